@@ -1,0 +1,231 @@
+"""Benchmark of the mesh-measurement hot path on one MI355X.
+
+Workload (one step = one mock): 256^3 float32 particles from a Zel'dovich field
+(Gaussian field P(k) = 1e4 k^-1.5, L = 1000 Mpc/h, seeded) resident in HBM ->
+TSC onto a 512^3 float32 mesh -> r2c FFT -> P(k) in 128 shells (ndel=2) with
+MAS deconvolution and shot-noise subtraction -> T-web of the R=2 Mpc/h
+Gaussian-smoothed density, lambda_th=0, int8 labels.
+
+Prints one JSON line: mocks/s of the optimised path, p50/p99 latency, the
+plain GPU path on the same inputs, the float32-vs-float64 P(k) error of the
+measured mock, the roofline of the dominant kernel (HIP events on the
+library's stream) and a CPU baseline (the OpenMP restatement of the reference
+path in oracle/cpu_ref.c, timed on a bounded sample on the host cores).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TSC+P(k) mocks/s & p50/p99 latency at 512^3 vs plain GPU path, P(k) err≤1e-4"
+HBM_PEAK_GBS = 8000.0
+
+NL, N, L = 256, 512, 1000.0
+NDEL, R_SMOOTH, LAMBDA_TH = 2, 2.0, 0.0
+
+
+def algorithmic_bytes(name, n_part, n_rec):
+    """Bytes a kernel must move once (each array read or written once)."""
+    n3 = N ** 3
+    nh = N * N * (N // 2 + 1)
+    table = {
+        "mas_tile_deposit": 16 * n_rec + 4 * n3,
+        "mas_tile_scatter": 12 * n_part + 16 * n_rec,
+        "mas_tile_count": 12 * n_part,
+        "mas_plain": 12 * n_part + 4 * n3,
+        "fft_z_r2c": 4 * n3 + 8 * nh,
+        "fft_y_fwd": 16 * nh,
+        "fft_x_fwd": 16 * nh,
+        "fft_x_inv": 16 * nh,
+        "fft_y_inv": 16 * nh,
+        "fft_z_c2r": 8 * nh + 4 * n3,
+        "pk_shells": 8 * nh,
+        "tweb_kmul": 16 * nh,
+        "tweb_grad": 4 * n3 + 12 * n3,
+        "tweb_hess_eig": 12 * n3 + n3,
+        "mesh_sum": 4 * n3,
+    }
+    return table.get(name)
+
+
+KERNELS = ["mas_tile_count", "mas_tile_scan", "mas_tile_items", "mas_tile_scatter", "mas_tile_zero_split",
+           "mas_tile_deposit", "mas_plain", "fill", "mesh_sum", "mesh_sum_final", "fft_z_r2c", "fft_y_fwd",
+           "fft_x_fwd", "pk_zero", "pk_shells", "tweb_kmul", "fft_x_inv", "fft_y_inv", "fft_z_c2r", "tweb_grad",
+           "tweb_zero", "tweb_hess_eig"]
+
+
+def cpu_baseline(pos, quick=False):
+    """OpenMP restatement of the reference path on the host cores, on a bounded
+    sample: the whole pipeline at NS^3 with NS^3/8 particles (1/8 of the mesh and
+    particle count per axis-halving), scaled to the 512^3 mock by the ratio of
+    work (N^3 log N for the FFT, N^3 for the mesh passes, N_p for the deposit)."""
+    try:
+        import oracle.cpu_ref as cr
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "unit": "mocks/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    threads = int(os.environ.get("CPU_BASELINE_THREADS", min(16, os.cpu_count() or 1)))
+    NS = 128 if quick else 256
+    npart = (NS // 2) ** 3
+    sub = pos[:: max(1, len(pos) // npart)][:npart].astype(np.float32) * np.float32(1.0)
+    Ls = L * NS / N  # same cell size d = L/N
+    sub = np.mod(sub, Ls).astype(np.float32)
+    t = cr.time_pipeline(sub, NS, Ls, scheme=2, ndel=NDEL, R=R_SMOOTH, lambda_th=LAMBDA_TH, threads=threads,
+                         repeats=1)
+    # scale: mesh passes ~N^3, FFT ~N^3 log N, deposit ~N_p
+    f_mesh = (N / NS) ** 3
+    f_fft = f_mesh * math.log2(N) / math.log2(NS)
+    f_part = (NL ** 3) / npart
+    est = t["mas"] * f_part + t["fft"] * f_fft + t["pk"] * f_mesh + t["tweb"] * (
+        f_mesh * (t["tweb_fft"] / max(t["tweb"], 1e-12)) * math.log2(N) / math.log2(NS)
+        + f_mesh * (1 - t["tweb_fft"] / max(t["tweb"], 1e-12)))
+    return {"value": 1.0 / est, "unit": "mocks/s", "cores": threads, "kind": "port",
+            "sample": f"full pipeline at {NS}^3 mesh with {npart} particles (same cell size, TSC, "
+                      f"{NS // NDEL // 2} shells, T-web), {sum(t[k] for k in ('mas', 'fft', 'pk', 'tweb')):.2f} s, "
+                      f"scaled to 512^3 / 256^3 particles by N^3, N^3 log N and N_p",
+            "stage_seconds_sample": {k: round(v, 4) for k, v in t.items()}}
+
+
+def run_steps(ctx, posd, b, fused, steps, warmup, labels):
+    import torch
+    kw = dict(do_tweb=True, R_smooth=R_SMOOTH, lambda_th=LAMBDA_TH, margin_tol=1e-6, fused=fused, labels=labels)
+    for _ in range(warmup):
+        out = ctx.measure(posd, N, L, "TSC", b["nshells"], b["kmin"], b["dk"], **kw)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        out = ctx.measure(posd, N, L, "TSC", b["nshells"], b["kmin"], b["dk"], **kw)
+        lat.append(time.perf_counter() - s)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    return out, total, np.array(lat)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--quick-cpu", action="store_true", help="smaller CPU sample (128^3)")
+    ap.add_argument("--plain-steps", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import cosmicatlas_amd as cm
+    from cosmicatlas_amd import synth
+
+    t = time.time()
+    pos = synth.zeldovich(NL, L, A=1e4, ns=-1.5, seed=1)
+    gen_s = time.time() - t
+    dev = torch.device("cuda:0")
+    posd = torch.from_numpy(pos).to(dev)
+    b = cm.binning(N, L, ndel=NDEL, DeltaKmin=0.0)
+    labels = torch.empty((N, N, N), dtype=torch.int8, device=dev)
+    ctx = cm.Context(0)
+
+    # optimised path
+    out, total, lat = run_steps(ctx, posd, b, True, args.steps, args.warmup, labels)
+    value = args.steps / total
+    # per-kernel times (HIP events on the library stream) over a few more steps
+    ctx.profile(True)
+    ctx.profile_reset()
+    nprof = 3
+    run_steps(ctx, posd, b, True, nprof, 0, labels)
+    ktimes = {}
+    for k in KERNELS:
+        ms, cnt = ctx.kernel_time(k)
+        if cnt:
+            ktimes[k] = {"mean_ms": ms, "launches_per_step": cnt / nprof}
+    ctx.profile(False)
+    n_rec = None
+    # dominant kernel: largest total time per step
+    dom = max(ktimes.items(), key=lambda kv: kv[1]["mean_ms"] * kv[1]["launches_per_step"])
+    # records per deposit: estimated from the ghost fraction measured on the host
+    d = L / N
+    c = np.floor(pos.astype(np.float64) / d).astype(np.int64) % 16
+    n_rec = int(np.prod(1 + (c == 0) + (c == 15), axis=1).sum())
+    ab = algorithmic_bytes(dom[0], len(pos), n_rec)
+    roof = None
+    if ab is not None:
+        ach = ab / (dom[1]["mean_ms"] * 1e-3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tf):
+            traffic = json.load(open(tf)).get(dom[0])
+        roof = {"kernel": dom[0], "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
+                "mean_ms": round(dom[1]["mean_ms"], 4)}
+
+    # plain GPU path on the same inputs
+    psteps = args.plain_steps or max(3, args.steps // 2)
+    pout, ptotal, plat = run_steps(ctx, posd, b, False, psteps, 1, labels)
+    plain_value = psteps / ptotal
+
+    # float32 vs float64 P(k) of this mock: float64 mesh, FFT and shells on the GPU
+    mesh64 = ctx.mesh_assign(posd, N, L, "TSC", mesh_dtype=torch.float64)
+    nobj = ctx.mesh_sum(mesh64)
+    dk64 = ctx.fft_r2c(mesh64, subtract=nobj / N ** 3)
+    del mesh64
+    V = L ** 3
+    r0, _, _, rn = ctx.pk_shells(dk64, N, L, b["nshells"], b["kmin"], b["dk"], "TSC", V / nobj, nobj ** 2 / V)
+    del dk64
+    torch.cuda.empty_cache()
+    valid = rn > 0
+    pk_err = float(np.max(np.abs(out["pk0"][valid] - r0[valid]) / np.abs(r0[valid])))
+    modes_equal = bool(np.array_equal(rn, out["nmodes"]))
+
+    cpu = None if args.no_cpu else cpu_baseline(pos, quick=args.quick_cpu)
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "mocks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single mock: 256^3 Zel'dovich float32 particles (P=1e4 k^-1.5, L=1000 Mpc/h) -> "
+                               "TSC 512^3 float32 mesh -> r2c -> P(k) 128 shells (ndel=2, FP64 sums) -> "
+                               "T-web (Gaussian R=2 Mpc/h, lambda_th=0, int8 labels)",
+                   "N": N, "particles": int(len(pos)), "L": L, "nshells": b["nshells"]},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "plain_gpu": {"value": round(plain_value, 3), "unit": "mocks/s",
+                      "p50_ms": round(1e3 * float(np.percentile(plat, 50)), 3),
+                      "p99_ms": round(1e3 * float(np.percentile(plat, 99)), 3)},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "pk_rel_err_vs_fp64": pk_err,
+        "nmodes_equal_fp64": modes_equal,
+        "tweb_counts": out["tweb_counts"],
+        "tweb_marginal": out["tweb_marginal"],
+        "kernels_ms_per_step": {k: round(v["mean_ms"] * v["launches_per_step"], 4) for k, v in ktimes.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "input_gen_s": round(gen_s, 2),
+    }
+    if cpu and cpu.get("value"):
+        res["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    ctx.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,252 @@
+"""cosmicatlas_amd -- MI355X (gfx950) mesh-measurement path of COSMICATLAS/BAM.
+
+Python front end of libcosmo_mesh.so (C ABI in include/cosmo_mesh.h).  Device
+buffers are torch tensors on ``cuda`` (used only as HIP memory); every compute
+call goes through the HIP library.  The C++ host classes that keep the
+reference's interface (Params, FftwFunctions, PowerSpectrumF, Cwclass, the
+``-p`` CLI) live in cosmicatlas_amd/host and call the same C ABI.
+
+The entry points mirror the reference functions they replace:
+
+=====================================  ==========================================
+``Context.mesh_assign``                grid_assignment_* / getDensity_*
+                                       (Source/Miscelanious.cpp:2597-2868, 65-815)
+``Context.fft_r2c`` / ``fft_c2r``      do_fftw_r2c / do_fftw_c2r
+                                       (Source/NumericalMethods.cpp:1718-1856)
+``Context.pk_shells``                  FftwFunctions::power_spectrum_fkp
+                                       (Source/FftwFunctions.cpp:958-1657)
+``Context.poisson``                    PoissonSolver (Source/Miscelanious.cpp:1189)
+``Context.tweb`` / ``tweb_from_phi``   Cwclass::get_CWC / EigenValuesTweb
+                                       (Source/Cwclass.cpp:235-894)
+``Context.measure``                    PowerSpectrumF::compute_power_spectrum +
+                                       Cwclass T-web on the same mesh
+=====================================  ==========================================
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from ._abi import (CM_AOS, CM_CIC, CM_F32, CM_F64, CM_MAS_AUTO, CM_MAS_PLAIN, CM_MAS_TILED,  # noqa: F401
+                   CM_NGP, CM_PCS, CM_SOA, CM_TSC)
+
+SCHEMES = {"NGP": CM_NGP, "CIC": CM_CIC, "TSC": CM_TSC, "PCS": CM_PCS}
+ALGOS = {"auto": CM_MAS_AUTO, "plain": CM_MAS_PLAIN, "tiled": CM_MAS_TILED}
+
+
+class CosmoError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _scheme(s):
+    return SCHEMES[s.upper()] if isinstance(s, str) else int(s)
+
+
+def fft_supported(N: int) -> bool:
+    return bool(_abi.load().cm_fft_supported(int(N)))
+
+
+class Context:
+    """One library context (device, stream, resident buffers)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, use_torch_stream: bool = True):
+        self.lib = _abi.load()
+        self.device = device
+        h = C.c_void_p()
+        st = self.lib.cm_ctx_create(device, hbm_budget, C.byref(h))
+        if st != 0:
+            raise CosmoError(st, f"cm_ctx_create(device={device}) failed: "
+                             f"{self.lib.cm_status_string(st).decode()}")
+        self.h = h
+        if use_torch_stream:
+            torch = _torch()
+            s = torch.cuda.current_stream(device).cuda_stream
+            self._check(self.lib.cm_set_stream(self.h, C.c_void_p(s)))
+
+    # ------------------------------------------------------------------ utils
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            raise CosmoError(st, self.lib.cm_last_error(self.h).decode())
+
+    def synchronize(self):
+        self._check(self.lib.cm_synchronize(self.h))
+
+    def profile(self, on: bool = True):
+        self._check(self.lib.cm_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._check(self.lib.cm_profile_reset(self.h))
+
+    def kernel_time(self, name: str):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._check(self.lib.cm_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    # --------------------------------------------------------- mass assignment
+    def mesh_assign(self, pos, N, L, scheme, weights=None, xmin=(0.0, 0.0, 0.0), soa=False,
+                    interlace_shift=0.0, algo="auto", mesh=None, mesh_dtype=None, accumulate=False):
+        """Deposit particles on an N^3 mesh (device tensors).  ``pos`` is (n,3)
+        (AoS) or (3,n) with ``soa``; float32 or float64."""
+        torch = _torch()
+        assert pos.is_cuda and pos.is_contiguous()
+        n = pos.shape[1] if soa else pos.shape[0]
+        pdt = CM_F32 if pos.dtype == torch.float32 else CM_F64
+        if mesh is None:
+            mdt = mesh_dtype or torch.float32
+            mesh = torch.empty((N, N, N), dtype=mdt, device=pos.device)
+        mcode = CM_F32 if mesh.dtype == torch.float32 else CM_F64
+        if weights is not None:
+            assert weights.dtype == pos.dtype and weights.is_contiguous() and weights.numel() == n
+        d = _abi.MasDesc(N, float(L), (C.c_double * 3)(*xmin), CM_SOA if soa else CM_AOS, pdt,
+                         _scheme(scheme), float(interlace_shift), ALGOS[algo])
+        self._check(self.lib.cm_mesh_assign(self.h, C.byref(d), _ptr(pos), _ptr(weights), n,
+                                            _ptr(mesh), mcode, int(accumulate)))
+        return mesh
+
+    def mesh_sum(self, mesh):
+        torch = _torch()
+        out = C.c_double()
+        code = CM_F32 if mesh.dtype == torch.float32 else CM_F64
+        self._check(self.lib.cm_mesh_sum(self.h, _ptr(mesh), code, mesh.numel(), C.byref(out)))
+        return out.value
+
+    def mesh_affine(self, mesh, scale, shift):
+        torch = _torch()
+        code = CM_F32 if mesh.dtype == torch.float32 else CM_F64
+        self._check(self.lib.cm_mesh_affine(self.h, _ptr(mesh), code, mesh.numel(), float(scale),
+                                            float(shift)))
+        return mesh
+
+    # -------------------------------------------------------------------- FFT
+    def fft_r2c(self, x, subtract=0.0, out=None):
+        """Unnormalised forward r2c of an (N,N,N) real tensor -> (N,N,N/2+1) complex."""
+        torch = _torch()
+        N = x.shape[0]
+        cdt = torch.complex64 if x.dtype == torch.float32 else torch.complex128
+        if out is None:
+            out = torch.empty((N, N, N // 2 + 1), dtype=cdt, device=x.device)
+        code = CM_F32 if x.dtype == torch.float32 else CM_F64
+        self._check(self.lib.cm_fft_r2c(self.h, N, code, _ptr(x), _ptr(out), float(subtract)))
+        return out
+
+    def fft_c2r(self, h, N, out=None):
+        """Inverse of a Hermitian half spectrum, divided by N^3 (h is overwritten)."""
+        torch = _torch()
+        rdt = torch.float32 if h.dtype == torch.complex64 else torch.float64
+        if out is None:
+            out = torch.empty((N, N, N), dtype=rdt, device=h.device)
+        code = CM_F32 if rdt == torch.float32 else CM_F64
+        self._check(self.lib.cm_fft_c2r(self.h, N, code, _ptr(h), _ptr(out)))
+        return out
+
+    # ------------------------------------------------------------------- P(k)
+    def pk_shells(self, dk, N, L, nshells, kmin, dk_bin, scheme, shot_noise, normal_power,
+                  mas_correction=True, float_shell_rule=False):
+        torch = _torch()
+        code = CM_F32 if dk.dtype == torch.complex64 else CM_F64
+        o = _abi.PkOpts(N, float(L), nshells, float(kmin), float(dk_bin), _scheme(scheme),
+                        int(mas_correction), float(shot_noise), float(normal_power),
+                        int(float_shell_rule))
+        p0, p2, p4 = (np.zeros(nshells) for _ in range(3))
+        nm = np.zeros(nshells, np.int64)
+        self._check(self.lib.cm_pk_shells(self.h, C.byref(o), code, _ptr(dk),
+                                          p0.ctypes.data_as(C.c_void_p), p2.ctypes.data_as(C.c_void_p),
+                                          p4.ctypes.data_as(C.c_void_p), nm.ctypes.data_as(C.c_void_p)))
+        return p0, p2, p4, nm
+
+    # ------------------------------------------------------------------ T-web
+    def _tweb_opts(self, N, L, R, lambda_th, margin_tol, delta_scale):
+        return _abi.TwebOpts(N, float(L), float(R), float(lambda_th), float(margin_tol),
+                             float(delta_scale))
+
+    def poisson(self, delta, L, R=0.0, delta_scale=1.0):
+        torch = _torch()
+        N = delta.shape[0]
+        phi = torch.empty_like(delta)
+        o = self._tweb_opts(N, L, R, 0.0, 0.0, delta_scale)
+        self._check(self.lib.cm_poisson(self.h, C.byref(o), _ptr(delta), _ptr(phi)))
+        return phi
+
+    def tweb(self, delta, L, R, lambda_th, margin_tol=0.0, delta_scale=1.0, want_lambdas=False):
+        torch = _torch()
+        N = delta.shape[0]
+        labels = torch.empty((N, N, N), dtype=torch.int8, device=delta.device)
+        lam = torch.empty((3, N, N, N), dtype=torch.float32, device=delta.device) if want_lambdas else None
+        st = _abi.TwebStats()
+        o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale)
+        self._check(self.lib.cm_tweb(self.h, C.byref(o), _ptr(delta), _ptr(labels), _ptr(lam), C.byref(st)))
+        return labels, lam, {"counts": list(st.counts), "n_marginal": st.n_marginal}
+
+    def tweb_from_phi(self, phi, L, lambda_th, margin_tol=0.0, want_lambdas=False):
+        torch = _torch()
+        N = phi.shape[0]
+        labels = torch.empty((N, N, N), dtype=torch.int8, device=phi.device)
+        lam = torch.empty((3, N, N, N), dtype=torch.float32, device=phi.device) if want_lambdas else None
+        st = _abi.TwebStats()
+        o = self._tweb_opts(N, L, 0.0, lambda_th, margin_tol, 1.0)
+        self._check(self.lib.cm_tweb_from_phi(self.h, C.byref(o), _ptr(phi), _ptr(labels), _ptr(lam),
+                                              C.byref(st)))
+        return labels, lam, {"counts": list(st.counts), "n_marginal": st.n_marginal}
+
+    # ---------------------------------------------------------- whole mock
+    def measure(self, pos, N, L, scheme, nshells, kmin, dk_bin, weights=None, mas_correction=True,
+                sn_correction=True, float_shell_rule=False, do_tweb=False, R_smooth=0.0,
+                lambda_th=0.0, margin_tol=0.0, fused=True, interlace_shift=0.0, labels=None):
+        torch = _torch()
+        n = pos.shape[0]
+        md = _abi.MasDesc(N, float(L), (C.c_double * 3)(0.0, 0.0, 0.0), CM_AOS, CM_F32, _scheme(scheme),
+                          float(interlace_shift), CM_MAS_AUTO)
+        o = _abi.MeasureOpts(md, nshells, float(kmin), float(dk_bin), int(mas_correction),
+                             int(sn_correction), int(float_shell_rule), int(do_tweb), float(R_smooth),
+                             float(lambda_th), float(margin_tol), int(fused))
+        p0, p2, p4 = (np.zeros(nshells) for _ in range(3))
+        nm = np.zeros(nshells, np.int64)
+        if do_tweb and labels is None:
+            labels = torch.empty((N, N, N), dtype=torch.int8, device=pos.device)
+        info = _abi.MeasureInfo()
+        self._check(self.lib.cm_measure(self.h, C.byref(o), _ptr(pos), _ptr(weights), n,
+                                        p0.ctypes.data_as(C.c_void_p), p2.ctypes.data_as(C.c_void_p),
+                                        p4.ctypes.data_as(C.c_void_p), nm.ctypes.data_as(C.c_void_p),
+                                        _ptr(labels), C.byref(info)))
+        return {"pk0": p0, "pk2": p2, "pk4": p4, "nmodes": nm, "labels": labels,
+                "n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
+                "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal}
+
+
+def binning(N, L, ndel=1, DeltaKmin=0.0):
+    """Params::derived_pars binning (Source/Params.cpp:4091-4104): k_f=2pi/L,
+    kmin = DeltaKmin*k_f, dk = ndel*k_f, N_shells = N/ndel/2."""
+    kf = 2.0 * math.pi / L
+    return {"kmin": DeltaKmin * kf, "dk": ndel * kf, "nshells": N // ndel // 2, "kf": kf}

@@ -1,0 +1,114 @@
+// Internal declarations of libcosmo_mesh (gfx950).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmo_mesh.h"
+
+namespace cm {
+
+// Per-kernel event timing (cm_profile_enable / cm_kernel_time).
+struct KernelTimes {
+    double sum_ms = 0.0;
+    int64_t count = 0;
+};
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace cm
+
+struct cm_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;
+    size_t resident = 0;  // bytes held in `scratch` and tables
+    std::string err;
+    bool profiling = false;
+    std::map<std::string, cm::KernelTimes> ktimes;
+    std::vector<cm::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, cm::Buffer> scratch;   // named resident work buffers
+    std::map<uint64_t, void *> tables;           // twiddle / sinc tables
+    int cu_count = 256;
+    size_t lds_per_block = 65536;
+};
+
+namespace cm {
+
+cm_status fail(cm_ctx *ctx, cm_status s, const char *fmt, ...);
+cm_status hip_fail(cm_ctx *ctx, hipError_t e, const char *what);
+
+#define CM_HIP(ctx, call)                                              \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return cm::hip_fail((ctx), e_, #call);   \
+    } while (0)
+
+// Named persistent device buffer of at least `bytes` (grown on demand).
+cm_status scratch(cm_ctx *ctx, const char *name, size_t bytes, void **out);
+
+// Events around a launch when profiling is on.
+void prof_begin(cm_ctx *ctx, const char *name, hipEvent_t *start);
+void prof_end(cm_ctx *ctx, const char *name, hipEvent_t start);
+cm_status check_launch(cm_ctx *ctx, const char *name);
+
+template <typename K, typename... Args>
+inline cm_status launch(cm_ctx *ctx, const char *name, K kernel, dim3 grid, dim3 block,
+                        size_t lds, Args... args) {
+    hipEvent_t ev = nullptr;
+    if (ctx->profiling) prof_begin(ctx, name, &ev);
+    hipLaunchKernelGGL(kernel, grid, block, lds, ctx->stream, args...);
+    if (ctx->profiling) prof_end(ctx, name, ev);
+    return check_launch(ctx, name);
+}
+
+// Twiddle table exp(-2 pi i t / n), t in [0, n), as T2 (float2/double2), cached.
+const void *twiddles(cm_ctx *ctx, int n, cm_dtype dt, cm_status *st);
+
+// FFT internals shared with the fused measurement path.
+struct FftPlan {
+    int n = 0;
+    int nst = 0;
+    int r[8] = {0};
+};
+FftPlan make_plan(int n);
+bool fft_size_ok(int N);
+
+cm_status fft_r2c_impl(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
+                       double subtract);
+cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale);
+// Strided column passes over the half spectrum, axis 1 (y) or 0 (x).
+cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward);
+cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
+                       double subtract);
+cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
+                       double scale);
+
+cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w,
+                           size_t n, void *mesh, cm_dtype mdt, int accumulate);
+cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
+                        double *host_out);
+
+// P(k) helpers (pk.hip)
+void pk_tables(int N, int scheme, int mas_correction, std::vector<double> &tab);
+cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const void *dk, double **sums,
+                        unsigned long long **cnts);
+void pk_finalize(int ns, double normal_power, double sn_norm, const double *sum, const unsigned long long *cnt,
+                 double *pk0, double *pk2, double *pk4, int64_t *nmodes);
+
+}  // namespace cm

@@ -1,0 +1,149 @@
+// Whole-mock measurement: particles -> mesh -> r2c -> P(k) shells (+ T-web).
+//
+// plain (fused=0): the step-by-step GPU path -- global-atomic deposit, FP64
+//   mesh sum read back to the host, r2c with the mean removed, shell kernel,
+//   then the T-web as separate Poisson / gradient / Hessian+eigen kernels.
+// fused (fused=1): tiled LDS deposit; the object count stays on the device
+//   (no host round trip until the shell sums are read at the end).
+#include <cmath>
+#include <vector>
+
+#include "cm_internal.hpp"
+#include "tweb_common.hpp"
+
+namespace cm {
+
+cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi, double subtract);
+cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
+                             cm_tweb_stats *st);
+
+// scale the half spectrum by -W(k)/k^2 * N^3/N_gal (delta = rho/mean - 1 with
+// mean = N_gal/N^3), reading N_gal on the device.
+__global__ void __launch_bounds__(256) k_kmul_dev(float2 *dk, int N, double kf, double R2h,
+                                                  const double *__restrict__ nobj) {
+    const int M1 = N / 2 + 1;
+    const size_t total = (size_t)N * N * M1;
+    const double n3 = (double)N * N * N;
+    const double scale = nobj[0] > 0.0 ? n3 / nobj[0] : 0.0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(g % M1);
+        const size_t r = g / M1;
+        const int iy = (int)(r % N), ix = (int)(r / N);
+        const double f = smooth_poisson_factor(ix, iy, iz, N, kf, R2h, scale);
+        float2 v = dk[g];
+        v.x = (float)((double)v.x * f);
+        v.y = (float)((double)v.y * f);
+        dk[g] = v;
+    }
+}
+
+static unsigned grid_for(size_t n, int block, size_t cap) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                              double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
+                              cm_measure_info *info) {
+    const int N = o->mas.N;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t nh = (size_t)N * N * (N / 2 + 1);
+    const double V = o->mas.L * o->mas.L * o->mas.L;
+    cm_status s;
+    void *mesh, *dk, *nobj;
+    if ((s = scratch(ctx, "m.mesh", sizeof(float) * n3, &mesh)) != CM_OK) return s;
+    if ((s = scratch(ctx, "m.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
+    if ((s = scratch(ctx, "m.nobj", sizeof(double), &nobj)) != CM_OK) return s;
+
+    cm_mas_desc md = o->mas;
+    md.algo = o->fused ? CM_MAS_AUTO : CM_MAS_PLAIN;
+    if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0)) != CM_OK) return s;
+    // N_gal = FP64 sum of the mesh (get_nobjects without its float sum)
+    double nobj_h = 0.0;
+    if (o->fused) {
+        if ((s = mesh_sum_impl(ctx, mesh, CM_F32, n3, (double *)nobj, nullptr)) != CM_OK) return s;
+    } else {
+        if ((s = mesh_sum_impl(ctx, mesh, CM_F32, n3, (double *)nobj, &nobj_h)) != CM_OK) return s;
+    }
+    // remove the mean before the transform; only the k=0 mode depends on it, and
+    // that mode is excluded from every shell, so an estimate is enough
+    const double mean_est = o->fused ? (double)n / (double)n3 : nobj_h / (double)n3;
+    if ((s = fft_r2c_impl(ctx, N, CM_F32, mesh, dk, mean_est)) != CM_OK) return s;
+
+    cm_pk_opts po;
+    po.N = N;
+    po.L = o->mas.L;
+    po.nshells = o->nshells;
+    po.kmin = o->kmin;
+    po.dk = o->dk;
+    po.scheme = o->mas.scheme;
+    po.mas_correction = o->mas_correction;
+    po.shot_noise = 0.0;  // applied at finalisation from the split sums
+    po.normal_power = 1.0;
+    po.float_shell_rule = o->float_shell_rule;
+    double *sums;
+    unsigned long long *cnts;
+    if ((s = pk_accumulate(ctx, &po, CM_F32, dk, &sums, &cnts)) != CM_OK) return s;
+    const int ns = o->nshells;
+    std::vector<double> hs(4 * (size_t)ns);
+    std::vector<unsigned long long> hc((size_t)ns);
+    CM_HIP(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipMemcpyAsync(hc.data(), cnts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+
+    if (o->do_tweb) {
+        if (!labels) return fail(ctx, CM_ERR_INVALID, "cm_measure: do_tweb needs a labels buffer");
+        void *phi;
+        if ((s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
+        const double kf = 2.0 * M_PI / o->mas.L;
+        const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
+        if ((s = launch(ctx, "tweb_kmul", k_kmul_dev, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, (float2 *)dk, N,
+                        kf, R2h, (const double *)nobj)) != CM_OK)
+            return s;
+        if ((s = fft_c2r_impl(ctx, N, CM_F32, dk, phi, 1.0 / (double)n3)) != CM_OK) return s;
+        cm_tweb_opts to;
+        to.N = N;
+        to.L = o->mas.L;
+        to.R_smooth = o->R_smooth;
+        to.lambda_th = o->lambda_th;
+        to.margin_tol = o->margin_tol;
+        to.delta_scale = 1.0;
+        cm_tweb_stats ts;
+        if ((s = tweb_from_phi_impl(ctx, &to, (const float *)phi, labels, nullptr, info ? &ts : nullptr)) != CM_OK)
+            return s;
+        if (info) info->tweb = ts;
+    }
+    if (o->fused) {
+        CM_HIP(ctx, hipMemcpyAsync(&nobj_h, nobj, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+    // normalisation: normal_power = N_gal^2/V, SN = V/N_gal (raw_sampling,
+    // FftwFunctions.cpp:159-174); SN*normal_power = N_gal
+    const double normal = nobj_h * nobj_h / V;
+    const double SN = (o->sn_correction && nobj_h > 0) ? V / nobj_h : 0.0;
+    if (info) {
+        info->n_obj = nobj_h;
+        info->shot_noise = SN;
+        info->normal_power = normal;
+    }
+    pk_finalize(ns, normal > 0 ? normal : 1.0, SN * normal, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
+    return CM_OK;
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                                double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
+                                cm_measure_info *info) {
+    if (!ctx) return CM_ERR_INVALID;
+    if (!o || (!pos && n)) return fail(ctx, CM_ERR_INVALID, "cm_measure: null pointer");
+    if (!fft_size_ok(o->mas.N)) return fail(ctx, CM_ERR_INVALID, "cm_measure: N=%d not supported", o->mas.N);
+    if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure: float32 positions only");
+    if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure: binning");
+    return measure_impl(ctx, o, pos, w, n, pk0, pk2, pk4, nmodes, labels, info);
+}

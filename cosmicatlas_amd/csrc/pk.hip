@@ -1,0 +1,247 @@
+// P(k) shell estimator over an r2c half spectrum (gfx950).
+//
+// One thread per stored mode (coalesced 8/16-byte loads).  Each mode is mapped
+// to the reference's octant enumeration (which stored modes are counted) and
+// its |k| shell (exact integer rule or the reference's float32 rule).  Four FP64
+// sums per shell are kept: |d|^2 W^-2, Sq W^-2 (the aliased shot-noise weight),
+// and the ell=2,4 Legendre-weighted |d|^2 W^-2.  The shot noise is subtracted
+// at finalisation, sum(|d|^2 - SN*n*Sq)W^-2 = S0 - SN*n*S1, which is the
+// reference's per-mode subtraction (FftwFunctions.cpp:1176) summed.  Modes of
+// one wavefront that share
+// (row, shell) are combined with a segmented scan over the wavefront (shells
+// are monotone along kz, so runs are contiguous); the last lane of each run adds
+// the run total into FP64 LDS accumulators, and each workgroup adds its
+// accumulators into the FP64 global sums once.
+#include <cmath>
+#include <vector>
+
+#include "cm_internal.hpp"
+#include "pk_common.hpp"
+
+namespace cm {
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pk_shells(const v2pk<T> *__restrict__ dk, PkK p,
+                                                   const double *__restrict__ tabs,
+                                                   double *__restrict__ gsum,
+                                                   unsigned long long *__restrict__ gcnt) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int M1 = p.N / 2 + 1;
+    double *s_ic = reinterpret_cast<double *>(smem_raw);
+    double *s_sq = s_ic + M1;
+    double *s_acc = s_sq + M1;                                        // 4*nshells
+    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * p.nshells);  // nshells
+    for (int i = threadIdx.x; i < M1; i += blockDim.x) {
+        s_ic[i] = tabs[i];
+        s_sq[i] = tabs[M1 + i];
+    }
+    for (int i = threadIdx.x; i < 4 * p.nshells; i += blockDim.x) s_acc[i] = 0.0;
+    for (int i = threadIdx.x; i < p.nshells; i += blockDim.x) s_cnt[i] = 0u;
+    __syncthreads();
+
+    const size_t total = (size_t)p.N * p.N * M1;
+    const int lane = threadIdx.x & 63;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    // all lanes of a wavefront iterate together (uniform trip count)
+    for (size_t base = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += step) {
+        const size_t g = base + lane;
+        int row = -1, sh = -1;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        unsigned c = 0;
+        if (g < total) {
+            const int iz = (int)(g % M1);
+            const size_t r = g / M1;
+            const int iy = (int)(r % p.N), ix = (int)(r / p.N);
+            int fx, fy;
+            if (mode_counted(p.N, ix, iy, iz, fx, fy)) {
+                sh = shell_of(p, fx, fy, iz);
+                if (sh >= 0) {
+                    row = (int)r;
+                    const v2pk<T> v = dk[g];
+                    const double pw = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                    const double ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
+                    const int q = fx * fx + fy * fy + iz * iz;
+                    const double mu2 = (double)iz * (double)iz / (double)q;
+                    const double pc = pw * ic;
+                    a0 = pc;
+                    a1 = s_sq[fx] * s_sq[fy] * s_sq[iz] * ic;
+                    a2 = 0.5 * (3.0 * mu2 - 1.0) * pc;
+                    a3 = (35.0 * mu2 * mu2 - 30.0 * mu2 + 3.0) * 0.125 * pc;
+                    c = 1;
+                }
+            }
+        }
+        // segmented inclusive scan over equal (row, shell) runs
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int urow = __shfl_up(row, off, 64);
+            const int ush = __shfl_up(sh, off, 64);
+            const double u0 = __shfl_up(a0, off, 64);
+            const double u1 = __shfl_up(a1, off, 64);
+            const double u2 = __shfl_up(a2, off, 64);
+            const double u3 = __shfl_up(a3, off, 64);
+            const unsigned uc = __shfl_up(c, off, 64);
+            if (lane >= off && urow == row && ush == sh) {
+                a0 += u0;
+                a1 += u1;
+                a2 += u2;
+                a3 += u3;
+                c += uc;
+            }
+        }
+        const int nrow = __shfl_down(row, 1, 64);
+        const int nsh = __shfl_down(sh, 1, 64);
+        const bool tail = (lane == 63) || nrow != row || nsh != sh;
+        if (tail && sh >= 0) {
+            atomicAdd(&s_acc[sh], a0);
+            atomicAdd(&s_acc[p.nshells + sh], a1);
+            atomicAdd(&s_acc[2 * p.nshells + sh], a2);
+            atomicAdd(&s_acc[3 * p.nshells + sh], a3);
+            atomicAdd(&s_cnt[sh], c);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < p.nshells; i += blockDim.x) {
+        if (s_cnt[i]) {
+            unsafeAtomicAdd(&gsum[i], s_acc[i]);
+            unsafeAtomicAdd(&gsum[p.nshells + i], s_acc[p.nshells + i]);
+            unsafeAtomicAdd(&gsum[2 * p.nshells + i], s_acc[2 * p.nshells + i]);
+            unsafeAtomicAdd(&gsum[3 * p.nshells + i], s_acc[3 * p.nshells + i]);
+            atomicAdd(&gcnt[i], (unsigned long long)s_cnt[i]);
+        }
+    }
+}
+
+__global__ void k_pk_zero(double *s, unsigned long long *c, int ns) {
+    for (int i = threadIdx.x; i < 4 * ns; i += blockDim.x) s[i] = 0.0;
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) c[i] = 0ull;
+}
+
+// Per-axis tables: inverse squared MAS window sinc(pi i/N)^-2(p+1) and the
+// aliased shot-noise factor Sq(p, sin(pi i/N)) (Miscelanious.cpp:2532-2577).
+void pk_tables(int N, int scheme, int mas_correction, std::vector<double> &tab) {
+    const int M1 = N / 2 + 1;
+    tab.assign(2 * (size_t)M1, 0.0);
+    for (int i = 0; i < M1; ++i) {
+        double a = M_PI * (double)i / (double)N;
+        double sinc = i == 0 ? 1.0 : std::sin(a) / a;
+        double ic = 1.0;
+        if (mas_correction) {
+            double cp = std::pow(sinc, scheme + 1);
+            ic = 1.0 / (cp * cp);
+        }
+        double s = std::sin(a), s2 = s * s;
+        double sq = 1.0;
+        switch (scheme) {
+        case 0: sq = 1.0; break;
+        case 1: sq = 1.0 - (2.0 / 3.0) * s2; break;
+        case 2: sq = 1.0 - s2 + (2.0 / 15.0) * s2 * s2; break;
+        case 3: sq = 1.0 - (4.0 / 3.0) * s2 + (2.0 / 5.0) * s2 * s2 - (4.0 / 315.0) * s2 * s2 * s2; break;
+        }
+        tab[i] = ic;
+        tab[M1 + i] = sq;
+    }
+}
+
+static bool near_int(double v, long &out) {
+    double r = std::floor(v + 0.5);
+    if (std::fabs(v - r) < 1e-9) {
+        out = (long)r;
+        return true;
+    }
+    return false;
+}
+
+PkK make_pkk(const cm_pk_opts *o) {
+    PkK p;
+    p.N = o->N;
+    p.nshells = o->nshells;
+    const double kf = 2.0 * M_PI / o->L;
+    p.Dmin = o->kmin / kf;
+    p.ndel = o->dk / kf;
+    long di = 0, ni = 0;
+    p.int_rule = (near_int(p.Dmin, di) && near_int(p.ndel, ni) && ni > 0) ? 1 : 0;
+    p.Dmin_i = (int)di;
+    p.ndel_i = (int)ni;
+    p.float_rule = o->float_shell_rule ? 1 : 0;
+    p.dkx_f = (float)kf;
+    p.kmin_f = (float)o->kmin;
+    p.rdk_f = (float)(1.0 / (double)(float)o->dk);
+    return p;
+}
+
+cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const void *dk, double **sums,
+                        unsigned long long **cnts) {
+    const int ns = o->nshells;
+    const int M1 = o->N / 2 + 1;
+    std::vector<double> tab;
+    pk_tables(o->N, (int)o->scheme, o->mas_correction, tab);
+    void *dtab, *dsum, *dcnt;
+    cm_status s;
+    if ((s = scratch(ctx, "pk.tab", sizeof(double) * tab.size(), &dtab)) != CM_OK) return s;
+    if ((s = scratch(ctx, "pk.sum", sizeof(double) * 4 * (size_t)ns, &dsum)) != CM_OK) return s;
+    if ((s = scratch(ctx, "pk.cnt", sizeof(unsigned long long) * (size_t)ns, &dcnt)) != CM_OK) return s;
+    CM_HIP(ctx, hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    if ((s = launch(ctx, "pk_zero", k_pk_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
+                    ns)) != CM_OK)
+        return s;
+    PkK p = make_pkk(o);
+    const size_t total = (size_t)o->N * o->N * M1;
+    size_t lds = sizeof(double) * (2 * (size_t)M1 + 4 * (size_t)ns) + sizeof(unsigned) * (size_t)ns;
+    if (lds > ctx->lds_per_block)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "pk: %d shells need %zu bytes of LDS", ns, lds);
+    size_t blocks = (total + 255) / 256;
+    unsigned g = (unsigned)std::min<size_t>(blocks, (size_t)ctx->cu_count * 8);
+    if (dt == CM_F32)
+        s = launch(ctx, "pk_shells", k_pk_shells<float>, dim3(g), dim3(256), lds, (const v2pk<float> *)dk, p,
+                   (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt);
+    else
+        s = launch(ctx, "pk_shells", k_pk_shells<double>, dim3(g), dim3(256), lds, (const v2pk<double> *)dk, p,
+                   (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt);
+    if (s != CM_OK) return s;
+    *sums = (double *)dsum;
+    *cnts = (unsigned long long *)dcnt;
+    return CM_OK;
+}
+
+// Normalisation of power_spectrum_fkp (FftwFunctions.cpp:1613-1628) from the
+// four shell sums; sn_norm = shot_noise * normal_power (= N_gal in a box).
+void pk_finalize(int ns, double normal_power, double sn_norm, const double *sum, const unsigned long long *cnt,
+                 double *pk0, double *pk2, double *pk4, int64_t *nmodes) {
+    for (int i = 0; i < ns; ++i) {
+        double den = (double)cnt[i] * normal_power;
+        if (pk0) pk0[i] = cnt[i] ? (sum[i] - sn_norm * sum[ns + i]) / den : 0.0;
+        if (pk2) pk2[i] = cnt[i] ? 5.0 * sum[2 * ns + i] / den : 0.0;
+        if (pk4) pk4[i] = cnt[i] ? 9.0 * sum[3 * ns + i] / den : 0.0;
+        if (nmodes) nmodes[i] = (int64_t)cnt[i];
+    }
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" cm_status cm_pk_shells(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dtype, const void *dk, double *pk0,
+                                  double *pk2, double *pk4, int64_t *nmodes) {
+    if (!ctx) return CM_ERR_INVALID;
+    if (!o || !dk) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: null pointer");
+    if (o->N < 2 || (o->N & 1)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: N=%d must be even", o->N);
+    if (o->nshells < 1) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: nshells=%d", o->nshells);
+    if (!(o->dk > 0) || !(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: dk and L must be > 0");
+    if ((int)o->scheme < 0 || (int)o->scheme > 3) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: scheme");
+    if (!(o->normal_power > 0) && (pk0 || pk2 || pk4))
+        return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: normal_power must be > 0");
+    double *sums;
+    unsigned long long *cnts;
+    cm_status s = pk_accumulate(ctx, o, dtype, dk, &sums, &cnts);
+    if (s != CM_OK) return s;
+    std::vector<double> hs(4 * (size_t)o->nshells);
+    std::vector<unsigned long long> hc((size_t)o->nshells);
+    CM_HIP(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipMemcpyAsync(hc.data(), cnts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    pk_finalize(o->nshells, o->normal_power, o->shot_noise * o->normal_power, hs.data(), hc.data(), pk0, pk2, pk4,
+                nmodes);
+    return CM_OK;
+}

@@ -1,0 +1,72 @@
+// Mode enumeration and shell index of the P(k) estimator, shared by the shell
+// kernel and the fused FFT/shell pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace cm {
+
+template <typename T> struct v2pk_t;
+template <> struct v2pk_t<float> { using type = float2; };
+template <> struct v2pk_t<double> { using type = double2; };
+template <typename T> using v2pk = typename v2pk_t<T>::type;
+
+struct PkK {
+    int N;
+    int nshells;
+    int int_rule;    // Dmin and ndel are integers: exact integer edges
+    int float_rule;  // the reference's float32 shell index
+    int Dmin_i, ndel_i;
+    double Dmin, ndel;  // kmin/k_f, dk/k_f
+    float dkx_f, kmin_f, rdk_f;
+    float pad;
+};
+
+// Which stored half-spectrum modes power_spectrum_fkp visits
+// (FftwFunctions.cpp:1061-1084,1180,1273,1373,1474): the loop runs i,j < N/2,
+// k <= N/2 and adds the copies (i,j,k) [not the zero mode], (i,N-j,k) if
+// j>0 && k>0, (N-i,j,k) if i>0 && (j>0 || k>0), (N-i,N-j,k) if i,j,k>0.  So the
+// x- and y-Nyquist planes are never visited.  Returns the folded |i|,|j|.
+__device__ __forceinline__ bool mode_counted(int N, int ix, int iy, int iz, int &fx, int &fy) {
+    const int h = N / 2;
+    if (ix == h || iy == h) return false;
+    const bool xl = ix < h, yl = iy < h;
+    fx = xl ? ix : N - ix;
+    fy = yl ? iy : N - iy;
+    if (xl && yl) return (ix | iy | iz) != 0;
+    if (xl) return iz > 0;           // (i, N-j, k): j>0 && k>0
+    if (yl) return iy > 0 || iz > 0;  // (N-i, j, k): i>0 && (j>0 || k>0)
+    return iz > 0;                    // (N-i, N-j, k): i,j,k>0
+}
+
+// Shell of the folded mode (i,j,k); -1 outside [0, nshells).
+__device__ __forceinline__ int shell_of(const PkK &p, int i, int j, int k) {
+    if (p.float_rule) {
+        // real_prec=float arithmetic of FftwFunctions.cpp:1068-1097, no contraction
+        float ti = __fmul_rn(__fmul_rn((float)(i * i), p.dkx_f), p.dkx_f);
+        float tj = __fmul_rn(__fmul_rn((float)(j * j), p.dkx_f), p.dkx_f);
+        float tk = __fmul_rn(__fmul_rn((float)(k * k), p.dkx_f), p.dkx_f);
+        float kv = __fsqrt_rn(__fadd_rn(__fadd_rn(ti, tj), tk));
+        float t = __fmul_rn(__fsub_rn(kv, p.kmin_f), p.rdk_f);
+        int s = (int)floorf(t);
+        return (s < 0 || s >= p.nshells) ? -1 : s;
+    }
+    const int q = i * i + j * j + k * k;
+    const double t = (sqrt((double)q) - p.Dmin) / p.ndel;
+    int s = (int)floor(t);
+    if (p.int_rule) {
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            long lo = (long)s * p.ndel_i + p.Dmin_i, hi = (long)(s + 1) * p.ndel_i + p.Dmin_i;
+            if (lo > 0 && (long)q < lo * lo) --s;
+            else if (hi >= 0 && (long)q >= hi * hi) ++s;
+        }
+    }
+    if (t < 0.0 && s >= 0) s = -1;
+    return (s < 0 || s >= p.nshells) ? -1 : s;
+}
+
+}  // namespace cm

@@ -1,0 +1,193 @@
+// T-web classification for gfx950 (plain kernels).
+//   delta -> r2c -> exp(-k^2R^2/2)(-1/k^2) -> c2r -> phi            (Poisson)
+//   phi -> G_a = D_a phi (a = x,y,z)                                 (k_grad3)
+//   G -> T_ab = D_b G_a -> FP64 eigenvalues -> label, counts         (k_hess_eig)
+// The lower triangle the reference feeds to GSL is A10 = D_x G_y,
+// A20 = D_x G_z, A21 = D_y G_z (Miscelanious.cpp:1684-1740).
+#include <cmath>
+
+#include "cm_internal.hpp"
+#include "tweb_common.hpp"
+
+namespace cm {
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_kmul(typename std::conditional<sizeof(T) == 4, float2, double2>::type *dk,
+                                              int N, double kf, double R2h, double scale) {
+    const int M1 = N / 2 + 1;
+    const size_t total = (size_t)N * N * M1;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(g % M1);
+        const size_t r = g / M1;
+        const int iy = (int)(r % N), ix = (int)(r / N);
+        const double f = smooth_poisson_factor(ix, iy, iz, N, kf, R2h, scale);
+        auto v = dk[g];
+        v.x = (T)((double)v.x * f);
+        v.y = (T)((double)v.y * f);
+        dk[g] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_grad3(const float *__restrict__ phi, int N, double ih,
+                                               float *__restrict__ gx, float *__restrict__ gy,
+                                               float *__restrict__ gz) {
+    const size_t n3 = (size_t)N * N * N;
+    const size_t NN = (size_t)N;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(c % N), y = (int)((c / N) % N), x = (int)(c / (NN * N));
+        auto at = [&](int i, int j, int k) { return phi[((size_t)i * NN + j) * NN + k]; };
+        const int xm2 = (x - 2 + N) % N, xm1 = (x - 1 + N) % N, xp1 = (x + 1) % N, xp2 = (x + 2) % N;
+        const int ym2 = (y - 2 + N) % N, ym1 = (y - 1 + N) % N, yp1 = (y + 1) % N, yp2 = (y + 2) % N;
+        const int zm2 = (z - 2 + N) % N, zm1 = (z - 1 + N) % N, zp1 = (z + 1) % N, zp2 = (z + 2) % N;
+        gx[c] = fd4(at(xm2, y, z), at(xm1, y, z), at(xp1, y, z), at(xp2, y, z), ih);
+        gy[c] = fd4(at(x, ym2, z), at(x, ym1, z), at(x, yp1, z), at(x, yp2, z), ih);
+        gz[c] = fd4(at(x, y, zm2), at(x, y, zm1), at(x, y, zp1), at(x, y, zp2), ih);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, const float *__restrict__ gy,
+                                                  const float *__restrict__ gz, int N, double ih,
+                                                  double lth, double tol, int8_t *__restrict__ labels,
+                                                  float *__restrict__ lam, unsigned long long *__restrict__ stats) {
+    __shared__ unsigned cnt[5];
+    if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t n3 = (size_t)N * N * N;
+    const size_t NN = (size_t)N;
+    unsigned loc[5] = {0, 0, 0, 0, 0};
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(c % N), y = (int)((c / N) % N), x = (int)(c / (NN * N));
+        const int xm2 = (x - 2 + N) % N, xm1 = (x - 1 + N) % N, xp1 = (x + 1) % N, xp2 = (x + 2) % N;
+        const int ym2 = (y - 2 + N) % N, ym1 = (y - 1 + N) % N, yp1 = (y + 1) % N, yp2 = (y + 2) % N;
+        const int zm2 = (z - 2 + N) % N, zm1 = (z - 1 + N) % N, zp1 = (z + 1) % N, zp2 = (z + 2) % N;
+        auto id = [&](int i, int j, int k) { return ((size_t)i * NN + j) * NN + k; };
+        // T_xx = D_x G_x, A10 = D_x G_y, A20 = D_x G_z
+        const float txx = fd4(gx[id(xm2, y, z)], gx[id(xm1, y, z)], gx[id(xp1, y, z)], gx[id(xp2, y, z)], ih);
+        const float tyx = fd4(gy[id(xm2, y, z)], gy[id(xm1, y, z)], gy[id(xp1, y, z)], gy[id(xp2, y, z)], ih);
+        const float tzx = fd4(gz[id(xm2, y, z)], gz[id(xm1, y, z)], gz[id(xp1, y, z)], gz[id(xp2, y, z)], ih);
+        // T_yy = D_y G_y, A21 = D_y G_z
+        const float tyy = fd4(gy[id(x, ym2, z)], gy[id(x, ym1, z)], gy[id(x, yp1, z)], gy[id(x, yp2, z)], ih);
+        const float tzy = fd4(gz[id(x, ym2, z)], gz[id(x, ym1, z)], gz[id(x, yp1, z)], gz[id(x, yp2, z)], ih);
+        // T_zz = D_z G_z
+        const float tzz = fd4(gz[id(x, y, zm2)], gz[id(x, y, zm1)], gz[id(x, y, zp1)], gz[id(x, y, zp2)], ih);
+        double e1, e2, e3;
+        eig3(txx, tyy, tzz, tyx, tzx, tzy, e1, e2, e3);
+        const int v = (e1 > lth) + (e2 > lth) + (e3 > lth);
+        const int lab = 4 - v;
+        labels[c] = (int8_t)lab;
+        if (lam) {
+            lam[c] = (float)e1;
+            lam[n3 + c] = (float)e2;
+            lam[2 * n3 + c] = (float)e3;
+        }
+        loc[lab - 1]++;
+        const double m = fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth)));
+        if (m < tol) loc[4]++;
+    }
+    for (int i = 0; i < 5; ++i)
+        if (loc[i]) atomicAdd(&cnt[i], loc[i]);
+    __syncthreads();
+    if (threadIdx.x < 5 && cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+__global__ void k_zero_u64(unsigned long long *a, int n) {
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0ull;
+}
+
+static unsigned grid_for(size_t n, int block, size_t cap) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// phi = c2r(W(k)(-1/k^2) r2c(delta)*scale), float32.
+cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi, double subtract) {
+    const int N = o->N;
+    const size_t nh = (size_t)N * N * (N / 2 + 1);
+    void *dk;
+    cm_status s;
+    if ((s = scratch(ctx, "tweb.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
+    if ((s = fft_r2c_impl(ctx, N, CM_F32, delta, dk, subtract)) != CM_OK) return s;
+    const double kf = 2.0 * M_PI / o->L;
+    const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
+    const double scale = o->delta_scale == 0.0 ? 1.0 : o->delta_scale;
+    if ((s = launch(ctx, "tweb_kmul", k_kmul<float>, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, (float2 *)dk, N,
+                    kf, R2h, scale)) != CM_OK)
+        return s;
+    const double n3 = (double)N * N * N;
+    return fft_c2r_impl(ctx, N, CM_F32, dk, phi, 1.0 / n3);
+}
+
+cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
+                             cm_tweb_stats *st) {
+    const int N = o->N;
+    const size_t n3 = (size_t)N * N * N;
+    void *g, *stats;
+    cm_status s;
+    if ((s = scratch(ctx, "tweb.grad", sizeof(float) * 3 * n3, &g)) != CM_OK) return s;
+    if ((s = scratch(ctx, "tweb.stats", sizeof(unsigned long long) * 5, &stats)) != CM_OK) return s;
+    const double ih = 1.0 / (12.0 * (o->L / (double)N));
+    float *gx = (float *)g, *gy = gx + n3, *gz = gy + n3;
+    unsigned gr = grid_for(n3, 256, 65536);
+    if ((s = launch(ctx, "tweb_grad", k_grad3, dim3(gr), dim3(256), 0, phi, N, ih, gx, gy, gz)) != CM_OK) return s;
+    if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, (unsigned long long *)stats, 5)) != CM_OK)
+        return s;
+    if ((s = launch(ctx, "tweb_hess_eig", k_hess_eig, dim3(gr), dim3(256), 0, (const float *)gx, (const float *)gy,
+                    (const float *)gz, N, ih, o->lambda_th, o->margin_tol, labels, lam,
+                    (unsigned long long *)stats)) != CM_OK)
+        return s;
+    if (st) {
+        unsigned long long h[5];
+        CM_HIP(ctx, hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+        CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < 4; ++i) st->counts[i] = (int64_t)h[i];
+        st->n_marginal = (int64_t)h[4];
+    }
+    return CM_OK;
+}
+
+static cm_status check_opts(cm_ctx *ctx, const cm_tweb_opts *o) {
+    if (!o) return fail(ctx, CM_ERR_INVALID, "tweb: null options");
+    if (!fft_size_ok(o->N)) return fail(ctx, CM_ERR_INVALID, "tweb: N=%d not supported by the FFT", o->N);
+    if (!(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "tweb: L must be > 0");
+    return CM_OK;
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+cm_status cm_poisson(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi) {
+    if (!ctx) return CM_ERR_INVALID;
+    cm_status s = check_opts(ctx, o);
+    if (s != CM_OK) return s;
+    if (!delta || !phi) return fail(ctx, CM_ERR_INVALID, "cm_poisson: null pointer");
+    return poisson_impl(ctx, o, delta, phi, 0.0);
+}
+
+cm_status cm_tweb_from_phi(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lambdas,
+                           cm_tweb_stats *stats) {
+    if (!ctx) return CM_ERR_INVALID;
+    cm_status s = check_opts(ctx, o);
+    if (s != CM_OK) return s;
+    if (!phi || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb_from_phi: null pointer");
+    return tweb_from_phi_impl(ctx, o, phi, labels, lambdas, stats);
+}
+
+cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t *labels, float *lambdas,
+                  cm_tweb_stats *stats) {
+    if (!ctx) return CM_ERR_INVALID;
+    cm_status s = check_opts(ctx, o);
+    if (s != CM_OK) return s;
+    if (!delta || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb: null pointer");
+    void *phi;
+    const size_t n3 = (size_t)o->N * o->N * o->N;
+    if ((s = scratch(ctx, "tweb.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
+    if ((s = poisson_impl(ctx, o, delta, (float *)phi, 0.0)) != CM_OK) return s;
+    return tweb_from_phi_impl(ctx, o, (const float *)phi, labels, lambdas, stats);
+}
+
+}  // extern "C"

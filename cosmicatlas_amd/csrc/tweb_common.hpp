@@ -1,0 +1,61 @@
+// Device helpers of the T-web: k-space multiplier, FD stencil, 3x3 eigenvalues.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cm {
+
+// Wavenumber of index i on an N grid: i<=N/2 -> +i k_f, else -(N-i) k_f
+// (PoissonSolver, Miscelanious.cpp:1236-1241).
+__device__ __forceinline__ double kcoord(int i, int N, double kf) {
+    return kf * (i <= N / 2 ? (double)i : -(double)(N - i));
+}
+
+// scale * exp(-k^2 R^2/2) * (-1/k^2), zero at k=0 (kernelcomp filtertype 1,
+// NumericalMethods.cpp:2263-2264, times PoissonSolver's -1/k^2).
+__device__ __forceinline__ double smooth_poisson_factor(int ix, int iy, int iz, int N, double kf, double R2h,
+                                                        double scale) {
+    const double kx = kcoord(ix, N, kf), ky = kcoord(iy, N, kf), kz = kf * (double)iz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 <= 0.0) return 0.0;
+    double f = -scale / k2;
+    if (R2h > 0.0) f *= exp(-k2 * R2h);
+    return f;
+}
+
+// gradfindif stencil (NumericalMethods.cpp:1123-1217); the reference's float
+// build evaluates it in double (8.0 is a double literal) and stores to float.
+__device__ __forceinline__ float fd4(float fll, float fl, float fr, float frr, double ih) {
+    return (float)((-(double)frr + 8.0 * (double)fr - 8.0 * (double)fl + (double)fll) * ih);
+}
+
+// Eigenvalues (descending) of the symmetric matrix with diagonal a00,a11,a22 and
+// lower triangle a10,a20,a21: trigonometric closed form in FP64.
+__device__ __forceinline__ void eig3(double a00, double a11, double a22, double a10, double a20, double a21,
+                                     double &e1, double &e2, double &e3) {
+    const double p1 = a10 * a10 + a20 * a20 + a21 * a21;
+    const double q = (a00 + a11 + a22) * (1.0 / 3.0);
+    if (p1 == 0.0) {
+        double x = a00, y = a11, z = a22, t;
+        if (x < y) { t = x; x = y; y = t; }
+        if (y < z) { t = y; y = z; z = t; }
+        if (x < y) { t = x; x = y; y = t; }
+        e1 = x; e2 = y; e3 = z;
+        return;
+    }
+    const double b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+    const double p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * p1;
+    const double p = sqrt(p2 * (1.0 / 6.0));
+    const double ip = 1.0 / p;
+    const double c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip;
+    const double c10 = a10 * ip, c20 = a20 * ip, c21 = a21 * ip;
+    const double det = c00 * (c11 * c22 - c21 * c21) - c10 * (c10 * c22 - c21 * c20) + c20 * (c10 * c21 - c11 * c20);
+    double r = 0.5 * det;
+    r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+    const double phi = acos(r) * (1.0 / 3.0);
+    e1 = q + 2.0 * p * cos(phi);
+    e3 = q + 2.0 * p * cos(phi + 2.0943951023931954923);
+    e2 = 3.0 * q - e1 - e3;
+}
+
+}  // namespace cm

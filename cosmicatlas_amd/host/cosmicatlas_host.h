@@ -1,0 +1,236 @@
+// Host-side C++ interface of the mesh-measurement path, keeping the names,
+// argument meaning and data layouts of the COSMICATLAS classes and free
+// functions it replaces.  Everything computes on the GPU through the C ABI in
+// include/cosmo_mesh.h; this layer only holds host vectors, parameters and
+// files.  Errors throw cmh::Error (the CLI turns them into the reference's
+// message-and-exit(1)).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmo_mesh.h"
+
+typedef float real_prec;              // def.h:491,580 (float build)
+typedef real_prec complex_prec[2];    // fftwf_complex (def.h:587)
+typedef unsigned long ULONG;
+#ifndef REAL
+#define REAL 0
+#define IMAG 1
+#endif
+// Cosmic-web labels (def.h:644-647)
+#define I_KNOT 1
+#define I_FILAMENT 2
+#define I_SHEET 3
+#define I_VOID 4
+
+namespace cmh {
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+// Process-wide library context on device 0 (created on first use).
+cm_ctx *ctx();
+void check(cm_status s, const char *what);
+}  // namespace cmh
+
+// --------------------------------------------------------------------- Params
+// .ini reader (Source/Params.cpp:452-481: "name = value", first token after '='
+// is the value, '#' lines skipped, empty value keeps the default) and the
+// derived constants of derived_pars (Source/Params.cpp:4079-4115).  Keys the
+// reference fixes at compile time (def.h) are run-time keys here:
+//   shell_rule = exact | float   (float = FftwFunctions.cpp:1097 float32 index)
+//   compute_tweb = true | false  (run Cwclass on the measured mesh, -p only)
+class Params {
+   public:
+    Params();
+    explicit Params(const std::string &par_file);
+    void init_pars();
+    void read_pars(const std::string &file);
+    void derived_pars();
+    void show_params() const;
+    std::string get(const std::string &key, const std::string &def = "") const;
+
+    ULONG _Nft() const { return Nft; }
+    real_prec _Lbox() const { return Lbox; }
+    ULONG _NGRID() const { return NGRID; }
+    ULONG _NGRID_h() const { return NGRID_h; }
+    std::string _mass_assignment_scheme() const { return mass_assignment_scheme; }
+    int _mass_assignment() const { return mass_assignment; }
+    bool _MAS_correction() const { return MAS_correction; }
+    std::string _type_of_binning() const { return type_of_binning; }
+    real_prec _DeltaKmin() const { return DeltaKmin; }
+    int _ndel_data() const { return ndel_data; }
+    int _N_log_bins() const { return N_log_bins; }
+    bool _SN_correction() const { return SN_correction; }
+    real_prec _lambdath() const { return lambdath; }
+    real_prec _smscale() const { return smscale; }
+    std::string _statistics() const { return statistics; }
+    std::string _input_type() const { return input_type; }
+    std::string _Output_directory() const { return Output_directory; }
+    std::string _file_power() const { return file_power; }
+    std::string _Name_survey() const { return Name_survey; }
+    std::string _delta_grid_file() const { return delta_grid_file; }
+    std::string _file_catalogue() const { return Input_dir_cat + file_catalogue; }
+    bool _use_random_catalog() const { return use_random_catalog; }
+    bool _measure_cross() const { return measure_cross; }
+    int _i_coord1_g() const { return i_coord1_g; }
+    int _i_coord2_g() const { return i_coord2_g; }
+    int _i_coord3_g() const { return i_coord3_g; }
+    int _i_weight1_g() const { return i_weight1_g; }
+    real_prec _Xoffset() const { return Xoffset; }
+    real_prec _Yoffset() const { return Yoffset; }
+    real_prec _Zoffset() const { return Zoffset; }
+    double _d_delta_x() const { return delta_x; }
+    double _d_deltak_x() const { return deltak_x; }
+    double _d_deltak_0() const { return deltak_0; }
+    double _d_kmin() const { return kmin; }
+    double _d_kmax() const { return kmax; }
+    double _d_DeltaK_data() const { return DeltaK_data; }
+    int _d_Nnp_data() const { return Nnp_data; }
+    double _d_Deltal() const { return Deltal; }
+    bool _float_shell_rule() const { return shell_rule == "float"; }
+    bool _compute_tweb() const { return compute_tweb; }
+
+   private:
+    std::map<std::string, std::string> raw;
+    ULONG Nft = 1, NGRID = 1, NGRID_h = 1;
+    real_prec Lbox = 100;
+    std::string mass_assignment_scheme = "NGP";
+    int mass_assignment = 0;
+    bool MAS_correction = false;
+    std::string type_of_binning = "linear";
+    real_prec DeltaKmin = 0;
+    int ndel_data = 1, N_log_bins = 10;
+    bool SN_correction = false;
+    real_prec lambdath = 0, smscale = 0;
+    std::string statistics = "Pk_fkp", input_type = "catalog", Output_directory = "../output";
+    std::string file_power = "power", Name_survey = "survey", delta_grid_file = "delta_file";
+    std::string file_catalogue = "cat.dat", Input_dir_cat = "";
+    bool use_random_catalog = false, measure_cross = false;
+    int i_coord1_g = 0, i_coord2_g = 1, i_coord3_g = 2, i_weight1_g = -1;
+    real_prec Xoffset = 0, Yoffset = 0, Zoffset = 0;
+    double delta_x = 0, deltak_x = 0, deltak_0 = 0, kmin = 0, kmax = 0, DeltaK_data = 0, Deltal = 0;
+    int Nnp_data = 0;
+    std::string shell_rule = "exact";
+    bool compute_tweb = false;
+};
+
+// ------------------------------------------------------------ free functions
+// Source/Miscelanious.cpp:65-147,454-557,693-815 (array form, SoA positions)
+void getDensity_NGP(ULONG N1, ULONG N2, ULONG N3, real_prec L1, real_prec L2, real_prec L3, real_prec d1,
+                    real_prec d2, real_prec d3, real_prec min1, real_prec min2, real_prec min3,
+                    const std::vector<real_prec> &xp, const std::vector<real_prec> &yp,
+                    const std::vector<real_prec> &zp, const std::vector<real_prec> &Par_mass,
+                    std::vector<real_prec> &delta, bool weightmass);
+void getDensity_CIC(ULONG N1, ULONG N2, ULONG N3, real_prec L1, real_prec L2, real_prec L3, real_prec d1,
+                    real_prec d2, real_prec d3, real_prec min1, real_prec min2, real_prec min3,
+                    const std::vector<real_prec> &xp, const std::vector<real_prec> &yp,
+                    const std::vector<real_prec> &zp, const std::vector<real_prec> &Par_mass,
+                    std::vector<real_prec> &delta, bool weightmass);
+void getDensity_TSC(ULONG N1, ULONG N2, ULONG N3, real_prec L1, real_prec L2, real_prec L3, real_prec d1,
+                    real_prec d2, real_prec d3, real_prec min1, real_prec min2, real_prec min3,
+                    const std::vector<real_prec> &xp, const std::vector<real_prec> &yp,
+                    const std::vector<real_prec> &zp, const std::vector<real_prec> &Par_mass,
+                    std::vector<real_prec> &delta, bool weightmass);
+// Catalogue-to-mesh for all tracers (grid_assignment_* per tracer,
+// Source/Miscelanious.cpp:2597-2868 as called from Catalog.cpp:14186-14211).
+void grid_assignment(const Params &params, const std::vector<real_prec> &xyz /* AoS */,
+                     const std::vector<real_prec> &w /* empty = 1 */, std::vector<real_prec> &field);
+// Source/NumericalMethods.cpp:1718-1753 / :1818-1856
+void do_fftw_r2c(ULONG Nft, std::vector<real_prec> &in, complex_prec *out);
+void do_fftw_c2r(int Nft, complex_prec *in, std::vector<real_prec> &out);
+// Source/Miscelanious.cpp:1189-1274
+void PoissonSolver(real_prec Lbox, ULONG Nft, std::vector<real_prec> &in, std::vector<real_prec> &out);
+// Source/Miscelanious.cpp:1668-1769 (FD branch)
+void EigenValuesTweb(ULONG Nft, real_prec L1, std::vector<real_prec> &delta, std::vector<real_prec> &phi,
+                     std::vector<real_prec> &out1, std::vector<real_prec> &out2, std::vector<real_prec> &out3);
+// Source/Miscelanious.cpp:1421-1426 (summed in FP64 here)
+real_prec get_nobjects(const std::vector<real_prec> &density_field);
+
+// --------------------------------------------------------------- FftwFunctions
+// Source/FftwFunctions.cpp:28-406,958-1875 (box estimator).
+class FftwFunctions {
+   public:
+    FftwFunctions() = default;
+    explicit FftwFunctions(const Params &p) : params(p) {}
+    ~FftwFunctions();
+    void set_params(const Params &p) { params = p; }
+    void resize_fftw_vectors();
+    void free_fftw_vectors();
+    void set_n_gal(real_prec n) { n_gal = n; }
+    void set_w_g(real_prec w) { w_g = w; }
+    void set_s_g(real_prec s) { s_g = s; }
+    void set_normal_power(real_prec v) { normal_power = v; }
+    void raw_sampling(real_prec vol);
+    void get_parameters_estimator(bool verbose);
+    void get_fluctuation();
+    // monopole/quadrupole/hexadecapole of the box FKP estimator (r2c inside)
+    void get_power_spectrum_fkp(std::vector<real_prec> &power_g0, std::vector<real_prec> &power_g2,
+                                std::vector<real_prec> &power_g4, std::vector<int> &mod_g);
+    // grid-input monopole (r2c inside); the reference's D2 defect at
+    // FftwFunctions.cpp:1853 is not reproduced
+    void power_spectrum(std::vector<real_prec> &powerk_g0, std::vector<int> &mod_g);
+    real_prec _normal_power() const { return normal_power; }
+    real_prec _shot_noise() const { return shot_noise; }
+
+    std::vector<real_prec> data_g;    // FftwFunctions.h:1179
+    complex_prec *data_out_g = nullptr;
+    real_prec shot_noise = 0;
+
+   private:
+    void shells(std::vector<real_prec> &p0, std::vector<real_prec> *p2, std::vector<real_prec> *p4,
+                std::vector<int> &mod);
+    Params params;
+    real_prec n_gal = 0, w_g = 0, s_g = 0, normal_power = 1, alpha = 1, normal_p = 1, s_r = 0;
+};
+
+// --------------------------------------------------------------------- Cwclass
+// Source/Cwclass.cpp:235-894 (T-web), Cwclass.h:138,171-183.
+class Cwclass {
+   public:
+    Cwclass() = default;
+    explicit Cwclass(const Params &p) : params(p) {}
+    void set_params(const Params &p) { params = p; }
+    // delta = overdensity; smoothing R = smscale (kernelcomp, applied here)
+    void get_CWC(std::vector<real_prec> &delta);
+    std::vector<int> CWClass;
+    std::vector<real_prec> lambda1, lambda2, lambda3;
+    real_prec volume_knots = 0, volume_filaments = 0, volume_sheets = 0, volume_voids = 0;
+    int64_t n_marginal = 0;
+
+   private:
+    Params params;
+};
+
+// --------------------------------------------------------------- PowerSpectrumF
+// Source/PowerSpectrumF.cpp:15-46,350-355,741-1363,4188-4290.
+class PowerSpectrumF {
+   public:
+    explicit PowerSpectrumF(const Params &p);
+    void set_output_filenames();
+    void compute_power_spectrum(bool verbose, bool mcut);
+    void compute_power_spectrum_grid();
+    void write_power_and_modes();
+    std::vector<real_prec> kvector_data, pk0, pk2, pk4;
+    std::vector<int> modes_g;
+    std::string file_power, file_cwc;
+    FftwFunctions fftw_functions;
+    Cwclass cwclass;
+
+   private:
+    void kvector();
+    Params params;
+};
+
+// Raw float32 meshes (FileOutput.cpp:727-753,858-912) and the k P N text file
+// (FileOutput.cpp:490-511: row 0 dropped when P(k_0) < 1).
+void read_array(const std::string &fname, std::vector<real_prec> &out);
+void write_array(const std::string &fname, const std::vector<real_prec> &in);
+void write_to_file2(const std::string &fname, const std::vector<real_prec> &kve, const std::vector<real_prec> &pk,
+                    const std::vector<int> &si, bool cero);
+// ASCII catalogue: whitespace-separated columns (Catalog::read_catalog).
+void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w);

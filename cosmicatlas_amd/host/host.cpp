@@ -1,0 +1,516 @@
+// Host implementation of the reference-named interface (see cosmicatlas_host.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "cosmicatlas_host.h"
+
+namespace cmh {
+
+static cm_ctx *g_ctx = nullptr;
+
+cm_ctx *ctx() {
+    if (!g_ctx) {
+        cm_status s = cm_ctx_create(0, 0, &g_ctx);
+        if (s != CM_OK) throw Error(std::string("cannot open the GPU context: ") + cm_status_string(s));
+    }
+    return g_ctx;
+}
+
+void check(cm_status s, const char *what) {
+    if (s != CM_OK) throw Error(std::string(what) + ": " + cm_last_error(g_ctx));
+}
+
+// Device buffer owned for the duration of one call.
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf(size_t bytes) { check(cm_dev_alloc(ctx(), bytes, &p), "cm_dev_alloc"); }
+    ~DevBuf() {
+        if (p) cm_dev_free(ctx(), p);
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+static void h2d(void *d, const void *h, size_t b) { check(cm_copy_h2d(ctx(), d, h, b), "cm_copy_h2d"); }
+static void d2h(void *h, const void *d, size_t b) { check(cm_copy_d2h(ctx(), h, d, b), "cm_copy_d2h"); }
+
+}  // namespace cmh
+
+using cmh::DevBuf;
+using cmh::check;
+using cmh::ctx;
+
+// ======================================================================= Params
+Params::Params() { init_pars(); }
+
+Params::Params(const std::string &par_file) {
+    init_pars();
+    read_pars(par_file);
+    derived_pars();
+}
+
+void Params::init_pars() { raw.clear(); }
+
+static bool to_bool(const std::string &v) { return v == "true"; }
+
+void Params::read_pars(const std::string &file) {
+    std::ifstream fin(file.c_str());
+    if (fin.fail()) throw cmh::Error("Error at opening the parameter file " + file);
+    std::string line;
+    while (std::getline(fin, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::stringstream ls(line);
+        std::string name, eq, value;
+        ls >> name;
+        if (name.empty()) continue;
+        ls >> eq;
+        if (eq != "=")
+            throw cmh::Error("Error in parameters file at " + name + " = ???\nSymbol '=' not properly written");
+        ls >> value;
+        if (value.empty()) {
+            std::cerr << "Warning from parameter file: value of " << name << " not specified in " << file
+                      << "; assuming a default value" << std::endl;
+            continue;
+        }
+        raw[name] = value;
+        if (name == "Nft") Nft = std::stoul(value);
+        else if (name == "Lbox") Lbox = (real_prec)std::atof(value.c_str());
+        else if (name == "mass_assignment_scheme") {
+            if (value != "NGP" && value != "CIC" && value != "TSC" && value != "PCS")
+                throw cmh::Error("Unknown mass assigment scheme");
+            mass_assignment_scheme = value;
+        } else if (name == "MAS_correction") MAS_correction = to_bool(value);
+        else if (name == "type_of_binning") {
+            if (value != "linear" && value != "log") throw cmh::Error("Unknown type of binning");
+            type_of_binning = value;
+        } else if (name == "DeltaKmin") DeltaKmin = (real_prec)std::atof(value.c_str());
+        else if (name == "ndel_data") ndel_data = std::atoi(value.c_str());
+        else if (name == "N_log_bins") N_log_bins = std::atoi(value.c_str());
+        else if (name == "SN_correction") SN_correction = to_bool(value);
+        else if (name == "lambdath") lambdath = (real_prec)std::atof(value.c_str());
+        else if (name == "smscale") smscale = (real_prec)std::atof(value.c_str());
+        else if (name == "Statistics") statistics = value;
+        else if (name == "Input_type") input_type = value;
+        else if (name == "Output_directory") Output_directory = value;
+        else if (name == "file_power") file_power = value;
+        else if (name == "Name_survey") Name_survey = value;
+        else if (name == "delta_grid_file") delta_grid_file = value;
+        else if (name == "Catalogue_file") file_catalogue = value;
+        else if (name == "Input_dir_cat") Input_dir_cat = value;
+        else if (name == "use_random_catalog") use_random_catalog = to_bool(value);
+        else if (name == "measure_cross") measure_cross = to_bool(value);
+        else if (name == "i_coord1_g") i_coord1_g = std::atoi(value.c_str());
+        else if (name == "i_coord2_g") i_coord2_g = std::atoi(value.c_str());
+        else if (name == "i_coord3_g") i_coord3_g = std::atoi(value.c_str());
+        else if (name == "i_weight1_g") i_weight1_g = std::atoi(value.c_str());
+        else if (name == "shell_rule") {
+            if (value != "exact" && value != "float") throw cmh::Error("shell_rule must be exact or float");
+            shell_rule = value;
+        } else if (name == "compute_tweb") compute_tweb = to_bool(value);
+        // unknown keys are ignored, like the reference
+    }
+}
+
+void Params::derived_pars() {
+    // Params.cpp:3531-3542
+    mass_assignment = mass_assignment_scheme == "NGP" ? 0 : mass_assignment_scheme == "CIC" ? 1
+                      : mass_assignment_scheme == "TSC" ? 2 : 3;
+    // Params.cpp:4079-4115
+    NGRID = Nft * Nft * Nft;
+    NGRID_h = Nft * Nft * (Nft / 2 + 1);
+    delta_x = (double)Lbox / (double)Nft;
+    deltak_x = 2.0 * M_PI / (double)Lbox;
+    deltak_0 = std::sqrt(3.0 * deltak_x * deltak_x) / std::sqrt(3.0);
+    kmin = DeltaKmin * deltak_0;
+    kmax = std::sqrt(3.0 * std::pow(0.5 * Nft * deltak_x, 2)) / std::sqrt(3.0);
+    DeltaK_data = ndel_data * deltak_0;
+    Nnp_data = type_of_binning == "log" ? N_log_bins : (int)(Nft / ndel_data / 2);
+    Deltal = std::log10(kmax / (kmin > 0 ? kmin : deltak_0)) / (double)N_log_bins;
+    Xoffset = Yoffset = Zoffset = Lbox / 2;  // Params.cpp:3612
+}
+
+std::string Params::get(const std::string &key, const std::string &def) const {
+    auto it = raw.find(key);
+    return it == raw.end() ? def : it->second;
+}
+
+void Params::show_params() const {
+    std::printf("Nft = %lu\nLbox = %.9g\nNGRID = %lu\nNGRID_h = %lu\nmass_assignment_scheme = %s\n"
+                "mass_assignment = %d\nMAS_correction = %s\ntype_of_binning = %s\nDeltaKmin = %.9g\n"
+                "ndel_data = %d\nSN_correction = %s\nlambdath = %.9g\nsmscale = %.9g\nStatistics = %s\n"
+                "Input_type = %s\nOutput_directory = %s\nfile_power = %s\nName_survey = %s\n"
+                "delta_x = %.17g\ndeltak_x = %.17g\ndeltak_0 = %.17g\nkmin = %.17g\nkmax = %.17g\n"
+                "DeltaK_data = %.17g\nNnp_data = %d\nshell_rule = %s\ncompute_tweb = %s\n",
+                Nft, (double)Lbox, NGRID, NGRID_h, mass_assignment_scheme.c_str(), mass_assignment,
+                MAS_correction ? "true" : "false", type_of_binning.c_str(), (double)DeltaKmin, ndel_data,
+                SN_correction ? "true" : "false", (double)lambdath, (double)smscale, statistics.c_str(),
+                input_type.c_str(), Output_directory.c_str(), file_power.c_str(), Name_survey.c_str(), delta_x,
+                deltak_x, deltak_0, kmin, kmax, DeltaK_data, Nnp_data, shell_rule.c_str(),
+                compute_tweb ? "true" : "false");
+}
+
+// =============================================================== mass assignment
+static void density_soa(int scheme, ULONG N, real_prec L, real_prec min1, real_prec min2, real_prec min3,
+                        const std::vector<real_prec> &xp, const std::vector<real_prec> &yp,
+                        const std::vector<real_prec> &zp, const std::vector<real_prec> &mass,
+                        std::vector<real_prec> &delta, bool weightmass) {
+    const size_t n = xp.size();
+    if (yp.size() != n || zp.size() != n) throw cmh::Error("getDensity: coordinate arrays differ in size");
+    if (weightmass && mass.size() != n) throw cmh::Error("getDensity: mass array size");
+    const size_t n3 = (size_t)N * N * N;
+    std::vector<real_prec> soa(3 * n);
+    std::copy(xp.begin(), xp.end(), soa.begin());
+    std::copy(yp.begin(), yp.end(), soa.begin() + n);
+    std::copy(zp.begin(), zp.end(), soa.begin() + 2 * n);
+    DevBuf dpos(sizeof(real_prec) * soa.size() + 4), dmesh(sizeof(real_prec) * n3);
+    cmh::h2d(dpos.p, soa.data(), sizeof(real_prec) * soa.size());
+    void *dw = nullptr;
+    DevBuf dmass(weightmass ? sizeof(real_prec) * n + 4 : 4);
+    if (weightmass) {
+        cmh::h2d(dmass.p, mass.data(), sizeof(real_prec) * n);
+        dw = dmass.p;
+    }
+    cm_mas_desc d;
+    d.N = (int)N;
+    d.L = L;
+    d.xmin[0] = min1;
+    d.xmin[1] = min2;
+    d.xmin[2] = min3;
+    d.layout = CM_SOA;
+    d.pos_dtype = CM_F32;
+    d.scheme = (cm_mas)scheme;
+    d.interlace_shift = 0.0;
+    d.algo = CM_MAS_AUTO;
+    check(cm_mesh_assign(ctx(), &d, dpos.p, dw, n, dmesh.p, CM_F32, 0), "cm_mesh_assign");
+    delta.resize(n3);
+    cmh::d2h(delta.data(), dmesh.p, sizeof(real_prec) * n3);
+}
+
+#define GETDENSITY(NAME, SCHEME)                                                                                \
+    void NAME(ULONG N1, ULONG N2, ULONG N3, real_prec L1, real_prec L2, real_prec L3, real_prec, real_prec,     \
+              real_prec, real_prec min1, real_prec min2, real_prec min3, const std::vector<real_prec> &xp,     \
+              const std::vector<real_prec> &yp, const std::vector<real_prec> &zp,                              \
+              const std::vector<real_prec> &Par_mass, std::vector<real_prec> &delta, bool weightmass) {        \
+        if (N1 != N2 || N2 != N3 || L1 != L2 || L2 != L3) throw cmh::Error(#NAME ": cubic meshes only");       \
+        density_soa(SCHEME, N1, L1, min1, min2, min3, xp, yp, zp, Par_mass, delta, weightmass);                \
+    }
+GETDENSITY(getDensity_NGP, 0)
+GETDENSITY(getDensity_CIC, 1)
+GETDENSITY(getDensity_TSC, 2)
+
+void grid_assignment(const Params &p, const std::vector<real_prec> &xyz, const std::vector<real_prec> &w,
+                     std::vector<real_prec> &field) {
+    const size_t n = xyz.size() / 3;
+    const size_t n3 = p._NGRID();
+    DevBuf dpos(sizeof(real_prec) * xyz.size() + 4), dmesh(sizeof(real_prec) * n3);
+    cmh::h2d(dpos.p, xyz.data(), sizeof(real_prec) * xyz.size());
+    DevBuf dw(w.empty() ? 4 : sizeof(real_prec) * w.size());
+    if (!w.empty()) cmh::h2d(dw.p, w.data(), sizeof(real_prec) * w.size());
+    cm_mas_desc d;
+    d.N = (int)p._Nft();
+    d.L = p._Lbox();
+    // x -= Xoffset - L/2 (Miscelanious.cpp:2604-2606)
+    d.xmin[0] = (double)p._Xoffset() - 0.5 * p._Lbox();
+    d.xmin[1] = (double)p._Yoffset() - 0.5 * p._Lbox();
+    d.xmin[2] = (double)p._Zoffset() - 0.5 * p._Lbox();
+    d.layout = CM_AOS;
+    d.pos_dtype = CM_F32;
+    d.scheme = (cm_mas)p._mass_assignment();
+    d.interlace_shift = 0.0;
+    d.algo = CM_MAS_AUTO;
+    check(cm_mesh_assign(ctx(), &d, dpos.p, w.empty() ? nullptr : dw.p, n, dmesh.p, CM_F32, 0), "cm_mesh_assign");
+    field.resize(n3);
+    cmh::d2h(field.data(), dmesh.p, sizeof(real_prec) * n3);
+}
+
+real_prec get_nobjects(const std::vector<real_prec> &f) {
+    DevBuf d(sizeof(real_prec) * f.size() + 4);
+    cmh::h2d(d.p, f.data(), sizeof(real_prec) * f.size());
+    double s = 0;
+    check(cm_mesh_sum(ctx(), d.p, CM_F32, f.size(), &s), "cm_mesh_sum");
+    return (real_prec)s;
+}
+
+// ======================================================================== FFT
+void do_fftw_r2c(ULONG Nft, std::vector<real_prec> &in, complex_prec *out) {
+    const size_t n3 = (size_t)Nft * Nft * Nft, nh = (size_t)Nft * Nft * (Nft / 2 + 1);
+    if (in.size() < n3) throw cmh::Error("do_fftw_r2c: input smaller than Nft^3");
+    DevBuf din(sizeof(real_prec) * n3), dout(sizeof(complex_prec) * nh);
+    cmh::h2d(din.p, in.data(), sizeof(real_prec) * n3);
+    check(cm_fft_r2c(ctx(), (int)Nft, CM_F32, din.p, dout.p, 0.0), "cm_fft_r2c");
+    cmh::d2h(out, dout.p, sizeof(complex_prec) * nh);
+}
+
+void do_fftw_c2r(int Nft, complex_prec *in, std::vector<real_prec> &out) {
+    const size_t n3 = (size_t)Nft * Nft * Nft, nh = (size_t)Nft * Nft * (Nft / 2 + 1);
+    DevBuf din(sizeof(complex_prec) * nh), dout(sizeof(real_prec) * n3);
+    cmh::h2d(din.p, in, sizeof(complex_prec) * nh);
+    check(cm_fft_c2r(ctx(), Nft, CM_F32, din.p, dout.p), "cm_fft_c2r");
+    out.resize(n3);
+    cmh::d2h(out.data(), dout.p, sizeof(real_prec) * n3);
+}
+
+// ======================================================================= T-web
+void PoissonSolver(real_prec Lbox, ULONG Nft, std::vector<real_prec> &in, std::vector<real_prec> &out) {
+    const size_t n3 = (size_t)Nft * Nft * Nft;
+    DevBuf din(sizeof(real_prec) * n3), dout(sizeof(real_prec) * n3);
+    cmh::h2d(din.p, in.data(), sizeof(real_prec) * n3);
+    cm_tweb_opts o{(int)Nft, (double)Lbox, 0.0, 0.0, 0.0, 1.0};
+    check(cm_poisson(ctx(), &o, (const float *)din.p, (float *)dout.p), "cm_poisson");
+    out.resize(n3);
+    cmh::d2h(out.data(), dout.p, sizeof(real_prec) * n3);
+}
+
+void EigenValuesTweb(ULONG Nft, real_prec L1, std::vector<real_prec> &, std::vector<real_prec> &phi,
+                     std::vector<real_prec> &out1, std::vector<real_prec> &out2, std::vector<real_prec> &out3) {
+    const size_t n3 = (size_t)Nft * Nft * Nft;
+    DevBuf dphi(sizeof(real_prec) * n3), dlab(n3), dlam(3 * sizeof(real_prec) * n3);
+    cmh::h2d(dphi.p, phi.data(), sizeof(real_prec) * n3);
+    cm_tweb_opts o{(int)Nft, (double)L1, 0.0, 0.0, 0.0, 1.0};
+    check(cm_tweb_from_phi(ctx(), &o, (const float *)dphi.p, (int8_t *)dlab.p, (float *)dlam.p, nullptr),
+          "cm_tweb_from_phi");
+    out1.resize(n3);
+    out2.resize(n3);
+    out3.resize(n3);
+    cmh::d2h(out1.data(), dlam.p, sizeof(real_prec) * n3);
+    cmh::d2h(out2.data(), (char *)dlam.p + sizeof(real_prec) * n3, sizeof(real_prec) * n3);
+    cmh::d2h(out3.data(), (char *)dlam.p + 2 * sizeof(real_prec) * n3, sizeof(real_prec) * n3);
+}
+
+void Cwclass::get_CWC(std::vector<real_prec> &delta) {
+    const ULONG N = params._Nft();
+    const size_t n3 = (size_t)N * N * N;
+    DevBuf dd(sizeof(real_prec) * n3), dlab(n3), dlam(3 * sizeof(real_prec) * n3);
+    cmh::h2d(dd.p, delta.data(), sizeof(real_prec) * n3);
+    cm_tweb_opts o{(int)N, (double)params._Lbox(), (double)params._smscale(), (double)params._lambdath(), 1e-6,
+                   1.0};
+    cm_tweb_stats st;
+    check(cm_tweb(ctx(), &o, (const float *)dd.p, (int8_t *)dlab.p, (float *)dlam.p, &st), "cm_tweb");
+    std::vector<int8_t> lab(n3);
+    cmh::d2h(lab.data(), dlab.p, n3);
+    CWClass.assign(lab.begin(), lab.end());
+    lambda1.resize(n3);
+    lambda2.resize(n3);
+    lambda3.resize(n3);
+    cmh::d2h(lambda1.data(), dlam.p, sizeof(real_prec) * n3);
+    cmh::d2h(lambda2.data(), (char *)dlam.p + sizeof(real_prec) * n3, sizeof(real_prec) * n3);
+    cmh::d2h(lambda3.data(), (char *)dlam.p + 2 * sizeof(real_prec) * n3, sizeof(real_prec) * n3);
+    // volume fractions (Cwclass.cpp:891-894; the reference swaps the sheet and
+    // filament names at :987-988 -- reported correctly here)
+    volume_knots = (real_prec)st.counts[0] / (real_prec)n3;
+    volume_filaments = (real_prec)st.counts[1] / (real_prec)n3;
+    volume_sheets = (real_prec)st.counts[2] / (real_prec)n3;
+    volume_voids = (real_prec)st.counts[3] / (real_prec)n3;
+    n_marginal = st.n_marginal;
+}
+
+// =============================================================== FftwFunctions
+FftwFunctions::~FftwFunctions() { free_fftw_vectors(); }
+
+void FftwFunctions::resize_fftw_vectors() {
+    free_fftw_vectors();
+    data_g.assign(params._NGRID(), 0);
+    data_out_g = new complex_prec[params._NGRID_h()];
+}
+
+void FftwFunctions::free_fftw_vectors() {
+    delete[] data_out_g;
+    data_out_g = nullptr;
+}
+
+// FftwFunctions.cpp:159-174
+void FftwFunctions::raw_sampling(real_prec vol) {
+    normal_p = n_gal * n_gal / vol;
+    s_r = vol / n_gal;
+    alpha = 1.0;
+    normal_power = normal_p;
+    shot_noise = s_r;
+}
+
+// FftwFunctions.cpp:177-212 (box branch; randoms are not part of this path)
+void FftwFunctions::get_parameters_estimator(bool verbose) {
+    if (params._use_random_catalog())
+        throw cmh::Error("use_random_catalog: the FKP-with-randoms estimator is not part of this build");
+    alpha = 1.0;
+    shot_noise = s_r;
+    if (verbose)
+        std::printf("** Ngal = %.9g\n** Normalization = %.9g\n** Shot Noise = %.9g\n", (double)n_gal,
+                    (double)normal_power, (double)shot_noise);
+}
+
+// FftwFunctions.cpp:293-294
+void FftwFunctions::get_fluctuation() {
+    const real_prec m = n_gal / (real_prec)params._NGRID();
+    for (auto &v : data_g) v -= m;
+}
+
+void FftwFunctions::shells(std::vector<real_prec> &p0, std::vector<real_prec> *p2, std::vector<real_prec> *p4,
+                           std::vector<int> &mod) {
+    const ULONG N = params._Nft();
+    const size_t n3 = (size_t)N * N * N, nh = params._NGRID_h();
+    DevBuf din(sizeof(real_prec) * n3), dk(sizeof(complex_prec) * nh);
+    cmh::h2d(din.p, data_g.data(), sizeof(real_prec) * n3);
+    check(cm_fft_r2c(ctx(), (int)N, CM_F32, din.p, dk.p, 0.0), "cm_fft_r2c");
+    if (data_out_g) cmh::d2h(data_out_g, dk.p, sizeof(complex_prec) * nh);
+    const int ns = params._d_Nnp_data();
+    if (params._type_of_binning() != "linear") throw cmh::Error("log binning is not part of this build");
+    cm_pk_opts o;
+    o.N = (int)N;
+    o.L = params._Lbox();
+    o.nshells = ns;
+    o.kmin = params._d_kmin();
+    o.dk = params._d_DeltaK_data();
+    o.scheme = (cm_mas)params._mass_assignment();
+    o.mas_correction = params._MAS_correction();
+    o.shot_noise = params._SN_correction() ? shot_noise : 0.0;
+    o.normal_power = normal_power;
+    o.float_shell_rule = params._float_shell_rule();
+    std::vector<double> a0(ns), a2(ns), a4(ns);
+    std::vector<int64_t> nm(ns);
+    check(cm_pk_shells(ctx(), &o, CM_F32, dk.p, a0.data(), a2.data(), a4.data(), nm.data()), "cm_pk_shells");
+    p0.assign(a0.begin(), a0.end());
+    if (p2) p2->assign(a2.begin(), a2.end());
+    if (p4) p4->assign(a4.begin(), a4.end());
+    mod.assign(nm.begin(), nm.end());
+}
+
+void FftwFunctions::get_power_spectrum_fkp(std::vector<real_prec> &power_g0, std::vector<real_prec> &power_g2,
+                                           std::vector<real_prec> &power_g4, std::vector<int> &mod_g) {
+    shells(power_g0, &power_g2, &power_g4, mod_g);
+}
+
+void FftwFunctions::power_spectrum(std::vector<real_prec> &powerk_g0, std::vector<int> &mod_g) {
+    shells(powerk_g0, nullptr, nullptr, mod_g);
+}
+
+// ================================================================== file I/O
+void read_array(const std::string &fname, std::vector<real_prec> &out) {
+    std::ifstream f(fname.c_str(), std::ios::binary);
+    if (!f) throw cmh::Error("File " + fname + " not found");
+    // sized from the mesh, not read to EOF (the reference's read_array_fast
+    // pushes one extra element, FileOutput.cpp:756-778)
+    f.read(reinterpret_cast<char *>(out.data()), (std::streamsize)(sizeof(real_prec) * out.size()));
+    if ((size_t)f.gcount() != sizeof(real_prec) * out.size())
+        throw cmh::Error("File " + fname + " holds fewer than Nft^3 float32 values");
+}
+
+void write_array(const std::string &fname, const std::vector<real_prec> &in) {
+    std::ofstream f((fname + ".dat").c_str(), std::ios::binary);
+    if (!f) throw cmh::Error("cannot write " + fname + ".dat");
+    f.write(reinterpret_cast<const char *>(in.data()), (std::streamsize)(sizeof(real_prec) * in.size()));
+}
+
+void write_to_file2(const std::string &fname, const std::vector<real_prec> &kve, const std::vector<real_prec> &pk,
+                    const std::vector<int> &si, bool cero) {
+    std::ofstream out(fname.c_str());
+    if (!out) throw cmh::Error("cannot write " + fname);
+    size_t first = (cero && !pk.empty() && pk[0] < 1) ? 1 : 0;
+    for (size_t i = first; i < kve.size(); ++i) out << kve[i] << "\t" << pk[i] << "\t" << si[i] << "\n";
+}
+
+void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w) {
+    std::ifstream f(p._file_catalogue().c_str());
+    if (!f) throw cmh::Error("File " + p._file_catalogue() + " not found");
+    const int c[3] = {p._i_coord1_g(), p._i_coord2_g(), p._i_coord3_g()};
+    const int cw = p._i_weight1_g();
+    int need = std::max(std::max(c[0], c[1]), std::max(c[2], cw)) + 1;
+    std::string line;
+    std::vector<double> col;
+    xyz.clear();
+    w.clear();
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::stringstream ls(line);
+        col.clear();
+        double v;
+        while (ls >> v) col.push_back(v);
+        if ((int)col.size() < need) continue;
+        for (int a = 0; a < 3; ++a) xyz.push_back((real_prec)col[c[a]]);
+        if (cw >= 0) w.push_back((real_prec)col[cw]);
+    }
+}
+
+// ============================================================== PowerSpectrumF
+PowerSpectrumF::PowerSpectrumF(const Params &p) : fftw_functions(p), cwclass(p), params(p) {
+    set_output_filenames();
+}
+
+// PowerSpectrumF.cpp:15-46 (real-space file name)
+void PowerSpectrumF::set_output_filenames() {
+    const std::string stem = params._Output_directory() + params._statistics() + "_" + params._Name_survey() +
+                             "_Nft" + std::to_string(params._Nft()) + "_" + params._mass_assignment_scheme() + "_" +
+                             params._file_power();
+    file_power = stem + ".txt";
+    file_cwc = params._Output_directory() + "CWC_" + params._Name_survey() + "_Nft" + std::to_string(params._Nft()) +
+               "_" + params._mass_assignment_scheme() + "_" + params._file_power();
+}
+
+void PowerSpectrumF::kvector() {
+    // PowerSpectrumF.cpp:1266-1272 / 4259-4264: bin centres kmin + dk (i+1/2)
+    kvector_data.resize(params._d_Nnp_data());
+    for (int i = 0; i < params._d_Nnp_data(); ++i)
+        kvector_data[i] = (real_prec)(params._d_kmin() + params._d_DeltaK_data() * (i + 0.5));
+}
+
+// -p with Input_type=catalog (PowerSpectrumF.cpp:741-1363, box, no randoms)
+void PowerSpectrumF::compute_power_spectrum(bool verbose, bool) {
+    std::vector<real_prec> xyz, w;
+    read_catalog(params, xyz, w);
+    const size_t n = xyz.size() / 3;
+    if (n == 0) throw cmh::Error("the catalogue " + params._file_catalogue() + " holds no objects");
+    fftw_functions.resize_fftw_vectors();
+    grid_assignment(params, xyz, w, fftw_functions.data_g);
+    double ngal = 0;
+    if (w.empty()) ngal = (double)n;
+    else
+        for (real_prec v : w) ngal += v;
+    fftw_functions.set_n_gal((real_prec)ngal);
+    fftw_functions.raw_sampling(params._Lbox() * params._Lbox() * params._Lbox());
+    fftw_functions.get_parameters_estimator(verbose);
+    std::vector<real_prec> mesh = fftw_functions.data_g;  // kept for the T-web
+    fftw_functions.get_fluctuation();
+    kvector();
+    fftw_functions.get_power_spectrum_fkp(pk0, pk2, pk4, modes_g);
+    write_power_and_modes();
+    if (params._compute_tweb()) {
+        const real_prec mean = (real_prec)(ngal / (double)params._NGRID());
+        for (auto &v : mesh) v = v / mean - 1;
+        cwclass.get_CWC(mesh);
+        std::vector<real_prec> lab(cwclass.CWClass.begin(), cwclass.CWClass.end());
+        write_array(file_cwc, lab);
+        if (verbose)
+            std::printf("T-web volume fractions: knots %.6f filaments %.6f sheets %.6f voids %.6f\n",
+                        (double)cwclass.volume_knots, (double)cwclass.volume_filaments,
+                        (double)cwclass.volume_sheets, (double)cwclass.volume_voids);
+    }
+}
+
+// -p with Input_type=density_grid / delta_grid (PowerSpectrumF.cpp:4188-4290)
+void PowerSpectrumF::compute_power_spectrum_grid() {
+    std::vector<real_prec> data_in(params._NGRID(), 0);
+    read_array(params._delta_grid_file(), data_in);
+    fftw_functions.resize_fftw_vectors();
+    const real_prec ngal = get_nobjects(data_in);
+    if (params._input_type() == "density_grid") {
+        const real_prec nmean = ngal / (real_prec)params._NGRID();
+        for (size_t i = 0; i < data_in.size(); ++i) fftw_functions.data_g[i] = data_in[i] / nmean - 1;
+    } else {
+        fftw_functions.data_g = data_in;
+    }
+    fftw_functions.set_n_gal(ngal);
+    const double factor = std::pow((double)params._Lbox(), 1.5) / (double)params._NGRID();
+    fftw_functions.set_normal_power((real_prec)std::pow(factor, -2));
+    fftw_functions.shot_noise = 0;
+    if (params._SN_correction())
+        fftw_functions.shot_noise = (real_prec)(std::pow((double)params._Lbox(), 3) / (double)ngal);
+    kvector();
+    fftw_functions.power_spectrum(pk0, modes_g);
+    write_power_and_modes();
+}
+
+void PowerSpectrumF::write_power_and_modes() { write_to_file2(file_power, kvector_data, pk0, modes_g, true); }

@@ -1,0 +1,223 @@
+/*
+ * cosmo_mesh.h -- C ABI of the MI355X (gfx950) mesh-measurement library
+ * libcosmo_mesh.so: particle->mesh mass assignment, 3-D r2c/c2r FFT, the P(k)
+ * shell estimator and the T-web classification.
+ *
+ * Plain C: pointers, sizes and PODs only.  Every compute entry point takes
+ * DEVICE pointers (allocate them with cm_dev_alloc or any HIP allocator) and
+ * runs on the context's stream; small results (shell sums, counts) come back to
+ * host memory and those calls synchronise that stream.  Errors are reported as
+ * cm_status codes with a message in cm_last_error(); nothing exits the process.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to
+ * the COSMICATLAS source tree).
+ */
+#ifndef COSMO_MESH_H
+#define COSMO_MESH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CM_OK = 0,
+    CM_ERR_INVALID = 1,     /* bad argument (null pointer, N not supported, ...) */
+    CM_ERR_UNSUPPORTED = 2, /* valid request that this build does not implement */
+    CM_ERR_NOMEM = 3,       /* device allocation failed */
+    CM_ERR_DEVICE = 4,      /* HIP runtime error */
+    CM_ERR_BUDGET = 5,      /* request larger than the context's HBM budget */
+    CM_ERR_TIMEOUT = 6,     /* serving: request not finished before its deadline */
+    CM_ERR_CANCELLED = 7    /* serving: request cancelled */
+} cm_status;
+
+typedef enum { CM_F32 = 0, CM_F64 = 1 } cm_dtype;
+
+/* Mass-assignment schemes numbered like Params::mass_assignment
+ * (Source/Params.cpp:3531-3542). */
+typedef enum { CM_NGP = 0, CM_CIC = 1, CM_TSC = 2, CM_PCS = 3 } cm_mas;
+
+typedef enum { CM_AOS = 0, CM_SOA = 1 } cm_layout;
+
+/* Deposit algorithm: plain = one thread per particle with global atomics;
+ * tiled = particles binned to 16^3-cell tiles (with ghost copies for the
+ * stencil halo) and accumulated in LDS, each tile written once. */
+typedef enum { CM_MAS_AUTO = 0, CM_MAS_PLAIN = 1, CM_MAS_TILED = 2 } cm_mas_algo;
+
+typedef struct cm_ctx cm_ctx;
+
+/* ---------------------------------------------------------------- context */
+/* Opens a context on HIP device `device`.  hbm_budget (bytes, 0 = 90% of the
+ * device) bounds the scratch the context may keep resident.  No reference
+ * counterpart: the reference holds state in Params/FftwFunctions objects
+ * (Headers/FftwFunctions.h:1083, Source/FftwFunctions.cpp:28-156). */
+cm_status cm_ctx_create(int device, size_t hbm_budget, cm_ctx **out);
+void cm_ctx_destroy(cm_ctx *ctx);
+const char *cm_last_error(const cm_ctx *ctx);
+const char *cm_status_string(cm_status s);
+/* Run subsequent work on `stream` (a hipStream_t; NULL = the context's own). */
+cm_status cm_set_stream(cm_ctx *ctx, void *stream);
+void *cm_get_stream(cm_ctx *ctx);
+cm_status cm_synchronize(cm_ctx *ctx);
+
+/* Device memory and copies for hosts that do not include HIP headers. */
+cm_status cm_dev_alloc(cm_ctx *ctx, size_t bytes, void **ptr);
+cm_status cm_dev_free(cm_ctx *ctx, void *ptr);
+cm_status cm_memset(cm_ctx *ctx, void *dptr, int value, size_t bytes);
+cm_status cm_copy_h2d(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
+cm_status cm_copy_d2h(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* Per-kernel timing with HIP events recorded on the stream each kernel is
+ * launched on.  While enabled, every launch of the named kernels is bracketed
+ * by events; cm_kernel_time returns the mean duration (ms) and count since the
+ * last reset. */
+cm_status cm_profile_enable(cm_ctx *ctx, int on);
+cm_status cm_kernel_time(cm_ctx *ctx, const char *kernel, double *mean_ms, int64_t *count);
+cm_status cm_profile_reset(cm_ctx *ctx);
+
+/* ------------------------------------------------------------ mass assignment */
+typedef struct {
+    int N;                  /* Nft: cells per side */
+    double L;               /* Lbox */
+    double xmin[3];         /* box corner, Xoffset - Lbox/2 (Source/Params.cpp:3612) */
+    cm_layout layout;       /* AoS xyz or SoA x[n],y[n],z[n] */
+    cm_dtype pos_dtype;     /* float32 / float64 positions and weights */
+    cm_mas scheme;
+    double interlace_shift; /* cells added to every coordinate (0 or 0.5) */
+    cm_mas_algo algo;
+} cm_mas_desc;
+
+/* Replaces grid_assignment_{NGP,CIC,TSC,PCS} over a catalogue
+ * (Source/Miscelanious.cpp:2597-2868, driven by
+ * Catalog::get_interpolated_density_field, Source/Catalog.cpp:14186-14211) and
+ * getDensity_{NGP,CIC,TSC} (Source/Miscelanious.cpp:65-147,454-557,693-815).
+ * pos/w/mesh are device pointers; w may be NULL (unit weights).  The mesh
+ * (N^3, C order, z fastest) is overwritten unless `accumulate`.  Positions are
+ * wrapped periodically in their own dtype; the cell index is
+ * floor(double(x)/double(L/N)) as in get_bin (Miscelanious.cpp:1447-1457).
+ * NGP with unit weights gives exact integer counts. */
+cm_status cm_mesh_assign(cm_ctx *ctx, const cm_mas_desc *desc, const void *pos,
+                         const void *w, size_t n, void *mesh, cm_dtype mesh_dtype,
+                         int accumulate);
+
+/* FP64 sum of a mesh: replaces get_nobjects (Source/Miscelanious.cpp:1421-1426),
+ * which sums in float. */
+cm_status cm_mesh_sum(cm_ctx *ctx, const void *mesh, cm_dtype dtype, size_t n, double *sum);
+
+/* mesh[i] = mesh[i]*scale + shift: FftwFunctions::get_fluctuation box branch
+ * (Source/FftwFunctions.cpp:293-294, shift = -N_gal/NGRID) and delta=rho/mean-1. */
+cm_status cm_mesh_affine(cm_ctx *ctx, void *mesh, cm_dtype dtype, size_t n, double scale,
+                         double shift);
+
+/* ---------------------------------------------------------------------- FFT */
+/* Sizes supported: N = 2^a * 3^b with b <= 1, N a multiple of 8, 8 <= N <= 1024. */
+int cm_fft_supported(int N);
+
+/* Replaces do_fftw_r2c (Source/NumericalMethods.cpp:1718-1753): unnormalised
+ * forward transform e^{-ik.x} of a real N^3 mesh into the FFTW half spectrum
+ * N x N x (N/2+1) interleaved (re,im).  `in` is not modified.  `subtract` is
+ * removed from every input value on load (0 for a plain transform). */
+cm_status cm_fft_r2c(cm_ctx *ctx, int N, cm_dtype dtype, const void *in, void *out,
+                     double subtract);
+
+/* Replaces do_fftw_c2r (Source/NumericalMethods.cpp:1818-1856): inverse of a
+ * Hermitian half spectrum, divided by N^3.  `in` is used as scratch and is
+ * overwritten (FFTW's c2r also destroys its input). */
+cm_status cm_fft_c2r(cm_ctx *ctx, int N, cm_dtype dtype, void *in, void *out);
+
+/* ---------------------------------------------------------------- P(k) shells */
+typedef struct {
+    int N;                /* Nft */
+    double L;             /* Lbox */
+    int nshells;          /* Nnp_data (Source/Params.cpp:4104) */
+    double kmin;          /* DeltaKmin*deltak_0 (Source/Params.cpp:4099) */
+    double dk;            /* ndel_data*deltak_0 (Source/Params.cpp:4101) */
+    cm_mas scheme;        /* mass_assignment */
+    int mas_correction;   /* MAS_correction */
+    double shot_noise;    /* SN = V/N_gal, 0 to switch off (SN_correction) */
+    double normal_power;  /* N_gal^2/V (FftwFunctions::raw_sampling, :159-174) */
+    int float_shell_rule; /* 1 = the reference's float32 shell index
+                             (FftwFunctions.cpp:1097); 0 = exact integer rule */
+} cm_pk_opts;
+
+/* Replaces FftwFunctions::power_spectrum_fkp box branch
+ * (Source/FftwFunctions.cpp:958-1657): octant enumeration, |k| shells, aliased
+ * shot noise and MAS deconvolution, normalisation; monopole pk0 and the z-LOS
+ * multipoles pk2, pk4.  dk is a device half spectrum (cm_dtype); outputs are
+ * host arrays of nshells (any may be NULL).  Shell sums are FP64; mode counts
+ * exact. */
+cm_status cm_pk_shells(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dtype, const void *dk,
+                       double *pk0, double *pk2, double *pk4, int64_t *nmodes);
+
+/* ---------------------------------------------------------------------- T-web */
+typedef struct {
+    int N;
+    double L;
+    double R_smooth;     /* Gaussian smoothing radius (smscale, Params.cpp:572); <=0: none */
+    double lambda_th;    /* lambdath (Params.cpp:552) */
+    double margin_tol;   /* cells with min_j |lambda_j - lambda_th| < tol are counted */
+    double delta_scale;  /* input multiplied by this before the Poisson solve (1/mean for rho-mean) */
+} cm_tweb_opts;
+
+typedef struct {
+    int64_t counts[4];   /* cells labelled knot, filament, sheet, void (1..4) */
+    int64_t n_marginal;  /* cells whose eigenvalue margin is below margin_tol */
+} cm_tweb_stats;
+
+/* Replaces Cwclass::get_CWC (Source/Cwclass.cpp:235-894) with the Gaussian
+ * pre-smoothing of kernelcomp (Source/NumericalMethods.cpp:2197-2306):
+ * delta (device N^3 float32) -> r2c -> W(k)(-1/k^2) (PoissonSolver,
+ * Source/Miscelanious.cpp:1189-1274) -> c2r -> 4th-order FD Hessian
+ * (gradfindif o gradfindif, EigenValuesTweb FD branch,
+ * Source/Miscelanious.cpp:1668-1769) -> FP64 3x3 eigenvalues -> label
+ * 4 - #{lambda_j > lambda_th}.  labels: device int8 N^3; lambdas: device float32
+ * 3*N^3 (planes l1|l2|l3, descending) or NULL; stats: host, may be NULL. */
+cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t *labels,
+                  float *lambdas, cm_tweb_stats *stats);
+
+/* Replaces PoissonSolver (Source/Miscelanious.cpp:1189-1274) with optional
+ * Gaussian smoothing: phi = c2r(W(k) * (-1/k^2) * r2c(delta*delta_scale)). */
+cm_status cm_poisson(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi);
+
+/* Replaces EigenValuesTweb FD branch + the label loop
+ * (Source/Miscelanious.cpp:1668-1769, Source/Cwclass.cpp:874-894) for a given
+ * potential phi (device float32 N^3). */
+cm_status cm_tweb_from_phi(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi,
+                           int8_t *labels, float *lambdas, cm_tweb_stats *stats);
+
+/* ------------------------------------------------- whole-mock measurement */
+typedef struct {
+    cm_mas_desc mas;     /* positions -> mesh */
+    int nshells;
+    double kmin, dk;     /* binning, as in cm_pk_opts */
+    int mas_correction;
+    int sn_correction;
+    int float_shell_rule;
+    int do_tweb;         /* 0: P(k) only */
+    double R_smooth, lambda_th, margin_tol;
+    int fused;           /* 1: fused pipeline; 0: the plain step-by-step GPU path */
+} cm_measure_opts;
+
+typedef struct {
+    double n_obj;        /* FP64 mesh sum (N_gal) */
+    double shot_noise, normal_power;
+    cm_tweb_stats tweb;
+} cm_measure_info;
+
+/* One mock: the -p workflow for Input_type=catalog (PowerSpectrumF::
+ * compute_power_spectrum, Source/PowerSpectrumF.cpp:741-1363) followed by the
+ * T-web of delta = rho/mean - 1 (Bam's cwclass.do_CWC call,
+ * Source/Bam.cpp:2210-2283).  pos (and w) are device arrays of n particles;
+ * outputs: host pk0/pk2/pk4/nmodes[nshells] (may be NULL), device int8 labels
+ * N^3 (NULL unless do_tweb).  The context keeps the meshes resident between
+ * calls of the same N. */
+cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w,
+                     size_t n, double *pk0, double *pk2, double *pk4, int64_t *nmodes,
+                     int8_t *labels, cm_measure_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,40 @@
+"""ctypes front end of libcpuref.so, the OpenMP CPU baseline (test/bench
+infrastructure only; see cpu_ref.c)."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import HERE, build
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(HERE, "libcpuref.so")
+        if not os.path.exists(so):
+            build()
+        L = C.CDLL(so)
+        L.cr_time_pipeline.argtypes = [C.c_int, C.c_float, C.c_void_p, C.c_size_t, C.c_int, C.c_float,
+                                       C.c_float, C.c_float, C.c_int, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def time_pipeline(pos, N, L, scheme=2, ndel=1, R=2.0, lambda_th=0.0, threads=0, repeats=1):
+    """Seconds per stage (mas, fft, pk, tweb, tweb_fft) of the TSC pipeline at N^3."""
+    assert scheme == 2, "the CPU baseline restates the TSC path"
+    pos = np.ascontiguousarray(pos, np.float32)
+    kf = 2 * math.pi / L
+    ns = N // ndel // 2
+    best = None
+    for _ in range(repeats):
+        out = np.zeros(5)
+        lib().cr_time_pipeline(N, L, pos.ctypes.data_as(C.c_void_p), len(pos), ns, ndel * kf, R, lambda_th,
+                               threads, out.ctypes.data_as(C.c_void_p))
+        if best is None or out[:4].sum() < best[:4].sum():
+            best = out
+    return {"mas": best[0], "fft": best[1], "pk": best[2], "tweb": best[3], "tweb_fft": best[4]}

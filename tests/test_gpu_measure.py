@@ -1,0 +1,88 @@
+"""Whole-mock measurement (particles -> P(k) + T-web) on the GPU against the
+oracle pipeline, plain and fused paths."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import cosmicatlas_amd as cm
+    c = cm.Context(0)
+    yield c
+    c.close()
+
+
+def _oracle_pk(orc, pos, N, L, scheme, b, mas_correction=True):
+    mesh = orc.mas(pos, N, L, scheme)
+    nobj = mesh.sum()
+    V = L ** 3
+    dk = orc.fft_r2c(mesh - nobj / N ** 3)
+    o = orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, V / nobj, nobj ** 2 / V,
+                    mas_correction=mas_correction)
+    p0, p2, p4, nm = orc.pk_shells(dk, o)
+    return p0, p2, p4, nm, mesh, nobj, V / nobj
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_measure_zeldovich_pk(ctx, orc, fused, scheme):
+    import cosmicatlas_amd as cm
+    from cosmicatlas_amd import synth
+    NL, N, L = 32, 64, 250.0
+    pos = synth.zeldovich(NL, L, A=2e3, seed=11)
+    b = cm.binning(N, L, ndel=1, DeltaKmin=0.0)
+    out = ctx.measure(torch.from_numpy(pos).cuda(), N, L, scheme, b["nshells"], b["kmin"], b["dk"], fused=fused)
+    p0, p2, p4, nm, _, nobj, SN = _oracle_pk(orc, pos, N, L, scheme, b)
+    assert np.array_equal(out["nmodes"], nm)
+    assert out["n_obj"] == pytest.approx(nobj, rel=1e-6)
+    # error relative to the un-subtracted power (P + SN*Sq/W^2 >= P): 1e-4
+    denom = np.abs(p0) + SN
+    assert np.max(np.abs(out["pk0"] - p0) / denom) < 1e-4
+    assert np.max(np.abs(out["pk2"] - p2) / denom) < 1e-4
+
+
+def test_measure_tweb_fused_equals_plain_and_oracle(ctx, orc):
+    import cosmicatlas_amd as cm
+    from cosmicatlas_amd import synth
+    NL, N, L, R = 32, 32, 200.0, 12.0
+    pos = synth.zeldovich(NL, L, A=2e3, seed=3)
+    b = cm.binning(N, L)
+    posd = torch.from_numpy(pos).cuda()
+    kw = dict(do_tweb=True, R_smooth=R, lambda_th=0.0, margin_tol=1e-4)
+    a = ctx.measure(posd, N, L, 2, b["nshells"], b["kmin"], b["dk"], fused=False, **kw)
+    la = a["labels"].cpu().numpy()
+    c = ctx.measure(posd, N, L, 2, b["nshells"], b["kmin"], b["dk"], fused=True, **kw)
+    lc = c["labels"].cpu().numpy()
+    mesh = orc.mas(pos, N, L, orc.TSC)
+    mean = mesh.sum() / N ** 3
+    lab_ref, lam_ref, _, _ = orc.tweb(mesh / mean - 1.0, L, R, 0.0)
+    ok = np.abs(lam_ref).min(axis=0) > 1e-4 * np.abs(lam_ref).max()
+    assert np.array_equal(la[ok], lab_ref[ok])
+    assert np.array_equal(lc[ok], lab_ref[ok])
+    assert sum(c["tweb_counts"]) == N ** 3
+
+
+def test_c1_plain_pipeline_f64(ctx, orc):
+    """Config 1: 200,000 float64 uniform positions, L=500, CIC on a 128^3 float64
+    mesh, r2c, 64 shells with DeltaKmin=1; NGP counts bit-exact."""
+    import cosmicatlas_amd as cm
+    N, L, n = 128, 500.0, 200000
+    pos = np.random.default_rng(1).uniform(0, L, (n, 3))
+    posd = torch.from_numpy(pos).cuda()
+    ngp = ctx.mesh_assign(posd, N, L, "NGP", mesh_dtype=torch.float64).cpu().numpy()
+    assert np.array_equal(ngp, orc.mas(pos, N, L, orc.NGP))
+    mesh = ctx.mesh_assign(posd, N, L, "CIC", mesh_dtype=torch.float64)
+    nobj = ctx.mesh_sum(mesh)
+    dk = ctx.fft_r2c(mesh, subtract=nobj / N ** 3)
+    b = cm.binning(N, L, ndel=1, DeltaKmin=1.0)
+    V = L ** 3
+    p0, p2, p4, nm = ctx.pk_shells(dk, N, L, b["nshells"], b["kmin"], b["dk"], 1, V / nobj, nobj ** 2 / V)
+    ref_mesh = orc.mas(pos, N, L, orc.CIC)
+    rdk = orc.fft_r2c(ref_mesh - ref_mesh.sum() / N ** 3)
+    o = orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], 1, V / ref_mesh.sum(), ref_mesh.sum() ** 2 / V)
+    r0, _, _, rn = orc.pk_shells(rdk, o)
+    assert np.array_equal(nm, rn)
+    assert np.max(np.abs(p0 - r0) / (np.abs(r0) + V / nobj)) < 1e-10

@@ -1,0 +1,68 @@
+"""GPU P(k) shells against the oracle estimator on the same spectrum: mode
+counts bit-exact (exact and float32 rules), sums to FP64 round-off."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import cosmicatlas_amd as cm
+    c = cm.Context(0)
+    yield c
+    c.close()
+
+
+def _spectrum(N, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+
+
+@pytest.mark.parametrize("N,ndel,dmin,scheme", [(16, 1, 0, 0), (32, 1, 1, 1), (64, 2, 0, 2), (48, 3, 0, 3),
+                                                (128, 1, 0, 1), (96, 1, 1, 2)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_shells_match_oracle(ctx, orc, N, ndel, dmin, scheme, dtype):
+    L = 500.0
+    kf = 2 * np.pi / L
+    ns = N // ndel // 2
+    dk = _spectrum(N, N + ndel)
+    SN, normal = 2.5, 3.0
+    o = orc.pk_opts(N, L, ns, dmin * kf, ndel * kf, scheme, SN, normal)
+    r0, r2, r4, rn = orc.pk_shells(dk, o)
+    dkt = torch.from_numpy(dk).to(dtype).cuda()
+    p0, p2, p4, nm = ctx.pk_shells(dkt, N, L, ns, dmin * kf, ndel * kf, scheme, SN, normal)
+    assert np.array_equal(nm, rn)
+    tol = 1e-11 if dtype == torch.complex128 else 2e-6
+    scale = np.abs(r0).max()
+    assert np.max(np.abs(p0 - r0)) < tol * scale
+    assert np.max(np.abs(p2 - r2)) < tol * scale
+    assert np.max(np.abs(p4 - r4)) < tol * scale
+
+
+@pytest.mark.parametrize("N,L,ndel", [(128, 500.0, 1), (64, 1000.0, 2)])
+def test_float_shell_rule_counts(ctx, orc, N, L, ndel):
+    kf = 2 * np.pi / L
+    ns = N // ndel // 2
+    ones = torch.ones((N, N, N // 2 + 1), dtype=torch.complex64, device="cuda")
+    for rule in (False, True):
+        o = orc.pk_opts(N, L, ns, 0.0, ndel * kf, 1, 0.0, 1.0, float_shell_rule=rule)
+        rn = orc.pk_shells(np.ones((N, N, N // 2 + 1), complex), o)[3]
+        nm = ctx.pk_shells(ones, N, L, ns, 0.0, ndel * kf, 1, 0.0, 1.0, float_shell_rule=rule)[3]
+        assert np.array_equal(nm, rn), rule
+
+
+def test_single_mode_closed_form_on_gpu(ctx):
+    """A cos wave through the GPU FFT and shells lands in one shell with
+    P = (A N^3/2)^2 / n_s (normal_power 1, no SN, no MAS correction)."""
+    N, L, A = 32, 100.0, 0.3
+    kf = 2 * np.pi / L
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    x = A * np.cos(2 * np.pi * (3 * i + 2 * j + 5 * k) / N)
+    h = ctx.fft_r2c(torch.from_numpy(x).cuda())
+    p0, _, _, nm = ctx.pk_shells(h, N, L, N // 2, 0.0, kf, 0, 0.0, 1.0, mas_correction=False)
+    s = int(np.floor(np.sqrt(9 + 4 + 25)))
+    exp = np.zeros(N // 2)
+    exp[s] = (A * N ** 3 / 2) ** 2 / nm[s]
+    assert np.max(np.abs(p0 - exp)) < 1e-9 * exp.max()

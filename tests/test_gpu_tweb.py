@@ -1,0 +1,61 @@
+"""GPU T-web against the oracle: eigenvalues to float32 round-off, labels
+identical outside the eigenvalue margin (those cells are counted)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import cosmicatlas_amd as cm
+    c = cm.Context(0)
+    yield c
+    c.close()
+
+
+def _smooth_field(N, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((N, N, N))
+    k = np.fft.fftfreq(N)
+    kz = np.fft.rfftfreq(N)
+    w = np.exp(-20 * (k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2))
+    return np.fft.irfftn(np.fft.rfftn(x) * w, s=(N, N, N)) * 30
+
+
+@pytest.mark.parametrize("N,L,R,lth", [(32, 64.0, 3.0, 0.0), (48, 100.0, 4.0, 0.02), (64, 128.0, 0.0, -0.01)])
+def test_tweb_matches_oracle(ctx, orc, N, L, R, lth):
+    delta = _smooth_field(N, N).astype(np.float32)
+    lab_ref, lam_ref, _, cnt_ref = orc.tweb(delta.astype(np.float64), L, R, lth)
+    tol = 1e-4 * np.abs(lam_ref).max()
+    labels, lam, st = ctx.tweb(torch.from_numpy(delta).cuda(), L, R, lth, margin_tol=tol, want_lambdas=True)
+    lam = lam.cpu().numpy()
+    labels = labels.cpu().numpy()
+    assert np.max(np.abs(lam - lam_ref)) < 2e-5 * np.abs(lam_ref).max()
+    margin = np.abs(lam_ref - lth).min(axis=0)
+    ok = margin >= tol
+    assert np.array_equal(labels[ok], lab_ref[ok])
+    assert sum(st["counts"]) == N ** 3
+    assert np.array_equal(np.bincount(labels.ravel(), minlength=5)[1:], st["counts"])
+    # cells whose labels differ are all among the marginal cells the kernel counts
+    assert int(np.count_nonzero(labels != lab_ref)) <= st["n_marginal"] + int((~ok).sum())
+
+
+def test_tweb_from_phi_matches_oracle(ctx, orc):
+    N, L = 32, 50.0
+    phi = _smooth_field(N, 3)
+    lab_ref, lam_ref, _ = orc.tweb_from_phi(phi, L, 0.0)
+    labels, lam, _ = ctx.tweb_from_phi(torch.from_numpy(phi.astype(np.float32)).cuda(), L, 0.0, want_lambdas=True)
+    assert np.max(np.abs(lam.cpu().numpy() - lam_ref)) < 2e-5 * np.abs(lam_ref).max()
+    ok = np.abs(lam_ref).min(axis=0) > 1e-4 * np.abs(lam_ref).max()
+    assert np.array_equal(labels.cpu().numpy()[ok], lab_ref[ok])
+
+
+def test_poisson_matches_oracle(ctx, orc):
+    N, L, R = 48, 100.0, 5.0
+    delta = _smooth_field(N, 5)
+    dk = orc.fft_r2c(delta)
+    ref = orc.fft_c2r(orc.smooth_poisson(dk, N, L, R), N)
+    phi = ctx.poisson(torch.from_numpy(delta.astype(np.float32)).cuda(), L, R).cpu().numpy()
+    assert np.max(np.abs(phi - ref)) < 1e-5 * np.abs(ref).max()

@@ -1,0 +1,115 @@
+"""The C++ host layer: .ini grammar and derived constants (CPU), and the -p
+workflow on the GPU against the oracle."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "cosmicatlas_amd", "bin", "cosmicatlas")
+
+
+def _ini(tmp_path, **kv):
+    base = dict(Nft="128", Lbox="500.0", mass_assignment_scheme="CIC", MAS_correction="true",
+                type_of_binning="linear", DeltaKmin="1", ndel_data="1", SN_correction="true",
+                Statistics="Pk_fkp", Input_type="catalog", Output_directory=str(tmp_path) + "/",
+                file_power="test", Name_survey="BOX", Input_dir_cat=str(tmp_path) + "/",
+                Catalogue_file="cat.txt", i_coord1_g="0", i_coord2_g="1", i_coord3_g="2")
+    base.update(kv)
+    lines = ["# comment line", "Unknown_key = 3   # ignored like the reference"]
+    lines += [f"{k} = {v}   # trailing text after the value is ignored" for k, v in base.items()]
+    p = tmp_path / "run.ini"
+    p.write_text("\n".join(lines) + "\n")
+    return str(p)
+
+
+def _show(ini):
+    r = subprocess.run([CLI, "-i", ini], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return dict(line.split(" = ", 1) for line in r.stdout.strip().splitlines())
+
+
+def test_derived_parameters(tmp_path):
+    d = _show(_ini(tmp_path))
+    kf = 2 * math.pi / 500.0
+    assert int(d["Nft"]) == 128 and int(d["NGRID"]) == 128 ** 3 and int(d["NGRID_h"]) == 128 * 128 * 65
+    assert float(d["delta_x"]) == 500.0 / 128
+    assert math.isclose(float(d["deltak_0"]), kf, rel_tol=1e-15)
+    assert math.isclose(float(d["kmin"]), kf, rel_tol=1e-15)
+    assert math.isclose(float(d["DeltaK_data"]), kf, rel_tol=1e-15)
+    assert int(d["Nnp_data"]) == 64 and int(d["mass_assignment"]) == 1
+    d2 = _show(_ini(tmp_path, Nft="512", ndel_data="2", mass_assignment_scheme="TSC"))
+    assert int(d2["Nnp_data"]) == 128 and int(d2["mass_assignment"]) == 2
+
+
+@pytest.mark.parametrize("line,msg", [("Nft 128", "'=' not properly written"),
+                                      ("mass_assignment_scheme = QQQ", "Unknown mass assigment scheme"),
+                                      ("type_of_binning = cubic", "Unknown type of binning")])
+def test_ini_errors_exit_1(tmp_path, line, msg):
+    p = tmp_path / "bad.ini"
+    p.write_text("Nft = 64\n" + line + "\n")
+    r = subprocess.run([CLI, "-i", str(p)], capture_output=True, text=True)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_missing_file_and_usage():
+    r = subprocess.run([CLI, "-i", "/nonexistent.ini"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Error at opening the parameter file" in r.stderr
+    assert subprocess.run([CLI], capture_output=True).returncode == 1
+
+
+def test_empty_value_keeps_default(tmp_path):
+    p = tmp_path / "e.ini"
+    p.write_text("Nft = 64\nLbox =\n")
+    d = _show(str(p))
+    assert float(d["Lbox"]) == 100.0
+
+
+@pytest.mark.gpu
+def test_cli_catalog_power_spectrum(tmp_path, orc):
+    N, L, n = 64, 500.0, 100000
+    pos = np.random.default_rng(21).uniform(0, L, (n, 3)).astype(np.float32)
+    np.savetxt(tmp_path / "cat.txt", pos, fmt="%.7f")
+    ini = _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", compute_tweb="true", smscale="20",
+               lambdath="0.0")
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.loadtxt(tmp_path / f"Pk_fkp_BOX_Nft{N}_TSC_test.txt")
+    pos32 = np.loadtxt(tmp_path / "cat.txt").astype(np.float32)
+    mesh = orc.mas(pos32, N, L, orc.TSC)
+    V = L ** 3
+    kf = 2 * np.pi / L
+    dk = orc.fft_r2c(mesh - n / N ** 3)
+    o = orc.pk_opts(N, L, N // 2, kf, kf, orc.TSC, V / n, n * n / V)
+    p0, _, _, nm = orc.pk_shells(dk, o)
+    k = kf + kf * (np.arange(N // 2) + 0.5)
+    first = 1 if p0[0] < 1 else 0
+    np.testing.assert_allclose(out[:, 0], k[first:], rtol=1e-6)
+    assert np.array_equal(out[:, 2].astype(int), nm[first:])
+    assert np.max(np.abs(out[:, 1] - p0[first:]) / (np.abs(p0[first:]) + V / n)) < 1e-4
+    labels = np.fromfile(tmp_path / f"CWC_BOX_Nft{N}_TSC_test.dat", np.float32)
+    assert labels.size == N ** 3 and set(np.unique(labels)) <= {1, 2, 3, 4}
+
+
+@pytest.mark.gpu
+def test_cli_density_grid(tmp_path, orc):
+    N, L = 32, 300.0
+    rng = np.random.default_rng(5)
+    grid = rng.poisson(3.0, (N, N, N)).astype(np.float32)
+    grid.tofile(tmp_path / "grid.dat")
+    ini = _ini(tmp_path, Nft=str(N), Lbox=str(L), Input_type="density_grid", mass_assignment_scheme="NGP",
+               delta_grid_file=str(tmp_path / "grid.dat"), DeltaKmin="0")
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.loadtxt(tmp_path / f"Pk_fkp_BOX_Nft{N}_NGP_test.txt")
+    ng = grid.astype(np.float64).sum()
+    delta = grid / (ng / N ** 3) - 1.0
+    dk = orc.fft_r2c(delta)
+    kf = 2 * np.pi / L
+    o = orc.pk_opts(N, L, N // 2, 0.0, kf, orc.NGP, L ** 3 / ng, (N ** 3 / L ** 1.5) ** 2)
+    p0, _, _, nm = orc.pk_shells(dk, o)
+    first = 1 if p0[0] < 1 else 0
+    assert np.array_equal(out[:, 2].astype(int), nm[first:])
+    assert np.max(np.abs(out[:, 1] - p0[first:]) / (np.abs(p0[first:]) + L ** 3 / ng)) < 1e-4
