@@ -95,7 +95,8 @@ cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
 // Strided column passes over the half spectrum, axis 1 (y) or 0 (x).
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward);
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
-                       double subtract);
+                       double subtract, double *part);
+int fft_rows_blocks(int N, cm_dtype dt);
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double scale);
 

@@ -59,26 +59,42 @@ static int plan_E(const FftPlan &p) {
 }
 
 // ---------------------------------------------------------------- kernels
+// part != nullptr: each workgroup also writes the FP64 sum of its input rows
+// (before `sub` is removed) to part[blockIdx.x] -- the mesh sum rides along.
 template <typename T, int E>
 __global__ void __launch_bounds__(512) k_rows_r2c(const T *__restrict__ in, v2<T> *__restrict__ out,
                                                    DevPlan pl, int N, int nb, T sub,
                                                    const v2<T> *__restrict__ twM,
-                                                   const v2<T> *__restrict__ twN) {
+                                                   const v2<T> *__restrict__ twN,
+                                                   double *__restrict__ part) {
     using C = v2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[8];
     C *lds = reinterpret_cast<C *>(smem_raw);
     const int M = N / 2, stride = M + 1;
     const size_t row0 = (size_t)blockIdx.x * nb;
     const C *inrow = reinterpret_cast<const C *>(in + row0 * (size_t)N);
+    double acc = 0.0;
     auto src = [&](int b, int m) {
         C z = inrow[(size_t)b * M + m];
+        acc += (double)z.x + (double)z.y;
         z.x -= sub;
         z.y -= sub;
         return z;
     };
     auto dst = [&](int b, int k, C v) { lds[b * stride + k] = v; };
     fft_run<T, E, false, true>(lds, stride, pl, nb, twM, src, dst);
+    if (part) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    }
     __syncthreads();
+    if (part && threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
     C *orow = out + row0 * (size_t)(M + 1);
     const T half = (T)0.5;
     for (int t = threadIdx.x; t < nb * (M + 1); t += blockDim.x) {
@@ -161,7 +177,8 @@ static RowCfg row_cfg(int N, cm_dtype dt) {
     int target = dt == CM_F32 ? 4096 : 2048;
     int nb = 1;
     const int rows = N * N;
-    while (nb * 2 * M <= target && nb * 2 <= rows) nb *= 2;
+    const int rows_pow2 = rows & -rows;  // nb must divide the row count
+    while (nb * 2 * M <= target && nb * 2 <= rows_pow2) nb *= 2;
     while ((nb * M) % c.E) nb *= 2;
     while (nb * M / c.E > 512 && nb > 1) nb /= 2;
     c.nb = nb;
@@ -191,7 +208,7 @@ static ColCfg col_cfg(int N, cm_dtype dt) {
 }
 
 template <typename T>
-static cm_status rows_r2c_t(cm_ctx *ctx, int N, const void *in, void *out, double subtract) {
+static cm_status rows_r2c_t(cm_ctx *ctx, int N, const void *in, void *out, double subtract, double *part) {
     cm_dtype dt = sizeof(T) == 4 ? CM_F32 : CM_F64;
     RowCfg c = row_cfg(N, dt);
     cm_status st;
@@ -207,9 +224,9 @@ static cm_status rows_r2c_t(cm_ctx *ctx, int N, const void *in, void *out, doubl
     auto *a = (const v2<T> *)twM;
     auto *b = (const v2<T> *)twN;
     switch (c.E) {
-    case 8: return launch(ctx, "fft_z_r2c", k_rows_r2c<T, 8>, grid, dim3(c.NT), c.lds, ip, op, pl, N, c.nb, (T)subtract, a, b);
-    case 16: return launch(ctx, "fft_z_r2c", k_rows_r2c<T, 16>, grid, dim3(c.NT), c.lds, ip, op, pl, N, c.nb, (T)subtract, a, b);
-    case 24: return launch(ctx, "fft_z_r2c", k_rows_r2c<T, 24>, grid, dim3(c.NT), c.lds, ip, op, pl, N, c.nb, (T)subtract, a, b);
+    case 8: return launch(ctx, "fft_z_r2c", k_rows_r2c<T, 8>, grid, dim3(c.NT), c.lds, ip, op, pl, N, c.nb, (T)subtract, a, b, part);
+    case 16: return launch(ctx, "fft_z_r2c", k_rows_r2c<T, 16>, grid, dim3(c.NT), c.lds, ip, op, pl, N, c.nb, (T)subtract, a, b, part);
+    case 24: return launch(ctx, "fft_z_r2c", k_rows_r2c<T, 24>, grid, dim3(c.NT), c.lds, ip, op, pl, N, c.nb, (T)subtract, a, b, part);
     }
     return fail(ctx, CM_ERR_UNSUPPORTED, "fft: no kernel for E=%d", c.E);
 }
@@ -267,10 +284,13 @@ static cm_status cols_t(cm_ctx *ctx, int N, void *data, int axis) {
     return fail(ctx, CM_ERR_UNSUPPORTED, "fft: no kernel for E=%d", c.E);
 }
 
-cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double subtract) {
-    return dt == CM_F32 ? rows_r2c_t<float>(ctx, N, in, out, subtract)
-                        : rows_r2c_t<double>(ctx, N, in, out, subtract);
+cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double subtract,
+                       double *part) {
+    return dt == CM_F32 ? rows_r2c_t<float>(ctx, N, in, out, subtract, part)
+                        : rows_r2c_t<double>(ctx, N, in, out, subtract, part);
 }
+
+int fft_rows_blocks(int N, cm_dtype dt) { return (N * N) / row_cfg(N, dt).nb; }
 
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale) {
     return dt == CM_F32 ? rows_c2r_t<float>(ctx, N, in, out, scale)
@@ -284,7 +304,7 @@ cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool f
 }
 
 cm_status fft_r2c_impl(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double subtract) {
-    cm_status s = fft_rows_r2c(ctx, N, dt, in, out, subtract);
+    cm_status s = fft_rows_r2c(ctx, N, dt, in, out, subtract, nullptr);
     if (s != CM_OK) return s;
     if ((s = fft_cols(ctx, N, dt, out, 1, true)) != CM_OK) return s;
     return fft_cols(ctx, N, dt, out, 0, true);

@@ -1,0 +1,277 @@
+// Fused x pass of the whole-mock measurement (gfx950).
+//
+// After the z and y passes, one workgroup owns nb adjacent kz columns of one ky
+// row and all N kx values.  In one visit of that tile it
+//   1. runs the forward x FFT (global -> registers/LDS),
+//   2. bins every mode of the tile into the P(k) shells (the same enumeration,
+//      shell rule, MAS deconvolution and shot-noise weight as k_pk_shells; runs
+//      of equal (kx, shell) are combined with a wavefront segmented scan),
+//   3. multiplies by exp(-k^2R^2/2)(-1/k^2) N^3/N_gal for the T-web potential,
+//   4. runs the inverse x FFT (LDS -> global).
+// The spectrum is read and written once instead of four times (x pass, shell
+// pass, multiply pass, inverse x pass).  Workgroups are persistent: each loops
+// over tiles and keeps its shell sums in LDS until the end, so the global FP64
+// atomics are one set per workgroup.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cm_internal.hpp"
+#include "fft_engine.hpp"
+#include "pk_common.hpp"
+#include "tweb_common.hpp"
+
+namespace cm {
+
+template <int E>
+__global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevPlan pl, int nb, int ncols, int N,
+                                                 const float2 *__restrict__ tw, PkK pk,
+                                                 const double *__restrict__ tabs, double *__restrict__ gsum,
+                                                 unsigned long long *__restrict__ gcnt, int do_tweb, double kf,
+                                                 double R2h, const double *__restrict__ nobj) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int M1 = N / 2 + 1;
+    const int ns = pk.nshells;
+    float2 *lds = reinterpret_cast<float2 *>(smem_raw);
+    double *s_ic = reinterpret_cast<double *>(lds + (size_t)nb * N);
+    double *s_sq = s_ic + M1;
+    double *s_acc = s_sq + M1;
+    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);
+    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+    // last active lane of this wavefront (small N launches fewer than 64 threads)
+    const int last_lane = min(63, NT - 1 - (tid & ~63));
+    for (int i = tid; i < M1; i += NT) {
+        s_ic[i] = tabs[i];
+        s_sq[i] = tabs[M1 + i];
+    }
+    for (int i = tid; i < 4 * ns; i += NT) s_acc[i] = 0.0;
+    for (int i = tid; i < ns; i += NT) s_cnt[i] = 0u;
+    const double n3 = (double)N * N * N;
+    const double scale = (do_tweb && nobj[0] > 0.0) ? n3 / nobj[0] : 0.0;
+    const size_t xstride = (size_t)N * ncols;
+    const int nkzb = (ncols + nb - 1) / nb;
+    const int ntiles = nkzb * N;
+    __syncthreads();
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int kzb = tile % nkzb, iy = tile / nkzb;
+        const int kz0 = kzb * nb;
+        const size_t base = (size_t)iy * ncols + kz0;
+        const int valid = min(nb, ncols - kz0);
+        auto srcG = [&](int b, int e) {
+            return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
+        };
+        auto dstL = [&](int b, int e, float2 v) { lds[e * nb + b] = v; };
+        fft_run<float, E, true, true>(lds, 0, pl, nb, tw, srcG, dstL);
+        __syncthreads();
+        // shells (+ multiply), element t = e*nb + b (kz fastest)
+        const int total = nb * N;
+        for (int t0 = tid & ~63; t0 < total; t0 += NT) {
+            const int t = t0 + lane;
+            int key = -1, sh = -1;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            unsigned c = 0;
+            if (t < total) {
+                const int b = t % nb, ix = t / nb, iz = kz0 + b;
+                if (b < valid) {
+                    float2 v = lds[t];
+                    int fx, fy;
+                    if (mode_counted(N, ix, iy, iz, fx, fy)) {
+                        sh = shell_of(pk, fx, fy, iz);
+                        if (sh >= 0) {
+                            key = ix;
+                            const double pw = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                            const double ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
+                            const int q = fx * fx + fy * fy + iz * iz;
+                            const double mu2 = (double)iz * (double)iz / (double)q;
+                            const double pc = pw * ic;
+                            a0 = pc;
+                            a1 = s_sq[fx] * s_sq[fy] * s_sq[iz] * ic;
+                            a2 = 0.5 * (3.0 * mu2 - 1.0) * pc;
+                            a3 = (35.0 * mu2 * mu2 - 30.0 * mu2 + 3.0) * 0.125 * pc;
+                            c = 1;
+                        }
+                    }
+                    if (do_tweb) {
+                        const double f = smooth_poisson_factor(ix, iy, iz, N, kf, R2h, scale);
+                        v.x = (float)((double)v.x * f);
+                        v.y = (float)((double)v.y * f);
+                        lds[t] = v;
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int ukey = __shfl_up(key, off, 64);
+                const int ush = __shfl_up(sh, off, 64);
+                const double u0 = __shfl_up(a0, off, 64);
+                const double u1 = __shfl_up(a1, off, 64);
+                const double u2 = __shfl_up(a2, off, 64);
+                const double u3 = __shfl_up(a3, off, 64);
+                const unsigned uc = __shfl_up(c, off, 64);
+                if (lane >= off && ukey == key && ush == sh) {
+                    a0 += u0;
+                    a1 += u1;
+                    a2 += u2;
+                    a3 += u3;
+                    c += uc;
+                }
+            }
+            const int nkey = __shfl_down(key, 1, 64);
+            const int nsh = __shfl_down(sh, 1, 64);
+            if (sh >= 0 && (lane == last_lane || nkey != key || nsh != sh)) {
+                atomicAdd(&s_acc[sh], a0);
+                atomicAdd(&s_acc[ns + sh], a1);
+                atomicAdd(&s_acc[2 * ns + sh], a2);
+                atomicAdd(&s_acc[3 * ns + sh], a3);
+                atomicAdd(&s_cnt[sh], c);
+            }
+        }
+        __syncthreads();
+        if (do_tweb) {
+            auto srcL = [&](int b, int e) { return lds[e * nb + b]; };
+            auto dstG = [&](int b, int e, float2 v) {
+                if (b < valid) data[base + (size_t)e * xstride + b] = v;
+            };
+            fft_run<float, E, true, false>(lds, 0, pl, nb, tw, srcL, dstG);
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < ns; i += NT) {
+        if (s_cnt[i]) {
+            unsafeAtomicAdd(&gsum[i], s_acc[i]);
+            unsafeAtomicAdd(&gsum[ns + i], s_acc[ns + i]);
+            unsafeAtomicAdd(&gsum[2 * ns + i], s_acc[2 * ns + i]);
+            unsafeAtomicAdd(&gsum[3 * ns + i], s_acc[3 * ns + i]);
+            atomicAdd(&gcnt[i], (unsigned long long)s_cnt[i]);
+        }
+    }
+}
+
+__global__ void k_fused_zero(double *s, unsigned long long *c, int ns) {
+    for (int i = threadIdx.x; i < 4 * ns; i += blockDim.x) s[i] = 0.0;
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) c[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) k_fused_sum(const double *__restrict__ part, int np, double *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+PkK make_pkk(const cm_pk_opts *o);
+cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
+                            float *lam, unsigned long long *stats);
+
+struct FusedColCfg {
+    FftPlan plan;
+    int E, nb, NT;
+};
+
+// Fused measurement: mesh (float32, device) -> shells (+ labels).
+// Device outputs: sums[4*ns], cnts[ns], nobj[1], stats[5].
+cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean_est, int8_t *labels,
+                             double **sums, unsigned long long **cnts, double **nobj, unsigned long long **stats) {
+    const int N = o->mas.N;
+    const size_t n3 = (size_t)N * N * N;
+    const size_t nh = (size_t)N * N * (N / 2 + 1);
+    const int ns = o->nshells;
+    const int M1 = N / 2 + 1;
+    cm_status s;
+    void *dk, *part, *dsum, *dcnt, *dnobj, *dtab, *dstats;
+    const int nparts = fft_rows_blocks(N, CM_F32);
+    if ((s = scratch(ctx, "m.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
+    if ((s = scratch(ctx, "f.part", sizeof(double) * (size_t)nparts, &part)) != CM_OK) return s;
+    if ((s = scratch(ctx, "f.sum", sizeof(double) * 4 * (size_t)ns, &dsum)) != CM_OK) return s;
+    if ((s = scratch(ctx, "f.cnt", sizeof(unsigned long long) * (size_t)ns, &dcnt)) != CM_OK) return s;
+    if ((s = scratch(ctx, "f.nobj", sizeof(double), &dnobj)) != CM_OK) return s;
+    if ((s = scratch(ctx, "f.stats", sizeof(unsigned long long) * 5, &dstats)) != CM_OK) return s;
+    std::vector<double> tab;
+    pk_tables(N, (int)o->mas.scheme, o->mas_correction, tab);
+    if ((s = scratch(ctx, "f.tab", sizeof(double) * tab.size(), &dtab)) != CM_OK) return s;
+    CM_HIP(ctx, hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+
+    // z pass with the mesh sum, then the FP64 total
+    if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part)) != CM_OK) return s;
+    if ((s = launch(ctx, "fused_sum", k_fused_sum, dim3(1), dim3(256), 0, (const double *)part, nparts,
+                    (double *)dnobj)) != CM_OK)
+        return s;
+    // y pass
+    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true)) != CM_OK) return s;
+    // x pass + shells + multiply + inverse x
+    if ((s = launch(ctx, "fused_zero", k_fused_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
+                    ns)) != CM_OK)
+        return s;
+    cm_pk_opts po;
+    po.N = N;
+    po.L = o->mas.L;
+    po.nshells = ns;
+    po.kmin = o->kmin;
+    po.dk = o->dk;
+    po.scheme = o->mas.scheme;
+    po.mas_correction = o->mas_correction;
+    po.shot_noise = 0.0;
+    po.normal_power = 1.0;
+    po.float_shell_rule = o->float_shell_rule;
+    PkK pk = make_pkk(&po);
+    FftPlan plan = make_plan(N);
+    DevPlan dp;
+    dp.n = plan.n;
+    dp.nst = plan.nst;
+    for (int i = 0; i < 8; ++i) dp.r[i] = plan.r[i];
+    bool has3 = false, has8 = false;
+    for (int i = 0; i < plan.nst; ++i) {
+        has3 |= plan.r[i] == 3;
+        has8 |= plan.r[i] == 8;
+    }
+    const int E = has3 ? 24 : ((has8 && N >= 64) ? 16 : 8);
+    int nb = N > 512 ? 8 : 16;
+    while (nb * N / E > 512 && nb > 1) nb /= 2;
+    while ((nb * N) % E) nb *= 2;
+    const int NT = nb * N / E;
+    const size_t lds = sizeof(float2) * (size_t)nb * N + sizeof(double) * (2 * (size_t)M1 + 4 * (size_t)ns) +
+                       sizeof(unsigned) * (size_t)ns;
+    if (lds > ctx->lds_per_block)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
+    cm_status st;
+    const void *tw = twiddles(ctx, N, CM_F32, &st);
+    if (!tw) return st;
+    const int ncols = N / 2 + 1;
+    const int ntiles = ((ncols + nb - 1) / nb) * N;
+    const int per_cu = lds > 80 * 1024 ? 1 : 2;
+    const unsigned grid = (unsigned)std::min(ntiles, ctx->cu_count * per_cu);
+    const double kf = 2.0 * M_PI / o->mas.L;
+    const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
+    const int do_tweb = o->do_tweb ? 1 : 0;
+    auto *dkp = (float2 *)dk;
+    auto *twp = (const float2 *)tw;
+    switch (E) {
+    case 8: s = launch(ctx, "fused_x", k_x_fused<8>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, kf, R2h, (const double *)dnobj); break;
+    case 16: s = launch(ctx, "fused_x", k_x_fused<16>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, kf, R2h, (const double *)dnobj); break;
+    case 24: s = launch(ctx, "fused_x", k_x_fused<24>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, kf, R2h, (const double *)dnobj); break;
+    default: return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: E=%d", E);
+    }
+    if (s != CM_OK) return s;
+    if (do_tweb) {
+        void *phi;
+        if ((s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
+        if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false)) != CM_OK) return s;
+        if ((s = fft_rows_c2r(ctx, N, CM_F32, dk, phi, 1.0 / (double)n3)) != CM_OK) return s;
+        if ((s = tweb_fused_launch(ctx, N, o->mas.L, (const float *)phi, o->lambda_th, o->margin_tol, labels, nullptr,
+                                   (unsigned long long *)dstats)) != CM_OK)
+            return s;
+    }
+    *sums = (double *)dsum;
+    *cnts = (unsigned long long *)dcnt;
+    *nobj = (double *)dnobj;
+    *stats = (unsigned long long *)dstats;
+    return CM_OK;
+}
+
+}  // namespace cm

@@ -3,8 +3,10 @@
 // plain (fused=0): the step-by-step GPU path -- global-atomic deposit, FP64
 //   mesh sum read back to the host, r2c with the mean removed, shell kernel,
 //   then the T-web as separate Poisson / gradient / Hessian+eigen kernels.
-// fused (fused=1): tiled LDS deposit; the object count stays on the device
-//   (no host round trip until the shell sums are read at the end).
+// fused (fused=1): tiled LDS deposit; z pass carrying the FP64 mesh sum; y
+//   pass; one x pass doing forward FFT + shells + T-web multiplier + inverse
+//   FFT; inverse y and c2r z passes; the marching FD-Hessian/label kernel.  The
+//   object count stays on the device until the sums are read at the end.
 #include <cmath>
 #include <vector>
 
@@ -15,7 +17,7 @@ namespace cm {
 
 cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi, double subtract);
 cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
-                             cm_tweb_stats *st);
+                             cm_tweb_stats *st, bool plain);
 
 // scale the half spectrum by -W(k)/k^2 * N^3/N_gal (delta = rho/mean - 1 with
 // mean = N_gal/N^3), reading N_gal on the device.
@@ -44,6 +46,54 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
     return (unsigned)g;
 }
 
+cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean_est, int8_t *labels,
+                             double **sums, unsigned long long **cnts, double **nobj, unsigned long long **stats);
+
+// Finalisation shared by both paths: normal_power = N_gal^2/V, SN = V/N_gal
+// (raw_sampling, FftwFunctions.cpp:159-174), so SN*normal_power = N_gal.
+static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<double> &hs,
+                     const std::vector<unsigned long long> &hc, double *pk0, double *pk2, double *pk4,
+                     int64_t *nmodes, cm_measure_info *info) {
+    const double L = o->mas.L, V = L * L * L;
+    const double normal = nobj_h * nobj_h / V;
+    const double SN = (o->sn_correction && nobj_h > 0) ? V / nobj_h : 0.0;
+    if (info) {
+        info->n_obj = nobj_h;
+        info->shot_noise = SN;
+        info->normal_power = normal;
+    }
+    pk_finalize(o->nshells, normal > 0 ? normal : 1.0, SN * normal, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
+}
+
+static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, size_t n, double *pk0,
+                                    double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
+                                    cm_measure_info *info) {
+    const int N = o->mas.N;
+    const double n3 = (double)N * N * N;
+    double *sums, *nobj;
+    unsigned long long *cnts, *stats;
+    cm_status s = fused_measure_mesh(ctx, o, mesh, (double)n / n3, labels, &sums, &cnts, &nobj, &stats);
+    if (s != CM_OK) return s;
+    const int ns = o->nshells;
+    std::vector<double> hs(4 * (size_t)ns);
+    std::vector<unsigned long long> hc((size_t)ns);
+    double nobj_h = 0.0;
+    unsigned long long hst[5] = {0, 0, 0, 0, 0};
+    CM_HIP(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipMemcpyAsync(hc.data(), cnts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CM_HIP(ctx, hipMemcpyAsync(&nobj_h, nobj, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (o->do_tweb)
+        CM_HIP(ctx, hipMemcpyAsync(hst, stats, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (info) {
+        for (int i = 0; i < 4; ++i) info->tweb.counts[i] = (int64_t)hst[i];
+        info->tweb.n_marginal = (int64_t)hst[4];
+    }
+    finalize(o, nobj_h, hs, hc, pk0, pk2, pk4, nmodes, info);
+    return CM_OK;
+}
+
 static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
                               double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
                               cm_measure_info *info) {
@@ -60,16 +110,11 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     cm_mas_desc md = o->mas;
     md.algo = o->fused ? CM_MAS_AUTO : CM_MAS_PLAIN;
     if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0)) != CM_OK) return s;
+    if (o->fused) return measure_fused_tail(ctx, o, mesh, n, pk0, pk2, pk4, nmodes, labels, info);
     // N_gal = FP64 sum of the mesh (get_nobjects without its float sum)
     double nobj_h = 0.0;
-    if (o->fused) {
-        if ((s = mesh_sum_impl(ctx, mesh, CM_F32, n3, (double *)nobj, nullptr)) != CM_OK) return s;
-    } else {
-        if ((s = mesh_sum_impl(ctx, mesh, CM_F32, n3, (double *)nobj, &nobj_h)) != CM_OK) return s;
-    }
-    // remove the mean before the transform; only the k=0 mode depends on it, and
-    // that mode is excluded from every shell, so an estimate is enough
-    const double mean_est = o->fused ? (double)n / (double)n3 : nobj_h / (double)n3;
+    if ((s = mesh_sum_impl(ctx, mesh, CM_F32, n3, (double *)nobj, &nobj_h)) != CM_OK) return s;
+    const double mean_est = nobj_h / (double)n3;  // get_fluctuation (FftwFunctions.cpp:293-294)
     if ((s = fft_r2c_impl(ctx, N, CM_F32, mesh, dk, mean_est)) != CM_OK) return s;
 
     cm_pk_opts po;
@@ -111,25 +156,14 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
         to.margin_tol = o->margin_tol;
         to.delta_scale = 1.0;
         cm_tweb_stats ts;
-        if ((s = tweb_from_phi_impl(ctx, &to, (const float *)phi, labels, nullptr, info ? &ts : nullptr)) != CM_OK)
+        if ((s = tweb_from_phi_impl(ctx, &to, (const float *)phi, labels, nullptr, info ? &ts : nullptr,
+                                    !o->fused)) != CM_OK)
             return s;
         if (info) info->tweb = ts;
     }
-    if (o->fused) {
-        CM_HIP(ctx, hipMemcpyAsync(&nobj_h, nobj, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    }
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
 
-    // normalisation: normal_power = N_gal^2/V, SN = V/N_gal (raw_sampling,
-    // FftwFunctions.cpp:159-174); SN*normal_power = N_gal
-    const double normal = nobj_h * nobj_h / V;
-    const double SN = (o->sn_correction && nobj_h > 0) ? V / nobj_h : 0.0;
-    if (info) {
-        info->n_obj = nobj_h;
-        info->shot_noise = SN;
-        info->normal_power = normal;
-    }
-    pk_finalize(ns, normal > 0 ? normal : 1.0, SN * normal, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
+    finalize(o, nobj_h, hs, hc, pk0, pk2, pk4, nmodes, info);
     return CM_OK;
 }
 

@@ -45,12 +45,23 @@ __device__ __forceinline__ bool mode_counted(int N, int ix, int iy, int iz, int 
 // Shell of the folded mode (i,j,k); -1 outside [0, nshells).
 __device__ __forceinline__ int shell_of(const PkK &p, int i, int j, int k) {
     if (p.float_rule) {
-        // real_prec=float arithmetic of FftwFunctions.cpp:1068-1097, no contraction
-        float ti = __fmul_rn(__fmul_rn((float)(i * i), p.dkx_f), p.dkx_f);
-        float tj = __fmul_rn(__fmul_rn((float)(j * j), p.dkx_f), p.dkx_f);
-        float tk = __fmul_rn(__fmul_rn((float)(k * k), p.dkx_f), p.dkx_f);
-        float kv = __fsqrt_rn(__fadd_rn(__fadd_rn(ti, tj), tk));
-        float t = __fmul_rn(__fsub_rn(kv, p.kmin_f), p.rdk_f);
+#pragma clang fp contract(off)
+        // real_prec=float arithmetic of FftwFunctions.cpp:1068-1097.  Each float
+        // operation is evaluated in double and rounded to float, which gives the
+        // correctly rounded float result (double has more than 2*24+2 bits); FP
+        // contraction is off in this block so no multiply-add becomes an FMA.
+        const double dk = (double)p.dkx_f;
+        float ti = (float)((double)(float)(i * i) * dk);
+        ti = (float)((double)ti * dk);
+        float tj = (float)((double)(float)(j * j) * dk);
+        tj = (float)((double)tj * dk);
+        float tk = (float)((double)(float)(k * k) * dk);
+        tk = (float)((double)tk * dk);
+        const float s1 = (float)((double)ti + (double)tj);
+        const float s2 = (float)((double)s1 + (double)tk);
+        const float kv = (float)sqrt((double)s2);
+        const float d = (float)((double)kv - (double)p.kmin_f);
+        const float t = (float)((double)d * (double)p.rdk_f);
         int s = (int)floorf(t);
         return (s < 0 || s >= p.nshells) ? -1 : s;
     }

@@ -90,6 +90,119 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
     if (threadIdx.x < 5 && cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
 }
 
+
+// Fused FD Hessian + labels.  One workgroup owns a TY x TZ column of cells in
+// (y,z) and a segment of x planes; it marches along x holding, per (y,z)
+// position of the haloed region (TY+4) x (TZ+4), the last five phi values and
+// the G_a histories in registers, and only the current phi plane (halo 4) and
+// the centre plane's G_y, G_z in LDS.  Every phi value is read once per
+// workgroup (plus the y/z halo); G and T never touch HBM.  The composition
+// order of the reference is kept: G_a = D_a phi, then D_x G_x, D_x G_y, D_x G_z,
+// D_y G_y, D_y G_z, D_z G_z (Miscelanious.cpp:1684-1700).  Labels come from the
+// inertia of T - lambda_th I (positive_eigs); the closed-form eigenvalues are
+// evaluated only for the lambda output and for cells whose margin bound is
+// below tol.
+template <int P>
+__global__ void __launch_bounds__(512) k_tweb_fused(const float *__restrict__ phi, int N, int TY, int TZ,
+                                                    int segLen, double ih, double lth, double tol,
+                                                    int8_t *__restrict__ labels, float *__restrict__ lam,
+                                                    unsigned long long *__restrict__ stats) {
+    extern __shared__ float sm[];
+    const int HY = TY + 8, HZ = TZ + 8, GY = TY + 4, GZ = TZ + 4;
+    float *sphi = sm;
+    float *sgy = sphi + HY * HZ;
+    float *sgz = sgy + GY * GZ;
+    __shared__ unsigned scnt[5];
+    const int tid = threadIdx.x, NT = blockDim.x;
+    if (tid < 5) scnt[tid] = 0;
+    const int ntz = N / TZ;
+    const int ty0 = (blockIdx.x / ntz) * TY, tz0 = (blockIdx.x % ntz) * TZ;
+    const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
+    const int npos = GY * GZ;
+    const size_t NN = (size_t)N, n3 = NN * NN * NN;
+    float ph[P][5], gx[P][5], gy[P][7], gz[P][7];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) ph[k][i] = gx[k][i] = 0.f;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) gy[k][i] = gz[k][i] = 0.f;
+    }
+    unsigned loc[5] = {0, 0, 0, 0, 0};
+    for (int s = x0 - 4; s < x1 + 4; ++s) {
+        const int sp = (s % N + N) % N;
+        const float *plane = phi + (size_t)sp * NN * NN;
+        for (int q = tid; q < HY * HZ; q += NT) {
+            const int yy = q / HZ, zz = q - yy * HZ;
+            const int y = (ty0 - 4 + yy + N) % N, z = (tz0 - 4 + zz + N) % N;
+            sphi[q] = plane[(size_t)y * NN + z];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const int q = tid + k * NT;
+            if (q < npos) {
+                const int a = q / GZ, b = q - a * GZ;  // G-region coords; phi region = +2
+                const float *r = sphi + (a + 2) * HZ + (b + 2);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) ph[k][i] = ph[k][i + 1];
+                ph[k][4] = r[0];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) gx[k][i] = gx[k][i + 1];
+                gx[k][4] = fd4(ph[k][0], ph[k][1], ph[k][3], ph[k][4], ih);  // G_x at plane s-2
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    gy[k][i] = gy[k][i + 1];
+                    gz[k][i] = gz[k][i + 1];
+                }
+                gy[k][6] = fd4(r[-2 * HZ], r[-HZ], r[HZ], r[2 * HZ], ih);  // G_y at plane s
+                gz[k][6] = fd4(r[-2], r[-1], r[1], r[2], ih);              // G_z at plane s
+                sgy[q] = gy[k][2];                                         // centre plane c = s-4
+                sgz[q] = gz[k][2];
+            }
+        }
+        __syncthreads();
+        const int c = s - 4;
+        if (c < x0) continue;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const int q = tid + k * NT;
+            if (q >= npos) continue;
+            const int a = q / GZ, b = q - a * GZ;
+            if (a < 2 || a >= TY + 2 || b < 2 || b >= TZ + 2) continue;
+            const float txx = fd4(gx[k][0], gx[k][1], gx[k][3], gx[k][4], ih);
+            const float a10 = fd4(gy[k][0], gy[k][1], gy[k][3], gy[k][4], ih);
+            const float a20 = fd4(gz[k][0], gz[k][1], gz[k][3], gz[k][4], ih);
+            const float tyy = fd4(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ], ih);
+            const float a21 = fd4(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ], ih);
+            const float tzz = fd4(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2], ih);
+            double adet, fro2;
+            const int v = positive_eigs((double)txx - lth, (double)tyy - lth, (double)tzz - lth, a10, a20, a21,
+                                        adet, fro2);
+            const int lab = 4 - v;
+            const size_t cell = ((size_t)c * NN + (ty0 + a - 2)) * NN + (tz0 + b - 2);
+            labels[cell] = (int8_t)lab;
+            loc[lab - 1]++;
+            const bool maybe_marginal = adet < tol * fro2 || fro2 < tol * tol;
+            if (lam || maybe_marginal) {
+                double e1, e2, e3;
+                eig3(txx, tyy, tzz, a10, a20, a21, e1, e2, e3);
+                if (lam) {
+                    lam[cell] = (float)e1;
+                    lam[n3 + cell] = (float)e2;
+                    lam[2 * n3 + cell] = (float)e3;
+                }
+                if (maybe_marginal && fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth))) < tol) loc[4]++;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        if (loc[i]) atomicAdd(&scnt[i], loc[i]);
+    __syncthreads();
+    if (tid < 5 && scnt[tid]) atomicAdd(&stats[tid], (unsigned long long)scnt[tid]);
+}
+
 __global__ void k_zero_u64(unsigned long long *a, int n) {
     if ((int)threadIdx.x < n) a[threadIdx.x] = 0ull;
 }
@@ -99,6 +212,37 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
     if (g < 1) g = 1;
     if (g > cap) g = cap;
     return (unsigned)g;
+}
+
+
+static int pick_tile(int N, const int *cands, int nc) {
+    for (int i = 0; i < nc; ++i)
+        if (cands[i] <= N && N % cands[i] == 0) return cands[i];
+    return 0;
+}
+
+// Fused T-web from phi: labels (and optional lambdas) plus the five counters.
+cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
+                            float *lam, unsigned long long *stats) {
+    static const int cz[] = {64, 32, 16, 8}, cy[] = {16, 8};
+    const int TZ = pick_tile(N, cz, 4), TY = pick_tile(N, cy, 2);
+    if (!TZ || !TY) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
+    const int npos = (TY + 4) * (TZ + 4);
+    const int NT = 512;
+    const int P = (npos + NT - 1) / NT;
+    const int nseg = N >= 256 ? 4 : (N >= 64 ? 2 : 1);
+    const int segLen = (N + nseg - 1) / nseg;
+    dim3 grid((unsigned)((N / TY) * (N / TZ)), (unsigned)nseg);
+    const size_t lds = sizeof(float) * ((size_t)(TY + 8) * (TZ + 8) + 2 * (size_t)npos);
+    const double ih = 1.0 / (12.0 * (L / (double)N));
+    cm_status s;
+    if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
+    switch (P) {
+    case 1: return launch(ctx, "tweb_fused", k_tweb_fused<1>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ih, lth, tol, labels, lam, stats);
+    case 2: return launch(ctx, "tweb_fused", k_tweb_fused<2>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ih, lth, tol, labels, lam, stats);
+    case 3: return launch(ctx, "tweb_fused", k_tweb_fused<3>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ih, lth, tol, labels, lam, stats);
+    }
+    return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile needs %d positions per thread", P);
 }
 
 // phi = c2r(W(k)(-1/k^2) r2c(delta)*scale), float32.
@@ -120,23 +264,30 @@ cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, f
 }
 
 cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
-                             cm_tweb_stats *st) {
+                             cm_tweb_stats *st, bool plain) {
     const int N = o->N;
     const size_t n3 = (size_t)N * N * N;
-    void *g, *stats;
+    void *stats;
     cm_status s;
-    if ((s = scratch(ctx, "tweb.grad", sizeof(float) * 3 * n3, &g)) != CM_OK) return s;
     if ((s = scratch(ctx, "tweb.stats", sizeof(unsigned long long) * 5, &stats)) != CM_OK) return s;
     const double ih = 1.0 / (12.0 * (o->L / (double)N));
-    float *gx = (float *)g, *gy = gx + n3, *gz = gy + n3;
-    unsigned gr = grid_for(n3, 256, 65536);
-    if ((s = launch(ctx, "tweb_grad", k_grad3, dim3(gr), dim3(256), 0, phi, N, ih, gx, gy, gz)) != CM_OK) return s;
-    if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, (unsigned long long *)stats, 5)) != CM_OK)
-        return s;
-    if ((s = launch(ctx, "tweb_hess_eig", k_hess_eig, dim3(gr), dim3(256), 0, (const float *)gx, (const float *)gy,
-                    (const float *)gz, N, ih, o->lambda_th, o->margin_tol, labels, lam,
-                    (unsigned long long *)stats)) != CM_OK)
-        return s;
+    if (plain) {
+        void *g;
+        if ((s = scratch(ctx, "tweb.grad", sizeof(float) * 3 * n3, &g)) != CM_OK) return s;
+        float *gx = (float *)g, *gy = gx + n3, *gz = gy + n3;
+        unsigned gr = grid_for(n3, 256, 65536);
+        if ((s = launch(ctx, "tweb_grad", k_grad3, dim3(gr), dim3(256), 0, phi, N, ih, gx, gy, gz)) != CM_OK) return s;
+        if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, (unsigned long long *)stats, 5)) != CM_OK)
+            return s;
+        if ((s = launch(ctx, "tweb_hess_eig", k_hess_eig, dim3(gr), dim3(256), 0, (const float *)gx,
+                        (const float *)gy, (const float *)gz, N, ih, o->lambda_th, o->margin_tol, labels, lam,
+                        (unsigned long long *)stats)) != CM_OK)
+            return s;
+    } else {
+        if ((s = tweb_fused_launch(ctx, N, o->L, phi, o->lambda_th, o->margin_tol, labels, lam,
+                                   (unsigned long long *)stats)) != CM_OK)
+            return s;
+    }
     if (st) {
         unsigned long long h[5];
         CM_HIP(ctx, hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
@@ -174,7 +325,7 @@ cm_status cm_tweb_from_phi(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi,
     cm_status s = check_opts(ctx, o);
     if (s != CM_OK) return s;
     if (!phi || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb_from_phi: null pointer");
-    return tweb_from_phi_impl(ctx, o, phi, labels, lambdas, stats);
+    return tweb_from_phi_impl(ctx, o, phi, labels, lambdas, stats, false);
 }
 
 cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t *labels, float *lambdas,
@@ -187,7 +338,7 @@ cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t
     const size_t n3 = (size_t)o->N * o->N * o->N;
     if ((s = scratch(ctx, "tweb.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
     if ((s = poisson_impl(ctx, o, delta, (float *)phi, 0.0)) != CM_OK) return s;
-    return tweb_from_phi_impl(ctx, o, (const float *)phi, labels, lambdas, stats);
+    return tweb_from_phi_impl(ctx, o, (const float *)phi, labels, lambdas, stats, false);
 }
 
 }  // extern "C"

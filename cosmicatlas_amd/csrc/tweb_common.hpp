@@ -59,3 +59,34 @@ __device__ __forceinline__ void eig3(double a00, double a11, double a22, double 
 }
 
 }  // namespace cm
+
+namespace cm {
+
+// Number of eigenvalues of the symmetric matrix B (diagonal b00,b11,b22, lower
+// triangle a10,a20,a21) that are > 0, from the signs of the coefficients of its
+// characteristic polynomial t^3 - e1 t^2 + e2 t - e3 (Descartes' rule of signs,
+// exact for a polynomial whose roots are all real; zero roots are factored
+// out).  FP64.  Also returns |e3| and the squared Frobenius norm, which bound
+// the smallest |eigenvalue| from below: min|mu| >= |e3| / ||B||_F^2.
+__device__ __forceinline__ int positive_eigs(double b00, double b11, double b22, double a10, double a20,
+                                             double a21, double &abs_det, double &fro2) {
+    const double e1 = b00 + b11 + b22;
+    const double e2 = b00 * b11 + b00 * b22 + b11 * b22 - a10 * a10 - a20 * a20 - a21 * a21;
+    const double e3 = b00 * (b11 * b22 - a21 * a21) - a10 * (a10 * b22 - a21 * a20) + a20 * (a10 * a21 - b11 * a20);
+    abs_det = fabs(e3);
+    fro2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * (a10 * a10 + a20 * a20 + a21 * a21);
+    // coefficient sequence 1, -e1, e2, -e3 with trailing zero roots removed
+    double c[4] = {1.0, -e1, e2, -e3};
+    int deg = 3;
+    while (deg > 0 && c[deg] == 0.0) --deg;
+    int changes = 0;
+    double prev = 1.0;
+    for (int i = 1; i <= deg; ++i) {
+        if (c[i] == 0.0) continue;
+        if ((c[i] > 0.0) != (prev > 0.0)) ++changes;
+        prev = c[i];
+    }
+    return changes;
+}
+
+}  // namespace cm
