@@ -57,7 +57,7 @@ def algorithmic_bytes(name, n_part, n_rec):
     return table.get(name)
 
 
-KERNELS = ["mas_tile_count", "mas_tile_scan", "mas_tile_items", "mas_tile_scatter", "mas_tile_zero_split",
+KERNELS = ["mas_tile_count", "mas_tile_prefix", "mas_tile_scan", "mas_tile_items", "mas_tile_scatter", "mas_tile_zero_split",
            "mas_tile_deposit", "mas_plain", "fill", "mesh_sum", "mesh_sum_final", "fft_z_r2c", "fused_sum",
            "fft_y_fwd", "fused_zero", "fused_x", "fft_x_fwd", "pk_zero", "pk_shells", "tweb_kmul", "fft_x_inv",
            "fft_y_inv", "fft_z_c2r", "tweb_grad", "tweb_zero", "tweb_hess_eig", "tweb_fused"]

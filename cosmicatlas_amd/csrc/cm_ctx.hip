@@ -1,5 +1,6 @@
 // Context, device memory, error reporting, per-kernel event timing.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "cm_internal.hpp"
@@ -170,6 +171,29 @@ cm_status cm_ctx_create(int device, size_t hbm_budget, cm_ctx **out) {
         c->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)total);
     } else {
         c->budget = hbm_budget;
+    }
+    if (const char *env = getenv("CM_TUNE")) {
+        // comma-separated key=value pairs; unknown keys are ignored
+        std::string t(env);
+        size_t pos = 0;
+        while (pos < t.size()) {
+            size_t end = t.find(',', pos);
+            if (end == std::string::npos) end = t.size();
+            std::string kv = t.substr(pos, end - pos);
+            size_t eq = kv.find('=');
+            if (eq != std::string::npos) {
+                std::string k = kv.substr(0, eq);
+                int v = atoi(kv.c_str() + eq + 1);
+                if (k == "xnb") c->tune.xnb = v;
+                else if (k == "colnb") c->tune.colnb = v;
+                else if (k == "zrows") c->tune.zrows = v;
+                else if (k == "ty") c->tune.ty = v;
+                else if (k == "tz") c->tune.tz = v;
+                else if (k == "tseg") c->tune.tseg = v;
+                else if (k == "mas_tile") c->tune.mas_tile = v;
+            }
+            pos = end + 1;
+        }
     }
     if (hipStreamCreate(&c->own_stream) != hipSuccess) {
         delete c;

@@ -46,6 +46,15 @@ struct cm_ctx {
     std::map<uint64_t, void *> tables;           // twiddle / sinc tables
     int cu_count = 256;
     size_t lds_per_block = 65536;
+    // launch-shape tuning (0 = built-in choice); set from CM_TUNE at creation,
+    // e.g. CM_TUNE="xnb=8,zrows=16,ty=16,tz=64,tseg=4,colnb=16"
+    struct Tune {
+        int xnb = 0;    // kz columns per workgroup in the fused x pass
+        int colnb = 0;  // kz columns per workgroup in the y/x column passes
+        int zrows = 0;  // rows per workgroup in the z passes
+        int ty = 0, tz = 0, tseg = 0;  // T-web tile and x segments
+        int mas_tile = 0;              // cells per tile edge in the deposit
+    } tune;
 };
 
 namespace cm {
@@ -96,7 +105,7 @@ cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward);
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double subtract, double *part);
-int fft_rows_blocks(int N, cm_dtype dt);
+int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double scale);
 

@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(512) k_rows_r2c(const T *__restrict__ in, v2<T
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[8];
     C *lds = reinterpret_cast<C *>(smem_raw);
-    const int M = N / 2, stride = M + 1;
+    const int M = N / 2, stride = row_stride(M);
     const size_t row0 = (size_t)blockIdx.x * nb;
     const C *inrow = reinterpret_cast<const C *>(in + row0 * (size_t)N);
     double acc = 0.0;
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(512) k_rows_r2c(const T *__restrict__ in, v2<T
         z.y -= sub;
         return z;
     };
-    auto dst = [&](int b, int k, C v) { lds[b * stride + k] = v; };
+    auto dst = [&](int b, int k, C v) { lds[b * stride + rpad(k)] = v; };
     fft_run<T, E, false, true>(lds, stride, pl, nb, twM, src, dst);
     if (part) {
 #pragma unroll
@@ -99,8 +99,8 @@ __global__ void __launch_bounds__(512) k_rows_r2c(const T *__restrict__ in, v2<T
     const T half = (T)0.5;
     for (int t = threadIdx.x; t < nb * (M + 1); t += blockDim.x) {
         int b = t / (M + 1), k = t - b * (M + 1);
-        C zk = lds[b * stride + (k == M ? 0 : k)];
-        C zm = cconj(lds[b * stride + (k == 0 ? 0 : M - k)]);
+        C zk = lds[b * stride + rpad(k == M ? 0 : k)];
+        C zm = cconj(lds[b * stride + rpad(k == 0 ? 0 : M - k)]);
         C e = C{half * (zk.x + zm.x), half * (zk.y + zm.y)};
         C o = C{half * (zk.x - zm.x), half * (zk.y - zm.y)};
         // X_k = E_k + W^k O_k with O_k = (Z_k - conj Z_{M-k}) / (2i)
@@ -119,18 +119,18 @@ __global__ void __launch_bounds__(512) k_rows_c2r(const v2<T> *__restrict__ in, 
     using C = v2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *lds = reinterpret_cast<C *>(smem_raw);
-    const int M = N / 2, stride = M + 1;
+    const int M = N / 2, stride = row_stride(M);
     const size_t row0 = (size_t)blockIdx.x * nb;
     const C *irow = in + row0 * (size_t)(M + 1);
     for (int t = threadIdx.x; t < nb * (M + 1); t += blockDim.x) {
         int b = t / (M + 1), k = t - b * (M + 1);
-        lds[b * stride + k] = irow[(size_t)b * (M + 1) + k];
+        lds[b * stride + rpad(k)] = irow[(size_t)b * (M + 1) + k];
     }
     __syncthreads();
     const T half = (T)0.5;
     auto src = [&](int b, int m) {
-        C xm = lds[b * stride + m];
-        C xc = cconj(lds[b * stride + (M - m)]);
+        C xm = lds[b * stride + rpad(m)];
+        C xc = cconj(lds[b * stride + rpad(M - m)]);
         C e = C{half * (xm.x + xc.x), half * (xm.y + xc.y)};
         C o = C{half * (xm.x - xc.x), half * (xm.y - xc.y)};
         C w = cconj(twN[m]);
@@ -169,12 +169,13 @@ struct RowCfg {
     size_t lds;
 };
 
-static RowCfg row_cfg(int N, cm_dtype dt) {
+static RowCfg row_cfg(int N, cm_dtype dt, int want_rows = 0) {
     RowCfg c;
     int M = N / 2;
     c.plan = make_plan(M);
     c.E = plan_E(c.plan);
     int target = dt == CM_F32 ? 4096 : 2048;
+    if (want_rows > 0) target = want_rows * M;
     int nb = 1;
     const int rows = N * N;
     const int rows_pow2 = rows & -rows;  // nb must divide the row count
@@ -183,7 +184,7 @@ static RowCfg row_cfg(int N, cm_dtype dt) {
     while (nb * M / c.E > 512 && nb > 1) nb /= 2;
     c.nb = nb;
     c.NT = nb * M / c.E;
-    c.lds = (size_t)nb * (M + 1) * (dt == CM_F32 ? 8 : 16);
+    c.lds = (size_t)nb * row_stride(M) * (dt == CM_F32 ? 8 : 16);
     return c;
 }
 
@@ -193,12 +194,13 @@ struct ColCfg {
     size_t lds;
 };
 
-static ColCfg col_cfg(int N, cm_dtype dt) {
+static ColCfg col_cfg(int N, cm_dtype dt, int want_nb = 0) {
     ColCfg c;
     c.plan = make_plan(N);
     c.E = plan_E(c.plan);
     int nb = dt == CM_F32 ? 16 : 8;
     if (N > 512) nb /= 2;
+    if (want_nb > 0 && (want_nb & (want_nb - 1)) == 0 && want_nb <= 64) nb = want_nb;
     while (nb * N / c.E > 512 && nb > 1) nb /= 2;
     while ((nb * N) % c.E) nb *= 2;
     c.nb = nb;
@@ -210,13 +212,14 @@ static ColCfg col_cfg(int N, cm_dtype dt) {
 template <typename T>
 static cm_status rows_r2c_t(cm_ctx *ctx, int N, const void *in, void *out, double subtract, double *part) {
     cm_dtype dt = sizeof(T) == 4 ? CM_F32 : CM_F64;
-    RowCfg c = row_cfg(N, dt);
+    RowCfg c = row_cfg(N, dt, dt == CM_F32 ? ctx->tune.zrows : 0);
     cm_status st;
     const void *twM = twiddles(ctx, N / 2, dt, &st);
     if (!twM) return st;
     const void *twN = twiddles(ctx, N, dt, &st);
     if (!twN) return st;
     size_t rows = (size_t)N * N;
+    if (rows % c.nb || c.lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: bad row block");
     dim3 grid((unsigned)(rows / c.nb));
     DevPlan pl = dev_plan(c.plan);
     auto *ip = (const T *)in;
@@ -234,13 +237,14 @@ static cm_status rows_r2c_t(cm_ctx *ctx, int N, const void *in, void *out, doubl
 template <typename T>
 static cm_status rows_c2r_t(cm_ctx *ctx, int N, const void *in, void *out, double scale) {
     cm_dtype dt = sizeof(T) == 4 ? CM_F32 : CM_F64;
-    RowCfg c = row_cfg(N, dt);
+    RowCfg c = row_cfg(N, dt, dt == CM_F32 ? ctx->tune.zrows : 0);
     cm_status st;
     const void *twM = twiddles(ctx, N / 2, dt, &st);
     if (!twM) return st;
     const void *twN = twiddles(ctx, N, dt, &st);
     if (!twN) return st;
     size_t rows = (size_t)N * N;
+    if (rows % c.nb || c.lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: bad row block");
     dim3 grid((unsigned)(rows / c.nb));
     DevPlan pl = dev_plan(c.plan);
     auto *ip = (const v2<T> *)in;
@@ -258,7 +262,8 @@ static cm_status rows_c2r_t(cm_ctx *ctx, int N, const void *in, void *out, doubl
 template <typename T, bool FWD>
 static cm_status cols_t(cm_ctx *ctx, int N, void *data, int axis) {
     cm_dtype dt = sizeof(T) == 4 ? CM_F32 : CM_F64;
-    ColCfg c = col_cfg(N, dt);
+    ColCfg c = col_cfg(N, dt, dt == CM_F32 ? ctx->tune.colnb : 0);
+    if (c.NT > 512 || c.lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: bad block");
     cm_status st;
     const void *tw = twiddles(ctx, N, dt, &st);
     if (!tw) return st;
@@ -290,7 +295,9 @@ cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
                         : rows_r2c_t<double>(ctx, N, in, out, subtract, part);
 }
 
-int fft_rows_blocks(int N, cm_dtype dt) { return (N * N) / row_cfg(N, dt).nb; }
+int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt) {
+    return (N * N) / row_cfg(N, dt, dt == CM_F32 ? ctx->tune.zrows : 0).nb;
+}
 
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale) {
     return dt == CM_F32 ? rows_c2r_t<float>(ctx, N, in, out, scale)

@@ -107,12 +107,16 @@ template <> struct Items<true> {
     }
     __device__ __forceinline__ static int lds(int b, int e, int nb, int stride) { return e * nb + b; }
 };
+// Row layout: element e of row b at b*stride + rpad(e); one padding slot per 8
+// elements breaks the stride-8 stage writes across LDS banks.
+__device__ __forceinline__ int rpad(int e) { return e + (e >> 3); }
+__host__ __device__ inline int row_stride(int M) { return M + 1 + ((M + 1) >> 3) + 1; }
 template <> struct Items<false> {
     __device__ __forceinline__ static void split(int it, int nb, int nR, int &b, int &j) {
         j = it % nR;
         b = it / nR;
     }
-    __device__ __forceinline__ static int lds(int b, int e, int nb, int stride) { return b * stride + e; }
+    __device__ __forceinline__ static int lds(int b, int e, int nb, int stride) { return b * stride + rpad(e); }
 };
 
 template <typename T, int E, int R, bool COLS, bool FWD, class Src, class Dst>

@@ -23,52 +23,79 @@
 
 namespace cm {
 
+// The two FFT phases of a tile live in separate non-inlined functions so that
+// each gets its own register allocation (inlined together with the shell code
+// they exceed the 256-VGPR budget and spill).
+template <int E>
+__device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restrict__ data, size_t base,
+                                                            size_t xstride, int valid, float2 *lds,
+                                                            const DevPlan &pl, int nb,
+                                                            const float2 *__restrict__ tw) {
+    auto srcG = [&](int b, int e) {
+        return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
+    };
+    auto dstL = [&](int b, int e, float2 v) { lds[e * nb + b] = v; };
+    fft_run<float, E, true, true>(lds, 0, pl, nb, tw, srcG, dstL);
+}
+
+template <int E>
+__device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict__ data, size_t base, size_t xstride,
+                                                              int valid, float2 *lds, const DevPlan &pl, int nb,
+                                                              const float2 *__restrict__ tw) {
+    auto srcL = [&](int b, int e) { return lds[e * nb + b]; };
+    auto dstG = [&](int b, int e, float2 v) {
+        if (b < valid) data[base + (size_t)e * xstride + b] = v;
+    };
+    fft_run<float, E, true, false>(lds, 0, pl, nb, tw, srcL, dstG);
+}
+
 template <int E>
 __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevPlan pl, int nb, int ncols, int N,
                                                  const float2 *__restrict__ tw, PkK pk,
                                                  const double *__restrict__ tabs, double *__restrict__ gsum,
-                                                 unsigned long long *__restrict__ gcnt, int do_tweb, double kf,
-                                                 double R2h, const double *__restrict__ nobj) {
+                                                 unsigned long long *__restrict__ gcnt, int do_tweb, float inv_kf2,
+                                                 const float *__restrict__ gauss, const double *__restrict__ nobj) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M1 = N / 2 + 1;
     const int ns = pk.nshells;
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
-    double *s_ic = reinterpret_cast<double *>(lds + (size_t)nb * N);
-    double *s_sq = s_ic + M1;
-    double *s_acc = s_sq + M1;
-    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);
+    double *s_acc = reinterpret_cast<double *>(lds + (size_t)nb * N);  // 4*ns
+    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);     // ns
+    float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                // M1
+    float *s_sq = s_ic + M1;                                            // M1
+    float *s_g = s_sq + M1;                                             // N
     const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
-    // last active lane of this wavefront (small N launches fewer than 64 threads)
-    const int last_lane = min(63, NT - 1 - (tid & ~63));
     for (int i = tid; i < M1; i += NT) {
-        s_ic[i] = tabs[i];
-        s_sq[i] = tabs[M1 + i];
+        s_ic[i] = (float)tabs[i];
+        s_sq[i] = (float)tabs[M1 + i];
     }
+    for (int i = tid; i < N; i += NT) s_g[i] = gauss[i];
     for (int i = tid; i < 4 * ns; i += NT) s_acc[i] = 0.0;
     for (int i = tid; i < ns; i += NT) s_cnt[i] = 0u;
     const double n3 = (double)N * N * N;
-    const double scale = (do_tweb && nobj[0] > 0.0) ? n3 / nobj[0] : 0.0;
+    // -scale/k^2 = -(N^3/N_gal) / (kf^2 q)
+    const float fscale = (do_tweb && nobj[0] > 0.0) ? (float)(-(n3 / nobj[0]) * (double)inv_kf2) : 0.0f;
     const size_t xstride = (size_t)N * ncols;
     const int nkzb = (ncols + nb - 1) / nb;
     const int ntiles = nkzb * N;
+    const int h = N / 2;
     __syncthreads();
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int kzb = tile % nkzb, iy = tile / nkzb;
         const int kz0 = kzb * nb;
         const size_t base = (size_t)iy * ncols + kz0;
         const int valid = min(nb, ncols - kz0);
-        auto srcG = [&](int b, int e) {
-            return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
-        };
-        auto dstL = [&](int b, int e, float2 v) { lds[e * nb + b] = v; };
-        fft_run<float, E, true, true>(lds, 0, pl, nb, tw, srcG, dstL);
+        x_fft_fwd_to_lds<E>(data, base, xstride, valid, lds, pl, nb, tw);
         __syncthreads();
-        // shells (+ multiply), element t = e*nb + b (kz fastest)
+        // shells (+ multiply), element t = e*nb + b (kz fastest): runs of equal
+        // (kx, shell) lie inside one group of nb lanes
         const int total = nb * N;
+        const int ky = iy <= h ? iy : iy - N;
+        const float gy = s_g[iy];
         for (int t0 = tid & ~63; t0 < total; t0 += NT) {
             const int t = t0 + lane;
-            int key = -1, sh = -1;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int sh = -1;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
             unsigned c = 0;
             if (t < total) {
                 const int b = t % nb, ix = t / nb, iz = kz0 + b;
@@ -78,37 +105,36 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
                     if (mode_counted(N, ix, iy, iz, fx, fy)) {
                         sh = shell_of(pk, fx, fy, iz);
                         if (sh >= 0) {
-                            key = ix;
-                            const double pw = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-                            const double ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
+                            const float pw = v.x * v.x + v.y * v.y;
+                            const float ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
                             const int q = fx * fx + fy * fy + iz * iz;
-                            const double mu2 = (double)iz * (double)iz / (double)q;
-                            const double pc = pw * ic;
+                            const float mu2 = (float)(iz * iz) / (float)q;
+                            const float pc = pw * ic;
                             a0 = pc;
                             a1 = s_sq[fx] * s_sq[fy] * s_sq[iz] * ic;
-                            a2 = 0.5 * (3.0 * mu2 - 1.0) * pc;
-                            a3 = (35.0 * mu2 * mu2 - 30.0 * mu2 + 3.0) * 0.125 * pc;
+                            a2 = 0.5f * (3.0f * mu2 - 1.0f) * pc;
+                            a3 = (35.0f * mu2 * mu2 - 30.0f * mu2 + 3.0f) * 0.125f * pc;
                             c = 1;
                         }
                     }
                     if (do_tweb) {
-                        const double f = smooth_poisson_factor(ix, iy, iz, N, kf, R2h, scale);
-                        v.x = (float)((double)v.x * f);
-                        v.y = (float)((double)v.y * f);
+                        const int kx = ix <= h ? ix : ix - N;
+                        const int q2 = kx * kx + ky * ky + iz * iz;
+                        const float f = q2 > 0 ? fscale * s_g[ix] * gy * s_g[iz] / (float)q2 : 0.0f;
+                        v.x *= f;
+                        v.y *= f;
                         lds[t] = v;
                     }
                 }
             }
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int ukey = __shfl_up(key, off, 64);
-                const int ush = __shfl_up(sh, off, 64);
-                const double u0 = __shfl_up(a0, off, 64);
-                const double u1 = __shfl_up(a1, off, 64);
-                const double u2 = __shfl_up(a2, off, 64);
-                const double u3 = __shfl_up(a3, off, 64);
-                const unsigned uc = __shfl_up(c, off, 64);
-                if (lane >= off && ukey == key && ush == sh) {
+            for (int off = 1; off < nb; off <<= 1) {
+                const int ush = __shfl_up(sh, off, nb);
+                const float u0 = __shfl_up(a0, off, nb);
+                const float u1 = __shfl_up(a1, off, nb);
+                const float u2 = __shfl_up(a2, off, nb);
+                const float u3 = __shfl_up(a3, off, nb);
+                const unsigned uc = __shfl_up(c, off, nb);
+                if ((lane % nb) >= off && ush == sh) {
                     a0 += u0;
                     a1 += u1;
                     a2 += u2;
@@ -116,24 +142,17 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
                     c += uc;
                 }
             }
-            const int nkey = __shfl_down(key, 1, 64);
-            const int nsh = __shfl_down(sh, 1, 64);
-            if (sh >= 0 && (lane == last_lane || nkey != key || nsh != sh)) {
-                atomicAdd(&s_acc[sh], a0);
-                atomicAdd(&s_acc[ns + sh], a1);
-                atomicAdd(&s_acc[2 * ns + sh], a2);
-                atomicAdd(&s_acc[3 * ns + sh], a3);
+            const int nsh = __shfl_down(sh, 1, nb);
+            if (sh >= 0 && ((lane % nb) == nb - 1 || nsh != sh)) {
+                atomicAdd(&s_acc[sh], (double)a0);
+                atomicAdd(&s_acc[ns + sh], (double)a1);
+                atomicAdd(&s_acc[2 * ns + sh], (double)a2);
+                atomicAdd(&s_acc[3 * ns + sh], (double)a3);
                 atomicAdd(&s_cnt[sh], c);
             }
         }
         __syncthreads();
-        if (do_tweb) {
-            auto srcL = [&](int b, int e) { return lds[e * nb + b]; };
-            auto dstG = [&](int b, int e, float2 v) {
-                if (b < valid) data[base + (size_t)e * xstride + b] = v;
-            };
-            fft_run<float, E, true, false>(lds, 0, pl, nb, tw, srcL, dstG);
-        }
+        if (do_tweb) x_fft_inv_from_lds<E>(data, base, xstride, valid, lds, pl, nb, tw);
         __syncthreads();
     }
     for (int i = tid; i < ns; i += NT) {
@@ -185,7 +204,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     const int M1 = N / 2 + 1;
     cm_status s;
     void *dk, *part, *dsum, *dcnt, *dnobj, *dtab, *dstats;
-    const int nparts = fft_rows_blocks(N, CM_F32);
+    const int nparts = fft_rows_blocks(ctx, N, CM_F32);
     if ((s = scratch(ctx, "m.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
     if ((s = scratch(ctx, "f.part", sizeof(double) * (size_t)nparts, &part)) != CM_OK) return s;
     if ((s = scratch(ctx, "f.sum", sizeof(double) * 4 * (size_t)ns, &dsum)) != CM_OK) return s;
@@ -231,12 +250,13 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
         has8 |= plan.r[i] == 8;
     }
     const int E = has3 ? 24 : ((has8 && N >= 64) ? 16 : 8);
-    int nb = N > 512 ? 8 : 16;
+    int nb = 8;
+    if (ctx->tune.xnb > 0 && (ctx->tune.xnb & (ctx->tune.xnb - 1)) == 0 && ctx->tune.xnb <= 64) nb = ctx->tune.xnb;
     while (nb * N / E > 512 && nb > 1) nb /= 2;
     while ((nb * N) % E) nb *= 2;
     const int NT = nb * N / E;
-    const size_t lds = sizeof(float2) * (size_t)nb * N + sizeof(double) * (2 * (size_t)M1 + 4 * (size_t)ns) +
-                       sizeof(unsigned) * (size_t)ns;
+    const size_t lds = sizeof(float2) * (size_t)nb * N + sizeof(double) * 4 * (size_t)ns +
+                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (2 * (size_t)M1 + (size_t)N);
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
     cm_status st;
@@ -244,17 +264,28 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     if (!tw) return st;
     const int ncols = N / 2 + 1;
     const int ntiles = ((ncols + nb - 1) / nb) * N;
-    const int per_cu = lds > 80 * 1024 ? 1 : 2;
+    const int per_cu = lds > 80 * 1024 ? 1 : (lds > 53 * 1024 ? 2 : 3);
     const unsigned grid = (unsigned)std::min(ntiles, ctx->cu_count * per_cu);
     const double kf = 2.0 * M_PI / o->mas.L;
     const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
     const int do_tweb = o->do_tweb ? 1 : 0;
+    // per-axis Gaussian exp(-k_i^2 R^2/2), k_i = +i kf (i <= N/2) else -(N-i) kf;
+    // the kernel multiplies the three factors (exp is separable)
+    std::vector<float> g((size_t)N);
+    for (int i = 0; i < N; ++i) {
+        const double k = kf * (i <= N / 2 ? (double)i : -(double)(N - i));
+        g[i] = (float)std::exp(-k * k * R2h);
+    }
+    void *dgauss;
+    if ((s = scratch(ctx, "f.gauss", sizeof(float) * (size_t)N, &dgauss)) != CM_OK) return s;
+    CM_HIP(ctx, hipMemcpyAsync(dgauss, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice, ctx->stream));
+    const float inv_kf2 = (float)(1.0 / (kf * kf));
     auto *dkp = (float2 *)dk;
     auto *twp = (const float2 *)tw;
     switch (E) {
-    case 8: s = launch(ctx, "fused_x", k_x_fused<8>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, kf, R2h, (const double *)dnobj); break;
-    case 16: s = launch(ctx, "fused_x", k_x_fused<16>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, kf, R2h, (const double *)dnobj); break;
-    case 24: s = launch(ctx, "fused_x", k_x_fused<24>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, kf, R2h, (const double *)dnobj); break;
+    case 8: s = launch(ctx, "fused_x", k_x_fused<8>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj); break;
+    case 16: s = launch(ctx, "fused_x", k_x_fused<16>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj); break;
+    case 24: s = launch(ctx, "fused_x", k_x_fused<24>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj); break;
     default: return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: E=%d", E);
     }
     if (s != CM_OK) return s;

@@ -2,10 +2,13 @@
 //
 // plain : one thread per particle, 1/8/27/125 global atomics (the plain GPU
 //         path; any dtype).
-// tiled : float32 positions and mesh.  Particles are binned to T^3-cell tiles;
+// tiled : float32 positions and mesh.  Particles are binned to T^3-cell tiles
+//         (histogram and slot cursors in LDS per workgroup, so no global
+//         atomics; integer LDS atomics are full rate on gfx950);
 //         a particle whose stencil reaches a neighbouring tile is also copied to
 //         that tile ("ghost" copy).  One workgroup per (tile, chunk) accumulates
-//         its particles into an LDS grid (T+2R)^3 with LDS atomics and writes
+//         its particles into an LDS grid (T+2R)^3 with 64-bit fixed-point LDS
+//         atomics and writes
 //         the T^3 interior with plain coalesced stores, so the mesh needs no
 //         zero fill and sees no global atomics (tiles holding more than CH
 //         particles are split into chunks that add with global atomics).
@@ -94,16 +97,25 @@ __device__ __forceinline__ int axis_tiles(int c, const TileK &t, int out[2]) {
     return 1;
 }
 
+// Cell and in-cell offset f = x/d - c - 1/2 (FP64, as the plain kernel) of a
+// particle; s[] receives the offsets rounded to float.
 __device__ __forceinline__ bool particle_cells(const float *pos, size_t i, const MasK &p, int c[3], float s[3]) {
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float v = 0.0f;
         ok = ok && wrap_coord<float>(load_coord(pos, i, a, p.soa, p.n), p.xmin[a], p.L, v);
-        s[a] = v;
-        c[a] = cell_of(shifted((double)v, p.ishift, p.L), p.d, p.N);
+        const double x = shifted((double)v, p.ishift, p.L);
+        c[a] = cell_of(x, p.d, p.N);
+        s[a] = (float)(x / p.d - (double)c[a] - 0.5);
     }
     return ok;
+}
+
+// Sorted record: packed global cell (10 bits per axis) and the three offsets.
+__device__ __forceinline__ float4 make_record(const int c[3], const float f[3]) {
+    const unsigned pk = (unsigned)c[0] | ((unsigned)c[1] << 10) | ((unsigned)c[2] << 20);
+    return make_float4(__uint_as_float(pk), f[0], f[1], f[2]);
 }
 
 __global__ void __launch_bounds__(256) k_tile_count(const float *__restrict__ pos, MasK p, TileK t,
@@ -183,23 +195,108 @@ __global__ void __launch_bounds__(256) k_tile_items(const unsigned *__restrict__
 
 __global__ void __launch_bounds__(256) k_tile_scatter(const float *__restrict__ pos, const float *__restrict__ w,
                                                       MasK p, TileK t, unsigned *__restrict__ cursor,
-                                                      float4 *__restrict__ sorted, unsigned cap,
-                                                      unsigned *__restrict__ overflow) {
+                                                      float4 *__restrict__ sorted, float *__restrict__ wsorted,
+                                                      unsigned cap, unsigned *__restrict__ overflow) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
          i += (size_t)gridDim.x * blockDim.x) {
         int c[3];
         float s[3];
         if (!particle_cells(pos, i, p, c, s)) continue;
-        const float wt = w ? w[i] : 1.0f;
-        const float4 rec = make_float4(s[0], s[1], s[2], wt);
+        const float4 rec = make_record(c, s);
         int tx[2], ty[2], tz[2];
         int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
         for (int a = 0; a < nx; ++a)
             for (int b = 0; b < ny; ++b)
                 for (int e = 0; e < nz; ++e) {
                     unsigned slot = atomicAdd(&cursor[((size_t)tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
-                    if (slot < cap) sorted[slot] = rec;
-                    else atomicOr(overflow, 1u);
+                    if (slot < cap) {
+                        sorted[slot] = rec;
+                        if (w) wsorted[slot] = w[i];
+                    } else {
+                        atomicOr(overflow, 1u);
+                    }
+                }
+    }
+}
+
+
+// Count with an LDS histogram per workgroup (no global atomics): workgroup g
+// owns particles [g*chunk, (g+1)*chunk) and writes its histogram row
+// hist[g*ntiles + tile] with coalesced stores.
+__global__ void __launch_bounds__(1024) k_tile_count_lds(const float *__restrict__ pos, MasK p, TileK t, int ntiles,
+                                                         size_t chunk, unsigned *__restrict__ hist,
+                                                         unsigned long long *dropped) {
+    extern __shared__ unsigned lh[];
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) lh[i] = 0u;
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
+    unsigned ndrop = 0;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        int c[3];
+        float s[3];
+        if (!particle_cells(pos, i, p, c, s)) {
+            ++ndrop;
+            continue;
+        }
+        int tx[2], ty[2], tz[2];
+        int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
+        for (int a = 0; a < nx; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int e = 0; e < nz; ++e) atomicAdd(&lh[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+    }
+    if (ndrop) atomicAdd(dropped, (unsigned long long)ndrop);
+    __syncthreads();
+    unsigned *row = hist + (size_t)blockIdx.x * ntiles;
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) row[i] = lh[i];
+}
+
+// Per tile: exclusive prefix over the workgroups' histogram rows (in place) and
+// the tile total.  One thread per tile; for a fixed row consecutive threads
+// touch consecutive words.
+__global__ void __launch_bounds__(256) k_tile_prefix(unsigned *__restrict__ hist, int nrows, int ntiles,
+                                                     unsigned *__restrict__ totals) {
+    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= ntiles) return;
+    unsigned run = 0;
+    for (int g = 0; g < nrows; ++g) {
+        unsigned *h = hist + (size_t)g * ntiles + tile;
+        const unsigned v = *h;
+        *h = run;
+        run += v;
+    }
+    totals[tile] = run;
+}
+
+// Scatter with LDS cursors: cursor[tile] = offsets[tile] + prefix of earlier
+// workgroups; integer LDS atomics hand out the slots.
+__global__ void __launch_bounds__(1024) k_tile_scatter_lds(const float *__restrict__ pos, const float *__restrict__ w,
+                                                           MasK p, TileK t, int ntiles, size_t chunk,
+                                                           const unsigned *__restrict__ pref,
+                                                           const unsigned *__restrict__ offsets,
+                                                           float4 *__restrict__ sorted, float *__restrict__ wsorted,
+                                                           unsigned cap, unsigned *__restrict__ overflow) {
+    extern __shared__ unsigned cur[];
+    const unsigned *row = pref + (size_t)blockIdx.x * ntiles;
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cur[i] = offsets[i] + row[i];
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
+    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        int c[3];
+        float s[3];
+        if (!particle_cells(pos, i, p, c, s)) continue;
+        const float4 rec = make_record(c, s);
+        int tx[2], ty[2], tz[2];
+        int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
+        for (int a = 0; a < nx; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int e = 0; e < nz; ++e) {
+                    const unsigned slot = atomicAdd(&cur[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+                    if (slot < cap) {
+                        sorted[slot] = rec;
+                        if (w) wsorted[slot] = w[i];
+                    } else {
+                        atomicOr(overflow, 1u);
+                    }
                 }
     }
 }
@@ -219,65 +316,89 @@ __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restr
     }
 }
 
-__global__ void __launch_bounds__(256) k_tile_deposit(const float4 *__restrict__ sorted,
+// LDS accumulation is in 64-bit fixed point (2^-32 quantum): integer LDS
+// atomics run at full rate on gfx950 (float LDS atomic add does not), and the
+// integer sum does not depend on the order of the records, so every tile that
+// one workgroup deposits is bit-reproducible.
+__device__ __forceinline__ long long to_fixed(float v) { return __double2ll_rn((double)v * 4294967296.0); }
+__device__ __forceinline__ float from_fixed(long long q) { return (float)((double)q * (1.0 / 4294967296.0)); }
+
+// Stencil of one record into the LDS grid, unrolled for the scheme.  A record
+// of the tile sits in a cell c with local coordinate c-o in [-R, T+R) (ghosts
+// come from up to R cells outside), so its stencil spans [-2R, T+2R): the LDS
+// grid has S = T+4R cells per axis with local cell u at index u+2R, every
+// contribution lands in it without a range test, and only [0,T) is written
+// out.
+template <int SCHEME, bool UNIT>
+__device__ __forceinline__ void deposit_record(long long *lgrid, int S, int base, const float f[3], float wt) {
+    constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
+    constexpr int W = 2 * R + 1;
+    float wa[3][5];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) axis_weights<float>(SCHEME, f[a], wa[a]);
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const float vab = wt * wa[0][a + 2 - R] * wa[1][b + 2 - R];
+            unsigned long long *row = (unsigned long long *)&lgrid[base + (a * S + b) * S];
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                const float v = vab * wa[2][e + 2 - R];
+                // unit weights: |v| <= 1 fits v*2^30 in an int32 (one conversion);
+                // general weights go through FP64
+                const long long q = UNIT ? (long long)__float2int_rn(v * 1073741824.0f) * 4ll
+                                         : __double2ll_rn((double)v * 4294967296.0);
+                atomicAdd(&row[e], (unsigned long long)q);
+            }
+        }
+    }
+}
+
+template <int SCHEME, bool UNIT>
+__global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__ sorted,
+                                                      const float *__restrict__ wsorted,
                                                       const unsigned *__restrict__ offsets,
                                                       const uint2 *__restrict__ items,
                                                       const unsigned *__restrict__ nitems, MasK p, TileK t,
                                                       unsigned cap, float *__restrict__ mesh, int accumulate) {
-    extern __shared__ float lgrid[];
-    const int T = t.T, R = t.R, S = T + 2 * R, S3 = S * S * S;
+    extern __shared__ long long lgrid[];
+    constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
+    const int T = t.T, S = T + 4 * R, S3 = S * S * S;
     const unsigned total = *nitems;
     const int N = p.N;
     for (unsigned item = blockIdx.x; item < total; item += gridDim.x) {
         const uint2 it = items[item];
         const int tile = (int)it.x;
         const int tz = tile % t.ntx, ty = (tile / t.ntx) % t.ntx, tx = tile / (t.ntx * t.ntx);
-        const int ox = tx * T, oy = ty * T, oz = tz * T;
-        for (int q = threadIdx.x; q < S3; q += blockDim.x) lgrid[q] = 0.0f;
+        const int o[3] = {tx * T, ty * T, tz * T};
+        for (int q = threadIdx.x; q < S3; q += blockDim.x) lgrid[q] = 0ll;
         __syncthreads();
         const unsigned tb = offsets[tile], te = offsets[tile + 1];
         const unsigned beg = tb + it.y * (unsigned)t.CH;
         const unsigned end = min(min(te, beg + (unsigned)t.CH), cap);
         for (unsigned i = beg + threadIdx.x; i < end; i += blockDim.x) {
             const float4 r = sorted[i];
-            const float sv[3] = {r.x, r.y, r.z};
-            const int o[3] = {ox, oy, oz};
+            const unsigned pk = __float_as_uint(r.x);
+            const int cc[3] = {(int)(pk & 1023u), (int)((pk >> 10) & 1023u), (int)(pk >> 20)};
+            const float f[3] = {r.y, r.z, r.w};
             int l[3];
-            float wa[3][5];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                double x = shifted((double)sv[a], p.ishift, p.L);
-                int c = cell_of(x, p.d, N);
-                int lc = c - o[a];
+                int lc = cc[a] - o[a];
                 if (lc < -R) lc += N;
                 if (lc >= N - R) lc -= N;
-                l[a] = lc + R;
-                axis_weights<float>(p.scheme, (float)(x / p.d - (double)c - 0.5), wa[a]);
+                l[a] = lc + R;  // node c-R (local lc-R) sits at LDS index lc-R+2R
             }
-            const float wt = r.w;
-            for (int a = -R; a <= R; ++a) {
-                int ia = l[0] + a;
-                float va = wt * wa[0][a + 2];
-                if (va == 0.0f || ia < R || ia >= R + T) continue;
-                for (int b = -R; b <= R; ++b) {
-                    int ib = l[1] + b;
-                    float vb = va * wa[1][b + 2];
-                    if (vb == 0.0f || ib < R || ib >= R + T) continue;
-                    for (int e = -R; e <= R; ++e) {
-                        int ie = l[2] + e;
-                        float v = vb * wa[2][e + 2];
-                        if (v == 0.0f || ie < R || ie >= R + T) continue;
-                        atomicAdd(&lgrid[(ia * S + ib) * S + ie], v);
-                    }
-                }
-            }
+            const int base = (l[0] * S + l[1]) * S + l[2];
+            deposit_record<SCHEME, UNIT>(lgrid, S, base, f, UNIT ? 1.0f : wsorted[i]);
         }
         __syncthreads();
         const bool split = (te - tb) > (unsigned)t.CH;
         for (int q = threadIdx.x; q < T * T * T; q += blockDim.x) {
             int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
-            float v = lgrid[((ix + R) * S + (iy + R)) * S + (iz + R)];
-            size_t idx = ((size_t)(ox + ix) * N + (oy + iy)) * N + (oz + iz);
+            float v = from_fixed(lgrid[((ix + 2 * R) * S + (iy + 2 * R)) * S + (iz + 2 * R)]);
+            size_t idx = ((size_t)(o[0] + ix) * N + (o[1] + iy)) * N + (o[2] + iz);
             if (split) {
                 if (v != 0.0f) unsafeAtomicAdd(&mesh[idx], v);
             } else if (accumulate) {
@@ -353,6 +474,7 @@ static cm_status fill(cm_ctx *ctx, T *a, size_t n, T v) {
 }
 
 static int tile_edge(int N) {
+    if (N > 1024) return 0;  // records pack the cell index in 10 bits per axis
     if (N >= 32 && N % 16 == 0) return 16;
     if (N >= 16 && N % 8 == 0) return 8;
     return 0;
@@ -381,11 +503,27 @@ static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const f
     const size_t max_items = (size_t)ntiles + max_copies / t.CH + 1;
     if ((s = scratch(ctx, "mas.items", sizeof(uint2) * max_items, &items)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.nitems", sizeof(unsigned), &nitems)) != CM_OK) return s;
-    if ((s = fill(ctx, (unsigned *)counts, (size_t)ntiles, 0u)) != CM_OK) return s;
     const unsigned g = grid_for(p.n, 256, 8192);
-    if ((s = launch(ctx, "mas_tile_count", k_tile_count, dim3(g), dim3(256), 0, pos, p, t, (unsigned *)counts,
-                    dropped)) != CM_OK)
-        return s;
+    // LDS-privatised histogram when a workgroup's LDS holds one counter per tile
+    const size_t hist_lds = sizeof(unsigned) * (size_t)ntiles;
+    const bool lds_hist = hist_lds <= ctx->lds_per_block && ntiles <= 65536;
+    const int nrows = ctx->cu_count;
+    const size_t chunk = (p.n + nrows - 1) / nrows;
+    void *hist = nullptr;
+    if (lds_hist) {
+        if ((s = scratch(ctx, "mas.hist", sizeof(unsigned) * (size_t)nrows * ntiles, &hist)) != CM_OK) return s;
+        if ((s = launch(ctx, "mas_tile_count", k_tile_count_lds, dim3(nrows), dim3(1024), hist_lds, pos, p, t, ntiles,
+                        chunk, (unsigned *)hist, dropped)) != CM_OK)
+            return s;
+        if ((s = launch(ctx, "mas_tile_prefix", k_tile_prefix, dim3((ntiles + 255) / 256), dim3(256), 0,
+                        (unsigned *)hist, nrows, ntiles, (unsigned *)counts)) != CM_OK)
+            return s;
+    } else {
+        if ((s = fill(ctx, (unsigned *)counts, (size_t)ntiles, 0u)) != CM_OK) return s;
+        if ((s = launch(ctx, "mas_tile_count", k_tile_count, dim3(g), dim3(256), 0, pos, p, t, (unsigned *)counts,
+                        dropped)) != CM_OK)
+            return s;
+    }
     if ((s = launch(ctx, "mas_tile_scan", k_tile_scan, dim3(1), dim3(1024), 0, (const unsigned *)counts, ntiles,
                     t.CH, (unsigned *)offsets, (unsigned *)cursor, (unsigned *)item_off, (unsigned *)nitems)) != CM_OK)
         return s;
@@ -401,23 +539,48 @@ static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const f
         cap = std::min(max_copies, (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
     }
     if ((s = scratch(ctx, "mas.sorted", sizeof(float4) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
+    void *wsorted = nullptr;
+    if (w && (s = scratch(ctx, "mas.wsorted", sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3((ntiles + 255) / 256), dim3(256), 0,
                     (const unsigned *)item_off, ntiles, (uint2 *)items)) != CM_OK)
         return s;
-    if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter, dim3(g), dim3(256), 0, pos, w, p, t, (unsigned *)cursor,
-                    (float4 *)sorted, (unsigned)cap, overflow)) != CM_OK)
+    if (lds_hist) {
+        if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter_lds, dim3(nrows), dim3(1024), hist_lds, pos, w, p, t,
+                        ntiles, chunk, (const unsigned *)hist, (const unsigned *)offsets, (float4 *)sorted,
+                        (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
+            return s;
+    } else if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter, dim3(g), dim3(256), 0, pos, w, p, t,
+                           (unsigned *)cursor, (float4 *)sorted, (float *)wsorted, (unsigned)cap,
+                           overflow)) != CM_OK) {
         return s;
+    }
     if (!accumulate) {
         if ((s = launch(ctx, "mas_tile_zero_split", k_tile_zero_split, dim3(ntiles), dim3(256), 0,
                         (const unsigned *)offsets, t, p.N, mesh)) != CM_OK)
             return s;
     }
-    const int S = t.T + 2 * t.R;
-    const size_t lds = sizeof(float) * (size_t)S * S * S;
-    unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)ctx->cu_count * 16);
-    return launch(ctx, "mas_tile_deposit", k_tile_deposit, dim3(dg), dim3(256), lds, (const float4 *)sorted,
-                  (const unsigned *)offsets, (const uint2 *)items, (const unsigned *)nitems, p, t, (unsigned)cap,
-                  mesh, accumulate);
+    const int S = t.T + 4 * t.R;
+    const size_t lds = sizeof(long long) * (size_t)S * S * S;
+    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "mas: deposit grid of %zu bytes", lds);
+    unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)ctx->cu_count * 8);
+    const float *ws = (const float *)wsorted;
+    const dim3 gd(dg), bd(512);
+#define CM_DEPOSIT(SCH, UNIT)                                                                                   \
+    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT>, gd, bd, lds, (const float4 *)sorted, ws,        \
+           (const unsigned *)offsets, (const uint2 *)items, (const unsigned *)nitems, p, t, (unsigned)cap, mesh, \
+           accumulate)
+    switch (p.scheme * 2 + (w ? 0 : 1)) {
+    case 0: return CM_DEPOSIT(0, false);
+    case 1: return CM_DEPOSIT(0, true);
+    case 2: return CM_DEPOSIT(1, false);
+    case 3: return CM_DEPOSIT(1, true);
+    case 4: return CM_DEPOSIT(2, false);
+    case 5: return CM_DEPOSIT(2, true);
+    case 6: return CM_DEPOSIT(3, false);
+    case 7: return CM_DEPOSIT(3, true);
+    }
+#undef CM_DEPOSIT
+    return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
 }
 
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w, size_t n,

@@ -66,6 +66,19 @@ __device__ __forceinline__ int shell_of(const PkK &p, int i, int j, int k) {
         return (s < 0 || s >= p.nshells) ? -1 : s;
     }
     const int q = i * i + j * j + k * k;
+    if (p.int_rule) {
+        // integer DeltaKmin and ndel: float estimate, then exact integer edges
+        // (s*ndel + Dmin)^2 <= q < ((s+1)*ndel + Dmin)^2
+        int s = (int)floorf((sqrtf((float)q) - (float)p.Dmin_i) / (float)p.ndel_i);
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const long lo = (long)s * p.ndel_i + p.Dmin_i, hi = (long)(s + 1) * p.ndel_i + p.Dmin_i;
+            if (lo > 0 && (long)q < lo * lo) --s;
+            else if (hi >= 0 && (long)q >= hi * hi) ++s;
+        }
+        if ((long)q < (long)p.Dmin_i * p.Dmin_i) s = -1;
+        return (s < 0 || s >= p.nshells) ? -1 : s;
+    }
     const double t = (sqrt((double)q) - p.Dmin) / p.ndel;
     int s = (int)floor(t);
     if (p.int_rule) {

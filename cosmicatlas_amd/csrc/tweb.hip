@@ -92,25 +92,43 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
 
 
 // Fused FD Hessian + labels.  One workgroup owns a TY x TZ column of cells in
-// (y,z) and a segment of x planes; it marches along x holding, per (y,z)
-// position of the haloed region (TY+4) x (TZ+4), the last five phi values and
-// the G_a histories in registers, and only the current phi plane (halo 4) and
-// the centre plane's G_y, G_z in LDS.  Every phi value is read once per
-// workgroup (plus the y/z halo); G and T never touch HBM.  The composition
-// order of the reference is kept: G_a = D_a phi, then D_x G_x, D_x G_y, D_x G_z,
-// D_y G_y, D_y G_z, D_z G_z (Miscelanious.cpp:1684-1700).  Labels come from the
-// inertia of T - lambda_th I (positive_eigs); the closed-form eigenvalues are
-// evaluated only for the lambda output and for cells whose margin bound is
+// (y,z) and a segment of x planes and marches along x.  LDS holds a ring of
+// the last five phi planes (halo 4 in y,z) and the centre plane's G_y, G_z
+// (halo 2); each thread keeps, for its positions of the haloed region
+// (TY+4) x (TZ+4), the last five G_x, G_y, G_z values in registers.  The next
+// phi plane is loaded into registers while the current one is processed.  G and
+// T never touch HBM.  Composition order of the reference: G_a = D_a phi, then
+// D_x G_x, D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z
+// (Miscelanious.cpp:1684-1700).  Labels from the inertia of T - lambda_th I
+// (positive_eigs_f: float with a certified sign, else FP64); closed-form FP64
+// eigenvalues only for the lambda output and for cells whose margin bound is
 // below tol.
+// Rare path of the fused kernel (kept out of line so its FP64 registers do not
+// count against the marching loop): closed-form eigenvalues, optional lambda
+// output, and the exact margin test.  Returns 1 for a marginal cell.
+__device__ __attribute__((noinline)) unsigned eig_margin(float txx, float tyy, float tzz, float a10, float a20,
+                                                         float a21, double lth, double tol, bool test,
+                                                         float *lam, size_t cell, size_t n3) {
+    double e1, e2, e3;
+    eig3(txx, tyy, tzz, a10, a20, a21, e1, e2, e3);
+    if (lam) {
+        lam[cell] = (float)e1;
+        lam[n3 + cell] = (float)e2;
+        lam[2 * n3 + cell] = (float)e3;
+    }
+    return (test && fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth))) < tol) ? 1u : 0u;
+}
+
 template <int P>
 __global__ void __launch_bounds__(512) k_tweb_fused(const float *__restrict__ phi, int N, int TY, int TZ,
-                                                    int segLen, double ih, double lth, double tol,
+                                                    int segLen, float ih, double lth, double tol,
                                                     int8_t *__restrict__ labels, float *__restrict__ lam,
                                                     unsigned long long *__restrict__ stats) {
     extern __shared__ float sm[];
     const int HY = TY + 8, HZ = TZ + 8, GY = TY + 4, GZ = TZ + 4;
-    float *sphi = sm;
-    float *sgy = sphi + HY * HZ;
+    const int HP = HY * HZ;  // one phi plane
+    float *ring = sm;        // 5 planes
+    float *sgy = ring + 5 * HP;
     float *sgz = sgy + GY * GZ;
     __shared__ unsigned scnt[5];
     const int tid = threadIdx.x, NT = blockDim.x;
@@ -120,44 +138,58 @@ __global__ void __launch_bounds__(512) k_tweb_fused(const float *__restrict__ ph
     const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
     const int npos = GY * GZ;
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
-    float ph[P][5], gx[P][5], gy[P][7], gz[P][7];
+    const float lthf = (float)lth;
+    float gx[P][5], gy[P][5], gz[P][5];
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
+    for (int k = 0; k < P; ++k)
 #pragma unroll
-        for (int i = 0; i < 5; ++i) ph[k][i] = gx[k][i] = 0.f;
-#pragma unroll
-        for (int i = 0; i < 7; ++i) gy[k][i] = gz[k][i] = 0.f;
-    }
+        for (int i = 0; i < 5; ++i) gx[k][i] = gy[k][i] = gz[k][i] = 0.f;
     unsigned loc[5] = {0, 0, 0, 0, 0};
-    for (int s = x0 - 4; s < x1 + 4; ++s) {
+    constexpr int PF = 4;  // phi values per thread per plane (HP <= PF*NT)
+    float pf[PF];
+    auto fetch = [&](int s) {
         const int sp = (s % N + N) % N;
         const float *plane = phi + (size_t)sp * NN * NN;
-        for (int q = tid; q < HY * HZ; q += NT) {
-            const int yy = q / HZ, zz = q - yy * HZ;
-            const int y = (ty0 - 4 + yy + N) % N, z = (tz0 - 4 + zz + N) % N;
-            sphi[q] = plane[(size_t)y * NN + z];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int q = tid + k * NT;
+            if (q < HP) {
+                const int yy = q / HZ, zz = q - yy * HZ;
+                const int y = (ty0 - 4 + yy + N) % N, z = (tz0 - 4 + zz + N) % N;
+                pf[k] = plane[(size_t)y * NN + z];
+            }
         }
+    };
+    fetch(x0 - 4);
+    for (int s = x0 - 4; s < x1 + 4; ++s) {
+        float *cur = ring + ((s - x0 + 8) % 5) * HP;  // slot of plane s
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int q = tid + k * NT;
+            if (q < HP) cur[q] = pf[k];
+        }
+        if (s + 1 < x1 + 4) fetch(s + 1);  // in flight while plane s is processed
         __syncthreads();
+        // planes s-4..s live in the ring; G_x(s-2) from the x-stencil, G_y and
+        // G_z(s-2) from plane s-2's y/z neighbours
+        const float *pm4 = ring + ((s - x0 + 4) % 5) * HP, *pm3 = ring + ((s - x0 + 5) % 5) * HP;
+        const float *pm2 = ring + ((s - x0 + 6) % 5) * HP, *pm1 = ring + ((s - x0 + 7) % 5) * HP;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             const int q = tid + k * NT;
             if (q < npos) {
-                const int a = q / GZ, b = q - a * GZ;  // G-region coords; phi region = +2
-                const float *r = sphi + (a + 2) * HZ + (b + 2);
+                const int a = q / GZ, b = q - a * GZ;
+                const int o = (a + 2) * HZ + (b + 2);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) ph[k][i] = ph[k][i + 1];
-                ph[k][4] = r[0];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) gx[k][i] = gx[k][i + 1];
-                gx[k][4] = fd4(ph[k][0], ph[k][1], ph[k][3], ph[k][4], ih);  // G_x at plane s-2
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
+                for (int i = 0; i < 4; ++i) {
+                    gx[k][i] = gx[k][i + 1];
                     gy[k][i] = gy[k][i + 1];
                     gz[k][i] = gz[k][i + 1];
                 }
-                gy[k][6] = fd4(r[-2 * HZ], r[-HZ], r[HZ], r[2 * HZ], ih);  // G_y at plane s
-                gz[k][6] = fd4(r[-2], r[-1], r[1], r[2], ih);              // G_z at plane s
-                sgy[q] = gy[k][2];                                         // centre plane c = s-4
+                gx[k][4] = fd4f(pm4[o], pm3[o], pm1[o], cur[o], ih);
+                gy[k][4] = fd4f(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ], ih);
+                gz[k][4] = fd4f(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2], ih);
+                sgy[q] = gy[k][2];  // centre plane c = s-4
                 sgz[q] = gz[k][2];
             }
         }
@@ -170,30 +202,23 @@ __global__ void __launch_bounds__(512) k_tweb_fused(const float *__restrict__ ph
             if (q >= npos) continue;
             const int a = q / GZ, b = q - a * GZ;
             if (a < 2 || a >= TY + 2 || b < 2 || b >= TZ + 2) continue;
-            const float txx = fd4(gx[k][0], gx[k][1], gx[k][3], gx[k][4], ih);
-            const float a10 = fd4(gy[k][0], gy[k][1], gy[k][3], gy[k][4], ih);
-            const float a20 = fd4(gz[k][0], gz[k][1], gz[k][3], gz[k][4], ih);
-            const float tyy = fd4(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ], ih);
-            const float a21 = fd4(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ], ih);
-            const float tzz = fd4(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2], ih);
-            double adet, fro2;
-            const int v = positive_eigs((double)txx - lth, (double)tyy - lth, (double)tzz - lth, a10, a20, a21,
-                                        adet, fro2);
+            const float txx = fd4f(gx[k][0], gx[k][1], gx[k][3], gx[k][4], ih);
+            const float a10 = fd4f(gy[k][0], gy[k][1], gy[k][3], gy[k][4], ih);
+            const float a20 = fd4f(gz[k][0], gz[k][1], gz[k][3], gz[k][4], ih);
+            const float tyy = fd4f(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ], ih);
+            const float a21 = fd4f(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ], ih);
+            const float tzz = fd4f(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2], ih);
+            float det, det_err, fro2;
+            const int v = positive_eigs_f(txx - lthf, tyy - lthf, tzz - lthf, a10, a20, a21, det, det_err, fro2);
             const int lab = 4 - v;
             const size_t cell = ((size_t)c * NN + (ty0 + a - 2)) * NN + (tz0 + b - 2);
             labels[cell] = (int8_t)lab;
             loc[lab - 1]++;
-            const bool maybe_marginal = adet < tol * fro2 || fro2 < tol * tol;
-            if (lam || maybe_marginal) {
-                double e1, e2, e3;
-                eig3(txx, tyy, tzz, a10, a20, a21, e1, e2, e3);
-                if (lam) {
-                    lam[cell] = (float)e1;
-                    lam[n3 + cell] = (float)e2;
-                    lam[2 * n3 + cell] = (float)e3;
-                }
-                if (maybe_marginal && fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth))) < tol) loc[4]++;
-            }
+            // margin bound min|mu| >= |det B| / ||B||_F^2, taken conservatively
+            // (|det| minus its rounding bound); the FP64 eigenvalues settle it
+            const bool maybe_marginal = tol > 0.0 && (fabsf(det) - det_err) < (float)tol * fro2 * 1.0001f;
+            if (lam || maybe_marginal)
+                loc[4] += eig_margin(txx, tyy, tzz, a10, a20, a21, lth, tol, maybe_marginal, lam, cell, n3);
         }
     }
 #pragma unroll
@@ -225,22 +250,28 @@ static int pick_tile(int N, const int *cands, int nc) {
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
                             float *lam, unsigned long long *stats) {
     static const int cz[] = {64, 32, 16, 8}, cy[] = {16, 8};
-    const int TZ = pick_tile(N, cz, 4), TY = pick_tile(N, cy, 2);
+    int TZ = pick_tile(N, cz, 4), TY = pick_tile(N, cy, 2);
+    if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 && ctx->tune.tz <= 64) TZ = ctx->tune.tz;
+    if (ctx->tune.ty > 0 && N % ctx->tune.ty == 0 && ctx->tune.ty <= 32) TY = ctx->tune.ty;
     if (!TZ || !TY) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
     const int npos = (TY + 4) * (TZ + 4);
     const int NT = 512;
     const int P = (npos + NT - 1) / NT;
-    const int nseg = N >= 256 ? 4 : (N >= 64 ? 2 : 1);
+    int nseg = N >= 256 ? 4 : (N >= 64 ? 2 : 1);
+    if (ctx->tune.tseg > 0 && ctx->tune.tseg <= N / 4) nseg = ctx->tune.tseg;
     const int segLen = (N + nseg - 1) / nseg;
     dim3 grid((unsigned)((N / TY) * (N / TZ)), (unsigned)nseg);
-    const size_t lds = sizeof(float) * ((size_t)(TY + 8) * (TZ + 8) + 2 * (size_t)npos);
+    const size_t lds = sizeof(float) * (5 * (size_t)(TY + 8) * (TZ + 8) + 2 * (size_t)npos);
+    if ((size_t)(TY + 8) * (TZ + 8) > 4 * 512 || P > 3 || lds > ctx->lds_per_block)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d too large", TY, TZ);
     const double ih = 1.0 / (12.0 * (L / (double)N));
     cm_status s;
     if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
+    const float ihf = (float)ih;
     switch (P) {
-    case 1: return launch(ctx, "tweb_fused", k_tweb_fused<1>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ih, lth, tol, labels, lam, stats);
-    case 2: return launch(ctx, "tweb_fused", k_tweb_fused<2>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ih, lth, tol, labels, lam, stats);
-    case 3: return launch(ctx, "tweb_fused", k_tweb_fused<3>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ih, lth, tol, labels, lam, stats);
+    case 1: return launch(ctx, "tweb_fused", k_tweb_fused<1>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ihf, lth, tol, labels, lam, stats);
+    case 2: return launch(ctx, "tweb_fused", k_tweb_fused<2>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ihf, lth, tol, labels, lam, stats);
+    case 3: return launch(ctx, "tweb_fused", k_tweb_fused<3>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ihf, lth, tol, labels, lam, stats);
     }
     return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile needs %d positions per thread", P);
 }

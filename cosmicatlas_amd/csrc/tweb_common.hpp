@@ -58,6 +58,11 @@ __device__ __forceinline__ void eig3(double a00, double a11, double a22, double 
     e2 = 3.0 * q - e1 - e3;
 }
 
+// The same stencil in float (fused kernel): 3 FMAs and a multiply.
+__device__ __forceinline__ float fd4f(float fll, float fl, float fr, float frr, float ih) {
+    return ((fll - frr) + 8.0f * (fr - fl)) * ih;
+}
+
 }  // namespace cm
 
 namespace cm {
@@ -87,6 +92,40 @@ __device__ __forceinline__ int positive_eigs(double b00, double b11, double b22,
         prev = c[i];
     }
     return changes;
+}
+
+
+// Same count from float T: the coefficients are evaluated in float together
+// with a bound on their rounding error (a few ulps of the sum of absolute
+// values of the terms).  When every coefficient is farther from zero than its
+// bound its sign is certain and the float count equals the FP64 count;
+// otherwise the FP64 evaluation decides.
+__device__ __forceinline__ int positive_eigs_f(float b00, float b11, float b22, float a10, float a20, float a21,
+                                               float &det, float &det_err, float &fro2) {
+    const float e1 = b00 + b11 + b22;
+    const float m1 = fabsf(b00) + fabsf(b11) + fabsf(b22);
+    const float p01 = b00 * b11, p02 = b00 * b22, p12 = b11 * b22;
+    const float q10 = a10 * a10, q20 = a20 * a20, q21 = a21 * a21;
+    const float e2 = (p01 + p02 + p12) - (q10 + q20 + q21);
+    const float m2 = fabsf(p01) + fabsf(p02) + fabsf(p12) + q10 + q20 + q21;
+    const float c0 = b11 * b22 - q21, c1 = a10 * b22 - a21 * a20, c2 = a10 * a21 - b11 * a20;
+    const float e3 = b00 * c0 - a10 * c1 + a20 * c2;
+    const float m3 = fabsf(b00) * (fabsf(p12) + q21) + fabsf(a10) * (fabsf(a10 * b22) + fabsf(a21 * a20)) +
+                     fabsf(a20) * (fabsf(a10 * a21) + fabsf(b11 * a20));
+    const float eps = 8.0f * 5.9604645e-8f;  // 8 units of float roundoff
+    det = e3;
+    det_err = 3.0f * eps * m3;
+    fro2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * (q10 + q20 + q21);
+    if (fabsf(e1) <= eps * m1 || fabsf(e2) <= 2.0f * eps * m2 || fabsf(e3) <= 3.0f * eps * m3 || !(m3 < 1e30f)) {
+        double ad, f2;
+        return positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+    }
+    // no zero roots: sign changes of 1, -e1, e2, -e3
+    int ch = 0;
+    ch += (e1 > 0.f);            // 1 -> -e1
+    ch += ((-e1 > 0.f) != (e2 > 0.f));
+    ch += ((e2 > 0.f) != (-e3 > 0.f));
+    return ch;
 }
 
 }  // namespace cm
