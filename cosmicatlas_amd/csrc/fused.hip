@@ -49,8 +49,38 @@ __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict_
     fft_run<float, E, true, false>(lds, 0, pl, nb, tw, srcL, dstG);
 }
 
-template <int E>
-__global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevPlan pl, int nb, int ncols, int N,
+// Row-local lane shifts (DPP row_shr / row_shl inside rows of 16 lanes): a lane
+// whose source falls outside its row keeps its own value, which the callers
+// never use (they only combine lanes of the same nb-lane group, nb <= 16).
+template <int OFF> __device__ __forceinline__ int dpp_shr(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x110 + OFF, 0xf, 0xf, false);
+}
+template <int OFF> __device__ __forceinline__ float dpp_shr(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x110 + OFF, 0xf, 0xf,
+                                                      false));
+}
+template <int OFF> __device__ __forceinline__ int dpp_shl(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x100 + OFF, 0xf, 0xf, false);
+}
+
+// Segmented inclusive scan over groups of NB lanes (NB = 8 or 16) keyed by the
+// shell: lanes of one group with equal shells form one contiguous run.
+template <int NB, int OFF>
+__device__ __forceinline__ void seg_step(int gl, int &sh, float &a0, float &a1, float &a2, float &a3, int &c) {
+    const int ush = dpp_shr<OFF>(sh);
+    const float u0 = dpp_shr<OFF>(a0), u1 = dpp_shr<OFF>(a1), u2 = dpp_shr<OFF>(a2), u3 = dpp_shr<OFF>(a3);
+    const int uc = dpp_shr<OFF>(c);
+    if (gl >= OFF && ush == sh) {
+        a0 += u0;
+        a1 += u1;
+        a2 += u2;
+        a3 += u3;
+        c += uc;
+    }
+}
+
+template <int E, int NB>
+__global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevPlan pl, int ncols, int N,
                                                  const float2 *__restrict__ tw, PkK pk,
                                                  const double *__restrict__ tabs, double *__restrict__ gsum,
                                                  unsigned long long *__restrict__ gcnt, int do_tweb, float inv_kf2,
@@ -59,7 +89,7 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
     const int M1 = N / 2 + 1;
     const int ns = pk.nshells;
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
-    double *s_acc = reinterpret_cast<double *>(lds + (size_t)nb * N);  // 4*ns
+    double *s_acc = reinterpret_cast<double *>(lds + (size_t)NB * N);  // 4*ns
     unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);     // ns
     float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                // M1
     float *s_sq = s_ic + M1;                                            // M1
@@ -76,29 +106,29 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
     // -scale/k^2 = -(N^3/N_gal) / (kf^2 q)
     const float fscale = (do_tweb && nobj[0] > 0.0) ? (float)(-(n3 / nobj[0]) * (double)inv_kf2) : 0.0f;
     const size_t xstride = (size_t)N * ncols;
-    const int nkzb = (ncols + nb - 1) / nb;
+    const int nkzb = (ncols + NB - 1) / NB;
     const int ntiles = nkzb * N;
     const int h = N / 2;
     __syncthreads();
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int kzb = tile % nkzb, iy = tile / nkzb;
-        const int kz0 = kzb * nb;
+        const int kz0 = kzb * NB;
         const size_t base = (size_t)iy * ncols + kz0;
-        const int valid = min(nb, ncols - kz0);
-        x_fft_fwd_to_lds<E>(data, base, xstride, valid, lds, pl, nb, tw);
+        const int valid = min(NB, ncols - kz0);
+        x_fft_fwd_to_lds<E>(data, base, xstride, valid, lds, pl, NB, tw);
         __syncthreads();
         // shells (+ multiply), element t = e*nb + b (kz fastest): runs of equal
         // (kx, shell) lie inside one group of nb lanes
-        const int total = nb * N;
+        const int total = NB * N;
         const int ky = iy <= h ? iy : iy - N;
         const float gy = s_g[iy];
         for (int t0 = tid & ~63; t0 < total; t0 += NT) {
             const int t = t0 + lane;
             int sh = -1;
             float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-            unsigned c = 0;
+            int c = 0;
             if (t < total) {
-                const int b = t % nb, ix = t / nb, iz = kz0 + b;
+                const int b = t % NB, ix = t / NB, iz = kz0 + b;
                 if (b < valid) {
                     float2 v = lds[t];
                     int fx, fy;
@@ -108,51 +138,42 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
                             const float pw = v.x * v.x + v.y * v.y;
                             const float ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
                             const int q = fx * fx + fy * fy + iz * iz;
-                            const float mu2 = (float)(iz * iz) / (float)q;
+                            const float mu2 = (float)(iz * iz) * __builtin_amdgcn_rcpf((float)q);
                             const float pc = pw * ic;
                             a0 = pc;
                             a1 = s_sq[fx] * s_sq[fy] * s_sq[iz] * ic;
-                            a2 = 0.5f * (3.0f * mu2 - 1.0f) * pc;
-                            a3 = (35.0f * mu2 * mu2 - 30.0f * mu2 + 3.0f) * 0.125f * pc;
+                            a2 = (1.5f * mu2 - 0.5f) * pc;
+                            a3 = ((4.375f * mu2 - 3.75f) * mu2 + 0.375f) * pc;
                             c = 1;
                         }
                     }
                     if (do_tweb) {
                         const int kx = ix <= h ? ix : ix - N;
                         const int q2 = kx * kx + ky * ky + iz * iz;
-                        const float f = q2 > 0 ? fscale * s_g[ix] * gy * s_g[iz] / (float)q2 : 0.0f;
+                        const float f = q2 > 0 ? fscale * s_g[ix] * gy * s_g[iz] * __builtin_amdgcn_rcpf((float)q2)
+                                               : 0.0f;
                         v.x *= f;
                         v.y *= f;
                         lds[t] = v;
                     }
                 }
             }
-            for (int off = 1; off < nb; off <<= 1) {
-                const int ush = __shfl_up(sh, off, nb);
-                const float u0 = __shfl_up(a0, off, nb);
-                const float u1 = __shfl_up(a1, off, nb);
-                const float u2 = __shfl_up(a2, off, nb);
-                const float u3 = __shfl_up(a3, off, nb);
-                const unsigned uc = __shfl_up(c, off, nb);
-                if ((lane % nb) >= off && ush == sh) {
-                    a0 += u0;
-                    a1 += u1;
-                    a2 += u2;
-                    a3 += u3;
-                    c += uc;
-                }
-            }
-            const int nsh = __shfl_down(sh, 1, nb);
-            if (sh >= 0 && ((lane % nb) == nb - 1 || nsh != sh)) {
+            const int gl = lane & (NB - 1);
+            seg_step<NB, 1>(gl, sh, a0, a1, a2, a3, c);
+            seg_step<NB, 2>(gl, sh, a0, a1, a2, a3, c);
+            seg_step<NB, 4>(gl, sh, a0, a1, a2, a3, c);
+            if constexpr (NB == 16) seg_step<NB, 8>(gl, sh, a0, a1, a2, a3, c);
+            const int nsh = dpp_shl<1>(sh);
+            if (sh >= 0 && (gl == NB - 1 || nsh != sh)) {
                 atomicAdd(&s_acc[sh], (double)a0);
                 atomicAdd(&s_acc[ns + sh], (double)a1);
                 atomicAdd(&s_acc[2 * ns + sh], (double)a2);
                 atomicAdd(&s_acc[3 * ns + sh], (double)a3);
-                atomicAdd(&s_cnt[sh], c);
+                atomicAdd(&s_cnt[sh], (unsigned)c);
             }
         }
         __syncthreads();
-        if (do_tweb) x_fft_inv_from_lds<E>(data, base, xstride, valid, lds, pl, nb, tw);
+        if (do_tweb) x_fft_inv_from_lds<E>(data, base, xstride, valid, lds, pl, NB, tw);
         __syncthreads();
     }
     for (int i = tid; i < ns; i += NT) {
@@ -250,10 +271,10 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
         has8 |= plan.r[i] == 8;
     }
     const int E = has3 ? 24 : ((has8 && N >= 64) ? 16 : 8);
-    int nb = 8;
-    if (ctx->tune.xnb > 0 && (ctx->tune.xnb & (ctx->tune.xnb - 1)) == 0 && ctx->tune.xnb <= 64) nb = ctx->tune.xnb;
-    while (nb * N / E > 512 && nb > 1) nb /= 2;
-    while ((nb * N) % E) nb *= 2;
+    int nb = (ctx->tune.xnb == 16) ? 16 : 8;  // lane groups of the DPP scan: 8 or 16
+    if (nb * N / E > 512) nb = 8;
+    if (N % E || nb * N / E > 512)  // whole lane groups per workgroup
+        return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: N=%d has no 8/16-column block", N);
     const int NT = nb * N / E;
     const size_t lds = sizeof(float2) * (size_t)nb * N + sizeof(double) * 4 * (size_t)ns +
                        sizeof(unsigned) * (size_t)ns + sizeof(float) * (2 * (size_t)M1 + (size_t)N);
@@ -283,9 +304,18 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     auto *dkp = (float2 *)dk;
     auto *twp = (const float2 *)tw;
     switch (E) {
-    case 8: s = launch(ctx, "fused_x", k_x_fused<8>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj); break;
-    case 16: s = launch(ctx, "fused_x", k_x_fused<16>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj); break;
-    case 24: s = launch(ctx, "fused_x", k_x_fused<24>, dim3(grid), dim3(NT), lds, dkp, dp, nb, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj); break;
+    case 8:
+        s = nb == 8 ? launch(ctx, "fused_x", k_x_fused<8, 8>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj)
+                    : launch(ctx, "fused_x", k_x_fused<8, 16>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj);
+        break;
+    case 16:
+        s = nb == 8 ? launch(ctx, "fused_x", k_x_fused<16, 8>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj)
+                    : launch(ctx, "fused_x", k_x_fused<16, 16>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj);
+        break;
+    case 24:
+        s = nb == 8 ? launch(ctx, "fused_x", k_x_fused<24, 8>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj)
+                    : launch(ctx, "fused_x", k_x_fused<24, 16>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj);
+        break;
     default: return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: E=%d", E);
     }
     if (s != CM_OK) return s;

@@ -167,6 +167,7 @@ PkK make_pkk(const cm_pk_opts *o) {
     p.dkx_f = (float)kf;
     p.kmin_f = (float)o->kmin;
     p.rdk_f = (float)(1.0 / (double)(float)o->dk);
+    p.rndel_f = p.int_rule ? (float)(1.0 / (double)p.ndel_i) : 0.0f;
     return p;
 }
 

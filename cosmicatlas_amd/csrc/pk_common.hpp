@@ -22,7 +22,7 @@ struct PkK {
     int Dmin_i, ndel_i;
     double Dmin, ndel;  // kmin/k_f, dk/k_f
     float dkx_f, kmin_f, rdk_f;
-    float pad;
+    float rndel_f;  // 1/ndel for the integer rule's estimate
 };
 
 // Which stored half-spectrum modes power_spectrum_fkp visits
@@ -67,16 +67,18 @@ __device__ __forceinline__ int shell_of(const PkK &p, int i, int j, int k) {
     }
     const int q = i * i + j * j + k * k;
     if (p.int_rule) {
-        // integer DeltaKmin and ndel: float estimate, then exact integer edges
-        // (s*ndel + Dmin)^2 <= q < ((s+1)*ndel + Dmin)^2
-        int s = (int)floorf((sqrtf((float)q) - (float)p.Dmin_i) / (float)p.ndel_i);
+        // integer DeltaKmin and ndel: float estimate (hardware sqrt and a
+        // reciprocal are enough), then exact integer edges
+        // (s*ndel + Dmin)^2 <= q < ((s+1)*ndel + Dmin)^2 in 32-bit arithmetic
+        // (q <= 3 (N/2)^2 and the edges stay below 2^31 for N <= 1024)
+        int s = (int)floorf((__builtin_sqrtf((float)q) - (float)p.Dmin_i) * p.rndel_f);
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
-            const long lo = (long)s * p.ndel_i + p.Dmin_i, hi = (long)(s + 1) * p.ndel_i + p.Dmin_i;
-            if (lo > 0 && (long)q < lo * lo) --s;
-            else if (hi >= 0 && (long)q >= hi * hi) ++s;
+            const int lo = s * p.ndel_i + p.Dmin_i, hi = (s + 1) * p.ndel_i + p.Dmin_i;
+            if (lo > 0 && q < lo * lo) --s;
+            else if (hi >= 0 && q >= hi * hi) ++s;
         }
-        if ((long)q < (long)p.Dmin_i * p.Dmin_i) s = -1;
+        if (q < p.Dmin_i * p.Dmin_i) s = -1;
         return (s < 0 || s >= p.nshells) ? -1 : s;
     }
     const double t = (sqrt((double)q) - p.Dmin) / p.ndel;

@@ -4,7 +4,9 @@
 //   G -> T_ab = D_b G_a -> FP64 eigenvalues -> label, counts         (k_hess_eig)
 // The lower triangle the reference feeds to GSL is A10 = D_x G_y,
 // A20 = D_x G_z, A21 = D_y G_z (Miscelanious.cpp:1684-1740).
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "cm_internal.hpp"
 #include "tweb_common.hpp"
@@ -94,11 +96,13 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
 // Fused FD Hessian + labels.  One workgroup owns a TY x TZ column of cells in
 // (y,z) and a segment of x planes and marches along x.  LDS holds a ring of
 // the last five phi planes (halo 4 in y,z) and the centre plane's G_y, G_z
-// (halo 2); each thread keeps, for its positions of the haloed region
-// (TY+4) x (TZ+4), the last five G_x, G_y, G_z values in registers.  The next
-// phi plane is loaded into registers while the current one is processed.  G and
-// T never touch HBM.  Composition order of the reference: G_a = D_a phi, then
-// D_x G_x, D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z
+// (halo 2).  Each thread owns IP interior cells and keeps their last five
+// G_x, G_y, G_z values in registers; the march is unrolled by five so the ring
+// slots and register histories are addressed with compile-time indices (no
+// moves).  Halo values of G_y, G_z are computed directly on the centre plane.
+// The next phi plane is loaded into registers while the current one is
+// processed.  G and T never touch HBM.  Composition order of the reference:
+// G_a = D_a phi, then D_x G_x, D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z
 // (Miscelanious.cpp:1684-1700).  Labels from the inertia of T - lambda_th I
 // (positive_eigs_f: float with a certified sign, else FP64); closed-form FP64
 // eigenvalues only for the lambda output and for cells whose margin bound is
@@ -119,99 +123,121 @@ __device__ __attribute__((noinline)) unsigned eig_margin(float txx, float tyy, f
     return (test && fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth))) < tol) ? 1u : 0u;
 }
 
-template <int P>
-__global__ void __launch_bounds__(512) k_tweb_fused(const float *__restrict__ phi, int N, int TY, int TZ,
-                                                    int segLen, float ih, double lth, double tol,
-                                                    int8_t *__restrict__ labels, float *__restrict__ lam,
-                                                    unsigned long long *__restrict__ stats) {
+template <int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi, int N, int segLen, float ih,
+                                                   double lth, double tol, int8_t *__restrict__ labels,
+                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats) {
+    constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
+    constexpr int GZ = TZ + 4;                             // centre-plane G (halo 2), rows TY + 4
+    constexpr int IP = TY * TZ / NT;                       // interior cells per thread
+    constexpr int PF = (HP + NT - 1) / NT;                 // phi values per thread per plane
+    constexpr int NH1 = 4 * TZ, NH2 = 4 * TY;              // halo positions: y rows, z columns
+    static_assert(IP * NT == TY * TZ && NH1 + NH2 <= NT, "tile does not match the workgroup");
     extern __shared__ float sm[];
-    const int HY = TY + 8, HZ = TZ + 8, GY = TY + 4, GZ = TZ + 4;
-    const int HP = HY * HZ;  // one phi plane
-    float *ring = sm;        // 5 planes
+    float *ring = sm;  // 5 planes
     float *sgy = ring + 5 * HP;
-    float *sgz = sgy + GY * GZ;
+    float *sgz = sgy + (TY + 4) * GZ;
     __shared__ unsigned scnt[5];
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tid = threadIdx.x;
     if (tid < 5) scnt[tid] = 0;
     const int ntz = N / TZ;
     const int ty0 = (blockIdx.x / ntz) * TY, tz0 = (blockIdx.x % ntz) * TZ;
     const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
-    const int npos = GY * GZ;
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
     const float lthf = (float)lth;
-    float gx[P][5], gy[P][5], gz[P][5];
+
+    // this thread's phi loads: in-plane offsets with periodic wrap in y and z
+    int foff[PF];
 #pragma unroll
-    for (int k = 0; k < P; ++k)
+    for (int k = 0; k < PF; ++k) {
+        const int q = tid + k * NT;
+        const int yy = q / HZ, zz = q - (q / HZ) * HZ;
+        foff[k] = q < HP ? ((ty0 - 4 + yy + N) % N) * N + (tz0 - 4 + zz + N) % N : 0;
+    }
+    // interior cells: phi-plane index, centre-G index, in-plane output offset
+    int po[IP], gq[IP], coff[IP];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) gx[k][i] = gy[k][i] = gz[k][i] = 0.f;
+    for (int k = 0; k < IP; ++k) {
+        const int i = tid + k * NT, ia = i / TZ, ib = i % TZ;
+        po[k] = (ia + 4) * HZ + ib + 4;
+        gq[k] = (ia + 2) * GZ + ib + 2;
+        coff[k] = (ty0 + ia) * N + tz0 + ib;
+    }
+    // at most one halo position per thread: kind 1 = y-halo row (G_y, G_z),
+    // kind 2 = z-halo column (G_z)
+    int hkind = 0, hq = 0, hpo = 0;
+    if (tid < NH1) {
+        const int r = tid / TZ, a = r < 2 ? r : TY + r, b = 2 + tid % TZ;
+        hkind = 1;
+        hq = a * GZ + b;
+        hpo = (a + 2) * HZ + b + 2;
+    } else if (tid < NH1 + NH2) {
+        const int j = tid - NH1, r = j & 3, a = 2 + (j >> 2), b = r < 2 ? r : TZ + r;
+        hkind = 2;
+        hq = a * GZ + b;
+        hpo = (a + 2) * HZ + b + 2;
+    }
+
+    float hx[IP][5], hy[IP][5], hz[IP][5];
+#pragma unroll
+    for (int k = 0; k < IP; ++k)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) hx[k][i] = hy[k][i] = hz[k][i] = 0.f;
     unsigned loc[5] = {0, 0, 0, 0, 0};
-    constexpr int PF = 4;  // phi values per thread per plane (HP <= PF*NT)
     float pf[PF];
     auto fetch = [&](int s) {
         const int sp = (s % N + N) % N;
         const float *plane = phi + (size_t)sp * NN * NN;
 #pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            const int q = tid + k * NT;
-            if (q < HP) {
-                const int yy = q / HZ, zz = q - yy * HZ;
-                const int y = (ty0 - 4 + yy + N) % N, z = (tz0 - 4 + zz + N) % N;
-                pf[k] = plane[(size_t)y * NN + z];
-            }
-        }
+        for (int k = 0; k < PF; ++k)
+            if (tid + k * NT < HP) pf[k] = plane[foff[k]];
     };
-    fetch(x0 - 4);
-    for (int s = x0 - 4; s < x1 + 4; ++s) {
-        float *cur = ring + ((s - x0 + 8) % 5) * HP;  // slot of plane s
+
+    // Step s (J = (s - x0 + 4) mod 5): plane s enters ring slot (J+4)%5; G at
+    // plane s-2 goes to history slot J; cells of plane c = s-4 are output, with
+    // G(c-2..c+2) in history slots (J+1)%5, (J+2)%5, (J+3)%5, (J+4)%5, J.
+    auto step = [&](auto Jc, int s) {
+        constexpr int J = decltype(Jc)::value;
+        float *cur = ring + ((J + 4) % 5) * HP;
 #pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            const int q = tid + k * NT;
-            if (q < HP) cur[q] = pf[k];
-        }
+        for (int k = 0; k < PF; ++k)
+            if (tid + k * NT < HP) cur[tid + k * NT] = pf[k];
         if (s + 1 < x1 + 4) fetch(s + 1);  // in flight while plane s is processed
         __syncthreads();
-        // planes s-4..s live in the ring; G_x(s-2) from the x-stencil, G_y and
-        // G_z(s-2) from plane s-2's y/z neighbours
-        const float *pm4 = ring + ((s - x0 + 4) % 5) * HP, *pm3 = ring + ((s - x0 + 5) % 5) * HP;
-        const float *pm2 = ring + ((s - x0 + 6) % 5) * HP, *pm1 = ring + ((s - x0 + 7) % 5) * HP;
+        const float *pm4 = ring + (J % 5) * HP, *pm3 = ring + ((J + 1) % 5) * HP;
+        const float *pm2 = ring + ((J + 2) % 5) * HP, *pm1 = ring + ((J + 3) % 5) * HP;
+        if (s >= x0) {
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const int q = tid + k * NT;
-            if (q < npos) {
-                const int a = q / GZ, b = q - a * GZ;
-                const int o = (a + 2) * HZ + (b + 2);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    gx[k][i] = gx[k][i + 1];
-                    gy[k][i] = gy[k][i + 1];
-                    gz[k][i] = gz[k][i + 1];
-                }
-                gx[k][4] = fd4f(pm4[o], pm3[o], pm1[o], cur[o], ih);
-                gy[k][4] = fd4f(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ], ih);
-                gz[k][4] = fd4f(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2], ih);
-                sgy[q] = gy[k][2];  // centre plane c = s-4
-                sgz[q] = gz[k][2];
+            for (int k = 0; k < IP; ++k) {
+                const int o = po[k];
+                hx[k][J] = fd4f(pm4[o], pm3[o], pm1[o], cur[o], ih);
+                hy[k][J] = fd4f(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ], ih);
+                hz[k][J] = fd4f(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2], ih);
+                sgy[gq[k]] = hy[k][(J + 3) % 5];
+                sgz[gq[k]] = hz[k][(J + 3) % 5];
             }
+        }
+        if (s >= x0 + 4 && hkind) {
+            const int o = hpo;
+            if (hkind == 1) sgy[hq] = fd4f(pm4[o - 2 * HZ], pm4[o - HZ], pm4[o + HZ], pm4[o + 2 * HZ], ih);
+            sgz[hq] = fd4f(pm4[o - 2], pm4[o - 1], pm4[o + 1], pm4[o + 2], ih);
         }
         __syncthreads();
         const int c = s - 4;
-        if (c < x0) continue;
+        if (c < x0) return;
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const int q = tid + k * NT;
-            if (q >= npos) continue;
-            const int a = q / GZ, b = q - a * GZ;
-            if (a < 2 || a >= TY + 2 || b < 2 || b >= TZ + 2) continue;
-            const float txx = fd4f(gx[k][0], gx[k][1], gx[k][3], gx[k][4], ih);
-            const float a10 = fd4f(gy[k][0], gy[k][1], gy[k][3], gy[k][4], ih);
-            const float a20 = fd4f(gz[k][0], gz[k][1], gz[k][3], gz[k][4], ih);
+        for (int k = 0; k < IP; ++k) {
+            const int q = gq[k];
+            const float txx = fd4f(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J], ih);
+            const float a10 = fd4f(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J], ih);
+            const float a20 = fd4f(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J], ih);
             const float tyy = fd4f(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ], ih);
             const float a21 = fd4f(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ], ih);
             const float tzz = fd4f(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2], ih);
             float det, det_err, fro2;
             const int v = positive_eigs_f(txx - lthf, tyy - lthf, tzz - lthf, a10, a20, a21, det, det_err, fro2);
             const int lab = 4 - v;
-            const size_t cell = ((size_t)c * NN + (ty0 + a - 2)) * NN + (tz0 + b - 2);
+            const size_t cell = (size_t)c * NN * NN + (size_t)coff[k];
             labels[cell] = (int8_t)lab;
             loc[lab - 1]++;
             // margin bound min|mu| >= |det B| / ||B||_F^2, taken conservatively
@@ -220,6 +246,20 @@ __global__ void __launch_bounds__(512) k_tweb_fused(const float *__restrict__ ph
             if (lam || maybe_marginal)
                 loc[4] += eig_margin(txx, tyy, tzz, a10, a20, a21, lth, tol, maybe_marginal, lam, cell, n3);
         }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    fetch(x0 - 4);
+    const int send = x1 + 4;
+    for (int s = x0 - 4; s < send; s += 5) {
+        step(I0{}, s);
+        if (s + 1 < send) step(I1{}, s + 1);
+        if (s + 2 < send) step(I2{}, s + 2);
+        if (s + 3 < send) step(I3{}, s + 3);
+        if (s + 4 < send) step(I4{}, s + 4);
     }
 #pragma unroll
     for (int i = 0; i < 5; ++i)
@@ -240,40 +280,57 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
 }
 
 
-static int pick_tile(int N, const int *cands, int nc) {
-    for (int i = 0; i < nc; ++i)
-        if (cands[i] <= N && N % cands[i] == 0) return cands[i];
-    return 0;
+template <int TY, int TZ, int NT>
+static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol,
+                                int8_t *labels, float *lam, unsigned long long *stats) {
+    const size_t lds = sizeof(float) * (5 * (size_t)(TY + 8) * (TZ + 8) + 2 * (size_t)(TY + 4) * (TZ + 4));
+    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
+    // x segments: a segment re-reads 8 halo planes; pick the count that
+    // minimises (rounds of resident workgroups) x (planes per segment)
+    const int tiles = (N / TY) * (N / TZ);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tweb_fused<TY, TZ, NT>, NT, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const int slots = std::max(1, ctx->cu_count) * per_cu;
+    int nseg = 1;
+    double best = 1e300;
+    for (int g = 1; g <= std::max(1, N / 16); ++g) {
+        const int seg = (N + g - 1) / g;
+        const double rounds = std::ceil((double)tiles * g / slots);
+        const double cost = rounds * (seg + 8);
+        if (cost < best * 0.999) {
+            best = cost;
+            nseg = g;
+        }
+    }
+    if (ctx->tune.tseg > 0 && ctx->tune.tseg <= N / 4) nseg = ctx->tune.tseg;
+    const int segLen = (N + nseg - 1) / nseg;
+    nseg = (N + segLen - 1) / segLen;
+    dim3 grid((unsigned)tiles, (unsigned)nseg);
+    const float ihf = (float)(1.0 / (12.0 * (L / (double)N)));
+    cm_status s;
+    if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
+    return launch(ctx, "tweb_fused", k_tweb_fused<TY, TZ, NT>, grid, dim3(NT), lds, phi, N, segLen, ihf, lth, tol,
+                  labels, lam, stats);
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.
+// Tile (TY x TZ cells, threads): 16x64/512 when 64 | N, else 8x32/256,
+// 8x16/128 or 8x8/64 by the largest of 32, 16, 8 dividing N.
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
                             float *lam, unsigned long long *stats) {
-    static const int cz[] = {64, 32, 16, 8}, cy[] = {16, 8};
-    int TZ = pick_tile(N, cz, 4), TY = pick_tile(N, cy, 2);
-    if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 && ctx->tune.tz <= 64) TZ = ctx->tune.tz;
-    if (ctx->tune.ty > 0 && N % ctx->tune.ty == 0 && ctx->tune.ty <= 32) TY = ctx->tune.ty;
-    if (!TZ || !TY) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
-    const int npos = (TY + 4) * (TZ + 4);
-    const int NT = 512;
-    const int P = (npos + NT - 1) / NT;
-    int nseg = N >= 256 ? 4 : (N >= 64 ? 2 : 1);
-    if (ctx->tune.tseg > 0 && ctx->tune.tseg <= N / 4) nseg = ctx->tune.tseg;
-    const int segLen = (N + nseg - 1) / nseg;
-    dim3 grid((unsigned)((N / TY) * (N / TZ)), (unsigned)nseg);
-    const size_t lds = sizeof(float) * (5 * (size_t)(TY + 8) * (TZ + 8) + 2 * (size_t)npos);
-    if ((size_t)(TY + 8) * (TZ + 8) > 4 * 512 || P > 3 || lds > ctx->lds_per_block)
-        return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d too large", TY, TZ);
-    const double ih = 1.0 / (12.0 * (L / (double)N));
-    cm_status s;
-    if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
-    const float ihf = (float)ih;
-    switch (P) {
-    case 1: return launch(ctx, "tweb_fused", k_tweb_fused<1>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ihf, lth, tol, labels, lam, stats);
-    case 2: return launch(ctx, "tweb_fused", k_tweb_fused<2>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ihf, lth, tol, labels, lam, stats);
-    case 3: return launch(ctx, "tweb_fused", k_tweb_fused<3>, grid, dim3(NT), lds, phi, N, TY, TZ, segLen, ihf, lth, tol, labels, lam, stats);
+    int TZ = N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : (N % 16 == 0 ? 16 : (N % 8 == 0 ? 8 : 0)));
+    if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 && ctx->tune.tz <= TZ &&
+        (ctx->tune.tz == 64 || ctx->tune.tz == 32 || ctx->tune.tz == 16 || ctx->tune.tz == 8))
+        TZ = ctx->tune.tz;
+    switch (TZ) {
+    case 64: return tweb_fused_run<16, 64, 512>(ctx, N, L, phi, lth, tol, labels, lam, stats);
+    case 32: return tweb_fused_run<8, 32, 256>(ctx, N, L, phi, lth, tol, labels, lam, stats);
+    case 16: return tweb_fused_run<8, 16, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats);
+    case 8: return tweb_fused_run<8, 8, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats);
     }
-    return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile needs %d positions per thread", P);
+    return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
 }
 
 // phi = c2r(W(k)(-1/k^2) r2c(delta)*scale), float32.
