@@ -63,6 +63,60 @@ KERNELS = ["mas_tile_count", "mas_tile_prefix", "mas_tile_scan", "mas_tile_items
            "fft_y_inv", "fft_z_c2r", "tweb_grad", "tweb_zero", "tweb_hess_eig", "tweb_fused"]
 
 
+# rocprofv3 kernel names (demangled prefix) -> the library's kernel names, for
+# the fused path (the only path run with --fused-only)
+ROCPROF_NAMES = [("cm::k_tile_count_lds", "mas_tile_count"), ("cm::k_tile_count", "mas_tile_count"),
+                 ("cm::k_tile_prefix", "mas_tile_prefix"), ("cm::k_tile_scan", "mas_tile_scan"),
+                 ("cm::k_tile_items", "mas_tile_items"), ("cm::k_tile_scatter", "mas_tile_scatter"),
+                 ("cm::k_tile_zero_split", "mas_tile_zero_split"), ("cm::k_tile_deposit", "mas_tile_deposit"),
+                 ("cm::k_rows_r2c", "fft_z_r2c"), ("cm::k_rows_c2r", "fft_z_c2r"), ("cm::k_x_fused", "fused_x"),
+                 ("cm::k_tweb_fused", "tweb_fused")]
+
+
+def lib_name(kname):
+    k = kname.replace("void ", "", 1)
+    if k.startswith("cm::k_cols<"):
+        return "fft_y_fwd" if ", true>" in k.split("(")[0] else "fft_y_inv"
+    for pre, name in ROCPROF_NAMES:
+        if k.startswith(pre):
+            return name
+    return None
+
+
+def traffic_from_pmc(fetch_csv, write_csv, out_json):
+    """HBM bytes per dispatch from two rocprofv3 --pmc passes (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch) of `bench.py --fused-only --copy-calibration`.
+    Each counter is scaled by what it reads for the 512 MiB device copy made by
+    --copy-calibration (the largest __amd_rocclr_copyBuffer dispatch)."""
+    import csv
+    copy_kib = 4 * N ** 3 / 1024.0
+
+    def load(path, counter):
+        vals, copy = {}, []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+            if "copyBuffer" in r["Kernel_Name"]:
+                copy.append(v)
+                continue
+            name = lib_name(r["Kernel_Name"])
+            if name:
+                vals.setdefault(name, []).append(v)
+        return {k: float(np.median(v)) for k, v in vals.items()}, max(copy)
+
+    fetch, cf = load(fetch_csv, "FETCH_SIZE")
+    write, cw = load(write_csv, "WRITE_SIZE")
+    ff, fw = copy_kib / cf, copy_kib / cw
+    out = {"_factors": {"fetch": round(ff, 4), "write": round(fw, 4), "copy_fetch_kib": cf, "copy_write_kib": cw,
+                        "copy_kib": copy_kib}}
+    for k in sorted(set(fetch) & set(write)):
+        out[k] = int(round((fetch[k] * ff + write[k] * fw) * 1024))
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    return out
+
+
 def cpu_baseline(pos, quick=False):
     """OpenMP restatement of the reference path on the host cores, on a bounded
     sample: the whole pipeline at NS^3 with NS^3/8 particles (1/8 of the mesh and
@@ -119,7 +173,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick-cpu", action="store_true", help="smaller CPU sample (128^3)")
     ap.add_argument("--plain-steps", type=int, default=None)
+    ap.add_argument("--fused-only", action="store_true",
+                    help="time only the optimised path (no plain path, FP64 check or CPU leg): for profiler runs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy a 512 MiB device buffer once before the steps (reference for PMC byte counters)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="convert two rocprofv3 --pmc counter files into profiles/traffic.json and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        print(json.dumps(traffic_from_pmc(args.traffic_from[0], args.traffic_from[1],
+                                          os.path.join(ROOT, "profiles", "traffic.json"))))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -137,6 +201,12 @@ def main():
     b = cm.binning(N, L, ndel=NDEL, DeltaKmin=0.0)
     labels = torch.empty((N, N, N), dtype=torch.int8, device=dev)
     ctx = cm.Context(0)
+    if args.copy_calibration:
+        xa = torch.ones(N ** 3, dtype=torch.float32, device=dev)
+        xb = torch.empty_like(xa)
+        xb.copy_(xa)
+        torch.cuda.synchronize()
+        del xa, xb
 
     # optimised path
     out, total, lat = run_steps(ctx, posd, b, True, args.steps, args.warmup, labels)
@@ -170,6 +240,17 @@ def main():
         roof = {"kernel": dom[0], "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
                 "mean_ms": round(dom[1]["mean_ms"], 4)}
+
+    if args.fused_only:
+        print(json.dumps({"metric": METRIC, "value": round(value, 3), "unit": "mocks/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(1e3 * total / args.steps, 3), "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "fused path only (profiler run)", "N": N},
+                          "kernels_ms_per_step": {k: round(v["mean_ms"] * v["launches_per_step"], 4)
+                                                  for k, v in ktimes.items()}, "roofline": roof}))
+        ctx.close()
+        return
 
     # plain GPU path on the same inputs
     psteps = args.plain_steps or max(3, args.steps // 2)
