@@ -101,13 +101,15 @@ bool fft_size_ok(int N);
 cm_status fft_r2c_impl(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double subtract);
 cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale);
-// Strided column passes over the half spectrum, axis 1 (y) or 0 (x).
-cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward);
+// Strided column passes over the half spectrum, axis 1 (y) or 0 (x).  pitch:
+// kz row pitch of the spectrum in complex values (0 = N/2+1, the FFTW layout;
+// float32 only may use a padded pitch).
+cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch = 0);
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
-                       double subtract, double *part);
+                       double subtract, double *part, int pitch = 0);
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
-                       double scale);
+                       double scale, int pitch = 0);
 
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w,
                            size_t n, void *mesh, cm_dtype mdt, int accumulate);

@@ -7,6 +7,7 @@
 #include <cmath>
 
 #include "cm_internal.hpp"
+#include "fft_ct.hpp"
 #include "fft_engine.hpp"
 
 namespace cm {
@@ -162,6 +163,208 @@ __global__ void __launch_bounds__(512) k_cols(v2<T> *__restrict__ data, DevPlan 
     fft_run<T, E, true, FWD>(lds, 0, pl, nb, tw, src, dst);
 }
 
+// ---------------------------------------------------- float32, compile-time N
+// The same three passes with the plan, block shape and strides fixed at
+// compile time (fft_ct.hpp).  `pitch` is the kz row pitch of the half
+// spectrum in complex values (N/2+1 for the FFTW layout, padded_pitch(N) for
+// the internal one).
+template <int N>
+__global__ void __launch_bounds__(RowShape<N>::NT) k_rows_r2c_ct(const float *__restrict__ in,
+                                                                 float2 *__restrict__ out, int pitch, float sub,
+                                                                 const float2 *__restrict__ twM,
+                                                                 const float2 *__restrict__ twN,
+                                                                 double *__restrict__ part) {
+    using RS = RowShape<N>;
+    constexpr int M = RS::M, NB = RS::NB, STRIDE = RS::STRIDE, NT = RS::NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[(NT + 63) / 64];
+    float2 *lds = reinterpret_cast<float2 *>(smem_raw);
+    const size_t row0 = (size_t)blockIdx.x * NB;
+    const float2 *inrow = reinterpret_cast<const float2 *>(in + row0 * (size_t)N);
+    float acc = 0.f;  // at most E values per thread; the workgroup total is FP64
+    auto src = [&](int b, int m) {
+        float2 z = inrow[b * M + m];
+        acc += z.x + z.y;
+        z.x -= sub;
+        z.y -= sub;
+        return z;
+    };
+    auto dst = [&](int b, int k, float2 v) { lds[b * STRIDE + rpad(k)] = v; };
+    ct_fft<float, M, NB, STRIDE, false, true>(lds, twM, src, dst);
+    if (part) {
+        double a = (double)acc;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (part && threadIdx.x == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < (NT + 63) / 64; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
+    float2 *orow = out + row0 * (size_t)pitch;
+    for (int t = threadIdx.x; t < NB * (M + 1); t += NT) {
+        const int b = t / (M + 1), k = t - b * (M + 1);
+        const float2 zk = lds[b * STRIDE + rpad(k == M ? 0 : k)];
+        const float2 zm = cconj(lds[b * STRIDE + rpad(k == 0 ? 0 : M - k)]);
+        const float2 e = float2{0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y)};
+        const float2 o = float2{0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y)};
+        // X_k = E_k + W^k O_k with O_k = (Z_k - conj Z_{M-k}) / (2i)
+        const float2 oi = float2{o.y, -o.x};
+        orow[(size_t)b * pitch + k] = cadd(e, cmul(twN[k], oi));
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *__restrict__ in,
+                                                                 float *__restrict__ out, int pitch, float scale,
+                                                                 const float2 *__restrict__ twM,
+                                                                 const float2 *__restrict__ twN) {
+    using RS = RowShape<N>;
+    constexpr int M = RS::M, NB = RS::NB, STRIDE = RS::STRIDE, NT = RS::NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(smem_raw);
+    const size_t row0 = (size_t)blockIdx.x * NB;
+    const float2 *irow = in + row0 * (size_t)pitch;
+    for (int t = threadIdx.x; t < NB * (M + 1); t += NT) {
+        const int b = t / (M + 1), k = t - b * (M + 1);
+        lds[b * STRIDE + rpad(k)] = irow[(size_t)b * pitch + k];
+    }
+    __syncthreads();
+    auto src = [&](int b, int m) {
+        const float2 xm = lds[b * STRIDE + rpad(m)];
+        const float2 xc = cconj(lds[b * STRIDE + rpad(M - m)]);
+        const float2 e = float2{0.5f * (xm.x + xc.x), 0.5f * (xm.y + xc.y)};
+        const float2 o = cmul(float2{0.5f * (xm.x - xc.x), 0.5f * (xm.y - xc.y)}, cconj(twN[m]));
+        return float2{e.x - o.y, e.y + o.x};  // Z = E + i O
+    };
+    float2 *orow = reinterpret_cast<float2 *>(out + row0 * (size_t)N);
+    const float s2 = 2.0f * scale;
+    auto dst = [&](int b, int m, float2 z) { orow[b * M + m] = float2{s2 * z.x, s2 * z.y}; };
+    ct_fft<float, M, NB, STRIDE, false, false>(lds, twM, src, dst);
+}
+
+template <int N, int NB, bool FWD>
+__global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__restrict__ data, int ncols, size_t stride,
+                                                                 size_t planeStride, const float2 *__restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(smem_raw);
+    const size_t base = (size_t)blockIdx.y * planeStride + (size_t)blockIdx.x * NB;
+    const int valid = min(NB, ncols - (int)blockIdx.x * NB);
+    auto src = [&](int b, int e) { return b < valid ? data[base + (size_t)e * stride + b] : float2{0.f, 0.f}; };
+    auto dst = [&](int b, int e, float2 v) {
+        if (b < valid) data[base + (size_t)e * stride + b] = v;
+    };
+    ct_fft<float, N, NB, 0, true, FWD>(lds, tw, src, dst);
+}
+
+// column block width of the float32 y/x passes
+template <int N> constexpr int col_nb() { return N > 768 ? 8 : 16; }
+
+template <int N>
+static cm_status rows_r2c_ct(cm_ctx *ctx, const float *in, float2 *out, int pitch, float sub, const float2 *twM,
+                             const float2 *twN, double *part) {
+    using RS = RowShape<N>;
+    if (RS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: LDS");
+    return launch(ctx, "fft_z_r2c", k_rows_r2c_ct<N>, dim3((unsigned)((size_t)N * N / RS::NB)), dim3(RS::NT), RS::LDS,
+                  in, out, pitch, sub, twM, twN, part);
+}
+
+template <int N>
+static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitch, float scale, const float2 *twM,
+                             const float2 *twN) {
+    using RS = RowShape<N>;
+    if (RS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: LDS");
+    return launch(ctx, "fft_z_c2r", k_rows_c2r_ct<N>, dim3((unsigned)((size_t)N * N / RS::NB)), dim3(RS::NT), RS::LDS,
+                  in, out, pitch, scale, twM, twN);
+}
+
+template <int N, bool FWD>
+static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw) {
+    constexpr int NB = col_nb<N>();
+    using CS = ColShape<N, NB>;
+    if (CS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: LDS");
+    const int ncols = N / 2 + 1;
+    size_t stride, planeStride;
+    if (axis == 1) {  // y: fixed x plane, stride over kz rows
+        stride = (size_t)pitch;
+        planeStride = (size_t)N * pitch;
+    } else {  // x: fixed y, stride over planes
+        stride = (size_t)N * pitch;
+        planeStride = (size_t)pitch;
+    }
+    dim3 grid((unsigned)((ncols + NB - 1) / NB), (unsigned)N);
+    const char *name = axis == 1 ? (FWD ? "fft_y_fwd" : "fft_y_inv") : (FWD ? "fft_x_fwd" : "fft_x_inv");
+    return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw);
+}
+
+static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **twN) {
+    cm_status st;
+    *twM = (const float2 *)twiddles(ctx, N / 2, CM_F32, &st);
+    if (!*twM) return st;
+    *twN = (const float2 *)twiddles(ctx, N, CM_F32, &st);
+    if (!*twN) return st;
+    return CM_OK;
+}
+
+static cm_status rows_r2c_f32(cm_ctx *ctx, int N, const void *in, void *out, int pitch, double subtract,
+                              double *part) {
+    const float2 *twM, *twN;
+    cm_status s = tw_pair(ctx, N, &twM, &twN);
+    if (s != CM_OK) return s;
+    auto *ip = (const float *)in;
+    auto *op = (float2 *)out;
+    switch (N) {
+#define CM_CASE(n) \
+    case n: return rows_r2c_ct<n>(ctx, ip, op, pitch, (float)subtract, twM, twN, part);
+        CM_FFT_SIZES(CM_CASE)
+#undef CM_CASE
+    }
+    return fail(ctx, CM_ERR_UNSUPPORTED, "fft: N=%d", N);
+}
+
+static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int pitch, double scale) {
+    const float2 *twM, *twN;
+    cm_status s = tw_pair(ctx, N, &twM, &twN);
+    if (s != CM_OK) return s;
+    auto *ip = (const float2 *)in;
+    auto *op = (float *)out;
+    switch (N) {
+#define CM_CASE(n) \
+    case n: return rows_c2r_ct<n>(ctx, ip, op, pitch, (float)scale, twM, twN);
+        CM_FFT_SIZES(CM_CASE)
+#undef CM_CASE
+    }
+    return fail(ctx, CM_ERR_UNSUPPORTED, "fft: N=%d", N);
+}
+
+static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch) {
+    cm_status st;
+    const float2 *tw = (const float2 *)twiddles(ctx, N, CM_F32, &st);
+    if (!tw) return st;
+    auto *dp = (float2 *)data;
+    switch (N) {
+#define CM_CASE(n)                                                                     \
+    case n:                                                                            \
+        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw) : cols_ct<n, false>(ctx, dp, axis, pitch, tw);
+        CM_FFT_SIZES(CM_CASE)
+#undef CM_CASE
+    }
+    return fail(ctx, CM_ERR_UNSUPPORTED, "fft: N=%d", N);
+}
+
+static int rows_blocks_f32(int N) {
+    switch (N) {
+#define CM_CASE(n) \
+    case n: return (n * n) / RowShape<n>::NB;
+        CM_FFT_SIZES(CM_CASE)
+#undef CM_CASE
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------- host side
 struct RowCfg {
     FftPlan plan;
@@ -290,23 +493,26 @@ static cm_status cols_t(cm_ctx *ctx, int N, void *data, int axis) {
 }
 
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double subtract,
-                       double *part) {
-    return dt == CM_F32 ? rows_r2c_t<float>(ctx, N, in, out, subtract, part)
-                        : rows_r2c_t<double>(ctx, N, in, out, subtract, part);
+                       double *part, int pitch) {
+    if (dt == CM_F32) return rows_r2c_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, subtract, part);
+    if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
+    return rows_r2c_t<double>(ctx, N, in, out, subtract, part);
 }
 
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt) {
-    return (N * N) / row_cfg(N, dt, dt == CM_F32 ? ctx->tune.zrows : 0).nb;
+    if (dt == CM_F32) return rows_blocks_f32(N);
+    return (N * N) / row_cfg(N, dt, 0).nb;
 }
 
-cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale) {
-    return dt == CM_F32 ? rows_c2r_t<float>(ctx, N, in, out, scale)
-                        : rows_c2r_t<double>(ctx, N, in, out, scale);
+cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale, int pitch) {
+    if (dt == CM_F32) return rows_c2r_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, scale);
+    if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
+    return rows_c2r_t<double>(ctx, N, in, out, scale);
 }
 
-cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward) {
-    if (dt == CM_F32)
-        return forward ? cols_t<float, true>(ctx, N, data, axis) : cols_t<float, false>(ctx, N, data, axis);
+cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch) {
+    if (dt == CM_F32) return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1);
+    if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return forward ? cols_t<double, true>(ctx, N, data, axis) : cols_t<double, false>(ctx, N, data, axis);
 }
 

@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "cm_internal.hpp"
-#include "fft_engine.hpp"
+#include "fft_ct.hpp"
 #include "pk_common.hpp"
 #include "tweb_common.hpp"
 
@@ -25,28 +25,27 @@ namespace cm {
 
 // The two FFT phases of a tile live in separate non-inlined functions so that
 // each gets its own register allocation (inlined together with the shell code
-// they exceed the 256-VGPR budget and spill).
-template <int E>
+// they exceed the VGPR budget and spill).
+template <int N, int NB>
 __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restrict__ data, size_t base,
                                                             size_t xstride, int valid, float2 *lds,
-                                                            const DevPlan &pl, int nb,
                                                             const float2 *__restrict__ tw) {
     auto srcG = [&](int b, int e) {
         return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
     };
-    auto dstL = [&](int b, int e, float2 v) { lds[e * nb + b] = v; };
-    fft_run<float, E, true, true>(lds, 0, pl, nb, tw, srcG, dstL);
+    auto dstL = [&](int b, int e, float2 v) { lds[e * NB + b] = v; };
+    ct_fft<float, N, NB, 0, true, true>(lds, tw, srcG, dstL);
 }
 
-template <int E>
+template <int N, int NB>
 __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict__ data, size_t base, size_t xstride,
-                                                              int valid, float2 *lds, const DevPlan &pl, int nb,
+                                                              int valid, float2 *lds,
                                                               const float2 *__restrict__ tw) {
-    auto srcL = [&](int b, int e) { return lds[e * nb + b]; };
+    auto srcL = [&](int b, int e) { return lds[e * NB + b]; };
     auto dstG = [&](int b, int e, float2 v) {
         if (b < valid) data[base + (size_t)e * xstride + b] = v;
     };
-    fft_run<float, E, true, false>(lds, 0, pl, nb, tw, srcL, dstG);
+    ct_fft<float, N, NB, 0, true, false>(lds, tw, srcL, dstG);
 }
 
 // Row-local lane shifts (DPP row_shr / row_shl inside rows of 16 lanes): a lane
@@ -79,14 +78,14 @@ __device__ __forceinline__ void seg_step(int gl, int &sh, float &a0, float &a1, 
     }
 }
 
-template <int E, int NB>
-__global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevPlan pl, int ncols, int N,
+template <int N, int NB>
+__global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__restrict__ data, int pitch,
                                                  const float2 *__restrict__ tw, PkK pk,
                                                  const double *__restrict__ tabs, double *__restrict__ gsum,
                                                  unsigned long long *__restrict__ gcnt, int do_tweb, float inv_kf2,
                                                  const float *__restrict__ gauss, const double *__restrict__ nobj) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int M1 = N / 2 + 1;
+    constexpr int M1 = N / 2 + 1;
     const int ns = pk.nshells;
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     double *s_acc = reinterpret_cast<double *>(lds + (size_t)NB * N);  // 4*ns
@@ -94,7 +93,9 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
     float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                // M1
     float *s_sq = s_ic + M1;                                            // M1
     float *s_g = s_sq + M1;                                             // N
-    const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63;
+    constexpr int NT = ColShape<N, NB>::NT;
+    const int ncols = N / 2 + 1;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int i = tid; i < M1; i += NT) {
         s_ic[i] = (float)tabs[i];
         s_sq[i] = (float)tabs[M1 + i];
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
     const double n3 = (double)N * N * N;
     // -scale/k^2 = -(N^3/N_gal) / (kf^2 q)
     const float fscale = (do_tweb && nobj[0] > 0.0) ? (float)(-(n3 / nobj[0]) * (double)inv_kf2) : 0.0f;
-    const size_t xstride = (size_t)N * ncols;
+    const size_t xstride = (size_t)N * pitch;
     const int nkzb = (ncols + NB - 1) / NB;
     const int ntiles = nkzb * N;
     const int h = N / 2;
@@ -113,9 +114,9 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int kzb = tile % nkzb, iy = tile / nkzb;
         const int kz0 = kzb * NB;
-        const size_t base = (size_t)iy * ncols + kz0;
+        const size_t base = (size_t)iy * pitch + kz0;
         const int valid = min(NB, ncols - kz0);
-        x_fft_fwd_to_lds<E>(data, base, xstride, valid, lds, pl, NB, tw);
+        x_fft_fwd_to_lds<N, NB>(data, base, xstride, valid, lds, tw);
         __syncthreads();
         // shells (+ multiply), element t = e*nb + b (kz fastest): runs of equal
         // (kx, shell) lie inside one group of nb lanes
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(512) k_x_fused(float2 *__restrict__ data, DevP
             }
         }
         __syncthreads();
-        if (do_tweb) x_fft_inv_from_lds<E>(data, base, xstride, valid, lds, pl, NB, tw);
+        if (do_tweb) x_fft_inv_from_lds<N, NB>(data, base, xstride, valid, lds, tw);
         __syncthreads();
     }
     for (int i = tid; i < ns; i += NT) {
@@ -209,10 +210,34 @@ PkK make_pkk(const cm_pk_opts *o);
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
                             float *lam, unsigned long long *stats);
 
-struct FusedColCfg {
-    FftPlan plan;
-    int E, nb, NT;
-};
+template <int N, int NB>
+static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, int pitch, const float2 *tw, const PkK &pk, const double *dtab,
+                             double *dsum, unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
+                             const double *dnobj, int ns) {
+    constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB>::NT;
+    const size_t lds = sizeof(float2) * (size_t)NB * N + sizeof(double) * 4 * (size_t)ns +
+                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (2 * (size_t)M1 + (size_t)N);
+    if (lds > ctx->lds_per_block)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_fused<N, NB>, NT, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int ntiles = ((M1 + NB - 1) / NB) * N;
+    const unsigned grid = (unsigned)std::min(ntiles, std::max(1, ctx->cu_count) * per_cu);
+    return launch(ctx, "fused_x", k_x_fused<N, NB>, dim3(grid), dim3(NT), lds, dk, pitch, tw, pk, dtab, dsum, dcnt,
+                  do_tweb, inv_kf2, dgauss, dnobj);
+}
+
+template <int N>
+static cm_status x_fused_launch(cm_ctx *ctx, int nb, float2 *dk, int pitch, const float2 *tw, const PkK &pk,
+                                const double *dtab, double *dsum, unsigned long long *dcnt, int do_tweb,
+                                float inv_kf2, const float *dgauss, const double *dnobj, int ns) {
+    if constexpr (ColShape<N, 8>::NT * 2 <= 512) {
+        if (nb == 16)
+            return x_fused_run<N, 16>(ctx, dk, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj, ns);
+    }
+    return x_fused_run<N, 8>(ctx, dk, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj, ns);
+}
 
 // Fused measurement: mesh (float32, device) -> shells (+ labels).
 // Device outputs: sums[4*ns], cnts[ns], nobj[1], stats[5].
@@ -220,9 +245,9 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
                              double **sums, unsigned long long **cnts, double **nobj, unsigned long long **stats) {
     const int N = o->mas.N;
     const size_t n3 = (size_t)N * N * N;
-    const size_t nh = (size_t)N * N * (N / 2 + 1);
+    const int pitch = padded_pitch(N);  // internal spectrum: kz rows padded to 128-byte multiples
+    const size_t nh = (size_t)N * N * pitch;
     const int ns = o->nshells;
-    const int M1 = N / 2 + 1;
     cm_status s;
     void *dk, *part, *dsum, *dcnt, *dnobj, *dtab, *dstats;
     const int nparts = fft_rows_blocks(ctx, N, CM_F32);
@@ -238,12 +263,12 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     CM_HIP(ctx, hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
 
     // z pass with the mesh sum, then the FP64 total
-    if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part)) != CM_OK) return s;
+    if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part, pitch)) != CM_OK) return s;
     if ((s = launch(ctx, "fused_sum", k_fused_sum, dim3(1), dim3(256), 0, (const double *)part, nparts,
                     (double *)dnobj)) != CM_OK)
         return s;
     // y pass
-    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true)) != CM_OK) return s;
+    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch)) != CM_OK) return s;
     // x pass + shells + multiply + inverse x
     if ((s = launch(ctx, "fused_zero", k_fused_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
                     ns)) != CM_OK)
@@ -260,33 +285,9 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     po.normal_power = 1.0;
     po.float_shell_rule = o->float_shell_rule;
     PkK pk = make_pkk(&po);
-    FftPlan plan = make_plan(N);
-    DevPlan dp;
-    dp.n = plan.n;
-    dp.nst = plan.nst;
-    for (int i = 0; i < 8; ++i) dp.r[i] = plan.r[i];
-    bool has3 = false, has8 = false;
-    for (int i = 0; i < plan.nst; ++i) {
-        has3 |= plan.r[i] == 3;
-        has8 |= plan.r[i] == 8;
-    }
-    const int E = has3 ? 24 : ((has8 && N >= 64) ? 16 : 8);
-    int nb = (ctx->tune.xnb == 16) ? 16 : 8;  // lane groups of the DPP scan: 8 or 16
-    if (nb * N / E > 512) nb = 8;
-    if (N % E || nb * N / E > 512)  // whole lane groups per workgroup
-        return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: N=%d has no 8/16-column block", N);
-    const int NT = nb * N / E;
-    const size_t lds = sizeof(float2) * (size_t)nb * N + sizeof(double) * 4 * (size_t)ns +
-                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (2 * (size_t)M1 + (size_t)N);
-    if (lds > ctx->lds_per_block)
-        return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
     cm_status st;
     const void *tw = twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;
-    const int ncols = N / 2 + 1;
-    const int ntiles = ((ncols + nb - 1) / nb) * N;
-    const int per_cu = lds > 80 * 1024 ? 1 : (lds > 53 * 1024 ? 2 : 3);
-    const unsigned grid = (unsigned)std::min(ntiles, ctx->cu_count * per_cu);
     const double kf = 2.0 * M_PI / o->mas.L;
     const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
     const int do_tweb = o->do_tweb ? 1 : 0;
@@ -301,29 +302,24 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     if ((s = scratch(ctx, "f.gauss", sizeof(float) * (size_t)N, &dgauss)) != CM_OK) return s;
     CM_HIP(ctx, hipMemcpyAsync(dgauss, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice, ctx->stream));
     const float inv_kf2 = (float)(1.0 / (kf * kf));
-    auto *dkp = (float2 *)dk;
-    auto *twp = (const float2 *)tw;
-    switch (E) {
-    case 8:
-        s = nb == 8 ? launch(ctx, "fused_x", k_x_fused<8, 8>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj)
-                    : launch(ctx, "fused_x", k_x_fused<8, 16>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj);
+    const int nb = ctx->tune.xnb == 16 ? 16 : 8;  // lane groups of the DPP shell scan
+    switch (N) {
+#define CM_CASE(n)                                                                                                     \
+    case n:                                                                                                            \
+        s = x_fused_launch<n>(ctx, nb, (float2 *)dk, pitch, (const float2 *)tw, pk, (const double *)dtab,             \
+                              (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss,     \
+                              (const double *)dnobj, ns);                                                              \
         break;
-    case 16:
-        s = nb == 8 ? launch(ctx, "fused_x", k_x_fused<16, 8>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj)
-                    : launch(ctx, "fused_x", k_x_fused<16, 16>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj);
-        break;
-    case 24:
-        s = nb == 8 ? launch(ctx, "fused_x", k_x_fused<24, 8>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj)
-                    : launch(ctx, "fused_x", k_x_fused<24, 16>, dim3(grid), dim3(NT), lds, dkp, dp, ncols, N, twp, pk, (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss, (const double *)dnobj);
-        break;
-    default: return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: E=%d", E);
+        CM_FFT_SIZES(CM_CASE)
+#undef CM_CASE
+    default: return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass: N=%d", N);
     }
     if (s != CM_OK) return s;
     if (do_tweb) {
         void *phi;
         if ((s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
-        if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false)) != CM_OK) return s;
-        if ((s = fft_rows_c2r(ctx, N, CM_F32, dk, phi, 1.0 / (double)n3)) != CM_OK) return s;
+        if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false, pitch)) != CM_OK) return s;
+        if ((s = fft_rows_c2r(ctx, N, CM_F32, dk, phi, 1.0 / (double)n3, pitch)) != CM_OK) return s;
         if ((s = tweb_fused_launch(ctx, N, o->mas.L, (const float *)phi, o->lambda_th, o->margin_tol, labels, nullptr,
                                    (unsigned long long *)dstats)) != CM_OK)
             return s;

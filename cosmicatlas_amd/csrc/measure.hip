@@ -100,7 +100,6 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     const int N = o->mas.N;
     const size_t n3 = (size_t)N * N * N;
     const size_t nh = (size_t)N * N * (N / 2 + 1);
-    const double V = o->mas.L * o->mas.L * o->mas.L;
     cm_status s;
     void *mesh, *dk, *nobj;
     if ((s = scratch(ctx, "m.mesh", sizeof(float) * n3, &mesh)) != CM_OK) return s;
