@@ -111,8 +111,13 @@ int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double scale, int pitch = 0);
 
+// exact = false (tiled path only, needs overflow_out): the record buffer is
+// sized from an estimate instead of a read-back of the count (no host
+// synchronisation); *overflow_out then points to a device flag that is
+// non-zero when the estimate was too small and the mesh must be redone.
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w,
-                           size_t n, void *mesh, cm_dtype mdt, int accumulate);
+                           size_t n, void *mesh, cm_dtype mdt, int accumulate, bool exact = true,
+                           unsigned **overflow_out = nullptr);
 cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
                         double *host_out);
 

@@ -228,9 +228,24 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     const size_t row0 = (size_t)blockIdx.x * NB;
     const float2 *irow = in + row0 * (size_t)pitch;
-    for (int t = threadIdx.x; t < NB * (M + 1); t += NT) {
-        const int b = t / (M + 1), k = t - b * (M + 1);
-        lds[b * STRIDE + rpad(k)] = irow[(size_t)b * pitch + k];
+    // stage the rows in LDS: all loads first (one latency), then the stores
+    constexpr int NL = (NB * (M + 1) + NT - 1) / NT;
+    float2 ld[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int t = threadIdx.x + i * NT;
+        if (t < NB * (M + 1)) {
+            const int b = t / (M + 1), k = t - b * (M + 1);
+            ld[i] = irow[(size_t)b * pitch + k];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int t = threadIdx.x + i * NT;
+        if (t < NB * (M + 1)) {
+            const int b = t / (M + 1), k = t - b * (M + 1);
+            lds[b * STRIDE + rpad(k)] = ld[i];
+        }
     }
     __syncthreads();
     auto src = [&](int b, int m) {

@@ -302,7 +302,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     if ((s = scratch(ctx, "f.gauss", sizeof(float) * (size_t)N, &dgauss)) != CM_OK) return s;
     CM_HIP(ctx, hipMemcpyAsync(dgauss, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice, ctx->stream));
     const float inv_kf2 = (float)(1.0 / (kf * kf));
-    const int nb = ctx->tune.xnb == 16 ? 16 : 8;  // lane groups of the DPP shell scan
+    const int nb = ctx->tune.xnb == 8 ? 8 : 16;  // lane groups of the DPP shell scan (16 when it fits)
     switch (N) {
 #define CM_CASE(n)                                                                                                     \
     case n:                                                                                                            \

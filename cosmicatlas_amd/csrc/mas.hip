@@ -83,29 +83,37 @@ struct TileK {
     int T;      // tile edge in cells
     int R;      // stencil radius
     int ntx;    // tiles per side
-    int CH;     // max particles per workgroup
+    int CH;     // max records per deposit workgroup
 };
+
+constexpr int kTileCH = 4096;
 
 // Tiles touched by a particle in cell c along one axis: home tile and, when the
 // stencil crosses the tile edge, the neighbour.  Returns the count (1 or 2).
-__device__ __forceinline__ int axis_tiles(int c, const TileK &t, int out[2]) {
-    int home = c / t.T;
-    int lc = c - home * t.T;
+template <int T>
+__device__ __forceinline__ int axis_tiles(int c, int R, int ntx, int out[2]) {
+    const int home = c / T, lc = c % T;
     out[0] = home;
-    if (lc < t.R) { out[1] = (home - 1 + t.ntx) % t.ntx; return 2; }
-    if (lc >= t.T - t.R) { out[1] = (home + 1) % t.ntx; return 2; }
+    if (lc < R) {
+        out[1] = home == 0 ? ntx - 1 : home - 1;
+        return 2;
+    }
+    if (lc >= T - R) {
+        out[1] = home + 1 == ntx ? 0 : home + 1;
+        return 2;
+    }
     return 1;
 }
 
 // Cell and in-cell offset f = x/d - c - 1/2 (FP64, as the plain kernel) of a
-// particle; s[] receives the offsets rounded to float.
-__device__ __forceinline__ bool particle_cells(const float *pos, size_t i, const MasK &p, int c[3], float s[3]) {
+// particle from its loaded coordinates; s[] receives the offsets in float.
+__device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int c[3], float s[3]) {
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float v = 0.0f;
-        ok = ok && wrap_coord<float>(load_coord(pos, i, a, p.soa, p.n), p.xmin[a], p.L, v);
-        const double x = shifted((double)v, p.ishift, p.L);
+        float u = 0.0f;
+        ok = ok && wrap_coord<float>(v[a], p.xmin[a], p.L, u);
+        const double x = shifted((double)u, p.ishift, p.L);
         c[a] = cell_of(x, p.d, p.N);
         s[a] = (float)(x / p.d - (double)c[a] - 0.5);
     }
@@ -118,201 +126,265 @@ __device__ __forceinline__ float4 make_record(const int c[3], const float f[3]) 
     return make_float4(__uint_as_float(pk), f[0], f[1], f[2]);
 }
 
-__global__ void __launch_bounds__(256) k_tile_count(const float *__restrict__ pos, MasK p, TileK t,
-                                                    unsigned *__restrict__ counts,
-                                                    unsigned long long *dropped) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        int c[3];
-        float s[3];
-        if (!particle_cells(pos, i, p, c, s)) {
-            atomicAdd(dropped, 1ull);
-            continue;
-        }
-        int tx[2], ty[2], tz[2];
-        int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
-                for (int e = 0; e < nz; ++e)
-                    atomicAdd(&counts[((size_t)tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+// Particles i = base + u*stride (u < U, i < i1) are loaded before any is used,
+// so one thread keeps U loads in flight.
+constexpr int kBatch = 4;
+template <int U>
+__device__ __forceinline__ void load_batch(const float *__restrict__ pos, const MasK &p, size_t base, size_t stride,
+                                           size_t i1, float v[U][3]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t i = base + (size_t)u * stride;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[u][a] = i < i1 ? load_coord(pos, i, a, p.soa, p.n) : 0.0f;
     }
 }
 
-// Single-workgroup exclusive scan of the tile counts into offsets (and cursors),
-// and of the per-tile chunk counts into the work list.
-__global__ void __launch_bounds__(1024) k_tile_scan(const unsigned *__restrict__ counts, int ntiles, int CH,
-                                                    unsigned *__restrict__ offsets,
-                                                    unsigned *__restrict__ cursor,
-                                                    unsigned *__restrict__ item_off,
-                                                    unsigned *__restrict__ nitems) {
-    __shared__ unsigned long long sA[1024];
-    __shared__ unsigned sB[1024];
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const int per = (ntiles + NT - 1) / NT;
-    const int beg = min(ntiles, tid * per), end = min(ntiles, beg + per);
-    unsigned long long a = 0;
-    unsigned b = 0;
-    for (int i = beg; i < end; ++i) {
-        unsigned cnt = counts[i];
-        a += cnt;
-        b += cnt == 0 ? 1u : (cnt + CH - 1) / CH;
-    }
-    sA[tid] = a;
-    sB[tid] = b;
-    __syncthreads();
-    for (int off = 1; off < NT; off <<= 1) {
-        unsigned long long va = tid >= off ? sA[tid - off] : 0;
-        unsigned vb = tid >= off ? sB[tid - off] : 0;
-        __syncthreads();
-        sA[tid] += va;
-        sB[tid] += vb;
-        __syncthreads();
-    }
-    unsigned long long ra = sA[tid] - a;
-    unsigned rb = sB[tid] - b;
-    for (int i = beg; i < end; ++i) {
-        unsigned cnt = counts[i];
-        offsets[i] = (unsigned)ra;
-        cursor[i] = (unsigned)ra;
-        item_off[i] = rb;
-        ra += cnt;
-        rb += cnt == 0 ? 1u : (cnt + CH - 1) / CH;
-    }
-    if (tid == NT - 1) {
-        offsets[ntiles] = (unsigned)sA[NT - 1];
-        item_off[ntiles] = sB[NT - 1];
-        *nitems = sB[NT - 1];
-    }
-}
-
-__global__ void __launch_bounds__(256) k_tile_items(const unsigned *__restrict__ item_off, int ntiles,
-                                                    uint2 *__restrict__ items) {
-    int tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= ntiles) return;
-    unsigned b = item_off[tile], e = item_off[tile + 1];
-    for (unsigned k = b; k < e; ++k) items[k] = make_uint2((unsigned)tile, k - b);
-}
-
-__global__ void __launch_bounds__(256) k_tile_scatter(const float *__restrict__ pos, const float *__restrict__ w,
-                                                      MasK p, TileK t, unsigned *__restrict__ cursor,
-                                                      float4 *__restrict__ sorted, float *__restrict__ wsorted,
-                                                      unsigned cap, unsigned *__restrict__ overflow) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        int c[3];
-        float s[3];
-        if (!particle_cells(pos, i, p, c, s)) continue;
-        const float4 rec = make_record(c, s);
-        int tx[2], ty[2], tz[2];
-        int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
-                for (int e = 0; e < nz; ++e) {
-                    unsigned slot = atomicAdd(&cursor[((size_t)tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
-                    if (slot < cap) {
-                        sorted[slot] = rec;
-                        if (w) wsorted[slot] = w[i];
-                    } else {
-                        atomicOr(overflow, 1u);
-                    }
-                }
-    }
-}
-
-
-// Count with an LDS histogram per workgroup (no global atomics): workgroup g
-// owns particles [g*chunk, (g+1)*chunk) and writes its histogram row
-// hist[g*ntiles + tile] with coalesced stores.
-__global__ void __launch_bounds__(1024) k_tile_count_lds(const float *__restrict__ pos, MasK p, TileK t, int ntiles,
-                                                         size_t chunk, unsigned *__restrict__ hist,
-                                                         unsigned long long *dropped) {
+// Count records per tile.  LDS form: workgroup g owns particles
+// [g*chunk, (g+1)*chunk) and writes its histogram row hist[g*ntiles + tile];
+// global form (hist_lds == false, too many tiles for LDS): one row, global
+// atomics.
+template <int T, bool LDSH>
+__global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ pos, MasK p, TileK t, int ntiles,
+                                                     size_t chunk, unsigned *__restrict__ hist,
+                                                     unsigned long long *dropped) {
     extern __shared__ unsigned lh[];
-    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) lh[i] = 0u;
-    __syncthreads();
-    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
+    unsigned *h = hist;
+    size_t i0 = 0, i1 = p.n, step = (size_t)gridDim.x * blockDim.x;
+    if constexpr (LDSH) {
+        for (int i = threadIdx.x; i < ntiles; i += blockDim.x) lh[i] = 0u;
+        __syncthreads();
+        h = lh;
+        i0 = (size_t)blockIdx.x * chunk;
+        i1 = min(p.n, i0 + chunk);
+        step = blockDim.x;
+    } else {
+        i0 = (size_t)blockIdx.x * blockDim.x;
+    }
     unsigned ndrop = 0;
-    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        int c[3];
-        float s[3];
-        if (!particle_cells(pos, i, p, c, s)) {
-            ++ndrop;
-            continue;
+    for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * step) {
+        float v[kBatch][3];
+        load_batch<kBatch>(pos, p, base, step, i1, v);
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (base + (size_t)u * step >= i1) continue;
+            int c[3];
+            float f[3];
+            if (!cells_from(v[u], p, c, f)) {
+                ++ndrop;
+                continue;
+            }
+            int tx[2], ty[2], tz[2];
+            const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
+                      nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
+            for (int a = 0; a < nx; ++a)
+                for (int b = 0; b < ny; ++b)
+                    for (int e = 0; e < nz; ++e) atomicAdd(&h[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
         }
-        int tx[2], ty[2], tz[2];
-        int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
-                for (int e = 0; e < nz; ++e) atomicAdd(&lh[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
     }
     if (ndrop) atomicAdd(dropped, (unsigned long long)ndrop);
-    __syncthreads();
-    unsigned *row = hist + (size_t)blockIdx.x * ntiles;
-    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) row[i] = lh[i];
+    if constexpr (LDSH) {
+        __syncthreads();
+        unsigned *row = hist + (size_t)blockIdx.x * ntiles;
+        for (int i = threadIdx.x; i < ntiles; i += blockDim.x) row[i] = lh[i];
+    }
 }
 
-// Per tile: exclusive prefix over the workgroups' histogram rows (in place) and
-// the tile total.  One thread per tile; for a fixed row consecutive threads
-// touch consecutive words.
+// Per tile (one thread each): exclusive prefix over the histogram rows, in
+// place, and the tile total; then a workgroup scan of the totals and of the
+// deposit chunk counts.  offsets/item_off receive the workgroup-local
+// exclusive values, bsum[2*blk..] the workgroup sums.
 __global__ void __launch_bounds__(256) k_tile_prefix(unsigned *__restrict__ hist, int nrows, int ntiles,
-                                                     unsigned *__restrict__ totals) {
-    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= ntiles) return;
+                                                     unsigned *__restrict__ offsets, unsigned *__restrict__ item_off,
+                                                     unsigned *__restrict__ counts, unsigned *__restrict__ bsum) {
+    __shared__ unsigned sa[256], sb[256];
+    const int tid = threadIdx.x, tile = blockIdx.x * 256 + tid;
     unsigned run = 0;
-    for (int g = 0; g < nrows; ++g) {
-        unsigned *h = hist + (size_t)g * ntiles + tile;
-        const unsigned v = *h;
-        *h = run;
-        run += v;
+    if (tile < ntiles) {
+        int g = 0;
+        for (; g + 8 <= nrows; g += 8) {
+            unsigned v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = hist[(size_t)(g + k) * ntiles + tile];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                hist[(size_t)(g + k) * ntiles + tile] = run;
+                run += v[k];
+            }
+        }
+        for (; g < nrows; ++g) {
+            unsigned *hp = hist + (size_t)g * ntiles + tile;
+            const unsigned v = *hp;
+            *hp = run;
+            run += v;
+        }
     }
-    totals[tile] = run;
+    const unsigned chunks = tile < ntiles ? (run == 0 ? 1u : (run + kTileCH - 1) / kTileCH) : 0u;
+    sa[tid] = run;
+    sb[tid] = chunks;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const unsigned va = tid >= off ? sa[tid - off] : 0u, vb = tid >= off ? sb[tid - off] : 0u;
+        __syncthreads();
+        sa[tid] += va;
+        sb[tid] += vb;
+        __syncthreads();
+    }
+    if (tile < ntiles) {
+        offsets[tile] = sa[tid] - run;
+        item_off[tile] = sb[tid] - chunks;
+        counts[tile] = run;
+    }
+    if (tid == 255) {
+        bsum[2 * blockIdx.x] = sa[255];
+        bsum[2 * blockIdx.x + 1] = sb[255];
+    }
 }
 
-// Scatter with LDS cursors: cursor[tile] = offsets[tile] + prefix of earlier
-// workgroups; integer LDS atomics hand out the slots.
-__global__ void __launch_bounds__(1024) k_tile_scatter_lds(const float *__restrict__ pos, const float *__restrict__ w,
-                                                           MasK p, TileK t, int ntiles, size_t chunk,
-                                                           const unsigned *__restrict__ pref,
-                                                           const unsigned *__restrict__ offsets,
-                                                           float4 *__restrict__ sorted, float *__restrict__ wsorted,
-                                                           unsigned cap, unsigned *__restrict__ overflow) {
-    extern __shared__ unsigned cur[];
-    const unsigned *row = pref + (size_t)blockIdx.x * ntiles;
-    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cur[i] = offsets[i] + row[i];
-    __syncthreads();
-    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
-    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        int c[3];
-        float s[3];
-        if (!particle_cells(pos, i, p, c, s)) continue;
-        const float4 rec = make_record(c, s);
-        int tx[2], ty[2], tz[2];
-        int nx = axis_tiles(c[0], t, tx), ny = axis_tiles(c[1], t, ty), nz = axis_tiles(c[2], t, tz);
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
-                for (int e = 0; e < nz; ++e) {
-                    const unsigned slot = atomicAdd(&cur[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
-                    if (slot < cap) {
-                        sorted[slot] = rec;
-                        if (w) wsorted[slot] = w[i];
-                    } else {
-                        atomicOr(overflow, 1u);
-                    }
-                }
+// Exclusive scan of the workgroup sums (one workgroup, nblk <= 4096).
+__global__ void __launch_bounds__(1024) k_tile_bscan(unsigned *__restrict__ bsum, int nblk) {
+    __shared__ unsigned sa[1024], sb[1024];
+    const int tid = threadIdx.x;
+    constexpr int PER = 4;
+    unsigned a[PER], b[PER], ta = 0, tb = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid * PER + k;
+        a[k] = i < nblk ? bsum[2 * i] : 0u;
+        b[k] = i < nblk ? bsum[2 * i + 1] : 0u;
+        ta += a[k];
+        tb += b[k];
     }
+    sa[tid] = ta;
+    sb[tid] = tb;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned va = tid >= off ? sa[tid - off] : 0u, vb = tid >= off ? sb[tid - off] : 0u;
+        __syncthreads();
+        sa[tid] += va;
+        sb[tid] += vb;
+        __syncthreads();
+    }
+    unsigned ra = sa[tid] - ta, rb = sb[tid] - tb;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid * PER + k;
+        if (i < nblk) {
+            bsum[2 * i] = ra;
+            bsum[2 * i + 1] = rb;
+        }
+        ra += a[k];
+        rb += b[k];
+    }
+    if (tid == 1023) {
+        bsum[2 * nblk] = sa[1023];
+        bsum[2 * nblk + 1] = sb[1023];
+    }
+}
+
+// Final offsets, slot cursors and the deposit work list: one item
+// {tile, first record, end record, split} per chunk of at most kTileCH records,
+// clamped to the record buffer's capacity.  counts[tile] holds the tile total
+// on entry and the slot cursor (= first record) on exit.
+__global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned *__restrict__ bsum,
+                                                    unsigned *__restrict__ offsets, unsigned *__restrict__ item_off,
+                                                    unsigned *__restrict__ counts, uint4 *__restrict__ items,
+                                                    unsigned *__restrict__ nitems, unsigned cap) {
+    const int tile = blockIdx.x * 256 + threadIdx.x;
+    const int nblk = (ntiles + 255) / 256;
+    if (tile == 0) {
+        offsets[ntiles] = bsum[2 * nblk];
+        item_off[ntiles] = bsum[2 * nblk + 1];
+        *nitems = bsum[2 * nblk + 1];
+    }
+    if (tile >= ntiles) return;
+    const int blk = tile / 256;
+    const unsigned cnt = counts[tile];
+    const unsigned tb = offsets[tile] + bsum[2 * blk];
+    const unsigned ib = item_off[tile] + bsum[2 * blk + 1];
+    const unsigned te = tb + cnt;
+    offsets[tile] = tb;
+    item_off[tile] = ib;
+    counts[tile] = tb;
+    const unsigned nch = cnt == 0 ? 1u : (cnt + kTileCH - 1) / kTileCH;
+    const unsigned split = cnt > (unsigned)kTileCH ? 1u : 0u;
+    for (unsigned k = 0; k < nch; ++k) {
+        const unsigned b = min(tb + k * (unsigned)kTileCH, cap), e = min(min(te, tb + (k + 1) * (unsigned)kTileCH), cap);
+        items[ib + k] = make_uint4((unsigned)tile, b, e, split);
+    }
+}
+
+// Scatter records to their tiles.  LDS form: cursor[tile] = offsets[tile] +
+// prefix of earlier workgroups, integer LDS atomics hand out the slots; global
+// form: global cursors.
+template <int T, bool LDSH>
+__global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__ pos, const float *__restrict__ w,
+                                                       MasK p, TileK t, int ntiles, size_t chunk,
+                                                       const unsigned *__restrict__ pref,
+                                                       const unsigned *__restrict__ offsets,
+                                                       unsigned *__restrict__ gcursor, float4 *__restrict__ sorted,
+                                                       float *__restrict__ wsorted, unsigned cap,
+                                                       unsigned *__restrict__ overflow) {
+    extern __shared__ unsigned cur[];
+    unsigned *cu = gcursor;
+    size_t i0 = 0, i1 = p.n, step = (size_t)gridDim.x * blockDim.x;
+    if constexpr (LDSH) {
+        const unsigned *row = pref + (size_t)blockIdx.x * ntiles;
+        for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cur[i] = offsets[i] + row[i];
+        __syncthreads();
+        cu = cur;
+        i0 = (size_t)blockIdx.x * chunk;
+        i1 = min(p.n, i0 + chunk);
+        step = blockDim.x;
+    } else {
+        i0 = (size_t)blockIdx.x * blockDim.x;
+    }
+    unsigned ovf = 0;
+    for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * step) {
+        float v[kBatch][3], wv[kBatch];
+        load_batch<kBatch>(pos, p, base, step, i1, v);
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const size_t i = base + (size_t)u * step;
+            wv[u] = (w && i < i1) ? w[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (base + (size_t)u * step >= i1) continue;
+            int c[3];
+            float f[3];
+            if (!cells_from(v[u], p, c, f)) continue;
+            const float4 rec = make_record(c, f);
+            int tx[2], ty[2], tz[2];
+            const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
+                      nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
+            for (int a = 0; a < nx; ++a)
+                for (int b = 0; b < ny; ++b)
+                    for (int e = 0; e < nz; ++e) {
+                        const unsigned slot = atomicAdd(&cu[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+                        if (slot < cap) {
+                            sorted[slot] = rec;
+                            if (w) wsorted[slot] = wv[u];
+                        } else {
+                            ovf = 1u;
+                        }
+                    }
+        }
+    }
+    if (ovf) atomicOr(overflow, 1u);
 }
 
 // Zero the mesh cells of tiles that will be deposited by several workgroups.
+template <int T>
 __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restrict__ offsets, TileK t, int N,
                                                          float *__restrict__ mesh) {
     const int tile = blockIdx.x;
-    unsigned cnt = offsets[tile + 1] - offsets[tile];
-    if (cnt <= (unsigned)t.CH) return;
+    const unsigned cnt = offsets[tile + 1] - offsets[tile];
+    if (cnt <= (unsigned)kTileCH) return;
     const int tz = tile % t.ntx, ty = (tile / t.ntx) % t.ntx, tx = tile / (t.ntx * t.ntx);
-    const int T = t.T;
     for (int q = threadIdx.x; q < T * T * T; q += blockDim.x) {
-        int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
-        size_t idx = ((size_t)(tx * T + ix) * N + (ty * T + iy)) * N + (tz * T + iz);
-        mesh[idx] = 0.0f;
+        const int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
+        mesh[((size_t)(tx * T + ix) * N + (ty * T + iy)) * N + (tz * T + iz)] = 0.0f;
     }
 }
 
@@ -320,7 +392,6 @@ __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restr
 // atomics run at full rate on gfx950 (float LDS atomic add does not), and the
 // integer sum does not depend on the order of the records, so every tile that
 // one workgroup deposits is bit-reproducible.
-__device__ __forceinline__ long long to_fixed(float v) { return __double2ll_rn((double)v * 4294967296.0); }
 __device__ __forceinline__ float from_fixed(long long q) { return (float)((double)q * (1.0 / 4294967296.0)); }
 
 // Stencil of one record into the LDS grid, unrolled for the scheme.  A record
@@ -329,8 +400,8 @@ __device__ __forceinline__ float from_fixed(long long q) { return (float)((doubl
 // grid has S = T+4R cells per axis with local cell u at index u+2R, every
 // contribution lands in it without a range test, and only [0,T) is written
 // out.
-template <int SCHEME, bool UNIT>
-__device__ __forceinline__ void deposit_record(long long *lgrid, int S, int base, const float f[3], float wt) {
+template <int SCHEME, bool UNIT, int S>
+__device__ __forceinline__ void deposit_record(long long *lgrid, int base, const float f[3], float wt) {
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
     constexpr int W = 2 * R + 1;
     float wa[3][5];
@@ -355,33 +426,52 @@ __device__ __forceinline__ void deposit_record(long long *lgrid, int S, int base
     }
 }
 
-template <int SCHEME, bool UNIT>
+// One workgroup per work item (a tile, or a kTileCH-record chunk of a crowded
+// tile), persistent over items.  The records of the next item are loaded into
+// registers while the current tile is written out.
+template <int SCHEME, bool UNIT, int T>
 __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__ sorted,
                                                       const float *__restrict__ wsorted,
-                                                      const unsigned *__restrict__ offsets,
-                                                      const uint2 *__restrict__ items,
-                                                      const unsigned *__restrict__ nitems, MasK p, TileK t,
-                                                      unsigned cap, float *__restrict__ mesh, int accumulate) {
+                                                      const uint4 *__restrict__ items,
+                                                      const unsigned *__restrict__ nitems, int N, int ntx,
+                                                      float *__restrict__ mesh, int accumulate) {
     extern __shared__ long long lgrid[];
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
-    const int T = t.T, S = T + 4 * R, S3 = S * S * S;
+    constexpr int S = T + 4 * R, S3 = S * S * S, NT = 512, RPT = kTileCH / NT;
+    static_assert(S3 % 2 == 0, "LDS grid is cleared in 16-byte words");
+    const int tid = threadIdx.x;
     const unsigned total = *nitems;
-    const int N = p.N;
-    for (unsigned item = blockIdx.x; item < total; item += gridDim.x) {
-        const uint2 it = items[item];
-        const int tile = (int)it.x;
-        const int tz = tile % t.ntx, ty = (tile / t.ntx) % t.ntx, tx = tile / (t.ntx * t.ntx);
-        const int o[3] = {tx * T, ty * T, tz * T};
-        for (int q = threadIdx.x; q < S3; q += blockDim.x) lgrid[q] = 0ll;
+    unsigned item = blockIdx.x;
+    uint4 it = item < total ? items[item] : make_uint4(0u, 0u, 0u, 0u);
+    float4 rec[RPT];
+    float wr[RPT];
+    auto load_item = [&](const uint4 &x) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const unsigned idx = x.y + (unsigned)(tid + r * NT);
+            if (idx < x.z) {
+                rec[r] = sorted[idx];
+                wr[r] = UNIT ? 1.0f : wsorted[idx];
+            }
+        }
+    };
+    load_item(it);
+    for (; item < total; item += gridDim.x) {
+        const unsigned nxt = item + gridDim.x;
+        const uint4 itn = nxt < total ? items[nxt] : make_uint4(0u, 0u, 0u, 0u);
+        uint4 *lz = reinterpret_cast<uint4 *>(lgrid);
+        for (int q = tid; q < S3 / 2; q += NT) lz[q] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
-        const unsigned tb = offsets[tile], te = offsets[tile + 1];
-        const unsigned beg = tb + it.y * (unsigned)t.CH;
-        const unsigned end = min(min(te, beg + (unsigned)t.CH), cap);
-        for (unsigned i = beg + threadIdx.x; i < end; i += blockDim.x) {
-            const float4 r = sorted[i];
-            const unsigned pk = __float_as_uint(r.x);
+        const int tile = (int)it.x;
+        const int tz = tile % ntx, ty = (tile / ntx) % ntx, tx = tile / (ntx * ntx);
+        const int o[3] = {tx * T, ty * T, tz * T};
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const unsigned idx = it.y + (unsigned)(tid + r * NT);
+            if (idx >= it.z) continue;
+            const unsigned pk = __float_as_uint(rec[r].x);
             const int cc[3] = {(int)(pk & 1023u), (int)((pk >> 10) & 1023u), (int)(pk >> 20)};
-            const float f[3] = {r.y, r.z, r.w};
+            const float f[3] = {rec[r].y, rec[r].z, rec[r].w};
             int l[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -390,15 +480,20 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
                 if (lc >= N - R) lc -= N;
                 l[a] = lc + R;  // node c-R (local lc-R) sits at LDS index lc-R+2R
             }
-            const int base = (l[0] * S + l[1]) * S + l[2];
-            deposit_record<SCHEME, UNIT>(lgrid, S, base, f, UNIT ? 1.0f : wsorted[i]);
+            deposit_record<SCHEME, UNIT, S>(lgrid, (l[0] * S + l[1]) * S + l[2], f, wr[r]);
         }
+        const bool split = it.w != 0u;
+        it = itn;
+        load_item(it);  // in flight during the write-out
         __syncthreads();
-        const bool split = (te - tb) > (unsigned)t.CH;
-        for (int q = threadIdx.x; q < T * T * T; q += blockDim.x) {
-            int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
-            float v = from_fixed(lgrid[((ix + 2 * R) * S + (iy + 2 * R)) * S + (iz + 2 * R)]);
-            size_t idx = ((size_t)(o[0] + ix) * N + (o[1] + iy)) * N + (o[2] + iz);
+        constexpr int TT = T * T * T;
+#pragma unroll
+        for (int k = 0; k < (TT + NT - 1) / NT; ++k) {
+            const int q = tid + k * NT;
+            if (TT % NT != 0 && q >= TT) break;
+            const int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
+            const float v = from_fixed(lgrid[((ix + 2 * R) * S + (iy + 2 * R)) * S + (iz + 2 * R)]);
+            const size_t idx = ((size_t)(o[0] + ix) * N + (o[1] + iy)) * N + (o[2] + iz);
             if (split) {
                 if (v != 0.0f) unsafeAtomicAdd(&mesh[idx], v);
             } else if (accumulate) {
@@ -480,95 +575,98 @@ static int tile_edge(int N) {
     return 0;
 }
 
-// Capacity of the sorted-record buffer: when `exact`, the total number of
-// records is read back after the scan (one host synchronisation) and the buffer
-// sized to it; otherwise a generous estimate is used and the scatter reports
-// overflow through `overflow` (checked by the caller).
-static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
-                           int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
+template <int T>
+static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
+                             int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
     TileK t;
-    t.T = tile_edge(p.N);
+    t.T = T;
     t.R = mas_radius(p.scheme);
-    t.ntx = p.N / t.T;
-    t.CH = 4096;
+    t.ntx = p.N / T;
+    t.CH = kTileCH;
     const int ntiles = t.ntx * t.ntx * t.ntx;
     const size_t max_copies = p.n * (t.R > 0 ? 8 : 1);
     if (max_copies > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many particles");
-    void *counts, *offsets, *cursor, *item_off, *items, *nitems, *sorted;
+    const int nblk = (ntiles + 255) / 256;
+    if (nblk > 4096) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many tiles");
+    // LDS-privatised histogram and cursors when a workgroup's LDS holds one
+    // counter per tile, else one global row
+    const size_t hist_lds = sizeof(unsigned) * (size_t)ntiles;
+    const bool lds_hist = hist_lds <= ctx->lds_per_block;
+    const int nrows = lds_hist ? std::max(1, ctx->cu_count) : 1;
+    const size_t chunk = (p.n + nrows - 1) / nrows;
+    void *hist, *counts, *offsets, *item_off, *bsum, *items, *nitems, *sorted;
     cm_status s;
+    if ((s = scratch(ctx, "mas.hist", sizeof(unsigned) * (size_t)nrows * ntiles, &hist)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.counts", sizeof(unsigned) * ntiles, &counts)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.offsets", sizeof(unsigned) * (ntiles + 1), &offsets)) != CM_OK) return s;
-    if ((s = scratch(ctx, "mas.cursor", sizeof(unsigned) * ntiles, &cursor)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.item_off", sizeof(unsigned) * (ntiles + 1), &item_off)) != CM_OK) return s;
-    const size_t max_items = (size_t)ntiles + max_copies / t.CH + 1;
-    if ((s = scratch(ctx, "mas.items", sizeof(uint2) * max_items, &items)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.bsum", sizeof(unsigned) * 2 * (size_t)(nblk + 1), &bsum)) != CM_OK) return s;
+    const size_t max_items = (size_t)ntiles + max_copies / kTileCH + 1;
+    if ((s = scratch(ctx, "mas.items", sizeof(uint4) * max_items, &items)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.nitems", sizeof(unsigned), &nitems)) != CM_OK) return s;
-    const unsigned g = grid_for(p.n, 256, 8192);
-    // LDS-privatised histogram when a workgroup's LDS holds one counter per tile
-    const size_t hist_lds = sizeof(unsigned) * (size_t)ntiles;
-    const bool lds_hist = hist_lds <= ctx->lds_per_block && ntiles <= 65536;
-    const int nrows = ctx->cu_count;
-    const size_t chunk = (p.n + nrows - 1) / nrows;
-    void *hist = nullptr;
+    auto *hp = (unsigned *)hist;
     if (lds_hist) {
-        if ((s = scratch(ctx, "mas.hist", sizeof(unsigned) * (size_t)nrows * ntiles, &hist)) != CM_OK) return s;
-        if ((s = launch(ctx, "mas_tile_count", k_tile_count_lds, dim3(nrows), dim3(1024), hist_lds, pos, p, t, ntiles,
-                        chunk, (unsigned *)hist, dropped)) != CM_OK)
-            return s;
-        if ((s = launch(ctx, "mas_tile_prefix", k_tile_prefix, dim3((ntiles + 255) / 256), dim3(256), 0,
-                        (unsigned *)hist, nrows, ntiles, (unsigned *)counts)) != CM_OK)
+        if ((s = launch(ctx, "mas_tile_count", k_tile_count<T, true>, dim3(nrows), dim3(1024), hist_lds, pos, p, t,
+                        ntiles, chunk, hp, dropped)) != CM_OK)
             return s;
     } else {
-        if ((s = fill(ctx, (unsigned *)counts, (size_t)ntiles, 0u)) != CM_OK) return s;
-        if ((s = launch(ctx, "mas_tile_count", k_tile_count, dim3(g), dim3(256), 0, pos, p, t, (unsigned *)counts,
-                        dropped)) != CM_OK)
+        if ((s = fill(ctx, hp, (size_t)ntiles, 0u)) != CM_OK) return s;
+        const unsigned g = grid_for(p.n, 1024 * kBatch, 2048);
+        if ((s = launch(ctx, "mas_tile_count", k_tile_count<T, false>, dim3(g), dim3(1024), 0, pos, p, t, ntiles,
+                        chunk, hp, dropped)) != CM_OK)
             return s;
     }
-    if ((s = launch(ctx, "mas_tile_scan", k_tile_scan, dim3(1), dim3(1024), 0, (const unsigned *)counts, ntiles,
-                    t.CH, (unsigned *)offsets, (unsigned *)cursor, (unsigned *)item_off, (unsigned *)nitems)) != CM_OK)
+    if ((s = launch(ctx, "mas_tile_prefix", k_tile_prefix, dim3(nblk), dim3(256), 0, hp, nrows, ntiles,
+                    (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts, (unsigned *)bsum)) != CM_OK)
+        return s;
+    if ((s = launch(ctx, "mas_tile_scan", k_tile_bscan, dim3(1), dim3(1024), 0, (unsigned *)bsum, nblk)) != CM_OK)
         return s;
     size_t cap;
     if (exact) {
         unsigned total = 0;
-        CM_HIP(ctx, hipMemcpyAsync(&total, (unsigned *)offsets + ntiles, sizeof(unsigned), hipMemcpyDeviceToHost,
+        CM_HIP(ctx, hipMemcpyAsync(&total, (unsigned *)bsum + 2 * nblk, sizeof(unsigned), hipMemcpyDeviceToHost,
                                    ctx->stream));
         CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         cap = (size_t)total;
     } else {
-        const double g1 = 1.0 + 2.0 * t.R / t.T;
+        // geometric ghost fraction (1 + 2R/T)^3 with 50% headroom; a larger
+        // count sets *overflow and the caller repeats with exact = true
+        const double g1 = 1.0 + 2.0 * t.R / T;
         cap = std::min(max_copies, (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
     }
     if ((s = scratch(ctx, "mas.sorted", sizeof(float4) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
     void *wsorted = nullptr;
     if (w && (s = scratch(ctx, "mas.wsorted", sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK) return s;
-    if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3((ntiles + 255) / 256), dim3(256), 0,
-                    (const unsigned *)item_off, ntiles, (uint2 *)items)) != CM_OK)
+    if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3(nblk), dim3(256), 0, ntiles, (const unsigned *)bsum,
+                    (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts, (uint4 *)items, (unsigned *)nitems,
+                    (unsigned)cap)) != CM_OK)
         return s;
     if (lds_hist) {
-        if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter_lds, dim3(nrows), dim3(1024), hist_lds, pos, w, p, t,
-                        ntiles, chunk, (const unsigned *)hist, (const unsigned *)offsets, (float4 *)sorted,
-                        (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
+        if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter<T, true>, dim3(nrows), dim3(1024), hist_lds, pos, w, p,
+                        t, ntiles, chunk, (const unsigned *)hp, (const unsigned *)offsets, (unsigned *)counts,
+                        (float4 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
             return s;
-    } else if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter, dim3(g), dim3(256), 0, pos, w, p, t,
-                           (unsigned *)cursor, (float4 *)sorted, (float *)wsorted, (unsigned)cap,
-                           overflow)) != CM_OK) {
-        return s;
+    } else {
+        const unsigned g = grid_for(p.n, 1024 * kBatch, 2048);
+        if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter<T, false>, dim3(g), dim3(1024), 0, pos, w, p, t,
+                        ntiles, chunk, (const unsigned *)hp, (const unsigned *)offsets, (unsigned *)counts,
+                        (float4 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
+            return s;
     }
     if (!accumulate) {
-        if ((s = launch(ctx, "mas_tile_zero_split", k_tile_zero_split, dim3(ntiles), dim3(256), 0,
+        if ((s = launch(ctx, "mas_tile_zero_split", k_tile_zero_split<T>, dim3(ntiles), dim3(256), 0,
                         (const unsigned *)offsets, t, p.N, mesh)) != CM_OK)
             return s;
     }
-    const int S = t.T + 4 * t.R;
+    const int S = T + 4 * t.R;
     const size_t lds = sizeof(long long) * (size_t)S * S * S;
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "mas: deposit grid of %zu bytes", lds);
-    unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)ctx->cu_count * 8);
+    const unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * 8);
     const float *ws = (const float *)wsorted;
     const dim3 gd(dg), bd(512);
-#define CM_DEPOSIT(SCH, UNIT)                                                                                   \
-    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT>, gd, bd, lds, (const float4 *)sorted, ws,        \
-           (const unsigned *)offsets, (const uint2 *)items, (const unsigned *)nitems, p, t, (unsigned)cap, mesh, \
-           accumulate)
+#define CM_DEPOSIT(SCH, UNIT)                                                                                    \
+    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const float4 *)sorted, ws,     \
+           (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, mesh, accumulate)
     switch (p.scheme * 2 + (w ? 0 : 1)) {
     case 0: return CM_DEPOSIT(0, false);
     case 1: return CM_DEPOSIT(0, true);
@@ -583,8 +681,22 @@ static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const f
     return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
 }
 
+// Capacity of the sorted-record buffer: when `exact`, the total number of
+// records is read back after the scan (one host synchronisation) and the buffer
+// sized to it; otherwise a generous estimate is used and the scatter reports
+// overflow through `overflow` (checked by the caller).
+static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
+                           int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
+    switch (tile_edge(p.N)) {
+    case 16: return mas_tiled_t<16>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow);
+    case 8: return mas_tiled_t<8>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow);
+    }
+    return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: N=%d", p.N);
+}
+
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w, size_t n,
-                           void *mesh, cm_dtype mdt, int accumulate) {
+                           void *mesh, cm_dtype mdt, int accumulate, bool exact, unsigned **overflow_out) {
+    if (overflow_out) *overflow_out = nullptr;
     MasK p = make_mask(d, n);
     const size_t n3 = (size_t)d->N * d->N * d->N;
     void *dropped;
@@ -599,8 +711,9 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
         void *ovf;
         if ((s = scratch(ctx, "mas.overflow", sizeof(unsigned), &ovf)) != CM_OK) return s;
         if ((s = fill(ctx, (unsigned *)ovf, 1, 0u)) != CM_OK) return s;
+        if (overflow_out) *overflow_out = (unsigned *)ovf;
         return mas_tiled(ctx, p, (const float *)pos, (const float *)w, (float *)mesh, accumulate,
-                         (unsigned long long *)dropped, true, (unsigned *)ovf);
+                         (unsigned long long *)dropped, exact || !overflow_out, (unsigned *)ovf);
     }
     if (!accumulate) {
         s = mdt == CM_F32 ? fill(ctx, (float *)mesh, n3, 0.0f) : fill(ctx, (double *)mesh, n3, 0.0);

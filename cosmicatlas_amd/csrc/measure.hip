@@ -65,9 +65,11 @@ static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<
     pk_finalize(o->nshells, normal > 0 ? normal : 1.0, SN * normal, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
 }
 
+// Returns CM_OK with *redo set when the tiled deposit's record estimate was too
+// small (the results are then discarded and the caller repeats exactly).
 static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, size_t n, double *pk0,
                                     double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
-                                    cm_measure_info *info) {
+                                    cm_measure_info *info, const unsigned *overflow, bool *redo) {
     const int N = o->mas.N;
     const double n3 = (double)N * N * N;
     double *sums, *nobj;
@@ -85,7 +87,11 @@ static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void 
     CM_HIP(ctx, hipMemcpyAsync(&nobj_h, nobj, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (o->do_tweb)
         CM_HIP(ctx, hipMemcpyAsync(hst, stats, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned ovf = 0;
+    if (overflow) CM_HIP(ctx, hipMemcpyAsync(&ovf, overflow, sizeof ovf, hipMemcpyDeviceToHost, ctx->stream));
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *redo = ovf != 0;
+    if (*redo) return CM_OK;
     if (info) {
         for (int i = 0; i < 4; ++i) info->tweb.counts[i] = (int64_t)hst[i];
         info->tweb.n_marginal = (int64_t)hst[4];
@@ -108,8 +114,19 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
 
     cm_mas_desc md = o->mas;
     md.algo = o->fused ? CM_MAS_AUTO : CM_MAS_PLAIN;
+    if (o->fused) {
+        // first attempt without the record-count read-back; repeat exactly on overflow
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            unsigned *ovf = nullptr;
+            if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, attempt > 0, &ovf)) != CM_OK) return s;
+            bool redo = false;
+            if ((s = measure_fused_tail(ctx, o, mesh, n, pk0, pk2, pk4, nmodes, labels, info, ovf, &redo)) != CM_OK)
+                return s;
+            if (!redo) return CM_OK;
+        }
+        return fail(ctx, CM_ERR_DEVICE, "cm_measure: record buffer overflow after exact sizing");
+    }
     if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0)) != CM_OK) return s;
-    if (o->fused) return measure_fused_tail(ctx, o, mesh, n, pk0, pk2, pk4, nmodes, labels, info);
     // N_gal = FP64 sum of the mesh (get_nobjects without its float sum)
     double nobj_h = 0.0;
     if ((s = mesh_sum_impl(ctx, mesh, CM_F32, n3, (double *)nobj, &nobj_h)) != CM_OK) return s;
