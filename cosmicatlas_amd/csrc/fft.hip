@@ -272,7 +272,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__res
     auto dst = [&](int b, int e, float2 v) {
         if (b < valid) data[base + (size_t)e * stride + b] = v;
     };
-    ct_fft<float, N, NB, 0, true, FWD>(lds, tw, src, dst);
+    ct_fft<float, N, NB, NB, true, FWD>(lds, tw, src, dst);
 }
 
 // column block width of the float32 y/x passes

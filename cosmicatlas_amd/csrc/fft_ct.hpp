@@ -63,7 +63,8 @@ constexpr int ct_E(int n) {
 constexpr int ct_row_stride(int M) { return M + 1 + ((M + 1) >> 3) + 1; }
 
 // Stage S of the plan for length n (Ns = product of the earlier radices).
-// COLS: element e of sequence b at LDS e*NB + b; rows: b*STRIDE + rpad(e).
+// COLS: element e of sequence b at LDS e*STRIDE + b (STRIDE >= NB; NB+1 keeps
+// per-element rows on distinct banks); rows: b*STRIDE + rpad(e).
 // The first stage reads through src(b, e), the last writes through
 // dst(b, e, v); the caller owns the barrier before the first stage.
 template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int S, int Ns, class Src, class Dst>
@@ -92,7 +93,7 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds, const v2<T> *__restrict__ t
             if constexpr (FIRST)
                 v[i * R + r] = src(bb[i], e);
             else
-                v[i * R + r] = lds[COLS ? e * NB + bb[i] : bb[i] * STRIDE + rpad(e)];
+                v[i * R + r] = lds[COLS ? e * STRIDE + bb[i] : bb[i] * STRIDE + rpad(e)];
         }
     }
     __syncthreads();
@@ -115,7 +116,7 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds, const v2<T> *__restrict__ t
             if constexpr (LAST)
                 dst(bb[i], e, v[i * R + q]);
             else
-                lds[COLS ? e * NB + bb[i] : bb[i] * STRIDE + rpad(e)] = v[i * R + q];
+                lds[COLS ? e * STRIDE + bb[i] : bb[i] * STRIDE + rpad(e)] = v[i * R + q];
         }
     }
     if constexpr (!LAST) {

@@ -4,8 +4,9 @@
 // row and all N kx values.  In one visit of that tile it
 //   1. runs the forward x FFT (global -> registers/LDS),
 //   2. bins every mode of the tile into the P(k) shells (the same enumeration,
-//      shell rule, MAS deconvolution and shot-noise weight as k_pk_shells; runs
-//      of equal (kx, shell) are combined with a wavefront segmented scan),
+//      shell rule, MAS deconvolution and shot-noise weight as k_pk_shells): one
+//      thread per kx row walks the tile's kz values in order, so |k| grows and
+//      a run of equal shells is summed in registers and added once,
 //   3. multiplies by exp(-k^2R^2/2)(-1/k^2) N^3/N_gal for the T-web potential,
 //   4. runs the inverse x FFT (LDS -> global).
 // The spectrum is read and written once instead of four times (x pass, shell
@@ -33,50 +34,40 @@ __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restr
     auto srcG = [&](int b, int e) {
         return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
     };
-    auto dstL = [&](int b, int e, float2 v) { lds[e * NB + b] = v; };
-    ct_fft<float, N, NB, 0, true, true>(lds, tw, srcG, dstL);
+    auto dstL = [&](int b, int e, float2 v) { lds[e * (NB + 1) + b] = v; };
+    ct_fft<float, N, NB, NB + 1, true, true>(lds, tw, srcG, dstL);
 }
 
 template <int N, int NB>
 __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict__ data, size_t base, size_t xstride,
                                                               int valid, float2 *lds,
                                                               const float2 *__restrict__ tw) {
-    auto srcL = [&](int b, int e) { return lds[e * NB + b]; };
+    auto srcL = [&](int b, int e) { return lds[e * (NB + 1) + b]; };
     auto dstG = [&](int b, int e, float2 v) {
         if (b < valid) data[base + (size_t)e * xstride + b] = v;
     };
-    ct_fft<float, N, NB, 0, true, false>(lds, tw, srcL, dstG);
+    ct_fft<float, N, NB, NB + 1, true, false>(lds, tw, srcL, dstG);
 }
 
-// Row-local lane shifts (DPP row_shr / row_shl inside rows of 16 lanes): a lane
-// whose source falls outside its row keeps its own value, which the callers
-// never use (they only combine lanes of the same nb-lane group, nb <= 16).
-template <int OFF> __device__ __forceinline__ int dpp_shr(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, 0x110 + OFF, 0xf, 0xf, false);
-}
-template <int OFF> __device__ __forceinline__ float dpp_shr(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x110 + OFF, 0xf, 0xf,
-                                                      false));
-}
-template <int OFF> __device__ __forceinline__ int dpp_shl(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, 0x100 + OFF, 0xf, 0xf, false);
-}
-
-// Segmented inclusive scan over groups of NB lanes (NB = 8 or 16) keyed by the
-// shell: lanes of one group with equal shells form one contiguous run.
-template <int NB, int OFF>
-__device__ __forceinline__ void seg_step(int gl, int &sh, float &a0, float &a1, float &a2, float &a3, int &c) {
-    const int ush = dpp_shr<OFF>(sh);
-    const float u0 = dpp_shr<OFF>(a0), u1 = dpp_shr<OFF>(a1), u2 = dpp_shr<OFF>(a2), u3 = dpp_shr<OFF>(a3);
-    const int uc = dpp_shr<OFF>(c);
-    if (gl >= OFF && ush == sh) {
-        a0 += u0;
-        a1 += u1;
-        a2 += u2;
-        a3 += u3;
-        c += uc;
+// Shell accumulator of one thread: the modes a thread visits along kz have
+// non-decreasing |k|, so equal shells form one run that is added to the
+// workgroup's LDS sums when the shell changes.
+struct ShellAcc {
+    int sh;
+    float a0, a1, a2, a3;
+    unsigned c;
+    __device__ __forceinline__ void flush(double *s_acc, unsigned *s_cnt, int ns) {
+        if (sh >= 0 && c) {
+            atomicAdd(&s_acc[sh], (double)a0);
+            atomicAdd(&s_acc[ns + sh], (double)a1);
+            atomicAdd(&s_acc[2 * ns + sh], (double)a2);
+            atomicAdd(&s_acc[3 * ns + sh], (double)a3);
+            atomicAdd(&s_cnt[sh], c);
+        }
+        a0 = a1 = a2 = a3 = 0.f;
+        c = 0;
     }
-}
+};
 
 template <int N, int NB>
 __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__restrict__ data, int pitch,
@@ -88,17 +79,19 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
     constexpr int M1 = N / 2 + 1;
     const int ns = pk.nshells;
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
-    double *s_acc = reinterpret_cast<double *>(lds + (size_t)NB * N);  // 4*ns
-    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);     // ns
-    float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                // M1
-    float *s_sq = s_ic + M1;                                            // M1
-    float *s_g = s_sq + M1;                                             // N
+    double *s_acc = reinterpret_cast<double *>(lds + (size_t)(NB + 1) * N);  // 4*ns
+    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);           // ns
+    float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                      // M1
+    float *s_sq = s_ic + M1;                                                  // M1
+    float *s_sqic = s_sq + M1;                                                // M1
+    float *s_g = s_sqic + M1;                                                 // N
     constexpr int NT = ColShape<N, NB>::NT;
     const int ncols = N / 2 + 1;
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x;
     for (int i = tid; i < M1; i += NT) {
         s_ic[i] = (float)tabs[i];
         s_sq[i] = (float)tabs[M1 + i];
+        s_sqic[i] = (float)(tabs[M1 + i] * tabs[i]);
     }
     for (int i = tid; i < N; i += NT) s_g[i] = gauss[i];
     for (int i = tid; i < 4 * ns; i += NT) s_acc[i] = 0.0;
@@ -118,60 +111,84 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
         const int valid = min(NB, ncols - kz0);
         x_fft_fwd_to_lds<N, NB>(data, base, xstride, valid, lds, tw);
         __syncthreads();
-        // shells (+ multiply), element t = e*nb + b (kz fastest): runs of equal
-        // (kx, shell) lie inside one group of nb lanes
-        const int total = NB * N;
+        // shells (+ multiply): one thread per kx row, serial along the tile's kz
+        // (|k| grows with kz, so shells are visited in order)
         const int ky = iy <= h ? iy : iy - N;
+        const bool yl = iy < h;
+        const int fy = yl ? iy : N - iy;
         const float gy = s_g[iy];
-        for (int t0 = tid & ~63; t0 < total; t0 += NT) {
-            const int t = t0 + lane;
-            int sh = -1;
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-            int c = 0;
-            if (t < total) {
-                const int b = t % NB, ix = t / NB, iz = kz0 + b;
-                if (b < valid) {
-                    float2 v = lds[t];
-                    int fx, fy;
-                    if (mode_counted(N, ix, iy, iz, fx, fy)) {
-                        sh = shell_of(pk, fx, fy, iz);
-                        if (sh >= 0) {
-                            const float pw = v.x * v.x + v.y * v.y;
-                            const float ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
-                            const int q = fx * fx + fy * fy + iz * iz;
-                            const float mu2 = (float)(iz * iz) * __builtin_amdgcn_rcpf((float)q);
-                            const float pc = pw * ic;
-                            a0 = pc;
-                            a1 = s_sq[fx] * s_sq[fy] * s_sq[iz] * ic;
-                            a2 = (1.5f * mu2 - 0.5f) * pc;
-                            a3 = ((4.375f * mu2 - 3.75f) * mu2 + 0.375f) * pc;
-                            c = 1;
+        for (int ix = tid; ix < N; ix += NT) {
+            const bool xl = ix < h;
+            const int fx = xl ? ix : N - ix;
+            // mode_counted: Nyquist x/y planes never; the kz = 0 plane only for
+            // (i, j) with xl && yl && (i|j) != 0, or !xl && yl && j > 0
+            const bool excl = ix == h || iy == h;
+            const bool zero_ok = !excl && yl && (xl ? (ix | iy) != 0 : iy > 0);
+            const int qxy = fx * fx + fy * fy;
+            const float icxy = s_ic[fx] * s_ic[fy], sqxy = s_sq[fx] * s_sq[fy] * icxy;
+            const int kx = xl ? ix : ix - N;
+            const int q2xy = kx * kx + ky * ky;
+            const float gxy = fscale * s_g[ix] * gy;
+            float2 *row = lds + ix * (NB + 1);
+            ShellAcc acc{-1, 0.f, 0.f, 0.f, 0.f, 0u};
+            // integer shell rule: raw index r with (r ndel + Dmin)^2 <= q < hiq
+            int r = -1, hiq = 0;
+            if (pk.int_rule && !excl) {
+                const int q0 = qxy + kz0 * kz0;
+                r = (int)floorf((__builtin_sqrtf((float)q0) - (float)pk.Dmin_i) * pk.rndel_f);
+                r = max(r, -1);
+#pragma unroll
+                for (int it = 0; it < 2; ++it) {
+                    const int lo = r * pk.ndel_i + pk.Dmin_i, hi = (r + 1) * pk.ndel_i + pk.Dmin_i;
+                    if (r >= 0 && q0 < lo * lo) --r;
+                    else if (q0 >= hi * hi) ++r;
+                }
+                if (q0 < pk.Dmin_i * pk.Dmin_i) r = -1;
+                const int hi = (r + 1) * pk.ndel_i + pk.Dmin_i;
+                hiq = hi * hi;
+            }
+#pragma unroll 4
+            for (int b = 0; b < NB; ++b) {
+                if (b >= valid) break;
+                const int iz = kz0 + b;
+                float2 v = row[b];
+                const int q = qxy + iz * iz;
+                int sh = -1;
+                if (!excl && (iz > 0 || zero_ok)) {
+                    if (pk.int_rule) {
+                        while (q >= hiq && r < ns) {
+                            ++r;
+                            const int hi = (r + 1) * pk.ndel_i + pk.Dmin_i;
+                            hiq = hi * hi;
                         }
-                    }
-                    if (do_tweb) {
-                        const int kx = ix <= h ? ix : ix - N;
-                        const int q2 = kx * kx + ky * ky + iz * iz;
-                        const float f = q2 > 0 ? fscale * s_g[ix] * gy * s_g[iz] * __builtin_amdgcn_rcpf((float)q2)
-                                               : 0.0f;
-                        v.x *= f;
-                        v.y *= f;
-                        lds[t] = v;
+                        sh = (r >= 0 && r < ns) ? r : -1;
+                    } else {
+                        sh = shell_of(pk, fx, fy, iz);
                     }
                 }
+                if (sh != acc.sh) {
+                    acc.flush(s_acc, s_cnt, ns);
+                    acc.sh = sh;
+                }
+                const int q2 = q2xy + iz * iz;
+                const float rq = q2 > 0 ? __builtin_amdgcn_rcpf((float)q2) : 0.0f;
+                if (sh >= 0) {
+                    const float pc = (v.x * v.x + v.y * v.y) * icxy * s_ic[iz];
+                    const float mu2 = (float)(iz * iz) * rq;
+                    acc.a0 += pc;
+                    acc.a1 += sqxy * s_sqic[iz];
+                    acc.a2 += (1.5f * mu2 - 0.5f) * pc;
+                    acc.a3 += ((4.375f * mu2 - 3.75f) * mu2 + 0.375f) * pc;
+                    acc.c += 1u;
+                }
+                if (do_tweb) {
+                    const float f = gxy * s_g[iz] * rq;
+                    v.x *= f;
+                    v.y *= f;
+                    row[b] = v;
+                }
             }
-            const int gl = lane & (NB - 1);
-            seg_step<NB, 1>(gl, sh, a0, a1, a2, a3, c);
-            seg_step<NB, 2>(gl, sh, a0, a1, a2, a3, c);
-            seg_step<NB, 4>(gl, sh, a0, a1, a2, a3, c);
-            if constexpr (NB == 16) seg_step<NB, 8>(gl, sh, a0, a1, a2, a3, c);
-            const int nsh = dpp_shl<1>(sh);
-            if (sh >= 0 && (gl == NB - 1 || nsh != sh)) {
-                atomicAdd(&s_acc[sh], (double)a0);
-                atomicAdd(&s_acc[ns + sh], (double)a1);
-                atomicAdd(&s_acc[2 * ns + sh], (double)a2);
-                atomicAdd(&s_acc[3 * ns + sh], (double)a3);
-                atomicAdd(&s_cnt[sh], (unsigned)c);
-            }
+            acc.flush(s_acc, s_cnt, ns);
         }
         __syncthreads();
         if (do_tweb) x_fft_inv_from_lds<N, NB>(data, base, xstride, valid, lds, tw);
@@ -215,8 +232,8 @@ static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, int pitch, const float2 *t
                              double *dsum, unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
                              const double *dnobj, int ns) {
     constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB>::NT;
-    const size_t lds = sizeof(float2) * (size_t)NB * N + sizeof(double) * 4 * (size_t)ns +
-                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (2 * (size_t)M1 + (size_t)N);
+    const size_t lds = sizeof(float2) * (size_t)(NB + 1) * N + sizeof(double) * 4 * (size_t)ns +
+                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (3 * (size_t)M1 + (size_t)N);
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
     int per_cu = 0;
@@ -302,7 +319,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     if ((s = scratch(ctx, "f.gauss", sizeof(float) * (size_t)N, &dgauss)) != CM_OK) return s;
     CM_HIP(ctx, hipMemcpyAsync(dgauss, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice, ctx->stream));
     const float inv_kf2 = (float)(1.0 / (kf * kf));
-    const int nb = ctx->tune.xnb == 8 ? 8 : 16;  // lane groups of the DPP shell scan (16 when it fits)
+    const int nb = ctx->tune.xnb == 8 ? 8 : 16;  // kz columns per tile (16 when the workgroup fits)
     switch (N) {
 #define CM_CASE(n)                                                                                                     \
     case n:                                                                                                            \

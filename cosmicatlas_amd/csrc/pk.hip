@@ -160,7 +160,11 @@ PkK make_pkk(const cm_pk_opts *o) {
     p.Dmin = o->kmin / kf;
     p.ndel = o->dk / kf;
     long di = 0, ni = 0;
-    p.int_rule = (near_int(p.Dmin, di) && near_int(p.ndel, ni) && ni > 0) ? 1 : 0;
+    // the integer rule squares shell edges in 32-bit arithmetic: keep them below 40000
+    p.int_rule = (near_int(p.Dmin, di) && near_int(p.ndel, ni) && ni > 0 && di >= 0 &&
+                  di + (long)(o->nshells + 1) * ni < 40000)
+                     ? 1
+                     : 0;
     p.Dmin_i = (int)di;
     p.ndel_i = (int)ni;
     p.float_rule = o->float_shell_rule ? 1 : 0;

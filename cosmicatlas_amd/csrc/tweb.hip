@@ -110,11 +110,12 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
 // Rare path of the fused kernel (kept out of line so its FP64 registers do not
 // count against the marching loop): closed-form eigenvalues, optional lambda
 // output, and the exact margin test.  Returns 1 for a marginal cell.
+// The fused kernel passes T' = T/ih^2 and ih2 = ih^2.
 __device__ __attribute__((noinline)) unsigned eig_margin(float txx, float tyy, float tzz, float a10, float a20,
-                                                         float a21, double lth, double tol, bool test,
+                                                         float a21, float ih2, double lth, double tol, bool test,
                                                          float *lam, size_t cell, size_t n3) {
     double e1, e2, e3;
-    eig3(txx, tyy, tzz, a10, a20, a21, e1, e2, e3);
+    eig3(txx * ih2, tyy * ih2, tzz * ih2, a10 * ih2, a20 * ih2, a21 * ih2, e1, e2, e3);
     if (lam) {
         lam[cell] = (float)e1;
         lam[n3 + cell] = (float)e2;
@@ -124,7 +125,7 @@ __device__ __attribute__((noinline)) unsigned eig_margin(float txx, float tyy, f
 }
 
 template <int TY, int TZ, int NT>
-__global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi, int N, int segLen, float ih,
+__global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi, int N, int segLen, float ih2,
                                                    double lth, double tol, int8_t *__restrict__ labels,
                                                    float *__restrict__ lam, unsigned long long *__restrict__ stats) {
     constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
@@ -144,7 +145,9 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     const int ty0 = (blockIdx.x / ntz) * TY, tz0 = (blockIdx.x % ntz) * TZ;
     const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
-    const float lthf = (float)lth;
+    // T' = T/ih^2: threshold and margin scaled to match
+    const float lthf = (float)(lth / (double)ih2);
+    const float tols = (float)(tol / (double)ih2);
 
     // this thread's phi loads: in-plane offsets with periodic wrap in y and z
     int foff[PF];
@@ -183,7 +186,8 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     for (int k = 0; k < IP; ++k)
 #pragma unroll
         for (int i = 0; i < 5; ++i) hx[k][i] = hy[k][i] = hz[k][i] = 0.f;
-    unsigned loc[5] = {0, 0, 0, 0, 0};
+    unsigned long long cls = 0;  // class counts, 16 bits each (at most 2048 cells per thread)
+    unsigned nmarg = 0;
     float pf[PF];
     auto fetch = [&](int s) {
         const int sp = (s % N + N) % N;
@@ -210,17 +214,17 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
 #pragma unroll
             for (int k = 0; k < IP; ++k) {
                 const int o = po[k];
-                hx[k][J] = fd4f(pm4[o], pm3[o], pm1[o], cur[o], ih);
-                hy[k][J] = fd4f(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ], ih);
-                hz[k][J] = fd4f(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2], ih);
+                hx[k][J] = fd4s(pm4[o], pm3[o], pm1[o], cur[o]);
+                hy[k][J] = fd4s(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ]);
+                hz[k][J] = fd4s(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2]);
                 sgy[gq[k]] = hy[k][(J + 3) % 5];
                 sgz[gq[k]] = hz[k][(J + 3) % 5];
             }
         }
         if (s >= x0 + 4 && hkind) {
             const int o = hpo;
-            if (hkind == 1) sgy[hq] = fd4f(pm4[o - 2 * HZ], pm4[o - HZ], pm4[o + HZ], pm4[o + 2 * HZ], ih);
-            sgz[hq] = fd4f(pm4[o - 2], pm4[o - 1], pm4[o + 1], pm4[o + 2], ih);
+            if (hkind == 1) sgy[hq] = fd4s(pm4[o - 2 * HZ], pm4[o - HZ], pm4[o + HZ], pm4[o + 2 * HZ]);
+            sgz[hq] = fd4s(pm4[o - 2], pm4[o - 1], pm4[o + 1], pm4[o + 2]);
         }
         __syncthreads();
         const int c = s - 4;
@@ -228,23 +232,23 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
 #pragma unroll
         for (int k = 0; k < IP; ++k) {
             const int q = gq[k];
-            const float txx = fd4f(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J], ih);
-            const float a10 = fd4f(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J], ih);
-            const float a20 = fd4f(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J], ih);
-            const float tyy = fd4f(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ], ih);
-            const float a21 = fd4f(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ], ih);
-            const float tzz = fd4f(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2], ih);
-            float det, det_err, fro2;
-            const int v = positive_eigs_f(txx - lthf, tyy - lthf, tzz - lthf, a10, a20, a21, det, det_err, fro2);
+            const float txx = fd4s(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J]);
+            const float a10 = fd4s(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J]);
+            const float a20 = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
+            const float tyy = fd4s(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ]);
+            const float a21 = fd4s(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ]);
+            const float tzz = fd4s(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2]);
+            float det_lo, fro2;
+            const int v = positive_eigs_s(txx - lthf, tyy - lthf, tzz - lthf, a10, a20, a21, det_lo, fro2);
             const int lab = 4 - v;
             const size_t cell = (size_t)c * NN * NN + (size_t)coff[k];
             labels[cell] = (int8_t)lab;
-            loc[lab - 1]++;
+            cls += 1ull << (16 * (lab - 1));
             // margin bound min|mu| >= |det B| / ||B||_F^2, taken conservatively
             // (|det| minus its rounding bound); the FP64 eigenvalues settle it
-            const bool maybe_marginal = tol > 0.0 && (fabsf(det) - det_err) < (float)tol * fro2 * 1.0001f;
+            const bool maybe_marginal = tols > 0.f && det_lo < tols * fro2 * 1.0001f;
             if (lam || maybe_marginal)
-                loc[4] += eig_margin(txx, tyy, tzz, a10, a20, a21, lth, tol, maybe_marginal, lam, cell, n3);
+                nmarg += eig_margin(txx, tyy, tzz, a10, a20, a21, ih2, lth, tol, maybe_marginal, lam, cell, n3);
         }
     };
     using I0 = std::integral_constant<int, 0>;
@@ -262,8 +266,11 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         if (s + 4 < send) step(I4{}, s + 4);
     }
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
-        if (loc[i]) atomicAdd(&scnt[i], loc[i]);
+    for (int i = 0; i < 4; ++i) {
+        const unsigned ci = (unsigned)((cls >> (16 * i)) & 0xffffull);
+        if (ci) atomicAdd(&scnt[i], ci);
+    }
+    if (nmarg) atomicAdd(&scnt[4], nmarg);
     __syncthreads();
     if (tid < 5 && scnt[tid]) atomicAdd(&stats[tid], (unsigned long long)scnt[tid]);
 }
@@ -308,10 +315,11 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
     const int segLen = (N + nseg - 1) / nseg;
     nseg = (N + segLen - 1) / segLen;
     dim3 grid((unsigned)tiles, (unsigned)nseg);
-    const float ihf = (float)(1.0 / (12.0 * (L / (double)N)));
+    const double ih = 1.0 / (12.0 * (L / (double)N));
+    const float ih2 = (float)(ih * ih);
     cm_status s;
     if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
-    return launch(ctx, "tweb_fused", k_tweb_fused<TY, TZ, NT>, grid, dim3(NT), lds, phi, N, segLen, ihf, lth, tol,
+    return launch(ctx, "tweb_fused", k_tweb_fused<TY, TZ, NT>, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol,
                   labels, lam, stats);
 }
 

@@ -63,6 +63,12 @@ __device__ __forceinline__ float fd4f(float fll, float fl, float fr, float frr, 
     return ((fll - frr) + 8.0f * (fr - fl)) * ih;
 }
 
+// The stencil without the 1/(12 d) factor: the fused T-web works with
+// T' = T (12 d)^2, whose inertia relative to lambda_th (12 d)^2 is that of T.
+__device__ __forceinline__ float fd4s(float fll, float fl, float fr, float frr) {
+    return (fll - frr) + 8.0f * (fr - fl);
+}
+
 }  // namespace cm
 
 namespace cm {
@@ -126,6 +132,34 @@ __device__ __forceinline__ int positive_eigs_f(float b00, float b11, float b22, 
     ch += ((-e1 > 0.f) != (e2 > 0.f));
     ch += ((e2 > 0.f) != (-e3 > 0.f));
     return ch;
+}
+
+
+// Same count with one scale for all rounding bounds: s = sum of |entries|
+// bounds the sums of absolute values of the terms of e1, e2, e3 by s, s^2, s^3,
+// so the float signs are certain when |e1| > 8u s, |e2| > 16u s^2 and
+// |e3| > 24u s^3 (u = 2^-24; the factors cover every rounding of the
+// evaluation with room to spare).  Otherwise FP64 decides.  Also returns |e3|
+// minus its bound (a lower bound of |det B|) and ||B||_F^2.
+__device__ __forceinline__ int positive_eigs_s(float b00, float b11, float b22, float a10, float a20, float a21,
+                                               float &det_lo, float &fro2) {
+    const float e1 = b00 + b11 + b22;
+    const float q10 = a10 * a10, q20 = a20 * a20, q21 = a21 * a21;
+    const float e2 = (b00 * b11 + b00 * b22 + b11 * b22) - (q10 + q20 + q21);
+    const float c0 = b11 * b22 - q21, c1 = a10 * b22 - a21 * a20, c2 = a10 * a21 - b11 * a20;
+    const float e3 = b00 * c0 - a10 * c1 + a20 * c2;
+    const float sc = (fabsf(b00) + fabsf(b11) + fabsf(b22)) + (fabsf(a10) + fabsf(a20) + fabsf(a21));
+    const float u = 5.9604645e-8f;
+    const float t1 = 8.0f * u * sc, t2 = 2.0f * t1 * sc, t3 = 1.5f * t2 * sc;
+    const float ae3 = fabsf(e3);
+    det_lo = ae3 - t3;
+    fro2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * (q10 + q20 + q21);
+    if (fabsf(e1) <= t1 || fabsf(e2) <= t2 || ae3 <= t3 || !(sc < 1e12f)) {
+        double ad, f2;
+        return positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+    }
+    // no zero roots: sign changes of 1, -e1, e2, -e3
+    return (e1 > 0.f) + ((e1 < 0.f) != (e2 > 0.f)) + ((e2 > 0.f) != (e3 < 0.f));
 }
 
 }  // namespace cm

@@ -86,3 +86,27 @@ def test_c1_plain_pipeline_f64(ctx, orc):
     r0, _, _, rn = orc.pk_shells(rdk, o)
     assert np.array_equal(nm, rn)
     assert np.max(np.abs(p0 - r0) / (np.abs(r0) + V / nobj)) < 1e-10
+
+
+def test_measure_fused_record_overflow_retry(ctx, orc):
+    """Every particle on a tile corner (8 records each with TSC on 16^3 tiles)
+    exceeds the fused path's record estimate: the measurement is repeated with
+    the exact record count and must equal the oracle and the plain path."""
+    import cosmicatlas_amd as cm
+    N, L = 64, 256.0
+    d = L / N
+    rng = np.random.default_rng(5)
+    # cells with every coordinate == 0 mod 16 (tile corners), jittered inside the cell
+    corners = np.array([(i, j, k) for i in range(0, N, 16) for j in range(0, N, 16) for k in range(0, N, 16)])
+    cells = corners[rng.integers(0, len(corners), 60000)]
+    pos = ((cells + rng.uniform(0.05, 0.95, cells.shape)) * d).astype(np.float32)
+    b = cm.binning(N, L, ndel=1, DeltaKmin=0.0)
+    posd = torch.from_numpy(pos).cuda()
+    fz = ctx.measure(posd, N, L, 2, b["nshells"], b["kmin"], b["dk"], fused=True)
+    pl = ctx.measure(posd, N, L, 2, b["nshells"], b["kmin"], b["dk"], fused=False)
+    p0, _, _, nm, _, nobj, SN = _oracle_pk(orc, pos, N, L, 2, b)
+    assert np.array_equal(fz["nmodes"], nm)
+    assert fz["n_obj"] == pytest.approx(nobj, rel=1e-6)
+    denom = np.abs(p0) + SN
+    assert np.max(np.abs(fz["pk0"] - p0) / denom) < 1e-4
+    assert np.max(np.abs(fz["pk0"] - pl["pk0"]) / denom) < 1e-4
