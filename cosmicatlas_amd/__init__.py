@@ -220,6 +220,25 @@ class Context:
                                               C.byref(st)))
         return labels, lam, {"counts": list(st.counts), "n_marginal": st.n_marginal}
 
+    # ------------------------------------------------------- synthetic mocks
+    def zeldovich(self, NL, L, A=1e4, ns=-1.5, seed=1, out=None):
+        """NL^3 Zel'dovich particles (float32, (NL^3, 3), device) from a seeded
+        fixed-amplitude Gaussian field with P(k) = A k^ns."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty((NL ** 3, 3), dtype=torch.float32, device=f"cuda:{self.device}")
+        o = _abi.ZeldovichOpts(int(NL), float(L), float(A), float(ns), int(seed))
+        self._check(self.lib.cm_zeldovich(self.h, C.byref(o), _ptr(out)))
+        return out
+
+    def uniform(self, n, L, seed=1, out=None):
+        """n uniform positions in [0, L)^3 (float32, (n, 3), device)."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty((n, 3), dtype=torch.float32, device=f"cuda:{self.device}")
+        self._check(self.lib.cm_uniform(self.h, int(n), float(L), int(seed), _ptr(out)))
+        return out
+
     # ---------------------------------------------------------- whole mock
     def measure(self, pos, N, L, scheme, nshells, kmin, dk_bin, weights=None, mas_correction=True,
                 sn_correction=True, float_shell_rule=False, do_tweb=False, R_smooth=0.0,

@@ -90,6 +90,16 @@ class MeasureInfo(C.Structure):
     ]
 
 
+class ZeldovichOpts(C.Structure):
+    _fields_ = [
+        ("NL", C.c_int),
+        ("L", C.c_double),
+        ("A", C.c_double),
+        ("ns", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
 _LIB = None
 
 
@@ -124,6 +134,8 @@ def _declare(L):
         "cm_tweb_from_phi": (st, [P, C.POINTER(TwebOpts), P, P, P, C.POINTER(TwebStats)]),
         "cm_measure": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, P, P, P, P,
                             C.POINTER(MeasureInfo)]),
+        "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),
+        "cm_uniform": (st, [P, C.c_size_t, C.c_double, C.c_uint64, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

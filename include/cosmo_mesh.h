@@ -217,6 +217,27 @@ cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, con
                      size_t n, double *pk0, double *pk2, double *pk4, int64_t *nmodes,
                      int8_t *labels, cm_measure_info *info);
 
+/* ----------------------------------------------------------- synthetic mocks */
+typedef struct {
+    int NL;          /* Lagrangian grid per side (an FFT size) */
+    double L;        /* box side */
+    double A, ns;    /* P(k) = A k^ns */
+    uint64_t seed;
+} cm_zeldovich_opts;
+
+/* GPU replacement of create_GARFIELD_FIXED_AMP (Source/NumericalMethods.cpp:
+ * 2147-2189) followed by the first-order LPT displacement
+ * (LPT::get_displacement, Source/LPT.cpp:285): fixed-amplitude Gaussian field
+ * with Philox4x32-10 phases keyed by (seed, mode index), Psi(k) = i k delta/k^2,
+ * particle (i,j,k) at row (i NL + j) NL + k of pos (device, NL^3 x 3 float32,
+ * AoS), positions wrapped into [0, L). */
+cm_status cm_zeldovich(cm_ctx *ctx, const cm_zeldovich_opts *o, float *pos);
+
+/* n uniform positions in [0, L)^3 (device, n x 3 float32 AoS), Philox4x32-10
+ * of counter (i, 0, 1, 0) and key seed: the Poisson randoms of an FKP
+ * catalogue (no reference generator; Catalog reads them from files). */
+cm_status cm_uniform(cm_ctx *ctx, size_t n, double L, uint64_t seed, float *pos);
+
 #ifdef __cplusplus
 }
 #endif

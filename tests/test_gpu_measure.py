@@ -30,7 +30,7 @@ def _oracle_pk(orc, pos, N, L, scheme, b, mas_correction=True):
 @pytest.mark.parametrize("scheme", [0, 1, 2])
 def test_measure_zeldovich_pk(ctx, orc, fused, scheme):
     import cosmicatlas_amd as cm
-    from cosmicatlas_amd import synth
+    from oracle import zeldovich as synth
     NL, N, L = 32, 64, 250.0
     pos = synth.zeldovich(NL, L, A=2e3, seed=11)
     b = cm.binning(N, L, ndel=1, DeltaKmin=0.0)
@@ -46,7 +46,7 @@ def test_measure_zeldovich_pk(ctx, orc, fused, scheme):
 
 def test_measure_tweb_fused_equals_plain_and_oracle(ctx, orc):
     import cosmicatlas_amd as cm
-    from cosmicatlas_amd import synth
+    from oracle import zeldovich as synth
     NL, N, L, R = 32, 32, 200.0, 12.0
     pos = synth.zeldovich(NL, L, A=2e3, seed=3)
     b = cm.binning(N, L)
