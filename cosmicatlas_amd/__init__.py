@@ -264,6 +264,116 @@ class Context:
                 "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal}
 
 
+def _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=True, sn_correction=True,
+                  float_shell_rule=False, do_tweb=False, R_smooth=0.0, lambda_th=0.0, margin_tol=0.0, fused=True,
+                  interlace_shift=0.0):
+    md = _abi.MasDesc(N, float(L), (C.c_double * 3)(0.0, 0.0, 0.0), CM_AOS, CM_F32, _scheme(scheme),
+                      float(interlace_shift), CM_MAS_AUTO)
+    return _abi.MeasureOpts(md, nshells, float(kmin), float(dk_bin), int(mas_correction), int(sn_correction),
+                            int(float_shell_rule), int(do_tweb), float(R_smooth), float(lambda_th),
+                            float(margin_tol), int(fused))
+
+
+class Ticket:
+    """A submitted request and its host/device result buffers."""
+
+    def __init__(self, server, req, keep, out):
+        self.server, self.req, self._keep, self.out = server, req, keep, out
+        self.id = None
+
+    def result(self, timeout=0.0):
+        lat = C.c_double()
+        err = C.create_string_buffer(512)
+        st = self.server.lib.cm_server_wait(self.server.h, self.id, float(timeout), C.byref(lat), err, 512)
+        if st != 0:
+            raise CosmoError(st, err.value.decode() or self.server.lib.cm_status_string(st).decode())
+        info = self.out["_info"]
+        self.out.update({"n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
+                         "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal,
+                         "latency_ms": lat.value})
+        return self.out
+
+
+class Server:
+    """Continuous-batching measurement server (cm_server_*): `slots` requests
+    run concurrently, each on its own stream, admitted by HBM footprint."""
+
+    def __init__(self, device=0, slots=2, hbm_budget=0):
+        self.lib = _abi.load()
+        self.device = device
+        h = C.c_void_p()
+        st = self.lib.cm_server_create(device, slots, hbm_budget, C.byref(h))
+        if st != 0:
+            raise CosmoError(st, f"cm_server_create failed: {self.lib.cm_status_string(st).decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def request(self, N, L, scheme, nshells, kmin, dk_bin, pos=None, gen=None, labels=None, **kw):
+        """Build a request: device particles `pos` ((n,3) float32) or
+        gen=dict(NL=, A=, ns=, seed=) for a GPU-generated mock in the same box."""
+        torch = _torch()
+        o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, **kw)
+        out = {"pk0": np.zeros(nshells), "pk2": np.zeros(nshells), "pk4": np.zeros(nshells),
+               "nmodes": np.zeros(nshells, np.int64), "_info": _abi.MeasureInfo()}
+        if o.do_tweb and labels is None:
+            labels = torch.empty((N, N, N), dtype=torch.int8, device=f"cuda:{self.device}")
+        out["labels"] = labels
+        g = _abi.ZeldovichOpts(0, float(L), 0.0, 0.0, 0)
+        if gen is not None:
+            g = _abi.ZeldovichOpts(int(gen["NL"]), float(L), float(gen.get("A", 1e4)), float(gen.get("ns", -1.5)),
+                                   int(gen.get("seed", 1)))
+        req = _abi.Request(o, _ptr(pos), None, 0 if pos is None else int(pos.shape[0]), g,
+                           out["pk0"].ctypes.data_as(C.c_void_p), out["pk2"].ctypes.data_as(C.c_void_p),
+                           out["pk4"].ctypes.data_as(C.c_void_p), out["nmodes"].ctypes.data_as(C.c_void_p),
+                           _ptr(labels), C.pointer(out["_info"]))
+        return Ticket(self, req, (pos, labels), out)
+
+    def footprint(self, ticket):
+        return int(self.lib.cm_request_footprint(C.byref(ticket.req)))
+
+    def submit(self, ticket):
+        t = C.c_int64()
+        st = self.lib.cm_server_submit(self.h, C.byref(ticket.req), C.byref(t))
+        if st != 0:
+            raise CosmoError(st, f"cm_server_submit: {self.lib.cm_status_string(st).decode()}")
+        ticket.id = t.value
+        return ticket
+
+    def batch(self, tickets):
+        """Run tickets with cm_measure_batch; returns their results."""
+        n = len(tickets)
+        arr = (_abi.Request * n)(*[t.req for t in tickets])
+        sts = np.zeros(n, np.int32)
+        lat = np.zeros(n)
+        self.lib.cm_measure_batch(self.h, arr, n, sts.ctypes.data_as(C.c_void_p), lat.ctypes.data_as(C.c_void_p))
+        res = []
+        for t, s_, l_ in zip(tickets, sts, lat):
+            if s_ != 0:
+                raise CosmoError(int(s_), f"batch request failed: {self.lib.cm_status_string(int(s_)).decode()}")
+            info = t.out["_info"]
+            t.out.update({"n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
+                          "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal,
+                          "latency_ms": float(l_)})
+            res.append(t.out)
+        return res
+
+
 def binning(N, L, ndel=1, DeltaKmin=0.0):
     """Params::derived_pars binning (Source/Params.cpp:4091-4104): k_f=2pi/L,
     kmin = DeltaKmin*k_f, dk = ndel*k_f, N_shells = N/ndel/2."""

@@ -100,6 +100,22 @@ class ZeldovichOpts(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("opts", MeasureOpts),
+        ("pos", C.c_void_p),
+        ("w", C.c_void_p),
+        ("n", C.c_size_t),
+        ("gen", ZeldovichOpts),
+        ("pk0", C.c_void_p),
+        ("pk2", C.c_void_p),
+        ("pk4", C.c_void_p),
+        ("nmodes", C.c_void_p),
+        ("labels", C.c_void_p),
+        ("info", C.POINTER(MeasureInfo)),
+    ]
+
+
 _LIB = None
 
 
@@ -135,6 +151,12 @@ def _declare(L):
         "cm_measure": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, P, P, P, P,
                             C.POINTER(MeasureInfo)]),
         "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),
+        "cm_request_footprint": (C.c_size_t, [C.POINTER(Request)]),
+        "cm_server_create": (st, [C.c_int, C.c_int, C.c_size_t, C.POINTER(P)]),
+        "cm_server_submit": (st, [P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+        "cm_server_wait": (st, [P, C.c_int64, C.c_double, C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
+        "cm_server_destroy": (st, [P]),
+        "cm_measure_batch": (st, [P, C.POINTER(Request), C.c_int, P, P]),
         "cm_uniform": (st, [P, C.c_size_t, C.c_double, C.c_uint64, P]),
     }
     for name, (res, args) in sig.items():

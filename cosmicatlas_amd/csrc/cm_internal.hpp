@@ -121,6 +121,10 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
 cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
                         double *host_out);
 
+// Whole-mock measurement with argument checks (cm_measure's body).
+cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n, double *pk0,
+                      double *pk2, double *pk4, int64_t *nmodes, int8_t *labels, cm_measure_info *info);
+
 // P(k) helpers (pk.hip)
 void pk_tables(int N, int scheme, int mas_correction, std::vector<double> &tab);
 cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const void *dk, double **sums,

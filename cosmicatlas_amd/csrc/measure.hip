@@ -187,13 +187,21 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
 
 using namespace cm;
 
-extern "C" cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
-                                double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
-                                cm_measure_info *info) {
+namespace cm {
+cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n, double *pk0,
+                      double *pk2, double *pk4, int64_t *nmodes, int8_t *labels, cm_measure_info *info) {
     if (!ctx) return CM_ERR_INVALID;
     if (!o || (!pos && n)) return fail(ctx, CM_ERR_INVALID, "cm_measure: null pointer");
     if (!fft_size_ok(o->mas.N)) return fail(ctx, CM_ERR_INVALID, "cm_measure: N=%d not supported", o->mas.N);
     if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure: float32 positions only");
     if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure: binning");
+    if (o->do_tweb && !labels) return fail(ctx, CM_ERR_INVALID, "cm_measure: do_tweb needs a labels buffer");
     return measure_impl(ctx, o, pos, w, n, pk0, pk2, pk4, nmodes, labels, info);
+}
+}  // namespace cm
+
+extern "C" cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                                double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
+                                cm_measure_info *info) {
+    return cm::measure_run(ctx, o, pos, w, n, pk0, pk2, pk4, nmodes, labels, info);
 }

@@ -238,6 +238,56 @@ cm_status cm_zeldovich(cm_ctx *ctx, const cm_zeldovich_opts *o, float *pos);
  * catalogue (no reference generator; Catalog reads them from files). */
 cm_status cm_uniform(cm_ctx *ctx, size_t n, double L, uint64_t seed, float *pos);
 
+/* ------------------------------------------------- batching and serving */
+/* One measurement request: the options of cm_measure, the input (device
+ * particles, or pos == NULL and gen.NL > 0 for a seeded Zel'dovich mock made on
+ * the GPU), and the outputs (host pk0/pk2/pk4/nmodes[nshells] and info, device
+ * labels when opts.do_tweb).  The buffers must stay valid until the request
+ * completes.  No counterpart in the reference, whose drivers
+ * (PowerSpectrumF::compute_power_spectrum, Source/PowerSpectrumF.cpp:741-1363;
+ * Bam, Source/Bam.cpp:2210-2283) measure one catalogue per call. */
+typedef struct {
+    cm_measure_opts opts;
+    const void *pos;
+    const void *w;
+    size_t n;
+    cm_zeldovich_opts gen;
+    double *pk0, *pk2, *pk4;
+    int64_t *nmodes;
+    int8_t *labels;
+    cm_measure_info *info;
+} cm_request;
+
+typedef struct cm_server cm_server;
+
+/* Device bytes a request holds while it runs (the admission footprint). */
+size_t cm_request_footprint(const cm_request *r);
+
+/* Continuous-batching server on `device`: `slots` (1..16) concurrent requests,
+ * each on its own stream with its own resident meshes; a queued request starts
+ * when a slot is free and its footprint fits in the HBM budget beside the
+ * running ones (hbm_budget 0: 90% of the device). */
+cm_status cm_server_create(int device, int slots, size_t hbm_budget, cm_server **out);
+
+/* Queue a request (thread-safe, returns at once).  CM_ERR_BUDGET if the
+ * request alone exceeds the budget. */
+cm_status cm_server_submit(cm_server *s, const cm_request *r, int64_t *ticket);
+
+/* Wait for a ticket (timeout_s <= 0: no limit; CM_ERR_TIMEOUT leaves it
+ * pending).  Returns the request's status; latency_ms = completion - submission;
+ * err receives the request's error message.  A ticket is waited for once. */
+cm_status cm_server_wait(cm_server *s, int64_t ticket, double timeout_s, double *latency_ms, char *err,
+                         size_t errlen);
+
+/* Cancel queued requests (their waits return CM_ERR_CANCELLED), finish the
+ * running ones, free the slots. */
+cm_status cm_server_destroy(cm_server *s);
+
+/* Submit n requests and wait for all: per-request status and latency (either
+ * array may be NULL); returns the first failure or CM_OK. */
+cm_status cm_measure_batch(cm_server *s, const cm_request *reqs, int n, cm_status *statuses,
+                           double *latency_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,138 @@
+"""Batching and serving (cm_measure_batch, cm_server_*): every request gives
+the same result in a mix of concurrent requests as alone, admission by HBM
+footprint, error and cancellation paths."""
+import threading
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import zeldovich as zref
+
+pytestmark = pytest.mark.gpu
+
+# (N, L, scheme, ndel, do_tweb, input): input ("gen", NL, seed) or ("uniform", n, seed)
+MIX = [
+    (32, 200.0, "TSC", 1, True, ("gen", 16, 1)),
+    (48, 300.0, "CIC", 1, False, ("gen", 24, 2)),
+    (64, 250.0, "NGP", 2, True, ("uniform", 40000, 3)),
+    (64, 500.0, "TSC", 1, True, ("gen", 32, 4)),
+    (24, 120.0, "PCS", 1, False, ("uniform", 5000, 5)),
+    (96, 400.0, "TSC", 3, True, ("gen", 48, 6)),
+]
+
+
+def _kw(N, L, scheme, ndel, do_tweb):
+    import cosmicatlas_amd as cm
+    b = cm.binning(N, L, ndel=ndel)
+    kw = dict(do_tweb=do_tweb, R_smooth=2.0 * L / N if do_tweb else 0.0, margin_tol=1e-6)
+    return b, kw
+
+
+def _alone(ctx, spec):
+    N, L, scheme, ndel, do_tweb, inp = spec
+    b, kw = _kw(N, L, scheme, ndel, do_tweb)
+    if inp[0] == "gen":
+        pos = ctx.zeldovich(inp[1], L, A=1e4, ns=-1.5, seed=inp[2])
+    else:
+        pos = torch.from_numpy(zref.uniform(inp[1], L, inp[2])).cuda()
+    return ctx.measure(pos, N, L, scheme, b["nshells"], b["kmin"], b["dk"], **kw)
+
+
+def _ticket(srv, spec):
+    N, L, scheme, ndel, do_tweb, inp = spec
+    b, kw = _kw(N, L, scheme, ndel, do_tweb)
+    if inp[0] == "gen":
+        return srv.request(N, L, scheme, b["nshells"], b["kmin"], b["dk"],
+                           gen=dict(NL=inp[1], A=1e4, ns=-1.5, seed=inp[2]), **kw)
+    pos = torch.from_numpy(zref.uniform(inp[1], L, inp[2])).cuda()
+    return srv.request(N, L, scheme, b["nshells"], b["kmin"], b["dk"], pos=pos, **kw)
+
+
+def _same(a, b, spec):
+    N, L = spec[0], spec[1]
+    assert np.array_equal(a["nmodes"], b["nmodes"])
+    assert a["n_obj"] == b["n_obj"]
+    sn = L ** 3 / a["n_obj"]
+    denom = np.abs(a["pk0"]) + sn
+    assert np.max(np.abs(a["pk0"] - b["pk0"]) / denom) < 1e-9
+    assert np.max(np.abs(a["pk2"] - b["pk2"]) / denom) < 1e-9
+    if spec[4]:
+        assert torch.equal(a["labels"], b["labels"])
+        assert a["tweb_counts"] == b["tweb_counts"]
+        assert sum(a["tweb_counts"]) == N ** 3
+
+
+@pytest.fixture(scope="module")
+def alone():
+    import cosmicatlas_amd as cm
+    with cm.Context(0) as ctx:
+        res = [_alone(ctx, s) for s in MIX]
+        torch.cuda.synchronize()
+    return res
+
+
+def test_batch_equals_alone(alone):
+    import cosmicatlas_amd as cm
+    with cm.Server(0, slots=3) as srv:
+        tickets = [_ticket(srv, s) for s in MIX * 2]
+        out = srv.batch(tickets)
+    for i, r in enumerate(out):
+        _same(r, alone[i % len(MIX)], MIX[i % len(MIX)])
+        assert r["latency_ms"] > 0
+
+
+def test_continuous_arrivals_equal_alone(alone):
+    """Requests arrive from another thread while earlier ones are running."""
+    import cosmicatlas_amd as cm
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(MIX) * 3) % len(MIX)
+    with cm.Server(0, slots=2) as srv:
+        tickets = [None] * len(order)
+
+        def producer():
+            for k, i in enumerate(order):
+                tickets[k] = srv.submit(_ticket(srv, MIX[i]))
+                time.sleep(float(rng.exponential(0.002)))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        res = [t.result(timeout=60.0) for t in tickets]
+    for k, i in enumerate(order):
+        _same(res[k], alone[i], MIX[i])
+
+
+def test_admission_by_footprint(alone):
+    import cosmicatlas_amd as cm
+    with cm.Server(0, slots=3, hbm_budget=1 << 30) as srv:
+        t = _ticket(srv, MIX[3])
+        fp = srv.footprint(t)
+        assert 0 < fp < (1 << 30)
+        # three slots but room for one request at a time: all still complete
+        srv.close()
+    with cm.Server(0, slots=3, hbm_budget=int(fp * 1.5)) as srv:
+        tickets = [srv.submit(_ticket(srv, MIX[3])) for _ in range(4)]
+        for t in tickets:
+            _same(t.result(timeout=60.0), alone[3], MIX[3])
+        big = _ticket(srv, (512, 1000.0, "TSC", 2, True, ("gen", 256, 1)))
+        with pytest.raises(cm.CosmoError) as e:
+            srv.submit(big)
+        assert e.value.status == 5  # CM_ERR_BUDGET
+
+
+def test_request_errors_and_cancel():
+    import cosmicatlas_amd as cm
+    srv = cm.Server(0, slots=1)
+    b = cm.binning(40, 100.0)
+    bad = srv.submit(srv.request(40, 100.0, "TSC", b["nshells"], b["kmin"], b["dk"], gen=dict(NL=16, seed=1)))
+    with pytest.raises(cm.CosmoError) as e:
+        bad.result(timeout=30.0)
+    assert "N=40" in str(e.value)
+    # a long queue, then destroy: queued requests are cancelled, running ones finish
+    b = cm.binning(64, 100.0)
+    ts = [srv.submit(srv.request(64, 100.0, "TSC", b["nshells"], b["kmin"], b["dk"], gen=dict(NL=32, seed=i)))
+          for i in range(20)]
+    srv.close()
+    assert ts[0].server.h is None
