@@ -148,6 +148,84 @@ def cpu_baseline(pos, quick=False):
             "stage_seconds_sample": {k: round(v, 4) for k, v in t.items()}}
 
 
+def served(cm, posd, b, slots, nreq, rate=None, seed=0):
+    """512^3 mocks through the continuous-batching server: `nreq` requests on
+    the resident particles, all at once (rate None) or with Poisson arrivals
+    at `rate` per second.  Returns mocks/s and latency percentiles (ms)."""
+    import threading
+    import torch
+    rng = np.random.default_rng(seed)
+    kw = dict(do_tweb=True, R_smooth=R_SMOOTH, lambda_th=LAMBDA_TH, margin_tol=1e-6)
+    with cm.Server(0, slots=slots) as srv:
+        labels = [torch.empty((N, N, N), dtype=torch.int8, device="cuda") for _ in range(slots + 1)]
+        mk = lambda i: srv.request(N, L, "TSC", b["nshells"], b["kmin"], b["dk"], pos=posd,
+                                   labels=labels[i % len(labels)], **kw)
+        for t in [srv.submit(mk(i)) for i in range(2 * slots)]:  # warm the slots' buffers
+            t.result()
+        torch.cuda.synchronize()
+        tickets = [None] * nreq
+        t0 = time.perf_counter()
+        if rate is None:
+            for i in range(nreq):
+                tickets[i] = srv.submit(mk(i))
+        else:
+            def producer():
+                nxt = time.perf_counter()
+                for i in range(nreq):
+                    nxt += float(rng.exponential(1.0 / rate))
+                    d = nxt - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    tickets[i] = srv.submit(mk(i))
+            th = threading.Thread(target=producer)
+            th.start()
+            th.join()
+        lat = np.array([t.result()["latency_ms"] for t in tickets])
+        T = time.perf_counter() - t0
+    return {"value": round(nreq / T, 2), "unit": "mocks/s", "slots": slots, "requests": nreq,
+            "arrivals": "all at t=0" if rate is None else f"Poisson, {rate:.0f}/s",
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+
+def batched(cm, nmocks=64, slots=3, seed=7):
+    """64 uneven mocks (N in {128,192,256,384,512}, NL^3 particles with NL in
+    {48..384} ~ log-uniform 1e5..5e7, L in [250,2000], NGP/CIC/TSC, 64 shells,
+    T-web labels), each generated on the GPU from its own seed, through
+    cm_measure_batch.  N=192 uses ndel=1 and keeps 64 of its 96 shells."""
+    import torch
+    rng = np.random.default_rng(seed)
+    Ns = [128, 192, 256, 384, 512]
+    NLs = [48, 64, 96, 128, 192, 256, 384]
+    specs = []
+    for i in range(nmocks):
+        Nm = int(rng.choice(Ns))
+        NLm = int(rng.choice(NLs))
+        Lm = float(rng.uniform(250.0, 2000.0))
+        sch = ["NGP", "CIC", "TSC"][int(rng.integers(0, 3))]
+        ndel = max(1, Nm // 128) if Nm != 192 else 1
+        specs.append((Nm, NLm, Lm, sch, ndel, 1000 + i))
+    npart = sum(s[1] ** 3 for s in specs)
+    with cm.Server(0, slots=slots) as srv:
+        def tickets():
+            out = []
+            for (Nm, NLm, Lm, sch, ndel, sd) in specs:
+                bm = cm.binning(Nm, Lm, ndel=ndel)
+                out.append(srv.request(Nm, Lm, sch, 64, bm["kmin"], bm["dk"], gen=dict(NL=NLm, seed=sd),
+                                       do_tweb=True, R_smooth=2.0 * Lm / Nm, margin_tol=1e-6))
+            return out
+        srv.batch(tickets()[:slots * 2])  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = srv.batch(tickets())
+        T = time.perf_counter() - t0
+    lat = np.array([r["latency_ms"] for r in res])
+    return {"value": round(nmocks / T, 2), "unit": "mocks/s", "mocks": nmocks, "slots": slots,
+            "particles": int(npart), "seconds": round(T, 3),
+            "workload": "64 uneven mocks generated on the GPU (N 128-512, NL^3 particles 1.1e5-5.7e7, "
+                        "NGP/CIC/TSC, 64 shells, T-web)",
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+
 def run_steps(ctx, posd, b, fused, steps, warmup, labels):
     import torch
     kw = dict(do_tweb=True, R_smooth=R_SMOOTH, lambda_th=LAMBDA_TH, margin_tol=1e-6, fused=fused, labels=labels)
@@ -177,6 +255,7 @@ def main():
                     help="time only the optimised path (no plain path, FP64 check or CPU leg): for profiler runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy a 512 MiB device buffer once before the steps (reference for PMC byte counters)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served and batched legs")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="convert two rocprofv3 --pmc counter files into profiles/traffic.json and exit")
     args = ap.parse_args()
@@ -279,6 +358,12 @@ def main():
     pk_err_sig = float(np.max(np.abs(out["pk0"][sig] - r0[sig]) / r0[sig])) if sig.any() else None
     modes_equal = bool(np.array_equal(rn, out["nmodes"]))
 
+    serve = None
+    if not args.no_serve:
+        closed = served(cm, posd, b, slots=2, nreq=40)
+        poisson = served(cm, posd, b, slots=2, nreq=60, rate=0.8 * closed["value"], seed=1)
+        serve = {"closed_loop": closed, "poisson": poisson, "batched_c4": batched(cm)}
+
     cpu = None if args.no_cpu else cpu_baseline(pos, quick=args.quick_cpu)
 
     res = {
@@ -314,6 +399,7 @@ def main():
         "tweb_marginal": out["tweb_marginal"],
         "kernels_ms_per_step": {k: round(v["mean_ms"] * v["launches_per_step"], 4) for k, v in ktimes.items()},
         "roofline": roof,
+        "served": serve,
         "cpu_baseline": cpu,
         "input_gen_ms": round(1e3 * gen_s, 2),
     }
