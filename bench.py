@@ -226,6 +226,64 @@ def batched(cm, nmocks=64, slots=3, seed=7):
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
 
+def large(cm, steps=3):
+    """Large single mock: 768^3 Zel'dovich particles (P = 1e4 k^-1.5, L = 2000)
+    plus 20% uniform randoms (90,596,966), all float32 and generated on the GPU,
+    interlaced TSC onto 1024^3, 256 shells (ndel = 2).  Reports the time per
+    measurement and the per-shell float32-vs-FP64 error of the raw power (the
+    FP64 leg: two FP64 meshes, FP64 FFTs, combination and FP64 shells)."""
+    import torch
+    NLl, Nl, Ll = 768, 1024, 2000.0
+    n1 = NLl ** 3
+    n2 = n1 // 5
+    ctx = cm.Context(0)
+    pos = torch.empty((n1 + n2, 3), dtype=torch.float32, device="cuda")
+    ctx.zeldovich(NLl, Ll, A=1e4, ns=-1.5, seed=11, out=pos[:n1])
+    ctx.uniform(n2, Ll, seed=12, out=pos[n1:])
+    torch.cuda.synchronize()
+    bl = cm.binning(Nl, Ll, ndel=2)
+    kw = dict(interlace=True)
+    out = ctx.measure(pos, Nl, Ll, "TSC", bl["nshells"], bl["kmin"], bl["dk"], **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ctx.measure(pos, Nl, Ll, "TSC", bl["nshells"], bl["kmin"], bl["dk"], **kw)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    raw32 = ctx.measure(pos, Nl, Ll, "TSC", bl["nshells"], bl["kmin"], bl["dk"], sn_correction=False, **kw)
+    # FP64: both meshes, FFTs and the interlaced combination in double
+    nobj = float(n1 + n2)
+    V = Ll ** 3
+    specs = []
+    for sh in (0.0, 0.5):
+        m = ctx.mesh_assign(pos, Nl, Ll, "TSC", interlace_shift=sh, mesh_dtype=torch.float64)
+        specs.append(ctx.fft_r2c(m, subtract=nobj / Nl ** 3))
+        del m
+    j = torch.arange(Nl, device="cuda", dtype=torch.float64)
+    sj = torch.where(j <= Nl // 2, j, j - Nl)
+    kz = torch.arange(Nl // 2 + 1, device="cuda", dtype=torch.float64)
+    a = specs[1]
+    for ax, v in ((0, sj), (1, sj), (2, kz)):
+        shape = [1, 1, 1]
+        shape[ax] = -1
+        a.mul_(torch.polar(torch.ones_like(v), torch.pi * v / Nl).view(shape))
+    specs[0].add_(a).mul_(0.5)
+    del a, specs[1]
+    raw64, _, _, nm64 = ctx.pk_shells(specs[0], Nl, Ll, bl["nshells"], bl["kmin"], bl["dk"], "TSC", 0.0, nobj ** 2 / V)
+    del specs
+    torch.cuda.empty_cache()
+    ok = nm64 > 0
+    err = np.abs(raw32["pk0"][ok] - raw64[ok]) / np.abs(raw64[ok])
+    ctx.close()
+    return {"metric": "interlaced TSC 1024^3 P(k) measurement", "value": round(1e3 / ms, 3), "unit": "mocks/s",
+            "ms_per_mock": round(ms, 2), "particles": n1 + n2,
+            "config": {"workload": "768^3 Zel'dovich (P=1e4 k^-1.5, L=2000) + 20% uniform randoms, float32, "
+                                   "generated on the GPU; interlaced TSC 1024^3; 256 shells"},
+            "pk_rel_err_fp32_vs_fp64_max": float(err.max()), "pk_rel_err_fp32_vs_fp64_median": float(np.median(err)),
+            "pk_rel_err_per_shell": [float(f"{e:.3g}") for e in err],
+            "nmodes_equal_fp64": bool(np.array_equal(nm64, raw32["nmodes"]))}
+
+
 def run_steps(ctx, posd, b, fused, steps, warmup, labels):
     import torch
     kw = dict(do_tweb=True, R_smooth=R_SMOOTH, lambda_th=LAMBDA_TH, margin_tol=1e-6, fused=fused, labels=labels)
@@ -256,6 +314,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy a 512 MiB device buffer once before the steps (reference for PMC byte counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served and batched legs")
+    ap.add_argument("--large", action="store_true",
+                    help="run the large interlaced 1024^3 configuration instead (one JSON line)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="convert two rocprofv3 --pmc counter files into profiles/traffic.json and exit")
     args = ap.parse_args()
@@ -271,6 +331,9 @@ def main():
     import torch
     import cosmicatlas_amd as cm
 
+    if args.large:
+        print(json.dumps(large(cm)))
+        return
     dev = torch.device("cuda:0")
     ctx = cm.Context(0)
     # synthetic input generated on the GPU (seeded Gaussian field + Zel'dovich)

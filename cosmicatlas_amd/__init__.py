@@ -242,14 +242,13 @@ class Context:
     # ---------------------------------------------------------- whole mock
     def measure(self, pos, N, L, scheme, nshells, kmin, dk_bin, weights=None, mas_correction=True,
                 sn_correction=True, float_shell_rule=False, do_tweb=False, R_smooth=0.0,
-                lambda_th=0.0, margin_tol=0.0, fused=True, interlace_shift=0.0, labels=None):
+                lambda_th=0.0, margin_tol=0.0, fused=True, interlace_shift=0.0, labels=None, interlace=False):
         torch = _torch()
         n = pos.shape[0]
-        md = _abi.MasDesc(N, float(L), (C.c_double * 3)(0.0, 0.0, 0.0), CM_AOS, CM_F32, _scheme(scheme),
-                          float(interlace_shift), CM_MAS_AUTO)
-        o = _abi.MeasureOpts(md, nshells, float(kmin), float(dk_bin), int(mas_correction),
-                             int(sn_correction), int(float_shell_rule), int(do_tweb), float(R_smooth),
-                             float(lambda_th), float(margin_tol), int(fused))
+        o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=mas_correction,
+                          sn_correction=sn_correction, float_shell_rule=float_shell_rule, do_tweb=do_tweb,
+                          R_smooth=R_smooth, lambda_th=lambda_th, margin_tol=margin_tol, fused=fused,
+                          interlace_shift=interlace_shift, interlace=interlace)
         p0, p2, p4 = (np.zeros(nshells) for _ in range(3))
         nm = np.zeros(nshells, np.int64)
         if do_tweb and labels is None:
@@ -266,12 +265,12 @@ class Context:
 
 def _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=True, sn_correction=True,
                   float_shell_rule=False, do_tweb=False, R_smooth=0.0, lambda_th=0.0, margin_tol=0.0, fused=True,
-                  interlace_shift=0.0):
+                  interlace_shift=0.0, interlace=False):
     md = _abi.MasDesc(N, float(L), (C.c_double * 3)(0.0, 0.0, 0.0), CM_AOS, CM_F32, _scheme(scheme),
                       float(interlace_shift), CM_MAS_AUTO)
     return _abi.MeasureOpts(md, nshells, float(kmin), float(dk_bin), int(mas_correction), int(sn_correction),
                             int(float_shell_rule), int(do_tweb), float(R_smooth), float(lambda_th),
-                            float(margin_tol), int(fused))
+                            float(margin_tol), int(fused), int(interlace))
 
 
 class Ticket:

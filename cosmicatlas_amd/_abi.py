@@ -78,6 +78,7 @@ class MeasureOpts(C.Structure):
         ("lambda_th", C.c_double),
         ("margin_tol", C.c_double),
         ("fused", C.c_int),
+        ("interlace", C.c_int),
     ]
 
 

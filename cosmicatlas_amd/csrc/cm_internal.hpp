@@ -104,9 +104,12 @@ cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
 // Strided column passes over the half spectrum, axis 1 (y) or 0 (x).  pitch:
 // kz row pitch of the spectrum in complex values (0 = N/2+1, the FFTW layout;
 // float32 only may use a padded pitch).
-cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch = 0);
+// phase (float32 only, nullable): per-frequency factors float2[N] multiplied
+// into the output of the pass (kz index for the z pass, ky for the y pass).
+cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch = 0,
+                   const void *phase = nullptr);
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
-                       double subtract, double *part, int pitch = 0);
+                       double subtract, double *part, int pitch = 0, const void *phase = nullptr);
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double scale, int pitch = 0);
@@ -117,7 +120,7 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 // non-zero when the estimate was too small and the mesh must be redone.
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w,
                            size_t n, void *mesh, cm_dtype mdt, int accumulate, bool exact = true,
-                           unsigned **overflow_out = nullptr);
+                           unsigned **overflow_out = nullptr, bool reset_overflow = true);
 cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
                         double *host_out);
 

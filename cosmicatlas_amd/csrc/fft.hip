@@ -173,7 +173,8 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_r2c_ct(const float *__
                                                                  float2 *__restrict__ out, int pitch, float sub,
                                                                  const float2 *__restrict__ twM,
                                                                  const float2 *__restrict__ twN,
-                                                                 double *__restrict__ part) {
+                                                                 double *__restrict__ part,
+                                                                 const float2 *__restrict__ phase) {
     using RS = RowShape<N>;
     constexpr int M = RS::M, NB = RS::NB, STRIDE = RS::STRIDE, NT = RS::NT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -213,7 +214,9 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_r2c_ct(const float *__
         const float2 o = float2{0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y)};
         // X_k = E_k + W^k O_k with O_k = (Z_k - conj Z_{M-k}) / (2i)
         const float2 oi = float2{o.y, -o.x};
-        orow[(size_t)b * pitch + k] = cadd(e, cmul(twN[k], oi));
+        float2 x = cadd(e, cmul(twN[k], oi));
+        if (phase) x = cmul(x, phase[k]);
+        orow[(size_t)b * pitch + k] = x;
     }
 }
 
@@ -263,13 +266,15 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
 
 template <int N, int NB, bool FWD>
 __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__restrict__ data, int ncols, size_t stride,
-                                                                 size_t planeStride, const float2 *__restrict__ tw) {
+                                                                 size_t planeStride, const float2 *__restrict__ tw,
+                                                                 const float2 *__restrict__ phase) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     const size_t base = (size_t)blockIdx.y * planeStride + (size_t)blockIdx.x * NB;
     const int valid = min(NB, ncols - (int)blockIdx.x * NB);
     auto src = [&](int b, int e) { return b < valid ? data[base + (size_t)e * stride + b] : float2{0.f, 0.f}; };
     auto dst = [&](int b, int e, float2 v) {
+        if (phase) v = cmul(v, phase[e]);
         if (b < valid) data[base + (size_t)e * stride + b] = v;
     };
     ct_fft<float, N, NB, NB, true, FWD>(lds, tw, src, dst);
@@ -280,11 +285,11 @@ template <int N> constexpr int col_nb() { return N > 768 ? 8 : 16; }
 
 template <int N>
 static cm_status rows_r2c_ct(cm_ctx *ctx, const float *in, float2 *out, int pitch, float sub, const float2 *twM,
-                             const float2 *twN, double *part) {
+                             const float2 *twN, double *part, const float2 *phase) {
     using RS = RowShape<N>;
     if (RS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: LDS");
     return launch(ctx, "fft_z_r2c", k_rows_r2c_ct<N>, dim3((unsigned)((size_t)N * N / RS::NB)), dim3(RS::NT), RS::LDS,
-                  in, out, pitch, sub, twM, twN, part);
+                  in, out, pitch, sub, twM, twN, part, phase);
 }
 
 template <int N>
@@ -297,7 +302,7 @@ static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitc
 }
 
 template <int N, bool FWD>
-static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw) {
+static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw, const float2 *phase) {
     constexpr int NB = col_nb<N>();
     using CS = ColShape<N, NB>;
     if (CS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: LDS");
@@ -312,7 +317,8 @@ static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const f
     }
     dim3 grid((unsigned)((ncols + NB - 1) / NB), (unsigned)N);
     const char *name = axis == 1 ? (FWD ? "fft_y_fwd" : "fft_y_inv") : (FWD ? "fft_x_fwd" : "fft_x_inv");
-    return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw);
+    return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw,
+                  phase);
 }
 
 static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **twN) {
@@ -325,7 +331,7 @@ static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **
 }
 
 static cm_status rows_r2c_f32(cm_ctx *ctx, int N, const void *in, void *out, int pitch, double subtract,
-                              double *part) {
+                              double *part, const float2 *phase) {
     const float2 *twM, *twN;
     cm_status s = tw_pair(ctx, N, &twM, &twN);
     if (s != CM_OK) return s;
@@ -333,7 +339,7 @@ static cm_status rows_r2c_f32(cm_ctx *ctx, int N, const void *in, void *out, int
     auto *op = (float2 *)out;
     switch (N) {
 #define CM_CASE(n) \
-    case n: return rows_r2c_ct<n>(ctx, ip, op, pitch, (float)subtract, twM, twN, part);
+    case n: return rows_r2c_ct<n>(ctx, ip, op, pitch, (float)subtract, twM, twN, part, phase);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -355,7 +361,7 @@ static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int
     return fail(ctx, CM_ERR_UNSUPPORTED, "fft: N=%d", N);
 }
 
-static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch) {
+static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch, const float2 *phase) {
     cm_status st;
     const float2 *tw = (const float2 *)twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;
@@ -363,7 +369,8 @@ static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward
     switch (N) {
 #define CM_CASE(n)                                                                     \
     case n:                                                                            \
-        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw) : cols_ct<n, false>(ctx, dp, axis, pitch, tw);
+        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase)                                        \
+                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -508,8 +515,10 @@ static cm_status cols_t(cm_ctx *ctx, int N, void *data, int axis) {
 }
 
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double subtract,
-                       double *part, int pitch) {
-    if (dt == CM_F32) return rows_r2c_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, subtract, part);
+                       double *part, int pitch, const void *phase) {
+    if (dt == CM_F32)
+        return rows_r2c_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, subtract, part, (const float2 *)phase);
+    if (phase) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: phase on an FP64 spectrum");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return rows_r2c_t<double>(ctx, N, in, out, subtract, part);
 }
@@ -525,8 +534,11 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
     return rows_c2r_t<double>(ctx, N, in, out, scale);
 }
 
-cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch) {
-    if (dt == CM_F32) return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1);
+cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch,
+                   const void *phase) {
+    if (dt == CM_F32)
+        return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1, (const float2 *)phase);
+    if (phase) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: phase on an FP64 spectrum");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return forward ? cols_t<double, true>(ctx, N, data, axis) : cols_t<double, false>(ctx, N, data, axis);
 }

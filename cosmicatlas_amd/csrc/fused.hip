@@ -69,10 +69,12 @@ struct ShellAcc {
     }
 };
 
-template <int N, int NB>
-__global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__restrict__ data, int pitch,
+template <int N, int NB, bool IL>
+__global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__restrict__ data,
+                                                 const float2 *__restrict__ dataB, int pitch,
                                                  const float2 *__restrict__ tw, PkK pk,
-                                                 const double *__restrict__ tabs, double *__restrict__ gsum,
+                                                 const double *__restrict__ tabs, float wc, float wa,
+                                                 double *__restrict__ gsum,
                                                  unsigned long long *__restrict__ gcnt, int do_tweb, float inv_kf2,
                                                  const float *__restrict__ gauss, const double *__restrict__ nobj) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -82,16 +84,20 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
     double *s_acc = reinterpret_cast<double *>(lds + (size_t)(NB + 1) * N);  // 4*ns
     unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);           // ns
     float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                      // M1
-    float *s_sq = s_ic + M1;                                                  // M1
-    float *s_sqic = s_sq + M1;                                                // M1
-    float *s_g = s_sqic + M1;                                                 // N
+    float *s_sq = s_ic + M1;                                                  // M1: C1 (aliasing sum)
+    float *s_al = s_sq + M1;                                                  // M1: alternating sum
+    float *s_sqic = s_al + M1;                                                // M1
+    float *s_alic = s_sqic + M1;                                              // M1
+    float *s_g = s_alic + M1;                                                 // N
     constexpr int NT = ColShape<N, NB>::NT;
     const int ncols = N / 2 + 1;
     const int tid = threadIdx.x;
     for (int i = tid; i < M1; i += NT) {
         s_ic[i] = (float)tabs[i];
         s_sq[i] = (float)tabs[M1 + i];
+        s_al[i] = (float)tabs[2 * M1 + i];
         s_sqic[i] = (float)(tabs[M1 + i] * tabs[i]);
+        s_alic[i] = (float)(tabs[2 * M1 + i] * tabs[i]);
     }
     for (int i = tid; i < N; i += NT) s_g[i] = gauss[i];
     for (int i = tid; i < 4 * ns; i += NT) s_acc[i] = 0.0;
@@ -111,6 +117,19 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
         const int valid = min(NB, ncols - kz0);
         x_fft_fwd_to_lds<N, NB>(data, base, xstride, valid, lds, tw);
         __syncthreads();
+        if constexpr (IL) {
+            // interlaced meshes: average with the second mesh's finished
+            // spectrum (phase exp(i k.(d/2)(1,1,1)) already applied)
+            for (int t = tid; t < NB * N; t += NT) {
+                const int b = t % NB, e = t / NB;
+                if (b < valid) {
+                    float2 &a = lds[e * (NB + 1) + b];
+                    const float2 v = dataB[base + (size_t)e * xstride + b];
+                    a = float2{0.5f * (a.x + v.x), 0.5f * (a.y + v.y)};
+                }
+            }
+            __syncthreads();
+        }
         // shells (+ multiply): one thread per kx row, serial along the tile's kz
         // (|k| grows with kz, so shells are visited in order)
         const int ky = iy <= h ? iy : iy - N;
@@ -125,7 +144,10 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
             const bool excl = ix == h || iy == h;
             const bool zero_ok = !excl && yl && (xl ? (ix | iy) != 0 : iy > 0);
             const int qxy = fx * fx + fy * fy;
-            const float icxy = s_ic[fx] * s_ic[fy], sqxy = s_sq[fx] * s_sq[fy] * icxy;
+            // shot-noise weight per mode: (wc prod C1 + wa prod A) / W^2
+            // (wa = 0 for one mesh; wc = wa = 1/2 for interlaced meshes)
+            const float icxy = s_ic[fx] * s_ic[fy];
+            const float sqxy = wc * s_sq[fx] * s_sq[fy] * icxy, alxy = wa * s_al[fx] * s_al[fy] * icxy;
             const int kx = xl ? ix : ix - N;
             const int q2xy = kx * kx + ky * ky;
             const float gxy = fscale * s_g[ix] * gy;
@@ -176,7 +198,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
                     const float pc = (v.x * v.x + v.y * v.y) * icxy * s_ic[iz];
                     const float mu2 = (float)(iz * iz) * rq;
                     acc.a0 += pc;
-                    acc.a1 += sqxy * s_sqic[iz];
+                    acc.a1 += sqxy * s_sqic[iz] + alxy * s_alic[iz];
                     acc.a2 += (1.5f * mu2 - 0.5f) * pc;
                     acc.a3 += ((4.375f * mu2 - 3.75f) * mu2 + 0.375f) * pc;
                     acc.c += 1u;
@@ -228,38 +250,44 @@ cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, doub
                             float *lam, unsigned long long *stats);
 
 template <int N, int NB>
-static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, int pitch, const float2 *tw, const PkK &pk, const double *dtab,
-                             double *dsum, unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
+static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pitch,
+                             const float2 *tw, const PkK &pk, const double *dtab, double *dsum,
+                             unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
                              const double *dnobj, int ns) {
     constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB>::NT;
     const size_t lds = sizeof(float2) * (size_t)(NB + 1) * N + sizeof(double) * 4 * (size_t)ns +
-                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (3 * (size_t)M1 + (size_t)N);
+                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (5 * (size_t)M1 + (size_t)N);
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
+    auto kern = dkB ? k_x_fused<N, NB, true> : k_x_fused<N, NB, false>;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_x_fused<N, NB>, NT, lds) != hipSuccess || per_cu < 1)
-        per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1) per_cu = 1;
     const int ntiles = ((M1 + NB - 1) / NB) * N;
     const unsigned grid = (unsigned)std::min(ntiles, std::max(1, ctx->cu_count) * per_cu);
-    return launch(ctx, "fused_x", k_x_fused<N, NB>, dim3(grid), dim3(NT), lds, dk, pitch, tw, pk, dtab, dsum, dcnt,
+    const float wc = dkB ? 0.5f : 1.0f, wa = dkB ? 0.5f : 0.0f;
+    return launch(ctx, "fused_x", kern, dim3(grid), dim3(NT), lds, dk, dkB, pitch, tw, pk, dtab, wc, wa, dsum, dcnt,
                   do_tweb, inv_kf2, dgauss, dnobj);
 }
 
 template <int N>
-static cm_status x_fused_launch(cm_ctx *ctx, int nb, float2 *dk, int pitch, const float2 *tw, const PkK &pk,
-                                const double *dtab, double *dsum, unsigned long long *dcnt, int do_tweb,
-                                float inv_kf2, const float *dgauss, const double *dnobj, int ns) {
+static cm_status x_fused_launch(cm_ctx *ctx, int nb, float2 *dk, const float2 *dkB, int pitch,
+                                const float2 *tw, const PkK &pk, const double *dtab, double *dsum,
+                                unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
+                                const double *dnobj, int ns) {
     if constexpr (ColShape<N, 8>::NT * 2 <= 512) {
         if (nb == 16)
-            return x_fused_run<N, 16>(ctx, dk, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj, ns);
+            return x_fused_run<N, 16>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss,
+                                      dnobj, ns);
     }
-    return x_fused_run<N, 8>(ctx, dk, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj, ns);
+    return x_fused_run<N, 8>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj,
+                             ns);
 }
 
 // Fused measurement: mesh (float32, device) -> shells (+ labels).
 // Device outputs: sums[4*ns], cnts[ns], nobj[1], stats[5].
-cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean_est, int8_t *labels,
-                             double **sums, unsigned long long **cnts, double **nobj, unsigned long long **stats) {
+cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
+                             int8_t *labels, double **sums, unsigned long long **cnts, double **nobj,
+                             unsigned long long **stats) {
     const int N = o->mas.N;
     const size_t n3 = (size_t)N * N * N;
     const int pitch = padded_pitch(N);  // internal spectrum: kz rows padded to 128-byte multiples
@@ -286,6 +314,23 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
         return s;
     // y pass
     if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch)) != CM_OK) return s;
+    // interlaced mesh (particles shifted by d/2 on every axis): its full
+    // spectrum, multiplied by exp(+i k.(d/2)(1,1,1)) = prod_a exp(i pi k_a/N)
+    // one axis per pass, is averaged with the first in the fused x pass
+    void *dkB = nullptr, *phase = nullptr;
+    if (meshB) {
+        if ((s = scratch(ctx, "m.dkB", sizeof(float2) * nh, &dkB)) != CM_OK) return s;
+        if ((s = scratch(ctx, "f.phase", sizeof(float2) * (size_t)N, &phase)) != CM_OK) return s;
+        std::vector<float2> ph((size_t)N);
+        for (int j = 0; j < N; ++j) {
+            const double a = M_PI * (double)(j <= N / 2 ? j : j - N) / (double)N;
+            ph[j] = float2{(float)std::cos(a), (float)std::sin(a)};
+        }
+        CM_HIP(ctx, hipMemcpyAsync(phase, ph.data(), sizeof(float2) * ph.size(), hipMemcpyHostToDevice, ctx->stream));
+        if ((s = fft_rows_r2c(ctx, N, CM_F32, meshB, dkB, mean_est, nullptr, pitch, phase)) != CM_OK) return s;
+        if ((s = fft_cols(ctx, N, CM_F32, dkB, 1, true, pitch, phase)) != CM_OK) return s;
+        if ((s = fft_cols(ctx, N, CM_F32, dkB, 0, true, pitch, phase)) != CM_OK) return s;
+    }
     // x pass + shells + multiply + inverse x
     if ((s = launch(ctx, "fused_zero", k_fused_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
                     ns)) != CM_OK)
@@ -323,7 +368,8 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     switch (N) {
 #define CM_CASE(n)                                                                                                     \
     case n:                                                                                                            \
-        s = x_fused_launch<n>(ctx, nb, (float2 *)dk, pitch, (const float2 *)tw, pk, (const double *)dtab,             \
+        s = x_fused_launch<n>(ctx, nb, (float2 *)dk, (const float2 *)dkB, pitch,                                   \
+                              (const float2 *)tw, pk, (const double *)dtab,                                            \
                               (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss,     \
                               (const double *)dnobj, ns);                                                              \
         break;

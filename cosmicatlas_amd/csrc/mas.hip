@@ -196,8 +196,9 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
 // exclusive values, bsum[2*blk..] the workgroup sums.
 __global__ void __launch_bounds__(256) k_tile_prefix(unsigned *__restrict__ hist, int nrows, int ntiles,
                                                      unsigned *__restrict__ offsets, unsigned *__restrict__ item_off,
-                                                     unsigned *__restrict__ counts, unsigned *__restrict__ bsum) {
-    __shared__ unsigned sa[256], sb[256];
+                                                     unsigned *__restrict__ counts,
+                                                     unsigned long long *__restrict__ bsum) {
+    __shared__ unsigned long long sa[256], sb[256];  // 64-bit: a block of tiles may hold 2^32 records
     const int tid = threadIdx.x, tile = blockIdx.x * 256 + tid;
     unsigned run = 0;
     if (tile < ntiles) {
@@ -224,15 +225,17 @@ __global__ void __launch_bounds__(256) k_tile_prefix(unsigned *__restrict__ hist
     sb[tid] = chunks;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-        const unsigned va = tid >= off ? sa[tid - off] : 0u, vb = tid >= off ? sb[tid - off] : 0u;
+        const unsigned long long va = tid >= off ? sa[tid - off] : 0ull, vb = tid >= off ? sb[tid - off] : 0ull;
         __syncthreads();
         sa[tid] += va;
         sb[tid] += vb;
         __syncthreads();
     }
+    // workgroup-local exclusive offsets; a value of 2^32 or more only occurs
+    // with a total beyond the record capacity, which k_tile_items reports
     if (tile < ntiles) {
-        offsets[tile] = sa[tid] - run;
-        item_off[tile] = sb[tid] - chunks;
+        offsets[tile] = (unsigned)(sa[tid] - run);
+        item_off[tile] = (unsigned)(sb[tid] - chunks);
         counts[tile] = run;
     }
     if (tid == 255) {
@@ -241,17 +244,18 @@ __global__ void __launch_bounds__(256) k_tile_prefix(unsigned *__restrict__ hist
     }
 }
 
-// Exclusive scan of the workgroup sums (one workgroup, nblk <= 4096).
-__global__ void __launch_bounds__(1024) k_tile_bscan(unsigned *__restrict__ bsum, int nblk) {
-    __shared__ unsigned sa[1024], sb[1024];
+// Exclusive scan of the workgroup sums (one workgroup, nblk <= 4096), in
+// 64-bit so that a total of 2^32 records or more is seen and reported.
+__global__ void __launch_bounds__(1024) k_tile_bscan(unsigned long long *__restrict__ bsum, int nblk) {
+    __shared__ unsigned long long sa[1024], sb[1024];
     const int tid = threadIdx.x;
     constexpr int PER = 4;
-    unsigned a[PER], b[PER], ta = 0, tb = 0;
+    unsigned long long a[PER], b[PER], ta = 0, tb = 0;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = tid * PER + k;
-        a[k] = i < nblk ? bsum[2 * i] : 0u;
-        b[k] = i < nblk ? bsum[2 * i + 1] : 0u;
+        a[k] = i < nblk ? bsum[2 * i] : 0ull;
+        b[k] = i < nblk ? bsum[2 * i + 1] : 0ull;
         ta += a[k];
         tb += b[k];
     }
@@ -259,13 +263,13 @@ __global__ void __launch_bounds__(1024) k_tile_bscan(unsigned *__restrict__ bsum
     sb[tid] = tb;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned va = tid >= off ? sa[tid - off] : 0u, vb = tid >= off ? sb[tid - off] : 0u;
+        const unsigned long long va = tid >= off ? sa[tid - off] : 0ull, vb = tid >= off ? sb[tid - off] : 0ull;
         __syncthreads();
         sa[tid] += va;
         sb[tid] += vb;
         __syncthreads();
     }
-    unsigned ra = sa[tid] - ta, rb = sb[tid] - tb;
+    unsigned long long ra = sa[tid] - ta, rb = sb[tid] - tb;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = tid * PER + k;
@@ -286,31 +290,36 @@ __global__ void __launch_bounds__(1024) k_tile_bscan(unsigned *__restrict__ bsum
 // {tile, first record, end record, split} per chunk of at most kTileCH records,
 // clamped to the record buffer's capacity.  counts[tile] holds the tile total
 // on entry and the slot cursor (= first record) on exit.
-__global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned *__restrict__ bsum,
+__global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned long long *__restrict__ bsum,
                                                     unsigned *__restrict__ offsets, unsigned *__restrict__ item_off,
                                                     unsigned *__restrict__ counts, uint4 *__restrict__ items,
-                                                    unsigned *__restrict__ nitems, unsigned cap) {
+                                                    unsigned *__restrict__ nitems, unsigned cap,
+                                                    unsigned *__restrict__ overflow) {
     const int tile = blockIdx.x * 256 + threadIdx.x;
     const int nblk = (ntiles + 255) / 256;
     if (tile == 0) {
-        offsets[ntiles] = bsum[2 * nblk];
-        item_off[ntiles] = bsum[2 * nblk + 1];
-        *nitems = bsum[2 * nblk + 1];
+        const unsigned long long tot = bsum[2 * nblk];
+        offsets[ntiles] = (unsigned)min(tot, (unsigned long long)cap);
+        item_off[ntiles] = (unsigned)bsum[2 * nblk + 1];
+        *nitems = (unsigned)bsum[2 * nblk + 1];
+        if (tot > cap) atomicOr(overflow, 1u);
     }
     if (tile >= ntiles) return;
     const int blk = tile / 256;
     const unsigned cnt = counts[tile];
-    const unsigned tb = offsets[tile] + bsum[2 * blk];
-    const unsigned ib = item_off[tile] + bsum[2 * blk + 1];
-    const unsigned te = tb + cnt;
+    const unsigned long long tb64 = offsets[tile] + bsum[2 * blk];
+    const unsigned tb = (unsigned)min(tb64, (unsigned long long)cap);
+    const unsigned ib = item_off[tile] + (unsigned)bsum[2 * blk + 1];
+    const unsigned long long te64 = tb64 + cnt;
     offsets[tile] = tb;
     item_off[tile] = ib;
-    counts[tile] = tb;
+    counts[tile] = tb;  // slot cursor of the global scatter
     const unsigned nch = cnt == 0 ? 1u : (cnt + kTileCH - 1) / kTileCH;
     const unsigned split = cnt > (unsigned)kTileCH ? 1u : 0u;
     for (unsigned k = 0; k < nch; ++k) {
-        const unsigned b = min(tb + k * (unsigned)kTileCH, cap), e = min(min(te, tb + (k + 1) * (unsigned)kTileCH), cap);
-        items[ib + k] = make_uint4((unsigned)tile, b, e, split);
+        const unsigned long long b = min(tb64 + (unsigned long long)k * kTileCH, (unsigned long long)cap);
+        const unsigned long long e = min(min(te64, tb64 + (unsigned long long)(k + 1) * kTileCH), (unsigned long long)cap);
+        items[ib + k] = make_uint4((unsigned)tile, (unsigned)b, (unsigned)e, split);
     }
 }
 
@@ -585,7 +594,6 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     t.CH = kTileCH;
     const int ntiles = t.ntx * t.ntx * t.ntx;
     const size_t max_copies = p.n * (t.R > 0 ? 8 : 1);
-    if (max_copies > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many particles");
     const int nblk = (ntiles + 255) / 256;
     if (nblk > 4096) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many tiles");
     // LDS-privatised histogram and cursors when a workgroup's LDS holds one
@@ -600,8 +608,9 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     if ((s = scratch(ctx, "mas.counts", sizeof(unsigned) * ntiles, &counts)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.offsets", sizeof(unsigned) * (ntiles + 1), &offsets)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.item_off", sizeof(unsigned) * (ntiles + 1), &item_off)) != CM_OK) return s;
-    if ((s = scratch(ctx, "mas.bsum", sizeof(unsigned) * 2 * (size_t)(nblk + 1), &bsum)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.bsum", sizeof(unsigned long long) * 2 * (size_t)(nblk + 1), &bsum)) != CM_OK) return s;
     const size_t max_items = (size_t)ntiles + max_copies / kTileCH + 1;
+    if (max_items > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many particles");
     if ((s = scratch(ctx, "mas.items", sizeof(uint4) * max_items, &items)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.nitems", sizeof(unsigned), &nitems)) != CM_OK) return s;
     auto *hp = (unsigned *)hist;
@@ -617,29 +626,35 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
             return s;
     }
     if ((s = launch(ctx, "mas_tile_prefix", k_tile_prefix, dim3(nblk), dim3(256), 0, hp, nrows, ntiles,
-                    (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts, (unsigned *)bsum)) != CM_OK)
+                    (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts,
+                    (unsigned long long *)bsum)) != CM_OK)
         return s;
-    if ((s = launch(ctx, "mas_tile_scan", k_tile_bscan, dim3(1), dim3(1024), 0, (unsigned *)bsum, nblk)) != CM_OK)
+    if ((s = launch(ctx, "mas_tile_scan", k_tile_bscan, dim3(1), dim3(1024), 0, (unsigned long long *)bsum, nblk)) !=
+        CM_OK)
         return s;
+    // record indices are 32-bit
+    const size_t max_cap = 0xffffffffull;
     size_t cap;
     if (exact) {
-        unsigned total = 0;
-        CM_HIP(ctx, hipMemcpyAsync(&total, (unsigned *)bsum + 2 * nblk, sizeof(unsigned), hipMemcpyDeviceToHost,
-                                   ctx->stream));
+        unsigned long long total = 0;
+        CM_HIP(ctx, hipMemcpyAsync(&total, (unsigned long long *)bsum + 2 * nblk, sizeof(total),
+                                   hipMemcpyDeviceToHost, ctx->stream));
         CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (total > max_cap)
+            return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %llu records exceed 32-bit indices", total);
         cap = (size_t)total;
     } else {
         // geometric ghost fraction (1 + 2R/T)^3 with 50% headroom; a larger
         // count sets *overflow and the caller repeats with exact = true
         const double g1 = 1.0 + 2.0 * t.R / T;
-        cap = std::min(max_copies, (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
+        cap = std::min(std::min(max_copies, max_cap), (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
     }
     if ((s = scratch(ctx, "mas.sorted", sizeof(float4) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
     void *wsorted = nullptr;
     if (w && (s = scratch(ctx, "mas.wsorted", sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK) return s;
-    if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3(nblk), dim3(256), 0, ntiles, (const unsigned *)bsum,
-                    (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts, (uint4 *)items, (unsigned *)nitems,
-                    (unsigned)cap)) != CM_OK)
+    if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3(nblk), dim3(256), 0, ntiles,
+                    (const unsigned long long *)bsum, (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts,
+                    (uint4 *)items, (unsigned *)nitems, (unsigned)cap, overflow)) != CM_OK)
         return s;
     if (lds_hist) {
         if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter<T, true>, dim3(nrows), dim3(1024), hist_lds, pos, w, p,
@@ -695,7 +710,8 @@ static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const f
 }
 
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w, size_t n,
-                           void *mesh, cm_dtype mdt, int accumulate, bool exact, unsigned **overflow_out) {
+                           void *mesh, cm_dtype mdt, int accumulate, bool exact, unsigned **overflow_out,
+                           bool reset_overflow) {
     if (overflow_out) *overflow_out = nullptr;
     MasK p = make_mask(d, n);
     const size_t n3 = (size_t)d->N * d->N * d->N;
@@ -710,7 +726,7 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
     if (use_tiled) {
         void *ovf;
         if ((s = scratch(ctx, "mas.overflow", sizeof(unsigned), &ovf)) != CM_OK) return s;
-        if ((s = fill(ctx, (unsigned *)ovf, 1, 0u)) != CM_OK) return s;
+        if (reset_overflow && (s = fill(ctx, (unsigned *)ovf, 1, 0u)) != CM_OK) return s;
         if (overflow_out) *overflow_out = (unsigned *)ovf;
         return mas_tiled(ctx, p, (const float *)pos, (const float *)w, (float *)mesh, accumulate,
                          (unsigned long long *)dropped, exact || !overflow_out, (unsigned *)ovf);

@@ -46,8 +46,9 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
     return (unsigned)g;
 }
 
-cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean_est, int8_t *labels,
-                             double **sums, unsigned long long **cnts, double **nobj, unsigned long long **stats);
+cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
+                             int8_t *labels, double **sums, unsigned long long **cnts, double **nobj,
+                             unsigned long long **stats);
 
 // Finalisation shared by both paths: normal_power = N_gal^2/V, SN = V/N_gal
 // (raw_sampling, FftwFunctions.cpp:159-174), so SN*normal_power = N_gal.
@@ -67,14 +68,14 @@ static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<
 
 // Returns CM_OK with *redo set when the tiled deposit's record estimate was too
 // small (the results are then discarded and the caller repeats exactly).
-static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, size_t n, double *pk0,
-                                    double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
+static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, size_t n,
+                                    double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
                                     cm_measure_info *info, const unsigned *overflow, bool *redo) {
     const int N = o->mas.N;
     const double n3 = (double)N * N * N;
     double *sums, *nobj;
     unsigned long long *cnts, *stats;
-    cm_status s = fused_measure_mesh(ctx, o, mesh, (double)n / n3, labels, &sums, &cnts, &nobj, &stats);
+    cm_status s = fused_measure_mesh(ctx, o, mesh, meshB, (double)n / n3, labels, &sums, &cnts, &nobj, &stats);
     if (s != CM_OK) return s;
     const int ns = o->nshells;
     std::vector<double> hs(4 * (size_t)ns);
@@ -115,17 +116,28 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     cm_mas_desc md = o->mas;
     md.algo = o->fused ? CM_MAS_AUTO : CM_MAS_PLAIN;
     if (o->fused) {
+        void *meshB = nullptr;
+        if (o->interlace && (s = scratch(ctx, "m.meshB", sizeof(float) * n3, &meshB)) != CM_OK) return s;
         // first attempt without the record-count read-back; repeat exactly on overflow
         for (int attempt = 0; attempt < 2; ++attempt) {
             unsigned *ovf = nullptr;
             if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, attempt > 0, &ovf)) != CM_OK) return s;
+            if (meshB) {
+                cm_mas_desc mb = md;
+                mb.interlace_shift += 0.5;  // half a cell on every axis
+                unsigned *ovfB = nullptr;
+                if ((s = mesh_assign_impl(ctx, &mb, pos, w, n, meshB, CM_F32, 0, attempt > 0, &ovfB, false)) != CM_OK)
+                    return s;
+            }
             bool redo = false;
-            if ((s = measure_fused_tail(ctx, o, mesh, n, pk0, pk2, pk4, nmodes, labels, info, ovf, &redo)) != CM_OK)
+            if ((s = measure_fused_tail(ctx, o, mesh, meshB, n, pk0, pk2, pk4, nmodes, labels, info, ovf, &redo)) !=
+                CM_OK)
                 return s;
             if (!redo) return CM_OK;
         }
         return fail(ctx, CM_ERR_DEVICE, "cm_measure: record buffer overflow after exact sizing");
     }
+    if (o->interlace) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure: interlacing needs the fused path");
     if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0)) != CM_OK) return s;
     // N_gal = FP64 sum of the mesh (get_nobjects without its float sum)
     double nobj_h = 0.0;

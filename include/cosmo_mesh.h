@@ -198,6 +198,11 @@ typedef struct {
     int do_tweb;         /* 0: P(k) only */
     double R_smooth, lambda_th, margin_tol;
     int fused;           /* 1: fused pipeline; 0: the plain step-by-step GPU path */
+    int interlace;       /* 1 (fused only): second mesh of the particles shifted by
+                            half a cell on every axis, spectra averaged as
+                            (d1 + exp(i k.(d/2)(1,1,1)) d2)/2 and the aliased shot
+                            noise (prod C1 + prod A)/2 with A the alternating
+                            aliasing sum; no counterpart in the reference */
 } cm_measure_opts;
 
 typedef struct {
