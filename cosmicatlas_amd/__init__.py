@@ -220,6 +220,23 @@ class Context:
                                               C.byref(st)))
         return labels, lam, {"counts": list(st.counts), "n_marginal": st.n_marginal}
 
+    def measure_fkp(self, pos, rand, N, L, scheme, nshells, kmin, dk_bin, weights=None, rweights=None, nbar=0.0,
+                    P_est=0.0, fkp_weight=False, window=False, mas_correction=True, sn_correction=True,
+                    float_shell_rule=False):
+        """FKP estimator with a random catalogue ``rand`` ((nr,3) float32)."""
+        o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=mas_correction,
+                          sn_correction=sn_correction, float_shell_rule=float_shell_rule)
+        f = _abi.FkpOpts(float(nbar), float(P_est), int(fkp_weight), int(window))
+        p0, p2, p4, pw = (np.zeros(nshells) for _ in range(4))
+        nm = np.zeros(nshells, np.int64)
+        info = _abi.MeasureInfo()
+        self._check(self.lib.cm_measure_fkp(self.h, C.byref(o), C.byref(f), _ptr(pos), _ptr(weights), pos.shape[0],
+                                            _ptr(rand), _ptr(rweights), rand.shape[0],
+                                            p0.ctypes.data_as(C.c_void_p), p2.ctypes.data_as(C.c_void_p),
+                                            p4.ctypes.data_as(C.c_void_p), nm.ctypes.data_as(C.c_void_p),
+                                            pw.ctypes.data_as(C.c_void_p) if window else None, C.byref(info)))
+        return _fkp_result(p0, p2, p4, nm, pw if window else None, info)
+
     # ------------------------------------------------------- synthetic mocks
     def zeldovich(self, NL, L, A=1e4, ns=-1.5, seed=1, out=None):
         """NL^3 Zel'dovich particles (float32, (NL^3, 3), device) from a seeded
@@ -261,6 +278,11 @@ class Context:
         return {"pk0": p0, "pk2": p2, "pk4": p4, "nmodes": nm, "labels": labels,
                 "n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
                 "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal}
+
+
+def _fkp_result(p0, p2, p4, nm, pw, info):
+    return {"pk0": p0, "pk2": p2, "pk4": p4, "nmodes": nm, "pk_window": pw, "W_g": info.n_obj,
+            "shot_noise": info.shot_noise, "normal_power": info.normal_power}
 
 
 def _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=True, sn_correction=True,

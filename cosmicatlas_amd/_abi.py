@@ -101,6 +101,15 @@ class ZeldovichOpts(C.Structure):
     ]
 
 
+class FkpOpts(C.Structure):
+    _fields_ = [
+        ("nbar", C.c_double),
+        ("P_est", C.c_double),
+        ("fkp_weight", C.c_int),
+        ("window", C.c_int),
+    ]
+
+
 class Request(C.Structure):
     _fields_ = [
         ("opts", MeasureOpts),
@@ -152,6 +161,8 @@ def _declare(L):
         "cm_measure": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, P, P, P, P,
                             C.POINTER(MeasureInfo)]),
         "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),
+        "cm_measure_fkp": (st, [P, C.POINTER(MeasureOpts), C.POINTER(FkpOpts), P, P, C.c_size_t, P, P, C.c_size_t,
+                                P, P, P, P, P, C.POINTER(MeasureInfo)]),
         "cm_request_footprint": (C.c_size_t, [C.POINTER(Request)]),
         "cm_server_create": (st, [C.c_int, C.c_int, C.c_size_t, C.POINTER(P)]),
         "cm_server_submit": (st, [P, C.POINTER(Request), C.POINTER(C.c_int64)]),

@@ -1,0 +1,183 @@
+// FKP estimator with a random catalogue (gfx950).
+//
+// Per catalogue (Catalog::get_interpolated_density_field, Source/Catalog.cpp:
+// 14153-14171): total weight w = w_user * w_fkp, w_fkp = 1/(1 + nbar P_est);
+// W = sum w, S = sum w^2, normal = sum nbar w^2 (nbar constant here).
+// FftwFunctions::get_parameters_estimator random branch
+// (Source/FftwFunctions.cpp:180-200): alpha = W_g/W_r, normal_power =
+// alpha normal_r, shot noise = (alpha + 1) S_r/normal_r, window
+// normalisation normal_power/alpha^2.  get_fluctuation (:275-281):
+// delta = rho_g - alpha rho_r, built on one mesh by depositing the randoms with
+// weight -alpha w.  The shells are those of the fused path.
+#include <cmath>
+#include <vector>
+
+#include "cm_internal.hpp"
+
+namespace cm {
+
+cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
+                             int8_t *labels, double **sums, unsigned long long **cnts, double **nobj,
+                             unsigned long long **stats);
+
+// FP64 sums of w and w^2 (unit weights when w is NULL), one partial pair per
+// workgroup.
+__global__ void __launch_bounds__(256) k_wsums(const float *__restrict__ w, size_t n, double *__restrict__ part) {
+    __shared__ double r1[256], r2[256];
+    double a = 0.0, b = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = w ? (double)w[i] : 1.0;
+        a += v;
+        b += v * v;
+    }
+    r1[threadIdx.x] = a;
+    r2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            r1[threadIdx.x] += r1[threadIdx.x + o];
+            r2[threadIdx.x] += r2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r1[0];
+        part[2 * blockIdx.x + 1] = r2[0];
+    }
+}
+
+// out[i] = s * (w ? w[i] : 1)
+__global__ void __launch_bounds__(256) k_wscale(const float *__restrict__ w, size_t n, float s,
+                                                float *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = s * (w ? w[i] : 1.0f);
+}
+
+static unsigned grid_of(size_t n, size_t cap) {
+    size_t g = (n + 255) / 256;
+    return (unsigned)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+static cm_status weight_sums(cm_ctx *ctx, const float *w, size_t n, double *sum, double *sum2) {
+    *sum = *sum2 = 0.0;
+    if (!n) return CM_OK;
+    const unsigned g = grid_of(n, 1024);
+    void *part;
+    cm_status s;
+    if ((s = scratch(ctx, "fkp.part", sizeof(double) * 2 * g, &part)) != CM_OK) return s;
+    if ((s = launch(ctx, "fkp_wsums", k_wsums, dim3(g), dim3(256), 0, w, n, (double *)part)) != CM_OK) return s;
+    std::vector<double> h(2 * (size_t)g);
+    CM_HIP(ctx, hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (unsigned i = 0; i < g; ++i) {
+        *sum += h[2 * i];
+        *sum2 += h[2 * i + 1];
+    }
+    return CM_OK;
+}
+
+// deposit weights s * w (or s); NULL when they are all 1
+static cm_status scaled_weights(cm_ctx *ctx, const char *name, const float *w, size_t n, double s,
+                                const float **out) {
+    *out = nullptr;
+    if (!w && s == 1.0) return CM_OK;
+    void *buf;
+    cm_status st;
+    if ((st = scratch(ctx, name, sizeof(float) * (n ? n : 1), &buf)) != CM_OK) return st;
+    if (n && (st = launch(ctx, "fkp_wscale", k_wscale, dim3(grid_of(n, 16384)), dim3(256), 0, w, n, (float)s,
+                          (float *)buf)) != CM_OK)
+        return st;
+    *out = (const float *)buf;
+    return CM_OK;
+}
+
+// mesh -> shells through the fused path, host sums and counts
+static cm_status shells_of_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean,
+                                std::vector<double> &hs, std::vector<unsigned long long> &hc) {
+    double *sums, *dnobj;
+    unsigned long long *cnts, *stats;
+    cm_status s;
+    if ((s = fused_measure_mesh(ctx, o, mesh, nullptr, mean, nullptr, &sums, &cnts, &dnobj, &stats)) != CM_OK)
+        return s;
+    hs.assign(4 * (size_t)o->nshells, 0.0);
+    hc.assign((size_t)o->nshells, 0ull);
+    CM_HIP(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipMemcpyAsync(hc.data(), cnts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CM_OK;
+}
+
+static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const cm_fkp_opts *f, const void *pos,
+                                  const float *w, size_t n, const void *rpos, const float *rw, size_t nr,
+                                  double *pk0, double *pk2, double *pk4, int64_t *nmodes, double *pk_window,
+                                  cm_measure_info *info) {
+    const int N = o->mas.N;
+    const size_t n3 = (size_t)N * N * N;
+    const double L = o->mas.L, V = L * L * L;
+    cm_status s;
+    double Wg, Sg, Wr, Sr;
+    if ((s = weight_sums(ctx, w, n, &Wg, &Sg)) != CM_OK) return s;
+    if ((s = weight_sums(ctx, rw, nr, &Wr, &Sr)) != CM_OK) return s;
+    if (!(Wg > 0) || !(Wr > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: empty catalogue");
+    const double nbar = f->nbar > 0 ? f->nbar : Wg / V;
+    const double wf = f->fkp_weight ? 1.0 / (1.0 + nbar * f->P_est) : 1.0;
+    Wg *= wf;
+    Wr *= wf;
+    Sr *= wf * wf;
+    const double alpha = Wg / Wr;
+    const double normal_r = nbar * Sr;
+    const double normal_power = alpha * normal_r;
+    const double shot_noise = (alpha + 1.0) * Sr / normal_r;
+
+    void *mesh;
+    if ((s = scratch(ctx, "m.mesh", sizeof(float) * n3, &mesh)) != CM_OK) return s;
+    cm_mas_desc md = o->mas;
+    md.algo = CM_MAS_AUTO;
+    const float *wg, *wr;
+    if ((s = scaled_weights(ctx, "fkp.wg", w, n, wf, &wg)) != CM_OK) return s;
+    if ((s = mesh_assign_impl(ctx, &md, pos, wg, n, mesh, CM_F32, 0)) != CM_OK) return s;
+    if ((s = scaled_weights(ctx, "fkp.wr", rw, nr, -alpha * wf, &wr)) != CM_OK) return s;
+    if ((s = mesh_assign_impl(ctx, &md, rpos, wr, nr, mesh, CM_F32, 1)) != CM_OK) return s;
+    std::vector<double> hs;
+    std::vector<unsigned long long> hc;
+    if ((s = shells_of_mesh(ctx, o, mesh, 0.0, hs, hc)) != CM_OK) return s;
+    const double sn = o->sn_correction ? shot_noise : 0.0;
+    pk_finalize(o->nshells, normal_power, sn * normal_power, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
+    if (info) {
+        info->n_obj = Wg;
+        info->shot_noise = shot_noise;
+        info->normal_power = normal_power;
+        info->tweb = cm_tweb_stats{};
+    }
+    if (f->window && pk_window) {
+        const float *wr1;
+        if ((s = scaled_weights(ctx, "fkp.wr", rw, nr, wf, &wr1)) != CM_OK) return s;
+        if ((s = mesh_assign_impl(ctx, &md, rpos, wr1, nr, mesh, CM_F32, 0)) != CM_OK) return s;
+        if ((s = shells_of_mesh(ctx, o, mesh, Wr / (double)n3, hs, hc)) != CM_OK) return s;
+        std::vector<double> q2((size_t)o->nshells), q4((size_t)o->nshells);
+        std::vector<int64_t> nm((size_t)o->nshells);
+        pk_finalize(o->nshells, normal_power / (alpha * alpha), 0.0, hs.data(), hc.data(), pk_window, q2.data(),
+                    q4.data(), nm.data());
+    }
+    return CM_OK;
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" cm_status cm_measure_fkp(cm_ctx *ctx, const cm_measure_opts *o, const cm_fkp_opts *f, const void *pos,
+                                    const void *w, size_t n, const void *rpos, const void *rw, size_t nr, double *pk0,
+                                    double *pk2, double *pk4, int64_t *nmodes, double *pk_window,
+                                    cm_measure_info *info) {
+    if (!ctx) return CM_ERR_INVALID;
+    if (!o || !f || (!pos && n) || (!rpos && nr)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: null pointer");
+    if (!fft_size_ok(o->mas.N)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: N=%d not supported", o->mas.N);
+    if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure_fkp: float32 positions only");
+    if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: binning");
+    if (o->do_tweb || o->interlace)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure_fkp: the T-web and interlacing options must be off");
+    return measure_fkp_impl(ctx, o, f, pos, (const float *)w, n, rpos, (const float *)rw, nr, pk0, pk2, pk4, nmodes,
+                            pk_window, info);
+}

@@ -222,6 +222,31 @@ cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, con
                      size_t n, double *pk0, double *pk2, double *pk4, int64_t *nmodes,
                      int8_t *labels, cm_measure_info *info);
 
+/* ------------------------------------------------ FKP with a random catalogue */
+typedef struct {
+    double nbar;      /* mean number density n(r) of the sample (constant in the box);
+                         <= 0: W_g / L^3 */
+    double P_est;     /* FKP weight 1/(1 + nbar P_est) when fkp_weight (Params _Pest) */
+    int fkp_weight;   /* Params _FKP_weight */
+    int window;       /* also the window: shells of the randoms' power */
+} cm_fkp_opts;
+
+/* The random-catalogue branch of the estimator
+ * (FftwFunctions::get_parameters_estimator, Source/FftwFunctions.cpp:177-200;
+ * get_fluctuation, :275-281; per-object sums of Catalog::
+ * get_interpolated_density_field, Source/Catalog.cpp:14153-14171): weights
+ * w = w_user * w_fkp, W = sum w, S = sum w^2, normal = sum nbar w^2 per
+ * catalogue; alpha = W_g/W_r, delta = rho_g - alpha rho_r (one mesh: the randoms
+ * are deposited with weight -alpha w), normal_power = alpha normal_r,
+ * shot noise = (alpha + 1) S_r / normal_r.  pk_window (NULL unless
+ * f->window) receives the randoms' shells over normal_power/alpha^2
+ * (power_spectrum_fkp window branch, :1250-1262, :1635-1638).  Positions are
+ * device float32 AoS; weights may be NULL.  The T-web and interlacing options
+ * must be off. */
+cm_status cm_measure_fkp(cm_ctx *ctx, const cm_measure_opts *o, const cm_fkp_opts *f, const void *pos,
+                         const void *w, size_t n, const void *rpos, const void *rw, size_t nr, double *pk0,
+                         double *pk2, double *pk4, int64_t *nmodes, double *pk_window, cm_measure_info *info);
+
 /* ----------------------------------------------------------- synthetic mocks */
 typedef struct {
     int NL;          /* Lagrangian grid per side (an FFT size) */

@@ -1,0 +1,98 @@
+"""FKP estimator with a random catalogue (cm_measure_fkp) against the oracle
+pipeline, a closed form and the box estimator."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import zeldovich as zref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import cosmicatlas_amd as cm
+    c = cm.Context(0)
+    yield c
+    c.close()
+
+
+def _sq_table(N, scheme):
+    i = np.arange(N // 2 + 1)
+    s2 = np.sin(np.pi * i / N) ** 2
+    return [np.ones_like(s2), 1 - 2 / 3 * s2, 1 - s2 + 2 / 15 * s2 ** 2,
+            1 - 4 / 3 * s2 + 2 / 5 * s2 ** 2 - 4 / 315 * s2 ** 3][scheme]
+
+
+def test_randoms_equal_to_galaxies_closed_form(ctx, orc):
+    """Randoms at the galaxies' positions: alpha = 1 and delta = 0, so P0 is
+    minus the shot noise 2/nbar times the shell mean of prod Sq / W^2."""
+    import cosmicatlas_amd as cm
+    N, L, n = 32, 100.0, 30000
+    pos = zref.uniform(n, L, 3)
+    posd = torch.from_numpy(pos).cuda()
+    b = cm.binning(N, L)
+    out = ctx.measure_fkp(posd, posd, N, L, "CIC", b["nshells"], b["kmin"], b["dk"])
+    nbar = n / L ** 3
+    assert out["shot_noise"] == pytest.approx(2.0 / nbar, rel=1e-12)
+    assert out["normal_power"] == pytest.approx(n * nbar, rel=1e-12)
+    C = _sq_table(N, 1)
+    f = np.where(np.arange(N) <= N // 2, np.arange(N), N - np.arange(N))
+    X = C[f][:, None, None] * C[f][None, :, None] * C[None, None, :]
+    m = orc.pk_shells(np.sqrt(X), orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], 1, 0.0, 1.0))[0]
+    ref = -2.0 / nbar * m
+    assert np.allclose(out["pk0"], ref, rtol=1e-5, atol=0)
+    assert np.max(np.abs(out["pk2"])) < 1e-6 * 2.0 / nbar
+
+
+def test_fkp_matches_oracle(ctx, orc):
+    import cosmicatlas_amd as cm
+    N, L, scheme = 32, 200.0, 2
+    gal = zref.zeldovich(24, L, A=2e3, seed=5)
+    ran = zref.uniform(3 * len(gal), L, 6)
+    rng = np.random.default_rng(1)
+    wg = rng.uniform(0.5, 1.5, len(gal)).astype(np.float32)
+    wr = rng.uniform(0.5, 1.5, len(ran)).astype(np.float32)
+    nbar, P_est = 1.5e-3, 2e3
+    b = cm.binning(N, L)
+    out = ctx.measure_fkp(torch.from_numpy(gal).cuda(), torch.from_numpy(ran).cuda(), N, L, scheme, b["nshells"],
+                          b["kmin"], b["dk"], weights=torch.from_numpy(wg).cuda(),
+                          rweights=torch.from_numpy(wr).cuda(), nbar=nbar, P_est=P_est, fkp_weight=True, window=True)
+    wf = 1.0 / (1.0 + nbar * P_est)
+    Wg = wf * wg.astype(np.float64).sum()
+    Wr = wf * wr.astype(np.float64).sum()
+    Sr = wf ** 2 * (wr.astype(np.float64) ** 2).sum()
+    alpha = Wg / Wr
+    normal = alpha * nbar * Sr
+    SN = (alpha + 1) * Sr / (nbar * Sr)
+    mesh = (orc.mas(gal, N, L, scheme, weights=(wf * wg).astype(np.float32))
+            - alpha * orc.mas(ran, N, L, scheme, weights=(wf * wr).astype(np.float32)))
+    dk = orc.fft_r2c(mesh)
+    p0, p2, p4, nm = orc.pk_shells(dk, orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, SN, normal))
+    assert np.array_equal(out["nmodes"], nm)
+    assert out["shot_noise"] == pytest.approx(SN, rel=1e-9)
+    denom = np.abs(p0) + SN
+    assert np.max(np.abs(out["pk0"] - p0) / denom) < 1e-4
+    assert np.max(np.abs(out["pk2"] - p2) / denom) < 1e-4
+    assert np.max(np.abs(out["pk4"] - p4) / denom) < 1e-4
+    mr = orc.mas(ran, N, L, scheme, weights=(wf * wr).astype(np.float32))
+    dr = orc.fft_r2c(mr - mr.sum() / N ** 3)
+    pw = orc.pk_shells(dr, orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, 0.0, normal / alpha ** 2))[0]
+    assert np.allclose(out["pk_window"], pw, rtol=1e-4, atol=1e-6 * np.abs(pw).max())
+
+
+def test_fkp_tends_to_box_estimator(ctx):
+    """With many more randoms than galaxies the FKP estimate of a periodic box
+    approaches the box estimator on shells where the signal dominates."""
+    import cosmicatlas_amd as cm
+    N, L = 64, 300.0
+    gal = torch.from_numpy(zref.zeldovich(32, L, A=3e3, seed=8)).cuda()
+    ran = ctx.uniform(40 * gal.shape[0], L, seed=9)
+    b = cm.binning(N, L)
+    fk = ctx.measure_fkp(gal, ran, N, L, "TSC", b["nshells"], b["kmin"], b["dk"])
+    box = ctx.measure(gal, N, L, "TSC", b["nshells"], b["kmin"], b["dk"])
+    SN = L ** 3 / gal.shape[0]
+    sig = box["pk0"] > 5 * SN
+    assert sig.sum() >= 3
+    rel = np.abs(fk["pk0"][sig] - box["pk0"][sig]) / box["pk0"][sig]
+    assert rel.max() < 0.15, rel
