@@ -237,6 +237,31 @@ class Context:
                                             pw.ctypes.data_as(C.c_void_p) if window else None, C.byref(info)))
         return _fkp_result(p0, p2, p4, nm, pw if window else None, info)
 
+    def bispectrum(self, dk, N, L, nshells, kmin, dk_bin, scheme, n_obj, shell_min=0, mas_correction=True,
+                   sn_correction=True):
+        """Bispectrum of the triangle bins from a complex64 half spectrum; returns
+        dict with ijk (nbins,3), bk, sn, ntri."""
+        o = _abi.BkOpts(N, float(L), nshells, float(kmin), float(dk_bin), shell_min, _scheme(scheme),
+                        int(mas_correction), int(sn_correction), float(n_obj))
+        cap = 8 * 256
+        ijk = np.zeros((cap, 3), np.int32)
+        bk, sn, nt = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        nb = C.c_int()
+        self._check(self.lib.cm_bispectrum(self.h, C.byref(o), _ptr(dk), cap, ijk.ctypes.data_as(C.c_void_p),
+                                           bk.ctypes.data_as(C.c_void_p), sn.ctypes.data_as(C.c_void_p),
+                                           nt.ctypes.data_as(C.c_void_p), C.byref(nb)))
+        n = nb.value
+        return {"ijk": ijk[:n], "bk": bk[:n], "sn": sn[:n], "ntri": nt[:n]}
+
+    def tweb_tclass(self, labels, cwt_used, out=None):
+        """Cwclass::get_Tclassification for every cell: int32 tensor like labels."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty(labels.shape, dtype=torch.int32, device=labels.device)
+        codes = (C.c_int * len(cwt_used))(*[int(c) for c in cwt_used])
+        self._check(self.lib.cm_tweb_tclass(self.h, _ptr(labels), labels.numel(), codes, len(cwt_used), _ptr(out)))
+        return out
+
     # ------------------------------------------------------- synthetic mocks
     def zeldovich(self, NL, L, A=1e4, ns=-1.5, seed=1, out=None):
         """NL^3 Zel'dovich particles (float32, (NL^3, 3), device) from a seeded

@@ -110,6 +110,21 @@ class FkpOpts(C.Structure):
     ]
 
 
+class BkOpts(C.Structure):
+    _fields_ = [
+        ("N", C.c_int),
+        ("L", C.c_double),
+        ("nshells", C.c_int),
+        ("kmin", C.c_double),
+        ("dk", C.c_double),
+        ("shell_min", C.c_int),
+        ("scheme", C.c_int),
+        ("mas_correction", C.c_int),
+        ("sn_correction", C.c_int),
+        ("n_obj", C.c_double),
+    ]
+
+
 class Request(C.Structure):
     _fields_ = [
         ("opts", MeasureOpts),
@@ -161,6 +176,8 @@ def _declare(L):
         "cm_measure": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, P, P, P, P,
                             C.POINTER(MeasureInfo)]),
         "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),
+        "cm_tweb_tclass": (st, [P, P, C.c_size_t, C.POINTER(C.c_int), C.c_int, P]),
+        "cm_bispectrum": (st, [P, C.POINTER(BkOpts), P, C.c_int, P, P, P, P, C.POINTER(C.c_int)]),
         "cm_measure_fkp": (st, [P, C.POINTER(MeasureOpts), C.POINTER(FkpOpts), P, P, C.c_size_t, P, P, C.c_size_t,
                                 P, P, P, P, P, C.POINTER(MeasureInfo)]),
         "cm_request_footprint": (C.c_size_t, [C.POINTER(Request)]),

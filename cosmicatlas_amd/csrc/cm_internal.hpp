@@ -50,6 +50,7 @@ struct cm_ctx {
     // e.g. CM_TUNE="xnb=8,zrows=16,ty=16,tz=64,tseg=4,colnb=16"
     struct Tune {
         int xnb = 0;    // kz columns per workgroup in the fused x pass
+        int xe = 0;     // elements per thread in the fused x pass (8: twice the threads)
         int colnb = 0;  // kz columns per workgroup in the y/x column passes
         int zrows = 0;  // rows per workgroup in the z passes
         int ty = 0, tz = 0, tseg = 0;  // T-web tile and x segments

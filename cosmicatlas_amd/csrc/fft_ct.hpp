@@ -67,10 +67,10 @@ constexpr int ct_row_stride(int M) { return M + 1 + ((M + 1) >> 3) + 1; }
 // per-element rows on distinct banks); rows: b*STRIDE + rpad(e).
 // The first stage reads through src(b, e), the last writes through
 // dst(b, e, v); the caller owns the barrier before the first stage.
-template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int S, int Ns, class Src, class Dst>
+template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int S, int Ns, int E, class Src, class Dst>
 __device__ __forceinline__ void ct_stage(v2<T> *lds, const v2<T> *__restrict__ tw, const Src &src, const Dst &dst) {
     using C = v2<T>;
-    constexpr int E = ct_E(n), R = ct_radix(n, S), NST = ct_nst(n);
+    constexpr int R = ct_radix(n, S), NST = ct_nst(n);
     constexpr bool FIRST = S == 0, LAST = S == NST - 1;
     constexpr int IT = E / R, nR = n / R, NT = NB * n / E, tws = n / (Ns * R);
     static_assert(E % R == 0 && NT * E == NB * n, "plan does not tile the workgroup");
@@ -121,13 +121,15 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds, const v2<T> *__restrict__ t
     }
     if constexpr (!LAST) {
         __syncthreads();
-        ct_stage<T, n, NB, STRIDE, COLS, FWD, S + 1, Ns * R>(lds, tw, src, dst);
+        ct_stage<T, n, NB, STRIDE, COLS, FWD, S + 1, Ns * R, E>(lds, tw, src, dst);
     }
 }
 
-template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, class Src, class Dst>
+// E: elements per thread (ct_E(n) by default; any multiple of every radix of
+// the plan), so the workgroup has NB*n/E threads.
+template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int E = ct_E(n), class Src, class Dst>
 __device__ __forceinline__ void ct_fft(v2<T> *lds, const v2<T> *__restrict__ tw, const Src &src, const Dst &dst) {
-    ct_stage<T, n, NB, STRIDE, COLS, FWD, 0, 1>(lds, tw, src, dst);
+    ct_stage<T, n, NB, STRIDE, COLS, FWD, 0, 1, E>(lds, tw, src, dst);
 }
 
 // Workgroup shapes of the three passes for a float32 N^3 transform.
@@ -150,12 +152,12 @@ template <int N> struct RowShape {
 };
 
 // y / x columns (length N): NB adjacent kz columns per workgroup.
-template <int N, int NBC> struct ColShape {
-    static constexpr int E = ct_E(N);
+template <int N, int NBC, int EC = ct_E(N)> struct ColShape {
+    static constexpr int E = EC;
     static constexpr int NB = NBC;
     static constexpr int NT = NB * N / E;
     static constexpr size_t LDS = sizeof(float2) * (size_t)NB * N;
-    static_assert(NT <= 512, "column workgroup too large");
+    static_assert(NT <= 1024, "column workgroup too large");
 };
 
 // kz row pitch (complex values) of the internal half spectrum: N/2+1 rounded

@@ -27,7 +27,7 @@ namespace cm {
 // The two FFT phases of a tile live in separate non-inlined functions so that
 // each gets its own register allocation (inlined together with the shell code
 // they exceed the VGPR budget and spill).
-template <int N, int NB>
+template <int N, int NB, int E>
 __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restrict__ data, size_t base,
                                                             size_t xstride, int valid, float2 *lds,
                                                             const float2 *__restrict__ tw) {
@@ -35,10 +35,10 @@ __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restr
         return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
     };
     auto dstL = [&](int b, int e, float2 v) { lds[e * (NB + 1) + b] = v; };
-    ct_fft<float, N, NB, NB + 1, true, true>(lds, tw, srcG, dstL);
+    ct_fft<float, N, NB, NB + 1, true, true, E>(lds, tw, srcG, dstL);
 }
 
-template <int N, int NB>
+template <int N, int NB, int E>
 __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict__ data, size_t base, size_t xstride,
                                                               int valid, float2 *lds,
                                                               const float2 *__restrict__ tw) {
@@ -46,7 +46,7 @@ __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict_
     auto dstG = [&](int b, int e, float2 v) {
         if (b < valid) data[base + (size_t)e * xstride + b] = v;
     };
-    ct_fft<float, N, NB, NB + 1, true, false>(lds, tw, srcL, dstG);
+    ct_fft<float, N, NB, NB + 1, true, false, E>(lds, tw, srcL, dstG);
 }
 
 // Shell accumulator of one thread: the modes a thread visits along kz have
@@ -69,8 +69,8 @@ struct ShellAcc {
     }
 };
 
-template <int N, int NB, bool IL>
-__global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__restrict__ data,
+template <int N, int NB, bool IL, int E>
+__global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__restrict__ data,
                                                  const float2 *__restrict__ dataB, int pitch,
                                                  const float2 *__restrict__ tw, PkK pk,
                                                  const double *__restrict__ tabs, float wc, float wa,
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
     float *s_sqic = s_al + M1;                                                // M1
     float *s_alic = s_sqic + M1;                                              // M1
     float *s_g = s_alic + M1;                                                 // N
-    constexpr int NT = ColShape<N, NB>::NT;
+    constexpr int NT = ColShape<N, NB, E>::NT;
     const int ncols = N / 2 + 1;
     const int tid = threadIdx.x;
     for (int i = tid; i < M1; i += NT) {
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
         const int kz0 = kzb * NB;
         const size_t base = (size_t)iy * pitch + kz0;
         const int valid = min(NB, ncols - kz0);
-        x_fft_fwd_to_lds<N, NB>(data, base, xstride, valid, lds, tw);
+        x_fft_fwd_to_lds<N, NB, E>(data, base, xstride, valid, lds, tw);
         __syncthreads();
         if constexpr (IL) {
             // interlaced meshes: average with the second mesh's finished
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_x_fused(float2 *__res
             acc.flush(s_acc, s_cnt, ns);
         }
         __syncthreads();
-        if (do_tweb) x_fft_inv_from_lds<N, NB>(data, base, xstride, valid, lds, tw);
+        if (do_tweb) x_fft_inv_from_lds<N, NB, E>(data, base, xstride, valid, lds, tw);
         __syncthreads();
     }
     for (int i = tid; i < ns; i += NT) {
@@ -249,17 +249,17 @@ PkK make_pkk(const cm_pk_opts *o);
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
                             float *lam, unsigned long long *stats);
 
-template <int N, int NB>
+template <int N, int NB, int E>
 static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pitch,
                              const float2 *tw, const PkK &pk, const double *dtab, double *dsum,
                              unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
                              const double *dnobj, int ns) {
-    constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB>::NT;
+    constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB, E>::NT;
     const size_t lds = sizeof(float2) * (size_t)(NB + 1) * N + sizeof(double) * 4 * (size_t)ns +
                        sizeof(unsigned) * (size_t)ns + sizeof(float) * (5 * (size_t)M1 + (size_t)N);
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
-    auto kern = dkB ? k_x_fused<N, NB, true> : k_x_fused<N, NB, false>;
+    auto kern = dkB ? k_x_fused<N, NB, true, E> : k_x_fused<N, NB, false, E>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1) per_cu = 1;
     const int ntiles = ((M1 + NB - 1) / NB) * N;
@@ -270,17 +270,31 @@ static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pit
 }
 
 template <int N>
-static cm_status x_fused_launch(cm_ctx *ctx, int nb, float2 *dk, const float2 *dkB, int pitch,
+static cm_status x_fused_launch(cm_ctx *ctx, int nb, int xe, float2 *dk, const float2 *dkB, int pitch,
                                 const float2 *tw, const PkK &pk, const double *dtab, double *dsum,
                                 unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
                                 const double *dnobj, int ns) {
-    if constexpr (ColShape<N, 8>::NT * 2 <= 512) {
-        if (nb == 16)
-            return x_fused_run<N, 16>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss,
-                                      dnobj, ns);
+    constexpr int E0 = ct_E(N);
+    // E = 8 (twice the threads) where the plan allows it: radix 8/4/2 only
+    if constexpr (E0 == 16) {
+        if (xe == 8) {
+            if constexpr (16 * N / 8 <= 1024) {
+                if (nb == 16)
+                    return x_fused_run<N, 16, 8>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2,
+                                                 dgauss, dnobj, ns);
+            }
+            if constexpr (8 * N / 8 <= 1024)
+                return x_fused_run<N, 8, 8>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss,
+                                            dnobj, ns);
+        }
     }
-    return x_fused_run<N, 8>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj,
-                             ns);
+    if constexpr (ColShape<N, 8, E0>::NT * 2 <= 512) {
+        if (nb == 16)
+            return x_fused_run<N, 16, E0>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss,
+                                          dnobj, ns);
+    }
+    return x_fused_run<N, 8, E0>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj,
+                                 ns);
 }
 
 // Fused measurement: mesh (float32, device) -> shells (+ labels).
@@ -368,7 +382,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     switch (N) {
 #define CM_CASE(n)                                                                                                     \
     case n:                                                                                                            \
-        s = x_fused_launch<n>(ctx, nb, (float2 *)dk, (const float2 *)dkB, pitch,                                   \
+        s = x_fused_launch<n>(ctx, nb, ctx->tune.xe, (float2 *)dk, (const float2 *)dkB, pitch,                     \
                               (const float2 *)tw, pk, (const double *)dtab,                                            \
                               (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss,     \
                               (const double *)dnobj, ns);                                                              \

@@ -438,3 +438,61 @@ cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ label combos
+namespace cm {
+
+// 4-bit class mask of a combo code (bit c-1 for class c); 0: matches nothing
+static unsigned combo_mask(int code) {
+    switch (code) {
+    case 1: return 1u;
+    case 2: return 2u;
+    case 3: return 4u;
+    case 4: return 8u;
+    case 12: return 3u;
+    case 23: return 6u;
+    case 34: return 12u;
+    case 123: return 7u;
+    case 234: return 14u;
+    }
+    return 0u;
+}
+
+struct ComboTab {
+    unsigned mask[16];
+};
+
+__global__ void __launch_bounds__(256) k_tclass(const int8_t *__restrict__ labels, size_t n, ComboTab tab, int ncwt,
+                                                int single, int32_t *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (single) {
+            out[i] = single;
+            continue;
+        }
+        const int lab = labels[i];
+        const unsigned bit = (lab >= 1 && lab <= 4) ? (1u << (lab - 1)) : 0u;
+        int ans = 0;
+        for (int c = 0; c < ncwt; ++c)
+            if (tab.mask[c] & bit) ans = c;
+        out[i] = ans;
+    }
+}
+
+}  // namespace cm
+
+extern "C" cm_status cm_tweb_tclass(cm_ctx *ctx, const int8_t *labels, size_t n, const int *cwt_used, int ncwt,
+                                    int32_t *out) {
+    if (!ctx) return CM_ERR_INVALID;
+    if (!cwt_used || ncwt < 1 || ncwt > 16) return cm::fail(ctx, CM_ERR_INVALID, "cm_tweb_tclass: 1..16 codes");
+    if (n && (!labels || !out)) return cm::fail(ctx, CM_ERR_INVALID, "cm_tweb_tclass: null pointer");
+    if (!n) return CM_OK;
+    cm::ComboTab tab{};
+    for (int c = 0; c < ncwt; ++c) tab.mask[c] = cm::combo_mask(cwt_used[c]);
+    // one code: every cell gets the code itself (the reference returns
+    // cwt_used[0]; for code 0 the loop below also gives 0)
+    const int single = ncwt == 1 ? cwt_used[0] : 0;
+    size_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    return cm::launch(ctx, "tweb_tclass", cm::k_tclass, dim3((unsigned)g), dim3(256), 0, labels, n, tab, ncwt, single,
+                      out);
+}

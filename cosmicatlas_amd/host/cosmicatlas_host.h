@@ -197,6 +197,11 @@ class Cwclass {
     void set_params(const Params &p) { params = p; }
     // delta = overdensity; smoothing R = smscale (kernelcomp, applied here)
     void get_CWC(std::vector<real_prec> &delta);
+    // combos of classes requested by the parameter file (t_CWT_used): codes
+    // 1..4, 12, 23, 34, 123, 234 (Source/Cwclass.cpp:1925-2030)
+    std::vector<int> cwt_used;
+    bool get_cell_classified(int sua, ULONG ig) const;
+    int get_Tclassification(ULONG ig) const;
     std::vector<int> CWClass;
     std::vector<real_prec> lambda1, lambda2, lambda3;
     real_prec volume_knots = 0, volume_filaments = 0, volume_sheets = 0, volume_voids = 0;

@@ -282,6 +282,42 @@ void EigenValuesTweb(ULONG Nft, real_prec L1, std::vector<real_prec> &, std::vec
     cmh::d2h(out3.data(), (char *)dlam.p + 2 * sizeof(real_prec) * n3, sizeof(real_prec) * n3);
 }
 
+// classes of a combo code as a 4-bit mask (bit c-1 for class c)
+static unsigned combo_classes(int code) {
+    switch (code) {
+    case 1: return 1u;
+    case 2: return 2u;
+    case 3: return 4u;
+    case 4: return 8u;
+    case 12: return 3u;
+    case 23: return 6u;
+    case 34: return 12u;
+    case 123: return 7u;
+    case 234: return 14u;
+    }
+    return 0u;
+}
+
+// Source/Cwclass.cpp:1925-1968: code 0 (or a list {0}) takes every cell
+bool Cwclass::get_cell_classified(int sua, ULONG ig) const {
+    if (cwt_used.size() == 1 && cwt_used[0] == 0) return true;
+    const int code = cwt_used[(size_t)sua];
+    if (code == 0) return true;
+    const int lab = CWClass[ig];
+    return lab >= 1 && lab <= 4 && (combo_classes(code) >> (lab - 1) & 1u);
+}
+
+// Source/Cwclass.cpp:1983-2030: one code -> the code; several -> index of the
+// last code containing the cell's class, else 0
+int Cwclass::get_Tclassification(ULONG ig) const {
+    if (cwt_used.size() == 1) return cwt_used[0];
+    int ans = 0;
+    const int lab = CWClass[ig];
+    for (size_t ic = 0; ic < cwt_used.size(); ++ic)
+        if (lab >= 1 && lab <= 4 && (combo_classes(cwt_used[ic]) >> (lab - 1) & 1u)) ans = (int)ic;
+    return ans;
+}
+
 void Cwclass::get_CWC(std::vector<real_prec> &delta) {
     const ULONG N = params._Nft();
     const size_t n3 = (size_t)N * N * N;

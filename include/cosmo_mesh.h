@@ -187,6 +187,15 @@ cm_status cm_poisson(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, flo
 cm_status cm_tweb_from_phi(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi,
                            int8_t *labels, float *lambdas, cm_tweb_stats *stats);
 
+/* Per-cell index of the classification combos requested by the parameter file
+ * (Cwclass::get_Tclassification, Source/Cwclass.cpp:1983-2030): cwt_used lists
+ * codes 1..4 (one class: 1 knot, 2 filament, 3 sheet, 4 void) or the combos
+ * 12, 23, 34, 123, 234.  With one code, every cell gets that code; with
+ * several, a cell gets the index of the last code whose classes contain its
+ * label, else 0 (codes outside this list, 0 included, match no cell).  labels:
+ * device int8[n]; out: device int32[n]. */
+cm_status cm_tweb_tclass(cm_ctx *ctx, const int8_t *labels, size_t n, const int *cwt_used, int ncwt, int32_t *out);
+
 /* ------------------------------------------------- whole-mock measurement */
 typedef struct {
     cm_mas_desc mas;     /* positions -> mesh */
@@ -246,6 +255,28 @@ typedef struct {
 cm_status cm_measure_fkp(cm_ctx *ctx, const cm_measure_opts *o, const cm_fkp_opts *f, const void *pos,
                          const void *w, size_t n, const void *rpos, const void *rw, size_t nr, double *pk0,
                          double *pk2, double *pk4, int64_t *nmodes, double *pk_window, cm_measure_info *info);
+
+/* ------------------------------------------------------------- bispectrum */
+typedef struct {
+    int N;
+    double L;
+    int nshells;      /* Nshells_bk: shells [kmin + s dk, kmin + (s+1) dk) */
+    double kmin, dk;  /* d_kmin, DeltaK_Bis; the last edge must lie below N/3 k_f */
+    int shell_min;    /* first shell of the triangle loop (Nshell_min) */
+    cm_mas scheme;
+    int mas_correction, sn_correction;
+    double n_obj;     /* objects in the box: normal_b = n^3/V^2, nbar = n/V */
+} cm_bk_opts;
+
+/* Bispectrum of a box (FftwFunctions::get_bispectrum_fkp_fast and
+ * loop_shells_bispectrum, Source/FftwFunctions.cpp:3358-3718, 4878) from the
+ * device half spectrum dk (complex64, FFTW layout) of the fluctuation: for each
+ * triangle bin (i >= j >= k, the reference's loop) ijk[3b..3b+2], the
+ * bispectrum bk[b] with the Poisson shot noise sn[b] = <P_i+P_j+P_k>/nbar +
+ * 1/nbar^2 subtracted (when sn_correction), and the number of closed ordered
+ * triangles ntri[b].  *nbins receives the bin count; the arrays hold max_bins. */
+cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void *dk, int max_bins, int32_t *ijk,
+                        double *bk, double *sn, double *ntri, int *nbins);
 
 /* ----------------------------------------------------------- synthetic mocks */
 typedef struct {

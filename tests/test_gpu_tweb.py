@@ -59,3 +59,30 @@ def test_poisson_matches_oracle(ctx, orc):
     ref = orc.fft_c2r(orc.smooth_poisson(dk, N, L, R), N)
     phi = ctx.poisson(torch.from_numpy(delta.astype(np.float32)).cuda(), L, R).cpu().numpy()
     assert np.max(np.abs(phi - ref)) < 1e-5 * np.abs(ref).max()
+
+
+def _tclass_ref(labels, codes):
+    """get_Tclassification (Source/Cwclass.cpp:1983-2030), restated per cell."""
+    combos = {12: (1, 2), 23: (2, 3), 34: (3, 4), 123: (1, 2, 3), 234: (2, 3, 4)}
+    if len(codes) == 1:
+        return np.full(labels.shape, codes[0], np.int32)
+    out = np.zeros(labels.shape, np.int32)
+    for ic, c in enumerate(codes):
+        hit = (labels == c) if c < 5 else np.isin(labels, combos.get(c, ()))
+        out[hit] = ic
+    return out
+
+
+@pytest.mark.parametrize("codes", [[234], [0], [1, 234], [12, 34], [4, 3, 2, 1], [123, 4], [0, 2], [13, 2]])
+def test_tweb_tclass(codes):
+    import cosmicatlas_amd as cm
+    rng = np.random.default_rng(len(codes))
+    lab = rng.integers(1, 5, 5000).astype(np.int8)
+    with cm.Context(0) as ctx:
+        out = ctx.tweb_tclass(torch.from_numpy(lab).cuda(), codes).cpu().numpy()
+    assert np.array_equal(out, _tclass_ref(lab, codes))
+    # hand-checked cells: knot (1), filament (2), sheet (3), void (4)
+    if codes == [1, 234]:
+        assert [int(_tclass_ref(np.array([v], np.int8), codes)[0]) for v in (1, 2, 3, 4)] == [0, 1, 1, 1]
+    if codes == [12, 34]:
+        assert [int(_tclass_ref(np.array([v], np.int8), codes)[0]) for v in (1, 2, 3, 4)] == [0, 0, 1, 1]
