@@ -94,6 +94,12 @@ class Params {
     double _d_Deltal() const { return Deltal; }
     bool _float_shell_rule() const { return shell_rule == "float"; }
     bool _compute_tweb() const { return compute_tweb; }
+    std::string _file_bin_x_coord() const { return file_bin_x_coord; }
+    std::string _file_bin_y_coord() const { return file_bin_y_coord; }
+    std::string _file_bin_z_coord() const { return file_bin_z_coord; }
+    ULONG _N_lines_binary() const { return N_lines_binary; }
+    int _sys_of_coord_g() const { return sys_of_coord_g; }
+    std::string _angles_units_g() const { return angles_units_g; }
 
    private:
     std::map<std::string, std::string> raw;
@@ -112,6 +118,15 @@ class Params {
     std::string file_catalogue = "cat.dat", Input_dir_cat = "";
     bool use_random_catalog = false, measure_cross = false;
     int i_coord1_g = 0, i_coord2_g = 1, i_coord3_g = 2, i_weight1_g = -1;
+    // binary SoA catalogues (Catalog::read_catalog_bin, Source/Catalog.cpp:560):
+    // one headerless real_prec file per coordinate
+    std::string file_bin_x_coord, file_bin_y_coord, file_bin_z_coord;
+    ULONG N_lines_binary = 0;
+    // coordinate system of the catalogue: 0 Cartesian, 2 equatorial (RA, Dec,
+    // comoving distance) converted by equatorial_to_cartesian
+    // (Source/CoordinateSystem.cpp:12-18); angles in degrees ("D") or radians ("R")
+    int sys_of_coord_g = 0;
+    std::string angles_units_g = "D";
     real_prec Xoffset = 0, Yoffset = 0, Zoffset = 0;
     double delta_x = 0, deltak_x = 0, deltak_0 = 0, kmin = 0, kmax = 0, DeltaK_data = 0, Deltal = 0;
     int Nnp_data = 0;

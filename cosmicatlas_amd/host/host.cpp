@@ -108,6 +108,12 @@ void Params::read_pars(const std::string &file) {
         else if (name == "i_coord2_g") i_coord2_g = std::atoi(value.c_str());
         else if (name == "i_coord3_g") i_coord3_g = std::atoi(value.c_str());
         else if (name == "i_weight1_g") i_weight1_g = std::atoi(value.c_str());
+        else if (name == "file_bin_x_coord") file_bin_x_coord = value;
+        else if (name == "file_bin_y_coord") file_bin_y_coord = value;
+        else if (name == "file_bin_z_coord") file_bin_z_coord = value;
+        else if (name == "N_lines_binary") N_lines_binary = std::strtoul(value.c_str(), nullptr, 10);
+        else if (name == "sys_of_coord_g") sys_of_coord_g = std::atoi(value.c_str());
+        else if (name == "angles_units_g") angles_units_g = value;
         else if (name == "shell_rule") {
             if (value != "exact" && value != "float") throw cmh::Error("shell_rule must be exact or float");
             shell_rule = value;
@@ -449,7 +455,70 @@ void write_to_file2(const std::string &fname, const std::vector<real_prec> &kve,
     for (size_t i = first; i < kve.size(); ++i) out << kve[i] << "\t" << pk[i] << "\t" << si[i] << "\n";
 }
 
+// equatorial_to_cartesian (Source/CoordinateSystem.cpp:12-18) with the angle
+// units of Catalog::ang_to_cart_coordinates (Source/Catalog.cpp:13640-13652)
+static void equatorial_to_cartesian(double ra, double dec, double r, double &x, double &y, double &z) {
+    const double fac = M_PI / 180.0;
+    x = r * std::cos(ra * fac) * std::sin(0.5 * M_PI - dec * fac);
+    y = r * std::sin(ra * fac) * std::sin(0.5 * M_PI - dec * fac);
+    z = r * std::cos(0.5 * M_PI - dec * fac);
+}
+
+static void to_cartesian(const Params &p, std::vector<real_prec> &xyz) {
+    switch (p._sys_of_coord_g()) {
+    case 0: return;
+    case 2: {
+        const double fac = p._angles_units_g() == "R" ? 180.0 / M_PI : 1.0;  // radians -> degrees
+        for (size_t i = 0; i + 2 < xyz.size(); i += 3) {
+            double x, y, z;
+            equatorial_to_cartesian(xyz[i] * fac, xyz[i + 1] * fac, xyz[i + 2], x, y, z);
+            xyz[i] = (real_prec)x;
+            xyz[i + 1] = (real_prec)y;
+            xyz[i + 2] = (real_prec)z;
+        }
+        return;
+    }
+    case 1: throw cmh::Error("sys_of_coord_g = 1 (RA, Dec, redshift) needs the distance-redshift relation, not built");
+    }
+    throw cmh::Error("unknown sys_of_coord_g = " + std::to_string(p._sys_of_coord_g()));
+}
+
+// three headerless real_prec files x, y, z (N_lines_binary values each, or the
+// whole file when 0)
+static void read_catalog_bin(const Params &p, std::vector<real_prec> &xyz) {
+    std::vector<real_prec> c[3];
+    const std::string f[3] = {p._file_bin_x_coord(), p._file_bin_y_coord(), p._file_bin_z_coord()};
+    for (int a = 0; a < 3; ++a) {
+        std::ifstream in(f[a].c_str(), std::ios::binary | std::ios::ate);
+        if (!in) throw cmh::Error("File " + f[a] + " not found");
+        const size_t bytes = (size_t)in.tellg();
+        size_t n = bytes / sizeof(real_prec);
+        if (p._N_lines_binary() > 0) {
+            if (p._N_lines_binary() > n) throw cmh::Error("File " + f[a] + " holds fewer than N_lines_binary values");
+            n = p._N_lines_binary();
+        }
+        in.seekg(0);
+        c[a].resize(n);
+        in.read(reinterpret_cast<char *>(c[a].data()), (std::streamsize)(n * sizeof(real_prec)));
+    }
+    if (c[1].size() != c[0].size() || c[2].size() != c[0].size())
+        throw cmh::Error("binary coordinate files of different lengths");
+    xyz.resize(3 * c[0].size());
+    for (size_t i = 0; i < c[0].size(); ++i) {
+        xyz[3 * i] = c[0][i];
+        xyz[3 * i + 1] = c[1][i];
+        xyz[3 * i + 2] = c[2][i];
+    }
+}
+
 void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w) {
+    xyz.clear();
+    w.clear();
+    if (!p._file_bin_x_coord().empty()) {
+        read_catalog_bin(p, xyz);
+        to_cartesian(p, xyz);
+        return;
+    }
     std::ifstream f(p._file_catalogue().c_str());
     if (!f) throw cmh::Error("File " + p._file_catalogue() + " not found");
     const int c[3] = {p._i_coord1_g(), p._i_coord2_g(), p._i_coord3_g()};
@@ -469,6 +538,7 @@ void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real
         for (int a = 0; a < 3; ++a) xyz.push_back((real_prec)col[c[a]]);
         if (cw >= 0) w.push_back((real_prec)col[cw]);
     }
+    to_cartesian(p, xyz);
 }
 
 // ============================================================== PowerSpectrumF

@@ -114,3 +114,61 @@ def test_cli_density_grid(tmp_path, orc):
     first = 1 if p0[0] < 1 else 0
     assert np.array_equal(out[:, 2].astype(int), nm[first:])
     assert np.max(np.abs(out[:, 1] - p0[first:]) / (np.abs(p0[first:]) + L ** 3 / ng)) < 1e-4
+
+
+def _run_pk(tmp_path, ini, N):
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return np.loadtxt(tmp_path / f"Pk_fkp_BOX_Nft{N}_TSC_test.txt")
+
+
+def _oracle_pk(orc, pos32, N, L):
+    n = len(pos32)
+    mesh = orc.mas(pos32, N, L, orc.TSC)
+    V = L ** 3
+    kf = 2 * np.pi / L
+    dk = orc.fft_r2c(mesh - n / N ** 3)
+    p0, _, _, nm = orc.pk_shells(dk, orc.pk_opts(N, L, N // 2, kf, kf, orc.TSC, V / n, n * n / V))
+    first = 1 if p0[0] < 1 else 0
+    return p0[first:], nm[first:], V / n
+
+
+@pytest.mark.gpu
+def test_cli_binary_catalogue(tmp_path, orc):
+    """Catalog::read_catalog_bin: three headerless float32 coordinate files."""
+    N, L, n = 64, 500.0, 60000
+    pos = np.random.default_rng(22).uniform(0, L, (n + 500, 3)).astype(np.float32)
+    for a, ax in enumerate("xyz"):
+        pos[:, a].tofile(tmp_path / f"{ax}.dat")
+    ini = _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", N_lines_binary=str(n),
+               file_bin_x_coord=str(tmp_path / "x.dat"), file_bin_y_coord=str(tmp_path / "y.dat"),
+               file_bin_z_coord=str(tmp_path / "z.dat"))
+    out = _run_pk(tmp_path, ini, N)
+    p0, nm, sn = _oracle_pk(orc, pos[:n], N, L)
+    assert np.array_equal(out[:, 2].astype(int), nm)
+    assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
+
+
+@pytest.mark.gpu
+def test_cli_equatorial_catalogue(tmp_path, orc):
+    """Equatorial (RA, Dec, distance) columns in degrees and in radians give the
+    Cartesian measurement (equatorial_to_cartesian, CoordinateSystem.cpp:12-18)."""
+    N, L, n = 64, 500.0, 60000
+    rng = np.random.default_rng(23)
+    xyz = rng.uniform(20.0, 480.0, (n, 3))
+    r = np.sqrt((xyz ** 2).sum(1))
+    dec = np.degrees(np.arcsin(xyz[:, 2] / r))
+    ra = np.degrees(np.arctan2(xyz[:, 1], xyz[:, 0]))
+    np.savetxt(tmp_path / "cat.txt", np.stack([ra, dec, r], 1), fmt="%.12f")
+    ini = _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", sys_of_coord_g="2", angles_units_g="D")
+    deg = _run_pk(tmp_path, ini, N)
+    np.savetxt(tmp_path / "cat.txt", np.stack([np.radians(ra), np.radians(dec), r], 1), fmt="%.14f")
+    ini = _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", sys_of_coord_g="2", angles_units_g="R")
+    rad = _run_pk(tmp_path, ini, N)
+    p0, nm, sn = _oracle_pk(orc, xyz.astype(np.float32), N, L)
+    for out in (deg, rad):
+        assert np.array_equal(out[:, 2].astype(int), nm)
+        assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
+    bad = _ini(tmp_path, Nft=str(N), sys_of_coord_g="1")
+    rr = subprocess.run([CLI, "-p", bad], capture_output=True, text=True, timeout=120)
+    assert rr.returncode == 1 and "redshift" in rr.stderr + rr.stdout
