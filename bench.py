@@ -75,7 +75,7 @@ ROCPROF_NAMES = [("cm::k_tile_count_lds", "mas_tile_count"), ("cm::k_tile_count"
 
 def lib_name(kname):
     k = kname.replace("void ", "", 1)
-    if k.startswith("cm::k_cols<"):
+    if k.startswith("cm::k_cols<") or k.startswith("cm::k_cols_ct<"):
         return "fft_y_fwd" if ", true>" in k.split("(")[0] else "fft_y_inv"
     for pre, name in ROCPROF_NAMES:
         if k.startswith(pre):

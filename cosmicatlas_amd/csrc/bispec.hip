@@ -178,13 +178,11 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
     const size_t lds = sizeof(float) * 3 * (size_t)ns * kBkChunk;
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum: %d shells need %zu B LDS", ns, lds);
     cm_status s;
-    void *fields, *specs, *dtri, *part, *dsum, *dtab;
+    void *fields, *specs, *dtri, *part, *dsum;
     if ((s = scratch(ctx, "bk.fields", sizeof(float) * 3 * (size_t)ns * n3, &fields)) != CM_OK) return s;
     if ((s = scratch(ctx, "bk.specs", sizeof(float2) * 3 * nh, &specs)) != CM_OK) return s;
-    std::vector<double> tab;
-    pk_tables(N, (int)o->scheme, o->mas_correction, tab);
-    if ((s = scratch(ctx, "bk.tab", sizeof(double) * tab.size(), &dtab)) != CM_OK) return s;
-    CM_HIP(ctx, hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    const double *dtab = pk_tables_dev(ctx, N, (int)o->scheme, o->mas_correction, &s);
+    if (!dtab) return s;
     const double L = o->L, V = L * L * L, kf = 2.0 * M_PI / L;
     const double n = o->n_obj, normal = n * n / V;
     const double Dmin = o->kmin / kf, ndel = o->dk / kf;
@@ -205,7 +203,7 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
     if (g > 16384) g = 16384;
     for (int sh = 0; sh < ns; ++sh) {
         if ((s = launch(ctx, "bk_masks", k_bk_masks, dim3((unsigned)g), dim3(256), 0, (const float2 *)dk, N, sh,
-                        int_rule, (int)Di, (int)ndi, Dmin, ndel, (const double *)dtab, sn_norm, 1.0 / normal, sp,
+                        int_rule, (int)Di, (int)ndi, Dmin, ndel, dtab, sn_norm, 1.0 / normal, sp,
                         sp + nh, sp + 2 * nh)) != CM_OK)
             return s;
         for (int f = 0; f < 3; ++f)

@@ -95,6 +95,54 @@ cm_status scratch(cm_ctx *ctx, const char *name, size_t bytes, void **out) {
     return CM_OK;
 }
 
+const void *find_table(cm_ctx *ctx, const std::string &key) {
+    auto it = ctx->ctables.find(key);
+    return it == ctx->ctables.end() ? nullptr : it->second.ptr;
+}
+
+const void *upload_table(cm_ctx *ctx, const std::string &key, const void *host, size_t bytes, cm_status *st) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *st = hip_fail(ctx, e, "hipMalloc(table)");
+        return nullptr;
+    }
+    if (bytes && (e = hipMemcpy(p, host, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        hipFree(p);
+        *st = hip_fail(ctx, e, "table upload");
+        return nullptr;
+    }
+    ctx->ctables[key] = Buffer{p, bytes};
+    ctx->resident += bytes;
+    *st = CM_OK;
+    return p;
+}
+
+std::string table_key(const char *fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return std::string(buf);
+}
+
+cm_status pinned_host(cm_ctx *ctx, size_t bytes, void **out) {
+    if (ctx->pinned_bytes < bytes) {
+        if (ctx->pinned) {
+            CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            hipHostFree(ctx->pinned);
+            ctx->pinned = nullptr;
+            ctx->pinned_bytes = 0;
+        }
+        const size_t want = bytes < 4096 ? 4096 : bytes;
+        CM_HIP(ctx, hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+        ctx->pinned_bytes = want;
+    }
+    *out = ctx->pinned;
+    return CM_OK;
+}
+
 const void *twiddles(cm_ctx *ctx, int n, cm_dtype dt, cm_status *st) {
     uint64_t key = (uint64_t(n) << 8) | uint64_t(dt);
     auto it = ctx->tables.find(key);
@@ -217,6 +265,9 @@ void cm_ctx_destroy(cm_ctx *ctx) {
     for (auto &kv : ctx->scratch)
         if (kv.second.ptr) hipFree(kv.second.ptr);
     for (auto &kv : ctx->tables) hipFree(kv.second);
+    for (auto &kv : ctx->ctables)
+        if (kv.second.ptr) hipFree(kv.second.ptr);
+    if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

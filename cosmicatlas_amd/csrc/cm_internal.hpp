@@ -44,6 +44,9 @@ struct cm_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, cm::Buffer> scratch;   // named resident work buffers
     std::map<uint64_t, void *> tables;           // twiddle / sinc tables
+    std::map<std::string, cm::Buffer> ctables;   // host-computed tables by key (MAS windows, Gaussian, phases)
+    void *pinned = nullptr;                      // host staging of result read-backs (hipHostMalloc)
+    size_t pinned_bytes = 0;
     int cu_count = 256;
     size_t lds_per_block = 65536;
     // launch-shape tuning (0 = built-in choice); set from CM_TUNE at creation,
@@ -71,6 +74,25 @@ cm_status hip_fail(cm_ctx *ctx, hipError_t e, const char *what);
 
 // Named persistent device buffer of at least `bytes` (grown on demand).
 cm_status scratch(cm_ctx *ctx, const char *name, size_t bytes, void **out);
+
+// Device copy of a table computed on the host, uploaded on the first request of
+// `key` and kept for the context's lifetime (no host work or copy afterwards).
+const void *find_table(cm_ctx *ctx, const std::string &key);
+const void *upload_table(cm_ctx *ctx, const std::string &key, const void *host, size_t bytes, cm_status *st);
+template <typename T, typename F>
+const T *cached_table(cm_ctx *ctx, const std::string &key, F make, cm_status *st) {
+    if (const void *p = find_table(ctx, key)) {
+        *st = CM_OK;
+        return (const T *)p;
+    }
+    std::vector<T> h;
+    make(h);
+    return (const T *)upload_table(ctx, key, h.data(), sizeof(T) * h.size(), st);
+}
+std::string table_key(const char *fmt, ...);
+
+// Pinned host buffer of at least `bytes` owned by the context (result read-backs).
+cm_status pinned_host(cm_ctx *ctx, size_t bytes, void **out);
 
 // Events around a launch when profiling is on.
 void prof_begin(cm_ctx *ctx, const char *name, hipEvent_t *start);
@@ -119,11 +141,36 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 // sized from an estimate instead of a read-back of the count (no host
 // synchronisation); *overflow_out then points to a device flag that is
 // non-zero when the estimate was too small and the mesh must be redone.
+// overflow_at (nullable): device word used as the overflow flag instead of the
+// library's own.
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w,
                            size_t n, void *mesh, cm_dtype mdt, int accumulate, bool exact = true,
-                           unsigned **overflow_out = nullptr, bool reset_overflow = true);
+                           unsigned **overflow_out = nullptr, bool reset_overflow = true,
+                           unsigned *overflow_at = nullptr);
 cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
                         double *host_out);
+
+// Device results of one fused measurement in one block, so that a single copy
+// reads them back: shell sums[4 ns], mode counts[ns], object count, T-web
+// counters[5] (4 classes, marginal cells), deposit overflow flag (one 8-byte
+// word each for the last three groups).
+struct FusedOut {
+    double *sums = nullptr;
+    unsigned long long *cnts = nullptr;
+    double *nobj = nullptr;
+    unsigned long long *stats = nullptr;
+    unsigned *overflow = nullptr;
+    void *base = nullptr;
+    size_t bytes = 0;
+};
+cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f);
+// The fused tail of a measurement: mesh (float32, device) -> z/y/x passes with
+// the shells (+ potential and T-web labels when o->do_tweb).  meshB (nullable):
+// the interlaced mesh.  Results land in the FusedOut block of o->nshells.
+cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
+                             int8_t *labels, FusedOut *out);
+// Copy a FusedOut block to the pinned host buffer and wait; returns its host copy.
+cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host);
 
 // Whole-mock measurement with argument checks (cm_measure's body).
 cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n, double *pk0,
@@ -131,6 +178,8 @@ cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, co
 
 // P(k) helpers (pk.hip)
 void pk_tables(int N, int scheme, int mas_correction, std::vector<double> &tab);
+// pk_tables on the device, cached per (N, scheme, correction)
+const double *pk_tables_dev(cm_ctx *ctx, int N, int scheme, int mas_correction, cm_status *st);
 cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const void *dk, double **sums,
                         unsigned long long **cnts);
 void pk_finalize(int ns, double normal_power, double sn_norm, const double *sum, const unsigned long long *cnt,

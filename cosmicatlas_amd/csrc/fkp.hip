@@ -16,10 +16,6 @@
 
 namespace cm {
 
-cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, double **sums, unsigned long long **cnts, double **nobj,
-                             unsigned long long **stats);
-
 // FP64 sums of w and w^2 (unit weights when w is NULL), one partial pair per
 // workgroup.
 __global__ void __launch_bounds__(256) k_wsums(const float *__restrict__ w, size_t n, double *__restrict__ part) {
@@ -94,17 +90,12 @@ static cm_status scaled_weights(cm_ctx *ctx, const char *name, const float *w, s
 // mesh -> shells through the fused path, host sums and counts
 static cm_status shells_of_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean,
                                 std::vector<double> &hs, std::vector<unsigned long long> &hc) {
-    double *sums, *dnobj;
-    unsigned long long *cnts, *stats;
+    FusedOut dev, h;
     cm_status s;
-    if ((s = fused_measure_mesh(ctx, o, mesh, nullptr, mean, nullptr, &sums, &cnts, &dnobj, &stats)) != CM_OK)
-        return s;
-    hs.assign(4 * (size_t)o->nshells, 0.0);
-    hc.assign((size_t)o->nshells, 0ull);
-    CM_HIP(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(ctx, hipMemcpyAsync(hc.data(), cnts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((s = fused_measure_mesh(ctx, o, mesh, nullptr, mean, nullptr, &dev)) != CM_OK) return s;
+    if ((s = fused_readback(ctx, dev, o->nshells, &h)) != CM_OK) return s;
+    hs.assign(h.sums, h.sums + 4 * (size_t)o->nshells);
+    hc.assign(h.cnts, h.cnts + (size_t)o->nshells);
     return CM_OK;
 }
 

@@ -297,29 +297,59 @@ static cm_status x_fused_launch(cm_ctx *ctx, int nb, int xe, float2 *dk, const f
                                  ns);
 }
 
-// Fused measurement: mesh (float32, device) -> shells (+ labels).
-// Device outputs: sums[4*ns], cnts[ns], nobj[1], stats[5].
+cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f) {
+    const size_t words = 5 * (size_t)ns + 7;
+    void *b;
+    cm_status s;
+    if ((s = scratch(ctx, "f.out", 8 * words, &b)) != CM_OK) return s;
+    auto *w = (unsigned long long *)b;
+    f->base = b;
+    f->bytes = 8 * words;
+    f->sums = (double *)w;
+    f->cnts = w + 4 * (size_t)ns;
+    f->nobj = (double *)(w + 5 * (size_t)ns);
+    f->stats = w + 5 * (size_t)ns + 1;
+    f->overflow = (unsigned *)(w + 5 * (size_t)ns + 6);
+    return CM_OK;
+}
+
+cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host) {
+    void *h;
+    cm_status s;
+    if ((s = pinned_host(ctx, f.bytes, &h)) != CM_OK) return s;
+    CM_HIP(ctx, hipMemcpyAsync(h, f.base, f.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    auto *w = (unsigned long long *)h;
+    host->base = h;
+    host->bytes = f.bytes;
+    host->sums = (double *)w;
+    host->cnts = w + 4 * (size_t)ns;
+    host->nobj = (double *)(w + 5 * (size_t)ns);
+    host->stats = w + 5 * (size_t)ns + 1;
+    host->overflow = (unsigned *)(w + 5 * (size_t)ns + 6);
+    return CM_OK;
+}
+
+// Fused measurement: mesh (float32, device) -> shells (+ labels), results in
+// the FusedOut block.  Host tables (MAS windows, interlacing phases, Gaussian
+// factors) are cached on the device, so a repeated call issues kernels only.
 cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, double **sums, unsigned long long **cnts, double **nobj,
-                             unsigned long long **stats) {
+                             int8_t *labels, FusedOut *out) {
     const int N = o->mas.N;
     const size_t n3 = (size_t)N * N * N;
     const int pitch = padded_pitch(N);  // internal spectrum: kz rows padded to 128-byte multiples
     const size_t nh = (size_t)N * N * pitch;
     const int ns = o->nshells;
     cm_status s;
-    void *dk, *part, *dsum, *dcnt, *dnobj, *dtab, *dstats;
+    void *dk, *part;
     const int nparts = fft_rows_blocks(ctx, N, CM_F32);
     if ((s = scratch(ctx, "m.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
     if ((s = scratch(ctx, "f.part", sizeof(double) * (size_t)nparts, &part)) != CM_OK) return s;
-    if ((s = scratch(ctx, "f.sum", sizeof(double) * 4 * (size_t)ns, &dsum)) != CM_OK) return s;
-    if ((s = scratch(ctx, "f.cnt", sizeof(unsigned long long) * (size_t)ns, &dcnt)) != CM_OK) return s;
-    if ((s = scratch(ctx, "f.nobj", sizeof(double), &dnobj)) != CM_OK) return s;
-    if ((s = scratch(ctx, "f.stats", sizeof(unsigned long long) * 5, &dstats)) != CM_OK) return s;
-    std::vector<double> tab;
-    pk_tables(N, (int)o->mas.scheme, o->mas_correction, tab);
-    if ((s = scratch(ctx, "f.tab", sizeof(double) * tab.size(), &dtab)) != CM_OK) return s;
-    CM_HIP(ctx, hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    if ((s = fused_out(ctx, ns, out)) != CM_OK) return s;
+    double *dsum = out->sums, *dnobj = out->nobj;
+    unsigned long long *dcnt = out->cnts, *dstats = out->stats;
+    const double *dtab = pk_tables_dev(ctx, N, (int)o->mas.scheme, o->mas_correction, &s);
+    if (!dtab) return s;
 
     // z pass with the mesh sum, then the FP64 total
     if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part, pitch)) != CM_OK) return s;
@@ -331,16 +361,17 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     // interlaced mesh (particles shifted by d/2 on every axis): its full
     // spectrum, multiplied by exp(+i k.(d/2)(1,1,1)) = prod_a exp(i pi k_a/N)
     // one axis per pass, is averaged with the first in the fused x pass
-    void *dkB = nullptr, *phase = nullptr;
+    void *dkB = nullptr;
     if (meshB) {
         if ((s = scratch(ctx, "m.dkB", sizeof(float2) * nh, &dkB)) != CM_OK) return s;
-        if ((s = scratch(ctx, "f.phase", sizeof(float2) * (size_t)N, &phase)) != CM_OK) return s;
-        std::vector<float2> ph((size_t)N);
-        for (int j = 0; j < N; ++j) {
-            const double a = M_PI * (double)(j <= N / 2 ? j : j - N) / (double)N;
-            ph[j] = float2{(float)std::cos(a), (float)std::sin(a)};
-        }
-        CM_HIP(ctx, hipMemcpyAsync(phase, ph.data(), sizeof(float2) * ph.size(), hipMemcpyHostToDevice, ctx->stream));
+        const float2 *phase = cached_table<float2>(ctx, table_key("phase:%d", N), [&](std::vector<float2> &ph) {
+            ph.resize((size_t)N);
+            for (int j = 0; j < N; ++j) {
+                const double a = M_PI * (double)(j <= N / 2 ? j : j - N) / (double)N;
+                ph[j] = float2{(float)std::cos(a), (float)std::sin(a)};
+            }
+        }, &s);
+        if (!phase) return s;
         if ((s = fft_rows_r2c(ctx, N, CM_F32, meshB, dkB, mean_est, nullptr, pitch, phase)) != CM_OK) return s;
         if ((s = fft_cols(ctx, N, CM_F32, dkB, 1, true, pitch, phase)) != CM_OK) return s;
         if ((s = fft_cols(ctx, N, CM_F32, dkB, 0, true, pitch, phase)) != CM_OK) return s;
@@ -369,14 +400,16 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     const int do_tweb = o->do_tweb ? 1 : 0;
     // per-axis Gaussian exp(-k_i^2 R^2/2), k_i = +i kf (i <= N/2) else -(N-i) kf;
     // the kernel multiplies the three factors (exp is separable)
-    std::vector<float> g((size_t)N);
-    for (int i = 0; i < N; ++i) {
-        const double k = kf * (i <= N / 2 ? (double)i : -(double)(N - i));
-        g[i] = (float)std::exp(-k * k * R2h);
-    }
-    void *dgauss;
-    if ((s = scratch(ctx, "f.gauss", sizeof(float) * (size_t)N, &dgauss)) != CM_OK) return s;
-    CM_HIP(ctx, hipMemcpyAsync(dgauss, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice, ctx->stream));
+    const float *dgauss = cached_table<float>(ctx, table_key("gauss:%d:%.17g:%.17g", N, kf, R2h),
+                                              [&](std::vector<float> &g) {
+                                                  g.resize((size_t)N);
+                                                  for (int i = 0; i < N; ++i) {
+                                                      const double k = kf * (i <= N / 2 ? (double)i : -(double)(N - i));
+                                                      g[i] = (float)std::exp(-k * k * R2h);
+                                                  }
+                                              },
+                                              &s);
+    if (!dgauss) return s;
     const float inv_kf2 = (float)(1.0 / (kf * kf));
     const int nb = ctx->tune.xnb == 8 ? 8 : 16;  // kz columns per tile (16 when the workgroup fits)
     switch (N) {
@@ -401,10 +434,6 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
                                    (unsigned long long *)dstats)) != CM_OK)
             return s;
     }
-    *sums = (double *)dsum;
-    *cnts = (unsigned long long *)dcnt;
-    *nobj = (double *)dnobj;
-    *stats = (unsigned long long *)dstats;
     return CM_OK;
 }
 

@@ -711,7 +711,7 @@ static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const f
 
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w, size_t n,
                            void *mesh, cm_dtype mdt, int accumulate, bool exact, unsigned **overflow_out,
-                           bool reset_overflow) {
+                           bool reset_overflow, unsigned *overflow_at) {
     if (overflow_out) *overflow_out = nullptr;
     MasK p = make_mask(d, n);
     const size_t n3 = (size_t)d->N * d->N * d->N;
@@ -724,8 +724,8 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
     if (use_tiled && !tiled_ok)
         return fail(ctx, CM_ERR_UNSUPPORTED, "tiled mass assignment needs float32 positions and mesh and N%%8==0, N>=16");
     if (use_tiled) {
-        void *ovf;
-        if ((s = scratch(ctx, "mas.overflow", sizeof(unsigned), &ovf)) != CM_OK) return s;
+        void *ovf = overflow_at;
+        if (!ovf && (s = scratch(ctx, "mas.overflow", sizeof(unsigned), &ovf)) != CM_OK) return s;
         if (reset_overflow && (s = fill(ctx, (unsigned *)ovf, 1, 0u)) != CM_OK) return s;
         if (overflow_out) *overflow_out = (unsigned *)ovf;
         return mas_tiled(ctx, p, (const float *)pos, (const float *)w, (float *)mesh, accumulate,

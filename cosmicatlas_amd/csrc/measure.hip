@@ -46,10 +46,6 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
     return (unsigned)g;
 }
 
-cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, double **sums, unsigned long long **cnts, double **nobj,
-                             unsigned long long **stats);
-
 // Finalisation shared by both paths: normal_power = N_gal^2/V, SN = V/N_gal
 // (raw_sampling, FftwFunctions.cpp:159-174), so SN*normal_power = N_gal.
 static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<double> &hs,
@@ -67,37 +63,27 @@ static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<
 }
 
 // Returns CM_OK with *redo set when the tiled deposit's record estimate was too
-// small (the results are then discarded and the caller repeats exactly).
+// small (the results are then discarded and the caller repeats exactly).  One
+// copy reads back the shells, object count, T-web counters and overflow flag.
 static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, size_t n,
                                     double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
-                                    cm_measure_info *info, const unsigned *overflow, bool *redo) {
+                                    cm_measure_info *info, bool *redo) {
     const int N = o->mas.N;
     const double n3 = (double)N * N * N;
-    double *sums, *nobj;
-    unsigned long long *cnts, *stats;
-    cm_status s = fused_measure_mesh(ctx, o, mesh, meshB, (double)n / n3, labels, &sums, &cnts, &nobj, &stats);
-    if (s != CM_OK) return s;
     const int ns = o->nshells;
-    std::vector<double> hs(4 * (size_t)ns);
-    std::vector<unsigned long long> hc((size_t)ns);
-    double nobj_h = 0.0;
-    unsigned long long hst[5] = {0, 0, 0, 0, 0};
-    CM_HIP(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(ctx, hipMemcpyAsync(hc.data(), cnts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    CM_HIP(ctx, hipMemcpyAsync(&nobj_h, nobj, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if (o->do_tweb)
-        CM_HIP(ctx, hipMemcpyAsync(hst, stats, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
-    unsigned ovf = 0;
-    if (overflow) CM_HIP(ctx, hipMemcpyAsync(&ovf, overflow, sizeof ovf, hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    *redo = ovf != 0;
+    FusedOut dev, h;
+    cm_status s = fused_measure_mesh(ctx, o, mesh, meshB, (double)n / n3, labels, &dev);
+    if (s != CM_OK) return s;
+    if ((s = fused_readback(ctx, dev, ns, &h)) != CM_OK) return s;
+    *redo = *h.overflow != 0;
     if (*redo) return CM_OK;
     if (info) {
-        for (int i = 0; i < 4; ++i) info->tweb.counts[i] = (int64_t)hst[i];
-        info->tweb.n_marginal = (int64_t)hst[4];
+        for (int i = 0; i < 4; ++i) info->tweb.counts[i] = o->do_tweb ? (int64_t)h.stats[i] : 0;
+        info->tweb.n_marginal = o->do_tweb ? (int64_t)h.stats[4] : 0;
     }
-    finalize(o, nobj_h, hs, hc, pk0, pk2, pk4, nmodes, info);
+    const std::vector<double> hs(h.sums, h.sums + 4 * (size_t)ns);
+    const std::vector<unsigned long long> hc(h.cnts, h.cnts + (size_t)ns);
+    finalize(o, *h.nobj, hs, hc, pk0, pk2, pk4, nmodes, info);
     return CM_OK;
 }
 
@@ -118,20 +104,24 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     if (o->fused) {
         void *meshB = nullptr;
         if (o->interlace && (s = scratch(ctx, "m.meshB", sizeof(float) * n3, &meshB)) != CM_OK) return s;
-        // first attempt without the record-count read-back; repeat exactly on overflow
+        // first attempt without the record-count read-back; repeat exactly on
+        // overflow (flag in the result block, read back with the shells)
+        FusedOut fo;
+        if ((s = fused_out(ctx, o->nshells, &fo)) != CM_OK) return s;
         for (int attempt = 0; attempt < 2; ++attempt) {
             unsigned *ovf = nullptr;
-            if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, attempt > 0, &ovf)) != CM_OK) return s;
+            if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, attempt > 0, &ovf, true, fo.overflow)) !=
+                CM_OK)
+                return s;
             if (meshB) {
                 cm_mas_desc mb = md;
                 mb.interlace_shift += 0.5;  // half a cell on every axis
-                unsigned *ovfB = nullptr;
-                if ((s = mesh_assign_impl(ctx, &mb, pos, w, n, meshB, CM_F32, 0, attempt > 0, &ovfB, false)) != CM_OK)
+                if ((s = mesh_assign_impl(ctx, &mb, pos, w, n, meshB, CM_F32, 0, attempt > 0, &ovf, false,
+                                          fo.overflow)) != CM_OK)
                     return s;
             }
             bool redo = false;
-            if ((s = measure_fused_tail(ctx, o, mesh, meshB, n, pk0, pk2, pk4, nmodes, labels, info, ovf, &redo)) !=
-                CM_OK)
+            if ((s = measure_fused_tail(ctx, o, mesh, meshB, n, pk0, pk2, pk4, nmodes, labels, info, &redo)) != CM_OK)
                 return s;
             if (!redo) return CM_OK;
         }

@@ -161,6 +161,11 @@ void pk_tables(int N, int scheme, int mas_correction, std::vector<double> &tab) 
     }
 }
 
+const double *pk_tables_dev(cm_ctx *ctx, int N, int scheme, int mas_correction, cm_status *st) {
+    return cached_table<double>(ctx, table_key("pk:%d:%d:%d", N, scheme, mas_correction),
+                                [&](std::vector<double> &t) { pk_tables(N, scheme, mas_correction, t); }, st);
+}
+
 static bool near_int(double v, long &out) {
     double r = std::floor(v + 0.5);
     if (std::fabs(v - r) < 1e-9) {
@@ -197,14 +202,12 @@ cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const voi
                         unsigned long long **cnts) {
     const int ns = o->nshells;
     const int M1 = o->N / 2 + 1;
-    std::vector<double> tab;
-    pk_tables(o->N, (int)o->scheme, o->mas_correction, tab);
-    void *dtab, *dsum, *dcnt;
+    void *dsum, *dcnt;
     cm_status s;
-    if ((s = scratch(ctx, "pk.tab", sizeof(double) * tab.size(), &dtab)) != CM_OK) return s;
+    const double *dtab = pk_tables_dev(ctx, o->N, (int)o->scheme, o->mas_correction, &s);
+    if (!dtab) return s;
     if ((s = scratch(ctx, "pk.sum", sizeof(double) * 4 * (size_t)ns, &dsum)) != CM_OK) return s;
     if ((s = scratch(ctx, "pk.cnt", sizeof(unsigned long long) * (size_t)ns, &dcnt)) != CM_OK) return s;
-    CM_HIP(ctx, hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
     if ((s = launch(ctx, "pk_zero", k_pk_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
                     ns)) != CM_OK)
         return s;
@@ -217,10 +220,10 @@ cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const voi
     unsigned g = (unsigned)std::min<size_t>(blocks, (size_t)ctx->cu_count * 8);
     if (dt == CM_F32)
         s = launch(ctx, "pk_shells", k_pk_shells<float>, dim3(g), dim3(256), lds, (const v2pk<float> *)dk, p,
-                   (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt);
+                   dtab, (double *)dsum, (unsigned long long *)dcnt);
     else
         s = launch(ctx, "pk_shells", k_pk_shells<double>, dim3(g), dim3(256), lds, (const v2pk<double> *)dk, p,
-                   (const double *)dtab, (double *)dsum, (unsigned long long *)dcnt);
+                   dtab, (double *)dsum, (unsigned long long *)dcnt);
     if (s != CM_OK) return s;
     *sums = (double *)dsum;
     *cnts = (unsigned long long *)dcnt;
