@@ -232,13 +232,22 @@ __global__ void k_fused_zero(double *s, unsigned long long *c, int ns) {
     for (int i = threadIdx.x; i < ns; i += blockDim.x) c[i] = 0ull;
 }
 
-__global__ void __launch_bounds__(256) k_fused_sum(const double *__restrict__ part, int np, double *__restrict__ out) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
-    red[threadIdx.x] = s;
+// FP64 total of the z pass's partial sums (one workgroup, four independent
+// accumulators per thread so the loads overlap).
+__global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ part, int np, double *__restrict__ out) {
+    __shared__ double red[1024];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * 1024 < np; i += 4 * 1024) {
+        s0 += part[i];
+        s1 += part[i + 1024];
+        s2 += part[i + 2 * 1024];
+        s3 += part[i + 3 * 1024];
+    }
+    for (; i < np; i += 1024) s0 += part[i];
+    red[threadIdx.x] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    for (int o = 512; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
@@ -353,7 +362,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
 
     // z pass with the mesh sum, then the FP64 total
     if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part, pitch)) != CM_OK) return s;
-    if ((s = launch(ctx, "fused_sum", k_fused_sum, dim3(1), dim3(256), 0, (const double *)part, nparts,
+    if ((s = launch(ctx, "fused_sum", k_fused_sum, dim3(1), dim3(1024), 0, (const double *)part, nparts,
                     (double *)dnobj)) != CM_OK)
         return s;
     // y pass

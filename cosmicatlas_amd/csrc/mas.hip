@@ -107,15 +107,32 @@ __device__ __forceinline__ int axis_tiles(int c, int R, int ntx, int out[2]) {
 
 // Cell and in-cell offset f = x/d - c - 1/2 (FP64, as the plain kernel) of a
 // particle from its loaded coordinates; s[] receives the offsets in float.
+// Out-of-box coordinates (periodic images, non-finite values) take the full
+// wrap out of line; in-box ones (the common case) only the subtraction.
+// Returns NaN for a dropped coordinate.
+__device__ __attribute__((noinline)) float wrap_slow(float v, double xmin, double L) {
+    float o = 0.0f;
+    return wrap_coord<float>(v, xmin, L, o) ? o : __int_as_float(0x7fc00000);
+}
+
 __device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int c[3], float s[3]) {
     bool ok = true;
+    const float Lf = (float)p.L;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float u = 0.0f;
-        ok = ok && wrap_coord<float>(v[a], p.xmin[a], p.L, u);
+        // wrap_coord's first step; a value already in [0, L) is returned as is
+        float u = (float)((double)v[a] - p.xmin[a]);
+        if (__builtin_expect(!(u >= 0.0f && u < Lf), 0)) {
+            u = wrap_slow(v[a], p.xmin[a], p.L);
+            if (u != u) {
+                ok = false;
+                u = 0.0f;
+            }
+        }
         const double x = shifted((double)u, p.ishift, p.L);
-        c[a] = cell_of(x, p.d, p.N);
-        s[a] = (float)(x / p.d - (double)c[a] - 0.5);
+        double q;
+        c[a] = cell_fast(x, p.d, p.rd, p.N, q);
+        s[a] = (float)(q - (double)c[a] - 0.5);
     }
     return ok;
 }
@@ -128,7 +145,7 @@ __device__ __forceinline__ float4 make_record(const int c[3], const float f[3]) 
 
 // Particles i = base + u*stride (u < U, i < i1) are loaded before any is used,
 // so one thread keeps U loads in flight.
-constexpr int kBatch = 4;
+constexpr int kBatch = 8;
 template <int U>
 __device__ __forceinline__ void load_batch(const float *__restrict__ pos, const MasK &p, size_t base, size_t stride,
                                            size_t i1, float v[U][3]) {
@@ -397,20 +414,36 @@ __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restr
     }
 }
 
-// LDS accumulation is in 64-bit fixed point (2^-32 quantum): integer LDS
-// atomics run at full rate on gfx950 (float LDS atomic add does not), and the
-// integer sum does not depend on the order of the records, so every tile that
-// one workgroup deposits is bit-reproducible.
+// LDS accumulation is in fixed point: integer LDS atomics run at full rate on
+// gfx950 (float LDS atomic add does not), and an integer sum does not depend
+// on the order of the records, so every tile that one workgroup deposits is
+// bit-reproducible.
+//   unit weights: 32-bit cells with a per-item quantum 2^-k, k the largest
+//     integer with cnt * wmax * 2^k <= 2^31 (cnt = records of the item, wmax
+//     = the scheme's largest stencil weight), so no cell can overflow (the
+//     factor 2 covers rounding); an add rounds by at most 2^-(k+1) (2^-23 for
+//     a TSC chunk of 730 records, 2^-21 at the 4096-record maximum).  NGP adds
+//     2^k per record: counts stay exact.  32-bit atomics move half the LDS
+//     words of 64-bit ones and halve the grid, so more workgroups fit a CU.
+//   general weights: 64-bit cells, quantum 2^-32.
 __device__ __forceinline__ float from_fixed(long long q) { return (float)((double)q * (1.0 / 4294967296.0)); }
+
+template <int SCHEME>
+__device__ __forceinline__ int unit_quantum_exp(unsigned cnt) {
+    constexpr float wmax = SCHEME == 2 ? 0.421875f : (SCHEME == 3 ? 0.2962963f : 1.0f);
+    const float lim = 2147483648.0f / ((float)(cnt ? cnt : 1u) * wmax);
+    const int k = (int)floorf(log2f(lim));
+    return k < 0 ? 0 : (k > 31 ? 31 : k);
+}
 
 // Stencil of one record into the LDS grid, unrolled for the scheme.  A record
 // of the tile sits in a cell c with local coordinate c-o in [-R, T+R) (ghosts
 // come from up to R cells outside), so its stencil spans [-2R, T+2R): the LDS
 // grid has S = T+4R cells per axis with local cell u at index u+2R, every
 // contribution lands in it without a range test, and only [0,T) is written
-// out.
+// out.  Unit weights: 32-bit cells, quantum 1/qs (qs = 2^k).
 template <int SCHEME, bool UNIT, int S>
-__device__ __forceinline__ void deposit_record(long long *lgrid, int base, const float f[3], float wt) {
+__device__ __forceinline__ void deposit_record(void *lgrid, int base, const float f[3], float wt, float qs) {
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
     constexpr int W = 2 * R + 1;
     float wa[3][5];
@@ -421,15 +454,16 @@ __device__ __forceinline__ void deposit_record(long long *lgrid, int base, const
 #pragma unroll
         for (int b = 0; b < W; ++b) {
             const float vab = wt * wa[0][a + 2 - R] * wa[1][b + 2 - R];
-            unsigned long long *row = (unsigned long long *)&lgrid[base + (a * S + b) * S];
+            const int rowi = base + (a * S + b) * S;
 #pragma unroll
             for (int e = 0; e < W; ++e) {
                 const float v = vab * wa[2][e + 2 - R];
-                // unit weights: |v| <= 1 fits v*2^30 in an int32 (one conversion);
-                // general weights go through FP64
-                const long long q = UNIT ? (long long)__float2int_rn(v * 1073741824.0f) * 4ll
-                                         : __double2ll_rn((double)v * 4294967296.0);
-                atomicAdd(&row[e], (unsigned long long)q);
+                if constexpr (UNIT) {
+                    atomicAdd((unsigned *)lgrid + rowi + e, __float2uint_rn(v * qs));
+                } else {
+                    const long long q = __double2ll_rn((double)v * 4294967296.0);
+                    atomicAdd((unsigned long long *)lgrid + rowi + e, (unsigned long long)q);
+                }
             }
         }
     }
@@ -444,10 +478,12 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
                                                       const uint4 *__restrict__ items,
                                                       const unsigned *__restrict__ nitems, int N, int ntx,
                                                       float *__restrict__ mesh, int accumulate) {
-    extern __shared__ long long lgrid[];
+    extern __shared__ long long lgrid_raw[];
+    void *lgrid = lgrid_raw;
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
     constexpr int S = T + 4 * R, S3 = S * S * S, NT = 512, RPT = kTileCH / NT;
-    static_assert(S3 % 2 == 0, "LDS grid is cleared in 16-byte words");
+    constexpr int WB = UNIT ? 4 : 8;  // bytes per LDS cell
+    static_assert((S3 * WB) % 16 == 0, "LDS grid is cleared in 16-byte words");
     const int tid = threadIdx.x;
     const unsigned total = *nitems;
     unsigned item = blockIdx.x;
@@ -469,11 +505,13 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
         const unsigned nxt = item + gridDim.x;
         const uint4 itn = nxt < total ? items[nxt] : make_uint4(0u, 0u, 0u, 0u);
         uint4 *lz = reinterpret_cast<uint4 *>(lgrid);
-        for (int q = tid; q < S3 / 2; q += NT) lz[q] = make_uint4(0u, 0u, 0u, 0u);
+        for (int q = tid; q < S3 * WB / 16; q += NT) lz[q] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
         const int tile = (int)it.x;
         const int tz = tile % ntx, ty = (tile / ntx) % ntx, tx = tile / (ntx * ntx);
         const int o[3] = {tx * T, ty * T, tz * T};
+        const int qk = UNIT ? unit_quantum_exp<SCHEME>(it.z - it.y) : 0;
+        const float qs = ldexpf(1.0f, qk);
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const unsigned idx = it.y + (unsigned)(tid + r * NT);
@@ -489,7 +527,7 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
                 if (lc >= N - R) lc -= N;
                 l[a] = lc + R;  // node c-R (local lc-R) sits at LDS index lc-R+2R
             }
-            deposit_record<SCHEME, UNIT, S>(lgrid, (l[0] * S + l[1]) * S + l[2], f, wr[r]);
+            deposit_record<SCHEME, UNIT, S>(lgrid, (l[0] * S + l[1]) * S + l[2], f, wr[r], qs);
         }
         const bool split = it.w != 0u;
         it = itn;
@@ -501,7 +539,9 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
             const int q = tid + k * NT;
             if (TT % NT != 0 && q >= TT) break;
             const int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
-            const float v = from_fixed(lgrid[((ix + 2 * R) * S + (iy + 2 * R)) * S + (iz + 2 * R)]);
+            const int li = ((ix + 2 * R) * S + (iy + 2 * R)) * S + (iz + 2 * R);
+            const float v = UNIT ? ldexpf(__uint2float_rn(((const unsigned *)lgrid)[li]), -qk)
+                                 : from_fixed(((const long long *)lgrid)[li]);
             const size_t idx = ((size_t)(o[0] + ix) * N + (o[1] + iy)) * N + (o[2] + iz);
             if (split) {
                 if (v != 0.0f) unsafeAtomicAdd(&mesh[idx], v);
@@ -566,6 +606,7 @@ static MasK make_mask(const cm_mas_desc *d, size_t n) {
     p.pad = 0;
     p.L = d->L;
     p.d = d->L / (double)d->N;
+    p.rd = 1.0 / p.d;
     for (int a = 0; a < 3; ++a) p.xmin[a] = d->xmin[a];
     p.ishift = d->interlace_shift * p.d;
     p.n = n;
@@ -674,7 +715,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
             return s;
     }
     const int S = T + 4 * t.R;
-    const size_t lds = sizeof(long long) * (size_t)S * S * S;
+    const size_t lds = (w ? sizeof(long long) : sizeof(unsigned)) * (size_t)S * S * S;
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "mas: deposit grid of %zu bytes", lds);
     const unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * 8);
     const float *ws = (const float *)wsorted;

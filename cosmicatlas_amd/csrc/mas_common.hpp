@@ -16,6 +16,7 @@ struct MasK {
     int pad;
     double L;
     double d;      // L/N
+    double rd;     // 1/d rounded to double
     double xmin[3];
     double ishift; // interlace shift in length units
     size_t n;
@@ -55,6 +56,19 @@ __device__ __forceinline__ double shifted(double s, double ishift, double L) {
 // get_bin: floor(double(x)/double(d)), clamped to [0, N-1].
 __device__ __forceinline__ int cell_of(double x, double d, int N) {
     int c = (int)floor(x / d);
+    c = c >= N ? N - 1 : c;
+    return c < 0 ? 0 : c;
+}
+
+// get_bin's cell and the in-cell offset q - c - 1/2 without an FP64 division
+// in the common case.  For 0 <= x < L the product x*rd is within 2^-42 of
+// x/d and the rounded quotient within 2^-44 (x/d < N <= 1024), so when no
+// integer lies within 2^-40 of the product, both floor to the same cell;
+// otherwise the quotient is taken by division, as in cell_of.
+__device__ __forceinline__ int cell_fast(double x, double d, double rd, int N, double &q) {
+    q = x * rd;
+    if (floor(q - 0x1p-40) != floor(q + 0x1p-40)) q = x / d;
+    int c = (int)floor(q);
     c = c >= N ? N - 1 : c;
     return c < 0 ? 0 : c;
 }

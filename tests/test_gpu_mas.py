@@ -42,7 +42,9 @@ def test_cic_tsc_pcs_match_oracle(ctx, orc, scheme, algo):
     pos = uniform_pos(n, L, 5)
     mesh = ctx.mesh_assign(torch.from_numpy(pos).cuda(), N, L, scheme, algo=algo).cpu().numpy()
     ref = orc.mas(pos, N, L, {"CIC": orc.CIC, "TSC": orc.TSC, "PCS": orc.PCS}[scheme])
-    assert rel_err(mesh, ref) < 2e-6
+    # tiled, unit weights: 32-bit fixed point, quantum 2^-20 for the 4096-record
+    # chunks of this dense mesh (half a quantum per add, ~30 adds per cell)
+    assert rel_err(mesh, ref) < (2e-6 if algo == "plain" else 6e-6)
     assert abs(mesh.astype(np.float64).sum() - n) < 1e-5 * n
 
 
@@ -78,11 +80,14 @@ def test_weights_soa_interlace_and_accumulate(ctx, orc):
         soa = ctx.mesh_assign(torch.from_numpy(pos.T.copy()).cuda(), N, L, "CIC", weights=wd, soa=True,
                               algo=algo).cpu().numpy()
         assert rel_err(soa, m) < 1e-6
+        # unit weights on the tiled path: 32-bit fixed point, quantum 2^-20 at
+        # the ~3500-record tiles of this 32^3 mesh
+        tol = 2e-6 if algo == "plain" else 6e-6
         il = ctx.mesh_assign(posd, N, L, "TSC", interlace_shift=0.5, algo=algo).cpu().numpy()
-        assert rel_err(il, orc.mas(pos, N, L, orc.TSC, interlace_shift=0.5)) < 2e-6
+        assert rel_err(il, orc.mas(pos, N, L, orc.TSC, interlace_shift=0.5)) < tol
         acc = ctx.mesh_assign(posd, N, L, "TSC", algo=algo)
         ctx.mesh_assign(posd, N, L, "TSC", algo=algo, mesh=acc, accumulate=True)
-        assert rel_err(acc.cpu().numpy(), 2 * orc.mas(pos, N, L, orc.TSC)) < 2e-6
+        assert rel_err(acc.cpu().numpy(), 2 * orc.mas(pos, N, L, orc.TSC)) < tol
 
 
 def test_wrap_images_and_edges(ctx, orc):
