@@ -102,6 +102,46 @@ class Context:
     def synchronize(self):
         self._check(self.lib.cm_synchronize(self.h))
 
+    def set_graphs(self, on: bool = True):
+        """hipGraph replay of repeated fused measurements (on by default)."""
+        self._check(self.lib.cm_set_graphs(self.h, int(on)))
+
+    def graph_counts(self):
+        """(graphs captured, graph launches) of this context."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.lib.cm_graph_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, pos, N, L, scheme, nshells, kmin, dk_bin, weights=None, do_tweb=False, R_smooth=0.0,
+                 lambda_th=0.0, margin_tol=0.0, labels=None, **kw):
+        """Tune the fused path's launch shapes for meshes of N on this device
+        (process-wide); returns the shapes and the kernel ms per mock before and
+        after."""
+        torch = _torch()
+        o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, do_tweb=do_tweb, R_smooth=R_smooth,
+                          lambda_th=lambda_th, margin_tol=margin_tol, **kw)
+        if do_tweb and labels is None:
+            labels = torch.empty((N, N, N), dtype=torch.int8, device=pos.device)
+        t = _abi.Tune()
+        before, after = C.c_double(), C.c_double()
+        self._check(self.lib.cm_autotune(self.h, C.byref(o), _ptr(pos), _ptr(weights), pos.shape[0], _ptr(labels),
+                                         C.byref(t), C.byref(before), C.byref(after)))
+        return {"tune": {f: getattr(t, f) for f, _ in _abi.Tune._fields_}, "ms_before": before.value,
+                "ms_after": after.value}
+
+    def set_tune(self, N, tune=None):
+        """Set (None: clear) the process-wide launch shapes for meshes of N."""
+        if tune is None:
+            self._check(self.lib.cm_set_tune(self.h, int(N), None))
+            return
+        t = _abi.Tune(*[int(tune.get(f, 0)) for f, _ in _abi.Tune._fields_])
+        self._check(self.lib.cm_set_tune(self.h, int(N), C.byref(t)))
+
+    def get_tune(self, N):
+        t = _abi.Tune()
+        self._check(self.lib.cm_get_tune(self.h, int(N), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _abi.Tune._fields_}
+
     def profile(self, on: bool = True):
         self._check(self.lib.cm_profile_enable(self.h, int(on)))
 

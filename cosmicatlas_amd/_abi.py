@@ -91,6 +91,10 @@ class MeasureInfo(C.Structure):
     ]
 
 
+class Tune(C.Structure):
+    _fields_ = [("xnb", C.c_int), ("xe", C.c_int), ("tz", C.c_int), ("tseg", C.c_int), ("dep_wg", C.c_int)]
+
+
 class ZeldovichOpts(C.Structure):
     _fields_ = [
         ("NL", C.c_int),
@@ -154,6 +158,12 @@ def _declare(L):
         "cm_status_string": (C.c_char_p, [C.c_int]),
         "cm_set_stream": (st, [P, P]),
         "cm_get_stream": (P, [P]),
+        "cm_set_graphs": (st, [P, C.c_int]),
+        "cm_graph_counts": (st, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "cm_autotune": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, C.POINTER(Tune),
+                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "cm_set_tune": (st, [P, C.c_int, C.POINTER(Tune)]),
+        "cm_get_tune": (st, [P, C.c_int, C.POINTER(Tune)]),
         "cm_synchronize": (st, [P]),
         "cm_dev_alloc": (st, [P, C.c_size_t, C.POINTER(P)]),
         "cm_dev_free": (st, [P, P]),

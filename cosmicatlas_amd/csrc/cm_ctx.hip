@@ -69,13 +69,24 @@ static void resolve_pending(cm_ctx *ctx) {
     ctx->pending.clear();
 }
 
+double profiled_total_ms(cm_ctx *ctx) {
+    resolve_pending(ctx);
+    double t = 0.0;
+    for (auto &kv : ctx->ktimes) t += kv.second.sum_ms;
+    return t;
+}
+
 cm_status scratch(cm_ctx *ctx, const char *name, size_t bytes, void **out) {
     auto &b = ctx->scratch[name];
     if (b.bytes >= bytes && b.ptr) {
         *out = b.ptr;
         return CM_OK;
     }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return fail(ctx, CM_ERR_DEVICE, "scratch '%s' would be allocated during graph capture", name);
     if (b.ptr) {
+        ++ctx->scratch_gen;  // captured graphs that hold the old address are stale
         hipStreamSynchronize(ctx->stream);
         hipFree(b.ptr);
         ctx->resident -= b.bytes;
@@ -240,6 +251,8 @@ cm_status cm_ctx_create(int device, size_t hbm_budget, cm_ctx **out) {
                 else if (k == "tz") c->tune.tz = v;
                 else if (k == "tseg") c->tune.tseg = v;
                 else if (k == "mas_tile") c->tune.mas_tile = v;
+                else if (k == "graph") c->graphs = v != 0;
+                else if (k == "dep_wg") c->tune.dep_wg = v;
             }
             pos = end + 1;
         }
@@ -264,6 +277,8 @@ void cm_ctx_destroy(cm_ctx *ctx) {
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     for (auto &kv : ctx->scratch)
         if (kv.second.ptr) hipFree(kv.second.ptr);
+    for (auto &g : ctx->graph_cache)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     for (auto &kv : ctx->tables) hipFree(kv.second);
     for (auto &kv : ctx->ctables)
         if (kv.second.ptr) hipFree(kv.second.ptr);
@@ -281,6 +296,19 @@ cm_status cm_set_stream(cm_ctx *ctx, void *stream) {
 }
 
 void *cm_get_stream(cm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+cm_status cm_set_graphs(cm_ctx *ctx, int on) {
+    if (!ctx) return CM_ERR_INVALID;
+    ctx->graphs = on != 0;
+    return CM_OK;
+}
+
+cm_status cm_graph_counts(cm_ctx *ctx, int64_t *captured, int64_t *launched) {
+    if (!ctx) return CM_ERR_INVALID;
+    if (captured) *captured = ctx->graph_captures;
+    if (launched) *launched = ctx->graph_launches;
+    return CM_OK;
+}
 
 cm_status cm_synchronize(cm_ctx *ctx) {
     if (!ctx) return CM_ERR_INVALID;

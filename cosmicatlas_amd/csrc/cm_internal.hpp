@@ -29,6 +29,14 @@ struct Buffer {
     size_t bytes = 0;
 };
 
+// A captured fused measurement (see measure.hip).
+struct GraphEntry {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+    bool bad = false;
+};
+
 }  // namespace cm
 
 struct cm_ctx {
@@ -47,6 +55,10 @@ struct cm_ctx {
     std::map<std::string, cm::Buffer> ctables;   // host-computed tables by key (MAS windows, Gaussian, phases)
     void *pinned = nullptr;                      // host staging of result read-backs (hipHostMalloc)
     size_t pinned_bytes = 0;
+    uint64_t scratch_gen = 0;                    // bumped whenever a work buffer is (re)allocated
+    bool graphs = true;                          // hipGraph replay of repeated fused measurements
+    std::vector<cm::GraphEntry> graph_cache;
+    int64_t graph_captures = 0, graph_launches = 0;
     int cu_count = 256;
     size_t lds_per_block = 65536;
     // launch-shape tuning (0 = built-in choice); set from CM_TUNE at creation,
@@ -58,6 +70,7 @@ struct cm_ctx {
         int zrows = 0;  // rows per workgroup in the z passes
         int ty = 0, tz = 0, tseg = 0;  // T-web tile and x segments
         int mas_tile = 0;              // cells per tile edge in the deposit
+        int dep_wg = 0;                // deposit workgroups per CU
     } tune;
 };
 
@@ -171,6 +184,18 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
                              int8_t *labels, FusedOut *out);
 // Copy a FusedOut block to the pinned host buffer and wait; returns its host copy.
 cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host);
+
+// Sum of the profiled kernel times recorded since the last reset (ms).
+double profiled_total_ms(cm_ctx *ctx);
+
+// The process-wide shapes for (device, N) laid over the context's launch
+// shapes for the lifetime of the scope.
+struct TuneScope {
+    cm_ctx *ctx;
+    decltype(cm_ctx::tune) saved;
+    TuneScope(cm_ctx *c, int N);
+    ~TuneScope() { ctx->tune = saved; }
+};
 
 // Whole-mock measurement with argument checks (cm_measure's body).
 cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n, double *pk0,

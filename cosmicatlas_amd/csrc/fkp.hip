@@ -169,6 +169,7 @@ extern "C" cm_status cm_measure_fkp(cm_ctx *ctx, const cm_measure_opts *o, const
     if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: binning");
     if (o->do_tweb || o->interlace)
         return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure_fkp: the T-web and interlacing options must be off");
+    TuneScope ts(ctx, o->mas.N);
     return measure_fkp_impl(ctx, o, f, pos, (const float *)w, n, rpos, (const float *)rw, nr, pk0, pk2, pk4, nmodes,
                             pk_window, info);
 }

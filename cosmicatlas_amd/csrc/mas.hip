@@ -717,7 +717,8 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     const int S = T + 4 * t.R;
     const size_t lds = (w ? sizeof(long long) : sizeof(unsigned)) * (size_t)S * S * S;
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "mas: deposit grid of %zu bytes", lds);
-    const unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * 8);
+    const int wg_per_cu = ctx->tune.dep_wg > 0 && ctx->tune.dep_wg <= 64 ? ctx->tune.dep_wg : 8;
+    const unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * wg_per_cu);
     const float *ws = (const float *)wsorted;
     const dim3 gd(dg), bd(512);
 #define CM_DEPOSIT(SCH, UNIT)                                                                                    \

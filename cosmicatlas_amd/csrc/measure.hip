@@ -62,18 +62,14 @@ static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<
     pk_finalize(o->nshells, normal > 0 ? normal : 1.0, SN * normal, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
 }
 
-// Returns CM_OK with *redo set when the tiled deposit's record estimate was too
-// small (the results are then discarded and the caller repeats exactly).  One
-// copy reads back the shells, object count, T-web counters and overflow flag.
-static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, size_t n,
-                                    double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,
-                                    cm_measure_info *info, bool *redo) {
-    const int N = o->mas.N;
-    const double n3 = (double)N * N * N;
+// Read back the fused results (one copy) and finalise; *redo is set when the
+// tiled deposit's record estimate was too small (results discarded, the caller
+// repeats with exact sizing).
+static cm_status finish_fused(cm_ctx *ctx, const cm_measure_opts *o, const FusedOut &dev, double *pk0, double *pk2,
+                              double *pk4, int64_t *nmodes, cm_measure_info *info, bool *redo) {
     const int ns = o->nshells;
-    FusedOut dev, h;
-    cm_status s = fused_measure_mesh(ctx, o, mesh, meshB, (double)n / n3, labels, &dev);
-    if (s != CM_OK) return s;
+    FusedOut h;
+    cm_status s;
     if ((s = fused_readback(ctx, dev, ns, &h)) != CM_OK) return s;
     *redo = *h.overflow != 0;
     if (*redo) return CM_OK;
@@ -84,6 +80,106 @@ static cm_status measure_fused_tail(cm_ctx *ctx, const cm_measure_opts *o, void 
     const std::vector<double> hs(h.sums, h.sums + 4 * (size_t)ns);
     const std::vector<unsigned long long> hc(h.cnts, h.cnts + (size_t)ns);
     finalize(o, *h.nobj, hs, hc, pk0, pk2, pk4, nmodes, info);
+    return CM_OK;
+}
+
+// Enqueue one fused measurement: deposit(s) into the mesh (and the interlaced
+// mesh), then the fused tail.  exact = false sizes the record buffer from an
+// estimate (no host synchronisation; overflow is flagged in the result block).
+static cm_status enqueue_fused(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                               void *mesh, void *meshB, int8_t *labels, bool exact, FusedOut *dev) {
+    const int N = o->mas.N;
+    const double n3 = (double)N * N * N;
+    cm_mas_desc md = o->mas;
+    md.algo = CM_MAS_AUTO;
+    FusedOut fo;
+    cm_status s;
+    if ((s = fused_out(ctx, o->nshells, &fo)) != CM_OK) return s;
+    unsigned *ovf = nullptr;
+    if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, exact, &ovf, true, fo.overflow)) != CM_OK)
+        return s;
+    if (meshB) {
+        cm_mas_desc mb = md;
+        mb.interlace_shift += 0.5;  // half a cell on every axis
+        if ((s = mesh_assign_impl(ctx, &mb, pos, w, n, meshB, CM_F32, 0, exact, &ovf, false, fo.overflow)) != CM_OK)
+            return s;
+    }
+    return fused_measure_mesh(ctx, o, mesh, meshB, (double)n / n3, labels, dev);
+}
+
+// hipGraph replay of repeated fused measurements.  A call is keyed by its
+// options, pointers, particle count, stream and tuning; the first call of a key
+// runs eagerly (allocating every work buffer), the second captures the
+// sequence into a graph, later ones launch the graph.  The graph holds work
+// buffer addresses, so it is valid only while no buffer of the context has been
+// reallocated (ctx->scratch_gen).  Nothing in the sequence uses
+// hipMemsetAsync: buffers are cleared by fill kernels.
+static std::string graph_key(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                             const int8_t *labels) {
+    const cm_mas_desc &m = o->mas;
+    return table_key("%d|%d|%d|%d|%.17g|%.17g,%.17g,%.17g|%.17g|%d|%d|%.17g|%.17g|%d|%d|%d|%d|%.17g|%.17g|%.17g|%d|"
+                     "%p|%p|%zu|%p|%p|%d,%d,%d,%d,%d",
+                     m.N, (int)m.scheme, (int)m.layout, (int)m.pos_dtype, m.L, m.xmin[0], m.xmin[1], m.xmin[2],
+                     m.interlace_shift, o->nshells, o->mas_correction, o->kmin, o->dk, o->sn_correction,
+                     o->float_shell_rule, o->do_tweb, o->interlace, o->R_smooth, o->lambda_th, o->margin_tol,
+                     (int)m.algo, pos, w, n, (const void *)labels, (void *)ctx->stream, ctx->tune.xnb,
+                     ctx->tune.xe, ctx->tune.tz, ctx->tune.tseg, ctx->tune.dep_wg);
+}
+
+static cm_status measure_graph(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                               void *mesh, void *meshB, int8_t *labels, double *pk0, double *pk2, double *pk4,
+                               int64_t *nmodes, cm_measure_info *info, bool *done) {
+    *done = false;
+    const std::string key = graph_key(ctx, o, pos, w, n, labels);
+    GraphEntry *e = nullptr;
+    for (auto &g : ctx->graph_cache)
+        if (g.key == key) e = &g;
+    if (!e) {  // first sighting: the eager run warms the work buffers
+        if (ctx->graph_cache.size() >= 8) {
+            if (ctx->graph_cache.front().exec) hipGraphExecDestroy(ctx->graph_cache.front().exec);
+            ctx->graph_cache.erase(ctx->graph_cache.begin());
+        }
+        ctx->graph_cache.push_back(GraphEntry{key});
+        return CM_OK;
+    }
+    if (e->bad) return CM_OK;
+    if (!e->exec || e->gen != ctx->scratch_gen) {
+        if (e->exec) hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+        const uint64_t gen = ctx->scratch_gen;
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            e->bad = true;
+            return CM_OK;
+        }
+        FusedOut dev;
+        const cm_status s = enqueue_fused(ctx, o, pos, w, n, mesh, meshB, labels, false, &dev);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        hipGetLastError();
+        if (s != CM_OK || ec != hipSuccess || !g || gen != ctx->scratch_gen ||
+            hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            e->exec = nullptr;
+            e->bad = true;  // this key runs eagerly from now on
+            ctx->err.clear();
+            return CM_OK;
+        }
+        hipGraphDestroy(g);
+        e->gen = gen;
+        ++ctx->graph_captures;
+    }
+    CM_HIP(ctx, hipGraphLaunch(e->exec, ctx->stream));
+    ++ctx->graph_launches;
+    FusedOut dev;
+    cm_status s;
+    if ((s = fused_out(ctx, o->nshells, &dev)) != CM_OK) return s;
+    bool redo = false;
+    if ((s = finish_fused(ctx, o, dev, pk0, pk2, pk4, nmodes, info, &redo)) != CM_OK) return s;
+    if (redo) {
+        e->bad = true;  // the record estimate is too small for this input: eager exact path
+        return CM_OK;
+    }
+    *done = true;
     return CM_OK;
 }
 
@@ -104,25 +200,20 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     if (o->fused) {
         void *meshB = nullptr;
         if (o->interlace && (s = scratch(ctx, "m.meshB", sizeof(float) * n3, &meshB)) != CM_OK) return s;
-        // first attempt without the record-count read-back; repeat exactly on
-        // overflow (flag in the result block, read back with the shells)
-        FusedOut fo;
-        if ((s = fused_out(ctx, o->nshells, &fo)) != CM_OK) return s;
-        for (int attempt = 0; attempt < 2; ++attempt) {
-            unsigned *ovf = nullptr;
-            if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, attempt > 0, &ovf, true, fo.overflow)) !=
+        if (ctx->graphs && !ctx->profiling) {
+            bool done = false;
+            if ((s = measure_graph(ctx, o, pos, w, n, mesh, meshB, labels, pk0, pk2, pk4, nmodes, info, &done)) !=
                 CM_OK)
                 return s;
-            if (meshB) {
-                cm_mas_desc mb = md;
-                mb.interlace_shift += 0.5;  // half a cell on every axis
-                if ((s = mesh_assign_impl(ctx, &mb, pos, w, n, meshB, CM_F32, 0, attempt > 0, &ovf, false,
-                                          fo.overflow)) != CM_OK)
-                    return s;
-            }
+            if (done) return CM_OK;
+        }
+        // first attempt without the record-count read-back; repeat exactly on
+        // overflow (flag in the result block, read back with the shells)
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            FusedOut dev;
+            if ((s = enqueue_fused(ctx, o, pos, w, n, mesh, meshB, labels, attempt > 0, &dev)) != CM_OK) return s;
             bool redo = false;
-            if ((s = measure_fused_tail(ctx, o, mesh, meshB, n, pk0, pk2, pk4, nmodes, labels, info, &redo)) != CM_OK)
-                return s;
+            if ((s = finish_fused(ctx, o, dev, pk0, pk2, pk4, nmodes, info, &redo)) != CM_OK) return s;
             if (!redo) return CM_OK;
         }
         return fail(ctx, CM_ERR_DEVICE, "cm_measure: record buffer overflow after exact sizing");
@@ -198,6 +289,7 @@ cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, co
     if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure: float32 positions only");
     if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure: binning");
     if (o->do_tweb && !labels) return fail(ctx, CM_ERR_INVALID, "cm_measure: do_tweb needs a labels buffer");
+    TuneScope ts(ctx, o->mas.N);
     return measure_impl(ctx, o, pos, w, n, pk0, pk2, pk4, nmodes, labels, info);
 }
 }  // namespace cm

@@ -62,6 +62,17 @@ cm_status cm_set_stream(cm_ctx *ctx, void *stream);
 void *cm_get_stream(cm_ctx *ctx);
 cm_status cm_synchronize(cm_ctx *ctx);
 
+/* hipGraph replay of repeated fused measurements (cm_measure, fused = 1): the
+ * first call with given options, pointers, particle count, stream and launch
+ * shapes runs eagerly, the second is captured into a graph, later ones launch
+ * the graph (one submission instead of ~20 kernel launches).  A graph is
+ * recaptured when one of the context's work buffers has been reallocated.  On
+ * by default; 0 turns it off for this context.  No reference counterpart (the
+ * reference has no GPU path). */
+cm_status cm_set_graphs(cm_ctx *ctx, int on);
+/* Graphs captured and graph launches made by this context so far. */
+cm_status cm_graph_counts(cm_ctx *ctx, int64_t *captured, int64_t *launched);
+
 /* Device memory and copies for hosts that do not include HIP headers. */
 cm_status cm_dev_alloc(cm_ctx *ctx, size_t bytes, void **ptr);
 cm_status cm_dev_free(cm_ctx *ctx, void *ptr);
@@ -230,6 +241,32 @@ typedef struct {
 cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w,
                      size_t n, double *pk0, double *pk2, double *pk4, int64_t *nmodes,
                      int8_t *labels, cm_measure_info *info);
+
+/* ------------------------------------------------------------- autotuner */
+/* Launch shapes of the fused path; 0 = built-in choice. */
+typedef struct {
+    int xnb;     /* kz columns per workgroup of the fused x pass: 8 or 16 */
+    int xe;      /* elements per thread of the fused x pass: 8 or 16 */
+    int tz;      /* T-web tile width along z: 64, 32, 16 or 8 */
+    int tseg;    /* T-web x segments per tile column (0: occupancy model) */
+    int dep_wg;  /* deposit workgroups per CU (0: 8) */
+} cm_tune;
+
+/* Tile-shape autotuner (no reference counterpart).  Times the fused
+ * measurement of the given request (o->fused is taken as 1) under candidate
+ * shapes, one knob at a time in the order of the struct (coordinate descent on
+ * the summed kernel time per mock, best of two runs per candidate, a change
+ * kept only when it gains more than 1%), and stores the best shapes for meshes
+ * of this N on this device in a process-wide table that every context's
+ * cm_measure and cm_measure_fkp consult (server slots included).  best
+ * (nullable) receives the shapes, ms_before / ms_after (nullable) the kernel
+ * time per mock with the built-in and with the chosen shapes. */
+cm_status cm_autotune(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
+                      int8_t *labels, cm_tune *best, double *ms_before, double *ms_after);
+/* Set (t NULL: clear) or read the process-wide shapes for meshes of N on the
+ * context's device; cm_get_tune returns zeros when none are set. */
+cm_status cm_set_tune(cm_ctx *ctx, int N, const cm_tune *t);
+cm_status cm_get_tune(cm_ctx *ctx, int N, cm_tune *t);
 
 /* ------------------------------------------------ FKP with a random catalogue */
 typedef struct {
