@@ -160,7 +160,9 @@ def served(cm, posd, b, slots, nreq, rate=None, seed=0):
         labels = [torch.empty((N, N, N), dtype=torch.int8, device="cuda") for _ in range(slots + 1)]
         mk = lambda i: srv.request(N, L, "TSC", b["nshells"], b["kmin"], b["dk"], pos=posd,
                                    labels=labels[i % len(labels)], **kw)
-        for t in [srv.submit(mk(i)) for i in range(2 * slots)]:  # warm the slots' buffers
+        # warm the slots' buffers and graphs: every slot sees every labels
+        # buffer (part of a request's graph key) at least twice before timing
+        for t in [srv.submit(mk(i)) for i in range(4 * slots * len(labels))]:
             t.result()
         torch.cuda.synchronize()
         tickets = [None] * nreq
@@ -205,6 +207,21 @@ def batched(cm, nmocks=64, slots=3, seed=7):
         ndel = max(1, Nm // 128) if Nm != 192 else 1
         specs.append((Nm, NLm, Lm, sch, ndel, 1000 + i))
     npart = sum(s[1] ** 3 for s in specs)
+    # launch shapes per mesh size from the autotuner (process-wide table the
+    # server's slot contexts consult), tuned on a TSC + T-web mock of that N
+    tuned = {}
+    with cm.Context(0) as tctx:
+        for Nm in Ns:
+            NLm = {128: 64, 192: 96, 256: 128, 384: 192, 512: 256}[Nm]
+            Lm = 1000.0
+            posm = tctx.zeldovich(NLm, Lm, A=1e4, ns=-1.5, seed=77)
+            bm = cm.binning(Nm, Lm, ndel=max(1, Nm // 128) if Nm != 192 else 1)
+            r = tctx.autotune(posm, Nm, Lm, "TSC", 64, bm["kmin"], bm["dk"], do_tweb=True, R_smooth=2.0 * Lm / Nm,
+                              margin_tol=1e-6)
+            tuned[str(Nm)] = {"tune": r["tune"], "kernel_ms_before": round(r["ms_before"], 3),
+                              "kernel_ms_after": round(r["ms_after"], 3)}
+            del posm
+        torch.cuda.synchronize()
     with cm.Server(0, slots=slots) as srv:
         def tickets():
             out = []
@@ -223,6 +240,7 @@ def batched(cm, nmocks=64, slots=3, seed=7):
             "particles": int(npart), "seconds": round(T, 3),
             "workload": "64 uneven mocks generated on the GPU (N 128-512, NL^3 particles 1.1e5-5.7e7, "
                         "NGP/CIC/TSC, 64 shells, T-web)",
+            "autotune": tuned, "graphs": "hipGraph replay of repeated request shapes per slot",
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
 
@@ -352,8 +370,16 @@ def main():
         torch.cuda.synchronize()
         del xa, xb
 
-    # optimised path
+    # launch shapes from the autotuner (kernel time per mock before/after); the
+    # profiler runs (--fused-only) keep the built-in shapes so that every
+    # dispatch of a kernel in the trace has the same shape
+    tune = None
+    if not args.fused_only:
+        tune = ctx.autotune(posd, N, L, "TSC", b["nshells"], b["kmin"], b["dk"], do_tweb=True, R_smooth=R_SMOOTH,
+                            lambda_th=LAMBDA_TH, margin_tol=1e-6, labels=labels)
+    # optimised path (hipGraph replay after the first two steps)
     out, total, lat = run_steps(ctx, posd, b, True, args.steps, args.warmup, labels)
+    graph_counts = ctx.graph_counts()
     value = args.steps / total
     # per-kernel times (HIP events on the library stream) over a few more steps
     ctx.profile(True)
@@ -462,6 +488,9 @@ def main():
         "tweb_marginal": out["tweb_marginal"],
         "kernels_ms_per_step": {k: round(v["mean_ms"] * v["launches_per_step"], 4) for k, v in ktimes.items()},
         "roofline": roof,
+        "autotune": {"tune": tune["tune"], "kernel_ms_before": round(tune["ms_before"], 4),
+                     "kernel_ms_after": round(tune["ms_after"], 4)},
+        "graphs": {"captured": graph_counts[0], "launched": graph_counts[1]},
         "served": serve,
         "cpu_baseline": cpu,
         "input_gen_ms": round(1e3 * gen_s, 2),
