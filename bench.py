@@ -60,7 +60,7 @@ def algorithmic_bytes(name, n_part, n_rec):
 KERNELS = ["mas_tile_count", "mas_tile_prefix", "mas_tile_scan", "mas_tile_items", "mas_tile_scatter", "mas_tile_zero_split",
            "mas_tile_deposit", "mas_plain", "fill", "mesh_sum", "mesh_sum_final", "fft_z_r2c", "fused_sum",
            "fft_y_fwd", "fused_zero", "fused_x", "fft_x_fwd", "pk_zero", "pk_shells", "tweb_kmul", "fft_x_inv",
-           "fft_y_inv", "fft_z_c2r", "tweb_grad", "tweb_zero", "tweb_hess_eig", "tweb_fused"]
+           "fft_y_inv", "fft_z_c2r", "tweb_grad", "tweb_zero", "tweb_hess_eig", "tweb_fused", "tweb_rare"]
 
 
 # rocprofv3 kernel names (demangled prefix) -> the library's kernel names, for
@@ -70,7 +70,7 @@ ROCPROF_NAMES = [("cm::k_tile_count_lds", "mas_tile_count"), ("cm::k_tile_count"
                  ("cm::k_tile_items", "mas_tile_items"), ("cm::k_tile_scatter", "mas_tile_scatter"),
                  ("cm::k_tile_zero_split", "mas_tile_zero_split"), ("cm::k_tile_deposit", "mas_tile_deposit"),
                  ("cm::k_rows_r2c", "fft_z_r2c"), ("cm::k_rows_c2r", "fft_z_c2r"), ("cm::k_x_fused", "fused_x"),
-                 ("cm::k_tweb_fused", "tweb_fused")]
+                 ("cm::k_tweb_fused", "tweb_fused"), ("cm::k_tweb_rare", "tweb_rare")]
 
 
 def lib_name(kname):

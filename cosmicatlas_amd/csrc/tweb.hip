@@ -94,19 +94,27 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
 
 
 // Fused FD Hessian + labels.  One workgroup owns a TY x TZ column of cells in
-// (y,z) and a segment of x planes and marches along x.  LDS holds a ring of
-// the last five phi planes (halo 4 in y,z) and the centre plane's G_y, G_z
-// (halo 2).  Each thread owns IP interior cells and keeps their last five
-// G_x, G_y, G_z values in registers; the march is unrolled by five so the ring
-// slots and register histories are addressed with compile-time indices (no
-// moves).  Halo values of G_y, G_z are computed directly on the centre plane.
-// The next phi plane is loaded into registers while the current one is
-// processed.  G and T never touch HBM.  Composition order of the reference:
-// G_a = D_a phi, then D_x G_x, D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z
-// (Miscelanious.cpp:1684-1700).  Labels from the inertia of T - lambda_th I
-// (positive_eigs_f: float with a certified sign, else FP64); closed-form FP64
-// eigenvalues only for the lambda output and for cells whose margin bound is
-// below tol.
+// (y,z) and a segment of x planes and marches along x, one barrier per plane:
+//   step s: phi plane s enters the LDS ring; barrier; G_a = D_a phi of plane
+//   s-2 (halo 2 in y,z for G_y, G_z) from ring planes s-3..s and the thread's
+//   own plane s-4 values (kept in registers from the step before), G_y and
+//   G_z written to the G slot of plane s-2; T of plane c = s-4 from G_x, G_y,
+//   G_z of planes c-2..c+2 (x derivative, register histories) and the G slot
+//   of plane c (y, z derivatives, written two steps earlier).
+// Five phi slots (plane s-4 is no longer read from LDS in step s, so plane
+// s+1 may replace it before the next barrier) and three G slots make every
+// LDS slot's next write follow the barrier after its last read.  Each thread owns IP interior cells and keeps
+// their last five G_x, G_y, G_z values in registers; the march is unrolled by
+// five so the history slots are compile-time indices.  The next phi plane is
+// loaded into registers while the current one is processed.  G and T never
+// touch HBM.  Composition order of the reference: G_a = D_a phi, then D_x G_x,
+// D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z (Miscelanious.cpp:1684-1700).
+// Labels from the inertia of T - lambda_th I: float signs with a certified
+// rounding bound.  Cells whose signs are not certain or whose margin bound is
+// below tol are appended to a list (INL = false) that k_tweb_rare settles in
+// FP64 afterwards, so the marching loop makes no calls (a call site inside it
+// costs ~35 VGPRs of saved registers); with the lambda output (INL = true)
+// every cell takes the out-of-line FP64 branch.
 // Rare path of the fused kernel (kept out of line so its FP64 registers do not
 // count against the marching loop): closed-form eigenvalues, optional lambda
 // output, and the exact margin test.  Returns 1 for a marginal cell.
@@ -124,20 +132,33 @@ __device__ __attribute__((noinline)) unsigned eig_margin(float txx, float tyy, f
     return (test && fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth))) < tol) ? 1u : 0u;
 }
 
-template <int TY, int TZ, int NT>
+// FP64 count of positive eigenvalues of B (the uncertain float case).
+__device__ __attribute__((noinline)) int positive_eigs_d(float b00, float b11, float b22, float a10, float a20,
+                                                         float a21) {
+    double ad, f2;
+    return positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+}
+
+template <int TY, int TZ>
+constexpr size_t tweb_lds_floats() {
+    return 5 * (size_t)(TY + 8) * (TZ + 8) + 6 * (size_t)(TY + 4) * (TZ + 4);
+}
+
+template <int TY, int TZ, int NT, bool INL>
 __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi, int N, int segLen, float ih2,
                                                    double lth, double tol, int8_t *__restrict__ labels,
-                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats) {
+                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats,
+                                                   unsigned *__restrict__ rare, unsigned *__restrict__ nrare) {
     constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
-    constexpr int GZ = TZ + 4;                             // centre-plane G (halo 2), rows TY + 4
+    constexpr int GZ = TZ + 4, GP = (TY + 4) * GZ;         // G plane (halo 2)
     constexpr int IP = TY * TZ / NT;                       // interior cells per thread
     constexpr int PF = (HP + NT - 1) / NT;                 // phi values per thread per plane
     constexpr int NH1 = 4 * TZ, NH2 = 4 * TY;              // halo positions: y rows, z columns
     static_assert(IP * NT == TY * TZ && NH1 + NH2 <= NT, "tile does not match the workgroup");
     extern __shared__ float sm[];
-    float *ring = sm;  // 5 planes
-    float *sgy = ring + 5 * HP;
-    float *sgz = sgy + (TY + 4) * GZ;
+    float *ring = sm;             // 5 phi planes
+    float *sgy = ring + 5 * HP;   // 3 G_y planes
+    float *sgz = sgy + 3 * GP;    // 3 G_z planes
     __shared__ unsigned scnt[5];
     const int tid = threadIdx.x;
     if (tid < 5) scnt[tid] = 0;
@@ -157,7 +178,7 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         const int yy = q / HZ, zz = q - (q / HZ) * HZ;
         foff[k] = q < HP ? ((ty0 - 4 + yy + N) % N) * N + (tz0 - 4 + zz + N) % N : 0;
     }
-    // interior cells: phi-plane index, centre-G index, in-plane output offset
+    // interior cells: phi-plane index, G-plane index, in-plane output offset
     int po[IP], gq[IP], coff[IP];
 #pragma unroll
     for (int k = 0; k < IP; ++k) {
@@ -181,11 +202,11 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         hpo = (a + 2) * HZ + b + 2;
     }
 
-    float hx[IP][5], hy[IP][5], hz[IP][5];
+    float hx[IP][5], hy[IP][5], hz[IP][5], p3[IP][5];
 #pragma unroll
     for (int k = 0; k < IP; ++k)
 #pragma unroll
-        for (int i = 0; i < 5; ++i) hx[k][i] = hy[k][i] = hz[k][i] = 0.f;
+        for (int i = 0; i < 5; ++i) hx[k][i] = hy[k][i] = hz[k][i] = p3[k][i] = 0.f;
     unsigned long long cls = 0;  // class counts, 16 bits each (at most 2048 cells per thread)
     unsigned nmarg = 0;
     float pf[PF];
@@ -196,10 +217,14 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         for (int k = 0; k < PF; ++k)
             if (tid + k * NT < HP) pf[k] = plane[foff[k]];
     };
+    // G of plane p (p >= x0 - 2) in G slot (p - x0 + 3) % 3
+    auto gslot = [&](int p) { return (p - x0 + 3) % 3 * GP; };
 
-    // Step s (J = (s - x0 + 4) mod 5): plane s enters ring slot (J+4)%5; G at
-    // plane s-2 goes to history slot J; cells of plane c = s-4 are output, with
-    // G(c-2..c+2) in history slots (J+1)%5, (J+2)%5, (J+3)%5, (J+4)%5, J.
+    // Step s (J = (s - x0 + 4) mod 5): plane s enters ring slot (J+4)%5 (the
+    // slot of plane s-5); G at plane s-2 goes to history slot J; cells of
+    // plane c = s-4 are output, with G(c-2..c+2) in history slots (J+1)%5,
+    // (J+2)%5, (J+3)%5, (J+4)%5, J.  p3[k][J] keeps plane s-3 at the thread's
+    // cell for step s+1 (where it is plane s-4).
     auto step = [&](auto Jc, int s) {
         constexpr int J = decltype(Jc)::value;
         float *cur = ring + ((J + 4) % 5) * HP;
@@ -208,47 +233,83 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
             if (tid + k * NT < HP) cur[tid + k * NT] = pf[k];
         if (s + 1 < x1 + 4) fetch(s + 1);  // in flight while plane s is processed
         __syncthreads();
-        const float *pm4 = ring + (J % 5) * HP, *pm3 = ring + ((J + 1) % 5) * HP;
-        const float *pm2 = ring + ((J + 2) % 5) * HP, *pm1 = ring + ((J + 3) % 5) * HP;
-        if (s >= x0) {
-#pragma unroll
-            for (int k = 0; k < IP; ++k) {
-                const int o = po[k];
-                hx[k][J] = fd4s(pm4[o], pm3[o], pm1[o], cur[o]);
-                hy[k][J] = fd4s(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ]);
-                hz[k][J] = fd4s(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2]);
-                sgy[gq[k]] = hy[k][(J + 3) % 5];
-                sgz[gq[k]] = hz[k][(J + 3) % 5];
-            }
-        }
-        if (s >= x0 + 4 && hkind) {
-            const int o = hpo;
-            if (hkind == 1) sgy[hq] = fd4s(pm4[o - 2 * HZ], pm4[o - HZ], pm4[o + HZ], pm4[o + 2 * HZ]);
-            sgz[hq] = fd4s(pm4[o - 2], pm4[o - 1], pm4[o + 1], pm4[o + 2]);
-        }
-        __syncthreads();
-        const int c = s - 4;
-        if (c < x0) return;
+        const float *pm3 = ring + ((J + 1) % 5) * HP, *pm2 = ring + ((J + 2) % 5) * HP;
+        const float *pm1 = ring + ((J + 3) % 5) * HP;
+        // registers are written unconditionally (before x0 the values come from
+        // a partly filled ring and are never read), so no history value has to
+        // survive a skipped write
 #pragma unroll
         for (int k = 0; k < IP; ++k) {
+            const int o = po[k];
+            p3[k][J] = pm3[o];
+            hx[k][J] = fd4s(p3[k][(J + 4) % 5], p3[k][J], pm1[o], cur[o]);
+            hy[k][J] = fd4s(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ]);
+            hz[k][J] = fd4s(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2]);
+        }
+        if (s >= x0) {
+            float *gyw = sgy + gslot(s - 2), *gzw = sgz + gslot(s - 2);
+#pragma unroll
+            for (int k = 0; k < IP; ++k) {
+                gyw[gq[k]] = hy[k][J];
+                gzw[gq[k]] = hz[k][J];
+            }
+            if (hkind) {
+                const int o = hpo;
+                if (hkind == 1) gyw[hq] = fd4s(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ]);
+                gzw[hq] = fd4s(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2]);
+            }
+        }
+        const int c = s - 4;
+        if (c < x0) return;
+        const float *gy = sgy + gslot(c), *gz = sgz + gslot(c);
+        int8_t *lplane = labels + (size_t)c * NN * NN;
+        // T of cell k: txx, tyy, tzz, a10 = D_x G_y, a20 = D_x G_z, a21 = D_y G_z
+        auto tensor = [&](int k, float t[6]) {
             const int q = gq[k];
-            const float txx = fd4s(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J]);
-            const float a10 = fd4s(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J]);
-            const float a20 = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
-            const float tyy = fd4s(sgy[q - 2 * GZ], sgy[q - GZ], sgy[q + GZ], sgy[q + 2 * GZ]);
-            const float a21 = fd4s(sgz[q - 2 * GZ], sgz[q - GZ], sgz[q + GZ], sgz[q + 2 * GZ]);
-            const float tzz = fd4s(sgz[q - 2], sgz[q - 1], sgz[q + 1], sgz[q + 2]);
+            t[0] = fd4s(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J]);
+            t[1] = fd4s(gy[q - 2 * GZ], gy[q - GZ], gy[q + GZ], gy[q + 2 * GZ]);
+            t[2] = fd4s(gz[q - 2], gz[q - 1], gz[q + 1], gz[q + 2]);
+            t[3] = fd4s(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J]);
+            t[4] = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
+            t[5] = fd4s(gz[q - 2 * GZ], gz[q - GZ], gz[q + GZ], gz[q + 2 * GZ]);
+        };
+        int v[IP];
+        bool unc[IP], mm[IP];
+        bool any = INL && lam != nullptr;
+#pragma unroll
+        for (int k = 0; k < IP; ++k) {
+            float t[6];
+            tensor(k, t);
             float det_lo, fro2;
-            const int v = positive_eigs_s(txx - lthf, tyy - lthf, tzz - lthf, a10, a20, a21, det_lo, fro2);
-            const int lab = 4 - v;
-            const size_t cell = (size_t)c * NN * NN + (size_t)coff[k];
-            labels[cell] = (int8_t)lab;
-            cls += 1ull << (16 * (lab - 1));
+            v[k] = positive_eigs_nb(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], unc[k], det_lo, fro2);
             // margin bound min|mu| >= |det B| / ||B||_F^2, taken conservatively
             // (|det| minus its rounding bound); the FP64 eigenvalues settle it
-            const bool maybe_marginal = tols > 0.f && det_lo < tols * fro2 * 1.0001f;
-            if (lam || maybe_marginal)
-                nmarg += eig_margin(txx, tyy, tzz, a10, a20, a21, ih2, lth, tol, maybe_marginal, lam, cell, n3);
+            mm[k] = tols > 0.f && det_lo < tols * fro2 * 1.0001f;
+            any = any || unc[k] || mm[k];
+        }
+        if (__builtin_expect(any, 0)) {
+#pragma unroll
+            for (int k = 0; k < IP; ++k) {
+                if (!(unc[k] || mm[k] || (INL && lam))) continue;
+                const size_t cell = (size_t)c * NN * NN + (size_t)coff[k];
+                if constexpr (INL) {  // T is recomputed here, so it is not live across the common path
+                    float t[6];
+                    tensor(k, t);
+                    if (unc[k]) v[k] = positive_eigs_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5]);
+                    if (lam || mm[k])
+                        nmarg += eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, mm[k], lam, cell, n3);
+                } else {
+                    // provisional float label; k_tweb_rare recounts in FP64 and
+                    // moves the class count if the label changes
+                    rare[atomicAdd(nrare, 1u)] = (unsigned)cell;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IP; ++k) {
+            const int lab = 4 - v[k];
+            lplane[coff[k]] = (int8_t)lab;
+            cls += 1ull << (16 * (lab - 1));
         }
     };
     using I0 = std::integral_constant<int, 0>;
@@ -275,6 +336,53 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     if (tid < 5 && scnt[tid]) atomicAdd(&stats[tid], (unsigned long long)scnt[tid]);
 }
 
+// Settles the cells listed by k_tweb_fused<..., false>: recomputes their T
+// from phi with the same float operations in the same order (G = D phi, then
+// D G; periodic wrap), so the float tensor is the one the marching kernel saw;
+// then the FP64 sign count when the float one is not certain (fixing the
+// label and moving one count between classes), and the FP64 margin test.
+// The list holds at most N^3 cells (each cell is listed once).
+__global__ void __launch_bounds__(256) k_tweb_rare(const float *__restrict__ phi, int N, float ih2, double lth,
+                                                   double tol, int8_t *__restrict__ labels,
+                                                   unsigned long long *__restrict__ stats,
+                                                   const unsigned *__restrict__ rare, const unsigned *__restrict__ nrare) {
+    const unsigned n = *nrare;
+    const size_t NN = (size_t)N, n3 = NN * NN * NN;
+    const float lthf = (float)(lth / (double)ih2);
+    const float tols = (float)(tol / (double)ih2);
+    auto P = [&](int x, int y, int z) {
+        return phi[((size_t)((x + N) % N) * NN + (size_t)((y + N) % N)) * NN + (size_t)((z + N) % N)];
+    };
+    auto gx = [&](int x, int y, int z) { return fd4s(P(x - 2, y, z), P(x - 1, y, z), P(x + 1, y, z), P(x + 2, y, z)); };
+    auto gy = [&](int x, int y, int z) { return fd4s(P(x, y - 2, z), P(x, y - 1, z), P(x, y + 1, z), P(x, y + 2, z)); };
+    auto gz = [&](int x, int y, int z) { return fd4s(P(x, y, z - 2), P(x, y, z - 1), P(x, y, z + 1), P(x, y, z + 2)); };
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const size_t cell = rare[i];
+        const int z = (int)(cell % NN), y = (int)((cell / NN) % NN), x = (int)(cell / (NN * NN));
+        float t[6];
+        t[0] = fd4s(gx(x - 2, y, z), gx(x - 1, y, z), gx(x + 1, y, z), gx(x + 2, y, z));  // txx
+        t[1] = fd4s(gy(x, y - 2, z), gy(x, y - 1, z), gy(x, y + 1, z), gy(x, y + 2, z));  // tyy
+        t[2] = fd4s(gz(x, y, z - 2), gz(x, y, z - 1), gz(x, y, z + 1), gz(x, y, z + 2));  // tzz
+        t[3] = fd4s(gy(x - 2, y, z), gy(x - 1, y, z), gy(x + 1, y, z), gy(x + 2, y, z));  // a10 = D_x G_y
+        t[4] = fd4s(gz(x - 2, y, z), gz(x - 1, y, z), gz(x + 1, y, z), gz(x + 2, y, z));  // a20 = D_x G_z
+        t[5] = fd4s(gz(x, y - 2, z), gz(x, y - 1, z), gz(x, y + 1, z), gz(x, y + 2, z));  // a21 = D_y G_z
+        bool unc;
+        float det_lo, fro2;
+        const int vf = positive_eigs_nb(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], unc, det_lo, fro2);
+        const bool mm = tols > 0.f && det_lo < tols * fro2 * 1.0001f;
+        if (unc) {
+            const int v = positive_eigs_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5]);
+            if (v != vf) {
+                labels[cell] = (int8_t)(4 - v);
+                atomicAdd(&stats[3 - vf], ~0ull);  // class 4 - vf loses the cell
+                atomicAdd(&stats[3 - v], 1ull);
+            }
+        }
+        if (mm && eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, true, nullptr, cell, n3))
+            atomicAdd(&stats[4], 1ull);
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long *a, int n) {
     if ((int)threadIdx.x < n) a[threadIdx.x] = 0ull;
 }
@@ -290,14 +398,15 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
 template <int TY, int TZ, int NT>
 static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol,
                                 int8_t *labels, float *lam, unsigned long long *stats) {
-    const size_t lds = sizeof(float) * (5 * (size_t)(TY + 8) * (TZ + 8) + 2 * (size_t)(TY + 4) * (TZ + 4));
+    const bool inl = lam != nullptr;
+    auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
+    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
     // x segments: a segment re-reads 8 halo planes; pick the count that
     // minimises (rounds of resident workgroups) x (planes per segment)
     const int tiles = (N / TY) * (N / TZ);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tweb_fused<TY, TZ, NT>, NT, lds) != hipSuccess ||
-        per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     const int slots = std::max(1, ctx->cu_count) * per_cu;
     int nseg = 1;
@@ -318,9 +427,20 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
     const double ih = 1.0 / (12.0 * (L / (double)N));
     const float ih2 = (float)(ih * ih);
     cm_status s;
+    void *rare = nullptr, *nrare = nullptr;
+    if (!inl) {
+        if ((s = scratch(ctx, "tweb.rare", sizeof(unsigned) * (size_t)N * N * N, &rare)) != CM_OK) return s;
+        if ((s = scratch(ctx, "tweb.nrare", sizeof(unsigned long long), &nrare)) != CM_OK) return s;
+        if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, (unsigned long long *)nrare, 1)) != CM_OK)
+            return s;
+    }
     if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
-    return launch(ctx, "tweb_fused", k_tweb_fused<TY, TZ, NT>, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol,
-                  labels, lam, stats);
+    if ((s = launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats,
+                    (unsigned *)rare, (unsigned *)nrare)) != CM_OK)
+        return s;
+    if (inl) return CM_OK;
+    return launch(ctx, "tweb_rare", k_tweb_rare, dim3(std::max(1, ctx->cu_count) * 4), dim3(256), 0, phi, N, ih2, lth,
+                  tol, labels, stats, (const unsigned *)rare, (const unsigned *)nrare);
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.

@@ -141,6 +141,26 @@ __device__ __forceinline__ int positive_eigs_f(float b00, float b11, float b22, 
 // |e3| > 24u s^3 (u = 2^-24; the factors cover every rounding of the
 // evaluation with room to spare).  Otherwise FP64 decides.  Also returns |e3|
 // minus its bound (a lower bound of |det B|) and ||B||_F^2.
+// The float count of positive_eigs_s without its FP64 branch: unc is set when
+// a coefficient lies within its rounding bound (the count is then not
+// certain and the FP64 count must replace it).
+__device__ __forceinline__ int positive_eigs_nb(float b00, float b11, float b22, float a10, float a20, float a21,
+                                                bool &unc, float &det_lo, float &fro2) {
+    const float e1 = b00 + b11 + b22;
+    const float q10 = a10 * a10, q20 = a20 * a20, q21 = a21 * a21;
+    const float e2 = (b00 * b11 + b00 * b22 + b11 * b22) - (q10 + q20 + q21);
+    const float c0 = b11 * b22 - q21, c1 = a10 * b22 - a21 * a20, c2 = a10 * a21 - b11 * a20;
+    const float e3 = b00 * c0 - a10 * c1 + a20 * c2;
+    const float sc = (fabsf(b00) + fabsf(b11) + fabsf(b22)) + (fabsf(a10) + fabsf(a20) + fabsf(a21));
+    const float u = 5.9604645e-8f;
+    const float t1 = 8.0f * u * sc, t2 = 2.0f * t1 * sc, t3 = 1.5f * t2 * sc;
+    const float ae3 = fabsf(e3);
+    det_lo = ae3 - t3;
+    fro2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * (q10 + q20 + q21);
+    unc = fabsf(e1) <= t1 || fabsf(e2) <= t2 || ae3 <= t3 || !(sc < 1e12f);
+    return (e1 > 0.f) + ((e1 < 0.f) != (e2 > 0.f)) + ((e2 > 0.f) != (e3 < 0.f));
+}
+
 __device__ __forceinline__ int positive_eigs_s(float b00, float b11, float b22, float a10, float a20, float a21,
                                                float &det_lo, float &fro2) {
     const float e1 = b00 + b11 + b22;

@@ -86,3 +86,46 @@ def test_tweb_tclass(codes):
         assert [int(_tclass_ref(np.array([v], np.int8), codes)[0]) for v in (1, 2, 3, 4)] == [0, 1, 1, 1]
     if codes == [12, 34]:
         assert [int(_tclass_ref(np.array([v], np.int8), codes)[0]) for v in (1, 2, 3, 4)] == [0, 0, 1, 1]
+
+
+def test_tweb_rare_list_equals_inline_branch(ctx):
+    """Without the lambda output the marching kernel lists cells whose float
+    signs are uncertain or whose margin bound is below tol, and a second kernel
+    settles them in FP64; with it, every cell takes the inline FP64 branch.
+    Both must give the same labels, class counts and marginal count, also when
+    a large tol lists a good part of the mesh."""
+    N, L = 48, 100.0
+    phi = torch.from_numpy(_smooth_field(N, 11).astype(np.float32)).cuda()
+    for lth, frac in ((0.0, 1e-3), (0.01, 0.3)):
+        _, lam, _ = ctx.tweb_from_phi(phi, L, lth, want_lambdas=True)
+        tol = frac * float(lam.abs().max())
+        la, _, sa = ctx.tweb_from_phi(phi, L, lth, margin_tol=tol, want_lambdas=True)
+        lb, _, sb = ctx.tweb_from_phi(phi, L, lth, margin_tol=tol, want_lambdas=False)
+        assert torch.equal(la, lb)
+        assert sa == sb
+        assert sum(sb["counts"]) == N ** 3
+        if frac > 0.1:
+            assert sb["n_marginal"] > N ** 3 // 20  # the list held a good part of the mesh
+
+
+def test_tweb_uncertain_signs_settled_in_fp64(ctx):
+    """phi varying along x only: T_yy = T_zz = T_xy = ... = 0 exactly, so two
+    eigenvalues of T - 0 I are exactly zero and every float sign test is
+    uncertain; the FP64 count gives label 3 where T_xx > 0 and 4 elsewhere."""
+    N, L = 32, 64.0
+    x = np.arange(N)
+    phi1 = np.cos(2 * np.pi * 3 * x / N) + 0.3 * np.sin(2 * np.pi * 5 * x / N)
+    phi = np.broadcast_to(phi1[:, None, None], (N, N, N)).astype(np.float32)
+    d = L / N
+    c = np.array([1.0, -8.0, 0.0, 8.0, -1.0]) / (12 * d)
+    p = phi1.astype(np.float64)
+    g = sum(c[i] * np.roll(p, 2 - i) for i in range(5))
+    txx = sum(c[i] * np.roll(g, 2 - i) for i in range(5))
+    expect = np.where(txx > 0, 3, 4)
+    clear = np.abs(txx) > 1e-4 * np.abs(txx).max()
+    for want in (False, True):
+        lab, _, st = ctx.tweb_from_phi(torch.from_numpy(phi.copy()).cuda(), L, 0.0, want_lambdas=want)
+        lab = lab.cpu().numpy()
+        assert np.array_equal(lab[clear][:, 0, 0], expect[clear])
+        assert np.all(lab == lab[:, :1, :1])  # constant over y, z
+        assert sum(st["counts"]) == N ** 3
