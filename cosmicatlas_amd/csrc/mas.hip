@@ -143,17 +143,39 @@ __device__ __forceinline__ float4 make_record(const int c[3], const float f[3]) 
     return make_float4(__uint_as_float(pk), f[0], f[1], f[2]);
 }
 
-// Particles i = base + u*stride (u < U, i < i1) are loaded before any is used,
-// so one thread keeps U loads in flight.
 constexpr int kBatch = 8;
-template <int U>
-__device__ __forceinline__ void load_batch(const float *__restrict__ pos, const MasK &p, size_t base, size_t stride,
+// Particles i = base + u*stride (u < U, i < i1) are loaded before any is used,
+// so one thread keeps U loads in flight; an AoS particle is one 12-byte load.
+// A lane whose whole batch lies below i1 (all but the chunk's last batch)
+// takes the unguarded loads.
+template <int U, bool SOA>
+__device__ __forceinline__ void load_batch(const float *__restrict__ pos, size_t n, size_t base, size_t stride,
                                            size_t i1, float v[U][3]) {
+    auto one = [&](size_t i, float w[3]) {
+        if constexpr (SOA) {
+            w[0] = pos[i];
+            w[1] = pos[n + i];
+            w[2] = pos[2 * n + i];
+        } else {
+            const float3 q = reinterpret_cast<const float3 *>(pos)[i];
+            w[0] = q.x;
+            w[1] = q.y;
+            w[2] = q.z;
+        }
+    };
+    if (base + (size_t)(U - 1) * stride < i1) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const size_t i = base + (size_t)u * stride;
+        for (int u = 0; u < U; ++u) one(base + (size_t)u * stride, v[u]);
+    } else {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[u][a] = i < i1 ? load_coord(pos, i, a, p.soa, p.n) : 0.0f;
+        for (int u = 0; u < U; ++u) {
+            const size_t i = base + (size_t)u * stride;
+            if (i < i1) {
+                one(i, v[u]);
+            } else {
+                v[u][0] = v[u][1] = v[u][2] = 0.0f;
+            }
+        }
     }
 }
 
@@ -161,7 +183,7 @@ __device__ __forceinline__ void load_batch(const float *__restrict__ pos, const 
 // [g*chunk, (g+1)*chunk) and writes its histogram row hist[g*ntiles + tile];
 // global form (hist_lds == false, too many tiles for LDS): one row, global
 // atomics.
-template <int T, bool LDSH>
+template <int T, bool LDSH, bool SOA>
 __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ pos, MasK p, TileK t, int ntiles,
                                                      size_t chunk, unsigned *__restrict__ hist,
                                                      unsigned long long *dropped) {
@@ -181,7 +203,7 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
     unsigned ndrop = 0;
     for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * step) {
         float v[kBatch][3];
-        load_batch<kBatch>(pos, p, base, step, i1, v);
+        load_batch<kBatch, SOA>(pos, p.n, base, step, i1, v);
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             if (base + (size_t)u * step >= i1) continue;
@@ -343,7 +365,7 @@ __global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned l
 // Scatter records to their tiles.  LDS form: cursor[tile] = offsets[tile] +
 // prefix of earlier workgroups, integer LDS atomics hand out the slots; global
 // form: global cursors.
-template <int T, bool LDSH>
+template <int T, bool LDSH, bool SOA>
 __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__ pos, const float *__restrict__ w,
                                                        MasK p, TileK t, int ntiles, size_t chunk,
                                                        const unsigned *__restrict__ pref,
@@ -368,7 +390,7 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
     unsigned ovf = 0;
     for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * step) {
         float v[kBatch][3], wv[kBatch];
-        load_batch<kBatch>(pos, p, base, step, i1, v);
+        load_batch<kBatch, SOA>(pos, p.n, base, step, i1, v);
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const size_t i = base + (size_t)u * step;
@@ -656,14 +678,14 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     if ((s = scratch(ctx, "mas.nitems", sizeof(unsigned), &nitems)) != CM_OK) return s;
     auto *hp = (unsigned *)hist;
     if (lds_hist) {
-        if ((s = launch(ctx, "mas_tile_count", k_tile_count<T, true>, dim3(nrows), dim3(1024), hist_lds, pos, p, t,
-                        ntiles, chunk, hp, dropped)) != CM_OK)
+        if ((s = launch(ctx, "mas_tile_count", p.soa ? k_tile_count<T, true, true> : k_tile_count<T, true, false>,
+                        dim3(nrows), dim3(1024), hist_lds, pos, p, t, ntiles, chunk, hp, dropped)) != CM_OK)
             return s;
     } else {
         if ((s = fill(ctx, hp, (size_t)ntiles, 0u)) != CM_OK) return s;
         const unsigned g = grid_for(p.n, 1024 * kBatch, 2048);
-        if ((s = launch(ctx, "mas_tile_count", k_tile_count<T, false>, dim3(g), dim3(1024), 0, pos, p, t, ntiles,
-                        chunk, hp, dropped)) != CM_OK)
+        if ((s = launch(ctx, "mas_tile_count", p.soa ? k_tile_count<T, false, true> : k_tile_count<T, false, false>,
+                        dim3(g), dim3(1024), 0, pos, p, t, ntiles, chunk, hp, dropped)) != CM_OK)
             return s;
     }
     if ((s = launch(ctx, "mas_tile_prefix", k_tile_prefix, dim3(nblk), dim3(256), 0, hp, nrows, ntiles,
@@ -698,15 +720,18 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
                     (uint4 *)items, (unsigned *)nitems, (unsigned)cap, overflow)) != CM_OK)
         return s;
     if (lds_hist) {
-        if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter<T, true>, dim3(nrows), dim3(1024), hist_lds, pos, w, p,
-                        t, ntiles, chunk, (const unsigned *)hp, (const unsigned *)offsets, (unsigned *)counts,
-                        (float4 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
+        if ((s = launch(ctx, "mas_tile_scatter",
+                        p.soa ? k_tile_scatter<T, true, true> : k_tile_scatter<T, true, false>, dim3(nrows),
+                        dim3(1024), hist_lds, pos, w, p, t, ntiles, chunk, (const unsigned *)hp,
+                        (const unsigned *)offsets, (unsigned *)counts, (float4 *)sorted, (float *)wsorted,
+                        (unsigned)cap, overflow)) != CM_OK)
             return s;
     } else {
         const unsigned g = grid_for(p.n, 1024 * kBatch, 2048);
-        if ((s = launch(ctx, "mas_tile_scatter", k_tile_scatter<T, false>, dim3(g), dim3(1024), 0, pos, w, p, t,
-                        ntiles, chunk, (const unsigned *)hp, (const unsigned *)offsets, (unsigned *)counts,
-                        (float4 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
+        if ((s = launch(ctx, "mas_tile_scatter",
+                        p.soa ? k_tile_scatter<T, false, true> : k_tile_scatter<T, false, false>, dim3(g),
+                        dim3(1024), 0, pos, w, p, t, ntiles, chunk, (const unsigned *)hp, (const unsigned *)offsets,
+                        (unsigned *)counts, (float4 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
             return s;
     }
     if (!accumulate) {

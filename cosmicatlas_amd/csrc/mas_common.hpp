@@ -67,7 +67,10 @@ __device__ __forceinline__ int cell_of(double x, double d, int N) {
 // otherwise the quotient is taken by division, as in cell_of.
 __device__ __forceinline__ int cell_fast(double x, double d, double rd, int N, double &q) {
     q = x * rd;
-    if (floor(q - 0x1p-40) != floor(q + 0x1p-40)) q = x / d;
+    if (__builtin_expect(floor(q - 0x1p-40) != floor(q + 0x1p-40), 0)) {
+        q = x / d;
+        __asm__ volatile("" ::: "memory");  // keeps the division in the branch (not evaluated speculatively)
+    }
     int c = (int)floor(q);
     c = c >= N ? N - 1 : c;
     return c < 0 ? 0 : c;

@@ -110,11 +110,12 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
 // touch HBM.  Composition order of the reference: G_a = D_a phi, then D_x G_x,
 // D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z (Miscelanious.cpp:1684-1700).
 // Labels from the inertia of T - lambda_th I: float signs with a certified
-// rounding bound.  Cells whose signs are not certain or whose margin bound is
-// below tol are appended to a list (INL = false) that k_tweb_rare settles in
-// FP64 afterwards, so the marching loop makes no calls (a call site inside it
-// costs ~35 VGPRs of saved registers); with the lambda output (INL = true)
-// every cell takes the out-of-line FP64 branch.
+// rounding bound, recounted in FP64 in the kernel where the float signs are
+// not certain.  Cells whose margin bound is below tol are appended to the
+// workgroup's own slice of a list (INL = false; one LDS atomic per wave) that
+// k_tweb_rare settles with FP64 eigenvalues afterwards, so the marching loop makes no calls (a call site
+// inside it costs ~35 VGPRs of saved registers); with the lambda output
+// (INL = true) every cell takes the out-of-line FP64 branch.
 // Rare path of the fused kernel (kept out of line so its FP64 registers do not
 // count against the marching loop): closed-form eigenvalues, optional lambda
 // output, and the exact margin test.  Returns 1 for a marginal cell.
@@ -149,6 +150,7 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
                                                    double lth, double tol, int8_t *__restrict__ labels,
                                                    float *__restrict__ lam, unsigned long long *__restrict__ stats,
                                                    unsigned *__restrict__ rare, unsigned *__restrict__ nrare) {
+    // rare: this workgroup's slice of TY*TZ*segLen entries; nrare[wg]: its count
     constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
     constexpr int GZ = TZ + 4, GP = (TY + 4) * GZ;         // G plane (halo 2)
     constexpr int IP = TY * TZ / NT;                       // interior cells per thread
@@ -159,9 +161,12 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     float *ring = sm;             // 5 phi planes
     float *sgy = ring + 5 * HP;   // 3 G_y planes
     float *sgz = sgy + 3 * GP;    // 3 G_z planes
-    __shared__ unsigned scnt[5];
+    __shared__ unsigned scnt[5], s_nrare;
     const int tid = threadIdx.x;
     if (tid < 5) scnt[tid] = 0;
+    if (tid == 0) s_nrare = 0;
+    const unsigned wg = blockIdx.y * gridDim.x + blockIdx.x;
+    unsigned *myrare = INL ? nullptr : rare + (size_t)wg * ((size_t)TY * TZ * segLen);
     const int ntz = N / TZ;
     const int ty0 = (blockIdx.x / ntz) * TY, tz0 = (blockIdx.x % ntz) * TZ;
     const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
@@ -273,6 +278,14 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
             t[4] = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
             t[5] = fd4s(gz[q - 2 * GZ], gz[q - GZ], gz[q + GZ], gz[q + 2 * GZ]);
         };
+        struct T6 {
+            float t[6];
+        };
+        auto tt_of = [&](int k) {
+            T6 r;
+            tensor(k, r.t);
+            return r;
+        };
         int v[IP];
         bool unc[IP], mm[IP];
         bool any = INL && lam != nullptr;
@@ -280,17 +293,15 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         for (int k = 0; k < IP; ++k) {
             float t[6];
             tensor(k, t);
-            float det_lo, fro2;
-            v[k] = positive_eigs_nb(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], unc[k], det_lo, fro2);
-            // margin bound min|mu| >= |det B| / ||B||_F^2, taken conservatively
-            // (|det| minus its rounding bound); the FP64 eigenvalues settle it
-            mm[k] = tols > 0.f && det_lo < tols * fro2 * 1.0001f;
+            // margin pre-filter (a conservative root bound); the FP64
+            // eigenvalues settle the listed cells
+            v[k] = positive_eigs_nb(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], unc[k], tols, mm[k]);
             any = any || unc[k] || mm[k];
         }
         if (__builtin_expect(any, 0)) {
 #pragma unroll
             for (int k = 0; k < IP; ++k) {
-                if (!(unc[k] || mm[k] || (INL && lam))) continue;
+                if (!(unc[k] || (INL && (mm[k] || lam)))) continue;
                 const size_t cell = (size_t)c * NN * NN + (size_t)coff[k];
                 if constexpr (INL) {  // T is recomputed here, so it is not live across the common path
                     float t[6];
@@ -298,10 +309,27 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
                     if (unc[k]) v[k] = positive_eigs_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5]);
                     if (lam || mm[k])
                         nmarg += eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, mm[k], lam, cell, n3);
-                } else {
-                    // provisional float label; k_tweb_rare recounts in FP64 and
-                    // moves the class count if the label changes
-                    rare[atomicAdd(nrare, 1u)] = (unsigned)cell;
+                } else if (unc[k]) {
+                    const T6 r = tt_of(k);
+                    v[k] = positive_eigs_nb64(r.t[0] - lthf, r.t[1] - lthf, r.t[2] - lthf, r.t[3], r.t[4], r.t[5], tols,
+                                              mm[k]);
+                }
+            }
+            if constexpr (!INL) {
+                // margin candidates to the list: one atomic per wave per cell slot
+#pragma unroll
+                for (int k = 0; k < IP; ++k) {
+                    const unsigned long long bal = __ballot(mm[k]);
+                    if (bal) {
+                        const int lane = (int)__lane_id();
+                        const int leader = __ffsll((long long)bal) - 1;
+                        unsigned base = 0;
+                        if (lane == leader) base = atomicAdd(&s_nrare, (unsigned)__popcll(bal));
+                        base = __shfl(base, leader);
+                        if (mm[k])
+                            myrare[base + (unsigned)__popcll(bal & ((1ull << lane) - 1ull))] =
+                                (unsigned)((size_t)c * NN * NN + (size_t)coff[k]);
+                    }
                 }
             }
         }
@@ -334,53 +362,49 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     if (nmarg) atomicAdd(&scnt[4], nmarg);
     __syncthreads();
     if (tid < 5 && scnt[tid]) atomicAdd(&stats[tid], (unsigned long long)scnt[tid]);
+    if (!INL && tid == 0) nrare[wg] = s_nrare;
 }
 
-// Settles the cells listed by k_tweb_fused<..., false>: recomputes their T
-// from phi with the same float operations in the same order (G = D phi, then
-// D G; periodic wrap), so the float tensor is the one the marching kernel saw;
-// then the FP64 sign count when the float one is not certain (fixing the
-// label and moving one count between classes), and the FP64 margin test.
-// The list holds at most N^3 cells (each cell is listed once).
+// Settles the margin candidates listed by k_tweb_fused<..., false>:
+// recomputes their T from phi with the same float operations in the same
+// order (G = D phi, then D G; periodic wrap), so the float tensor is the one
+// the marching kernel saw, and counts the cells whose FP64 eigenvalues lie
+// within tol of lambda_th.  Block b reads the slice of marching workgroup b
+// (slice entries, nrare[b] of them used).
 __global__ void __launch_bounds__(256) k_tweb_rare(const float *__restrict__ phi, int N, float ih2, double lth,
-                                                   double tol, int8_t *__restrict__ labels,
-                                                   unsigned long long *__restrict__ stats,
-                                                   const unsigned *__restrict__ rare, const unsigned *__restrict__ nrare) {
-    const unsigned n = *nrare;
+                                                   double tol, unsigned long long *__restrict__ stats,
+                                                   const unsigned *__restrict__ rare, const unsigned *__restrict__ nrare,
+                                                   size_t slice) {
+    const unsigned n = nrare[blockIdx.x];
+    const unsigned *list = rare + (size_t)blockIdx.x * slice;
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
-    const float lthf = (float)(lth / (double)ih2);
-    const float tols = (float)(tol / (double)ih2);
-    auto P = [&](int x, int y, int z) {
-        return phi[((size_t)((x + N) % N) * NN + (size_t)((y + N) % N)) * NN + (size_t)((z + N) % N)];
-    };
-    auto gx = [&](int x, int y, int z) { return fd4s(P(x - 2, y, z), P(x - 1, y, z), P(x + 1, y, z), P(x + 2, y, z)); };
-    auto gy = [&](int x, int y, int z) { return fd4s(P(x, y - 2, z), P(x, y - 1, z), P(x, y + 1, z), P(x, y + 2, z)); };
-    auto gz = [&](int x, int y, int z) { return fd4s(P(x, y, z - 2), P(x, y, z - 1), P(x, y, z + 1), P(x, y, z + 2)); };
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const size_t cell = rare[i];
-        const int z = (int)(cell % NN), y = (int)((cell / NN) % NN), x = (int)(cell / (NN * NN));
-        float t[6];
-        t[0] = fd4s(gx(x - 2, y, z), gx(x - 1, y, z), gx(x + 1, y, z), gx(x + 2, y, z));  // txx
-        t[1] = fd4s(gy(x, y - 2, z), gy(x, y - 1, z), gy(x, y + 1, z), gy(x, y + 2, z));  // tyy
-        t[2] = fd4s(gz(x, y, z - 2), gz(x, y, z - 1), gz(x, y, z + 1), gz(x, y, z + 2));  // tzz
-        t[3] = fd4s(gy(x - 2, y, z), gy(x - 1, y, z), gy(x + 1, y, z), gy(x + 2, y, z));  // a10 = D_x G_y
-        t[4] = fd4s(gz(x - 2, y, z), gz(x - 1, y, z), gz(x + 1, y, z), gz(x + 2, y, z));  // a20 = D_x G_z
-        t[5] = fd4s(gz(x, y - 2, z), gz(x, y - 1, z), gz(x, y + 1, z), gz(x, y + 2, z));  // a21 = D_y G_z
-        bool unc;
-        float det_lo, fro2;
-        const int vf = positive_eigs_nb(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], unc, det_lo, fro2);
-        const bool mm = tols > 0.f && det_lo < tols * fro2 * 1.0001f;
-        if (unc) {
-            const int v = positive_eigs_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5]);
-            if (v != vf) {
-                labels[cell] = (int8_t)(4 - v);
-                atomicAdd(&stats[3 - vf], ~0ull);  // class 4 - vf loses the cell
-                atomicAdd(&stats[3 - v], 1ull);
-            }
+    const unsigned Nu = (unsigned)N;
+    unsigned marg = 0;
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned cell = list[i];  // N^3 <= 2^30
+        const int z = (int)(cell % Nu), y = (int)((cell / Nu) % Nu), x = (int)(cell / (Nu * Nu));
+        // wrapped coordinates c-4..c+4 along each axis, as plane/row offsets
+        unsigned xo[9], yo[9], zo[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            xo[d] = (unsigned)((x + d - 4 + N) % N) * Nu * Nu;
+            yo[d] = (unsigned)((y + d - 4 + N) % N) * Nu;
+            zo[d] = (unsigned)((z + d - 4 + N) % N);
         }
-        if (mm && eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, true, nullptr, cell, n3))
-            atomicAdd(&stats[4], 1ull);
+        auto P = [&](int dx, int dy, int dz) { return phi[xo[dx + 4] + yo[dy + 4] + zo[dz + 4]]; };
+        auto gx = [&](int a, int b, int c) { return fd4s(P(a - 2, b, c), P(a - 1, b, c), P(a + 1, b, c), P(a + 2, b, c)); };
+        auto gy = [&](int a, int b, int c) { return fd4s(P(a, b - 2, c), P(a, b - 1, c), P(a, b + 1, c), P(a, b + 2, c)); };
+        auto gz = [&](int a, int b, int c) { return fd4s(P(a, b, c - 2), P(a, b, c - 1), P(a, b, c + 1), P(a, b, c + 2)); };
+        float t[6];
+        t[0] = fd4s(gx(-2, 0, 0), gx(-1, 0, 0), gx(1, 0, 0), gx(2, 0, 0));  // txx
+        t[1] = fd4s(gy(0, -2, 0), gy(0, -1, 0), gy(0, 1, 0), gy(0, 2, 0));  // tyy
+        t[2] = fd4s(gz(0, 0, -2), gz(0, 0, -1), gz(0, 0, 1), gz(0, 0, 2));  // tzz
+        t[3] = fd4s(gy(-2, 0, 0), gy(-1, 0, 0), gy(1, 0, 0), gy(2, 0, 0));  // a10 = D_x G_y
+        t[4] = fd4s(gz(-2, 0, 0), gz(-1, 0, 0), gz(1, 0, 0), gz(2, 0, 0));  // a20 = D_x G_z
+        t[5] = fd4s(gz(0, -2, 0), gz(0, -1, 0), gz(0, 1, 0), gz(0, 2, 0));  // a21 = D_y G_z
+        marg += eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, true, nullptr, cell, n3);
     }
+    if (marg) atomicAdd(&stats[4], (unsigned long long)marg);
 }
 
 __global__ void k_zero_u64(unsigned long long *a, int n) {
@@ -428,19 +452,18 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
     const float ih2 = (float)(ih * ih);
     cm_status s;
     void *rare = nullptr, *nrare = nullptr;
+    const size_t nwg = (size_t)tiles * nseg, slice = (size_t)TY * TZ * segLen;
     if (!inl) {
-        if ((s = scratch(ctx, "tweb.rare", sizeof(unsigned) * (size_t)N * N * N, &rare)) != CM_OK) return s;
-        if ((s = scratch(ctx, "tweb.nrare", sizeof(unsigned long long), &nrare)) != CM_OK) return s;
-        if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, (unsigned long long *)nrare, 1)) != CM_OK)
-            return s;
+        if ((s = scratch(ctx, "tweb.rare", sizeof(unsigned) * nwg * slice, &rare)) != CM_OK) return s;
+        if ((s = scratch(ctx, "tweb.nrare", sizeof(unsigned) * nwg, &nrare)) != CM_OK) return s;
     }
     if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
     if ((s = launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats,
                     (unsigned *)rare, (unsigned *)nrare)) != CM_OK)
         return s;
     if (inl) return CM_OK;
-    return launch(ctx, "tweb_rare", k_tweb_rare, dim3(std::max(1, ctx->cu_count) * 4), dim3(256), 0, phi, N, ih2, lth,
-                  tol, labels, stats, (const unsigned *)rare, (const unsigned *)nrare);
+    return launch(ctx, "tweb_rare", k_tweb_rare, dim3((unsigned)nwg), dim3(256), 0, phi, N, ih2, lth, tol, stats,
+                  (const unsigned *)rare, (const unsigned *)nrare, slice);
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.

@@ -143,9 +143,18 @@ __device__ __forceinline__ int positive_eigs_f(float b00, float b11, float b22, 
 // minus its bound (a lower bound of |det B|) and ||B||_F^2.
 // The float count of positive_eigs_s without its FP64 branch: unc is set when
 // a coefficient lies within its rounding bound (the count is then not
-// certain and the FP64 count must replace it).
+// certain and the FP64 count must replace it).  marg is set when the
+// eigenvalue of B closest to zero may lie within tol of it, from a bound on
+// the roots of the characteristic polynomial: the reciprocals w = 1/mu solve
+// w^3 - (e2/e3) w^2 + (e1/e3) w - 1/e3 = 0, so by Fujiwara's bound
+// |w| <= 2 max(|e2/e3|, |e1/e3|^(1/2), |1/(2 e3)|^(1/3)), and min |mu| >= tol
+// whenever |e3| >= max(2 tol |e2|, 4 tol^2 |e1|, 4 tol^3) (each coefficient
+// taken at the unfavourable end of its rounding bound).  Unlike
+// |det|/||B||_F^2 this stays within a factor ~2 of min |mu| when the other two
+// eigenvalues differ widely.  Below tol = 1e-10 (tol^3 near float underflow)
+// the Frobenius bound min |mu| >= 2 |det| / ||B||_F^2 is used instead.
 __device__ __forceinline__ int positive_eigs_nb(float b00, float b11, float b22, float a10, float a20, float a21,
-                                                bool &unc, float &det_lo, float &fro2) {
+                                                bool &unc, float tol, bool &marg) {
     const float e1 = b00 + b11 + b22;
     const float q10 = a10 * a10, q20 = a20 * a20, q21 = a21 * a21;
     const float e2 = (b00 * b11 + b00 * b22 + b11 * b22) - (q10 + q20 + q21);
@@ -154,11 +163,36 @@ __device__ __forceinline__ int positive_eigs_nb(float b00, float b11, float b22,
     const float sc = (fabsf(b00) + fabsf(b11) + fabsf(b22)) + (fabsf(a10) + fabsf(a20) + fabsf(a21));
     const float u = 5.9604645e-8f;
     const float t1 = 8.0f * u * sc, t2 = 2.0f * t1 * sc, t3 = 1.5f * t2 * sc;
-    const float ae3 = fabsf(e3);
-    det_lo = ae3 - t3;
-    fro2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * (q10 + q20 + q21);
-    unc = fabsf(e1) <= t1 || fabsf(e2) <= t2 || ae3 <= t3 || !(sc < 1e12f);
+    const float ae3 = fabsf(e3), det_lo = ae3 - t3;
+    unc = (fabsf(e1) <= t1) | (fabsf(e2) <= t2) | (ae3 <= t3) | !(sc < 1e12f);
+    if (tol >= 1e-10f) {
+        const float thr = fmaxf(fmaxf(2.0f * tol * (fabsf(e2) + t2), 4.0f * tol * tol * (fabsf(e1) + t1)),
+                                4.0f * tol * tol * tol);
+        marg = tol > 0.f && det_lo < thr * 1.0001f;
+    } else {
+        const float fro2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * (q10 + q20 + q21);
+        marg = tol > 0.f && 2.0f * det_lo < tol * fro2 * 1.0001f;
+    }
     return (e1 > 0.f) + ((e1 < 0.f) != (e2 > 0.f)) + ((e2 > 0.f) != (e3 < 0.f));
+}
+
+// FP64 count for a cell whose float signs are not certain, with the margin
+// pre-filter of positive_eigs_nb evaluated on the FP64 coefficients (bounds
+// with u = 2^-53): cells flagged only because of float rounding drop out.
+__device__ __forceinline__ int positive_eigs_nb64(float b00, float b11, float b22, float a10, float a20, float a21,
+                                                  float tol, bool &marg) {
+    double ad, f2;
+    const int v = positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+    const double d00 = b00, d11 = b11, d22 = b22, d10 = a10, d20 = a20, d21 = a21;
+    const double e1 = d00 + d11 + d22;
+    const double e2 = d00 * d11 + d00 * d22 + d11 * d22 - d10 * d10 - d20 * d20 - d21 * d21;
+    const double sc = (fabs(d00) + fabs(d11) + fabs(d22)) + (fabs(d10) + fabs(d20) + fabs(d21));
+    const double u = 1.1102230246251565e-16;
+    const double t1 = 8.0 * u * sc, t2 = 2.0 * t1 * sc, t3 = 1.5 * t2 * sc;
+    const double det_lo = ad - t3, tl = (double)tol;
+    const double thr = fmax(fmax(2.0 * tl * (fabs(e2) + t2), 4.0 * tl * tl * (fabs(e1) + t1)), 4.0 * tl * tl * tl);
+    marg = tol > 0.f && det_lo < thr * 1.0001;
+    return v;
 }
 
 __device__ __forceinline__ int positive_eigs_s(float b00, float b11, float b22, float a10, float a20, float a21,

@@ -129,3 +129,24 @@ def test_tweb_uncertain_signs_settled_in_fp64(ctx):
         assert np.array_equal(lab[clear][:, 0, 0], expect[clear])
         assert np.all(lab == lab[:, :1, :1])  # constant over y, z
         assert sum(st["counts"]) == N ** 3
+
+
+@pytest.mark.parametrize("frac", [1e-4, 1e-2, 0.1])
+def test_tweb_margin_prefilter_is_conservative(ctx, frac):
+    """The marginal count (cells with an eigenvalue within tol of lambda_th)
+    comes from an exact FP64 test applied to the cells a float root bound lets
+    through; the bound must let every marginal cell through.  Reference: the
+    same count taken from the returned eigenvalues, up to cells within float
+    rounding of the tol boundary."""
+    N, L, lth = 48, 100.0, 0.005
+    phi = torch.from_numpy(_smooth_field(N, 21).astype(np.float32)).cuda()
+    _, lam, _ = ctx.tweb_from_phi(phi, L, lth, want_lambdas=True)
+    tol = frac * float(lam.abs().max())
+    _, lam, st_l = ctx.tweb_from_phi(phi, L, lth, margin_tol=tol, want_lambdas=True)
+    _, _, st = ctx.tweb_from_phi(phi, L, lth, margin_tol=tol, want_lambdas=False)
+    m = (lam.double() - lth).abs().min(dim=0).values
+    exact = int((m < tol).sum())
+    edge = int(((m - tol).abs() < 1e-5 * tol + 1e-6 * float(lam.abs().max())).sum())
+    assert exact > 0
+    assert abs(st["n_marginal"] - exact) <= edge
+    assert st["n_marginal"] == st_l["n_marginal"]
