@@ -76,7 +76,10 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
                                                  const double *__restrict__ tabs, float wc, float wa,
                                                  double *__restrict__ gsum,
                                                  unsigned long long *__restrict__ gcnt, int do_tweb, float inv_kf2,
-                                                 const float *__restrict__ gauss, const double *__restrict__ nobj) {
+                                                 const float *__restrict__ gauss, const double *__restrict__ nobj,
+                                                 const float4 *__restrict__ ztab) {
+    // ztab[iz] = {1/W^2, C1/W^2, A/W^2, Gaussian} of the kz index: uniform
+    // across the workgroup inside the kz walk, so it is read with scalar loads
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int M1 = N / 2 + 1;
     const int ns = pk.nshells;
@@ -86,9 +89,7 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
     float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                      // M1
     float *s_sq = s_ic + M1;                                                  // M1: C1 (aliasing sum)
     float *s_al = s_sq + M1;                                                  // M1: alternating sum
-    float *s_sqic = s_al + M1;                                                // M1
-    float *s_alic = s_sqic + M1;                                              // M1
-    float *s_g = s_alic + M1;                                                 // N
+    float *s_g = s_al + M1;                                                   // N
     constexpr int NT = ColShape<N, NB, E>::NT;
     const int ncols = N / 2 + 1;
     const int tid = threadIdx.x;
@@ -96,8 +97,6 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
         s_ic[i] = (float)tabs[i];
         s_sq[i] = (float)tabs[M1 + i];
         s_al[i] = (float)tabs[2 * M1 + i];
-        s_sqic[i] = (float)(tabs[M1 + i] * tabs[i]);
-        s_alic[i] = (float)(tabs[2 * M1 + i] * tabs[i]);
     }
     for (int i = tid; i < N; i += NT) s_g[i] = gauss[i];
     for (int i = tid; i < 4 * ns; i += NT) s_acc[i] = 0.0;
@@ -194,17 +193,18 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
                 }
                 const int q2 = q2xy + iz * iz;
                 const float rq = q2 > 0 ? __builtin_amdgcn_rcpf((float)q2) : 0.0f;
+                const float4 zt = ztab[iz];
                 if (sh >= 0) {
-                    const float pc = (v.x * v.x + v.y * v.y) * icxy * s_ic[iz];
+                    const float pc = (v.x * v.x + v.y * v.y) * icxy * zt.x;
                     const float mu2 = (float)(iz * iz) * rq;
                     acc.a0 += pc;
-                    acc.a1 += sqxy * s_sqic[iz] + alxy * s_alic[iz];
+                    acc.a1 += sqxy * zt.y + alxy * zt.z;
                     acc.a2 += (1.5f * mu2 - 0.5f) * pc;
                     acc.a3 += ((4.375f * mu2 - 3.75f) * mu2 + 0.375f) * pc;
                     acc.c += 1u;
                 }
                 if (do_tweb) {
-                    const float f = gxy * s_g[iz] * rq;
+                    const float f = gxy * zt.w * rq;
                     v.x *= f;
                     v.y *= f;
                     row[b] = v;
@@ -262,10 +262,10 @@ template <int N, int NB, int E>
 static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pitch,
                              const float2 *tw, const PkK &pk, const double *dtab, double *dsum,
                              unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
-                             const double *dnobj, int ns) {
+                             const double *dnobj, int ns, const float4 *ztab) {
     constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB, E>::NT;
     const size_t lds = sizeof(float2) * (size_t)(NB + 1) * N + sizeof(double) * 4 * (size_t)ns +
-                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (5 * (size_t)M1 + (size_t)N);
+                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (3 * (size_t)M1 + (size_t)N);
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
     auto kern = dkB ? k_x_fused<N, NB, true, E> : k_x_fused<N, NB, false, E>;
@@ -275,14 +275,14 @@ static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pit
     const unsigned grid = (unsigned)std::min(ntiles, std::max(1, ctx->cu_count) * per_cu);
     const float wc = dkB ? 0.5f : 1.0f, wa = dkB ? 0.5f : 0.0f;
     return launch(ctx, "fused_x", kern, dim3(grid), dim3(NT), lds, dk, dkB, pitch, tw, pk, dtab, wc, wa, dsum, dcnt,
-                  do_tweb, inv_kf2, dgauss, dnobj);
+                  do_tweb, inv_kf2, dgauss, dnobj, ztab);
 }
 
 template <int N>
 static cm_status x_fused_launch(cm_ctx *ctx, int nb, int xe, float2 *dk, const float2 *dkB, int pitch,
                                 const float2 *tw, const PkK &pk, const double *dtab, double *dsum,
                                 unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
-                                const double *dnobj, int ns) {
+                                const double *dnobj, int ns, const float4 *ztab) {
     constexpr int E0 = ct_E(N);
     // E = 8 (twice the threads) where the plan allows it: radix 8/4/2 only
     if constexpr (E0 == 16) {
@@ -290,20 +290,20 @@ static cm_status x_fused_launch(cm_ctx *ctx, int nb, int xe, float2 *dk, const f
             if constexpr (16 * N / 8 <= 1024) {
                 if (nb == 16)
                     return x_fused_run<N, 16, 8>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2,
-                                                 dgauss, dnobj, ns);
+                                                 dgauss, dnobj, ns, ztab);
             }
             if constexpr (8 * N / 8 <= 1024)
                 return x_fused_run<N, 8, 8>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss,
-                                            dnobj, ns);
+                                            dnobj, ns, ztab);
         }
     }
     if constexpr (ColShape<N, 8, E0>::NT * 2 <= 512) {
         if (nb == 16)
             return x_fused_run<N, 16, E0>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss,
-                                          dnobj, ns);
+                                          dnobj, ns, ztab);
     }
     return x_fused_run<N, 8, E0>(ctx, dk, dkB, pitch, tw, pk, dtab, dsum, dcnt, do_tweb, inv_kf2, dgauss, dnobj,
-                                 ns);
+                                 ns, ztab);
 }
 
 cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f) {
@@ -420,6 +420,22 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
                                               &s);
     if (!dgauss) return s;
     const float inv_kf2 = (float)(1.0 / (kf * kf));
+    // per-kz factors {1/W^2, C1/W^2, A/W^2, Gaussian} (kz = 0..N/2)
+    const float4 *ztab = cached_table<float4>(
+        ctx, table_key("ztab:%d:%d:%d:%.17g:%.17g", N, (int)o->mas.scheme, o->mas_correction, kf, R2h),
+        [&](std::vector<float4> &z) {
+            std::vector<double> t;
+            pk_tables(N, (int)o->mas.scheme, o->mas_correction, t);
+            const int M1 = N / 2 + 1;
+            z.resize((size_t)M1);
+            for (int i = 0; i < M1; ++i) {
+                const double k = kf * (double)i;
+                z[i] = float4{(float)t[i], (float)(t[M1 + i] * t[i]), (float)(t[2 * M1 + i] * t[i]),
+                              (float)std::exp(-k * k * R2h)};
+            }
+        },
+        &s);
+    if (!ztab) return s;
     const int nb = ctx->tune.xnb == 8 ? 8 : 16;  // kz columns per tile (16 when the workgroup fits)
     switch (N) {
 #define CM_CASE(n)                                                                                                     \
@@ -427,7 +443,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
         s = x_fused_launch<n>(ctx, nb, ctx->tune.xe, (float2 *)dk, (const float2 *)dkB, pitch,                     \
                               (const float2 *)tw, pk, (const double *)dtab,                                            \
                               (double *)dsum, (unsigned long long *)dcnt, do_tweb, inv_kf2, (const float *)dgauss,     \
-                              (const double *)dnobj, ns);                                                              \
+                              (const double *)dnobj, ns, ztab);                                                        \
         break;
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
