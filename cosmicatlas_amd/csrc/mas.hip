@@ -115,7 +115,11 @@ __device__ __attribute__((noinline)) float wrap_slow(float v, double xmin, doubl
     return wrap_coord<float>(v, xmin, L, o) ? o : __int_as_float(0x7fc00000);
 }
 
-__device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int c[3], float s[3]) {
+// Cell and in-cell offset of a particle from its loaded coordinates: fr[a] =
+// the fraction x/d - c in [0, 1) as a 27-bit fixed-point number (floored, so
+// it never reaches the next cell).
+constexpr int kFracBits = 27;
+__device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int c[3], unsigned fr[3]) {
     bool ok = true;
     const float Lf = (float)p.L;
 #pragma unroll
@@ -132,15 +136,29 @@ __device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int 
         const double x = shifted((double)u, p.ishift, p.L);
         double q;
         c[a] = cell_fast(x, p.d, p.rd, p.N, q);
-        s[a] = (float)(q - (double)c[a] - 0.5);
+        const unsigned qf = (unsigned)((q - (double)c[a]) * (double)(1u << kFracBits));
+        fr[a] = qf < (1u << kFracBits) ? qf : (1u << kFracBits) - 1u;
     }
     return ok;
 }
 
-// Sorted record: packed global cell (10 bits per axis) and the three offsets.
-__device__ __forceinline__ float4 make_record(const int c[3], const float f[3]) {
-    const unsigned pk = (unsigned)c[0] | ((unsigned)c[1] << 10) | ((unsigned)c[2] << 20);
-    return make_float4(__uint_as_float(pk), f[0], f[1], f[2]);
+// Sorted record: one 32-bit word per axis.  Bits 27..31 hold the cell's
+// coordinate in the destination tile plus 2 (the tile's own cells are 0..T-1,
+// ghosts from neighbours -R..T+R-1, R <= 2); bits 0..26 the fraction x/d - c.
+// The periodic wrap matches k_tile_deposit's: a coordinate below -R or at
+// least N-R is moved by one period.
+__device__ __forceinline__ unsigned record_word(int c, int o, unsigned fr, int N, int R) {
+    int lc = c - o;
+    if (lc < -R) lc += N;
+    if (lc >= N - R) lc -= N;
+    return ((unsigned)(lc + 2) << kFracBits) | fr;
+}
+
+// In-cell offset f = x/d - c - 1/2 from a record word (midpoint of the 2^-27
+// bin; the float conversion rounds by at most 2^-26).
+__device__ __forceinline__ float record_offset(unsigned w) {
+    const int q = (int)(w & ((1u << kFracBits) - 1u)) - (1 << (kFracBits - 1));
+    return (float)q * (1.0f / (float)(1u << kFracBits)) + (0.5f / (float)(1u << kFracBits));
 }
 
 constexpr int kBatch = 8;
@@ -208,7 +226,7 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
         for (int u = 0; u < kBatch; ++u) {
             if (base + (size_t)u * step >= i1) continue;
             int c[3];
-            float f[3];
+            unsigned f[3];
             if (!cells_from(v[u], p, c, f)) {
                 ++ndrop;
                 continue;
@@ -370,7 +388,7 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
                                                        MasK p, TileK t, int ntiles, size_t chunk,
                                                        const unsigned *__restrict__ pref,
                                                        const unsigned *__restrict__ offsets,
-                                                       unsigned *__restrict__ gcursor, float4 *__restrict__ sorted,
+                                                       unsigned *__restrict__ gcursor, uint3 *__restrict__ sorted,
                                                        float *__restrict__ wsorted, unsigned cap,
                                                        unsigned *__restrict__ overflow) {
     extern __shared__ unsigned cur[];
@@ -400,23 +418,26 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
         for (int u = 0; u < kBatch; ++u) {
             if (base + (size_t)u * step >= i1) continue;
             int c[3];
-            float f[3];
+            unsigned f[3];
             if (!cells_from(v[u], p, c, f)) continue;
-            const float4 rec = make_record(c, f);
             int tx[2], ty[2], tz[2];
             const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
                       nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
-            for (int a = 0; a < nx; ++a)
-                for (int b = 0; b < ny; ++b)
+            for (int a = 0; a < nx; ++a) {
+                const unsigned wx = record_word(c[0], tx[a] * T, f[0], p.N, t.R);
+                for (int b = 0; b < ny; ++b) {
+                    const unsigned wy = record_word(c[1], ty[b] * T, f[1], p.N, t.R);
                     for (int e = 0; e < nz; ++e) {
                         const unsigned slot = atomicAdd(&cu[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
                         if (slot < cap) {
-                            sorted[slot] = rec;
+                            sorted[slot] = make_uint3(wx, wy, record_word(c[2], tz[e] * T, f[2], p.N, t.R));
                             if (w) wsorted[slot] = wv[u];
                         } else {
                             ovf = 1u;
                         }
                     }
+                }
+            }
         }
     }
     if (ovf) atomicOr(overflow, 1u);
@@ -495,7 +516,7 @@ __device__ __forceinline__ void deposit_record(void *lgrid, int base, const floa
 // tile), persistent over items.  The records of the next item are loaded into
 // registers while the current tile is written out.
 template <int SCHEME, bool UNIT, int T>
-__global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__ sorted,
+__global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ sorted,
                                                       const float *__restrict__ wsorted,
                                                       const uint4 *__restrict__ items,
                                                       const unsigned *__restrict__ nitems, int N, int ntx,
@@ -510,7 +531,7 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
     const unsigned total = *nitems;
     unsigned item = blockIdx.x;
     uint4 it = item < total ? items[item] : make_uint4(0u, 0u, 0u, 0u);
-    float4 rec[RPT];
+    uint3 rec[RPT];
     float wr[RPT];
     auto load_item = [&](const uint4 &x) {
 #pragma unroll
@@ -538,17 +559,12 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const float4 *__restrict__
         for (int r = 0; r < RPT; ++r) {
             const unsigned idx = it.y + (unsigned)(tid + r * NT);
             if (idx >= it.z) continue;
-            const unsigned pk = __float_as_uint(rec[r].x);
-            const int cc[3] = {(int)(pk & 1023u), (int)((pk >> 10) & 1023u), (int)(pk >> 20)};
-            const float f[3] = {rec[r].y, rec[r].z, rec[r].w};
+            const unsigned wd[3] = {rec[r].x, rec[r].y, rec[r].z};
+            const float f[3] = {record_offset(wd[0]), record_offset(wd[1]), record_offset(wd[2])};
             int l[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                int lc = cc[a] - o[a];
-                if (lc < -R) lc += N;
-                if (lc >= N - R) lc -= N;
-                l[a] = lc + R;  // node c-R (local lc-R) sits at LDS index lc-R+2R
-            }
+            for (int a = 0; a < 3; ++a)  // node c-R (local lc-R) sits at LDS index lc-R+2R = lc+R
+                l[a] = (int)(wd[a] >> kFracBits) - 2 + R;
             deposit_record<SCHEME, UNIT, S>(lgrid, (l[0] * S + l[1]) * S + l[2], f, wr[r], qs);
         }
         const bool split = it.w != 0u;
@@ -712,7 +728,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
         const double g1 = 1.0 + 2.0 * t.R / T;
         cap = std::min(std::min(max_copies, max_cap), (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
     }
-    if ((s = scratch(ctx, "mas.sorted", sizeof(float4) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.sorted", sizeof(uint3) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
     void *wsorted = nullptr;
     if (w && (s = scratch(ctx, "mas.wsorted", sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3(nblk), dim3(256), 0, ntiles,
@@ -723,7 +739,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
         if ((s = launch(ctx, "mas_tile_scatter",
                         p.soa ? k_tile_scatter<T, true, true> : k_tile_scatter<T, true, false>, dim3(nrows),
                         dim3(1024), hist_lds, pos, w, p, t, ntiles, chunk, (const unsigned *)hp,
-                        (const unsigned *)offsets, (unsigned *)counts, (float4 *)sorted, (float *)wsorted,
+                        (const unsigned *)offsets, (unsigned *)counts, (uint3 *)sorted, (float *)wsorted,
                         (unsigned)cap, overflow)) != CM_OK)
             return s;
     } else {
@@ -731,7 +747,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
         if ((s = launch(ctx, "mas_tile_scatter",
                         p.soa ? k_tile_scatter<T, false, true> : k_tile_scatter<T, false, false>, dim3(g),
                         dim3(1024), 0, pos, w, p, t, ntiles, chunk, (const unsigned *)hp, (const unsigned *)offsets,
-                        (unsigned *)counts, (float4 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
+                        (unsigned *)counts, (uint3 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
             return s;
     }
     if (!accumulate) {
@@ -747,7 +763,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     const float *ws = (const float *)wsorted;
     const dim3 gd(dg), bd(512);
 #define CM_DEPOSIT(SCH, UNIT)                                                                                    \
-    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const float4 *)sorted, ws,     \
+    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const uint3 *)sorted, ws,     \
            (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, mesh, accumulate)
     switch (p.scheme * 2 + (w ? 0 : 1)) {
     case 0: return CM_DEPOSIT(0, false);
