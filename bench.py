@@ -36,8 +36,8 @@ def algorithmic_bytes(name, n_part, n_rec):
     n3 = N ** 3
     nh = N * N * (N // 2 + 1)
     table = {
-        "mas_tile_deposit": 16 * n_rec + 4 * n3,
-        "mas_tile_scatter": 12 * n_part + 16 * n_rec,
+        "mas_tile_deposit": 12 * n_rec + 4 * n3,
+        "mas_tile_scatter": 12 * n_part + 12 * n_rec,
         "mas_tile_count": 12 * n_part,
         "mas_plain": 12 * n_part + 4 * n3,
         "fft_z_r2c": 4 * n3 + 8 * nh,
