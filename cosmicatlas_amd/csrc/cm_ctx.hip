@@ -253,6 +253,7 @@ cm_status cm_ctx_create(int device, size_t hbm_budget, cm_ctx **out) {
                 else if (k == "mas_tile") c->tune.mas_tile = v;
                 else if (k == "graph") c->graphs = v != 0;
                 else if (k == "dep_wg") c->tune.dep_wg = v;
+                else if (k == "mas_hist") c->tune.mas_hist = v;
             }
             pos = end + 1;
         }

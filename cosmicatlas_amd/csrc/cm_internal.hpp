@@ -71,6 +71,7 @@ struct cm_ctx {
         int ty = 0, tz = 0, tseg = 0;  // T-web tile and x segments
         int mas_tile = 0;              // cells per tile edge in the deposit
         int dep_wg = 0;                // deposit workgroups per CU
+        int mas_hist = 0;              // tile counters per binning workgroup (0: its LDS capacity)
     } tune;
 };
 

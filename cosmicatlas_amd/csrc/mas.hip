@@ -84,6 +84,8 @@ struct TileK {
     int R;      // stencil radius
     int ntx;    // tiles per side
     int CH;     // max records per deposit workgroup
+    int tx0;    // first tile along x of the binned range
+    int nxl;    // tiles along x in the range (ntx: the whole mesh)
 };
 
 constexpr int kTileCH = 4096;
@@ -234,9 +236,12 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
             int tx[2], ty[2], tz[2];
             const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
                       nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
-            for (int a = 0; a < nx; ++a)
+            for (int a = 0; a < nx; ++a) {
+                const int lx = tx[a] - t.tx0;  // tiles outside the x range belong to another pass
+                if ((unsigned)lx >= (unsigned)t.nxl) continue;
                 for (int b = 0; b < ny; ++b)
-                    for (int e = 0; e < nz; ++e) atomicAdd(&h[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+                    for (int e = 0; e < nz; ++e) atomicAdd(&h[(lx * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+            }
         }
     }
     if (ndrop) atomicAdd(dropped, (unsigned long long)ndrop);
@@ -424,11 +429,13 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
             const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
                       nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
             for (int a = 0; a < nx; ++a) {
+                const int lx = tx[a] - t.tx0;
+                if ((unsigned)lx >= (unsigned)t.nxl) continue;
                 const unsigned wx = record_word(c[0], tx[a] * T, f[0], p.N, t.R);
                 for (int b = 0; b < ny; ++b) {
                     const unsigned wy = record_word(c[1], ty[b] * T, f[1], p.N, t.R);
                     for (int e = 0; e < nz; ++e) {
-                        const unsigned slot = atomicAdd(&cu[(tx[a] * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+                        const unsigned slot = atomicAdd(&cu[(lx * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
                         if (slot < cap) {
                             sorted[slot] = make_uint3(wx, wy, record_word(c[2], tz[e] * T, f[2], p.N, t.R));
                             if (w) wsorted[slot] = wv[u];
@@ -443,6 +450,118 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
     if (ovf) atomicOr(overflow, 1u);
 }
 
+// ------------------------------------------- x-slab groups (level-1 binning)
+// When the tiles of the mesh do not fit one workgroup's LDS histogram (N >= 768
+// with 16^3 tiles), particles are first split into G groups of gs consecutive
+// tile planes along x (gs N^2/T^2 tiles fit the LDS), with a copy for a group
+// whose first or last tile plane the particle's stencil reaches from the
+// neighbouring group; each group then goes through the LDS binning and deposit
+// restricted to its tile planes, so every record is emitted by exactly one
+// group.  Raw positions are copied (AoS float32), so the second level computes
+// the same cells.
+template <int T>
+__device__ __forceinline__ int particle_groups(int cx, int R, int ntx, int gs, int G, int out[2]) {
+    const int home = cx / T, lc = cx % T, g = home / gs;
+    out[0] = g;
+    if (G > 1 && lc < R && home % gs == 0) {
+        out[1] = g == 0 ? G - 1 : g - 1;
+        return 2;
+    }
+    if (G > 1 && lc >= T - R && (home % gs == gs - 1 || home == ntx - 1)) {
+        out[1] = g + 1 == G ? 0 : g + 1;
+        return 2;
+    }
+    return 1;
+}
+
+template <int T, bool SOA>
+__global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ pos, MasK p, int R, int ntx, int gs,
+                                                      int G, size_t chunk, unsigned *__restrict__ hist,
+                                                      unsigned long long *dropped) {
+    __shared__ unsigned lh[64];
+    if (threadIdx.x < 64) lh[threadIdx.x] = 0u;
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
+    unsigned ndrop = 0;
+    for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * (size_t)blockDim.x) {
+        float v[kBatch][3];
+        load_batch<kBatch, SOA>(pos, p.n, base, blockDim.x, i1, v);
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (base + (size_t)u * blockDim.x >= i1) continue;
+            int c[3];
+            unsigned f[3];
+            if (!cells_from(v[u], p, c, f)) {
+                ++ndrop;
+                continue;
+            }
+            int g[2];
+            const int ng = particle_groups<T>(c[0], R, ntx, gs, G, g);
+            for (int k = 0; k < ng; ++k) atomicAdd(&lh[g[k]], 1u);
+        }
+    }
+    if (ndrop) atomicAdd(dropped, (unsigned long long)ndrop);
+    __syncthreads();
+    if ((int)threadIdx.x < G) hist[(size_t)blockIdx.x * G + threadIdx.x] = lh[threadIdx.x];
+}
+
+// Exclusive prefix over the workgroups per group (in place) and the group
+// starts gstart[0..G] (64-bit), one workgroup.
+__global__ void __launch_bounds__(64) k_group_scan(unsigned *__restrict__ hist, int nrows, int G,
+                                                   unsigned long long *__restrict__ gstart) {
+    __shared__ unsigned long long tot[64];
+    const int g = threadIdx.x;
+    unsigned long long run = 0;
+    if (g < G)
+        for (int r = 0; r < nrows; ++r) {
+            const unsigned v = hist[(size_t)r * G + g];
+            hist[(size_t)r * G + g] = (unsigned)run;  // < 2^32: checked against the total on the host
+            run += v;
+        }
+    tot[g] = g < G ? run : 0ull;
+    __syncthreads();
+    if (g == 0) {
+        unsigned long long a = 0;
+        for (int k = 0; k < G; ++k) {
+            gstart[k] = a;
+            a += tot[k];
+        }
+        gstart[G] = a;
+    }
+}
+
+template <int T, bool SOA>
+__global__ void __launch_bounds__(1024) k_group_scatter(const float *__restrict__ pos, const float *__restrict__ w,
+                                                        MasK p, int R, int ntx, int gs, int G, size_t chunk,
+                                                        const unsigned *__restrict__ hist,
+                                                        const unsigned long long *__restrict__ gstart,
+                                                        float3 *__restrict__ out, float *__restrict__ wout) {
+    __shared__ unsigned long long cur[64];
+    if ((int)threadIdx.x < G) cur[threadIdx.x] = gstart[threadIdx.x] + hist[(size_t)blockIdx.x * G + threadIdx.x];
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
+    for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * (size_t)blockDim.x) {
+        float v[kBatch][3];
+        load_batch<kBatch, SOA>(pos, p.n, base, blockDim.x, i1, v);
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const size_t i = base + (size_t)u * blockDim.x;
+            if (i >= i1) continue;
+            int c[3];
+            unsigned f[3];
+            if (!cells_from(v[u], p, c, f)) continue;
+            int g[2];
+            const int ng = particle_groups<T>(c[0], R, ntx, gs, G, g);
+            const float wi = w ? w[i] : 0.0f;
+            for (int k = 0; k < ng; ++k) {
+                const unsigned long long slot = atomicAdd(&cur[g[k]], 1ull);
+                out[slot] = make_float3(v[u][0], v[u][1], v[u][2]);
+                if (w) wout[slot] = wi;
+            }
+        }
+    }
+}
+
 // Zero the mesh cells of tiles that will be deposited by several workgroups.
 template <int T>
 __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restrict__ offsets, TileK t, int N,
@@ -450,7 +569,7 @@ __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restr
     const int tile = blockIdx.x;
     const unsigned cnt = offsets[tile + 1] - offsets[tile];
     if (cnt <= (unsigned)kTileCH) return;
-    const int tz = tile % t.ntx, ty = (tile / t.ntx) % t.ntx, tx = tile / (t.ntx * t.ntx);
+    const int tz = tile % t.ntx, ty = (tile / t.ntx) % t.ntx, tx = t.tx0 + tile / (t.ntx * t.ntx);
     for (int q = threadIdx.x; q < T * T * T; q += blockDim.x) {
         const int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
         mesh[((size_t)(tx * T + ix) * N + (ty * T + iy)) * N + (tz * T + iz)] = 0.0f;
@@ -519,7 +638,7 @@ template <int SCHEME, bool UNIT, int T>
 __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ sorted,
                                                       const float *__restrict__ wsorted,
                                                       const uint4 *__restrict__ items,
-                                                      const unsigned *__restrict__ nitems, int N, int ntx,
+                                                      const unsigned *__restrict__ nitems, int N, int ntx, int tx0,
                                                       float *__restrict__ mesh, int accumulate) {
     extern __shared__ long long lgrid_raw[];
     void *lgrid = lgrid_raw;
@@ -551,7 +670,7 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
         for (int q = tid; q < S3 * WB / 16; q += NT) lz[q] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
         const int tile = (int)it.x;
-        const int tz = tile % ntx, ty = (tile / ntx) % ntx, tx = tile / (ntx * ntx);
+        const int tz = tile % ntx, ty = (tile / ntx) % ntx, tx = tx0 + tile / (ntx * ntx);
         const int o[3] = {tx * T, ty * T, tz * T};
         const int qk = UNIT ? unit_quantum_exp<SCHEME>(it.z - it.y) : 0;
         const float qs = ldexpf(1.0f, qk);
@@ -664,14 +783,17 @@ static int tile_edge(int N) {
 }
 
 template <int T>
-static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
-                             int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
+static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
+                                 int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow,
+                                 int tx0, int nxl) {
     TileK t;
     t.T = T;
     t.R = mas_radius(p.scheme);
     t.ntx = p.N / T;
     t.CH = kTileCH;
-    const int ntiles = t.ntx * t.ntx * t.ntx;
+    t.tx0 = tx0;
+    t.nxl = nxl;
+    const int ntiles = nxl * t.ntx * t.ntx;
     const size_t max_copies = p.n * (t.R > 0 ? 8 : 1);
     const int nblk = (ntiles + 255) / 256;
     if (nblk > 4096) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many tiles");
@@ -764,7 +886,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     const dim3 gd(dg), bd(512);
 #define CM_DEPOSIT(SCH, UNIT)                                                                                    \
     launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const uint3 *)sorted, ws,     \
-           (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, mesh, accumulate)
+           (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, t.tx0, mesh, accumulate)
     switch (p.scheme * 2 + (w ? 0 : 1)) {
     case 0: return CM_DEPOSIT(0, false);
     case 1: return CM_DEPOSIT(0, true);
@@ -777,6 +899,59 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     }
 #undef CM_DEPOSIT
     return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
+}
+
+// Tiled deposit of the whole mesh: one pass when one workgroup's LDS holds a
+// counter per tile, else x-slab groups (level-1 binning, one host
+// synchronisation for the group sizes, exact record sizing per group).
+template <int T>
+static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
+                             int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
+    const int ntx = p.N / T, R = mas_radius(p.scheme);
+    const size_t plane_tiles = (size_t)ntx * ntx;
+    size_t per_lds = ctx->lds_per_block / sizeof(unsigned);
+    if (ctx->tune.mas_hist > 0) per_lds = std::min(per_lds, (size_t)ctx->tune.mas_hist);
+    if ((size_t)ntx * plane_tiles <= per_lds || plane_tiles > per_lds)
+        return mas_tiled_range<T>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow, 0, ntx);
+    const int gs = (int)std::min<size_t>((size_t)ntx, per_lds / plane_tiles);  // tile planes per group
+    const int G = (ntx + gs - 1) / gs;
+    if (G > 64) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %d x-slab groups", G);
+    const int nrows = std::max(1, ctx->cu_count);
+    const size_t chunk = (p.n + nrows - 1) / nrows;
+    void *hist, *gstart;
+    cm_status s;
+    if ((s = scratch(ctx, "mas.ghist", sizeof(unsigned) * (size_t)nrows * G, &hist)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.gstart", sizeof(unsigned long long) * (size_t)(G + 1), &gstart)) != CM_OK) return s;
+    if ((s = launch(ctx, "mas_group_count", p.soa ? k_group_count<T, true> : k_group_count<T, false>, dim3(nrows),
+                    dim3(1024), 0, pos, p, R, ntx, gs, G, chunk, (unsigned *)hist, dropped)) != CM_OK)
+        return s;
+    if ((s = launch(ctx, "mas_group_scan", k_group_scan, dim3(1), dim3(64), 0, (unsigned *)hist, nrows, G,
+                    (unsigned long long *)gstart)) != CM_OK)
+        return s;
+    std::vector<unsigned long long> gs_h((size_t)G + 1);
+    CM_HIP(ctx, hipMemcpyAsync(gs_h.data(), gstart, sizeof(unsigned long long) * gs_h.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned long long total = gs_h[(size_t)G];
+    if (total > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %llu grouped particles", total);
+    void *gpos, *gw = nullptr;
+    if ((s = scratch(ctx, "mas.gpos", sizeof(float3) * (size_t)(total ? total : 1), &gpos)) != CM_OK) return s;
+    if (w && (s = scratch(ctx, "mas.gw", sizeof(float) * (size_t)(total ? total : 1), &gw)) != CM_OK) return s;
+    if ((s = launch(ctx, "mas_group_scatter", p.soa ? k_group_scatter<T, true> : k_group_scatter<T, false>,
+                    dim3(nrows), dim3(1024), 0, pos, w, p, R, ntx, gs, G, chunk, (const unsigned *)hist,
+                    (const unsigned long long *)gstart, (float3 *)gpos, (float *)gw)) != CM_OK)
+        return s;
+    for (int g = 0; g < G; ++g) {
+        MasK q = p;
+        q.soa = 0;
+        q.n = (size_t)(gs_h[(size_t)g + 1] - gs_h[(size_t)g]);
+        const float *qp = (const float *)gpos + 3 * (size_t)gs_h[(size_t)g];
+        const float *qw = gw ? (const float *)gw + (size_t)gs_h[(size_t)g] : nullptr;
+        if ((s = mas_tiled_range<T>(ctx, q, qp, qw, mesh, accumulate, dropped, true, overflow, g * gs,
+                                    std::min(gs, ntx - g * gs))) != CM_OK)
+            return s;
+    }
+    return CM_OK;
 }
 
 // Capacity of the sorted-record buffer: when `exact`, the total number of
