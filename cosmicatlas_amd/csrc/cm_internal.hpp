@@ -59,6 +59,7 @@ struct cm_ctx {
     bool graphs = true;                          // hipGraph replay of repeated fused measurements
     std::vector<cm::GraphEntry> graph_cache;
     int64_t graph_captures = 0, graph_launches = 0;
+    uint64_t host_syncs = 0;                     // host synchronisations inside an enqueued sequence
     int cu_count = 256;
     size_t lds_per_block = 65536;
     // launch-shape tuning (0 = built-in choice); set from CM_TUNE at creation,

@@ -841,6 +841,7 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
         CM_HIP(ctx, hipMemcpyAsync(&total, (unsigned long long *)bsum + 2 * nblk, sizeof(total),
                                    hipMemcpyDeviceToHost, ctx->stream));
         CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ++ctx->host_syncs;
         if (total > max_cap)
             return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %llu records exceed 32-bit indices", total);
         cap = (size_t)total;
@@ -932,6 +933,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     CM_HIP(ctx, hipMemcpyAsync(gs_h.data(), gstart, sizeof(unsigned long long) * gs_h.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ++ctx->host_syncs;
     const unsigned long long total = gs_h[(size_t)G];
     if (total > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %llu grouped particles", total);
     void *gpos, *gw = nullptr;

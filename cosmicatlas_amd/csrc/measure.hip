@@ -126,10 +126,15 @@ static std::string graph_key(cm_ctx *ctx, const cm_measure_opts *o, const void *
                      ctx->tune.xe, ctx->tune.tz, ctx->tune.tseg, ctx->tune.dep_wg);
 }
 
+// *first is set on the first sighting of a key: the caller's eager run then
+// decides whether the sequence may be captured at all (a sequence that
+// synchronises with the host inside, like the x-slab group binning of large
+// meshes, is never captured).
 static cm_status measure_graph(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
                                void *mesh, void *meshB, int8_t *labels, double *pk0, double *pk2, double *pk4,
-                               int64_t *nmodes, cm_measure_info *info, bool *done) {
+                               int64_t *nmodes, cm_measure_info *info, bool *done, bool *first) {
     *done = false;
+    *first = false;
     const std::string key = graph_key(ctx, o, pos, w, n, labels);
     GraphEntry *e = nullptr;
     for (auto &g : ctx->graph_cache)
@@ -140,6 +145,7 @@ static cm_status measure_graph(cm_ctx *ctx, const cm_measure_opts *o, const void
             ctx->graph_cache.erase(ctx->graph_cache.begin());
         }
         ctx->graph_cache.push_back(GraphEntry{key});
+        *first = true;
         return CM_OK;
     }
     if (e->bad) return CM_OK;
@@ -155,7 +161,8 @@ static cm_status measure_graph(cm_ctx *ctx, const cm_measure_opts *o, const void
         const cm_status s = enqueue_fused(ctx, o, pos, w, n, mesh, meshB, labels, false, &dev);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-        hipGetLastError();
+        for (int k = 0; k < 4 && hipGetLastError() != hipSuccess; ++k) {
+        }
         if (s != CM_OK || ec != hipSuccess || !g || gen != ctx->scratch_gen ||
             hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) {
             if (g) hipGraphDestroy(g);
@@ -200,10 +207,11 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     if (o->fused) {
         void *meshB = nullptr;
         if (o->interlace && (s = scratch(ctx, "m.meshB", sizeof(float) * n3, &meshB)) != CM_OK) return s;
+        bool first = false;
         if (ctx->graphs && !ctx->profiling) {
             bool done = false;
-            if ((s = measure_graph(ctx, o, pos, w, n, mesh, meshB, labels, pk0, pk2, pk4, nmodes, info, &done)) !=
-                CM_OK)
+            if ((s = measure_graph(ctx, o, pos, w, n, mesh, meshB, labels, pk0, pk2, pk4, nmodes, info, &done,
+                                   &first)) != CM_OK)
                 return s;
             if (done) return CM_OK;
         }
@@ -211,7 +219,9 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
         // overflow (flag in the result block, read back with the shells)
         for (int attempt = 0; attempt < 2; ++attempt) {
             FusedOut dev;
+            const uint64_t syncs0 = ctx->host_syncs;
             if ((s = enqueue_fused(ctx, o, pos, w, n, mesh, meshB, labels, attempt > 0, &dev)) != CM_OK) return s;
+            if (first && ctx->host_syncs != syncs0) ctx->graph_cache.back().bad = true;
             bool redo = false;
             if ((s = finish_fused(ctx, o, dev, pk0, pk2, pk4, nmodes, info, &redo)) != CM_OK) return s;
             if (!redo) return CM_OK;

@@ -137,3 +137,32 @@ def test_autotune_shapes_and_results(cm):
     finally:
         ctx.set_tune(N, None)
         ctx.close()
+
+
+def test_sequences_with_host_sync_are_not_captured(cm):
+    """The x-slab group binning of meshes with more tiles than one histogram
+    holds synchronises with the host inside the sequence; such a call must run
+    eagerly every time (no capture attempt) and give the eager results."""
+    import os
+    old = os.environ.get("CM_TUNE")
+    os.environ["CM_TUNE"] = "mas_hist=32"  # N = 64: 64 tiles, two x-slab groups
+    try:
+        cg = cm.Context(0)
+    finally:
+        if old is None:
+            del os.environ["CM_TUNE"]
+        else:
+            os.environ["CM_TUNE"] = old
+    ce = cm.Context(0)
+    ce.set_graphs(False)
+    try:
+        pos = cg.zeldovich(NL, L, A=A, seed=12)
+        lg = torch.empty((N, N, N), dtype=torch.int8, device="cuda")
+        le = torch.empty_like(lg)
+        ref = _measure(ce, pos, le)
+        for _ in range(4):
+            _same(ref, _measure(cg, pos, lg), le, lg)
+        assert cg.graph_counts() == (0, 0)
+    finally:
+        cg.close()
+        ce.close()
