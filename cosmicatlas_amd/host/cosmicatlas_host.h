@@ -100,6 +100,15 @@ class Params {
     ULONG _N_lines_binary() const { return N_lines_binary; }
     int _sys_of_coord_g() const { return sys_of_coord_g; }
     std::string _angles_units_g() const { return angles_units_g; }
+    double _om_matter() const { return om_matter; }
+    double _om_radiation() const { return om_radiation; }
+    double _om_vac() const { return om_vac; }
+    double _om_k() const { return om_k; }
+    double _wde_eos() const { return wde_eos; }
+    double _Hubble() const { return Hubble; }
+    int _Nbins_redshift() const { return Nbins_redshift; }
+    double _redshift_min_sample() const { return redshift_min_sample; }
+    double _redshift_max_sample() const { return redshift_max_sample; }
 
    private:
     std::map<std::string, std::string> raw;
@@ -127,6 +136,18 @@ class Params {
     // (Source/CoordinateSystem.cpp:12-18); angles in degrees ("D") or radians ("R")
     int sys_of_coord_g = 0;
     std::string angles_units_g = "D";
+    // background cosmology for the distance-redshift relation: the .ini keys of
+    // Source/Params.cpp:3168-3219, defaults of Cosmo_parameters_PLANCK
+    // (Headers/cosmo_parameters.h:37-50); H in km/s per Mpc/h
+    double om_matter = 0.3089;
+    double om_radiation = 1.3157e-5 * (1.0 + 0.2271 * 3.046);
+    double om_k = 0.0;
+    double om_vac = 1.0 - 0.3089 - 0.0 - 1.3157e-5 * (1.0 + 0.2271 * 3.046);
+    double wde_eos = -1.0;
+    double Hubble = 100.0;
+    // r(z) table nodes (Params.cpp:348-350)
+    int Nbins_redshift = 10;
+    double redshift_min_sample = 0.0, redshift_max_sample = 1.0;
     real_prec Xoffset = 0, Yoffset = 0, Zoffset = 0;
     double delta_x = 0, deltak_x = 0, deltak_0 = 0, kmin = 0, kmax = 0, DeltaK_data = 0, Deltal = 0;
     int Nnp_data = 0;
@@ -252,5 +273,11 @@ void read_array(const std::string &fname, std::vector<real_prec> &out);
 void write_array(const std::string &fname, const std::vector<real_prec> &in);
 void write_to_file2(const std::string &fname, const std::vector<real_prec> &kve, const std::vector<real_prec> &pk,
                     const std::vector<int> &si, bool cero);
+// Comoving distance in Mpc/h, c * int_0^z dz'/H(z') (Cosmology::comoving_distance,
+// Source/CosmologicalFunctions.cpp:51-53), and the r(z) table on Nbins_redshift
+// equally spaced redshifts from 0 to redshift_max_sample (Catalog::get_cosmo,
+// Source/Catalog.cpp:16-37).
+double comoving_distance(const Params &p, double z);
+void redshift_table(const Params &p, std::vector<double> &zc, std::vector<double> &rc);
 // ASCII catalogue: whitespace-separated columns (Catalog::read_catalog).
 void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w);

@@ -114,6 +114,15 @@ void Params::read_pars(const std::string &file) {
         else if (name == "N_lines_binary") N_lines_binary = std::strtoul(value.c_str(), nullptr, 10);
         else if (name == "sys_of_coord_g") sys_of_coord_g = std::atoi(value.c_str());
         else if (name == "angles_units_g") angles_units_g = value;
+        else if (name == "om_matter") om_matter = std::atof(value.c_str());
+        else if (name == "om_radiation") om_radiation = std::atof(value.c_str());
+        else if (name == "om_vac") om_vac = std::atof(value.c_str());
+        else if (name == "om_k") om_k = std::atof(value.c_str());
+        else if (name == "wde_eos") wde_eos = std::atof(value.c_str());
+        else if (name == "Hubble") Hubble = std::atof(value.c_str());
+        else if (name == "Nbins_redshift") Nbins_redshift = std::atoi(value.c_str());
+        else if (name == "redshift_min_sample") redshift_min_sample = std::atof(value.c_str());
+        else if (name == "redshift_max_sample") redshift_max_sample = std::atof(value.c_str());
         else if (name == "shell_rule") {
             if (value != "exact" && value != "float") throw cmh::Error("shell_rule must be exact or float");
             shell_rule = value;
@@ -158,6 +167,15 @@ void Params::show_params() const {
                 input_type.c_str(), Output_directory.c_str(), file_power.c_str(), Name_survey.c_str(), delta_x,
                 deltak_x, deltak_0, kmin, kmax, DeltaK_data, Nnp_data, shell_rule.c_str(),
                 compute_tweb ? "true" : "false");
+    if (sys_of_coord_g == 1) {
+        std::vector<double> zc, rc;
+        redshift_table(*this, zc, rc);
+        std::printf("zcosmo =");
+        for (double v : zc) std::printf(" %.17g", v);
+        std::printf("\nrcosmo =");
+        for (double v : rc) std::printf(" %.17g", v);
+        std::printf("\n");
+    }
 }
 
 // =============================================================== mass assignment
@@ -464,9 +482,84 @@ static void equatorial_to_cartesian(double ra, double dec, double r, double &x, 
     z = r * std::cos(0.5 * M_PI - dec * fac);
 }
 
+// Hubble_function (Source/CosmologicalFunctions.cpp:47-49)
+static double hubble_function(const Params &p, double z) {
+    const double a = 1.0 + z;
+    return p._Hubble() * std::sqrt(p._om_matter() * a * a * a + p._om_radiation() * a * a * a * a +
+                                   p._om_k() * a * a + p._om_vac() * std::pow(a, 3.0 * (1.0 + p._wde_eos())));
+}
+
+// adaptive Simpson on 1/H with Richardson's correction; 1/H is smooth and
+// positive on [0, z], so a relative tolerance of 1e-13 is met in a few hundred
+// evaluations
+static double simpson_hinv(const Params &p, double a, double b, double fa, double fm, double fb, double whole,
+                           double tol, int depth) {
+    const double m = 0.5 * (a + b);
+    const double lm = 0.5 * (a + m), rm = 0.5 * (m + b);
+    const double flm = 1.0 / hubble_function(p, lm), frm = 1.0 / hubble_function(p, rm);
+    const double left = (m - a) / 6.0 * (fa + 4.0 * flm + fm);
+    const double right = (b - m) / 6.0 * (fm + 4.0 * frm + fb);
+    const double delta = left + right - whole;
+    if (depth <= 0 || std::fabs(delta) <= 15.0 * tol) return left + right + delta / 15.0;
+    return simpson_hinv(p, a, m, fa, flm, fm, left, 0.5 * tol, depth - 1) +
+           simpson_hinv(p, m, b, fm, frm, fb, right, 0.5 * tol, depth - 1);
+}
+
+double comoving_distance(const Params &p, double z) {
+    if (z <= 0) return 0.0;
+    const double c_kms = 299792.458;  // Constants::speed_light (Headers/Constants.h:21)
+    const double fa = 1.0 / hubble_function(p, 0.0), fb = 1.0 / hubble_function(p, z);
+    const double fm = 1.0 / hubble_function(p, 0.5 * z);
+    const double whole = z / 6.0 * (fa + 4.0 * fm + fb);
+    return c_kms * simpson_hinv(p, 0.0, z, fa, fm, fb, whole, 1e-13 * z * fa, 40);
+}
+
+void redshift_table(const Params &p, std::vector<double> &zc, std::vector<double> &rc) {
+    const int nz = p._Nbins_redshift();
+    if (nz < 2) throw cmh::Error("Nbins_redshift must be at least 2 for the distance-redshift table");
+    if (!(p._redshift_max_sample() > 0)) throw cmh::Error("redshift_max_sample must be positive");
+    zc.resize(nz);
+    rc.resize(nz);
+    for (int i = 0; i < nz; ++i) {
+        zc[i] = i * p._redshift_max_sample() / (double)(nz - 1);
+        rc[i] = comoving_distance(p, zc[i]);
+    }
+}
+
+// linear interpolation in the r(z) table (gsl_interp_linear, Source/Catalog.cpp:13662-13669);
+// redshifts below the first node are raised to it (:13802), those above the last
+// one are an error (GSL's interpolation refuses them)
+static double distance_of_redshift(const std::vector<double> &zc, const std::vector<double> &rc, double z) {
+    if (z < zc[0]) z = zc[0];
+    if (z > zc.back())
+        throw cmh::Error("redshift " + std::to_string(z) + " above the r(z) table (redshift_max_sample = " +
+                         std::to_string(zc.back()) + ")");
+    size_t i = (size_t)((z - zc[0]) / (zc[1] - zc[0]));
+    if (i >= zc.size() - 1) i = zc.size() - 2;
+    while (i > 0 && z < zc[i]) --i;
+    while (i + 2 < zc.size() && z >= zc[i + 1]) ++i;
+    const double t = (z - zc[i]) / (zc[i + 1] - zc[i]);
+    return rc[i] + t * (rc[i + 1] - rc[i]);
+}
+
 static void to_cartesian(const Params &p, std::vector<real_prec> &xyz) {
     switch (p._sys_of_coord_g()) {
     case 0: return;
+    case 1: {
+        // Catalog::ang_to_cart_coordinates, I_EQZ branch (Source/Catalog.cpp:13786-13820)
+        std::vector<double> zc, rc;
+        redshift_table(p, zc, rc);
+        const double fac = p._angles_units_g() == "R" ? 180.0 / M_PI : 1.0;
+        for (size_t i = 0; i + 2 < xyz.size(); i += 3) {
+            double x, y, z;
+            const double r = distance_of_redshift(zc, rc, xyz[i + 2]);
+            equatorial_to_cartesian(xyz[i] * fac, xyz[i + 1] * fac, r, x, y, z);
+            xyz[i] = (real_prec)x;
+            xyz[i + 1] = (real_prec)y;
+            xyz[i + 2] = (real_prec)z;
+        }
+        return;
+    }
     case 2: {
         const double fac = p._angles_units_g() == "R" ? 180.0 / M_PI : 1.0;  // radians -> degrees
         for (size_t i = 0; i + 2 < xyz.size(); i += 3) {
@@ -478,7 +571,6 @@ static void to_cartesian(const Params &p, std::vector<real_prec> &xyz) {
         }
         return;
     }
-    case 1: throw cmh::Error("sys_of_coord_g = 1 (RA, Dec, redshift) needs the distance-redshift relation, not built");
     }
     throw cmh::Error("unknown sys_of_coord_g = " + std::to_string(p._sys_of_coord_g()));
 }

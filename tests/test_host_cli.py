@@ -169,6 +169,84 @@ def test_cli_equatorial_catalogue(tmp_path, orc):
     for out in (deg, rad):
         assert np.array_equal(out[:, 2].astype(int), nm)
         assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
-    bad = _ini(tmp_path, Nft=str(N), sys_of_coord_g="1")
-    rr = subprocess.run([CLI, "-p", bad], capture_output=True, text=True, timeout=120)
-    assert rr.returncode == 1 and "redshift" in rr.stderr + rr.stdout
+
+
+C_KMS = 299792.458
+
+
+def _rz_table(zmax, nz, om=0.3089, orad=1.3157e-5 * (1 + 0.2271 * 3.046), ok=0.0, ov=None, w=-1.0):
+    """r(z) nodes by scipy's adaptive quadrature of c/H (independent of the host's Simpson rule)."""
+    from scipy.integrate import quad
+    ov = 1 - om - ok - orad if ov is None else ov
+    hinv = lambda z: 1.0 / (100.0 * math.sqrt(om * (1 + z) ** 3 + orad * (1 + z) ** 4 + ok * (1 + z) ** 2
+                                               + ov * (1 + z) ** (3 * (1 + w))))
+    zc = np.arange(nz) * zmax / (nz - 1)  # z_i = i zmax/(nz-1), Source/Catalog.cpp:31
+    rc = np.array([C_KMS * quad(hinv, 0.0, z, epsabs=0, epsrel=1e-13, limit=200)[0] for z in zc])
+    return zc, rc
+
+
+def _table(tmp_path, **kv):
+    d = _show(_ini(tmp_path, sys_of_coord_g="1", **kv))
+    return np.array(d["zcosmo"].split(), float), np.array(d["rcosmo"].split(), float)
+
+
+def test_redshift_table_closed_forms(tmp_path):
+    """Comoving distance against the two closed forms: Einstein-de Sitter,
+    r = 2c/H0 (1 - (1+z)^-1/2), and de Sitter, r = c z/H0."""
+    z, r = _table(tmp_path, om_matter="1", om_radiation="0", om_vac="0", om_k="0", Nbins_redshift="9",
+                  redshift_max_sample="4")
+    assert np.array_equal(z, np.arange(9) * 4.0 / 8)
+    np.testing.assert_allclose(r, 2 * C_KMS / 100 * (1 - 1 / np.sqrt(1 + z)), rtol=1e-12, atol=0)
+    z, r = _table(tmp_path, om_matter="0", om_radiation="0", om_vac="1", om_k="0", Nbins_redshift="5",
+                  redshift_max_sample="2")
+    np.testing.assert_allclose(r, C_KMS * z / 100, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("kv", [{}, dict(om_matter="0.25", om_vac="0.68", om_k="0.07", wde_eos="-0.8",
+                                         om_radiation="9e-5", Nbins_redshift="13", redshift_max_sample="3")])
+def test_redshift_table_quadrature(tmp_path, kv):
+    """The r(z) table (Catalog::get_cosmo, Source/Catalog.cpp:16-37) against
+    scipy's quadrature, for the default Planck background and a curved w != -1 one;
+    the default r(1) is the textbook 2300.5 Mpc/h for Omega_m = 0.3089."""
+    z, r = _table(tmp_path, **kv)
+    args = {} if not kv else dict(om=0.25, orad=9e-5, ok=0.07, ov=0.68, w=-0.8)
+    zc, rc = _rz_table(float(kv.get("redshift_max_sample", 1)), int(kv.get("Nbins_redshift", 10)), **args)
+    assert np.array_equal(z, zc)
+    np.testing.assert_allclose(r, rc, rtol=1e-11, atol=0)
+    if not kv:
+        assert abs(r[-1] - 2300.5) < 0.1
+
+
+def test_redshift_above_table_is_an_error(tmp_path):
+    """Redshifts beyond the last table node stop the run (GSL's interpolation
+    refuses them); the catalogue is converted before any device work."""
+    (tmp_path / "cat.txt").write_text("10 20 0.5\n30 -10 1.5\n")
+    ini = _ini(tmp_path, Nft="32", sys_of_coord_g="1")
+    rr = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=120)
+    assert rr.returncode == 1 and "above the r(z) table" in rr.stderr + rr.stdout
+
+
+@pytest.mark.gpu
+def test_cli_redshift_catalogue(tmp_path, orc):
+    """(RA, Dec, redshift) columns (Catalog::ang_to_cart_coordinates, I_EQZ
+    branch, Source/Catalog.cpp:13786-13820): redshifts chosen so that linear
+    interpolation in the r(z) table returns the distances of known Cartesian
+    positions give those positions' measurement; redshifts below the first node
+    are raised to it (distance 0)."""
+    N, L, n = 64, 500.0, 60000
+    rng = np.random.default_rng(24)
+    xyz = rng.uniform(20.0, 480.0, (n, 3))
+    r = np.sqrt((xyz ** 2).sum(1))
+    zc, rc = _rz_table(1.0, 10)
+    i = np.searchsorted(rc, r, side="right") - 1
+    zz = zc[i] + (r - rc[i]) / (rc[i + 1] - rc[i]) * (zc[i + 1] - zc[i])
+    dec = np.degrees(np.arcsin(xyz[:, 2] / r))
+    ra = np.degrees(np.arctan2(xyz[:, 1], xyz[:, 0]))
+    rows = np.stack([ra, dec, zz], 1)
+    rows = np.vstack([rows, [[10.0, 20.0, -0.01]]])  # clamped to z = 0: the origin
+    np.savetxt(tmp_path / "cat.txt", rows, fmt="%.12f")
+    out = _run_pk(tmp_path, _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", sys_of_coord_g="1"), N)
+    ref = np.vstack([xyz, [[0.0, 0.0, 0.0]]]).astype(np.float32)
+    p0, nm, sn = _oracle_pk(orc, ref, N, L)
+    assert np.array_equal(out[:, 2].astype(int), nm)
+    assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
