@@ -293,6 +293,21 @@ class Context:
         n = nb.value
         return {"ijk": ijk[:n], "bk": bk[:n], "sn": sn[:n], "ntri": nt[:n]}
 
+    def bispectrum_direct(self, dk, N, L, nbins, kmin, dk_bin, scheme, n_obj, mas_correction=True,
+                          sn_correction=True):
+        """Brute-force bispectrum over ordered pairs of the (N+1)^3 mode grid
+        (FftwFunctions::get_bispectrum_fkp); returns dict of (nbins,)*3 arrays bk,
+        sn, ntri, bsum, snsum (filled at a <= b <= c) and (nbins,) arrays pk, nk."""
+        o = _abi.BkdOpts(N, float(L), nbins, float(kmin), float(dk_bin), _scheme(scheme), int(mas_correction),
+                         int(sn_correction), float(n_obj))
+        sh = (nbins, nbins, nbins)
+        bk, sn, bs, ss = np.zeros(sh), np.zeros(sh), np.zeros(sh), np.zeros(sh)
+        nt = np.zeros(sh, np.int64)
+        pk, nk = np.zeros(nbins), np.zeros(nbins, np.int64)
+        a = [x.ctypes.data_as(C.c_void_p) for x in (bk, sn, nt, pk, nk, bs, ss)]
+        self._check(self.lib.cm_bispectrum_direct(self.h, C.byref(o), _ptr(dk), *a))
+        return {"bk": bk, "sn": sn, "ntri": nt, "pk": pk, "nk": nk, "bsum": bs, "snsum": ss}
+
     def tweb_tclass(self, labels, cwt_used, out=None):
         """Cwclass::get_Tclassification for every cell: int32 tensor like labels."""
         torch = _torch()

@@ -114,6 +114,20 @@ class FkpOpts(C.Structure):
     ]
 
 
+class BkdOpts(C.Structure):
+    _fields_ = [
+        ("N", C.c_int),
+        ("L", C.c_double),
+        ("nbins", C.c_int),
+        ("kmin", C.c_double),
+        ("dk", C.c_double),
+        ("scheme", C.c_int),
+        ("mas_correction", C.c_int),
+        ("sn_correction", C.c_int),
+        ("n_obj", C.c_double),
+    ]
+
+
 class BkOpts(C.Structure):
     _fields_ = [
         ("N", C.c_int),
@@ -188,6 +202,7 @@ def _declare(L):
         "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),
         "cm_tweb_tclass": (st, [P, P, C.c_size_t, C.POINTER(C.c_int), C.c_int, P]),
         "cm_bispectrum": (st, [P, C.POINTER(BkOpts), P, C.c_int, P, P, P, P, C.POINTER(C.c_int)]),
+        "cm_bispectrum_direct": (st, [P, C.POINTER(BkdOpts), P, P, P, P, P, P, P, P]),
         "cm_measure_fkp": (st, [P, C.POINTER(MeasureOpts), C.POINTER(FkpOpts), P, P, C.c_size_t, P, P, C.c_size_t,
                                 P, P, P, P, P, C.POINTER(MeasureInfo)]),
         "cm_request_footprint": (C.c_size_t, [C.POINTER(Request)]),

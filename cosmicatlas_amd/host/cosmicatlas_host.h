@@ -72,6 +72,9 @@ class Params {
     std::string _input_type() const { return input_type; }
     std::string _Output_directory() const { return Output_directory; }
     std::string _file_power() const { return file_power; }
+    std::string _file_bispectrum() const { return file_bispectrum; }
+    double _kmax_bk() const { return kmax_bk; }
+    double _kmin_bk() const { return kmin_bk; }
     std::string _Name_survey() const { return Name_survey; }
     std::string _delta_grid_file() const { return delta_grid_file; }
     std::string _file_catalogue() const { return Input_dir_cat + file_catalogue; }
@@ -124,6 +127,9 @@ class Params {
     real_prec lambdath = 0, smscale = 0;
     std::string statistics = "Pk_fkp", input_type = "catalog", Output_directory = "../output";
     std::string file_power = "power", Name_survey = "survey", delta_grid_file = "delta_file";
+    // bispectrum output stem and shell range (Params.cpp:361-366)
+    std::string file_bispectrum = "bispectrum";
+    double kmax_bk = 0.2, kmin_bk = 0.1;
     std::string file_catalogue = "cat.dat", Input_dir_cat = "";
     bool use_random_catalog = false, measure_cross = false;
     int i_coord1_g = 0, i_coord2_g = 1, i_coord3_g = 2, i_weight1_g = -1;
@@ -210,6 +216,17 @@ class FftwFunctions {
     // grid-input monopole (r2c inside); the reference's D2 defect at
     // FftwFunctions.cpp:1853 is not reproduced
     void power_spectrum(std::vector<real_prec> &powerk_g0, std::vector<int> &mod_g);
+    // direct bispectrum (FftwFunctions.cpp:3105-3350; r2c inside): Nft^3 arrays at
+    // index_3d(a, b, c) = (a Nft + b) Nft + c, a <= b <= c over the Nnp_data bins:
+    // bispect = B minus the Poisson shot noise (SN_correction), sn_bispect = the raw
+    // sum of |d1|^2+|d2|^2+|d3|^2, mod = ordered pairs (k1, k2)
+    void get_bispectrum_fkp(char d_w, std::vector<real_prec> &bispect, std::vector<real_prec> &sn_bispect,
+                            std::vector<int> &mod);
+    // shell-transform bispectrum (FftwFunctions.cpp:3358-3718, 4878): shells of
+    // width DeltaK_data from kmin up to kmax_bk, triangle loop from kmin_bk; writes
+    // "i j k B SN ntri" rows to file; pk receives the shell power of the first shells
+    void get_bispectrum_fkp_fast(std::vector<real_prec> &pk, std::vector<real_prec> &bispect, std::vector<int> &mod,
+                                 const std::string &file);
     real_prec _normal_power() const { return normal_power; }
     real_prec _shot_noise() const { return shot_noise; }
 
@@ -256,9 +273,12 @@ class PowerSpectrumF {
     void compute_power_spectrum(bool verbose, bool mcut);
     void compute_power_spectrum_grid();
     void write_power_and_modes();
+    void write_bispectrum();
     std::vector<real_prec> kvector_data, pk0, pk2, pk4;
     std::vector<int> modes_g;
-    std::string file_power, file_cwc;
+    std::string file_power, file_cwc, file_bispectrum;
+    std::vector<real_prec> kvector_data_b, bispectrum, sn_bispectrum;
+    std::vector<int> modes_tri;
     FftwFunctions fftw_functions;
     Cwclass cwclass;
 

@@ -99,6 +99,9 @@ void Params::read_pars(const std::string &file) {
         else if (name == "Output_directory") Output_directory = value;
         else if (name == "file_power") file_power = value;
         else if (name == "Name_survey") Name_survey = value;
+        else if (name == "file_bispectrum") file_bispectrum = value;
+        else if (name == "kmax_bk") kmax_bk = std::atof(value.c_str());
+        else if (name == "kmin_bk") kmin_bk = std::atof(value.c_str());
         else if (name == "delta_grid_file") delta_grid_file = value;
         else if (name == "Catalogue_file") file_catalogue = value;
         else if (name == "Input_dir_cat") Input_dir_cat = value;
@@ -448,6 +451,94 @@ void FftwFunctions::power_spectrum(std::vector<real_prec> &powerk_g0, std::vecto
     shells(powerk_g0, nullptr, nullptr, mod_g);
 }
 
+void FftwFunctions::get_bispectrum_fkp(char, std::vector<real_prec> &bispect, std::vector<real_prec> &sn_bispect,
+                                       std::vector<int> &mod) {
+    const ULONG N = params._Nft();
+    const size_t n3 = (size_t)N * N * N, nh = params._NGRID_h();
+    if (params._type_of_binning() != "linear") throw cmh::Error("log binning is not part of this build");
+    const int nb = params._d_Nnp_data();
+    if (nb > 64) throw cmh::Error("Bk_fkp: at most 64 k bins (Nft/(2 ndel_data) = " + std::to_string(nb) + ")");
+    DevBuf din(sizeof(real_prec) * n3), dk(sizeof(complex_prec) * nh);
+    cmh::h2d(din.p, data_g.data(), sizeof(real_prec) * n3);
+    check(cm_fft_r2c(ctx(), (int)N, CM_F32, din.p, dk.p, 0.0), "cm_fft_r2c");
+    cm_bkd_opts o;
+    o.N = (int)N;
+    o.L = params._Lbox();
+    o.nbins = nb;
+    o.kmin = params._d_kmin();
+    o.dk = params._d_DeltaK_data();
+    o.scheme = (cm_mas)params._mass_assignment();
+    o.mas_correction = params._MAS_correction();
+    o.sn_correction = params._SN_correction();
+    o.n_obj = n_gal;
+    const size_t nb3 = (size_t)nb * nb * nb;
+    std::vector<double> bk(nb3), sn(nb3), snsum(nb3), pk(nb);
+    std::vector<int64_t> nt(nb3), nk(nb);
+    check(cm_bispectrum_direct(ctx(), &o, dk.p, bk.data(), sn.data(), nt.data(), pk.data(), nk.data(), nullptr,
+                               snsum.data()),
+          "cm_bispectrum_direct");
+    bispect.assign(n3, 0);
+    sn_bispect.assign(n3, 0);
+    mod.assign(n3, 0);
+    for (int a = 0; a < nb; ++a)
+        for (int b = a; b < nb; ++b)
+            for (int c = b; c < nb; ++c) {
+                const size_t e = ((size_t)a * nb + b) * nb + c, I = ((size_t)a * N + b) * N + c;
+                bispect[I] = (real_prec)bk[e];
+                sn_bispect[I] = (real_prec)snsum[e];
+                mod[I] = (int)nt[e];
+            }
+}
+
+void FftwFunctions::get_bispectrum_fkp_fast(std::vector<real_prec> &pk, std::vector<real_prec> &bispect,
+                                            std::vector<int> &mod, const std::string &file) {
+    const ULONG N = params._Nft();
+    const size_t n3 = (size_t)N * N * N, nh = params._NGRID_h();
+    if (params._type_of_binning() != "linear") throw cmh::Error("log binning is not part of this build");
+    // FftwFunctions::set_params (:5064-5095): DeltaK_Bis = DeltaK_data, Nshells_bk =
+    // int(kmax_bk / DeltaK_Bis), kmin_bk raised to the fundamental mode; the
+    // triangle loop starts at floor((kmin_bk - kmin)/DeltaK_Bis) (:3524)
+    const double dkb = params._d_DeltaK_data(), kf = 2.0 * M_PI / params._Lbox();
+    const int ns = (int)(params._kmax_bk() / dkb);
+    const double kmin_bk = std::max(params._kmin_bk(), kf);
+    const int smin = std::max(0, (int)std::floor((kmin_bk - params._d_kmin()) / dkb));
+    if (ns < 1 || smin >= ns) throw cmh::Error("Bk_fkp_fast: no shells between kmin_bk and kmax_bk");
+    DevBuf din(sizeof(real_prec) * n3), dk(sizeof(complex_prec) * nh);
+    cmh::h2d(din.p, data_g.data(), sizeof(real_prec) * n3);
+    check(cm_fft_r2c(ctx(), (int)N, CM_F32, din.p, dk.p, 0.0), "cm_fft_r2c");
+    cm_bk_opts o;
+    o.N = (int)N;
+    o.L = params._Lbox();
+    o.nshells = ns;
+    o.kmin = params._d_kmin();
+    o.dk = dkb;
+    o.shell_min = smin;
+    o.scheme = (cm_mas)params._mass_assignment();
+    o.mas_correction = params._MAS_correction();
+    o.sn_correction = params._SN_correction();
+    o.n_obj = n_gal;
+    const int cap = 8 * 256;
+    std::vector<int32_t> ijk(3 * (size_t)cap);
+    std::vector<double> bk(cap), sn(cap), nt(cap);
+    int nb = 0;
+    check(cm_bispectrum(ctx(), &o, dk.p, cap, ijk.data(), bk.data(), sn.data(), nt.data(), &nb), "cm_bispectrum");
+    std::ofstream out(file.c_str());
+    if (!out) throw cmh::Error("cannot write " + file);
+    out.precision(10);
+    out.setf(std::ios::showpoint);
+    out.setf(std::ios::scientific);
+    bispect.assign((size_t)ns * ns * ns, 0);
+    mod.assign((size_t)ns * ns * ns, 0);
+    for (int b = 0; b < nb; ++b) {
+        const int i = ijk[3 * b], j = ijk[3 * b + 1], k = ijk[3 * b + 2];
+        const size_t I = ((size_t)i * ns + j) * ns + k;
+        bispect[I] = (real_prec)bk[b];
+        mod[I] = (int)nt[b];
+        out << i << "\t" << j << "\t" << k << "\t" << bk[b] << "\t" << sn[b] << "\t" << (long long)nt[b] << "\n";
+    }
+    pk.assign(params._d_Nnp_data(), 0);
+}
+
 // ================================================================== file I/O
 void read_array(const std::string &fname, std::vector<real_prec> &out) {
     std::ifstream f(fname.c_str(), std::ios::binary);
@@ -644,6 +735,9 @@ void PowerSpectrumF::set_output_filenames() {
                              "_Nft" + std::to_string(params._Nft()) + "_" + params._mass_assignment_scheme() + "_" +
                              params._file_power();
     file_power = stem + ".txt";
+    file_bispectrum = params._Output_directory() + params._statistics() + "_" + params._Name_survey() + "_Nft" +
+                      std::to_string(params._Nft()) + "_" + params._mass_assignment_scheme() + "_" +
+                      params._file_bispectrum() + ".txt";
     file_cwc = params._Output_directory() + "CWC_" + params._Name_survey() + "_Nft" + std::to_string(params._Nft()) +
                "_" + params._mass_assignment_scheme() + "_" + params._file_power();
 }
@@ -673,8 +767,21 @@ void PowerSpectrumF::compute_power_spectrum(bool verbose, bool) {
     std::vector<real_prec> mesh = fftw_functions.data_g;  // kept for the T-web
     fftw_functions.get_fluctuation();
     kvector();
-    fftw_functions.get_power_spectrum_fkp(pk0, pk2, pk4, modes_g);
-    write_power_and_modes();
+    if (params._statistics() == "Bk_fkp") {
+        // PowerSpectrumF.cpp:1328-1343; the rows are the bin triples a <= b <= c
+        // with triangles, "a b c k_a k_b k_c B SN_sum ntri"
+        kvector_data_b.clear();
+        for (int i = 0; i < params._d_Nnp_data(); ++i)
+            kvector_data_b.push_back((real_prec)(params._d_kmin() + params._d_DeltaK_data() * (i + 0.5)));
+        fftw_functions.get_bispectrum_fkp('d', bispectrum, sn_bispectrum, modes_tri);
+        write_bispectrum();
+    } else if (params._statistics() == "Bk_fkp_fast") {
+        // PowerSpectrumF.cpp:1345-1356
+        fftw_functions.get_bispectrum_fkp_fast(pk0, bispectrum, modes_tri, file_bispectrum);
+    } else {
+        fftw_functions.get_power_spectrum_fkp(pk0, pk2, pk4, modes_g);
+        write_power_and_modes();
+    }
     if (params._compute_tweb()) {
         const real_prec mean = (real_prec)(ngal / (double)params._NGRID());
         for (auto &v : mesh) v = v / mean - 1;
@@ -712,3 +819,21 @@ void PowerSpectrumF::compute_power_spectrum_grid() {
 }
 
 void PowerSpectrumF::write_power_and_modes() { write_to_file2(file_power, kvector_data, pk0, modes_g, true); }
+
+void PowerSpectrumF::write_bispectrum() {
+    std::ofstream out(file_bispectrum.c_str());
+    if (!out) throw cmh::Error("cannot write " + file_bispectrum);
+    out.precision(10);
+    out.setf(std::ios::scientific);
+    const int nb = (int)kvector_data_b.size();
+    const size_t N = params._Nft();
+    for (int a = 0; a < nb; ++a)
+        for (int b = a; b < nb; ++b)
+            for (int c = b; c < nb; ++c) {
+                const size_t I = ((size_t)a * N + b) * N + c;
+                if (modes_tri[I] == 0) continue;
+                out << a << "\t" << b << "\t" << c << "\t" << kvector_data_b[a] << "\t" << kvector_data_b[b] << "\t"
+                    << kvector_data_b[c] << "\t" << bispectrum[I] << "\t" << sn_bispectrum[I] << "\t" << modes_tri[I]
+                    << "\n";
+            }
+}

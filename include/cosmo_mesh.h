@@ -315,6 +315,32 @@ typedef struct {
 cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void *dk, int max_bins, int32_t *ijk,
                         double *bk, double *sn, double *ntri, int *nbins);
 
+typedef struct {
+    int N;            /* mesh side of the half spectrum (even, at most 1024) */
+    double L;
+    int nbins;        /* k bins s = 0 .. nbins-1, at most 64 */
+    double kmin, dk;  /* bin s: kmin + s dk <= |k| < kmin + (s+1) dk (d_kmin, DeltaK_data) */
+    cm_mas scheme;
+    int mas_correction, sn_correction;
+    double n_obj;     /* objects in the box: normal_b = n^3/V^2, normal_p = n^2/V */
+} cm_bkd_opts;
+
+/* Direct (brute-force) bispectrum of a box (FftwFunctions::get_bispectrum_fkp
+ * and bispectrum_fkp, Source/FftwFunctions.cpp:3105-3350) from the device
+ * half spectrum dk (complex64, FFTW layout) of the fluctuation: every ordered
+ * pair (k1, k2) of the grid q in [-N/2, N/2]^3 (both Nyquist planes, like the
+ * reference's (N+1)^3 loops) with k3 = -k1-k2 in the same grid and no vector
+ * zero, binned by the sorted bin triple (a <= b <= c).  Host outputs of
+ * nbins^3 entries at (a nbins + b) nbins + c (zero where a <= b <= c fails):
+ * ntri (ordered pairs), bk = sum Re(d1 d2 d3)/(ntri normal_b) - sn with the
+ * Poisson shot noise sn = <P1+P2+P3>/nbar + 1/nbar^2 (P_i = |d_i|^2/normal_p -
+ * V/n) when sn_correction; optional raw sums bsum (sum Re(d1 d2 d3)) and snsum
+ * (sum |d1|^2+|d2|^2+|d3|^2), which may be NULL.  Per bin (nbins entries): the
+ * power pk = <|d|^2>/normal_p - V/n (sn_correction) over k1 and its mode count
+ * nk (:3194-3199, 3336).  d = delta/W with the MAS window W when mas_correction. */
+cm_status cm_bispectrum_direct(cm_ctx *ctx, const cm_bkd_opts *o, const void *dk, double *bk, double *sn,
+                               int64_t *ntri, double *pk, int64_t *nk, double *bsum, double *snsum);
+
 /* ----------------------------------------------------------- synthetic mocks */
 typedef struct {
     int NL;          /* Lagrangian grid per side (an FFT size) */

@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
         L.or_eigen3.argtypes = [P, P]
         L.or_tweb.argtypes = [C.c_int, C.c_double, P, C.c_double, C.c_double, P, P, P, P]
         L.or_tweb_from_phi.argtypes = [C.c_int, C.c_double, P, C.c_double, P, P, P]
+        L.or_bk_direct.argtypes = [C.c_int, P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, P, P, P, P, P]
         _LIB = L
     return _LIB
 
@@ -180,3 +181,20 @@ def tweb_from_phi(phi, L, lambda_th):
     counts = np.zeros(4, np.int64)
     lib().or_tweb_from_phi(N, L, _p(phi), lambda_th, _p(labels), _p(lam), _p(counts))
     return labels, lam, counts
+
+
+def bk_direct(dk, Dmin, ndel, nb, scheme=0, mas_correction=False):
+    """Brute-force bispectrum sums over ordered pairs of the (N+1)^3 grid
+    (FftwFunctions::bispectrum_fkp, FftwFunctions.cpp:3105-3297).  dk: complex
+    half spectrum N x N x (N/2+1).  Returns (bsum, snsum, cnt) of shape
+    (nb, nb, nb) filled at a <= b <= c, and (psum, pcnt) of shape (nb,)."""
+    dk = np.ascontiguousarray(dk, np.complex128)
+    N = dk.shape[0]
+    bsum = np.zeros((nb, nb, nb))
+    snsum = np.zeros((nb, nb, nb))
+    cnt = np.zeros((nb, nb, nb), np.int64)
+    psum = np.zeros(nb)
+    pcnt = np.zeros(nb, np.int64)
+    lib().or_bk_direct(N, _p(dk), float(Dmin), float(ndel), nb, int(scheme), int(mas_correction), _p(bsum),
+                       _p(snsum), _p(cnt), _p(psum), _p(pcnt))
+    return bsum, snsum, cnt, psum, pcnt

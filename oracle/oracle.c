@@ -526,3 +526,93 @@ void or_tweb(int N, double L, const double *delta, double R, double lambda_th,
     if (!phi) free(ph);
     free(dk);
 }
+
+/* bin of |q|^2 = q2 (units of k_f), -1 outside [Dmin, Dmin + nb ndel) */
+static int bkd_bin(long q2, double Dmin, double ndel, int nb, int int_rule, long Di, long ni)
+{
+    int s;
+    if (int_rule) {
+        if (q2 < Di * Di) return -1;
+        s = (int)floor((sqrt((double)q2) - (double)Di) / (double)ni);
+        while (s > 0 && q2 < (Di + s * ni) * (Di + s * ni)) --s;
+        while (q2 >= (Di + (s + 1) * ni) * (Di + (s + 1) * ni)) ++s;
+    } else {
+        double t = (sqrt((double)q2) - Dmin) / ndel;
+        if (t < 0) return -1;
+        s = (int)floor(t);
+    }
+    return s < nb ? s : -1;
+}
+
+/* the half-spectrum value of q (remap, Source/Miscelanious.cpp:43-62): stored
+ * at (q mod N) when q_z >= 0, else the conjugate of -q's */
+static void bkd_value(int N, const double *dk, int qx, int qy, int qz, double *re, double *im)
+{
+    const int M1 = N / 2 + 1;
+    double sg = 1.0;
+    if (qz < 0) { qx = -qx; qy = -qy; qz = -qz; sg = -1.0; }
+    const int ix = ((qx % N) + N) % N, iy = ((qy % N) + N) % N;
+    const size_t g = ((size_t)ix * N + iy) * M1 + qz;
+    *re = dk[2 * g];
+    *im = sg * dk[2 * g + 1];
+}
+
+void or_bk_direct(int N, const double *dk, double Dmin, double ndel, int nb, int scheme, int mas_corr,
+                  double *bsum, double *snsum, int64_t *cnt, double *psum, int64_t *pcnt)
+{
+    const int h = N / 2, M = N + 1;
+    long Di = 0, ni = 0;
+    const int int_rule = near_int(Dmin, &Di) && near_int(ndel, &ni) && ni > 0;
+    const size_t G = (size_t)M * M * M;
+    double *vr = (double *)malloc(sizeof(double) * G), *vi = (double *)malloc(sizeof(double) * G);
+    int *bin = (int *)malloc(sizeof(int) * G);
+    for (int a = -h; a <= h; ++a)
+        for (int b = -h; b <= h; ++b)
+            for (int c = -h; c <= h; ++c) {
+                const size_t t = ((size_t)(a + h) * M + (b + h)) * M + (c + h);
+                double re, im;
+                bkd_value(N, dk, a, b, c, &re, &im);
+                double w = 1.0;
+                if (mas_corr) w = pow(sinc_pi(a, N) * sinc_pi(b, N) * sinc_pi(c, N), scheme + 1);
+                vr[t] = re / w;
+                vi[t] = im / w;
+                const long q2 = (long)a * a + (long)b * b + (long)c * c;
+                bin[t] = q2 == 0 ? -1 : bkd_bin(q2, Dmin, ndel, nb, int_rule, Di, ni);
+            }
+    for (int a = -h; a <= h; ++a)
+        for (int b = -h; b <= h; ++b)
+            for (int c = -h; c <= h; ++c) {
+                const size_t t1 = ((size_t)(a + h) * M + (b + h)) * M + (c + h);
+                const int s1 = bin[t1];
+                if (s1 < 0) continue;
+                const double p1 = vr[t1] * vr[t1] + vi[t1] * vi[t1];
+                psum[s1] += p1;
+                pcnt[s1] += 1;
+                for (int x = -h; x <= h; ++x)
+                    for (int y = -h; y <= h; ++y)
+                        for (int z = -h; z <= h; ++z) {
+                            const int u = -a - x, v = -b - y, w = -c - z;
+                            if (abs(u) > h || abs(v) > h || abs(w) > h) continue;
+                            const size_t t2 = ((size_t)(x + h) * M + (y + h)) * M + (z + h);
+                            const size_t t3 = ((size_t)(u + h) * M + (v + h)) * M + (w + h);
+                            int s2 = bin[t2], s3 = bin[t3];
+                            if (s2 < 0 || s3 < 0) continue;
+                            /* Re(d1 d2 d3) */
+                            const double r12 = vr[t1] * vr[t2] - vi[t1] * vi[t2];
+                            const double i12 = vr[t1] * vi[t2] + vi[t1] * vr[t2];
+                            const double re = r12 * vr[t3] - i12 * vi[t3];
+                            const double sn = p1 + vr[t2] * vr[t2] + vi[t2] * vi[t2] + vr[t3] * vr[t3] + vi[t3] * vi[t3];
+                            int lo = s1, mid = s2, hi = s3, tmp;
+                            if (lo > mid) { tmp = lo; lo = mid; mid = tmp; }
+                            if (mid > hi) { tmp = mid; mid = hi; hi = tmp; }
+                            if (lo > mid) { tmp = lo; lo = mid; mid = tmp; }
+                            const size_t o = ((size_t)lo * nb + mid) * nb + hi;
+                            bsum[o] += re;
+                            snsum[o] += sn;
+                            cnt[o] += 1;
+                        }
+            }
+    free(vr);
+    free(vi);
+    free(bin);
+}

@@ -115,6 +115,25 @@ void or_tweb(int N, double L, const double *delta, double R, double lambda_th,
 void or_tweb_from_phi(int N, double L, const double *phi, double lambda_th,
                       int8_t *labels, double *lambdas, int64_t counts[4]);
 
+/* Direct (brute-force) bispectrum of a box, FftwFunctions::bispectrum_fkp
+ * (Source/FftwFunctions.cpp:3105-3297): every ordered pair (k1, k2) of the
+ * (N+1)^3 grid q in [-N/2, N/2]^3 (both Nyquist planes, :3146-3147; the
+ * half-spectrum lookup of remap, Source/Miscelanious.cpp:43-62) whose
+ * k3 = -k1-k2 lies in the same grid (:3262), none of the three zero (:3279),
+ * accumulated in the bin triple sorted ascending (:3284-3287):
+ *   bsum[(a nb + b) nb + c] += Re(d1 d2 d3), snsum += |d1|^2+|d2|^2+|d3|^2,
+ *   cnt += 1   (a <= b <= c < nb);
+ *   psum[a] += |d1|^2, pcnt[a] += 1 over k1 (:3194-3199).
+ * dk: complex128 half spectrum N x N x (N/2+1).  d = dk / W with
+ * W = prod_c sinc(pi q_c/N)^(scheme+1) when mas_corr (correction_MAS,
+ * Miscelanious.cpp:2565-2577, applied to all three vectors).  Bins: shell
+ * s of |q| with Dmin + s ndel <= |q| < Dmin + (s+1) ndel (|q| in units of k_f):
+ * integer comparisons of |q|^2 when Dmin and ndel are integers, else FP64;
+ * modes below Dmin or at s >= nb are left out.  Outputs are accumulated into
+ * (caller zeroes). */
+void or_bk_direct(int N, const double *dk, double Dmin, double ndel, int nb, int scheme, int mas_corr,
+                  double *bsum, double *snsum, int64_t *cnt, double *psum, int64_t *pcnt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,3 +93,88 @@ def test_shells_beyond_third_of_grid_rejected():
         dk = ctx.fft_r2c(torch.zeros((N, N, N), device="cuda"))
         with pytest.raises(cm.CosmoError):
             ctx.bispectrum(dk, N, L, 12, 0.0, 2 * np.pi / L, "NGP", 1.0)
+
+
+# ------------------------------------------------- direct (brute-force) estimator
+def _direct_case(N, L, seed, scheme, mas):
+    import cosmicatlas_amd as cm
+    rng = np.random.default_rng(seed)
+    mesh = rng.poisson(2.0, (N, N, N)).astype(np.float32)
+    n = float(mesh.astype(np.float64).sum())
+    return cm, mesh, n
+
+
+@pytest.mark.parametrize("N,nb,Dmin,ndel,scheme,mas", [
+    (16, 13, 1.0, 1.0, "NGP", False),   # every mode, both Nyquist copies
+    (16, 6, 0.5, 1.7, "CIC", True),     # fractional edges (FP64 bin rule)
+    (24, 9, 2.0, 1.0, "TSC", True),     # modes below the first edge left out
+])
+def test_direct_matches_oracle(orc, N, nb, Dmin, ndel, scheme, mas):
+    """cm_bispectrum_direct against or_bk_direct on the same complex64 half
+    spectrum: counts bit-exact, sums to FP64 summation-order rounding."""
+    L = 200.0
+    kf = 2 * np.pi / L
+    cm, mesh, n = _direct_case(N, L, N + nb, scheme, mas)
+    sid = {"NGP": 0, "CIC": 1, "TSC": 2, "PCS": 3}[scheme]
+    with cm.Context(0) as ctx:
+        dk = ctx.fft_r2c(torch.from_numpy(mesh).cuda(), subtract=n / N ** 3)
+        out = ctx.bispectrum_direct(dk, N, L, nb, Dmin * kf, ndel * kf, scheme, n, mas_correction=mas)
+        dkh = dk.cpu().numpy().astype(np.complex128)
+    b, sn, cnt, ps, pc = orc.bk_direct(dkh, Dmin, ndel, nb, sid, mas)
+    assert np.array_equal(out["ntri"], cnt)
+    assert np.array_equal(out["nk"], pc)
+    assert cnt.sum() > 1000
+    # |Re d1 d2 d3| <= ((|d1|^2+|d2|^2+|d3|^2)/3)^1.5 per term bounds the scale of each sum
+    scale = np.where(cnt > 0, cnt * (sn / np.maximum(cnt, 1) / 3) ** 1.5, 0)
+    assert np.all(np.abs(out["bsum"] - b) <= 1e-11 * scale + 1e-300)
+    np.testing.assert_allclose(out["snsum"], sn, rtol=1e-12, atol=0)
+    V = L ** 3
+    ok = pc > 0
+    np.testing.assert_allclose(out["pk"][ok], ps[ok] / pc[ok] / (n * n / V) - V / n, rtol=1e-11)
+    if N == 16 and nb == 13:
+        h = N // 2
+        assert cnt.sum() == ((N + 1) ** 2 - h * (h + 1)) ** 3 - 3 * (N + 1) ** 3 + 2
+
+
+def test_direct_single_particle_is_pure_shot_noise():
+    """One object (NGP, one cell): d_k = exp(-i k.x) for k != 0, so P = V/n^2 - V/n = 0
+    and B = V^2/n^3 minus the Poisson shot noise 3 V^2/n^3 - 3V^2/n^2 + V^2/n^2 = 0."""
+    import cosmicatlas_amd as cm
+    N, L = 16, 50.0
+    mesh = np.zeros((N, N, N), np.float32)
+    mesh[3, 11, 6] = 1.0
+    with cm.Context(0) as ctx:
+        dk = ctx.fft_r2c(torch.from_numpy(mesh).cuda(), subtract=1.0 / N ** 3)
+        out = ctx.bispectrum_direct(dk, N, L, 12, 2 * np.pi / L, 2 * np.pi / L, "NGP", 1.0, mas_correction=False)
+    V = L ** 3
+    ok = out["ntri"] > 0
+    assert ok.sum() > 100
+    assert np.max(np.abs(out["bk"][ok])) < 1e-5 * V ** 2
+    assert np.max(np.abs(out["pk"])) < 1e-5 * V
+
+
+def test_direct_equals_shell_transform_estimator():
+    """Below N/3 (no Nyquist component, no wrapped triangle) the direct sums are
+    the shell-transform estimator's over every ordering of the bin triple: same
+    B for every bin, and ntri = (number of distinct orderings) x closed triangles."""
+    import itertools
+    import cosmicatlas_amd as cm
+    N, L = 32, 120.0
+    kf = 2 * np.pi / L
+    rng = np.random.default_rng(9)
+    mesh = rng.poisson(3.0, (N, N, N)).astype(np.float32)
+    n = float(mesh.astype(np.float64).sum())
+    ns = 8  # shells [s, s+1) k_f + 1 k_f: last edge 9 < 32/3
+    with cm.Context(0) as ctx:
+        dk = ctx.fft_r2c(torch.from_numpy(mesh).cuda(), subtract=n / N ** 3)
+        fast = ctx.bispectrum(dk, N, L, ns, kf, kf, "CIC", n, mas_correction=True, sn_correction=False)
+        direct = ctx.bispectrum_direct(dk, N, L, ns, kf, kf, "CIC", n, mas_correction=True, sn_correction=False)
+    checked = 0
+    for (i, j, k), bk, nt in zip(fast["ijk"], fast["bk"], fast["ntri"]):
+        a, b, c = sorted((i, j, k))
+        perms = len(set(itertools.permutations((a, b, c))))
+        assert direct["ntri"][a, b, c] == perms * nt, (i, j, k)
+        if nt:
+            assert abs(direct["bk"][a, b, c] - bk) <= 2e-4 * (abs(bk) + 1e-3 * np.abs(fast["bk"]).max())
+            checked += 1
+    assert checked > 50

@@ -250,3 +250,76 @@ def test_cli_redshift_catalogue(tmp_path, orc):
     p0, nm, sn = _oracle_pk(orc, ref, N, L)
     assert np.array_equal(out[:, 2].astype(int), nm)
     assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
+
+
+def _bk_catalogue(tmp_path, N, L, n, seed):
+    pos = np.random.default_rng(seed).uniform(0, L, (n, 3)).astype(np.float32)
+    np.savetxt(tmp_path / "cat.txt", pos, fmt="%.7f")
+    return np.loadtxt(tmp_path / "cat.txt").astype(np.float32)
+
+
+@pytest.mark.gpu
+def test_cli_direct_bispectrum(tmp_path, orc):
+    """Statistics = Bk_fkp (FftwFunctions::get_bispectrum_fkp): rows a b c k_a k_b
+    k_c B SN_sum ntri for every bin triple with triangles, against the oracle's
+    direct sums on the oracle's own mesh."""
+    N, L, n = 16, 300.0, 20000
+    pos = _bk_catalogue(tmp_path, N, L, n, 31)
+    ini = _ini(tmp_path, Nft=str(N), Lbox=str(L), mass_assignment_scheme="CIC", Statistics="Bk_fkp",
+               file_bispectrum="bk")
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = np.loadtxt(tmp_path / f"Bk_fkp_BOX_Nft{N}_CIC_bk.txt")
+    mesh = orc.mas(pos, N, L, orc.CIC)
+    dk = orc.fft_r2c(mesh - n / N ** 3)
+    nb = N // 2
+    b, sn, cnt, _, _ = orc.bk_direct(dk, 1, 1, nb, 1, True)
+    a_, b_, c_ = rows[:, 0].astype(int), rows[:, 1].astype(int), rows[:, 2].astype(int)
+    assert np.array_equal(np.sort(np.ravel_multi_index((a_, b_, c_), (nb,) * 3)),
+                          np.sort(np.ravel_multi_index(np.nonzero(cnt), (nb,) * 3)))
+    assert np.array_equal(rows[:, 8].astype(np.int64), cnt[a_, b_, c_])
+    kf = 2 * np.pi / L
+    np.testing.assert_allclose(rows[:, 3], kf + kf * (a_ + 0.5), rtol=1e-6)
+    V = L ** 3
+    m = cnt[a_, b_, c_]
+    snb = (sn[a_, b_, c_] / m / (n * n / V) - 3 * V / n) * V / n + (V / n) ** 2
+    B = b[a_, b_, c_] / (m * n ** 3 / V ** 2) - snb
+    scale = (sn[a_, b_, c_] / m / 3) ** 1.5 / (n ** 3 / V ** 2) / np.sqrt(m) + np.abs(snb)
+    assert np.max(np.abs(rows[:, 6] - B) / scale) < 1e-3
+    np.testing.assert_allclose(rows[:, 7], sn[a_, b_, c_], rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_cli_fast_bispectrum(tmp_path, orc):
+    """Statistics = Bk_fkp_fast: rows i j k B SN ntri (i >= j >= k) of the shells
+    DeltaK_data wide up to kmax_bk, from kmin_bk; without shot noise B equals the
+    direct estimator's and ntri x (distinct orderings) its pair count."""
+    import itertools
+    N, L, n = 32, 300.0, 50000
+    kf = 2 * np.pi / L
+    pos = _bk_catalogue(tmp_path, N, L, n, 32)
+    ini = _ini(tmp_path, Nft=str(N), Lbox=str(L), mass_assignment_scheme="TSC", Statistics="Bk_fkp_fast",
+               SN_correction="false", kmax_bk=str(9.5 * kf), kmin_bk=str(3.2 * kf), file_bispectrum="bk")
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = np.atleast_2d(np.loadtxt(tmp_path / f"Bk_fkp_fast_BOX_Nft{N}_TSC_bk.txt"))
+    ns = 9
+    # the outer shell starts at floor((3.2 - 1)/1) = 2; the inner loops of
+    # loop_shells_bispectrum (FftwFunctions.cpp:3530-3541) may go below it
+    assert rows[:, 0].min() == 2 and rows[:, :3].min() >= 0 and rows[:, :3].max() == ns - 1
+    assert np.all((rows[:, 0] >= rows[:, 1]) & (rows[:, 1] >= rows[:, 2]))
+    mesh = orc.mas(pos, N, L, orc.TSC)
+    b, _, cnt, _, _ = orc.bk_direct(orc.fft_r2c(mesh - n / N ** 3), 1, 1, ns, 2, True)
+    V = L ** 3
+    checked = 0
+    for i, j, k, B, sn, nt in rows:
+        a, bb, c = sorted((int(i), int(j), int(k)))
+        perms = len(set(itertools.permutations((a, bb, c))))
+        assert cnt[a, bb, c] == perms * int(nt)
+        assert sn == 0
+        if nt:
+            ref = b[a, bb, c] / (float(cnt[a, bb, c]) * float(n) ** 3 / V ** 2)
+            # float32 shell fields: an error floor of a few 1e-7 of the largest bin
+            assert abs(B - ref) <= 2e-4 * abs(ref) + 2e-6 * np.abs(rows[:, 3]).max(), (i, j, k, B, ref)
+            checked += 1
+    assert checked > 20
