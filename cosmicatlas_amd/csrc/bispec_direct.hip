@@ -5,22 +5,24 @@
 // ordered pair (k1, k2) of the grid q in [-N/2, N/2]^3 (N+1 points per axis, so
 // the Nyquist planes appear as +N/2 and -N/2) whose k3 = -k1-k2 lies in the same
 // grid, no vector zero, summed in the bin triple sorted ascending:
-//   bsum += Re(d1 d2 d3),  snsum += |d1|^2 + |d2|^2 + |d3|^2,  ntri += 1.
+//   bsum += Re(d1 d2 d3),  snsum += |d1|^2 + |d2|^2 + |d3|^2,  ntri += 1
+// (snsum as three times the sum of |d1|^2, an identity of the pair set).
 //
 // Layout: one table over the (N+1)^3 grid, indexed by
-// t = ((qx+h) M + (qy+h)) M + (qz+h) (M = N+1, h = N/2), holding the
+// t = ((qx+h) M + (qz+h)) M + (qy+h) (M = N+1, h = N/2; y fastest), holding the
 // MAS-corrected mode d(q) in double (the half-spectrum lookup of remap,
 // Source/Miscelanious.cpp:43-62: q_z < 0 reads the conjugate of -q) and its bin
 // (int8, -1 for q = 0, modes below the first bin and at or past the last).
 //
-// Pair kernel: one workgroup per k1 (bin s1 fixed).  For a fixed k1 the valid
+// Pair kernel: one workgroup per chunk of k1 of one bin s1.  For a fixed k1 the valid
 // k2 form a box (k3 inside the grid) cut by two spheres (|k2| and |k3| below
-// the last bin edge); each thread walks whole z lines of that region, so
-// successive k2 (and the mirrored k3) are adjacent table entries, and keeps a
-// running sum while the pair (s2, s3) stays the same, flushing it into an
-// LDS tile [s2][s3] when it changes.  The tile is then added into the sorted
-// triple's global sums.  Cost: sum over kept k1 of the kept k2, about
-// (4/3 pi R^3)^2 pairs for a last bin edge of R k_f.
+// the last bin edge); the 64 lanes of a wave take 64 adjacent y lines of one x
+// and walk them in z, so each load of the wave reads adjacent table entries
+// (for k2 and the mirrored k3), and each lane keeps a running sum while its
+// pair (s2, s3) stays the same, flushing it into an LDS tile [s2][s3] when it
+// changes.  The tile is then added into the sorted triple's global sums.
+// Cost: sum over kept k1 of the kept k2, about (4/3 pi R^3)^2 pairs for a last
+// bin edge of R k_f.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -30,6 +32,10 @@
 namespace cm {
 
 constexpr int kBkdMaxBins = 64;
+constexpr int kBkdZ = 4;  // z steps whose table loads are in flight together
+// table padding (entries) before and after the grid: a round of kBkdZ steps
+// may read up to (kBkdZ-1) lines past either end of a z line
+constexpr unsigned kBkdPad = 4 * 1025;
 
 // bin of |q|^2 = q2 (units of k_f): exact integer comparisons when the bin
 // edges Di + s ni are integers, else FP64 floor((|q| - Dmin)/ndel)
@@ -72,8 +78,9 @@ __global__ void __launch_bounds__(256) k_bkd_table(const float2 *__restrict__ dk
     const size_t G = (size_t)M * M * M;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < G) {
-        const int c = (int)(t % M) - h;
-        const int b = (int)((t / M) % M) - h;
+        // table layout ((qx+h) M + (qz+h)) M + (qy+h): y fastest
+        const int b = (int)(t % M) - h;
+        const int c = (int)((t / M) % M) - h;
         const int a = (int)(t / ((size_t)M * M)) - h;
         int qx = a, qy = b, qz = c;
         double sg = 1.0;
@@ -89,10 +96,10 @@ __global__ void __launch_bounds__(256) k_bkd_table(const float2 *__restrict__ dk
         double w = 1.0;
         if (mas) w = pow(bkd_sinc(a, N) * bkd_sinc(b, N) * bkd_sinc(c, N), (double)expo);
         const double re = (double)v.x / w, im = sg * (double)v.y / w;
-        tv[t] = double2{re, im};
+        tv[t + kBkdPad] = double2{re, im};
         const long q2 = (long)a * a + (long)b * b + (long)c * c;
         const int s = q2 == 0 ? -1 : bkd_bin(q2, int_rule, Di, ni, Dmin, ndel, nb);
-        tb[t] = (int8_t)s;
+        tb[t + kBkdPad] = (int8_t)s;
         if (s >= 0) {
             atomicAdd(&s_p[s], re * re + im * im);
             atomicAdd(&s_n[s], 1u);
@@ -113,82 +120,157 @@ __device__ __forceinline__ int isqrt_floor(long r) {
     return (int)s;
 }
 
-// One workgroup per k1: table index blockIdx.y M^2 + blockIdx.x.  LDS tile: re[nb^2],
-// sn[nb^2] (double), n[nb^2] (u32).  r2max: the largest |q|^2 of any kept
-// mode (prunes k2 and k3 to spheres; the bins decide).
+// Table indices of the kept k1 grouped by bin: list[cursor[s]++] = t (cursor
+// starts at the bin's offset).  Positions are reserved per workgroup and bin
+// (one global atomic per nonempty bin), then ranked inside the workgroup.
+__global__ void __launch_bounds__(256) k_bkd_list(const int8_t *__restrict__ tb, unsigned G, int nb,
+                                                  unsigned *__restrict__ cursor, unsigned *__restrict__ list) {
+    __shared__ unsigned s_cnt[kBkdMaxBins], s_base[kBkdMaxBins];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s_cnt[i] = 0;
+    __syncthreads();
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = t < G ? (int)tb[t + kBkdPad] : -1;
+    unsigned rank = 0;
+    if (s >= 0) rank = atomicAdd(&s_cnt[s], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (s_cnt[i]) s_base[i] = atomicAdd(&cursor[i], s_cnt[i]);
+    __syncthreads();
+    if (s >= 0) list[s_base[s] + rank] = t;
+}
+
+// One workgroup per chunk of k1 of one bin: chunk[blockIdx.x] = {s1, first,
+// count} into list.  LDS tiles: re[nb^2], sum of p1 n[nb^2] (double), n[nb^2]
+// and the current k1's n[nb^2] (u32, folded after each k1); the chunk's tiles are
+// added once into the global sums.
+// r2max: the largest |q|^2 of any kept mode (prunes k2 and k3 to spheres; the
+// bins decide).  Work items are (x, 64 consecutive y) of k2, one per wave; lane
+// y walks its z line, so every table load of a wave reads 64 adjacent entries
+// (the table keeps y fastest), for k2 and for the mirrored k3 alike.
+//
+// Only sum p(k1) is accumulated: the ordered pairs of one sorted bin triple
+// are closed under (k1, k2, k3) -> (k2, k3, k1), so sum p1 = sum p2 = sum p3
+// over them and the caller's |d1|^2+|d2|^2+|d3|^2 sum is three times it.
 __global__ void __launch_bounds__(256) k_bkd_pairs(const double2 *__restrict__ tv, const int8_t *__restrict__ tb,
+                                                   const unsigned *__restrict__ list, const int4 *__restrict__ chunks,
                                                    int N, int nb, long r2max, double *__restrict__ bsum,
-                                                   double *__restrict__ snsum, unsigned long long *__restrict__ cnt) {
-    const size_t t1 = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-    const int s1 = tb[t1];
-    if (s1 < 0) return;  // uniform over the workgroup, before any barrier
+                                                   double *__restrict__ psum3, unsigned long long *__restrict__ cnt) {
+    const int M = N + 1, h = N / 2;
+    const int4 ch = chunks[blockIdx.x];
+    const int s1 = ch.x;
     extern __shared__ double s_tile[];
     const int nb2 = nb * nb;
     double *s_re = s_tile;
     double *s_sn = s_tile + nb2;
-    unsigned *s_n = (unsigned *)(s_tile + 2 * nb2);
+    unsigned *s_n = (unsigned *)(s_tile + 2 * nb2);  // pairs of the chunk
+    unsigned *s_nk = s_n + nb2;                       // pairs of the current k1
     for (int i = threadIdx.x; i < nb2; i += blockDim.x) {
         s_re[i] = 0.0;
         s_sn[i] = 0.0;
         s_n[i] = 0;
+        s_nk[i] = 0;
     }
     __syncthreads();
-    const int M = N + 1, h = N / 2;
-    const int c = (int)(t1 % M) - h;
-    const int b = (int)((t1 / M) % M) - h;
-    const int a = (int)(t1 / ((size_t)M * M)) - h;
-    const double2 v1 = tv[t1];
-    const double p1 = v1.x * v1.x + v1.y * v1.y;
-    // k3 = -k1-k2 inside the grid: per axis max(-h, -h - q1) <= q2 <= min(h, h - q1)
-    const int xlo = max(-h, -h - a), xhi = min(h, h - a);
-    const int ylo = max(-h, -h - b), yhi = min(h, h - b);
-    const int zlo0 = max(-h, -h - c), zhi0 = min(h, h - c);
-    const int nly = yhi - ylo + 1;
-    const int nlines = (xhi - xlo + 1) * nly;
-    for (int line = threadIdx.x; line < nlines; line += blockDim.x) {
-        const int x = xlo + line / nly, y = ylo + line % nly;
-        const long r2 = r2max - (long)x * x - (long)y * y;
-        const int u = -a - x, v = -b - y;
-        const long r3 = r2max - (long)u * u - (long)v * v;
-        if (r2 < 0 || r3 < 0) continue;
-        const int l2 = isqrt_floor(r2), l3 = isqrt_floor(r3);
-        const int zlo = max(zlo0, max(-l2, -c - l3));
-        const int zhi = min(zhi0, min(l2, -c + l3));
-        if (zlo > zhi) continue;
-        const size_t base2 = ((size_t)(x + h) * M + (y + h)) * M + h;  // + z
-        const size_t base3 = ((size_t)(u + h) * M + (v + h)) * M + h;  // + (-c - z)
-        int key = -1;
-        double re = 0.0, sn = 0.0;
-        unsigned n = 0;
-        for (int z = zlo; z <= zhi; ++z) {
-            const size_t t2 = base2 + z, t3 = base3 - c - z;
-            const int s2 = tb[t2], s3 = tb[t3];
-            if ((s2 | s3) < 0) continue;
-            const int k = s2 * nb + s3;
-            if (k != key) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned uM = (unsigned)M;
+    const int rmax = isqrt_floor(r2max);
+    int item = 0;  // work items continue across the chunk's k1, balancing the waves
+    for (int q = 0; q < ch.z; ++q) {
+        // table index ((qx+h) M + (qz+h)) M + (qy+h), stored kBkdPad entries in
+        const unsigned t1 = list[ch.y + q];
+        const int a = (int)(t1 / (uM * uM)) - h;
+        const int c = (int)((t1 / uM) % uM) - h;
+        const int b = (int)(t1 % uM) - h;
+        const double2 v1 = tv[t1 + kBkdPad];
+        const double p1 = v1.x * v1.x + v1.y * v1.y;
+        // k3 = -k1-k2 inside the grid: per axis max(-h, -h - q1) <= q2 <= min(h, h - q1)
+        // ... and |x| <= rmax, |u| = |a + x| <= rmax (the spheres)
+        const int xlo = max(max(-h, -h - a), max(-rmax, -a - rmax)), xhi = min(min(h, h - a), min(rmax, -a + rmax));
+        const int ylo = max(-h, -h - b), yhi = min(h, h - b);
+        const int zlo0 = max(-h, -h - c), zhi0 = min(h, h - c);
+        for (int x = xlo; x <= xhi; ++x) {
+            const int u = -a - x;
+            const long rx2 = r2max - (long)x * x, rx3 = r2max - (long)u * u;
+            if (rx2 < 0 || rx3 < 0) continue;
+            const int ly2 = isqrt_floor(rx2), ly3 = isqrt_floor(rx3);
+            const int y0 = max(ylo, max(-ly2, -b - ly3)), y1 = min(yhi, min(ly2, -b + ly3));
+            const int zx0 = max(zlo0, max(-ly2, -c - ly3)), zx1 = min(zhi0, min(ly2, -c + ly3));
+            if (y0 > y1 || zx0 > zx1) continue;
+            for (int yc = y0; yc <= y1; yc += 64, ++item) {
+                if (item % nw != wave) continue;
+                const int y = yc + lane, v = -b - y;
+                int zl = 0, zh = -1;
+                if (y <= y1) {
+                    const long r2 = rx2 - (long)y * y, r3 = rx3 - (long)v * v;
+                    if (r2 >= 0 && r3 >= 0) {
+                        const int l2 = isqrt_floor(r2), l3 = isqrt_floor(r3);
+                        zl = max(zx0, max(-l2, -c - l3));
+                        zh = min(zx1, min(l2, -c + l3));
+                    }
+                }
+                if (zl > zh) continue;
+                // padded indices of k2 = (x, y, z) and k3 = (u, v, -c-z); z + i
+                // past zh reads the padding or the next line and is skipped
+                unsigned t2 = ((unsigned)(x + h) * uM + (unsigned)(zl + h)) * uM + (unsigned)(y + h) + kBkdPad;
+                unsigned t3 = ((unsigned)(u + h) * uM + (unsigned)(-c - zl + h)) * uM + (unsigned)(v + h) + kBkdPad;
+                int key = -1;
+                double re = 0.0;
+                unsigned n = 0;
+                for (int z = zl; z <= zh; z += kBkdZ, t2 += kBkdZ * uM, t3 -= kBkdZ * uM) {
+                    int bb2[kBkdZ], bb3[kBkdZ];
+                    double2 w2[kBkdZ], w3[kBkdZ];
+#pragma unroll
+                    for (int i = 0; i < kBkdZ; ++i) {
+                        bb2[i] = tb[t2 + i * uM];
+                        bb3[i] = tb[t3 - i * uM];
+                        w2[i] = tv[t2 + i * uM];
+                        w3[i] = tv[t3 - i * uM];
+                    }
+                    // products for every step (keeps all loads ahead of the first use)
+                    double val[kBkdZ];
+#pragma unroll
+                    for (int i = 0; i < kBkdZ; ++i) {
+                        const double r12 = v1.x * w2[i].x - v1.y * w2[i].y;
+                        const double i12 = v1.x * w2[i].y + v1.y * w2[i].x;
+                        val[i] = r12 * w3[i].x - i12 * w3[i].y;
+                    }
+#pragma unroll
+                    for (int i = 0; i < kBkdZ; ++i) {
+                        // a skipped step keeps the key and adds nothing (no branch
+                        // around the products, so their loads stay hoisted)
+                        const bool ok = z + i <= zh && (bb2[i] | bb3[i]) >= 0;
+                        const int k = ok ? bb2[i] * nb + bb3[i] : key;
+                        if (k != key) {
+                            if (n) {
+                                atomicAdd(&s_re[key], re);
+                                atomicAdd(&s_nk[key], n);
+                            }
+                            key = k;
+                            re = 0.0;
+                            n = 0;
+                        }
+                        re += ok ? val[i] : 0.0;
+                        n += ok ? 1u : 0u;
+                    }
+                }
                 if (n) {
                     atomicAdd(&s_re[key], re);
-                    atomicAdd(&s_sn[key], sn);
-                    atomicAdd(&s_n[key], n);
+                    atomicAdd(&s_nk[key], n);
                 }
-                key = k;
-                re = sn = 0.0;
-                n = 0;
             }
-            const double2 v2 = tv[t2], v3 = tv[t3];
-            const double r12 = v1.x * v2.x - v1.y * v2.y;
-            const double i12 = v1.x * v2.y + v1.y * v2.x;
-            re += r12 * v3.x - i12 * v3.y;
-            sn += p1 + (v2.x * v2.x + v2.y * v2.y) + (v3.x * v3.x + v3.y * v3.y);
-            ++n;
         }
-        if (n) {
-            atomicAdd(&s_re[key], re);
-            atomicAdd(&s_sn[key], sn);
-            atomicAdd(&s_n[key], n);
+        // fold this k1's pair counts: p1 n into the p(k1) sums, n into the totals
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb2; i += blockDim.x) {
+            const unsigned nk = s_nk[i];
+            if (nk) {
+                s_sn[i] += p1 * (double)nk;
+                s_n[i] += nk;
+                s_nk[i] = 0;
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int e = threadIdx.x; e < nb2; e += blockDim.x) {
         const unsigned n = s_n[e];
         if (!n) continue;
@@ -198,7 +280,7 @@ __global__ void __launch_bounds__(256) k_bkd_pairs(const double2 *__restrict__ t
         if (lo > mid) { const int tt = lo; lo = mid; mid = tt; }
         const size_t o = ((size_t)lo * nb + mid) * nb + hi;
         unsafeAtomicAdd(&bsum[o], s_re[e]);
-        unsafeAtomicAdd(&snsum[o], s_sn[e]);
+        unsafeAtomicAdd(&psum3[o], s_sn[e]);
         atomicAdd(&cnt[o], (unsigned long long)n);
     }
 }
@@ -236,8 +318,9 @@ extern "C" cm_status cm_bispectrum_direct(cm_ctx *ctx, const cm_bkd_opts *o, con
     const size_t G = (size_t)M * M * M, nb3 = (size_t)nb * nb * nb;
     cm_status s;
     void *tv, *tb, *sums, *cnts;
-    if ((s = scratch(ctx, "bkd.tv", sizeof(double2) * G, &tv)) != CM_OK) return s;
-    if ((s = scratch(ctx, "bkd.tb", G, &tb)) != CM_OK) return s;
+    static_assert(kBkdPad >= (kBkdZ - 1) * 1025u, "padding below the largest read-ahead");
+    if ((s = scratch(ctx, "bkd.tv", sizeof(double2) * (G + 2 * kBkdPad), &tv)) != CM_OK) return s;
+    if ((s = scratch(ctx, "bkd.tb", G + 2 * kBkdPad, &tb)) != CM_OK) return s;
     if ((s = scratch(ctx, "bkd.sums", sizeof(double) * (2 * nb3 + nb), &sums)) != CM_OK) return s;
     if ((s = scratch(ctx, "bkd.cnts", sizeof(unsigned long long) * (nb3 + nb), &cnts)) != CM_OK) return s;
     auto *dsum = (double *)sums;  // bsum[nb3], snsum[nb3], psum[nb]
@@ -249,12 +332,46 @@ extern "C" cm_status cm_bispectrum_direct(cm_ctx *ctx, const cm_bkd_opts *o, con
                     N, nb, int_rule, Di, ni, Dmin, ndel, expo, o->mas_correction ? 1 : 0, (double2 *)tv, (int8_t *)tb,
                     dsum + 2 * nb3, dcnt + nb3)) != CM_OK)
         return s;
-    const size_t lds = (2 * sizeof(double) + sizeof(unsigned)) * (size_t)nb * nb;
+    const size_t lds = (2 * sizeof(double) + 2 * sizeof(unsigned)) * (size_t)nb * nb;
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum_direct: %d bins need %zu B LDS", nb, lds);
-    if ((s = launch(ctx, "bkd_pairs", k_bkd_pairs, dim3((unsigned)M * M, (unsigned)M), dim3(256), lds, (const double2 *)tv,
-                    (const int8_t *)tb, N, nb, r2max, dsum, dsum + nb3, dcnt)) != CM_OK)
-        return s;
+    // k1 grouped by bin, in chunks of K sharing one LDS tile (one global flush per
+    // chunk instead of per k1); K keeps about 32 chunks per CU
+    std::vector<unsigned long long> pc(nb);
+    CM_HIP(ctx, hipMemcpyAsync(pc.data(), dcnt + nb3, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long nk1 = 0;
+    for (int i = 0; i < nb; ++i) nk1 += pc[i];
+    if (nk1 > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum_direct: too many modes");
+    const unsigned long long target = (unsigned long long)std::max(1, ctx->cu_count) * 32;
+    const int K = (int)std::max<unsigned long long>(1, std::min<unsigned long long>(64, nk1 / target));
+    std::vector<unsigned> off(nb);
+    std::vector<int4> chunks;
+    unsigned acc = 0;
+    for (int i = 0; i < nb; ++i) {
+        off[i] = acc;
+        for (unsigned long long f = 0; f < pc[i]; f += (unsigned long long)K)
+            chunks.push_back(int4{i, (int)(acc + f), (int)std::min<unsigned long long>(K, pc[i] - f), 0});
+        acc += (unsigned)pc[i];
+    }
+    if (chunks.size() > (1u << 24)) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum_direct: too many modes");
+    if (!chunks.empty()) {
+        void *dlist, *dcur, *dch;
+        if ((s = scratch(ctx, "bkd.list", sizeof(unsigned) * (size_t)nk1, &dlist)) != CM_OK) return s;
+        if ((s = scratch(ctx, "bkd.cursor", sizeof(unsigned) * (size_t)nb, &dcur)) != CM_OK) return s;
+        if ((s = scratch(ctx, "bkd.chunks", sizeof(int4) * chunks.size(), &dch)) != CM_OK) return s;
+        CM_HIP(ctx, hipMemcpyAsync(dcur, off.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice, ctx->stream));
+        CM_HIP(ctx, hipMemcpyAsync(dch, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        if ((s = launch(ctx, "bkd_list", k_bkd_list, dim3((unsigned)((G + 255) / 256)), dim3(256), 0,
+                        (const int8_t *)tb, (unsigned)G, nb, (unsigned *)dcur, (unsigned *)dlist)) != CM_OK)
+            return s;
+        if ((s = launch(ctx, "bkd_pairs", k_bkd_pairs, dim3((unsigned)chunks.size()), dim3(256), lds,
+                        (const double2 *)tv, (const int8_t *)tb, (const unsigned *)dlist, (const int4 *)dch, N, nb,
+                        r2max, dsum, dsum + nb3, dcnt)) != CM_OK)
+            return s;
+    }
     std::vector<double> hs(2 * nb3 + nb);
     std::vector<unsigned long long> hc(nb3 + nb);
     CM_HIP(ctx, hipMemcpyAsync(hs.data(), dsum, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -272,13 +389,14 @@ extern "C" cm_status cm_bispectrum_direct(cm_ctx *ctx, const cm_bkd_opts *o, con
         const unsigned long long m = hc[e];
         ntri[e] = (int64_t)m;
         if (bsum_out) bsum_out[e] = hs[e];
-        if (snsum_out) snsum_out[e] = hs[nb3 + e];
+        const double snsum = 3.0 * hs[nb3 + e];  // see k_bkd_pairs
+        if (snsum_out) snsum_out[e] = snsum;
         if (!m) {
             bk[e] = sn[e] = 0.0;
             continue;
         }
         // <P1+P2+P3>/nbar + 1/nbar^2 with P_i = |d_i|^2/normal_p - V/n
-        const double snb = snc ? (hs[nb3 + e] / (double)m / normal_p - 3.0 * sn_p) * sn_p + sn_p * sn_p : 0.0;
+        const double snb = snc ? (snsum / (double)m / normal_p - 3.0 * sn_p) * sn_p + sn_p * sn_p : 0.0;
         sn[e] = snb;
         bk[e] = hs[e] / ((double)m * normal_b) - snb;
     }
