@@ -126,26 +126,47 @@ def cpu_baseline(pos, quick=False):
         import oracle.cpu_ref as cr
     except Exception as e:  # pragma: no cover
         return {"value": None, "unit": "mocks/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    # all host cores of the machine's share (16 per GPU), and the reference's
+    # hard-coded 12 OpenMP threads (_NTHREADS_)
     threads = int(os.environ.get("CPU_BASELINE_THREADS", min(16, os.cpu_count() or 1)))
     NS = 128 if quick else 256
+    reps = 1 if quick else 5
     npart = (NS // 2) ** 3
     sub = pos[:: max(1, len(pos) // npart)][:npart].astype(np.float32) * np.float32(1.0)
     Ls = L * NS / N  # same cell size d = L/N
     sub = np.mod(sub, Ls).astype(np.float32)
-    t = cr.time_pipeline(sub, NS, Ls, scheme=2, ndel=NDEL, R=R_SMOOTH, lambda_th=LAMBDA_TH, threads=threads,
-                         repeats=1)
-    # scale: mesh passes ~N^3, FFT ~N^3 log N, deposit ~N_p
-    f_mesh = (N / NS) ** 3
-    f_fft = f_mesh * math.log2(N) / math.log2(NS)
-    f_part = (NL ** 3) / npart
-    est = t["mas"] * f_part + t["fft"] * f_fft + t["pk"] * f_mesh + t["tweb"] * (
-        f_mesh * (t["tweb_fft"] / max(t["tweb"], 1e-12)) * math.log2(N) / math.log2(NS)
-        + f_mesh * (1 - t["tweb_fft"] / max(t["tweb"], 1e-12)))
-    return {"value": 1.0 / est, "unit": "mocks/s", "cores": threads, "kind": "port",
+
+    def mocks_per_s(nthreads):
+        t = cr.time_pipeline(sub, NS, Ls, scheme=2, ndel=NDEL, R=R_SMOOTH, lambda_th=LAMBDA_TH, threads=nthreads,
+                             repeats=reps, warmup=1)
+        # scale: mesh passes ~N^3, FFT ~N^3 log N, deposit ~N_p
+        f_mesh = (N / NS) ** 3
+        f_fft = f_mesh * math.log2(N) / math.log2(NS)
+        f_part = (NL ** 3) / npart
+        est = t["mas"] * f_part + t["fft"] * f_fft + t["pk"] * f_mesh + t["tweb"] * (
+            f_mesh * (t["tweb_fft"] / max(t["tweb"], 1e-12)) * math.log2(N) / math.log2(NS)
+            + f_mesh * (1 - t["tweb_fft"] / max(t["tweb"], 1e-12)))
+        return 1.0 / est, t
+
+    v, t = mocks_per_s(threads)
+    v12, t12 = mocks_per_s(12)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": v, "unit": "mocks/s", "cores": threads, "kind": "port",
             "sample": f"full pipeline at {NS}^3 mesh with {npart} particles (same cell size, TSC, "
-                      f"{NS // NDEL // 2} shells, T-web), {sum(t[k] for k in ('mas', 'fft', 'pk', 'tweb')):.2f} s, "
-                      f"scaled to 512^3 / 256^3 particles by N^3, N^3 log N and N_p",
-            "stage_seconds_sample": {k: round(v, 4) for k, v in t.items()}}
+                      f"{NS // NDEL // 2} shells, T-web), median of {reps} runs after 1 warm-up, "
+                      f"{sum(t[k] for k in ('mas', 'fft', 'pk', 'tweb')):.2f} s per run, scaled to 512^3 / 256^3 "
+                      f"particles by N^3, N^3 log N and N_p; the port's own radix-2 CPU FFT (no FFTW on the host)",
+            "stage_seconds_sample": {k: round(x, 4) for k, x in t.items()},
+            "threads_12": {"value": v12, "stage_seconds_sample": {k: round(x, 4) for k, x in t12.items()}},
+            "host": {"nproc": os.cpu_count(), "model": model}}
 
 
 def served(cm, posd, b, slots, nreq, rate=None, seed=0):
@@ -187,6 +208,72 @@ def served(cm, posd, b, slots, nreq, rate=None, seed=0):
     return {"value": round(nreq / T, 2), "unit": "mocks/s", "slots": slots, "requests": nreq,
             "arrivals": "all at t=0" if rate is None else f"Poisson, {rate:.0f}/s",
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+
+MIXED_NL = (48, 64, 96, 128, 192, 256, 384)  # FFT sizes for the generator's Lagrangian grid
+
+
+def served_mixed(cm, slots=3, nreq=96, rate=None, seed=5, specs=None):
+    """Mixed serving workload: N uniform in {128, 192, 256, 384, 512, 768}, NL^3
+    particles with NL^3 log-uniform in [1e5, 1e8] and NL rounded (in log) to the
+    nearest Lagrangian grid the generator's FFT takes (48 .. 384), scheme uniform in NGP/CIC/TSC,
+    T-web on half of the requests, each mock generated on the GPU from its own
+    seed; admission by HBM footprint.  All requests at once (rate None) or Poisson
+    arrivals at `rate` per second.  Returns mocks/s, latency percentiles (ms) and
+    the specs (so the Poisson leg replays the same requests)."""
+    import threading
+    import torch
+    rng = np.random.default_rng(seed)
+    if specs is None:
+        specs = []
+        for i in range(nreq):
+            Nm = int(rng.choice([128, 192, 256, 384, 512, 768]))
+            nl = math.exp(rng.uniform(math.log(1e5), math.log(1e8))) ** (1.0 / 3.0)
+            NLm = min(MIXED_NL, key=lambda g: abs(math.log(g / nl)))
+            Lm = float(rng.uniform(250.0, 2000.0))
+            sch = ["NGP", "CIC", "TSC"][int(rng.integers(0, 3))]
+            tw = bool(rng.integers(0, 2))
+            specs.append((Nm, NLm, Lm, sch, tw, 2000 + i))
+    nreq = len(specs)
+
+    def mk(spec):
+        Nm, NLm, Lm, sch, tw, sd = spec
+        ndel = 1 if Nm in (128, 192) else Nm // 128
+        bm = cm.binning(Nm, Lm, ndel=ndel)
+        kw = dict(do_tweb=True, R_smooth=2.0 * Lm / Nm, margin_tol=1e-6) if tw else {}
+        return srv.request(Nm, Lm, sch, 64, bm["kmin"], bm["dk"], gen=dict(NL=NLm, seed=sd), **kw)
+
+    with cm.Server(0, slots=slots) as srv:
+        for t in [srv.submit(mk(sp)) for sp in specs]:  # warm-up: buffers, tables, graphs
+            t.result()
+        torch.cuda.synchronize()
+        tickets = [None] * nreq
+        t0 = time.perf_counter()
+        if rate is None:
+            for i in range(nreq):
+                tickets[i] = srv.submit(mk(specs[i]))
+        else:
+            def producer():
+                nxt = time.perf_counter()
+                for i in range(nreq):
+                    nxt += float(rng.exponential(1.0 / rate))
+                    d = nxt - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    tickets[i] = srv.submit(mk(specs[i]))
+            th = threading.Thread(target=producer)
+            th.start()
+            th.join()
+        lat = np.array([t.result()["latency_ms"] for t in tickets])
+        T = time.perf_counter() - t0
+    npart = sum(sp[1] ** 3 for sp in specs)
+    return {"value": round(nreq / T, 2), "unit": "mocks/s", "slots": slots, "requests": nreq,
+            "particles": int(npart),
+            "workload": "N in {128..768}, NL^3 particles log-uniform 1e5-1e8 (NL rounded to 48..384) "
+                        "generated on the GPU, NGP/CIC/TSC, 64 shells, T-web on half",
+            "arrivals": "all at t=0" if rate is None else f"Poisson, {rate:.0f}/s",
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3)}, specs
 
 
 def batched(cm, nmocks=64, slots=3, seed=7):
@@ -451,7 +538,10 @@ def main():
     if not args.no_serve:
         closed = served(cm, posd, b, slots=2, nreq=40)
         poisson = served(cm, posd, b, slots=2, nreq=60, rate=0.8 * closed["value"], seed=1)
-        serve = {"closed_loop": closed, "poisson": poisson, "batched_c4": batched(cm)}
+        mixed, specs = served_mixed(cm)
+        mixed_p, _ = served_mixed(cm, rate=0.8 * mixed["value"], seed=6, specs=specs)
+        serve = {"closed_loop": closed, "poisson": poisson, "batched_c4": batched(cm),
+                 "mixed_c5": {"closed_loop": mixed, "poisson": mixed_p}}
 
     cpu = None if args.no_cpu else cpu_baseline(pos, quick=args.quick_cpu)
 

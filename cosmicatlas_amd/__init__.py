@@ -386,6 +386,8 @@ class Ticket:
         lat = C.c_double()
         err = C.create_string_buffer(512)
         st = self.server.lib.cm_server_wait(self.server.h, self.id, float(timeout), C.byref(lat), err, 512)
+        if st != _abi.CM_ERR_TIMEOUT:  # the job is finished: its buffers are no longer written
+            self.server._live.pop(self.id, None)
         if st != 0:
             raise CosmoError(st, err.value.decode() or self.server.lib.cm_status_string(st).decode())
         info = self.out["_info"]
@@ -402,6 +404,10 @@ class Server:
     def __init__(self, device=0, slots=2, hbm_budget=0):
         self.lib = _abi.load()
         self.device = device
+        # submitted tickets whose results have not been collected: the server
+        # writes into their host and device buffers until the job finishes, so
+        # they live until result() or close(), whatever the caller keeps
+        self._live = {}
         h = C.c_void_p()
         st = self.lib.cm_server_create(device, slots, hbm_budget, C.byref(h))
         if st != 0:
@@ -410,8 +416,9 @@ class Server:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.cm_server_destroy(self.h)
+            self.lib.cm_server_destroy(self.h)  # cancels queued jobs, waits for running ones
             self.h = None
+        getattr(self, "_live", {}).clear()
 
     def __del__(self):
         try:
@@ -454,6 +461,7 @@ class Server:
         if st != 0:
             raise CosmoError(st, f"cm_server_submit: {self.lib.cm_status_string(st).decode()}")
         ticket.id = t.value
+        self._live[ticket.id] = ticket
         return ticket
 
     def batch(self, tickets):

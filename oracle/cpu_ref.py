@@ -24,17 +24,20 @@ def lib():
     return _LIB
 
 
-def time_pipeline(pos, N, L, scheme=2, ndel=1, R=2.0, lambda_th=0.0, threads=0, repeats=1):
-    """Seconds per stage (mas, fft, pk, tweb, tweb_fft) of the TSC pipeline at N^3."""
+def time_pipeline(pos, N, L, scheme=2, ndel=1, R=2.0, lambda_th=0.0, threads=0, repeats=1, warmup=0):
+    """Seconds per stage (mas, fft, pk, tweb, tweb_fft) of the TSC pipeline at N^3:
+    the run with the median total of `repeats` timed runs after `warmup` untimed ones."""
     assert scheme == 2, "the CPU baseline restates the TSC path"
     pos = np.ascontiguousarray(pos, np.float32)
     kf = 2 * math.pi / L
     ns = N // ndel // 2
-    best = None
-    for _ in range(repeats):
+    runs = []
+    for r in range(warmup + repeats):
         out = np.zeros(5)
         lib().cr_time_pipeline(N, L, pos.ctypes.data_as(C.c_void_p), len(pos), ns, ndel * kf, R, lambda_th,
                                threads, out.ctypes.data_as(C.c_void_p))
-        if best is None or out[:4].sum() < best[:4].sum():
-            best = out
-    return {"mas": best[0], "fft": best[1], "pk": best[2], "tweb": best[3], "tweb_fft": best[4]}
+        if r >= warmup:
+            runs.append(out)
+    runs.sort(key=lambda o: o[:4].sum())
+    med = runs[len(runs) // 2]
+    return {"mas": med[0], "fft": med[1], "pk": med[2], "tweb": med[3], "tweb_fft": med[4]}

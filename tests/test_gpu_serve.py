@@ -136,3 +136,31 @@ def test_request_errors_and_cancel():
           for i in range(20)]
     srv.close()
     assert ts[0].server.h is None
+
+
+def test_dropped_tickets_outlive_their_jobs():
+    """A caller that drops its tickets while their jobs run (here: an exception
+    while collecting a failed request, with valid ones still in flight) must not
+    free the buffers the server writes: the server holds every submitted ticket
+    until its result is collected or the server is closed."""
+    import gc
+    import weakref
+    import cosmicatlas_amd as cm
+    b = cm.binning(64, 200.0)
+    with cm.Server(0, slots=2) as srv:
+        good = [srv.submit(srv.request(64, 200.0, "TSC", b["nshells"], b["kmin"], b["dk"], gen=dict(NL=64, seed=i),
+                                       do_tweb=True, R_smooth=6.0, margin_tol=1e-6)) for i in range(6)]
+        bad = srv.submit(srv.request(40, 200.0, "TSC", 20, b["kmin"], b["dk"], gen=dict(NL=16, seed=1)))
+        refs = [weakref.ref(t) for t in good]
+        kept = good[-1]
+        del good
+        gc.collect()
+        assert all(r() is not None for r in refs)
+        with pytest.raises(cm.CosmoError):
+            bad.result(timeout=60.0)
+        out = kept.result(timeout=60.0)
+        assert out["n_obj"] == pytest.approx(64 ** 3, rel=1e-6)
+        assert refs[-1]() is not None  # still referenced here, no longer by the server
+        assert kept.id not in srv._live
+    gc.collect()
+    assert all(r() is None for r in refs[:-1])
