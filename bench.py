@@ -331,6 +331,183 @@ def batched(cm, nmocks=64, slots=3, seed=7):
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
 
+ROW_KERNELS = KERNELS + ["zel_spectra", "zel_displace", "uniform", "fkp_wscale", "fkp_wsums", "bk_masks", "bk_sums",
+                         "bk_reduce", "bkd_table", "bkd_list", "bkd_pairs", "fft_x_inv", "fft_x_fwd"]
+FP64_PEAK_TFS = 78.6
+
+
+def _timed(fn, reps, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def _kernels(ctx, fn, nrep=3):
+    """Mean HIP-event time and launches per call of every library kernel fn runs."""
+    import torch
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(nrep):
+        fn()
+    torch.cuda.synchronize()
+    out = {}
+    for k in ROW_KERNELS:
+        ms, cnt = ctx.kernel_time(k)
+        if cnt:
+            out[k] = {"mean_ms": ms, "launches_per_call": cnt / nrep}
+    ctx.profile(False)
+    return out
+
+
+def _hbm_roof(kname, ktimes, nbytes):
+    ms = ktimes[kname]["mean_ms"]
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": int(nbytes),
+            "mean_ms": round(ms, 4)}
+
+
+def rows(cm):
+    """The rows either side of the measurement path, each timed on the GPU with
+    the roofline of its dominant kernel and a CPU leg on a bounded sample:
+    F1 the Zel'dovich generator (256^3 particles), F2 the FKP estimator with a
+    random catalogue (256^3 Zel'dovich data + 4x uniform randoms, TSC 512^3),
+    F4 the shell-transform bispectrum (512^3 mesh, 16 shells of 4 k_f) and the
+    direct bispectrum (64^3 mesh, 32 bins of k_f, all ordered pairs)."""
+    import torch
+    import oracle
+    import oracle.zeldovich as oz
+    res = {}
+    ctx = cm.Context(0)
+    # ---------------------------------------------------------------- F1
+    NLg, Lg = 256, 1000.0
+    buf = torch.empty((NLg ** 3, 3), dtype=torch.float32, device="cuda")
+    gen = lambda: ctx.zeldovich(NLg, Lg, A=1e4, ns=-1.5, seed=1, out=buf)
+    t = _timed(gen, 10)
+    kt = _kernels(ctx, gen)
+    nh = NLg * NLg * (NLg // 2 + 1)
+    ncpu = 64
+    tc = time.perf_counter()
+    oz.zeldovich(ncpu, Lg, seed=1)
+    tcpu = (time.perf_counter() - tc) * (NLg / ncpu) ** 3 * math.log2(NLg) / math.log2(ncpu)
+    res["F1_generator"] = {
+        "workload": "256^3 Zel'dovich particles (fixed-amplitude field P = 1e4 k^-1.5, L = 1000, Philox phases, three c2r)",
+        "value": round(NLg ** 3 / t, 1), "unit": "particles/s", "ms_per_call": round(1e3 * t, 4),
+        "kernels_ms_per_call": {k: round(v["mean_ms"] * v["launches_per_call"], 4) for k, v in kt.items()},
+        "roofline": _hbm_roof("zel_spectra", kt, 24 * nh),
+        "roofline_note": "zel_spectra writes three complex64 spectra (24 B per half-spectrum mode); it is bound by "
+                         "the Philox draws and FP64 trigonometry, not by HBM",
+        "cpu_baseline": {"value": round(NLg ** 3 / tcpu, 1), "unit": "particles/s", "cores": 1, "kind": "port",
+                         "sample": f"numpy restatement (oracle/zeldovich.py) at {ncpu}^3, "
+                                   f"{tcpu * (ncpu / NLg) ** 3 * math.log2(ncpu) / math.log2(NLg):.2f} s, "
+                                   "scaled by N^3 log N"}}
+    del buf
+    # ---------------------------------------------------------------- F2
+    posd = ctx.zeldovich(NL, L, A=1e4, ns=-1.5, seed=1)
+    nr = 4 * NL ** 3
+    rand = ctx.uniform(nr, L, seed=2)
+    b = cm.binning(N, L, ndel=NDEL, DeltaKmin=0.0)
+    nbar = NL ** 3 / L ** 3
+    fkp = lambda: ctx.measure_fkp(posd, rand, N, L, "TSC", b["nshells"], b["kmin"], b["dk"], nbar=nbar,
+                                  P_est=1e4, fkp_weight=True)
+    t = _timed(fkp, 5)
+    kt = _kernels(ctx, fkp)
+    dom = max(kt.items(), key=lambda kv: kv[1]["mean_ms"] * kv[1]["launches_per_call"])[0]
+    n_avg = (NL ** 3 + nr) / 2.0  # the MAS kernels run once on the data and once on the randoms
+    ab = algorithmic_bytes(dom, n_avg, 1.42 * n_avg)
+    # CPU: the oracle's serial MAS, FFT and shells on a 128^3 sample with 1/64 of both catalogues
+    Ns = 128
+    sub = posd[:: 64].cpu().numpy()
+    rsub = rand[:: 64].cpu().numpy()
+    Lsub = L * Ns / N
+    sub = np.mod(sub * np.float32(Ns / N), Lsub).astype(np.float32)
+    rsub = np.mod(rsub * np.float32(Ns / N), Lsub).astype(np.float32)
+    tc = time.perf_counter()
+    m1 = oracle.mas(sub, Ns, Lsub, oracle.TSC)
+    m2 = oracle.mas(rsub, Ns, Lsub, oracle.TSC)
+    dk = oracle.fft_r2c(m1 - m2 * (len(sub) / len(rsub)))
+    o = oracle.pk_opts(Ns, Lsub, Ns // NDEL // 2, 0.0, NDEL * 2 * np.pi / Lsub, oracle.TSC, 1.0, 1.0)
+    oracle.pk_shells(dk, o)
+    tcs = time.perf_counter() - tc
+    tcpu = tcs * max(64.0, (N / Ns) ** 3 * math.log2(N) / math.log2(Ns))
+    res["F2_fkp_randoms"] = {
+        "workload": f"256^3 Zel'dovich data + {nr} uniform randoms, FKP weights (P_est = 1e4), TSC 512^3, "
+                    "128 shells, monopole/quadrupole/hexadecapole",
+        "value": round(1.0 / t, 3), "unit": "measurements/s", "ms_per_call": round(1e3 * t, 4),
+        "kernels_ms_per_call": {k: round(v["mean_ms"] * v["launches_per_call"], 4) for k, v in kt.items()},
+        "roofline": _hbm_roof(dom, kt, ab) if ab else None,
+        "roofline_note": "per launch of the dominant kernel at the mean catalogue size (data and randoms launch once "
+                         "each), 1.42 records per particle for TSC on 16^3 tiles",
+        "cpu_baseline": {"value": round(1.0 / tcpu, 5), "unit": "measurements/s", "cores": 1, "kind": "port",
+                         "sample": f"the oracle's serial MAS of both catalogues, r2c and shells at {Ns}^3 with 1/64 "
+                                   f"of the particles, {tcs:.2f} s, scaled by the larger of the particle (x64) and "
+                                   "the N^3 log N mesh factor"}}
+    # ---------------------------------------------------------------- F4 fast
+    out = ctx.measure(posd, N, L, "TSC", b["nshells"], b["kmin"], b["dk"])
+    mesh = ctx.mesh_assign(posd, N, L, "TSC")
+    nobj = float(out["n_obj"])
+    dkd = ctx.fft_r2c(mesh, subtract=nobj / N ** 3)
+    kf = 2 * np.pi / L
+    nsb = 16
+    bkf = lambda: ctx.bispectrum(dkd, N, L, nsb, 4 * kf, 4 * kf, "TSC", nobj)
+    t = _timed(bkf, 3, warm=1)
+    kt = _kernels(ctx, bkf, nrep=2)
+    r = bkf()
+    res["F4_bispectrum_fast"] = {
+        "workload": f"512^3 TSC mesh of the 256^3 mock, {nsb} shells of 4 k_f from 4 k_f, "
+                    f"{len(r['bk'])} triangle bins, shell transforms",
+        "value": round(1.0 / t, 3), "unit": "bispectra/s", "ms_per_call": round(1e3 * t, 3),
+        "kernels_ms_per_call": {k: round(v["mean_ms"] * v["launches_per_call"], 4) for k, v in kt.items()},
+        "roofline": _hbm_roof("bk_sums", kt, 3 * nsb * N ** 3 * 4),
+        "roofline_note": "bk_sums reads the three float32 fields of every shell once (12 ns N^3 bytes)",
+        "cpu_baseline": None,
+        "cpu_note": "no CPU restatement of the shell-transform estimator; the direct estimator below carries the CPU leg"}
+    del mesh, dkd
+    # ---------------------------------------------------------------- F4 direct
+    Nd, nbd = 64, 32
+    Ld = 1000.0
+    kfd = 2 * np.pi / Ld
+    pd = ctx.zeldovich(64, Ld, A=1e4, ns=-1.5, seed=3)
+    md = ctx.mesh_assign(pd, Nd, Ld, "TSC")
+    nd = float(md.double().sum())
+    dkk = ctx.fft_r2c(md, subtract=nd / Nd ** 3)
+    bkd = lambda: ctx.bispectrum_direct(dkk, Nd, Ld, nbd, kfd, kfd, "TSC", nd)
+    t = _timed(bkd, 3, warm=1)
+    kt = _kernels(ctx, bkd, nrep=2)
+    rd = bkd()
+    pairs = int(rd["ntri"].sum())
+    flops = 11.0 * pairs  # Re(d1 d2 d3): two complex products, real part
+    ach = flops / (kt["bkd_pairs"]["mean_ms"] * 1e-3) / 1e12
+    Nc = 12
+    dkc = oracle.fft_r2c(np.random.default_rng(0).standard_normal((Nc, Nc, Nc)))
+    tc = time.perf_counter()
+    oracle.bk_direct(dkc, 1, 1, Nc)
+    tcs = time.perf_counter() - tc
+    cpu_pairs_s = float((Nc + 1) ** 6) / tcs
+    tcpu = float((Nd + 1) ** 6) / cpu_pairs_s
+    res["F4_bispectrum_direct"] = {
+        "workload": f"64^3 TSC mesh of a 64^3 mock, {nbd} bins of k_f from k_f, {pairs} ordered pairs",
+        "value": round(1.0 / t, 3), "unit": "bispectra/s", "ms_per_call": round(1e3 * t, 3),
+        "pairs_per_s": round(pairs / t, 1),
+        "kernels_ms_per_call": {k: round(v["mean_ms"] * v["launches_per_call"], 4) for k, v in kt.items()},
+        "roofline": {"kernel": "bkd_pairs", "bound": "fp64", "achieved": round(ach, 3), "peak": FP64_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": None,
+                     "mean_ms": round(kt["bkd_pairs"]["mean_ms"], 4)},
+        "roofline_note": "11 FP64 flops per pair; the kernel is bound by the LDS adds of its run-length sums, the "
+                         "table reads hit L1/L2",
+        "cpu_baseline": {"value": round(1.0 / tcpu, 8), "unit": "bispectra/s", "cores": 1, "kind": "port",
+                         "sample": f"the oracle's loop over all (N+1)^6 ordered pairs (the reference's) at N = {Nc}: "
+                                   f"{tcs:.2f} s, {cpu_pairs_s:.3e} pairs/s, scaled to the (N+1)^6 pairs of N = {Nd}"}}
+    ctx.close()
+    return {"metric": "rows either side of the measurement path on one MI355X", "n_gpus": 1, "rows": res}
+
+
 def large(cm, steps=3):
     """Large single mock: 768^3 Zel'dovich particles (P = 1e4 k^-1.5, L = 2000)
     plus 20% uniform randoms (90,596,966), all float32 and generated on the GPU,
@@ -419,6 +596,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy a 512 MiB device buffer once before the steps (reference for PMC byte counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served and batched legs")
+    ap.add_argument("--rows", action="store_true",
+                    help="time the generator, FKP-with-randoms and bispectrum rows instead (one JSON line)")
     ap.add_argument("--large", action="store_true",
                     help="run the large interlaced 1024^3 configuration instead (one JSON line)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -438,6 +617,9 @@ def main():
 
     if args.large:
         print(json.dumps(large(cm)))
+        return
+    if args.rows:
+        print(json.dumps(rows(cm)))
         return
     dev = torch.device("cuda:0")
     ctx = cm.Context(0)

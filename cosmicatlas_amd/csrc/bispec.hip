@@ -68,13 +68,16 @@ __global__ void __launch_bounds__(256) k_bk_masks(const float2 *__restrict__ dk,
 }
 
 constexpr int kBkChunk = 128;
+// LDS row pitch of one field: one word of padding, so that lanes reading the
+// same cell of different shells hit different banks
+constexpr int kBkPitch = kBkChunk + 1;
 
 // Triangle sums over cells.  fields: [3][ns][n3] (D, U, P); tri: (i, j, k) per
 // bin; part[blockIdx][bin][3] = partial sums of DDD, UUU and the cyclic PUU.
 __global__ void __launch_bounds__(256) k_bk_sums(const float *__restrict__ fields, int ns, size_t n3,
                                                  const int4 *__restrict__ tri, int nbins,
                                                  double *__restrict__ part) {
-    extern __shared__ float sf[];  // [3][ns][kBkChunk]
+    extern __shared__ float sf[];  // [3][ns][kBkPitch]
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
     // each thread owns bins tid, tid + NT, ... (at most 8 per thread here)
@@ -88,7 +91,7 @@ __global__ void __launch_bounds__(256) k_bk_sums(const float *__restrict__ field
         __syncthreads();
         for (int t = tid; t < 3 * ns * kBkChunk; t += NT) {
             const int f = t / kBkChunk, xi = t % kBkChunk;
-            sf[t] = xi < len ? fields[(size_t)f * n3 + x0 + xi] : 0.f;
+            sf[f * kBkPitch + xi] = xi < len ? fields[(size_t)f * n3 + x0 + xi] : 0.f;
         }
         __syncthreads();
 #pragma unroll
@@ -96,11 +99,11 @@ __global__ void __launch_bounds__(256) k_bk_sums(const float *__restrict__ field
             const int b = tid + a * NT;
             if (b >= nbins) break;
             const int4 t = tri[b];
-            const float *Di = sf + t.x * kBkChunk, *Dj = sf + t.y * kBkChunk, *Dk = sf + t.z * kBkChunk;
-            const float *Ui = sf + (ns + t.x) * kBkChunk, *Uj = sf + (ns + t.y) * kBkChunk,
-                        *Uk = sf + (ns + t.z) * kBkChunk;
-            const float *Pi = sf + (2 * ns + t.x) * kBkChunk, *Pj = sf + (2 * ns + t.y) * kBkChunk,
-                        *Pk = sf + (2 * ns + t.z) * kBkChunk;
+            const float *Di = sf + t.x * kBkPitch, *Dj = sf + t.y * kBkPitch, *Dk = sf + t.z * kBkPitch;
+            const float *Ui = sf + (ns + t.x) * kBkPitch, *Uj = sf + (ns + t.y) * kBkPitch,
+                        *Uk = sf + (ns + t.z) * kBkPitch;
+            const float *Pi = sf + (2 * ns + t.x) * kBkPitch, *Pj = sf + (2 * ns + t.y) * kBkPitch,
+                        *Pk = sf + (2 * ns + t.z) * kBkPitch;
             float s0 = 0.f, s1 = 0.f, s2 = 0.f;
             double d0 = 0.0, d1 = 0.0, d2 = 0.0;
             for (int xi = 0; xi < kBkChunk; ++xi) {
@@ -175,7 +178,7 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
     if (max_bins < nb || !ijk || !bk || !sn || !ntri)
         return fail(ctx, CM_ERR_INVALID, "cm_bispectrum: output arrays hold %d bins, %d needed", max_bins, nb);
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * (N / 2 + 1);
-    const size_t lds = sizeof(float) * 3 * (size_t)ns * kBkChunk;
+    const size_t lds = sizeof(float) * 3 * (size_t)ns * kBkPitch;
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum: %d shells need %zu B LDS", ns, lds);
     cm_status s;
     void *fields, *specs, *dtri, *part, *dsum;
