@@ -73,6 +73,7 @@ struct cm_ctx {
         int mas_tile = 0;              // cells per tile edge in the deposit
         int dep_wg = 0;                // deposit workgroups per CU
         int mas_hist = 0;              // tile counters per binning workgroup (0: its LDS capacity)
+        int mas_groups = 0;            // at least this many x-slab groups in the tiled deposit (0: as needed)
     } tune;
 };
 

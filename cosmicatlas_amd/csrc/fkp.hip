@@ -99,6 +99,19 @@ static cm_status shells_of_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mes
     return CM_OK;
 }
 
+// A random catalogue has no spatial order: its particles are deposited through
+// (at least) four x-slab groups, so that the binning records of one group are
+// written while that group's part of the record buffer stays in the caches
+// (one host synchronisation for the group sizes).
+static cm_status assign_randoms(cm_ctx *ctx, const cm_mas_desc *md, const void *rpos, const float *wr, size_t nr,
+                                void *mesh, int accumulate) {
+    const int saved = ctx->tune.mas_groups;
+    ctx->tune.mas_groups = 4;
+    const cm_status s = mesh_assign_impl(ctx, md, rpos, wr, nr, mesh, CM_F32, accumulate);
+    ctx->tune.mas_groups = saved;
+    return s;
+}
+
 static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const cm_fkp_opts *f, const void *pos,
                                   const float *w, size_t n, const void *rpos, const float *rw, size_t nr,
                                   double *pk0, double *pk2, double *pk4, int64_t *nmodes, double *pk_window,
@@ -129,7 +142,7 @@ static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const c
     if ((s = scaled_weights(ctx, "fkp.wg", w, n, wf, &wg)) != CM_OK) return s;
     if ((s = mesh_assign_impl(ctx, &md, pos, wg, n, mesh, CM_F32, 0)) != CM_OK) return s;
     if ((s = scaled_weights(ctx, "fkp.wr", rw, nr, -alpha * wf, &wr)) != CM_OK) return s;
-    if ((s = mesh_assign_impl(ctx, &md, rpos, wr, nr, mesh, CM_F32, 1)) != CM_OK) return s;
+    if ((s = assign_randoms(ctx, &md, rpos, wr, nr, mesh, 1)) != CM_OK) return s;
     std::vector<double> hs;
     std::vector<unsigned long long> hc;
     if ((s = shells_of_mesh(ctx, o, mesh, 0.0, hs, hc)) != CM_OK) return s;
@@ -144,7 +157,7 @@ static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const c
     if (f->window && pk_window) {
         const float *wr1;
         if ((s = scaled_weights(ctx, "fkp.wr", rw, nr, wf, &wr1)) != CM_OK) return s;
-        if ((s = mesh_assign_impl(ctx, &md, rpos, wr1, nr, mesh, CM_F32, 0)) != CM_OK) return s;
+        if ((s = assign_randoms(ctx, &md, rpos, wr1, nr, mesh, 0)) != CM_OK) return s;
         if ((s = shells_of_mesh(ctx, o, mesh, Wr / (double)n3, hs, hc)) != CM_OK) return s;
         std::vector<double> q2((size_t)o->nshells), q4((size_t)o->nshells);
         std::vector<int64_t> nm((size_t)o->nshells);

@@ -904,7 +904,10 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
 
 // Tiled deposit of the whole mesh: one pass when one workgroup's LDS holds a
 // counter per tile, else x-slab groups (level-1 binning, one host
-// synchronisation for the group sizes, exact record sizing per group).
+// synchronisation for the group sizes, exact record sizing per group).  A
+// caller may ask for groups on smaller meshes too (tune.mas_groups): for
+// particles in no spatial order the records of one group are then written
+// while that group's part of the record buffer stays in the caches.
 template <int T>
 static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
                              int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
@@ -912,6 +915,8 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     const size_t plane_tiles = (size_t)ntx * ntx;
     size_t per_lds = ctx->lds_per_block / sizeof(unsigned);
     if (ctx->tune.mas_hist > 0) per_lds = std::min(per_lds, (size_t)ctx->tune.mas_hist);
+    if (ctx->tune.mas_groups > 1 && ntx > 1)  // caller asks for x-slab groups (unordered inputs)
+        per_lds = std::min(per_lds, (size_t)((ntx + ctx->tune.mas_groups - 1) / ctx->tune.mas_groups) * plane_tiles);
     if ((size_t)ntx * plane_tiles <= per_lds || plane_tiles > per_lds)
         return mas_tiled_range<T>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow, 0, ntx);
     const int gs = (int)std::min<size_t>((size_t)ntx, per_lds / plane_tiles);  // tile planes per group
