@@ -902,6 +902,8 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
     return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
 }
 
+constexpr size_t kGroupTiles = 12288;
+
 // Tiled deposit of the whole mesh: one pass when one workgroup's LDS holds a
 // counter per tile, else x-slab groups (level-1 binning, one host
 // synchronisation for the group sizes, exact record sizing per group).  A
@@ -919,6 +921,10 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
         per_lds = std::min(per_lds, (size_t)((ntx + ctx->tune.mas_groups - 1) / ctx->tune.mas_groups) * plane_tiles);
     if ((size_t)ntx * plane_tiles <= per_lds || plane_tiles > per_lds)
         return mas_tiled_range<T>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow, 0, ntx);
+    // once grouped, groups of about kGroupTiles tiles: the record region of a group
+    // then stays cache-resident while it is written (1024^3 interlaced mock with
+    // 20% randoms: 7 groups 81.9 ms, 11: 79.1, 16: 77.3, 22: 76.3, 32: 76.5)
+    per_lds = std::min(per_lds, std::max(plane_tiles, kGroupTiles));
     const int gs = (int)std::min<size_t>((size_t)ntx, per_lds / plane_tiles);  // tile planes per group
     const int G = (ntx + gs - 1) / gs;
     if (G > 64) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %d x-slab groups", G);
