@@ -162,7 +162,7 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w,
                            size_t n, void *mesh, cm_dtype mdt, int accumulate, bool exact = true,
                            unsigned **overflow_out = nullptr, bool reset_overflow = true,
-                           unsigned *overflow_at = nullptr);
+                           unsigned *overflow_at = nullptr, void *meshB = nullptr);
 cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
                         double *host_out);
 

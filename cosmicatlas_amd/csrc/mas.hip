@@ -82,6 +82,8 @@ __global__ void k_fill(T *__restrict__ a, size_t n, T v) {
 struct TileK {
     int T;      // tile edge in cells
     int R;      // stencil radius
+    int Rup;    // ghost cells taken from the tile below: R, or R + 1 when the
+                // records also feed the half-cell-shifted (interlaced) mesh
     int ntx;    // tiles per side
     int CH;     // max records per deposit workgroup
     int tx0;    // first tile along x of the binned range
@@ -92,15 +94,18 @@ constexpr int kTileCH = 4096;
 
 // Tiles touched by a particle in cell c along one axis: home tile and, when the
 // stencil crosses the tile edge, the neighbour.  Returns the count (1 or 2).
+// The stencil spans cells c-R..c+R; shifted by half a cell (the interlaced
+// mesh) it spans c-R..c+R+1, so records shared by both meshes go to the tile
+// above from Rup = R+1 cells below its edge.
 template <int T>
-__device__ __forceinline__ int axis_tiles(int c, int R, int ntx, int out[2]) {
+__device__ __forceinline__ int axis_tiles(int c, int R, int Rup, int ntx, int out[2]) {
     const int home = c / T, lc = c % T;
     out[0] = home;
     if (lc < R) {
         out[1] = home == 0 ? ntx - 1 : home - 1;
         return 2;
     }
-    if (lc >= T - R) {
+    if (lc >= T - Rup) {
         out[1] = home + 1 == ntx ? 0 : home + 1;
         return 2;
     }
@@ -145,15 +150,16 @@ __device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int 
 }
 
 // Sorted record: one 32-bit word per axis.  Bits 27..31 hold the cell's
-// coordinate in the destination tile plus 2 (the tile's own cells are 0..T-1,
-// ghosts from neighbours -R..T+R-1, R <= 2); bits 0..26 the fraction x/d - c.
-// The periodic wrap matches k_tile_deposit's: a coordinate below -R or at
-// least N-R is moved by one period.
-__device__ __forceinline__ unsigned record_word(int c, int o, unsigned fr, int N, int R) {
+// coordinate in the destination tile plus kLcBias (the tile's own cells are
+// 0..T-1, ghosts from neighbours -Rup..T+R-1, R <= 2, Rup <= R+1); bits 0..26
+// the fraction x/d - c.  A coordinate below -Rup or at least N-Rup is moved by
+// one period.
+constexpr int kLcBias = 3;
+__device__ __forceinline__ unsigned record_word(int c, int o, unsigned fr, int N, int Rup) {
     int lc = c - o;
-    if (lc < -R) lc += N;
-    if (lc >= N - R) lc -= N;
-    return ((unsigned)(lc + 2) << kFracBits) | fr;
+    if (lc < -Rup) lc += N;
+    if (lc >= N - Rup) lc -= N;
+    return ((unsigned)(lc + kLcBias) << kFracBits) | fr;
 }
 
 // In-cell offset f = x/d - c - 1/2 from a record word (midpoint of the 2^-27
@@ -234,8 +240,9 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
                 continue;
             }
             int tx[2], ty[2], tz[2];
-            const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
-                      nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
+            const int nx = axis_tiles<T>(c[0], t.R, t.Rup, t.ntx, tx),
+                      ny = axis_tiles<T>(c[1], t.R, t.Rup, t.ntx, ty),
+                      nz = axis_tiles<T>(c[2], t.R, t.Rup, t.ntx, tz);
             for (int a = 0; a < nx; ++a) {
                 const int lx = tx[a] - t.tx0;  // tiles outside the x range belong to another pass
                 if ((unsigned)lx >= (unsigned)t.nxl) continue;
@@ -426,18 +433,19 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
             unsigned f[3];
             if (!cells_from(v[u], p, c, f)) continue;
             int tx[2], ty[2], tz[2];
-            const int nx = axis_tiles<T>(c[0], t.R, t.ntx, tx), ny = axis_tiles<T>(c[1], t.R, t.ntx, ty),
-                      nz = axis_tiles<T>(c[2], t.R, t.ntx, tz);
+            const int nx = axis_tiles<T>(c[0], t.R, t.Rup, t.ntx, tx),
+                      ny = axis_tiles<T>(c[1], t.R, t.Rup, t.ntx, ty),
+                      nz = axis_tiles<T>(c[2], t.R, t.Rup, t.ntx, tz);
             for (int a = 0; a < nx; ++a) {
                 const int lx = tx[a] - t.tx0;
                 if ((unsigned)lx >= (unsigned)t.nxl) continue;
-                const unsigned wx = record_word(c[0], tx[a] * T, f[0], p.N, t.R);
+                const unsigned wx = record_word(c[0], tx[a] * T, f[0], p.N, t.Rup);
                 for (int b = 0; b < ny; ++b) {
-                    const unsigned wy = record_word(c[1], ty[b] * T, f[1], p.N, t.R);
+                    const unsigned wy = record_word(c[1], ty[b] * T, f[1], p.N, t.Rup);
                     for (int e = 0; e < nz; ++e) {
                         const unsigned slot = atomicAdd(&cu[(lx * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
                         if (slot < cap) {
-                            sorted[slot] = make_uint3(wx, wy, record_word(c[2], tz[e] * T, f[2], p.N, t.R));
+                            sorted[slot] = make_uint3(wx, wy, record_word(c[2], tz[e] * T, f[2], p.N, t.Rup));
                             if (w) wsorted[slot] = wv[u];
                         } else {
                             ovf = 1u;
@@ -460,14 +468,14 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
 // group.  Raw positions are copied (AoS float32), so the second level computes
 // the same cells.
 template <int T>
-__device__ __forceinline__ int particle_groups(int cx, int R, int ntx, int gs, int G, int out[2]) {
+__device__ __forceinline__ int particle_groups(int cx, int R, int Rup, int ntx, int gs, int G, int out[2]) {
     const int home = cx / T, lc = cx % T, g = home / gs;
     out[0] = g;
     if (G > 1 && lc < R && home % gs == 0) {
         out[1] = g == 0 ? G - 1 : g - 1;
         return 2;
     }
-    if (G > 1 && lc >= T - R && (home % gs == gs - 1 || home == ntx - 1)) {
+    if (G > 1 && lc >= T - Rup && (home % gs == gs - 1 || home == ntx - 1)) {
         out[1] = g + 1 == G ? 0 : g + 1;
         return 2;
     }
@@ -475,7 +483,8 @@ __device__ __forceinline__ int particle_groups(int cx, int R, int ntx, int gs, i
 }
 
 template <int T, bool SOA>
-__global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ pos, MasK p, int R, int ntx, int gs,
+__global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ pos, MasK p, int R, int Rup, int ntx,
+                                                      int gs,
                                                       int G, size_t chunk, unsigned *__restrict__ hist,
                                                       unsigned long long *dropped) {
     __shared__ unsigned lh[64];
@@ -496,7 +505,7 @@ __global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ 
                 continue;
             }
             int g[2];
-            const int ng = particle_groups<T>(c[0], R, ntx, gs, G, g);
+            const int ng = particle_groups<T>(c[0], R, Rup, ntx, gs, G, g);
             for (int k = 0; k < ng; ++k) atomicAdd(&lh[g[k]], 1u);
         }
     }
@@ -532,7 +541,7 @@ __global__ void __launch_bounds__(64) k_group_scan(unsigned *__restrict__ hist, 
 
 template <int T, bool SOA>
 __global__ void __launch_bounds__(1024) k_group_scatter(const float *__restrict__ pos, const float *__restrict__ w,
-                                                        MasK p, int R, int ntx, int gs, int G, size_t chunk,
+                                                        MasK p, int R, int Rup, int ntx, int gs, int G, size_t chunk,
                                                         const unsigned *__restrict__ hist,
                                                         const unsigned long long *__restrict__ gstart,
                                                         float3 *__restrict__ out, float *__restrict__ wout) {
@@ -551,7 +560,7 @@ __global__ void __launch_bounds__(1024) k_group_scatter(const float *__restrict_
             unsigned f[3];
             if (!cells_from(v[u], p, c, f)) continue;
             int g[2];
-            const int ng = particle_groups<T>(c[0], R, ntx, gs, G, g);
+            const int ng = particle_groups<T>(c[0], R, Rup, ntx, gs, G, g);
             const float wi = w ? w[i] : 0.0f;
             for (int k = 0; k < ng; ++k) {
                 const unsigned long long slot = atomicAdd(&cur[g[k]], 1ull);
@@ -639,7 +648,7 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
                                                       const float *__restrict__ wsorted,
                                                       const uint4 *__restrict__ items,
                                                       const unsigned *__restrict__ nitems, int N, int ntx, int tx0,
-                                                      float *__restrict__ mesh, int accumulate) {
+                                                      float *__restrict__ mesh, int accumulate, int half) {
     extern __shared__ long long lgrid_raw[];
     void *lgrid = lgrid_raw;
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
@@ -678,12 +687,24 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
         for (int r = 0; r < RPT; ++r) {
             const unsigned idx = it.y + (unsigned)(tid + r * NT);
             if (idx >= it.z) continue;
-            const unsigned wd[3] = {rec[r].x, rec[r].y, rec[r].z};
+            unsigned wd[3] = {rec[r].x, rec[r].y, rec[r].z};
+            bool inside = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                // half a cell up: fraction + 1/2; a fraction past 1 carries into
+                // the cell field above it (bits 27..31 stay below 32 - kLcBias)
+                if (half) wd[a] += 1u << (kFracBits - 1);
+                // a record whose stencil misses the tile's cells (the extra ghosts
+                // kept for the other mesh) is skipped
+                const int lc = (int)(wd[a] >> kFracBits) - kLcBias;
+                inside = inside && lc >= -R && lc < T + R;
+            }
+            if (!inside) continue;
             const float f[3] = {record_offset(wd[0]), record_offset(wd[1]), record_offset(wd[2])};
             int l[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a)  // node c-R (local lc-R) sits at LDS index lc-R+2R = lc+R
-                l[a] = (int)(wd[a] >> kFracBits) - 2 + R;
+                l[a] = (int)(wd[a] >> kFracBits) - kLcBias + R;
             deposit_record<SCHEME, UNIT, S>(lgrid, (l[0] * S + l[1]) * S + l[2], f, wr[r], qs);
         }
         const bool split = it.w != 0u;
@@ -782,13 +803,16 @@ static int tile_edge(int N) {
     return 0;
 }
 
+// meshB (nullable): the half-cell-shifted mesh of the same particles, deposited
+// from the same records (binned with one more ghost cell towards the tile above).
 template <int T>
 static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
-                                 int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow,
-                                 int tx0, int nxl) {
+                                 float *meshB, int accumulate, unsigned long long *dropped, bool exact,
+                                 unsigned *overflow, int tx0, int nxl) {
     TileK t;
     t.T = T;
     t.R = mas_radius(p.scheme);
+    t.Rup = t.R + (meshB ? 1 : 0);
     t.ntx = p.N / T;
     t.CH = kTileCH;
     t.tx0 = tx0;
@@ -848,7 +872,7 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
     } else {
         // geometric ghost fraction (1 + 2R/T)^3 with 50% headroom; a larger
         // count sets *overflow and the caller repeats with exact = true
-        const double g1 = 1.0 + 2.0 * t.R / T;
+        const double g1 = 1.0 + (double)(t.R + t.Rup) / T;
         cap = std::min(std::min(max_copies, max_cap), (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
     }
     if ((s = scratch(ctx, "mas.sorted", sizeof(uint3) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
@@ -874,9 +898,11 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
             return s;
     }
     if (!accumulate) {
-        if ((s = launch(ctx, "mas_tile_zero_split", k_tile_zero_split<T>, dim3(ntiles), dim3(256), 0,
-                        (const unsigned *)offsets, t, p.N, mesh)) != CM_OK)
-            return s;
+        for (float *m : {mesh, meshB}) {
+            if (m && (s = launch(ctx, "mas_tile_zero_split", k_tile_zero_split<T>, dim3(ntiles), dim3(256), 0,
+                                 (const unsigned *)offsets, t, p.N, m)) != CM_OK)
+                return s;
+        }
     }
     const int S = T + 4 * t.R;
     const size_t lds = (w ? sizeof(long long) : sizeof(unsigned)) * (size_t)S * S * S;
@@ -887,19 +913,24 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
     const dim3 gd(dg), bd(512);
 #define CM_DEPOSIT(SCH, UNIT)                                                                                    \
     launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const uint3 *)sorted, ws,     \
-           (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, t.tx0, mesh, accumulate)
-    switch (p.scheme * 2 + (w ? 0 : 1)) {
-    case 0: return CM_DEPOSIT(0, false);
-    case 1: return CM_DEPOSIT(0, true);
-    case 2: return CM_DEPOSIT(1, false);
-    case 3: return CM_DEPOSIT(1, true);
-    case 4: return CM_DEPOSIT(2, false);
-    case 5: return CM_DEPOSIT(2, true);
-    case 6: return CM_DEPOSIT(3, false);
-    case 7: return CM_DEPOSIT(3, true);
+           (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, t.tx0, dst, accumulate, half)
+    for (int half = 0; half < (meshB ? 2 : 1); ++half) {
+        float *dst = half ? meshB : mesh;
+        switch (p.scheme * 2 + (w ? 0 : 1)) {
+        case 0: s = CM_DEPOSIT(0, false); break;
+        case 1: s = CM_DEPOSIT(0, true); break;
+        case 2: s = CM_DEPOSIT(1, false); break;
+        case 3: s = CM_DEPOSIT(1, true); break;
+        case 4: s = CM_DEPOSIT(2, false); break;
+        case 5: s = CM_DEPOSIT(2, true); break;
+        case 6: s = CM_DEPOSIT(3, false); break;
+        case 7: s = CM_DEPOSIT(3, true); break;
+        default: return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
+        }
+        if (s != CM_OK) return s;
     }
 #undef CM_DEPOSIT
-    return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
+    return CM_OK;
 }
 
 constexpr size_t kGroupTiles = 12288;
@@ -912,15 +943,16 @@ constexpr size_t kGroupTiles = 12288;
 // while that group's part of the record buffer stays in the caches.
 template <int T>
 static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
-                             int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
-    const int ntx = p.N / T, R = mas_radius(p.scheme);
+                             float *meshB, int accumulate, unsigned long long *dropped, bool exact,
+                             unsigned *overflow) {
+    const int ntx = p.N / T, R = mas_radius(p.scheme), Rup = R + (meshB ? 1 : 0);
     const size_t plane_tiles = (size_t)ntx * ntx;
     size_t per_lds = ctx->lds_per_block / sizeof(unsigned);
     if (ctx->tune.mas_hist > 0) per_lds = std::min(per_lds, (size_t)ctx->tune.mas_hist);
     if (ctx->tune.mas_groups > 1 && ntx > 1)  // caller asks for x-slab groups (unordered inputs)
         per_lds = std::min(per_lds, (size_t)((ntx + ctx->tune.mas_groups - 1) / ctx->tune.mas_groups) * plane_tiles);
     if ((size_t)ntx * plane_tiles <= per_lds || plane_tiles > per_lds)
-        return mas_tiled_range<T>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow, 0, ntx);
+        return mas_tiled_range<T>(ctx, p, pos, w, mesh, meshB, accumulate, dropped, exact, overflow, 0, ntx);
     // once grouped, groups of about kGroupTiles tiles: the record region of a group
     // then stays cache-resident while it is written (1024^3 interlaced mock with
     // 20% randoms: 7 groups 81.9 ms, 11: 79.1, 16: 77.3, 22: 76.3, 32: 76.5)
@@ -935,7 +967,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     if ((s = scratch(ctx, "mas.ghist", sizeof(unsigned) * (size_t)nrows * G, &hist)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.gstart", sizeof(unsigned long long) * (size_t)(G + 1), &gstart)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_group_count", p.soa ? k_group_count<T, true> : k_group_count<T, false>, dim3(nrows),
-                    dim3(1024), 0, pos, p, R, ntx, gs, G, chunk, (unsigned *)hist, dropped)) != CM_OK)
+                    dim3(1024), 0, pos, p, R, Rup, ntx, gs, G, chunk, (unsigned *)hist, dropped)) != CM_OK)
         return s;
     if ((s = launch(ctx, "mas_group_scan", k_group_scan, dim3(1), dim3(64), 0, (unsigned *)hist, nrows, G,
                     (unsigned long long *)gstart)) != CM_OK)
@@ -951,7 +983,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     if ((s = scratch(ctx, "mas.gpos", sizeof(float3) * (size_t)(total ? total : 1), &gpos)) != CM_OK) return s;
     if (w && (s = scratch(ctx, "mas.gw", sizeof(float) * (size_t)(total ? total : 1), &gw)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_group_scatter", p.soa ? k_group_scatter<T, true> : k_group_scatter<T, false>,
-                    dim3(nrows), dim3(1024), 0, pos, w, p, R, ntx, gs, G, chunk, (const unsigned *)hist,
+                    dim3(nrows), dim3(1024), 0, pos, w, p, R, Rup, ntx, gs, G, chunk, (const unsigned *)hist,
                     (const unsigned long long *)gstart, (float3 *)gpos, (float *)gw)) != CM_OK)
         return s;
     for (int g = 0; g < G; ++g) {
@@ -960,7 +992,7 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
         q.n = (size_t)(gs_h[(size_t)g + 1] - gs_h[(size_t)g]);
         const float *qp = (const float *)gpos + 3 * (size_t)gs_h[(size_t)g];
         const float *qw = gw ? (const float *)gw + (size_t)gs_h[(size_t)g] : nullptr;
-        if ((s = mas_tiled_range<T>(ctx, q, qp, qw, mesh, accumulate, dropped, true, overflow, g * gs,
+        if ((s = mas_tiled_range<T>(ctx, q, qp, qw, mesh, meshB, accumulate, dropped, true, overflow, g * gs,
                                     std::min(gs, ntx - g * gs))) != CM_OK)
             return s;
     }
@@ -972,17 +1004,18 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
 // sized to it; otherwise a generous estimate is used and the scatter reports
 // overflow through `overflow` (checked by the caller).
 static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
-                           int accumulate, unsigned long long *dropped, bool exact, unsigned *overflow) {
+                           float *meshB, int accumulate, unsigned long long *dropped, bool exact,
+                           unsigned *overflow) {
     switch (tile_edge(p.N)) {
-    case 16: return mas_tiled_t<16>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow);
-    case 8: return mas_tiled_t<8>(ctx, p, pos, w, mesh, accumulate, dropped, exact, overflow);
+    case 16: return mas_tiled_t<16>(ctx, p, pos, w, mesh, meshB, accumulate, dropped, exact, overflow);
+    case 8: return mas_tiled_t<8>(ctx, p, pos, w, mesh, meshB, accumulate, dropped, exact, overflow);
     }
     return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: N=%d", p.N);
 }
 
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w, size_t n,
                            void *mesh, cm_dtype mdt, int accumulate, bool exact, unsigned **overflow_out,
-                           bool reset_overflow, unsigned *overflow_at) {
+                           bool reset_overflow, unsigned *overflow_at, void *meshB) {
     if (overflow_out) *overflow_out = nullptr;
     MasK p = make_mask(d, n);
     const size_t n3 = (size_t)d->N * d->N * d->N;
@@ -999,8 +1032,15 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
         if (!ovf && (s = scratch(ctx, "mas.overflow", sizeof(unsigned), &ovf)) != CM_OK) return s;
         if (reset_overflow && (s = fill(ctx, (unsigned *)ovf, 1, 0u)) != CM_OK) return s;
         if (overflow_out) *overflow_out = (unsigned *)ovf;
-        return mas_tiled(ctx, p, (const float *)pos, (const float *)w, (float *)mesh, accumulate,
+        return mas_tiled(ctx, p, (const float *)pos, (const float *)w, (float *)mesh, (float *)meshB, accumulate,
                          (unsigned long long *)dropped, exact || !overflow_out, (unsigned *)ovf);
+    }
+    if (meshB) {  // plain path: the shifted mesh is a second deposit
+        cm_mas_desc db = *d;
+        db.interlace_shift += 0.5;
+        if ((s = mesh_assign_impl(ctx, &db, pos, w, n, meshB, mdt, accumulate, exact, nullptr, false, overflow_at)) !=
+            CM_OK)
+            return s;
     }
     if (!accumulate) {
         s = mdt == CM_F32 ? fill(ctx, (float *)mesh, n3, 0.0f) : fill(ctx, (double *)mesh, n3, 0.0);

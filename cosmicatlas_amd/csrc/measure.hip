@@ -96,14 +96,11 @@ static cm_status enqueue_fused(cm_ctx *ctx, const cm_measure_opts *o, const void
     cm_status s;
     if ((s = fused_out(ctx, o->nshells, &fo)) != CM_OK) return s;
     unsigned *ovf = nullptr;
-    if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, exact, &ovf, true, fo.overflow)) != CM_OK)
+    // with interlacing, meshB (half a cell on every axis) is deposited from the
+    // same binning records
+    if ((s = mesh_assign_impl(ctx, &md, pos, w, n, mesh, CM_F32, 0, exact, &ovf, true, fo.overflow, meshB)) !=
+        CM_OK)
         return s;
-    if (meshB) {
-        cm_mas_desc mb = md;
-        mb.interlace_shift += 0.5;  // half a cell on every axis
-        if ((s = mesh_assign_impl(ctx, &mb, pos, w, n, meshB, CM_F32, 0, exact, &ovf, false, fo.overflow)) != CM_OK)
-            return s;
-    }
     return fused_measure_mesh(ctx, o, mesh, meshB, (double)n / n3, labels, dev);
 }
 

@@ -128,18 +128,10 @@ def _alias_tables(N, scheme):
     return C, A
 
 
-@pytest.mark.parametrize("scheme,do_tweb", [(1, False), (2, True), (0, False)])
-def test_measure_interlaced_matches_oracle(ctx, orc, scheme, do_tweb):
-    """Interlaced meshes (no reference counterpart): the average of the plain
-    spectrum and the half-cell-shifted one times exp(i k.(d/2)(1,1,1)); aliased
+def _interlaced_ref(orc, pos, N, L, scheme, b):
+    """Oracle spectrum of interlaced meshes: the average of the plain spectrum
+    and the half-cell-shifted one times exp(i k.(d/2)(1,1,1)), minus the aliased
     shot noise (prod C1 + prod A)/2 per mode."""
-    import cosmicatlas_amd as cm
-    from oracle import zeldovich as zref
-    N, L = 32, 200.0
-    pos = zref.zeldovich(24, L, A=2e3, seed=21)
-    b = cm.binning(N, L)
-    out = ctx.measure(torch.from_numpy(pos).cuda(), N, L, scheme, b["nshells"], b["kmin"], b["dk"],
-                      interlace=True, do_tweb=do_tweb, R_smooth=12.0, margin_tol=1e-4)
     mA = orc.mas(pos, N, L, scheme)
     mB = orc.mas(pos, N, L, scheme, interlace_shift=0.5)
     nobj = mA.sum()
@@ -160,7 +152,22 @@ def test_measure_interlaced_matches_oracle(ctx, orc, scheme, do_tweb):
                + Ax[:, None, None] * Ax[None, :, None] * A[None, None, :])
     sn_shell = orc.pk_shells(np.sqrt(X), orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, 0.0, 1.0))[0]
     SN = V / nobj
-    ref0 = p0 - SN * sn_shell
+    return p0 - SN * sn_shell, p2, nm, SN, d, nobj, C, A
+
+
+@pytest.mark.parametrize("scheme,do_tweb", [(1, False), (2, True), (0, False), (3, False)])
+def test_measure_interlaced_matches_oracle(ctx, orc, scheme, do_tweb):
+    """Interlaced meshes (no reference counterpart), both deposited from one
+    binning: the records carry one more ghost cell towards the tile above, and
+    the second mesh moves each record's fraction by half a cell."""
+    import cosmicatlas_amd as cm
+    from oracle import zeldovich as zref
+    N, L = 32, 200.0
+    pos = zref.zeldovich(24, L, A=2e3, seed=21)
+    b = cm.binning(N, L)
+    out = ctx.measure(torch.from_numpy(pos).cuda(), N, L, scheme, b["nshells"], b["kmin"], b["dk"],
+                      interlace=True, do_tweb=do_tweb, R_smooth=12.0, margin_tol=1e-4)
+    ref0, p2, nm, SN, d, nobj, C, A = _interlaced_ref(orc, pos, N, L, scheme, b)
     assert np.array_equal(out["nmodes"], nm)
     denom = np.abs(ref0) + SN
     assert np.max(np.abs(out["pk0"] - ref0) / denom) < 1e-4
@@ -174,3 +181,34 @@ def test_measure_interlaced_matches_oracle(ctx, orc, scheme, do_tweb):
         lab_ref, lam_ref, _, _ = orc.tweb(delta, L, 12.0, 0.0)
         ok = np.abs(lam_ref).min(axis=0) > 1e-4 * np.abs(lam_ref).max()
         assert np.array_equal(out["labels"].cpu().numpy()[ok], lab_ref[ok])
+
+
+@pytest.mark.parametrize("scheme,hist", [(2, 32), (1, 48), (3, 40), (0, 16)])
+def test_measure_interlaced_grouped_binning(orc, scheme, hist):
+    """The shared binning of the interlaced meshes through x-slab groups (forced
+    on a small mesh by a small histogram budget): copies across group edges
+    include the extra ghost cell of the shifted mesh, with the periodic wrap."""
+    import os
+    import cosmicatlas_amd as cm
+    old = os.environ.get("CM_TUNE")
+    os.environ["CM_TUNE"] = f"mas_hist={hist}"
+    try:
+        c = cm.Context(0)
+    finally:
+        if old is None:
+            del os.environ["CM_TUNE"]
+        else:
+            os.environ["CM_TUNE"] = old
+    try:
+        N, L, n = 64, 300.0, 120000
+        rng = np.random.default_rng(scheme + hist)
+        pos = rng.uniform(0.0, L, (n, 3)).astype(np.float32)
+        b = cm.binning(N, L)
+        out = c.measure(torch.from_numpy(pos).cuda(), N, L, scheme, b["nshells"], b["kmin"], b["dk"], interlace=True)
+    finally:
+        c.close()
+    ref0, p2, nm, SN, _, _, _, _ = _interlaced_ref(orc, pos, N, L, scheme, b)
+    assert np.array_equal(out["nmodes"], nm)
+    denom = np.abs(ref0) + SN
+    assert np.max(np.abs(out["pk0"] - ref0) / denom) < 1e-4
+    assert np.max(np.abs(out["pk2"] - p2) / denom) < 1e-4
