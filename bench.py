@@ -57,7 +57,7 @@ def algorithmic_bytes(name, n_part, n_rec):
     return table.get(name)
 
 
-KERNELS = ["mas_tile_count", "mas_tile_prefix", "mas_tile_scan", "mas_tile_items", "mas_tile_scatter", "mas_tile_zero_split",
+KERNELS = ["mas_group_count", "mas_group_scan", "mas_group_scatter", "mas_tile_count", "mas_tile_prefix", "mas_tile_scan", "mas_tile_items", "mas_tile_scatter", "mas_tile_zero_split",
            "mas_tile_deposit", "mas_plain", "fill", "mesh_sum", "mesh_sum_final", "fft_z_r2c", "fused_sum",
            "fft_y_fwd", "fused_zero", "fused_x", "fft_x_fwd", "pk_zero", "pk_shells", "tweb_kmul", "fft_x_inv",
            "fft_y_inv", "fft_z_c2r", "tweb_grad", "tweb_zero", "tweb_hess_eig", "tweb_fused", "tweb_rare"]
@@ -532,6 +532,7 @@ def large(cm, steps=3):
         out = ctx.measure(pos, Nl, Ll, "TSC", bl["nshells"], bl["kmin"], bl["dk"], **kw)
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / steps
+    kt = _kernels(ctx, lambda: ctx.measure(pos, Nl, Ll, "TSC", bl["nshells"], bl["kmin"], bl["dk"], **kw), nrep=1)
     raw32 = ctx.measure(pos, Nl, Ll, "TSC", bl["nshells"], bl["kmin"], bl["dk"], sn_correction=False, **kw)
     # FP64: both meshes, FFTs and the interlaced combination in double
     nobj = float(n1 + n2)
@@ -563,7 +564,8 @@ def large(cm, steps=3):
                                    "generated on the GPU; interlaced TSC 1024^3; 256 shells"},
             "pk_rel_err_fp32_vs_fp64_max": float(err.max()), "pk_rel_err_fp32_vs_fp64_median": float(np.median(err)),
             "pk_rel_err_per_shell": [float(f"{e:.3g}") for e in err],
-            "nmodes_equal_fp64": bool(np.array_equal(nm64, raw32["nmodes"]))}
+            "nmodes_equal_fp64": bool(np.array_equal(nm64, raw32["nmodes"])),
+            "kernels_ms_per_mock": {k: round(v["mean_ms"] * v["launches_per_call"], 3) for k, v in kt.items()}}
 
 
 def run_steps(ctx, posd, b, fused, steps, warmup, labels):

@@ -460,35 +460,38 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
 
 // ------------------------------------------- x-slab groups (level-1 binning)
 // When the tiles of the mesh do not fit one workgroup's LDS histogram (N >= 768
-// with 16^3 tiles), particles are first split into G groups of gs consecutive
-// tile planes along x (gs N^2/T^2 tiles fit the LDS), with a copy for a group
-// whose first or last tile plane the particle's stencil reaches from the
-// neighbouring group; each group then goes through the LDS binning and deposit
-// restricted to its tile planes, so every record is emitted by exactly one
-// group.  Raw positions are copied (AoS float32), so the second level computes
-// the same cells.
+// with 16^3 tiles), or for particles in no spatial order, particles are first
+// sorted by their tile row (tx, ty): ntx^2 buckets, with a copy for the row of
+// the neighbouring x-slab group whose first or last tile plane the particle's
+// stencil reaches.  The rows of gs consecutive tile planes form a group; each
+// group then goes through the LDS binning and deposit restricted to its tile
+// planes, so every record is emitted by exactly one group.  Sorted by row, the
+// particles of one binning workgroup's chunk share a few tile rows, so its
+// records are written in runs even when the input has no order.  Raw positions
+// are copied (AoS float32), so the second level computes the same cells.
 template <int T>
-__device__ __forceinline__ int particle_groups(int cx, int R, int Rup, int ntx, int gs, int G, int out[2]) {
-    const int home = cx / T, lc = cx % T, g = home / gs;
-    out[0] = g;
+__device__ __forceinline__ int particle_rows(const int c[3], int R, int Rup, int ntx, int gs, int G, int out[2]) {
+    const int home = c[0] / T, lc = c[0] % T, ty = c[1] / T;
+    out[0] = home * ntx + ty;
     if (G > 1 && lc < R && home % gs == 0) {
-        out[1] = g == 0 ? G - 1 : g - 1;
+        out[1] = (home == 0 ? ntx - 1 : home - 1) * ntx + ty;
         return 2;
     }
     if (G > 1 && lc >= T - Rup && (home % gs == gs - 1 || home == ntx - 1)) {
-        out[1] = g + 1 == G ? 0 : g + 1;
+        out[1] = (home + 1 == ntx ? 0 : home + 1) * ntx + ty;
         return 2;
     }
     return 1;
 }
 
+// Per workgroup (a chunk of particles): count per tile row into hist[row][b].
 template <int T, bool SOA>
 __global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ pos, MasK p, int R, int Rup, int ntx,
-                                                      int gs,
-                                                      int G, size_t chunk, unsigned *__restrict__ hist,
+                                                      int gs, int G, size_t chunk, unsigned *__restrict__ hist,
                                                       unsigned long long *dropped) {
-    __shared__ unsigned lh[64];
-    if (threadIdx.x < 64) lh[threadIdx.x] = 0u;
+    extern __shared__ unsigned lh[];  // ntx^2
+    const int nb = ntx * ntx;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) lh[i] = 0u;
     __syncthreads();
     const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
     unsigned ndrop = 0;
@@ -505,48 +508,71 @@ __global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ 
                 continue;
             }
             int g[2];
-            const int ng = particle_groups<T>(c[0], R, Rup, ntx, gs, G, g);
+            const int ng = particle_rows<T>(c, R, Rup, ntx, gs, G, g);
             for (int k = 0; k < ng; ++k) atomicAdd(&lh[g[k]], 1u);
         }
     }
     if (ndrop) atomicAdd(dropped, (unsigned long long)ndrop);
     __syncthreads();
-    if ((int)threadIdx.x < G) hist[(size_t)blockIdx.x * G + threadIdx.x] = lh[threadIdx.x];
+    unsigned *row = hist + (size_t)blockIdx.x * nb;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) row[i] = lh[i];
 }
 
-// Exclusive prefix over the workgroups per group (in place) and the group
-// starts gstart[0..G] (64-bit), one workgroup.
-__global__ void __launch_bounds__(64) k_group_scan(unsigned *__restrict__ hist, int nrows, int G,
-                                                   unsigned long long *__restrict__ gstart) {
-    __shared__ unsigned long long tot[64];
-    const int g = threadIdx.x;
+// Exclusive prefix over the workgroups per tile row (in place, one thread per
+// row) and the row totals.
+__global__ void __launch_bounds__(256) k_group_colscan(unsigned *__restrict__ hist, int nrows, int nb,
+                                                       unsigned long long *__restrict__ tot) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
     unsigned long long run = 0;
-    if (g < G)
-        for (int r = 0; r < nrows; ++r) {
-            const unsigned v = hist[(size_t)r * G + g];
-            hist[(size_t)r * G + g] = (unsigned)run;  // < 2^32: checked against the total on the host
-            run += v;
-        }
-    tot[g] = g < G ? run : 0ull;
-    __syncthreads();
-    if (g == 0) {
-        unsigned long long a = 0;
-        for (int k = 0; k < G; ++k) {
-            gstart[k] = a;
-            a += tot[k];
-        }
-        gstart[G] = a;
+    for (int r = 0; r < nrows; ++r) {
+        const unsigned v = hist[(size_t)r * nb + b];
+        hist[(size_t)r * nb + b] = (unsigned)run;  // < 2^32: checked against the total on the host
+        run += v;
     }
+    tot[b] = run;
+}
+
+// Row starts rstart[0..nb] (64-bit exclusive scan of the totals), one workgroup
+// of 1024 threads, nb <= 4096.
+__global__ void __launch_bounds__(1024) k_group_scan(const unsigned long long *__restrict__ tot, int nb,
+                                                     unsigned long long *__restrict__ rstart) {
+    __shared__ unsigned long long sw[1024];
+    const int t = threadIdx.x;
+    unsigned long long loc[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int b = t * 4 + k;
+        loc[k] = b < nb ? tot[b] : 0ull;
+        sum += loc[k];
+    }
+    sw[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned long long x = t >= off ? sw[t - off] : 0ull;
+        __syncthreads();
+        sw[t] += x;
+        __syncthreads();
+    }
+    unsigned long long run = sw[t] - sum;  // exclusive
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int b = t * 4 + k;
+        if (b < nb) rstart[b] = run;
+        run += loc[k];
+    }
+    if (t == 1023) rstart[nb] = sw[1023];
 }
 
 template <int T, bool SOA>
 __global__ void __launch_bounds__(1024) k_group_scatter(const float *__restrict__ pos, const float *__restrict__ w,
                                                         MasK p, int R, int Rup, int ntx, int gs, int G, size_t chunk,
                                                         const unsigned *__restrict__ hist,
-                                                        const unsigned long long *__restrict__ gstart,
+                                                        const unsigned long long *__restrict__ rstart,
                                                         float3 *__restrict__ out, float *__restrict__ wout) {
-    __shared__ unsigned long long cur[64];
-    if ((int)threadIdx.x < G) cur[threadIdx.x] = gstart[threadIdx.x] + hist[(size_t)blockIdx.x * G + threadIdx.x];
+    extern __shared__ unsigned long long rcur[];  // ntx^2
+    const int nb = ntx * ntx;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) rcur[i] = rstart[i] + hist[(size_t)blockIdx.x * nb + i];
     __syncthreads();
     const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(p.n, i0 + chunk);
     for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * (size_t)blockDim.x) {
@@ -560,10 +586,10 @@ __global__ void __launch_bounds__(1024) k_group_scatter(const float *__restrict_
             unsigned f[3];
             if (!cells_from(v[u], p, c, f)) continue;
             int g[2];
-            const int ng = particle_groups<T>(c[0], R, Rup, ntx, gs, G, g);
+            const int ng = particle_rows<T>(c, R, Rup, ntx, gs, G, g);
             const float wi = w ? w[i] : 0.0f;
             for (int k = 0; k < ng; ++k) {
-                const unsigned long long slot = atomicAdd(&cur[g[k]], 1ull);
+                const unsigned long long slot = atomicAdd(&rcur[g[k]], 1ull);
                 out[slot] = make_float3(v[u][0], v[u][1], v[u][2]);
                 if (w) wout[slot] = wi;
             }
@@ -959,32 +985,42 @@ static cm_status mas_tiled_t(cm_ctx *ctx, const MasK &p, const float *pos, const
     per_lds = std::min(per_lds, std::max(plane_tiles, kGroupTiles));
     const int gs = (int)std::min<size_t>((size_t)ntx, per_lds / plane_tiles);  // tile planes per group
     const int G = (ntx + gs - 1) / gs;
-    if (G > 64) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %d x-slab groups", G);
+    const int nb = ntx * ntx;  // tile rows
+    if (nb > 4096) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %d tile rows", nb);
     const int nrows = std::max(1, ctx->cu_count);
     const size_t chunk = (p.n + nrows - 1) / nrows;
-    void *hist, *gstart;
+    void *hist, *rtot, *rstart;
     cm_status s;
-    if ((s = scratch(ctx, "mas.ghist", sizeof(unsigned) * (size_t)nrows * G, &hist)) != CM_OK) return s;
-    if ((s = scratch(ctx, "mas.gstart", sizeof(unsigned long long) * (size_t)(G + 1), &gstart)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.ghist", sizeof(unsigned) * (size_t)nrows * nb, &hist)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.rtot", sizeof(unsigned long long) * (size_t)nb, &rtot)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.gstart", sizeof(unsigned long long) * (size_t)(nb + 1), &rstart)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_group_count", p.soa ? k_group_count<T, true> : k_group_count<T, false>, dim3(nrows),
-                    dim3(1024), 0, pos, p, R, Rup, ntx, gs, G, chunk, (unsigned *)hist, dropped)) != CM_OK)
+                    dim3(1024), sizeof(unsigned) * (size_t)nb, pos, p, R, Rup, ntx, gs, G, chunk, (unsigned *)hist,
+                    dropped)) != CM_OK)
         return s;
-    if ((s = launch(ctx, "mas_group_scan", k_group_scan, dim3(1), dim3(64), 0, (unsigned *)hist, nrows, G,
-                    (unsigned long long *)gstart)) != CM_OK)
+    if ((s = launch(ctx, "mas_group_scan", k_group_colscan, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                    (unsigned *)hist, nrows, nb, (unsigned long long *)rtot)) != CM_OK)
         return s;
-    std::vector<unsigned long long> gs_h((size_t)G + 1);
-    CM_HIP(ctx, hipMemcpyAsync(gs_h.data(), gstart, sizeof(unsigned long long) * gs_h.size(), hipMemcpyDeviceToHost,
+    if ((s = launch(ctx, "mas_group_scan", k_group_scan, dim3(1), dim3(1024), 0, (const unsigned long long *)rtot, nb,
+                    (unsigned long long *)rstart)) != CM_OK)
+        return s;
+    std::vector<unsigned long long> rs_h((size_t)nb + 1);
+    CM_HIP(ctx, hipMemcpyAsync(rs_h.data(), rstart, sizeof(unsigned long long) * rs_h.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ++ctx->host_syncs;
+    // group g holds the rows of tile planes [g gs, (g+1) gs)
+    std::vector<unsigned long long> gs_h((size_t)G + 1);
+    for (int g = 0; g <= G; ++g) gs_h[(size_t)g] = rs_h[(size_t)std::min(g * gs, ntx) * ntx];
     const unsigned long long total = gs_h[(size_t)G];
     if (total > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: %llu grouped particles", total);
     void *gpos, *gw = nullptr;
     if ((s = scratch(ctx, "mas.gpos", sizeof(float3) * (size_t)(total ? total : 1), &gpos)) != CM_OK) return s;
     if (w && (s = scratch(ctx, "mas.gw", sizeof(float) * (size_t)(total ? total : 1), &gw)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_group_scatter", p.soa ? k_group_scatter<T, true> : k_group_scatter<T, false>,
-                    dim3(nrows), dim3(1024), 0, pos, w, p, R, Rup, ntx, gs, G, chunk, (const unsigned *)hist,
-                    (const unsigned long long *)gstart, (float3 *)gpos, (float *)gw)) != CM_OK)
+                    dim3(nrows), dim3(1024), sizeof(unsigned long long) * (size_t)nb, pos, w, p, R, Rup, ntx, gs, G,
+                    chunk, (const unsigned *)hist, (const unsigned long long *)rstart, (float3 *)gpos,
+                    (float *)gw)) != CM_OK)
         return s;
     for (int g = 0; g < G; ++g) {
         MasK q = p;
