@@ -84,6 +84,12 @@ class Params {
     int _i_coord2_g() const { return i_coord2_g; }
     int _i_coord3_g() const { return i_coord3_g; }
     int _i_weight1_g() const { return i_weight1_g; }
+    int _i_v1_g() const { return i_v1_g; }
+    int _i_v2_g() const { return i_v2_g; }
+    int _i_v3_g() const { return i_v3_g; }
+    std::string _vel_units_g() const { return vel_units_g; }
+    bool _redshift_space_coords_g() const { return redshift_space_coords_g; }
+    double _redshift() const { return redshift; }
     real_prec _Xoffset() const { return Xoffset; }
     real_prec _Yoffset() const { return Yoffset; }
     real_prec _Zoffset() const { return Zoffset; }
@@ -133,6 +139,12 @@ class Params {
     std::string file_catalogue = "cat.dat", Input_dir_cat = "";
     bool use_random_catalog = false, measure_cross = false;
     int i_coord1_g = 0, i_coord2_g = 1, i_coord3_g = 2, i_weight1_g = -1;
+    // velocity columns (-1: absent), their units and whether the coordinates
+    // already hold the redshift-space displacement (Params.cpp:170,215-227)
+    int i_v1_g = -1, i_v2_g = -1, i_v3_g = -1;
+    std::string vel_units_g = "kmps";
+    bool redshift_space_coords_g = false;
+    double redshift = 0.0;  // the snapshot's redshift (Params.cpp:127,567-570)
     // binary SoA catalogues (Catalog::read_catalog_bin, Source/Catalog.cpp:560):
     // one headerless real_prec file per coordinate
     std::string file_bin_x_coord, file_bin_y_coord, file_bin_z_coord;
@@ -299,5 +311,9 @@ void write_to_file2(const std::string &fname, const std::vector<real_prec> &kve,
 // Source/Catalog.cpp:16-37).
 double comoving_distance(const Params &p, double z);
 void redshift_table(const Params &p, std::vector<double> &zc, std::vector<double> &rc);
-// ASCII catalogue: whitespace-separated columns (Catalog::read_catalog).
+// Line-of-sight velocity -> displacement in Mpc/h for redshift-space
+// coordinates (Source/Catalog.cpp:14088-14099): (1+z)/H(z) for km/s.
+double rsd_conversion_factor(const Params &p);
+// ASCII catalogue: whitespace-separated columns (Catalog::read_catalog); a
+// velocity column i_v3_g moves objects along the third axis into redshift space.
 void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w);

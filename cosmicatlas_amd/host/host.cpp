@@ -111,6 +111,12 @@ void Params::read_pars(const std::string &file) {
         else if (name == "i_coord2_g") i_coord2_g = std::atoi(value.c_str());
         else if (name == "i_coord3_g") i_coord3_g = std::atoi(value.c_str());
         else if (name == "i_weight1_g") i_weight1_g = std::atoi(value.c_str());
+        else if (name == "i_v1_g") i_v1_g = std::atoi(value.c_str());
+        else if (name == "i_v2_g") i_v2_g = std::atoi(value.c_str());
+        else if (name == "i_v3_g") i_v3_g = std::atoi(value.c_str());
+        else if (name == "vel_units_g") vel_units_g = value;
+        else if (name == "redshift_space_coords_g") redshift_space_coords_g = to_bool(value);
+        else if (name == "Redshift") redshift = std::atof(value.c_str());
         else if (name == "file_bin_x_coord") file_bin_x_coord = value;
         else if (name == "file_bin_y_coord") file_bin_y_coord = value;
         else if (name == "file_bin_z_coord") file_bin_z_coord = value;
@@ -179,6 +185,9 @@ void Params::show_params() const {
         for (double v : rc) std::printf(" %.17g", v);
         std::printf("\n");
     }
+    if (i_v3_g > 0)
+        std::printf("redshift_space_coords_g = %s\nrsd_conversion_factor = %.9g\n",
+                    redshift_space_coords_g ? "true" : "false", (double)(real_prec)rsd_conversion_factor(*this));
 }
 
 // =============================================================== mass assignment
@@ -694,6 +703,29 @@ static void read_catalog_bin(const Params &p, std::vector<real_prec> &xyz) {
     }
 }
 
+// velocity units -> comoving Mpc/h along the line of sight
+// (Catalog.cpp:14088-14099, PowerSpectrumF.cpp:445-454): km/s times (1+z)/H(z);
+// "alpt" velocities divided by H(z); Mpc/h as they are; any other name leaves
+// the factor at 1. The velocity-bias power the reference multiplies in is 1.
+double rsd_conversion_factor(const Params &p) {
+    const double z = p._redshift();
+    if (p._vel_units_g() == "kmps") return (1.0 + z) / hubble_function(p, z);
+    if (p._vel_units_g() == "alpt") return 1.0 / hubble_function(p, z);
+    return 1.0;
+}
+
+// Catalog::get_interpolated_density_field (Source/Catalog.cpp:14071-14129):
+// with the coordinates in real space (redshift_space_coords_g false), every
+// object moves along the line of sight, the third Cartesian axis, by its
+// velocity times the conversion factor; the mesh assignment then wraps the
+// moved coordinate into the box (Miscelanious.cpp:2711-2716)
+static void to_redshift_space(const Params &p, std::vector<real_prec> &xyz, const std::vector<real_prec> &vlos) {
+    if (p._redshift_space_coords_g()) return;
+    if (vlos.size() * 3 != xyz.size()) throw cmh::Error("velocity column of a different length than the coordinates");
+    const real_prec f = (real_prec)rsd_conversion_factor(p);
+    for (size_t i = 0; i < vlos.size(); ++i) xyz[3 * i + 2] = xyz[3 * i + 2] + f * vlos[i];
+}
+
 void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w) {
     xyz.clear();
     w.clear();
@@ -706,9 +738,13 @@ void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real
     if (!f) throw cmh::Error("File " + p._file_catalogue() + " not found");
     const int c[3] = {p._i_coord1_g(), p._i_coord2_g(), p._i_coord3_g()};
     const int cw = p._i_weight1_g();
-    int need = std::max(std::max(c[0], c[1]), std::max(c[2], cw)) + 1;
+    // the line-of-sight velocity: the column is read when its index is
+    // positive (Catalog.cpp:2306, `if(i_vz>0)`)
+    const int cv = p._i_v3_g() > 0 ? p._i_v3_g() : -1;
+    int need = std::max(std::max(std::max(c[0], c[1]), std::max(c[2], cw)), cv) + 1;
     std::string line;
     std::vector<double> col;
+    std::vector<real_prec> vlos;
     xyz.clear();
     w.clear();
     while (std::getline(f, line)) {
@@ -720,8 +756,10 @@ void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real
         if ((int)col.size() < need) continue;
         for (int a = 0; a < 3; ++a) xyz.push_back((real_prec)col[c[a]]);
         if (cw >= 0) w.push_back((real_prec)col[cw]);
+        if (cv >= 0) vlos.push_back((real_prec)col[cv]);
     }
     to_cartesian(p, xyz);
+    if (!vlos.empty()) to_redshift_space(p, xyz, vlos);
 }
 
 // ============================================================== PowerSpectrumF

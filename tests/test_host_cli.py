@@ -252,6 +252,59 @@ def test_cli_redshift_catalogue(tmp_path, orc):
     assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
 
 
+def _hubble(z, om=0.3089, orad=1.3157e-5 * (1 + 0.2271 * 3.046)):
+    return 100.0 * math.sqrt(om * (1 + z) ** 3 + orad * (1 + z) ** 4 + (1 - om - orad))
+
+
+@pytest.mark.parametrize("units,z,factor", [
+    ("kmps", 0.0, lambda z: 1.0 / _hubble(z)),              # 1/H0 = 0.01 Mpc/h per km/s
+    ("kmps", 0.57, lambda z: (1 + z) / _hubble(z)),
+    ("alpt", 1.2, lambda z: 1.0 / _hubble(z)),
+    ("Mpcph", 0.57, lambda z: 1.0),
+    ("other", 0.57, lambda z: 1.0)])
+def test_rsd_conversion_factor(tmp_path, units, z, factor):
+    """Velocity to line-of-sight displacement (Catalog.cpp:14088-14099) against
+    (1+z)/H(z) written out for the default flat background."""
+    d = _show(_ini(tmp_path, i_v3_g="5", vel_units_g=units, Redshift=str(z)))
+    assert d["redshift_space_coords_g"] == "false"
+    assert float(d["rsd_conversion_factor"]) == pytest.approx(factor(z), rel=1e-7)
+    if units == "kmps" and z == 0.0:
+        assert float(d["rsd_conversion_factor"]) == pytest.approx(0.01, rel=1e-7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("coords_in_rsd", [False, True])
+def test_cli_redshift_space_catalogue(tmp_path, orc, coords_in_rsd):
+    """x y z vx vy vz columns: in real-space coordinates every object moves by
+    vz (1+z)/H(z) along the third axis (Catalog::get_interpolated_density_field,
+    Source/Catalog.cpp:14071-14129), wrapping through the box; with
+    redshift_space_coords_g the velocities are ignored. Column 0 is never a
+    velocity (the reference reads one when its index is positive)."""
+    N, L, n, z = 64, 500.0, 60000, 0.57
+    rng = np.random.default_rng(25)
+    pos = rng.uniform(0, L, (n, 3)).astype(np.float32)
+    vel = rng.normal(0, 600.0, (n, 3)).astype(np.float32)  # km/s: up to ~25 Mpc/h, many wrap
+    np.savetxt(tmp_path / "cat.txt", np.hstack([pos, vel]), fmt="%.7f")
+    rows = np.loadtxt(tmp_path / "cat.txt").astype(np.float32)
+    pos, vel = rows[:, :3].copy(), rows[:, 3:]
+    ini = _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", i_v1_g="3", i_v2_g="4", i_v3_g="5",
+               Redshift=str(z), redshift_space_coords_g="true" if coords_in_rsd else "false")
+    out = _run_pk(tmp_path, ini, N)
+    ref = pos.copy()
+    if not coords_in_rsd:
+        f = np.float32(float(_show(ini)["rsd_conversion_factor"]))
+        assert f == pytest.approx((1 + z) / _hubble(z), rel=1e-7)
+        ref[:, 2] = pos[:, 2] + f * vel[:, 2]  # float32 like the reference's real_prec
+        assert (ref[:, 2] < 0).sum() > 100 and (ref[:, 2] >= L).sum() > 100
+    p0, nm, sn = _oracle_pk(orc, ref, N, L)
+    assert np.array_equal(out[:, 2].astype(int), nm)
+    assert np.max(np.abs(out[:, 1] - p0) / (np.abs(p0) + sn)) < 1e-4
+    if not coords_in_rsd:
+        # the displacement matters at this tolerance: the real-space spectrum differs
+        p_real, _, _ = _oracle_pk(orc, pos, N, L)
+        assert np.max(np.abs(out[:, 1] - p_real) / (np.abs(p_real) + sn)) > 1e-2
+
+
 def _bk_catalogue(tmp_path, N, L, n, seed):
     pos = np.random.default_rng(seed).uniform(0, L, (n, 3)).astype(np.float32)
     np.savetxt(tmp_path / "cat.txt", pos, fmt="%.7f")
