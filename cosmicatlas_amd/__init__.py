@@ -262,11 +262,18 @@ class Context:
 
     def measure_fkp(self, pos, rand, N, L, scheme, nshells, kmin, dk_bin, weights=None, rweights=None, nbar=0.0,
                     P_est=0.0, fkp_weight=False, window=False, mas_correction=True, sn_correction=True,
-                    float_shell_rule=False):
-        """FKP estimator with a random catalogue ``rand`` ((nr,3) float32)."""
+                    float_shell_rule=False, nbar_g=None, nbar_r=None):
+        """FKP estimator with a random catalogue ``rand`` ((nr,3) float32);
+        ``nbar_g`` / ``nbar_r`` (device float32, one per object) give each
+        object's mean density in place of the constant ``nbar``."""
         o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=mas_correction,
                           sn_correction=sn_correction, float_shell_rule=float_shell_rule)
-        f = _abi.FkpOpts(float(nbar), float(P_est), int(fkp_weight), int(window))
+        torch = _torch()
+        for a, m in ((nbar_g, pos), (nbar_r, rand)):
+            if a is not None and (a.dtype != torch.float32 or not a.is_contiguous() or a.numel() != m.shape[0]):
+                raise ValueError("nbar_g / nbar_r: contiguous float32, one value per object")
+        f = _abi.FkpOpts(float(nbar), float(P_est), int(fkp_weight), int(window),                          nbar_g.data_ptr() if nbar_g is not None else None,
+                         nbar_r.data_ptr() if nbar_r is not None else None)
         p0, p2, p4, pw = (np.zeros(nshells) for _ in range(4))
         nm = np.zeros(nshells, np.int64)
         info = _abi.MeasureInfo()

@@ -111,6 +111,8 @@ class FkpOpts(C.Structure):
         ("P_est", C.c_double),
         ("fkp_weight", C.c_int),
         ("window", C.c_int),
+        ("nbar_g", C.c_void_p),
+        ("nbar_r", C.c_void_p),
     ]
 
 

@@ -2,7 +2,8 @@
 //
 // Per catalogue (Catalog::get_interpolated_density_field, Source/Catalog.cpp:
 // 14153-14171): total weight w = w_user * w_fkp, w_fkp = 1/(1 + nbar P_est);
-// W = sum w, S = sum w^2, normal = sum nbar w^2 (nbar constant here).
+// W = sum w, S = sum w^2, normal = sum nbar w^2, with nbar per object
+// (Halo.mean_density, :14150-14152) or one constant.
 // FftwFunctions::get_parameters_estimator random branch
 // (Source/FftwFunctions.cpp:180-200): alpha = W_g/W_r, normal_power =
 // alpha normal_r, shot noise = (alpha + 1) S_r/normal_r, window
@@ -16,37 +17,59 @@
 
 namespace cm {
 
-// FP64 sums of w and w^2 (unit weights when w is NULL), one partial pair per
-// workgroup.
-__global__ void __launch_bounds__(256) k_wsums(const float *__restrict__ w, size_t n, double *__restrict__ part) {
-    __shared__ double r1[256], r2[256];
-    double a = 0.0, b = 0.0;
+// The total weight of object i (Source/Catalog.cpp:14142-14160): its user
+// weight (1 without weights) times, when fkp, 1/(1 + P_est nbar) with the
+// object's own mean density nb[i], or nbar0 for all objects when nb is NULL.
+struct WeightRule {
+    const float *w;
+    const float *nb;
+    double nbar0, P_est;
+    int fkp;
+};
+
+__device__ inline double total_weight(const WeightRule &r, size_t i, double &nb) {
+    nb = r.nb ? (double)r.nb[i] : r.nbar0;
+    double v = r.w ? (double)r.w[i] : 1.0;
+    if (r.fkp) v *= 1.0 / (1.0 + r.P_est * nb);
+    return v;
+}
+
+// FP64 sums of w, w^2 and nbar w^2, one partial triple per workgroup.
+__global__ void __launch_bounds__(256) k_wsums(WeightRule r, size_t n, double *__restrict__ part) {
+    __shared__ double r1[256], r2[256], r3[256];
+    double a = 0.0, b = 0.0, c = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double v = w ? (double)w[i] : 1.0;
+        double nb;
+        const double v = total_weight(r, i, nb);
         a += v;
         b += v * v;
+        c += nb * v * v;
     }
     r1[threadIdx.x] = a;
     r2[threadIdx.x] = b;
+    r3[threadIdx.x] = c;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             r1[threadIdx.x] += r1[threadIdx.x + o];
             r2[threadIdx.x] += r2[threadIdx.x + o];
+            r3[threadIdx.x] += r3[threadIdx.x + o];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = r1[0];
-        part[2 * blockIdx.x + 1] = r2[0];
+        part[3 * blockIdx.x] = r1[0];
+        part[3 * blockIdx.x + 1] = r2[0];
+        part[3 * blockIdx.x + 2] = r3[0];
     }
 }
 
-// out[i] = s * (w ? w[i] : 1)
-__global__ void __launch_bounds__(256) k_wscale(const float *__restrict__ w, size_t n, float s,
-                                                float *__restrict__ out) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = s * (w ? w[i] : 1.0f);
+// deposit weights out[i] = s * (total weight of object i)
+__global__ void __launch_bounds__(256) k_wscale(WeightRule r, size_t n, double s, float *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double nb;
+        out[i] = (float)(s * total_weight(r, i, nb));
+    }
 }
 
 static unsigned grid_of(size_t n, size_t cap) {
@@ -54,33 +77,38 @@ static unsigned grid_of(size_t n, size_t cap) {
     return (unsigned)(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-static cm_status weight_sums(cm_ctx *ctx, const float *w, size_t n, double *sum, double *sum2) {
-    *sum = *sum2 = 0.0;
+struct WeightSums {
+    double W = 0, S = 0, normal = 0;  // sum w, sum w^2, sum nbar w^2
+};
+
+static cm_status weight_sums(cm_ctx *ctx, const WeightRule &r, size_t n, WeightSums *out) {
+    *out = WeightSums{};
     if (!n) return CM_OK;
     const unsigned g = grid_of(n, 1024);
     void *part;
     cm_status s;
-    if ((s = scratch(ctx, "fkp.part", sizeof(double) * 2 * g, &part)) != CM_OK) return s;
-    if ((s = launch(ctx, "fkp_wsums", k_wsums, dim3(g), dim3(256), 0, w, n, (double *)part)) != CM_OK) return s;
-    std::vector<double> h(2 * (size_t)g);
+    if ((s = scratch(ctx, "fkp.part", sizeof(double) * 3 * g, &part)) != CM_OK) return s;
+    if ((s = launch(ctx, "fkp_wsums", k_wsums, dim3(g), dim3(256), 0, r, n, (double *)part)) != CM_OK) return s;
+    std::vector<double> h(3 * (size_t)g);
     CM_HIP(ctx, hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (unsigned i = 0; i < g; ++i) {
-        *sum += h[2 * i];
-        *sum2 += h[2 * i + 1];
+        out->W += h[3 * i];
+        out->S += h[3 * i + 1];
+        out->normal += h[3 * i + 2];
     }
     return CM_OK;
 }
 
-// deposit weights s * w (or s); NULL when they are all 1
-static cm_status scaled_weights(cm_ctx *ctx, const char *name, const float *w, size_t n, double s,
+// deposit weights s * w_i; NULL when they are all 1
+static cm_status scaled_weights(cm_ctx *ctx, const char *name, const WeightRule &r, size_t n, double s,
                                 const float **out) {
     *out = nullptr;
-    if (!w && s == 1.0) return CM_OK;
+    if (!r.w && !r.fkp && s == 1.0) return CM_OK;
     void *buf;
     cm_status st;
     if ((st = scratch(ctx, name, sizeof(float) * (n ? n : 1), &buf)) != CM_OK) return st;
-    if (n && (st = launch(ctx, "fkp_wscale", k_wscale, dim3(grid_of(n, 16384)), dim3(256), 0, w, n, (float)s,
+    if (n && (st = launch(ctx, "fkp_wscale", k_wscale, dim3(grid_of(n, 16384)), dim3(256), 0, r, n, s,
                           (float *)buf)) != CM_OK)
         return st;
     *out = (const float *)buf;
@@ -120,28 +148,35 @@ static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const c
     const size_t n3 = (size_t)N * N * N;
     const double L = o->mas.L, V = L * L * L;
     cm_status s;
-    double Wg, Sg, Wr, Sr;
-    if ((s = weight_sums(ctx, w, n, &Wg, &Sg)) != CM_OK) return s;
-    if ((s = weight_sums(ctx, rw, nr, &Wr, &Sr)) != CM_OK) return s;
-    if (!(Wg > 0) || !(Wr > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: empty catalogue");
-    const double nbar = f->nbar > 0 ? f->nbar : Wg / V;
-    const double wf = f->fkp_weight ? 1.0 / (1.0 + nbar * f->P_est) : 1.0;
-    Wg *= wf;
-    Wr *= wf;
-    Sr *= wf * wf;
-    const double alpha = Wg / Wr;
-    const double normal_r = nbar * Sr;
+    // the constant mean density: given, or the objects' user weight over the volume
+    WeightRule rg{w, nullptr, 0.0, 0.0, 0};
+    WeightSums g0, g, r;
+    if ((s = weight_sums(ctx, rg, n, &g0)) != CM_OK) return s;
+    const double nbar = f->nbar > 0 ? f->nbar : g0.W / V;
+    rg = WeightRule{w, (const float *)f->nbar_g, nbar, f->P_est, f->fkp_weight};
+    const WeightRule rr{rw, (const float *)f->nbar_r, nbar, f->P_est, f->fkp_weight};
+    if (rg.fkp) {
+        if ((s = weight_sums(ctx, rg, n, &g)) != CM_OK) return s;
+    } else {
+        g = g0;
+    }
+    if ((s = weight_sums(ctx, rr, nr, &r)) != CM_OK) return s;
+    if (!(g.W > 0) || !(r.W > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: empty catalogue");
+    if (!(r.normal > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: the randoms' sum of nbar w^2 is not positive");
+    const double alpha = g.W / r.W;
+    const double normal_r = r.normal;
     const double normal_power = alpha * normal_r;
-    const double shot_noise = (alpha + 1.0) * Sr / normal_r;
+    const double shot_noise = (alpha + 1.0) * r.S / normal_r;
+    const double Wr = r.W;
 
     void *mesh;
     if ((s = scratch(ctx, "m.mesh", sizeof(float) * n3, &mesh)) != CM_OK) return s;
     cm_mas_desc md = o->mas;
     md.algo = CM_MAS_AUTO;
     const float *wg, *wr;
-    if ((s = scaled_weights(ctx, "fkp.wg", w, n, wf, &wg)) != CM_OK) return s;
+    if ((s = scaled_weights(ctx, "fkp.wg", rg, n, 1.0, &wg)) != CM_OK) return s;
     if ((s = mesh_assign_impl(ctx, &md, pos, wg, n, mesh, CM_F32, 0)) != CM_OK) return s;
-    if ((s = scaled_weights(ctx, "fkp.wr", rw, nr, -alpha * wf, &wr)) != CM_OK) return s;
+    if ((s = scaled_weights(ctx, "fkp.wr", rr, nr, -alpha, &wr)) != CM_OK) return s;
     if ((s = assign_randoms(ctx, &md, rpos, wr, nr, mesh, 1)) != CM_OK) return s;
     std::vector<double> hs;
     std::vector<unsigned long long> hc;
@@ -149,14 +184,14 @@ static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const c
     const double sn = o->sn_correction ? shot_noise : 0.0;
     pk_finalize(o->nshells, normal_power, sn * normal_power, hs.data(), hc.data(), pk0, pk2, pk4, nmodes);
     if (info) {
-        info->n_obj = Wg;
+        info->n_obj = g.W;
         info->shot_noise = shot_noise;
         info->normal_power = normal_power;
         info->tweb = cm_tweb_stats{};
     }
     if (f->window && pk_window) {
         const float *wr1;
-        if ((s = scaled_weights(ctx, "fkp.wr", rw, nr, wf, &wr1)) != CM_OK) return s;
+        if ((s = scaled_weights(ctx, "fkp.wr", rr, nr, 1.0, &wr1)) != CM_OK) return s;
         if ((s = assign_randoms(ctx, &md, rpos, wr1, nr, mesh, 0)) != CM_OK) return s;
         if ((s = shells_of_mesh(ctx, o, mesh, Wr / (double)n3, hs, hc)) != CM_OK) return s;
         std::vector<double> q2((size_t)o->nshells), q4((size_t)o->nshells);

@@ -275,14 +275,18 @@ typedef struct {
     double P_est;     /* FKP weight 1/(1 + nbar P_est) when fkp_weight (Params _Pest) */
     int fkp_weight;   /* Params _FKP_weight */
     int window;       /* also the window: shells of the randoms' power */
+    const void *nbar_g; /* device float32 n(r) of each object (Halo.mean_density,
+                           Source/Catalog.cpp:14150-14152); NULL: the constant nbar */
+    const void *nbar_r; /* the same for each random; NULL: the constant nbar */
 } cm_fkp_opts;
 
 /* The random-catalogue branch of the estimator
  * (FftwFunctions::get_parameters_estimator, Source/FftwFunctions.cpp:177-200;
  * get_fluctuation, :275-281; per-object sums of Catalog::
  * get_interpolated_density_field, Source/Catalog.cpp:14153-14171): weights
- * w = w_user * w_fkp, W = sum w, S = sum w^2, normal = sum nbar w^2 per
- * catalogue; alpha = W_g/W_r, delta = rho_g - alpha rho_r (one mesh: the randoms
+ * w = w_user * w_fkp, w_fkp = 1/(1 + nbar P_est), W = sum w, S = sum w^2,
+ * normal = sum nbar w^2 per catalogue, nbar per object when f->nbar_g / nbar_r
+ * are given; alpha = W_g/W_r, delta = rho_g - alpha rho_r (one mesh: the randoms
  * are deposited with weight -alpha w), normal_power = alpha normal_r,
  * shot noise = (alpha + 1) S_r / normal_r.  pk_window (NULL unless
  * f->window) receives the randoms' shells over normal_power/alpha^2

@@ -109,7 +109,7 @@ def zeldovich(NL: int, L: float, A: float = 1e4, ns: float = -1.5, seed: int = 1
     comps = (kd[:, None, None], kd[None, :, None], kdz[None, None, :])
     grids = np.meshgrid(q, q, q, indexing="ij")
     for a in range(3):
-        psi = np.fft.irfftn(1j * comps[a] * dk * inv, s=(NL, NL, NL))
+        psi = np.fft.irfftn(1j * comps[a] * dk * inv, s=(NL, NL, NL), axes=(0, 1, 2))
         pos[..., a] = grids[a] + psi
     pos = np.mod(pos, L).reshape(-1, 3)
     out = pos.astype(dtype)

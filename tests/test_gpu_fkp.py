@@ -96,3 +96,70 @@ def test_fkp_tends_to_box_estimator(ctx):
     assert sig.sum() >= 3
     rel = np.abs(fk["pk0"][sig] - box["pk0"][sig]) / box["pk0"][sig]
     assert rel.max() < 0.15, rel
+
+
+def _fkp_oracle(orc, gal, ran, wg, wr, nbg, nbr, P_est, N, L, scheme, b):
+    """The estimator assembled from oracle deposits, FFT and shells with the
+    per-object sums of Catalog::get_interpolated_density_field
+    (Source/Catalog.cpp:14142-14171) and the random branch of
+    get_parameters_estimator (Source/FftwFunctions.cpp:180-186)."""
+    fg = wg.astype(np.float64) / (1.0 + P_est * nbg.astype(np.float64))
+    fr = wr.astype(np.float64) / (1.0 + P_est * nbr.astype(np.float64))
+    alpha = fg.sum() / fr.sum()
+    normal_r = (nbr.astype(np.float64) * fr ** 2).sum()
+    SN = (alpha + 1) * (fr ** 2).sum() / normal_r
+    mesh = (orc.mas(gal, N, L, scheme, weights=fg.astype(np.float32))
+            - alpha * orc.mas(ran, N, L, scheme, weights=fr.astype(np.float32)))
+    p = orc.pk_shells(orc.fft_r2c(mesh), orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, SN,
+                                                     alpha * normal_r))
+    return p, SN, alpha * normal_r
+
+
+def test_per_object_mean_density(ctx, orc):
+    """n(r) per object (a gradient along x, as a selection function gives):
+    weights 1/(1 + P_est n_i) and normal = sum n_i w_i^2 against the oracle."""
+    import cosmicatlas_amd as cm
+    N, L, scheme, P_est = 32, 200.0, 1, 1e4
+    gal = zref.zeldovich(24, L, A=2e3, seed=11)
+    ran = zref.uniform(3 * len(gal), L, 12)
+    rng = np.random.default_rng(13)
+    wg = rng.uniform(0.5, 1.5, len(gal)).astype(np.float32)
+    wr = rng.uniform(0.5, 1.5, len(ran)).astype(np.float32)
+    nbg = (1e-4 * (1.0 + 4.0 * gal[:, 0] / L)).astype(np.float32)
+    nbr = (1e-4 * (1.0 + 4.0 * ran[:, 0] / L)).astype(np.float32)
+    b = cm.binning(N, L)
+    cu = lambda a: torch.from_numpy(a).cuda()
+    out = ctx.measure_fkp(cu(gal), cu(ran), N, L, scheme, b["nshells"], b["kmin"], b["dk"], weights=cu(wg),
+                          rweights=cu(wr), P_est=P_est, fkp_weight=True, nbar_g=cu(nbg), nbar_r=cu(nbr))
+    (p0, p2, p4, nm), SN, normal = _fkp_oracle(orc, gal, ran, wg, wr, nbg, nbr, P_est, N, L, scheme, b)
+    assert np.array_equal(out["nmodes"], nm)
+    assert out["shot_noise"] == pytest.approx(SN, rel=1e-9)
+    assert out["normal_power"] == pytest.approx(normal, rel=1e-9)
+    denom = np.abs(p0) + SN
+    for got, ref in ((out["pk0"], p0), (out["pk2"], p2), (out["pk4"], p4)):
+        assert np.max(np.abs(got - ref) / denom) < 1e-4
+    # the weights vary by a factor (1 + 5e-4 P_est) / (1 + 1e-4 P_est) = 3 across the box:
+    # the mean density taken constant changes the shot noise S_r / normal_r by ~30%
+    const = ctx.measure_fkp(cu(gal), cu(ran), N, L, scheme, b["nshells"], b["kmin"], b["dk"], weights=cu(wg),
+                            rweights=cu(wr), nbar=float(nbr.mean()), P_est=P_est, fkp_weight=True)
+    assert abs(out["shot_noise"] / const["shot_noise"] - 1) > 0.15
+
+
+def test_constant_per_object_density_equals_constant(ctx):
+    """Arrays that hold the constant nbar reproduce the constant-nbar result, with
+    FKP weights on and off (off: nbar enters the normalisation only)."""
+    import cosmicatlas_amd as cm
+    N, L, nbar = 32, 150.0, 2.5e-3
+    gal = torch.from_numpy(zref.zeldovich(20, L, A=2e3, seed=14)).cuda()
+    ran = ctx.uniform(4 * gal.shape[0], L, seed=15)
+    b = cm.binning(N, L)
+    for fkp in (True, False):
+        kw = dict(nbar=nbar, P_est=3e3, fkp_weight=fkp)
+        a = ctx.measure_fkp(gal, ran, N, L, "TSC", b["nshells"], b["kmin"], b["dk"], **kw)
+        c = ctx.measure_fkp(gal, ran, N, L, "TSC", b["nshells"], b["kmin"], b["dk"],
+                            nbar_g=torch.full((gal.shape[0],), nbar, device="cuda"),
+                            nbar_r=torch.full((ran.shape[0],), nbar, device="cuda"), **kw)
+        assert c["normal_power"] == pytest.approx(a["normal_power"], rel=1e-6)
+        assert c["shot_noise"] == pytest.approx(a["shot_noise"], rel=1e-6)
+        assert np.array_equal(a["nmodes"], c["nmodes"])
+        assert np.max(np.abs(a["pk0"] - c["pk0"]) / (np.abs(a["pk0"]) + a["shot_noise"])) < 1e-5
