@@ -163,3 +163,23 @@ def test_constant_per_object_density_equals_constant(ctx):
         assert c["shot_noise"] == pytest.approx(a["shot_noise"], rel=1e-6)
         assert np.array_equal(a["nmodes"], c["nmodes"])
         assert np.max(np.abs(a["pk0"] - c["pk0"]) / (np.abs(a["pk0"]) + a["shot_noise"])) < 1e-5
+
+
+def test_fkp_errors(ctx):
+    """Empty randoms, a zero mean density for every random, and per-object
+    arrays of the wrong length are refused before any deposit."""
+    import cosmicatlas_amd as cm
+    N, L = 32, 100.0
+    gal = ctx.uniform(1000, L, seed=16)
+    ran = ctx.uniform(3000, L, seed=17)
+    b = cm.binning(N, L)
+    args = (N, L, "CIC", b["nshells"], b["kmin"], b["dk"])
+    with pytest.raises(cm.CosmoError, match="empty catalogue"):
+        ctx.measure_fkp(gal, ran[:0], *args)
+    with pytest.raises(cm.CosmoError, match="nbar w\\^2"):
+        ctx.measure_fkp(gal, ran, *args, nbar_r=torch.zeros(3000, device="cuda"))
+    with pytest.raises(ValueError):
+        ctx.measure_fkp(gal, ran, *args, nbar_g=torch.ones(999, device="cuda"))
+    # the context still measures after the refusals
+    out = ctx.measure_fkp(gal, ran, *args)
+    assert np.isfinite(out["pk0"]).all()
