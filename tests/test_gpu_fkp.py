@@ -115,11 +115,13 @@ def _fkp_oracle(orc, gal, ran, wg, wr, nbg, nbr, P_est, N, L, scheme, b):
     return p, SN, alpha * normal_r
 
 
-def test_per_object_mean_density(ctx, orc):
+@pytest.mark.parametrize("scheme", [0, 1, 3])
+def test_per_object_mean_density(ctx, orc, scheme):
     """n(r) per object (a gradient along x, as a selection function gives):
-    weights 1/(1 + P_est n_i) and normal = sum n_i w_i^2 against the oracle."""
+    weights 1/(1 + P_est n_i) and normal = sum n_i w_i^2 against the oracle,
+    for NGP, CIC and PCS deposits."""
     import cosmicatlas_amd as cm
-    N, L, scheme, P_est = 32, 200.0, 1, 1e4
+    N, L, P_est = 32, 200.0, 1e4
     gal = zref.zeldovich(24, L, A=2e3, seed=11)
     ran = zref.uniform(3 * len(gal), L, 12)
     rng = np.random.default_rng(13)
