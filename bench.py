@@ -31,6 +31,9 @@ NL, N, L = 256, 512, 1000.0
 NDEL, R_SMOOTH, LAMBDA_TH = 2, 2.0, 0.0
 
 
+MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9
+
+
 def algorithmic_bytes(name, n_part, n_rec):
     """Bytes a kernel must move once (each array read or written once)."""
     n3 = N ** 3
@@ -681,6 +684,15 @@ def main():
         roof = {"kernel": dom[0], "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
                 "mean_ms": round(dom[1]["mean_ms"], 4)}
+        vf = os.path.join(ROOT, "profiles", "valu.json")
+        insts = json.load(open(vf))["sq_insts_valu"].get(dom[0]) if os.path.exists(vf) else None
+        if insts and N == 512:
+            # a wave64 VALU instruction holds one of the 4 SIMDs of a CU for 4 cycles
+            bound_ms = insts * 4 / (MI355X_CUS * 4) / MI355X_CLOCK_HZ * 1e3
+            roof["valu_issue"] = {"sq_insts_valu": insts, "bound_ms": round(bound_ms, 4),
+                                  "frac": round(bound_ms / dom[1]["mean_ms"], 3),
+                                  "note": "instruction count from a rocprofv3 SQ_INSTS_VALU pass of this workload; "
+                                          "frac = issue-bound time / measured mean"}
 
     if args.fused_only:
         print(json.dumps({"metric": METRIC, "value": round(value, 3), "unit": "mocks/s", "n_gpus": 1,
