@@ -272,6 +272,15 @@ def test_rsd_conversion_factor(tmp_path, units, z, factor):
         assert float(d["rsd_conversion_factor"]) == pytest.approx(0.01, rel=1e-7)
 
 
+def test_velocity_column_rules(tmp_path):
+    """A velocity column is read only at a positive index (Catalog.cpp:2306,
+    `if(i_vz>0)`), and redshift_space_coords_g is shown with it."""
+    assert "rsd_conversion_factor" not in _show(_ini(tmp_path, i_v3_g="0"))
+    assert "rsd_conversion_factor" not in _show(_ini(tmp_path))
+    d = _show(_ini(tmp_path, i_v3_g="4", redshift_space_coords_g="true"))
+    assert d["redshift_space_coords_g"] == "true"
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("coords_in_rsd", [False, True])
 def test_cli_redshift_space_catalogue(tmp_path, orc, coords_in_rsd):
