@@ -363,15 +363,15 @@ __global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned l
                                                     unsigned *__restrict__ offsets, unsigned *__restrict__ item_off,
                                                     unsigned *__restrict__ counts, uint4 *__restrict__ items,
                                                     unsigned *__restrict__ nitems, unsigned cap,
-                                                    unsigned *__restrict__ overflow) {
+                                                    unsigned max_items, unsigned *__restrict__ overflow) {
     const int tile = blockIdx.x * 256 + threadIdx.x;
     const int nblk = (ntiles + 255) / 256;
     if (tile == 0) {
-        const unsigned long long tot = bsum[2 * nblk];
+        const unsigned long long tot = bsum[2 * nblk], nit = bsum[2 * nblk + 1];
         offsets[ntiles] = (unsigned)min(tot, (unsigned long long)cap);
-        item_off[ntiles] = (unsigned)bsum[2 * nblk + 1];
-        *nitems = (unsigned)bsum[2 * nblk + 1];
-        if (tot > cap) atomicOr(overflow, 1u);
+        item_off[ntiles] = (unsigned)min(nit, (unsigned long long)max_items);
+        *nitems = (unsigned)min(nit, (unsigned long long)max_items);
+        if (tot > cap || nit > max_items) atomicOr(overflow, 1u);
     }
     if (tile >= ntiles) return;
     const int blk = tile / 256;
@@ -386,6 +386,7 @@ __global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned l
     const unsigned nch = cnt == 0 ? 1u : (cnt + kTileCH - 1) / kTileCH;
     const unsigned split = cnt > (unsigned)kTileCH ? 1u : 0u;
     for (unsigned k = 0; k < nch; ++k) {
+        if ((unsigned long long)ib + k >= max_items) break;  // reported through overflow above
         const unsigned long long b = min(tb64 + (unsigned long long)k * kTileCH, (unsigned long long)cap);
         const unsigned long long e = min(min(te64, tb64 + (unsigned long long)(k + 1) * kTileCH), (unsigned long long)cap);
         items[ib + k] = make_uint4((unsigned)tile, (unsigned)b, (unsigned)e, split);
@@ -844,7 +845,10 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
     t.tx0 = tx0;
     t.nxl = nxl;
     const int ntiles = nxl * t.ntx * t.ntx;
-    const size_t max_copies = p.n * (t.R > 0 ? 8 : 1);
+    // a particle is copied to at most two tiles per axis: below when its
+    // stencil reaches R cells under the tile edge, above when it reaches Rup
+    // cells over it (Rup = 1 for NGP with the shifted mesh)
+    const size_t max_copies = p.n * (t.R > 0 || t.Rup > 0 ? 8 : 1);
     const int nblk = (ntiles + 255) / 256;
     if (nblk > 4096) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many tiles");
     // LDS-privatised histogram and cursors when a workgroup's LDS holds one
@@ -906,7 +910,7 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
     if (w && (s = scratch(ctx, "mas.wsorted", sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK) return s;
     if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3(nblk), dim3(256), 0, ntiles,
                     (const unsigned long long *)bsum, (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts,
-                    (uint4 *)items, (unsigned *)nitems, (unsigned)cap, overflow)) != CM_OK)
+                    (uint4 *)items, (unsigned *)nitems, (unsigned)cap, (unsigned)max_items, overflow)) != CM_OK)
         return s;
     if (lds_hist) {
         if ((s = launch(ctx, "mas_tile_scatter",

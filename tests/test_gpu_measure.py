@@ -212,3 +212,31 @@ def test_measure_interlaced_grouped_binning(orc, scheme, hist):
     denom = np.abs(ref0) + SN
     assert np.max(np.abs(out["pk0"] - ref0) / denom) < 1e-4
     assert np.max(np.abs(out["pk2"] - p2) / denom) < 1e-4
+
+
+def test_measure_interlaced_ngp_dense_tiles(ctx, orc):
+    """NGP with the shifted mesh copies a particle to the tile above when it sits
+    in a tile's last cell plane, so a tile holds more records than particles.  A
+    dense small mesh (N = 32: 8 tiles of 16^3, ~30k records each) splits every
+    tile over several deposit workgroups; the records must be bounded by the
+    ghost copies, and the estimate-sized record buffer must not overflow (so
+    the third call replays a captured graph instead of the exact eager path)."""
+    import cosmicatlas_amd as cm
+    N, L, n = 32, 100.0, 200000
+    rng = np.random.default_rng(77)
+    pos = rng.uniform(0.0, L, (n, 3)).astype(np.float32)
+    b = cm.binning(N, L)
+    with cm.Context(0) as c:
+        pd = torch.from_numpy(pos).cuda()
+        outs = [c.measure(pd, N, L, 0, b["nshells"], b["kmin"], b["dk"], interlace=True) for _ in range(3)]
+        captured, launched = c.graph_counts()
+    # call 1 eager, call 2 captured and launched, call 3 launched
+    assert (captured, launched) == (1, 2), (captured, launched)
+    ref0, p2, nm, SN, _, nobj, _, _ = _interlaced_ref(orc, pos, N, L, 0, b)
+    assert nobj == n
+    for out in outs:
+        assert out["n_obj"] == n
+        assert np.array_equal(out["nmodes"], nm)
+        denom = np.abs(ref0) + SN
+        assert np.max(np.abs(out["pk0"] - ref0) / denom) < 1e-4
+        assert np.max(np.abs(out["pk2"] - p2) / denom) < 1e-4
