@@ -247,7 +247,21 @@ class Context:
         st = _abi.TwebStats()
         o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale)
         self._check(self.lib.cm_tweb(self.h, C.byref(o), _ptr(delta), _ptr(labels), _ptr(lam), C.byref(st)))
-        return labels, lam, {"counts": list(st.counts), "n_marginal": st.n_marginal}
+        return labels, lam, _tweb_stats(st)
+
+    def tweb_f64(self, delta, L, R, lambda_th, margin_tol=0.0, delta_scale=1.0, want_lambdas=False,
+                 want_phi=False):
+        """FP64 T-web of a float64 delta: (labels, lambdas or None, phi or None, stats)."""
+        torch = _torch()
+        N = delta.shape[0]
+        labels = torch.empty((N, N, N), dtype=torch.int8, device=delta.device)
+        lam = torch.empty((3, N, N, N), dtype=torch.float64, device=delta.device) if want_lambdas else None
+        phi = torch.empty((N, N, N), dtype=torch.float64, device=delta.device) if want_phi else None
+        st = _abi.TwebStats()
+        o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale)
+        self._check(self.lib.cm_tweb_f64(self.h, C.byref(o), _ptr(delta), _ptr(labels), _ptr(lam), _ptr(phi),
+                                         C.byref(st)))
+        return labels, lam, phi, _tweb_stats(st)
 
     def tweb_from_phi(self, phi, L, lambda_th, margin_tol=0.0, want_lambdas=False):
         torch = _torch()
@@ -258,7 +272,7 @@ class Context:
         o = self._tweb_opts(N, L, 0.0, lambda_th, margin_tol, 1.0)
         self._check(self.lib.cm_tweb_from_phi(self.h, C.byref(o), _ptr(phi), _ptr(labels), _ptr(lam),
                                               C.byref(st)))
-        return labels, lam, {"counts": list(st.counts), "n_marginal": st.n_marginal}
+        return labels, lam, _tweb_stats(st)
 
     def measure_fkp(self, pos, rand, N, L, scheme, nshells, kmin, dk_bin, weights=None, rweights=None, nbar=0.0,
                     P_est=0.0, fkp_weight=False, window=False, mas_correction=True, sn_correction=True,
@@ -364,7 +378,16 @@ class Context:
                                         _ptr(labels), C.byref(info)))
         return {"pk0": p0, "pk2": p2, "pk4": p4, "nmodes": nm, "labels": labels,
                 "n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
-                "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal}
+                "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal,
+                "tweb_tol": info.tweb.margin_tol}
+
+
+def _tweb_stats(st):
+    return {"counts": list(st.counts), "n_marginal": st.n_marginal, "margin_tol": st.margin_tol}
+
+
+# margin_tol asking for the rounding tolerance K(N) u max|phi| / d^2
+TOL_ROUNDING = -1.0
 
 
 def _fkp_result(p0, p2, p4, nm, pw, info):

@@ -61,7 +61,7 @@ class TwebOpts(C.Structure):
 
 
 class TwebStats(C.Structure):
-    _fields_ = [("counts", C.c_int64 * 4), ("n_marginal", C.c_int64)]
+    _fields_ = [("counts", C.c_int64 * 4), ("n_marginal", C.c_int64), ("margin_tol", C.c_double)]
 
 
 class MeasureOpts(C.Structure):
@@ -199,6 +199,7 @@ def _declare(L):
         "cm_tweb": (st, [P, C.POINTER(TwebOpts), P, P, P, C.POINTER(TwebStats)]),
         "cm_poisson": (st, [P, C.POINTER(TwebOpts), P, P]),
         "cm_tweb_from_phi": (st, [P, C.POINTER(TwebOpts), P, P, P, C.POINTER(TwebStats)]),
+        "cm_tweb_f64": (st, [P, C.POINTER(TwebOpts), P, P, P, P, C.POINTER(TwebStats)]),
         "cm_measure": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, P, P, P, P,
                             C.POINTER(MeasureInfo)]),
         "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),

@@ -150,8 +150,10 @@ cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool f
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double subtract, double *part, int pitch = 0, const void *phase = nullptr);
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);
+// sq (float32 only, nullable): per-workgroup sums of squares of the output,
+// fft_rows_blocks() FP64 words.
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
-                       double scale, int pitch = 0);
+                       double scale, int pitch = 0, double *sq = nullptr);
 
 // exact = false (tiled path only, needs overflow_out): the record buffer is
 // sized from an estimate instead of a read-back of the count (no host
@@ -168,14 +170,16 @@ cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, do
 
 // Device results of one fused measurement in one block, so that a single copy
 // reads them back: shell sums[4 ns], mode counts[ns], object count, T-web
-// counters[5] (4 classes, marginal cells), deposit overflow flag (one 8-byte
-// word each for the last three groups).
+// counters[5] (4 classes, marginal cells), deposit overflow flag, ||phi||_rms
+// (float bits) of the T-web potential (one 8-byte word each for the last four
+// groups).
 struct FusedOut {
     double *sums = nullptr;
     unsigned long long *cnts = nullptr;
     double *nobj = nullptr;
     unsigned long long *stats = nullptr;
     unsigned *overflow = nullptr;
+    unsigned *phirms = nullptr;
     void *base = nullptr;
     size_t bytes = 0;
 };

@@ -224,7 +224,8 @@ template <int N>
 __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *__restrict__ in,
                                                                  float *__restrict__ out, int pitch, float scale,
                                                                  const float2 *__restrict__ twM,
-                                                                 const float2 *__restrict__ twN) {
+                                                                 const float2 *__restrict__ twN,
+                                                                 double *__restrict__ sq) {
     using RS = RowShape<N>;
     constexpr int M = RS::M, NB = RS::NB, STRIDE = RS::STRIDE, NT = RS::NT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -260,8 +261,26 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
     };
     float2 *orow = reinterpret_cast<float2 *>(out + row0 * (size_t)N);
     const float s2 = 2.0f * scale;
-    auto dst = [&](int b, int m, float2 z) { orow[b * M + m] = float2{s2 * z.x, s2 * z.y}; };
+    double acc = 0.0;  // sum of squares of this thread's outputs
+    auto dst = [&](int b, int m, float2 z) {
+        const float2 o = float2{s2 * z.x, s2 * z.y};
+        orow[b * M + m] = o;
+        acc += (double)o.x * o.x + (double)o.y * o.y;
+    };
     ct_fft<float, M, NB, STRIDE, false, false>(lds, twM, src, dst);
+    if (sq) {  // per-workgroup sum of squares (one FP64 word per workgroup)
+        __shared__ double red[(NT + 63) / 64];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < (NT + 63) / 64; ++w) t += red[w];
+            sq[blockIdx.x] = t;
+        }
+    }
 }
 
 template <int N, int NB, bool FWD>
@@ -294,11 +313,11 @@ static cm_status rows_r2c_ct(cm_ctx *ctx, const float *in, float2 *out, int pitc
 
 template <int N>
 static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitch, float scale, const float2 *twM,
-                             const float2 *twN) {
+                             const float2 *twN, double *sq) {
     using RS = RowShape<N>;
     if (RS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: LDS");
     return launch(ctx, "fft_z_c2r", k_rows_c2r_ct<N>, dim3((unsigned)((size_t)N * N / RS::NB)), dim3(RS::NT), RS::LDS,
-                  in, out, pitch, scale, twM, twN);
+                  in, out, pitch, scale, twM, twN, sq);
 }
 
 template <int N, bool FWD>
@@ -346,7 +365,8 @@ static cm_status rows_r2c_f32(cm_ctx *ctx, int N, const void *in, void *out, int
     return fail(ctx, CM_ERR_UNSUPPORTED, "fft: N=%d", N);
 }
 
-static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int pitch, double scale) {
+static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int pitch, double scale,
+                              double *sq) {
     const float2 *twM, *twN;
     cm_status s = tw_pair(ctx, N, &twM, &twN);
     if (s != CM_OK) return s;
@@ -354,7 +374,7 @@ static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int
     auto *op = (float *)out;
     switch (N) {
 #define CM_CASE(n) \
-    case n: return rows_c2r_ct<n>(ctx, ip, op, pitch, (float)scale, twM, twN);
+    case n: return rows_c2r_ct<n>(ctx, ip, op, pitch, (float)scale, twM, twN, sq);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -528,8 +548,10 @@ int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt) {
     return (N * N) / row_cfg(N, dt, 0).nb;
 }
 
-cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale, int pitch) {
-    if (dt == CM_F32) return rows_c2r_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, scale);
+cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale, int pitch,
+                       double *sq) {
+    if (dt == CM_F32) return rows_c2r_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, scale, sq);
+    if (sq) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: sum of squares of an FP64 c2r");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return rows_c2r_t<double>(ctx, N, in, out, scale);
 }

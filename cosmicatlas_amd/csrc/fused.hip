@@ -255,8 +255,10 @@ __global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ p
 }
 
 PkK make_pkk(const cm_pk_opts *o);
-cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
+cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
                             float *lam, unsigned long long *stats);
+TolK tweb_tol(int N, double L, double margin_tol, bool fp64, const unsigned *norm);
+cm_status phi_rms_final(cm_ctx *ctx, const double *part, int np, size_t n, unsigned *out);
 
 template <int N, int NB, int E>
 static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pitch,
@@ -307,7 +309,7 @@ static cm_status x_fused_launch(cm_ctx *ctx, int nb, int xe, float2 *dk, const f
 }
 
 cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f) {
-    const size_t words = 5 * (size_t)ns + 7;
+    const size_t words = 5 * (size_t)ns + 8;
     void *b;
     cm_status s;
     if ((s = scratch(ctx, "f.out", 8 * words, &b)) != CM_OK) return s;
@@ -319,6 +321,7 @@ cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f) {
     f->nobj = (double *)(w + 5 * (size_t)ns);
     f->stats = w + 5 * (size_t)ns + 1;
     f->overflow = (unsigned *)(w + 5 * (size_t)ns + 6);
+    f->phirms = (unsigned *)(w + 5 * (size_t)ns + 7);
     return CM_OK;
 }
 
@@ -336,6 +339,7 @@ cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host)
     host->nobj = (double *)(w + 5 * (size_t)ns);
     host->stats = w + 5 * (size_t)ns + 1;
     host->overflow = (unsigned *)(w + 5 * (size_t)ns + 6);
+    host->phirms = (unsigned *)(w + 5 * (size_t)ns + 7);
     return CM_OK;
 }
 
@@ -454,8 +458,15 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
         void *phi;
         if ((s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
         if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false, pitch)) != CM_OK) return s;
-        if ((s = fft_rows_c2r(ctx, N, CM_F32, dk, phi, 1.0 / (double)n3, pitch)) != CM_OK) return s;
-        if ((s = tweb_fused_launch(ctx, N, o->mas.L, (const float *)phi, o->lambda_th, o->margin_tol, labels, nullptr,
+        // ||phi||_rms (for a tolerance derived from it): sums of squares ride in
+        // the c2r pass (the z pass's partial sums are consumed by now)
+        const bool rms = o->margin_tol < 0.0;
+        if ((s = fft_rows_c2r(ctx, N, CM_F32, dk, phi, 1.0 / (double)n3, pitch, rms ? (double *)part : nullptr)) !=
+            CM_OK)
+            return s;
+        if (rms && (s = phi_rms_final(ctx, (const double *)part, nparts, n3, out->phirms)) != CM_OK) return s;
+        const TolK tk = tweb_tol(N, o->mas.L, o->margin_tol, false, rms ? out->phirms : nullptr);
+        if ((s = tweb_fused_launch(ctx, N, o->mas.L, (const float *)phi, o->lambda_th, tk, labels, nullptr,
                                    (unsigned long long *)dstats)) != CM_OK)
             return s;
     }

@@ -8,6 +8,7 @@
 //   FFT; inverse y and c2r z passes; the marching FD-Hessian/label kernel.  The
 //   object count stays on the device until the sums are read at the end.
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "cm_internal.hpp"
@@ -65,6 +66,17 @@ static void finalize(const cm_measure_opts *o, double nobj_h, const std::vector<
 // Read back the fused results (one copy) and finalise; *redo is set when the
 // tiled deposit's record estimate was too small (results discarded, the caller
 // repeats with exact sizing).
+TolK tweb_tol(int N, double L, double margin_tol, bool fp64, const unsigned *norm);
+
+// The absolute tolerance a fused T-web applied, from the ||phi||_rms it read.
+static double tweb_tol_applied(int N, double L, double margin_tol, unsigned norm_bits) {
+    const TolK t = tweb_tol(N, L, margin_tol, false, nullptr);
+    if (margin_tol >= 0.0) return t.tol;
+    float pn;
+    std::memcpy(&pn, &norm_bits, sizeof pn);
+    return t.per_norm * (double)pn;
+}
+
 static cm_status finish_fused(cm_ctx *ctx, const cm_measure_opts *o, const FusedOut &dev, double *pk0, double *pk2,
                               double *pk4, int64_t *nmodes, cm_measure_info *info, bool *redo) {
     const int ns = o->nshells;
@@ -76,6 +88,7 @@ static cm_status finish_fused(cm_ctx *ctx, const cm_measure_opts *o, const Fused
     if (info) {
         for (int i = 0; i < 4; ++i) info->tweb.counts[i] = o->do_tweb ? (int64_t)h.stats[i] : 0;
         info->tweb.n_marginal = o->do_tweb ? (int64_t)h.stats[4] : 0;
+        info->tweb.margin_tol = o->do_tweb ? tweb_tol_applied(o->mas.N, o->mas.L, o->margin_tol, *h.phirms) : 0.0;
     }
     const std::vector<double> hs(h.sums, h.sums + 4 * (size_t)ns);
     const std::vector<unsigned long long> hc(h.cnts, h.cnts + (size_t)ns);

@@ -6,6 +6,7 @@
 // A20 = D_x G_z, A21 = D_y G_z (Miscelanious.cpp:1684-1740).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "cm_internal.hpp"
@@ -30,9 +31,19 @@ __global__ void __launch_bounds__(256) k_kmul(typename std::conditional<sizeof(T
     }
 }
 
-__global__ void __launch_bounds__(256) k_grad3(const float *__restrict__ phi, int N, double ih,
-                                               float *__restrict__ gx, float *__restrict__ gy,
-                                               float *__restrict__ gz) {
+// G_a = D_a phi, stored in the potential's type (float32: the stencil in
+// double, rounded to float, as the reference's float build; FP64: all double).
+template <typename T>
+__device__ __forceinline__ T fd4_t(T fll, T fl, T fr, T frr, double ih) {
+    if constexpr (sizeof(T) == 4)
+        return fd4(fll, fl, fr, frr, ih);
+    else
+        return fd4d(fll, fl, fr, frr, ih);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_grad3(const T *__restrict__ phi, int N, double ih, T *__restrict__ gx,
+                                               T *__restrict__ gy, T *__restrict__ gz) {
     const size_t n3 = (size_t)N * N * N;
     const size_t NN = (size_t)N;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += (size_t)gridDim.x * blockDim.x) {
@@ -41,19 +52,21 @@ __global__ void __launch_bounds__(256) k_grad3(const float *__restrict__ phi, in
         const int xm2 = (x - 2 + N) % N, xm1 = (x - 1 + N) % N, xp1 = (x + 1) % N, xp2 = (x + 2) % N;
         const int ym2 = (y - 2 + N) % N, ym1 = (y - 1 + N) % N, yp1 = (y + 1) % N, yp2 = (y + 2) % N;
         const int zm2 = (z - 2 + N) % N, zm1 = (z - 1 + N) % N, zp1 = (z + 1) % N, zp2 = (z + 2) % N;
-        gx[c] = fd4(at(xm2, y, z), at(xm1, y, z), at(xp1, y, z), at(xp2, y, z), ih);
-        gy[c] = fd4(at(x, ym2, z), at(x, ym1, z), at(x, yp1, z), at(x, yp2, z), ih);
-        gz[c] = fd4(at(x, y, zm2), at(x, y, zm1), at(x, y, zp1), at(x, y, zp2), ih);
+        gx[c] = fd4_t<T>(at(xm2, y, z), at(xm1, y, z), at(xp1, y, z), at(xp2, y, z), ih);
+        gy[c] = fd4_t<T>(at(x, ym2, z), at(x, ym1, z), at(x, yp1, z), at(x, yp2, z), ih);
+        gz[c] = fd4_t<T>(at(x, y, zm2), at(x, y, zm1), at(x, y, zp1), at(x, y, zp2), ih);
     }
 }
 
-__global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, const float *__restrict__ gy,
-                                                  const float *__restrict__ gz, int N, double ih,
-                                                  double lth, double tol, int8_t *__restrict__ labels,
-                                                  float *__restrict__ lam, unsigned long long *__restrict__ stats) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_hess_eig(const T *__restrict__ gx, const T *__restrict__ gy,
+                                                  const T *__restrict__ gz, int N, double ih, double lth, TolK tk,
+                                                  int8_t *__restrict__ labels, T *__restrict__ lam,
+                                                  unsigned long long *__restrict__ stats) {
     __shared__ unsigned cnt[5];
     if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
     __syncthreads();
+    const double tol = tol_value(tk);
     const size_t n3 = (size_t)N * N * N;
     const size_t NN = (size_t)N;
     unsigned loc[5] = {0, 0, 0, 0, 0};
@@ -64,23 +77,23 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
         const int zm2 = (z - 2 + N) % N, zm1 = (z - 1 + N) % N, zp1 = (z + 1) % N, zp2 = (z + 2) % N;
         auto id = [&](int i, int j, int k) { return ((size_t)i * NN + j) * NN + k; };
         // T_xx = D_x G_x, A10 = D_x G_y, A20 = D_x G_z
-        const float txx = fd4(gx[id(xm2, y, z)], gx[id(xm1, y, z)], gx[id(xp1, y, z)], gx[id(xp2, y, z)], ih);
-        const float tyx = fd4(gy[id(xm2, y, z)], gy[id(xm1, y, z)], gy[id(xp1, y, z)], gy[id(xp2, y, z)], ih);
-        const float tzx = fd4(gz[id(xm2, y, z)], gz[id(xm1, y, z)], gz[id(xp1, y, z)], gz[id(xp2, y, z)], ih);
+        const T txx = fd4_t<T>(gx[id(xm2, y, z)], gx[id(xm1, y, z)], gx[id(xp1, y, z)], gx[id(xp2, y, z)], ih);
+        const T tyx = fd4_t<T>(gy[id(xm2, y, z)], gy[id(xm1, y, z)], gy[id(xp1, y, z)], gy[id(xp2, y, z)], ih);
+        const T tzx = fd4_t<T>(gz[id(xm2, y, z)], gz[id(xm1, y, z)], gz[id(xp1, y, z)], gz[id(xp2, y, z)], ih);
         // T_yy = D_y G_y, A21 = D_y G_z
-        const float tyy = fd4(gy[id(x, ym2, z)], gy[id(x, ym1, z)], gy[id(x, yp1, z)], gy[id(x, yp2, z)], ih);
-        const float tzy = fd4(gz[id(x, ym2, z)], gz[id(x, ym1, z)], gz[id(x, yp1, z)], gz[id(x, yp2, z)], ih);
+        const T tyy = fd4_t<T>(gy[id(x, ym2, z)], gy[id(x, ym1, z)], gy[id(x, yp1, z)], gy[id(x, yp2, z)], ih);
+        const T tzy = fd4_t<T>(gz[id(x, ym2, z)], gz[id(x, ym1, z)], gz[id(x, yp1, z)], gz[id(x, yp2, z)], ih);
         // T_zz = D_z G_z
-        const float tzz = fd4(gz[id(x, y, zm2)], gz[id(x, y, zm1)], gz[id(x, y, zp1)], gz[id(x, y, zp2)], ih);
+        const T tzz = fd4_t<T>(gz[id(x, y, zm2)], gz[id(x, y, zm1)], gz[id(x, y, zp1)], gz[id(x, y, zp2)], ih);
         double e1, e2, e3;
         eig3(txx, tyy, tzz, tyx, tzx, tzy, e1, e2, e3);
         const int v = (e1 > lth) + (e2 > lth) + (e3 > lth);
         const int lab = 4 - v;
         labels[c] = (int8_t)lab;
         if (lam) {
-            lam[c] = (float)e1;
-            lam[n3 + c] = (float)e2;
-            lam[2 * n3 + c] = (float)e3;
+            lam[c] = (T)e1;
+            lam[n3 + c] = (T)e2;
+            lam[2 * n3 + c] = (T)e3;
         }
         loc[lab - 1]++;
         const double m = fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth)));
@@ -92,6 +105,38 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
     if (threadIdx.x < 5 && cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
 }
 
+// Per-workgroup FP64 sums of squares of a[0..n) into part[blockIdx.x].
+template <typename T>
+__global__ void __launch_bounds__(256) k_sumsq(const T *__restrict__ a, size_t n, double *__restrict__ part) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = (double)a[i];
+        s += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// *out = bits of the float sqrt(sum(part[0..np)) / n) (one workgroup).
+__global__ void __launch_bounds__(1024) k_rms_final(const double *__restrict__ part, int np, double n,
+                                                    unsigned *__restrict__ out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w) t += red[w];
+        *out = __float_as_uint((float)sqrt(t / n));
+    }
+}
 
 // Fused FD Hessian + labels.  One workgroup owns a TY x TZ column of cells in
 // (y,z) and a segment of x planes and marches along x, one barrier per plane:
@@ -111,33 +156,29 @@ __global__ void __launch_bounds__(256) k_hess_eig(const float *__restrict__ gx, 
 // D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z (Miscelanious.cpp:1684-1700).
 // Labels from the inertia of T - lambda_th I: float signs with a certified
 // rounding bound, recounted in FP64 in the kernel where the float signs are
-// not certain.  Cells whose margin bound is below tol are appended to the
-// workgroup's own slice of a list (INL = false; one LDS atomic per wave) that
-// k_tweb_rare settles with FP64 eigenvalues afterwards, so the marching loop makes no calls (a call site
-// inside it costs ~35 VGPRs of saved registers); with the lambda output
-// (INL = true) every cell takes the out-of-line FP64 branch.
-// Rare path of the fused kernel (kept out of line so its FP64 registers do not
-// count against the marching loop): closed-form eigenvalues, optional lambda
-// output, and the exact margin test.  Returns 1 for a marginal cell.
-// The fused kernel passes T' = T/ih^2 and ih2 = ih^2.
-__device__ __attribute__((noinline)) unsigned eig_margin(float txx, float tyy, float tzz, float a10, float a20,
-                                                         float a21, float ih2, double lth, double tol, bool test,
-                                                         float *lam, size_t cell, size_t n3) {
+// not certain.  The same coefficients, shifted by -+ tol, give the sign
+// counts of T - (lambda_th -+ tol) I: the cell is marginal when they differ
+// (an eigenvalue lies within tol of lambda_th), decided in FP64 when a shifted
+// sign is not certain either.  The marching loop makes no calls except for the
+// lambda output (INL = true), which takes the out-of-line FP64 branch for
+// every cell (a call site inside the loop costs ~35 VGPRs of saved registers).
+
+// Lambda output of the fused kernel (kept out of line so its FP64 registers do
+// not count against the marching loop): closed-form FP64 eigenvalues of T =
+// T' ih2 (the kernel works with T' = T/ih^2), descending.
+__device__ __attribute__((noinline)) void eig_store(float txx, float tyy, float tzz, float a10, float a20, float a21,
+                                                    float ih2, float *lam, size_t cell, size_t n3) {
     double e1, e2, e3;
     eig3(txx * ih2, tyy * ih2, tzz * ih2, a10 * ih2, a20 * ih2, a21 * ih2, e1, e2, e3);
-    if (lam) {
-        lam[cell] = (float)e1;
-        lam[n3 + cell] = (float)e2;
-        lam[2 * n3 + cell] = (float)e3;
-    }
-    return (test && fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth))) < tol) ? 1u : 0u;
+    lam[cell] = (float)e1;
+    lam[n3 + cell] = (float)e2;
+    lam[2 * n3 + cell] = (float)e3;
 }
 
-// FP64 count of positive eigenvalues of B (the uncertain float case).
-__device__ __attribute__((noinline)) int positive_eigs_d(float b00, float b11, float b22, float a10, float a20,
-                                                         float a21) {
-    double ad, f2;
-    return positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+// FP64 decision of the uncertain float case, out of line (lambda variant).
+__device__ __attribute__((noinline)) int positive_eigs_band_d(float b00, float b11, float b22, float a10, float a20,
+                                                              float a21, float tol, bool &band) {
+    return positive_eigs_band64(b00, b11, b22, a10, a20, a21, tol, band);
 }
 
 template <int TY, int TZ>
@@ -147,10 +188,8 @@ constexpr size_t tweb_lds_floats() {
 
 template <int TY, int TZ, int NT, bool INL>
 __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi, int N, int segLen, float ih2,
-                                                   double lth, double tol, int8_t *__restrict__ labels,
-                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats,
-                                                   unsigned *__restrict__ rare, unsigned *__restrict__ nrare) {
-    // rare: this workgroup's slice of TY*TZ*segLen entries; nrare[wg]: its count
+                                                   double lth, TolK tk, int8_t *__restrict__ labels,
+                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats) {
     constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
     constexpr int GZ = TZ + 4, GP = (TY + 4) * GZ;         // G plane (halo 2)
     constexpr int IP = TY * TZ / NT;                       // interior cells per thread
@@ -161,17 +200,15 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     float *ring = sm;             // 5 phi planes
     float *sgy = ring + 5 * HP;   // 3 G_y planes
     float *sgz = sgy + 3 * GP;    // 3 G_z planes
-    __shared__ unsigned scnt[5], s_nrare;
+    __shared__ unsigned scnt[5];
     const int tid = threadIdx.x;
     if (tid < 5) scnt[tid] = 0;
-    if (tid == 0) s_nrare = 0;
-    const unsigned wg = blockIdx.y * gridDim.x + blockIdx.x;
-    unsigned *myrare = INL ? nullptr : rare + (size_t)wg * ((size_t)TY * TZ * segLen);
     const int ntz = N / TZ;
     const int ty0 = (blockIdx.x / ntz) * TY, tz0 = (blockIdx.x % ntz) * TZ;
     const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
     // T' = T/ih^2: threshold and margin scaled to match
+    const double tol = tol_value(tk);
     const float lthf = (float)(lth / (double)ih2);
     const float tols = (float)(tol / (double)ih2);
 
@@ -278,58 +315,36 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
             t[4] = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
             t[5] = fd4s(gz[q - 2 * GZ], gz[q - GZ], gz[q + GZ], gz[q + 2 * GZ]);
         };
-        struct T6 {
-            float t[6];
-        };
-        auto tt_of = [&](int k) {
-            T6 r;
-            tensor(k, r.t);
-            return r;
-        };
         int v[IP];
-        bool unc[IP], mm[IP];
+        bool unc[IP];
         bool any = INL && lam != nullptr;
 #pragma unroll
         for (int k = 0; k < IP; ++k) {
             float t[6];
             tensor(k, t);
-            // margin pre-filter (a conservative root bound); the FP64
-            // eigenvalues settle the listed cells
-            v[k] = positive_eigs_nb(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], unc[k], tols, mm[k]);
-            any = any || unc[k] || mm[k];
+            // sign count and margin band, certified in float
+            bool bd;
+            v[k] = positive_eigs_band(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], tols, unc[k], bd);
+            nmarg += (!unc[k] && bd) ? 1u : 0u;
+            any = any || unc[k];
         }
         if (__builtin_expect(any, 0)) {
 #pragma unroll
             for (int k = 0; k < IP; ++k) {
-                if (!(unc[k] || (INL && (mm[k] || lam)))) continue;
-                const size_t cell = (size_t)c * NN * NN + (size_t)coff[k];
-                if constexpr (INL) {  // T is recomputed here, so it is not live across the common path
-                    float t[6];
-                    tensor(k, t);
-                    if (unc[k]) v[k] = positive_eigs_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5]);
-                    if (lam || mm[k])
-                        nmarg += eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, mm[k], lam, cell, n3);
-                } else if (unc[k]) {
-                    const T6 r = tt_of(k);
-                    v[k] = positive_eigs_nb64(r.t[0] - lthf, r.t[1] - lthf, r.t[2] - lthf, r.t[3], r.t[4], r.t[5], tols,
-                                              mm[k]);
+                if (!(unc[k] || (INL && lam))) continue;
+                // T is recomputed here, so it is not live across the common path
+                float t[6];
+                tensor(k, t);
+                if (unc[k]) {
+                    bool bd;
+                    if constexpr (INL)
+                        v[k] = positive_eigs_band_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], tols, bd);
+                    else
+                        v[k] = positive_eigs_band64(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], tols, bd);
+                    nmarg += bd ? 1u : 0u;
                 }
-            }
-            if constexpr (!INL) {
-                // margin candidates to the list: one atomic per wave per cell slot
-#pragma unroll
-                for (int k = 0; k < IP; ++k) {
-                    const unsigned long long bal = __ballot(mm[k]);
-                    if (bal) {
-                        const int lane = (int)__lane_id();
-                        const int leader = __ffsll((long long)bal) - 1;
-                        unsigned base = 0;
-                        if (lane == leader) base = atomicAdd(&s_nrare, (unsigned)__popcll(bal));
-                        base = __shfl(base, leader);
-                        if (mm[k])
-                            myrare[base + (unsigned)__popcll(bal & ((1ull << lane) - 1ull))] =
-                                (unsigned)((size_t)c * NN * NN + (size_t)coff[k]);
-                    }
+                if constexpr (INL) {
+                    if (lam) eig_store(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lam, (size_t)c * NN * NN + coff[k], n3);
                 }
             }
         }
@@ -362,49 +377,6 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     if (nmarg) atomicAdd(&scnt[4], nmarg);
     __syncthreads();
     if (tid < 5 && scnt[tid]) atomicAdd(&stats[tid], (unsigned long long)scnt[tid]);
-    if (!INL && tid == 0) nrare[wg] = s_nrare;
-}
-
-// Settles the margin candidates listed by k_tweb_fused<..., false>:
-// recomputes their T from phi with the same float operations in the same
-// order (G = D phi, then D G; periodic wrap), so the float tensor is the one
-// the marching kernel saw, and counts the cells whose FP64 eigenvalues lie
-// within tol of lambda_th.  Block b reads the slice of marching workgroup b
-// (slice entries, nrare[b] of them used).
-__global__ void __launch_bounds__(256) k_tweb_rare(const float *__restrict__ phi, int N, float ih2, double lth,
-                                                   double tol, unsigned long long *__restrict__ stats,
-                                                   const unsigned *__restrict__ rare, const unsigned *__restrict__ nrare,
-                                                   size_t slice) {
-    const unsigned n = nrare[blockIdx.x];
-    const unsigned *list = rare + (size_t)blockIdx.x * slice;
-    const size_t NN = (size_t)N, n3 = NN * NN * NN;
-    const unsigned Nu = (unsigned)N;
-    unsigned marg = 0;
-    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
-        const unsigned cell = list[i];  // N^3 <= 2^30
-        const int z = (int)(cell % Nu), y = (int)((cell / Nu) % Nu), x = (int)(cell / (Nu * Nu));
-        // wrapped coordinates c-4..c+4 along each axis, as plane/row offsets
-        unsigned xo[9], yo[9], zo[9];
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            xo[d] = (unsigned)((x + d - 4 + N) % N) * Nu * Nu;
-            yo[d] = (unsigned)((y + d - 4 + N) % N) * Nu;
-            zo[d] = (unsigned)((z + d - 4 + N) % N);
-        }
-        auto P = [&](int dx, int dy, int dz) { return phi[xo[dx + 4] + yo[dy + 4] + zo[dz + 4]]; };
-        auto gx = [&](int a, int b, int c) { return fd4s(P(a - 2, b, c), P(a - 1, b, c), P(a + 1, b, c), P(a + 2, b, c)); };
-        auto gy = [&](int a, int b, int c) { return fd4s(P(a, b - 2, c), P(a, b - 1, c), P(a, b + 1, c), P(a, b + 2, c)); };
-        auto gz = [&](int a, int b, int c) { return fd4s(P(a, b, c - 2), P(a, b, c - 1), P(a, b, c + 1), P(a, b, c + 2)); };
-        float t[6];
-        t[0] = fd4s(gx(-2, 0, 0), gx(-1, 0, 0), gx(1, 0, 0), gx(2, 0, 0));  // txx
-        t[1] = fd4s(gy(0, -2, 0), gy(0, -1, 0), gy(0, 1, 0), gy(0, 2, 0));  // tyy
-        t[2] = fd4s(gz(0, 0, -2), gz(0, 0, -1), gz(0, 0, 1), gz(0, 0, 2));  // tzz
-        t[3] = fd4s(gy(-2, 0, 0), gy(-1, 0, 0), gy(1, 0, 0), gy(2, 0, 0));  // a10 = D_x G_y
-        t[4] = fd4s(gz(-2, 0, 0), gz(-1, 0, 0), gz(1, 0, 0), gz(2, 0, 0));  // a20 = D_x G_z
-        t[5] = fd4s(gz(0, -2, 0), gz(0, -1, 0), gz(0, 1, 0), gz(0, 2, 0));  // a21 = D_y G_z
-        marg += eig_margin(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lth, tol, true, nullptr, cell, n3);
-    }
-    if (marg) atomicAdd(&stats[4], (unsigned long long)marg);
 }
 
 __global__ void k_zero_u64(unsigned long long *a, int n) {
@@ -420,7 +392,7 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
 
 
 template <int TY, int TZ, int NT>
-static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol,
+static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol,
                                 int8_t *labels, float *lam, unsigned long long *stats) {
     const bool inl = lam != nullptr;
     auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
@@ -451,25 +423,14 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
     const double ih = 1.0 / (12.0 * (L / (double)N));
     const float ih2 = (float)(ih * ih);
     cm_status s;
-    void *rare = nullptr, *nrare = nullptr;
-    const size_t nwg = (size_t)tiles * nseg, slice = (size_t)TY * TZ * segLen;
-    if (!inl) {
-        if ((s = scratch(ctx, "tweb.rare", sizeof(unsigned) * nwg * slice, &rare)) != CM_OK) return s;
-        if ((s = scratch(ctx, "tweb.nrare", sizeof(unsigned) * nwg, &nrare)) != CM_OK) return s;
-    }
     if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
-    if ((s = launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats,
-                    (unsigned *)rare, (unsigned *)nrare)) != CM_OK)
-        return s;
-    if (inl) return CM_OK;
-    return launch(ctx, "tweb_rare", k_tweb_rare, dim3((unsigned)nwg), dim3(256), 0, phi, N, ih2, lth, tol, stats,
-                  (const unsigned *)rare, (const unsigned *)nrare, slice);
+    return launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats);
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.
 // Tile (TY x TZ cells, threads): 16x64/512 when 64 | N, else 8x32/256,
 // 8x16/128 or 8x8/64 by the largest of 32, 16, 8 dividing N.
-cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, double tol, int8_t *labels,
+cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
                             float *lam, unsigned long long *stats) {
     int TZ = N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : (N % 16 == 0 ? 16 : (N % 8 == 0 ? 8 : 0)));
     if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 && ctx->tune.tz <= TZ &&
@@ -484,63 +445,140 @@ cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, doub
     return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
 }
 
-// phi = c2r(W(k)(-1/k^2) r2c(delta)*scale), float32.
-cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi, double subtract) {
+// Coefficient K(N) of the rounding tolerance K u ||phi||_rms / d^2: how far
+// rounding of the potential's arithmetic (unit roundoff u) may move an
+// eigenvalue of the FD tensor, from a random-rounding model.
+//   * The FFT chain (r2c, multiplier, c2r of N^3 points; phi rounded on store)
+//     leaves phi with a white error of rms g u ||phi||_rms, g = sqrt(log2 N^3):
+//     each of the log2 N^3 radix-2-equivalent stages adds an independent
+//     rounding of relative size at most u.
+//   * To first order an eigenvalue moves by v^T dT v = (v.D)^2 dphi, the
+//     second difference of the error along the eigenvector v.  The stencil of
+//     (v.D)^2 (4th-order gradfindif composed with itself) has an L2 norm of at
+//     most 1.5 / d^2 over all directions (1.12 / d^2 along an axis, 1.43 / d^2
+//     along a diagonal), so white noise of rms e moves it by rms <= 1.5 e / d^2.
+//   * The largest of the 3 N^3 eigenvalue errors of a Gaussian model lies
+//     below z = sqrt(2 ln(3 N^3)) standard deviations.
+// K(N) = 1.5 sqrt(log2 N^3) sqrt(2 ln(3 N^3)): 46.8 at N = 512.  The stencil
+// sums of the float32 T-web round in proportion to differences of nearby phi
+// values, far below the rounding of phi itself.
+double tweb_tol_coeff(int N) {
+    const double n3 = (double)N * N * N;
+    return 1.5 * std::sqrt(std::log2(n3)) * std::sqrt(2.0 * std::log(3.0 * n3));
+}
+
+TolK tweb_tol(int N, double L, double margin_tol, bool fp64, const unsigned *norm) {
+    TolK t{margin_tol, 0.0, nullptr};
+    if (margin_tol < 0.0) {
+        const double d = L / (double)N, u = fp64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+        t.tol = 0.0;
+        t.per_norm = -margin_tol * tweb_tol_coeff(N) * u / (d * d);
+        t.norm = norm;
+    }
+    return t;
+}
+
+// ||phi||_rms of n values from per-workgroup sums of squares, as float bits.
+cm_status phi_rms_final(cm_ctx *ctx, const double *part, int np, size_t n, unsigned *out) {
+    return launch(ctx, "tweb_rms", k_rms_final, dim3(1), dim3(1024), 0, part, np, (double)n, out);
+}
+
+cm_status tweb_zero_words(cm_ctx *ctx, unsigned long long *p, int n) {
+    return launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, p, n);
+}
+
+// phi = c2r(W(k)(-1/k^2) r2c(delta)*scale) in the potential's type.
+template <typename T>
+static cm_status poisson_t(cm_ctx *ctx, const cm_tweb_opts *o, const T *delta, T *phi, double subtract) {
+    using C2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    constexpr cm_dtype DT = sizeof(T) == 4 ? CM_F32 : CM_F64;
     const int N = o->N;
     const size_t nh = (size_t)N * N * (N / 2 + 1);
     void *dk;
     cm_status s;
-    if ((s = scratch(ctx, "tweb.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
-    if ((s = fft_r2c_impl(ctx, N, CM_F32, delta, dk, subtract)) != CM_OK) return s;
+    if ((s = scratch(ctx, DT == CM_F32 ? "tweb.dk" : "tweb.dk64", sizeof(C2) * nh, &dk)) != CM_OK) return s;
+    if ((s = fft_r2c_impl(ctx, N, DT, delta, dk, subtract)) != CM_OK) return s;
     const double kf = 2.0 * M_PI / o->L;
     const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
     const double scale = o->delta_scale == 0.0 ? 1.0 : o->delta_scale;
-    if ((s = launch(ctx, "tweb_kmul", k_kmul<float>, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, (float2 *)dk, N,
-                    kf, R2h, scale)) != CM_OK)
+    if ((s = launch(ctx, "tweb_kmul", k_kmul<T>, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, (C2 *)dk, N, kf, R2h,
+                    scale)) != CM_OK)
         return s;
     const double n3 = (double)N * N * N;
-    return fft_c2r_impl(ctx, N, CM_F32, dk, phi, 1.0 / n3);
+    return fft_c2r_impl(ctx, N, DT, dk, phi, 1.0 / n3);
 }
 
-cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
-                             cm_tweb_stats *st, bool plain) {
+cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi, double subtract) {
+    return poisson_t<float>(ctx, o, delta, phi, subtract);
+}
+
+// Labels, optional lambdas and the counters from phi.  plain: separate
+// gradient and Hessian+eigen kernels (float32 or FP64); otherwise the fused
+// marching kernel (float32).  A negative margin_tol takes max|phi| from one
+// reduction over phi first.
+template <typename T>
+static cm_status tweb_from_phi_t(cm_ctx *ctx, const cm_tweb_opts *o, const T *phi, int8_t *labels, T *lam,
+                                 cm_tweb_stats *st, bool plain) {
     const int N = o->N;
     const size_t n3 = (size_t)N * N * N;
+    constexpr bool F64 = sizeof(T) == 8;
     void *stats;
     cm_status s;
-    if ((s = scratch(ctx, "tweb.stats", sizeof(unsigned long long) * 5, &stats)) != CM_OK) return s;
-    const double ih = 1.0 / (12.0 * (o->L / (double)N));
-    if (plain) {
-        void *g;
-        if ((s = scratch(ctx, "tweb.grad", sizeof(float) * 3 * n3, &g)) != CM_OK) return s;
-        float *gx = (float *)g, *gy = gx + n3, *gz = gy + n3;
-        unsigned gr = grid_for(n3, 256, 65536);
-        if ((s = launch(ctx, "tweb_grad", k_grad3, dim3(gr), dim3(256), 0, phi, N, ih, gx, gy, gz)) != CM_OK) return s;
-        if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, (unsigned long long *)stats, 5)) != CM_OK)
+    // counters[5] and ||phi||_rms (one word each)
+    if ((s = scratch(ctx, "tweb.stats", sizeof(unsigned long long) * 6, &stats)) != CM_OK) return s;
+    auto *w = (unsigned long long *)stats;
+    auto *norm = (unsigned *)(w + 5);
+    if ((s = tweb_zero_words(ctx, w, 6)) != CM_OK) return s;
+    const unsigned gr = grid_for(n3, 256, 65536);
+    if (o->margin_tol < 0.0) {
+        constexpr unsigned NP = 2048;
+        void *part;
+        if ((s = scratch(ctx, "tweb.sq", sizeof(double) * NP, &part)) != CM_OK) return s;
+        const unsigned g = std::min(gr, NP);
+        if ((s = launch(ctx, "tweb_sumsq", k_sumsq<T>, dim3(g), dim3(256), 0, phi, n3, (double *)part)) != CM_OK)
             return s;
-        if ((s = launch(ctx, "tweb_hess_eig", k_hess_eig, dim3(gr), dim3(256), 0, (const float *)gx,
-                        (const float *)gy, (const float *)gz, N, ih, o->lambda_th, o->margin_tol, labels, lam,
-                        (unsigned long long *)stats)) != CM_OK)
+        if ((s = phi_rms_final(ctx, (const double *)part, (int)g, n3, norm)) != CM_OK) return s;
+    }
+    const TolK tk = tweb_tol(N, o->L, o->margin_tol, F64, norm);
+    const double ih = 1.0 / (12.0 * (o->L / (double)N));
+    if (plain || F64) {
+        void *g;
+        if ((s = scratch(ctx, F64 ? "tweb.grad64" : "tweb.grad", sizeof(T) * 3 * n3, &g)) != CM_OK) return s;
+        T *gx = (T *)g, *gy = gx + n3, *gz = gy + n3;
+        if ((s = launch(ctx, "tweb_grad", k_grad3<T>, dim3(gr), dim3(256), 0, phi, N, ih, gx, gy, gz)) != CM_OK)
+            return s;
+        if ((s = launch(ctx, "tweb_hess_eig", k_hess_eig<T>, dim3(gr), dim3(256), 0, (const T *)gx, (const T *)gy,
+                        (const T *)gz, N, ih, o->lambda_th, tk, labels, lam, w)) != CM_OK)
             return s;
     } else {
-        if ((s = tweb_fused_launch(ctx, N, o->L, phi, o->lambda_th, o->margin_tol, labels, lam,
-                                   (unsigned long long *)stats)) != CM_OK)
+        if ((s = tweb_fused_launch(ctx, N, o->L, (const float *)phi, o->lambda_th, tk, labels, (float *)lam, w)) !=
+            CM_OK)
             return s;
     }
     if (st) {
-        unsigned long long h[5];
+        unsigned long long h[6];
         CM_HIP(ctx, hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
         CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (int i = 0; i < 4; ++i) st->counts[i] = (int64_t)h[i];
         st->n_marginal = (int64_t)h[4];
+        float pn;
+        const unsigned bits = (unsigned)h[5];
+        std::memcpy(&pn, &bits, sizeof pn);
+        st->margin_tol = tk.norm ? tk.per_norm * (double)pn : tk.tol;
     }
     return CM_OK;
+}
+
+cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
+                             cm_tweb_stats *st, bool plain) {
+    return tweb_from_phi_t<float>(ctx, o, phi, labels, lam, st, plain);
 }
 
 static cm_status check_opts(cm_ctx *ctx, const cm_tweb_opts *o) {
     if (!o) return fail(ctx, CM_ERR_INVALID, "tweb: null options");
     if (!fft_size_ok(o->N)) return fail(ctx, CM_ERR_INVALID, "tweb: N=%d not supported by the FFT", o->N);
     if (!(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "tweb: L must be > 0");
+    if (!std::isfinite(o->margin_tol)) return fail(ctx, CM_ERR_INVALID, "tweb: margin_tol must be finite");
     return CM_OK;
 }
 
@@ -578,6 +616,19 @@ cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t
     if ((s = scratch(ctx, "tweb.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
     if ((s = poisson_impl(ctx, o, delta, (float *)phi, 0.0)) != CM_OK) return s;
     return tweb_from_phi_impl(ctx, o, (const float *)phi, labels, lambdas, stats, false);
+}
+
+cm_status cm_tweb_f64(cm_ctx *ctx, const cm_tweb_opts *o, const double *delta, int8_t *labels, double *lambdas,
+                      double *phi_out, cm_tweb_stats *stats) {
+    if (!ctx) return CM_ERR_INVALID;
+    cm_status s = check_opts(ctx, o);
+    if (s != CM_OK) return s;
+    if (!delta || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb_f64: null pointer");
+    void *phi = phi_out;
+    const size_t n3 = (size_t)o->N * o->N * o->N;
+    if (!phi && (s = scratch(ctx, "tweb.phi64", sizeof(double) * n3, &phi)) != CM_OK) return s;
+    if ((s = poisson_t<double>(ctx, o, delta, (double *)phi, 0.0)) != CM_OK) return s;
+    return tweb_from_phi_t<double>(ctx, o, (const double *)phi, labels, lambdas, stats, true);
 }
 
 }  // extern "C"

@@ -58,6 +58,25 @@ __device__ __forceinline__ void eig3(double a00, double a11, double a22, double 
     e2 = 3.0 * q - e1 - e3;
 }
 
+// The gradfindif stencil in the FP64 T-web (the reference's DOUBLE_PREC build).
+__device__ __forceinline__ double fd4d(double fll, double fl, double fr, double frr, double ih) {
+    return (-frr + 8.0 * fr - 8.0 * fl + fll) * ih;
+}
+
+// Margin tolerance of a T-web call.  tol >= 0 is absolute.  A negative tol
+// asks for the rounding tolerance |tol| K(N) u ||phi||_rms / d^2 of the
+// potential's arithmetic (u = 2^-24 for float32 phi, 2^-53 for FP64), taken on
+// the device from ||phi||_rms (a float's bits in *norm, written by a reduction
+// over phi), so no host synchronisation is needed; per_norm = |tol| K u / d^2.
+struct TolK {
+    double tol;
+    double per_norm;
+    const unsigned *norm;
+};
+__device__ __forceinline__ double tol_value(const TolK &t) {
+    return t.norm ? t.per_norm * (double)__uint_as_float(*t.norm) : t.tol;
+}
+
 // The same stencil in float (fused kernel): 3 FMAs and a multiply.
 __device__ __forceinline__ float fd4f(float fll, float fl, float fr, float frr, float ih) {
     return ((fll - frr) + 8.0f * (fr - fl)) * ih;
@@ -192,6 +211,72 @@ __device__ __forceinline__ int positive_eigs_nb64(float b00, float b11, float b2
     const double det_lo = ad - t3, tl = (double)tol;
     const double thr = fmax(fmax(2.0 * tl * (fabs(e2) + t2), 4.0 * tl * tl * (fabs(e1) + t1)), 4.0 * tl * tl * tl);
     marg = tol > 0.f && det_lo < thr * 1.0001;
+    return v;
+}
+
+// FP64 count of positive eigenvalues of B from float entries (inline).
+__device__ __forceinline__ int positive_eigs_count64(float b00, float b11, float b22, float a10, float a20,
+                                                     float a21) {
+    double ad, f2;
+    return positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+}
+
+// Sign count of B and its margin band from one set of coefficients.  With
+// e1, e2, e3 the coefficients of B's characteristic polynomial
+// t^3 - e1 t^2 + e2 t - e3, those of B - sI are
+//   e1 - 3s,  e2 - s (2 e1 - 3s),  e3 - s (e2 - s (e1 - s)),
+// so #{mu > s} for s = -tol and s = +tol costs a few FMAs; the cell is
+// marginal (an eigenvalue of B in (-tol, tol]) when the two counts differ.
+// Rounding: with S = sc + 3 tol (sc = sum of |entries| of B) the shifted
+// coefficients carry the errors of e1..e3 (8u sc, 16u sc^2, 24u sc^3, each
+// bounded by the same form in S) propagated through the shift, plus the
+// rounding of the shift itself; 16u S, 32u S^2 and 48u S^3 bound both with
+// room to spare.  unc is set when any of the nine signs is not certain (FP64
+// decides: positive_eigs of B and of B -+ tol I).  tol = 0: no band test.
+__device__ __forceinline__ int positive_eigs_band(float b00, float b11, float b22, float a10, float a20, float a21,
+                                                  float tol, bool &unc, bool &band) {
+    const float e1 = b00 + b11 + b22;
+    const float q10 = a10 * a10, q20 = a20 * a20, q21 = a21 * a21;
+    const float e2 = (b00 * b11 + b00 * b22 + b11 * b22) - (q10 + q20 + q21);
+    const float c0 = b11 * b22 - q21, c1 = a10 * b22 - a21 * a20, c2 = a10 * a21 - b11 * a20;
+    const float e3 = b00 * c0 - a10 * c1 + a20 * c2;
+    const float sc = (fabsf(b00) + fabsf(b11) + fabsf(b22)) + (fabsf(a10) + fabsf(a20) + fabsf(a21));
+    const float u = 5.9604645e-8f;
+    const float t1 = 8.0f * u * sc, t2 = 2.0f * t1 * sc, t3 = 1.5f * t2 * sc;
+    unc = (fabsf(e1) <= t1) || (fabsf(e2) <= t2) || (fabsf(e3) <= t3) || !(sc < 1e12f);
+    const int v = (e1 > 0.f) + ((e1 < 0.f) != (e2 > 0.f)) + ((e2 > 0.f) != (e3 < 0.f));
+    band = false;
+    if (tol > 0.f) {
+        const float S = sc + 3.0f * tol;
+        const float g1 = 16.0f * u * S, g2 = 2.0f * g1 * S, g3 = 1.5f * g2 * S;
+        int cnt[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float sh = i ? tol : -tol;
+            const float f1 = e1 - 3.0f * sh;
+            const float f2 = e2 - sh * (2.0f * e1 - 3.0f * sh);
+            const float f3 = e3 - sh * (e2 - sh * (e1 - sh));
+            unc = unc || (fabsf(f1) <= g1) || (fabsf(f2) <= g2) || (fabsf(f3) <= g3);
+            cnt[i] = (f1 > 0.f) + ((f1 < 0.f) != (f2 > 0.f)) + ((f2 > 0.f) != (f3 < 0.f));
+        }
+        band = cnt[0] != cnt[1];
+    }
+    return v;
+}
+
+// FP64 decision of positive_eigs_band for a cell whose float signs are not
+// all certain: the count of B and the band test from B -+ tol I in FP64.
+__device__ __forceinline__ int positive_eigs_band64(float b00, float b11, float b22, float a10, float a20,
+                                                    float a21, float tol, bool &band) {
+    double ad, f2;
+    const int v = positive_eigs(b00, b11, b22, a10, a20, a21, ad, f2);
+    band = false;
+    if (tol > 0.f) {
+        const double t = tol;
+        const int lo = positive_eigs((double)b00 + t, (double)b11 + t, (double)b22 + t, a10, a20, a21, ad, f2);
+        const int hi = positive_eigs((double)b00 - t, (double)b11 - t, (double)b22 - t, a10, a20, a21, ad, f2);
+        band = lo != hi;
+    }
     return v;
 }
 

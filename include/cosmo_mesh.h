@@ -168,13 +168,29 @@ typedef struct {
     double L;
     double R_smooth;     /* Gaussian smoothing radius (smscale, Params.cpp:572); <=0: none */
     double lambda_th;    /* lambdath (Params.cpp:552) */
-    double margin_tol;   /* cells with min_j |lambda_j - lambda_th| < tol are counted */
+    double margin_tol;   /* cells with min_j |lambda_j - lambda_th| < tol are counted:
+                            tol = margin_tol when >= 0; a negative value -s asks
+                            for s times the rounding tolerance
+                            K(N) u ||phi||_rms / d^2 (see CM_TWEB_TOL_ROUNDING) */
     double delta_scale;  /* input multiplied by this before the Poisson solve (1/mean for rho-mean) */
 } cm_tweb_opts;
 
+/* margin_tol value that counts the cells whose label the rounding of the
+ * potential's arithmetic can change: tol = K(N) u ||phi||_rms / d^2 with
+ * d = L/N, u the unit roundoff of phi (2^-24 float32, 2^-53 FP64) and
+ * K(N) = 1.5 sqrt(log2 N^3) sqrt(2 ln(3 N^3)) (46.8 at N = 512), from a
+ * random-rounding model: the FFT chain leaves phi a white error of rms
+ * sqrt(log2 N^3) u ||phi||_rms; an eigenvalue moves by the second difference
+ * of that error along its eigenvector, a stencil of L2 norm <= 1.5/d^2; and
+ * sqrt(2 ln(3 N^3)) standard deviations bound the largest of the 3 N^3
+ * eigenvalue errors.  ||phi||_rms is reduced on the device (no host
+ * synchronisation); cm_tweb_stats.margin_tol returns the value applied. */
+#define CM_TWEB_TOL_ROUNDING (-1.0)
+
 typedef struct {
     int64_t counts[4];   /* cells labelled knot, filament, sheet, void (1..4) */
-    int64_t n_marginal;  /* cells whose eigenvalue margin is below margin_tol */
+    int64_t n_marginal;  /* cells whose eigenvalue margin is below the tolerance */
+    double margin_tol;   /* the absolute tolerance applied */
 } cm_tweb_stats;
 
 /* Replaces Cwclass::get_CWC (Source/Cwclass.cpp:235-894) with the Gaussian
@@ -187,6 +203,15 @@ typedef struct {
  * 3*N^3 (planes l1|l2|l3, descending) or NULL; stats: host, may be NULL. */
 cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t *labels,
                   float *lambdas, cm_tweb_stats *stats);
+
+/* FP64 T-web: Cwclass::get_CWC of the reference's DOUBLE_PREC build
+ * (real_prec = double, Headers/def.h:594-595; Source/Cwclass.cpp:235-894), the
+ * same steps as cm_tweb with every step in double: FP64 r2c, multiplier, c2r,
+ * FD stencils (NumericalMethods.cpp:1123-1217) and closed-form eigenvalues.
+ * delta: device double N^3; labels: device int8 N^3; lambdas: device double
+ * 3*N^3 or NULL; phi_out: device double N^3 receiving the potential, or NULL. */
+cm_status cm_tweb_f64(cm_ctx *ctx, const cm_tweb_opts *o, const double *delta, int8_t *labels,
+                      double *lambdas, double *phi_out, cm_tweb_stats *stats);
 
 /* Replaces PoissonSolver (Source/Miscelanious.cpp:1189-1274) with optional
  * Gaussian smoothing: phi = c2r(W(k) * (-1/k^2) * r2c(delta*delta_scale)). */
@@ -216,7 +241,7 @@ typedef struct {
     int sn_correction;
     int float_shell_rule;
     int do_tweb;         /* 0: P(k) only */
-    double R_smooth, lambda_th, margin_tol;
+    double R_smooth, lambda_th, margin_tol; /* margin_tol as in cm_tweb_opts */
     int fused;           /* 1: fused pipeline; 0: the plain step-by-step GPU path */
     int interlace;       /* 1 (fused only): second mesh of the particles shifted by
                             half a cell on every axis, spectra averaged as

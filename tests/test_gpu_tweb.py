@@ -88,12 +88,12 @@ def test_tweb_tclass(codes):
         assert [int(_tclass_ref(np.array([v], np.int8), codes)[0]) for v in (1, 2, 3, 4)] == [0, 0, 1, 1]
 
 
-def test_tweb_rare_list_equals_inline_branch(ctx):
-    """Without the lambda output the marching kernel lists cells whose float
-    signs are uncertain or whose margin bound is below tol, and a second kernel
-    settles them in FP64; with it, every cell takes the inline FP64 branch.
-    Both must give the same labels, class counts and marginal count, also when
-    a large tol lists a good part of the mesh."""
+def test_tweb_lambda_variant_equals_count_variant(ctx):
+    """Without the lambda output the marching kernel decides uncertain signs and
+    margin bands in an inline FP64 branch; with it, every cell takes the
+    out-of-line FP64 branch.  Both must give the same labels, class counts and
+    marginal count, also when a large tol makes a good part of the mesh
+    marginal."""
     N, L = 48, 100.0
     phi = torch.from_numpy(_smooth_field(N, 11).astype(np.float32)).cuda()
     for lth, frac in ((0.0, 1e-3), (0.01, 0.3)):
@@ -105,7 +105,7 @@ def test_tweb_rare_list_equals_inline_branch(ctx):
         assert sa == sb
         assert sum(sb["counts"]) == N ** 3
         if frac > 0.1:
-            assert sb["n_marginal"] > N ** 3 // 20  # the list held a good part of the mesh
+            assert sb["n_marginal"] > N ** 3 // 20  # a good part of the mesh is marginal
 
 
 def test_tweb_uncertain_signs_settled_in_fp64(ctx):
@@ -132,12 +132,11 @@ def test_tweb_uncertain_signs_settled_in_fp64(ctx):
 
 
 @pytest.mark.parametrize("frac", [1e-4, 1e-2, 0.1])
-def test_tweb_margin_prefilter_is_conservative(ctx, frac):
+def test_tweb_marginal_count_matches_eigenvalues(ctx, frac):
     """The marginal count (cells with an eigenvalue within tol of lambda_th)
-    comes from an exact FP64 test applied to the cells a float root bound lets
-    through; the bound must let every marginal cell through.  Reference: the
-    same count taken from the returned eigenvalues, up to cells within float
-    rounding of the tol boundary."""
+    comes from the sign counts of T - (lambda_th -+ tol) I, certified in float
+    or decided in FP64.  Reference: the same count taken from the returned
+    eigenvalues, up to cells within float rounding of the tol boundary."""
     N, L, lth = 48, 100.0, 0.005
     phi = torch.from_numpy(_smooth_field(N, 21).astype(np.float32)).cuda()
     _, lam, _ = ctx.tweb_from_phi(phi, L, lth, want_lambdas=True)
