@@ -270,7 +270,8 @@ class Cwclass {
     std::vector<int> CWClass;
     std::vector<real_prec> lambda1, lambda2, lambda3;
     real_prec volume_knots = 0, volume_filaments = 0, volume_sheets = 0, volume_voids = 0;
-    int64_t n_marginal = 0;
+    int64_t n_marginal = 0;   // cells with an eigenvalue within margin_tol of lambdath
+    double margin_tol = 0.0;  // the tolerance applied (float32 rounding of phi)
 
    private:
     Params params;

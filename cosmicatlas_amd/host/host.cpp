@@ -359,8 +359,10 @@ void Cwclass::get_CWC(std::vector<real_prec> &delta) {
     const size_t n3 = (size_t)N * N * N;
     DevBuf dd(sizeof(real_prec) * n3), dlab(n3), dlam(3 * sizeof(real_prec) * n3);
     cmh::h2d(dd.p, delta.data(), sizeof(real_prec) * n3);
-    cm_tweb_opts o{(int)N, (double)params._Lbox(), (double)params._smscale(), (double)params._lambdath(), 1e-6,
-                   1.0};
+    // margin: the cells whose label the float32 rounding of the potential can
+    // change are counted (tol = K(N) u ||phi||_rms / d^2)
+    cm_tweb_opts o{(int)N, (double)params._Lbox(), (double)params._smscale(), (double)params._lambdath(),
+                   CM_TWEB_TOL_ROUNDING, 1.0};
     cm_tweb_stats st;
     check(cm_tweb(ctx(), &o, (const float *)dd.p, (int8_t *)dlab.p, (float *)dlam.p, &st), "cm_tweb");
     std::vector<int8_t> lab(n3);
@@ -379,6 +381,7 @@ void Cwclass::get_CWC(std::vector<real_prec> &delta) {
     volume_sheets = (real_prec)st.counts[2] / (real_prec)n3;
     volume_voids = (real_prec)st.counts[3] / (real_prec)n3;
     n_marginal = st.n_marginal;
+    margin_tol = st.margin_tol;
 }
 
 // =============================================================== FftwFunctions

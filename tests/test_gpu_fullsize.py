@@ -3,7 +3,8 @@
 properties that need no CPU pass at these sizes: float32 against a float64 mesh,
 FFT and shells on the GPU (the P(k) tolerance 1e-4), equal mode counts, the
 particle count, invariance under a translation by whole cells, and agreement of
-the fused and plain paths."""
+the fused and plain paths; the T-web labels against an FP64 T-web of the same
+particles, every differing label inside the counted margin band."""
 import numpy as np
 import pytest
 import torch
@@ -24,7 +25,7 @@ def ctx():
 def _measure(ctx, pos, N_, L_, ndel, fused=True, tweb=True):
     import cosmicatlas_amd as cm
     b = cm.binning(N_, L_, ndel=ndel)
-    kw = dict(do_tweb=True, R_smooth=2.0, lambda_th=0.0, margin_tol=1e-6) if tweb else {}
+    kw = dict(do_tweb=True, R_smooth=2.0, lambda_th=0.0, margin_tol=cm.TOL_ROUNDING) if tweb else {}
     return ctx.measure(pos, N_, L_, "TSC", b["nshells"], b["kmin"], b["dk"], fused=fused, **kw), b
 
 
@@ -56,6 +57,58 @@ def test_fullsize_pk_against_fp64(ctx, mock):
     ok = nm > 0
     # relative to the power before shot-noise subtraction (what the FFT computes)
     assert np.max(np.abs(out["pk0"][ok] - p0[ok]) / np.abs(raw[ok])) < 1e-4
+
+
+def test_fullsize_tweb_labels_against_fp64(ctx, mock):
+    """Labels equal the FP64 T-web (FP64 deposit, FFTs, FD Hessian and
+    eigenvalues) except for cells whose eigenvalue margin lies below the
+    rounding tolerance K(N) u ||phi||_rms / d^2, and those cells are counted.
+    The float32 T-web of cm_tweb_from_phi returns its eigenvalues, so there the
+    claim is checked cell by cell: every differing label has a float32 margin
+    below tol, the counted cells are those with such a margin, and tol bounds
+    the eigenvalue error everywhere.  The fused measurement keeps no
+    eigenvalues: every label it gets wrong must have an FP64 margin below tol
+    (an eigenvalue on the other side of lambda_th), and there are no more such
+    labels than counted cells."""
+    import cosmicatlas_amd as cm
+    pos, out, b = mock
+    n = pos.shape[0]
+    m64 = ctx.mesh_assign(pos, N, L, "TSC", algo="plain", mesh_dtype=torch.float64)
+    ctx.mesh_affine(m64, N ** 3 / n, -1.0)
+    lab64, lam64, _, st64 = ctx.tweb_f64(m64, L, 2.0, 0.0, margin_tol=cm.TOL_ROUNDING, want_lambdas=True)
+    del m64
+    assert sum(st64["counts"]) == N ** 3
+    marg64 = lam64.abs().min(dim=0).values
+    m32 = ctx.mesh_assign(pos, N, L, "TSC")
+    ctx.mesh_affine(m32, N ** 3 / n, -1.0)
+    phi32 = ctx.poisson(m32, L, R=2.0)
+    del m32
+    lab32, lam32, st32 = ctx.tweb_from_phi(phi32, L, 0.0, margin_tol=cm.TOL_ROUNDING, want_lambdas=True)
+    del phi32
+    tol = st32["margin_tol"]
+    # the tolerance is the one the fused path applied (same ||phi||_rms to float rounding)
+    assert out["tweb_tol"] == pytest.approx(tol, rel=1e-4)
+    # the eigenvalue error itself lies below tol (the claim the tolerance makes;
+    # measured 0.38 tol at this size)
+    err = max(float((lam32[j].double() - lam64[j]).abs().max()) for j in range(3))
+    del lam64
+    assert err < tol
+    # about 3% of the cells are counted at this size; far fewer labels differ
+    assert 0 < st32["n_marginal"] < N ** 3 // 20
+    marg32 = lam32.abs().min(dim=0).values
+    del lam32
+    in_band = marg32 < tol
+    edge = int(((marg32 - tol).abs() < 1e-5 * tol).sum())
+    assert abs(int(in_band.sum()) - st32["n_marginal"]) <= edge
+    differ = lab32 != lab64
+    assert int(differ.sum()) > 0
+    assert bool(in_band[differ].all()), "a float32 label differs from FP64 outside the counted band"
+    # fused path
+    differ_f = out["labels"] != lab64
+    nf = int(differ_f.sum())
+    assert nf <= out["tweb_marginal"]
+    assert bool((marg64[differ_f] < tol).all()), "a fused label differs from FP64 with an FP64 margin above tol"
+    assert abs(out["tweb_marginal"] - st32["n_marginal"]) < 0.01 * st32["n_marginal"]
 
 
 def test_fullsize_fused_equals_plain(ctx, mock):
