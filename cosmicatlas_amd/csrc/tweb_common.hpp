@@ -223,16 +223,18 @@ __device__ __forceinline__ int positive_eigs_count64(float b00, float b11, float
 
 // Sign count of B and its margin band from one set of coefficients.  With
 // e1, e2, e3 the coefficients of B's characteristic polynomial
-// t^3 - e1 t^2 + e2 t - e3, those of B - sI are
-//   e1 - 3s,  e2 - s (2 e1 - 3s),  e3 - s (e2 - s (e1 - s)),
-// so #{mu > s} for s = -tol and s = +tol costs a few FMAs; the cell is
-// marginal (an eigenvalue of B in (-tol, tol]) when the two counts differ.
-// Rounding: with S = sc + 3 tol (sc = sum of |entries| of B) the shifted
-// coefficients carry the errors of e1..e3 (8u sc, 16u sc^2, 24u sc^3, each
-// bounded by the same form in S) propagated through the shift, plus the
-// rounding of the shift itself; 16u S, 32u S^2 and 48u S^3 bound both with
-// room to spare.  unc is set when any of the nine signs is not certain (FP64
-// decides: positive_eigs of B and of B -+ tol I).  tol = 0: no band test.
+// t^3 - e1 t^2 + e2 t - e3, det(B - sI) = e3 - s (e2 - s (e1 - s)).  An odd
+// number of eigenvalues lies in (-tol, tol) exactly when det(B - sI) has
+// opposite signs at s = -tol and s = +tol.  Two eigenvalues mu1, mu2 in the
+// band would force both |e3| = |mu1 mu2 mu3| < tol^2 m and
+// |e2| = |mu1 mu2 + mu3 (mu1 + mu2)| < tol^2 + 2 tol m with
+// m = |e1| + 2 tol > |mu3| (e1 = mu1 + mu2 + mu3), so when either exceeds its
+// bound (widened by the rounding bounds of e1..e3) the cell is marginal (an
+// eigenvalue within tol of zero) exactly when the sign changes; otherwise
+// (3e-5 of the cells of the 512^3 mock) unc is set and FP64 decides.  The two
+// determinants are taken in float as they come: a cell whose eigenvalue lies
+// within their rounding of -+ tol may be counted either way, the float
+// rounding of the boundary itself.  tol = 0: no band test.
 __device__ __forceinline__ int positive_eigs_band(float b00, float b11, float b22, float a10, float a20, float a21,
                                                   float tol, bool &unc, bool &band) {
     const float e1 = b00 + b11 + b22;
@@ -247,19 +249,12 @@ __device__ __forceinline__ int positive_eigs_band(float b00, float b11, float b2
     const int v = (e1 > 0.f) + ((e1 < 0.f) != (e2 > 0.f)) + ((e2 > 0.f) != (e3 < 0.f));
     band = false;
     if (tol > 0.f) {
-        const float S = sc + 3.0f * tol;
-        const float g1 = 16.0f * u * S, g2 = 2.0f * g1 * S, g3 = 1.5f * g2 * S;
-        int cnt[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const float sh = i ? tol : -tol;
-            const float f1 = e1 - 3.0f * sh;
-            const float f2 = e2 - sh * (2.0f * e1 - 3.0f * sh);
-            const float f3 = e3 - sh * (e2 - sh * (e1 - sh));
-            unc = unc || (fabsf(f1) <= g1) || (fabsf(f2) <= g2) || (fabsf(f3) <= g3);
-            cnt[i] = (f1 > 0.f) + ((f1 < 0.f) != (f2 > 0.f)) + ((f2 > 0.f) != (f3 < 0.f));
-        }
-        band = cnt[0] != cnt[1];
+        const float dm = e3 + tol * (e2 + tol * (e1 + tol));  // det(B + tol I)
+        const float dp = e3 - tol * (e2 - tol * (e1 - tol));  // det(B - tol I)
+        const float m = fabsf(e1) + 2.0f * tol + t1;
+        const bool two = (fabsf(e3) <= tol * tol * m + t3) && (fabsf(e2) <= tol * (tol + 2.0f * m) + t2);
+        unc = unc || two;
+        band = (dm > 0.f) != (dp > 0.f);
     }
     return v;
 }
