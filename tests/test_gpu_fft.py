@@ -16,7 +16,7 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("N", [8, 16, 24, 32, 48, 64, 96, 128, 192])
+@pytest.mark.parametrize("N", [8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384])
 def test_r2c_f32_matches_oracle(ctx, orc, N):
     rng = np.random.default_rng(N)
     x = rng.standard_normal((N, N, N)).astype(np.float32)
@@ -36,6 +36,7 @@ def test_r2c_f64_matches_oracle(ctx, orc, N):
 
 
 @pytest.mark.parametrize("N,dt", [(16, torch.float32), (48, torch.float32), (128, torch.float32),
+                                  (256, torch.float32), (384, torch.float32),
                                   (32, torch.float64), (96, torch.float64)])
 def test_c2r_roundtrip_and_oracle(ctx, orc, N, dt):
     rng = np.random.default_rng(7 + N)

@@ -241,6 +241,33 @@ def test_shot_noise_closed_form(orc):
     assert np.isclose(p0[0], -SN * (1 - 2 / 3 * s1 ** 2), rtol=1e-14)
 
 
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+def test_aliased_shot_noise_all_schemes(orc, scheme):
+    """The aliased shot-noise factor per axis, C1(x) = sum_n W^2(k + 2 pi n/d),
+    W = sinc^(p) (p = scheme + 1, the B-spline window of NGP/CIC/TSC/PCS), at
+    x = pi i/N, against its brute-force sum over 400001 images plus the
+    integral of the 1/n^2 tail for NGP (Jing 2005; the closed forms the
+    estimator uses, Miscelanious.cpp:2532-2554).  Shells [i, i + 0.05) k_f with
+    i = 2^j hold only the six axis vectors (r3(4^j) = 6, and the next integer
+    |k|^2 = i^2 + 1 lies beyond (i + 0.05)^2 for i < 10), so the shell value of a
+    mode-free field is -SN C1(pi i/N) C1(0)^2 exactly."""
+    N, L, SN = 32, 100.0, 2.5
+    kf = TWO_PI / L
+    z = np.zeros((N, N, N // 2 + 1), complex)
+    M = 200000
+    n = np.arange(-M, M + 1)
+    for i in (1, 2, 4, 8):
+        o = orc.pk_opts(N, L, 1, i * kf, 0.05 * kf, scheme, SN, 1.0, mas_correction=False)
+        p0, _, _, nm = orc.pk_shells(z, o)
+        assert nm[0] == 3  # the +x, +y, +z axis modes of the half spectrum
+        x = np.pi * i / N
+        y = x + np.pi * n
+        brute = np.sum((np.sin(y) / y) ** (2 * (scheme + 1)))
+        if scheme == 0:  # sum over |n| > M of sin^2 x / (pi n)^2
+            brute += np.sin(x) ** 2 * 2.0 / (np.pi ** 2 * (M + 0.5))
+        np.testing.assert_allclose(p0[0], -SN * brute, rtol=1e-9)
+
+
 # --------------------------------------------------------------------------- T-web
 def test_eigen3_matches_eigvalsh(orc):
     rng = np.random.default_rng(7)
