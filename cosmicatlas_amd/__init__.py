@@ -213,12 +213,12 @@ class Context:
 
     # ------------------------------------------------------------------- P(k)
     def pk_shells(self, dk, N, L, nshells, kmin, dk_bin, scheme, shot_noise, normal_power,
-                  mas_correction=True, float_shell_rule=False):
+                  mas_correction=True, float_shell_rule=False, log_binning=False, dlog=0.0):
         torch = _torch()
         code = CM_F32 if dk.dtype == torch.complex64 else CM_F64
         o = _abi.PkOpts(N, float(L), nshells, float(kmin), float(dk_bin), _scheme(scheme),
                         int(mas_correction), float(shot_noise), float(normal_power),
-                        int(float_shell_rule))
+                        int(float_shell_rule), int(log_binning), float(dlog))
         p0, p2, p4 = (np.zeros(nshells) for _ in range(3))
         nm = np.zeros(nshells, np.int64)
         self._check(self.lib.cm_pk_shells(self.h, C.byref(o), code, _ptr(dk),
@@ -360,13 +360,15 @@ class Context:
     # ---------------------------------------------------------- whole mock
     def measure(self, pos, N, L, scheme, nshells, kmin, dk_bin, weights=None, mas_correction=True,
                 sn_correction=True, float_shell_rule=False, do_tweb=False, R_smooth=0.0,
-                lambda_th=0.0, margin_tol=0.0, fused=True, interlace_shift=0.0, labels=None, interlace=False):
+                lambda_th=0.0, margin_tol=0.0, fused=True, interlace_shift=0.0, labels=None, interlace=False,
+                log_binning=False, dlog=0.0):
         torch = _torch()
         n = pos.shape[0]
         o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=mas_correction,
                           sn_correction=sn_correction, float_shell_rule=float_shell_rule, do_tweb=do_tweb,
                           R_smooth=R_smooth, lambda_th=lambda_th, margin_tol=margin_tol, fused=fused,
-                          interlace_shift=interlace_shift, interlace=interlace)
+                          interlace_shift=interlace_shift, interlace=interlace, log_binning=log_binning,
+                          dlog=dlog)
         p0, p2, p4 = (np.zeros(nshells) for _ in range(3))
         nm = np.zeros(nshells, np.int64)
         if do_tweb and labels is None:
@@ -397,12 +399,12 @@ def _fkp_result(p0, p2, p4, nm, pw, info):
 
 def _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=True, sn_correction=True,
                   float_shell_rule=False, do_tweb=False, R_smooth=0.0, lambda_th=0.0, margin_tol=0.0, fused=True,
-                  interlace_shift=0.0, interlace=False):
+                  interlace_shift=0.0, interlace=False, log_binning=False, dlog=0.0):
     md = _abi.MasDesc(N, float(L), (C.c_double * 3)(0.0, 0.0, 0.0), CM_AOS, CM_F32, _scheme(scheme),
                       float(interlace_shift), CM_MAS_AUTO)
     return _abi.MeasureOpts(md, nshells, float(kmin), float(dk_bin), int(mas_correction), int(sn_correction),
                             int(float_shell_rule), int(do_tweb), float(R_smooth), float(lambda_th),
-                            float(margin_tol), int(fused), int(interlace))
+                            float(margin_tol), int(fused), int(interlace), int(log_binning), float(dlog))
 
 
 class Ticket:
@@ -518,3 +520,14 @@ def binning(N, L, ndel=1, DeltaKmin=0.0):
     kmin = DeltaKmin*k_f, dk = ndel*k_f, N_shells = N/ndel/2."""
     kf = 2.0 * math.pi / L
     return {"kmin": DeltaKmin * kf, "dk": ndel * kf, "nshells": N // ndel // 2, "kf": kf}
+
+
+def binning_log(N, L, DeltaKmin, N_log_bins):
+    """Params::derived_pars with type_of_binning = log (Source/Params.cpp:4099-4106):
+    kmin = DeltaKmin*k_f, kmax = N/2 k_f, Deltal = log10(kmax/kmin)/N_log_bins,
+    N_log_bins shells; the k of shell s is kmin 10^((s - 1/2) Deltal)."""
+    kf = 2.0 * math.pi / L
+    kmin = DeltaKmin * kf
+    dlog = math.log10((N / 2) * kf / kmin) / N_log_bins
+    return {"kmin": kmin, "dk": 0.0, "nshells": N_log_bins, "kf": kf, "dlog": dlog, "log_binning": True,
+            "k": [kmin * 10.0 ** ((s - 0.5) * dlog) for s in range(N_log_bins)]}

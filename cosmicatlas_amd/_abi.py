@@ -46,6 +46,8 @@ class PkOpts(C.Structure):
         ("shot_noise", C.c_double),
         ("normal_power", C.c_double),
         ("float_shell_rule", C.c_int),
+        ("log_binning", C.c_int),
+        ("dlog", C.c_double),
     ]
 
 
@@ -79,6 +81,8 @@ class MeasureOpts(C.Structure):
         ("margin_tol", C.c_double),
         ("fused", C.c_int),
         ("interlace", C.c_int),
+        ("log_binning", C.c_int),
+        ("dlog", C.c_double),
     ]
 
 

@@ -214,6 +214,9 @@ void pk_tables(int N, int scheme, int mas_correction, std::vector<double> &tab);
 const double *pk_tables_dev(cm_ctx *ctx, int N, int scheme, int mas_correction, cm_status *st);
 cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const void *dk, double **sums,
                         unsigned long long **cnts);
+// Argument check of a binning (linear: dk > 0; log: kmin > 0, dlog > 0).
+cm_status check_binning(cm_ctx *ctx, int nshells, double kmin, double dk, int log_binning, double dlog,
+                        int float_shell_rule, const char *who);
 void pk_finalize(int ns, double normal_power, double sn_norm, const double *sum, const unsigned long long *cnt,
                  double *pk0, double *pk2, double *pk4, int64_t *nmodes);
 

@@ -214,7 +214,9 @@ extern "C" cm_status cm_measure_fkp(cm_ctx *ctx, const cm_measure_opts *o, const
     if (!o || !f || (!pos && n) || (!rpos && nr)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: null pointer");
     if (!fft_size_ok(o->mas.N)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: N=%d not supported", o->mas.N);
     if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure_fkp: float32 positions only");
-    if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure_fkp: binning");
+    cm_status sb = check_binning(ctx, o->nshells, o->kmin, o->dk, o->log_binning, o->dlog, o->float_shell_rule,
+                                 "cm_measure_fkp");
+    if (sb != CM_OK) return sb;
     if (o->do_tweb || o->interlace)
         return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure_fkp: the T-web and interlacing options must be off");
     TuneScope ts(ctx, o->mas.N);

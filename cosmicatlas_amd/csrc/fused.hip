@@ -152,9 +152,16 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
             const float gxy = fscale * s_g[ix] * gy;
             float2 *row = lds + ix * (NB + 1);
             ShellAcc acc{-1, 0.f, 0.f, 0.f, 0.f, 0u};
-            // integer shell rule: raw index r with (r ndel + Dmin)^2 <= q < hiq
+            // integer shell rule: raw index r with (r ndel + Dmin)^2 <= q < hiq;
+            // log rule: qedge[r] <= q < hiq = qedge[r+1]
             int r = -1, hiq = 0;
-            if (pk.int_rule && !excl) {
+            const bool tab = pk.qedge != nullptr;
+            if (tab && !excl) {
+                const int q0 = qxy + kz0 * kz0;
+                r = q0 < pk.qedge[0] ? -1 : log_shell(pk.qedge, ns, q0);
+                if (r < 0 && q0 >= pk.qedge[0]) r = ns;  // beyond the last shell
+                hiq = r + 1 <= ns ? pk.qedge[r + 1] : 0x7fffffff;
+            } else if (pk.int_rule && !excl) {
                 const int q0 = qxy + kz0 * kz0;
                 r = (int)floorf((__builtin_sqrtf((float)q0) - (float)pk.Dmin_i) * pk.rndel_f);
                 r = max(r, -1);
@@ -176,7 +183,13 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
                 const int q = qxy + iz * iz;
                 int sh = -1;
                 if (!excl && (iz > 0 || zero_ok)) {
-                    if (pk.int_rule) {
+                    if (tab) {
+                        while (q >= hiq && r < ns) {
+                            ++r;
+                            hiq = r + 1 <= ns ? pk.qedge[r + 1] : 0x7fffffff;
+                        }
+                        sh = (r >= 0 && r < ns) ? r : -1;
+                    } else if (pk.int_rule) {
                         while (q >= hiq && r < ns) {
                             ++r;
                             const int hi = (r + 1) * pk.ndel_i + pk.Dmin_i;
@@ -254,7 +267,7 @@ __global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ p
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-PkK make_pkk(const cm_pk_opts *o);
+PkK make_pkk(cm_ctx *ctx, const cm_pk_opts *o, cm_status *st);
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
                             float *lam, unsigned long long *stats);
 TolK tweb_tol(int N, double L, double margin_tol, bool fp64, const unsigned *norm);
@@ -404,7 +417,10 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     po.shot_noise = 0.0;
     po.normal_power = 1.0;
     po.float_shell_rule = o->float_shell_rule;
-    PkK pk = make_pkk(&po);
+    po.log_binning = o->log_binning;
+    po.dlog = o->dlog;
+    PkK pk = make_pkk(ctx, &po, &s);
+    if (s != CM_OK) return s;
     cm_status st;
     const void *tw = twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;

@@ -128,12 +128,12 @@ static std::string graph_key(cm_ctx *ctx, const cm_measure_opts *o, const void *
                              const int8_t *labels) {
     const cm_mas_desc &m = o->mas;
     return table_key("%d|%d|%d|%d|%.17g|%.17g,%.17g,%.17g|%.17g|%d|%d|%.17g|%.17g|%d|%d|%d|%d|%.17g|%.17g|%.17g|%d|"
-                     "%p|%p|%zu|%p|%p|%d,%d,%d,%d,%d",
+                     "%p|%p|%zu|%p|%p|%d,%d,%d,%d,%d|%d|%.17g",
                      m.N, (int)m.scheme, (int)m.layout, (int)m.pos_dtype, m.L, m.xmin[0], m.xmin[1], m.xmin[2],
                      m.interlace_shift, o->nshells, o->mas_correction, o->kmin, o->dk, o->sn_correction,
                      o->float_shell_rule, o->do_tweb, o->interlace, o->R_smooth, o->lambda_th, o->margin_tol,
                      (int)m.algo, pos, w, n, (const void *)labels, (void *)ctx->stream, ctx->tune.xnb,
-                     ctx->tune.xe, ctx->tune.tz, ctx->tune.tseg, ctx->tune.dep_wg);
+                     ctx->tune.xe, ctx->tune.tz, ctx->tune.tseg, ctx->tune.dep_wg, o->log_binning, o->dlog);
 }
 
 // *first is set on the first sighting of a key: the caller's eager run then
@@ -257,6 +257,8 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     po.shot_noise = 0.0;  // applied at finalisation from the split sums
     po.normal_power = 1.0;
     po.float_shell_rule = o->float_shell_rule;
+    po.log_binning = o->log_binning;
+    po.dlog = o->dlog;
     double *sums;
     unsigned long long *cnts;
     if ((s = pk_accumulate(ctx, &po, CM_F32, dk, &sums, &cnts)) != CM_OK) return s;
@@ -307,7 +309,9 @@ cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, co
     if (!o || (!pos && n)) return fail(ctx, CM_ERR_INVALID, "cm_measure: null pointer");
     if (!fft_size_ok(o->mas.N)) return fail(ctx, CM_ERR_INVALID, "cm_measure: N=%d not supported", o->mas.N);
     if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure: float32 positions only");
-    if (o->nshells < 1 || !(o->dk > 0)) return fail(ctx, CM_ERR_INVALID, "cm_measure: binning");
+    cm_status sb = check_binning(ctx, o->nshells, o->kmin, o->dk, o->log_binning, o->dlog, o->float_shell_rule,
+                                 "cm_measure");
+    if (sb != CM_OK) return sb;
     if (o->do_tweb && !labels) return fail(ctx, CM_ERR_INVALID, "cm_measure: do_tweb needs a labels buffer");
     TuneScope ts(ctx, o->mas.N);
     return measure_impl(ctx, o, pos, w, n, pk0, pk2, pk4, nmodes, labels, info);

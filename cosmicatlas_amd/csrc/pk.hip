@@ -175,8 +175,30 @@ static bool near_int(double v, long &out) {
     return false;
 }
 
-PkK make_pkk(const cm_pk_opts *o) {
+// Integer thresholds of the logarithmic shells (FftwFunctions.cpp:1103-1108,
+// Params.cpp:4106): the mode q = |k/k_f|^2 lies in shell
+// s = floor(log10(sqrt(q)/Dmin)/dlog) + 1 exactly when
+// (Dmin 10^((s-1) dlog))^2 <= q < (Dmin 10^(s dlog))^2, so shell s starts at
+// Q[s] = the smallest integer at or above x_s = (Dmin 10^((s-1) dlog))^2.
+// x_s is evaluated in long double; a value within 1e-9 relative of an integer
+// (e.g. (N/2)^(2m/n) = 4^m) is that integer, so a mode on an edge goes to the
+// shell above as the exact rule says.  Thresholds are capped at qmax + 1.
+void log_shell_edges(double Dmin, double dlog, int ns, long qmax, std::vector<int> &Q) {
+    Q.resize((size_t)ns + 1);
+    for (int s = 0; s <= ns; ++s) {
+        const long double e = (long double)Dmin * powl(10.0L, (long double)(s - 1) * (long double)dlog);
+        const long double x = e * e;
+        long double r = nearbyintl(x);
+        long double v = fabsl(x - r) <= 1e-9L * (x > 1.0L ? x : 1.0L) ? r : ceill(x);
+        if (v > (long double)(qmax + 1)) v = (long double)(qmax + 1);
+        Q[(size_t)s] = (int)v;
+    }
+}
+
+PkK make_pkk(cm_ctx *ctx, const cm_pk_opts *o, cm_status *st) {
+    *st = CM_OK;
     PkK p;
+    p.qedge = nullptr;
     p.N = o->N;
     p.nshells = o->nshells;
     const double kf = 2.0 * M_PI / o->L;
@@ -195,7 +217,29 @@ PkK make_pkk(const cm_pk_opts *o) {
     p.kmin_f = (float)o->kmin;
     p.rdk_f = (float)(1.0 / (double)(float)o->dk);
     p.rndel_f = p.int_rule ? (float)(1.0 / (double)p.ndel_i) : 0.0f;
+    if (o->log_binning) {
+        const long h = o->N / 2, qmax = 3 * h * h;
+        p.int_rule = 0;
+        p.float_rule = 0;
+        p.qedge = cached_table<int>(ctx, table_key("logq:%d:%d:%.17g:%.17g", o->N, o->nshells, p.Dmin, o->dlog),
+                                    [&](std::vector<int> &q) { log_shell_edges(p.Dmin, o->dlog, o->nshells, qmax, q); },
+                                    st);
+    }
     return p;
+}
+
+cm_status check_binning(cm_ctx *ctx, int nshells, double kmin, double dk, int log_binning, double dlog,
+                        int float_shell_rule, const char *who) {
+    if (nshells < 1) return fail(ctx, CM_ERR_INVALID, "%s: nshells must be >= 1", who);
+    if (log_binning) {
+        if (!(kmin > 0) || !(dlog > 0) || !std::isfinite(dlog))
+            return fail(ctx, CM_ERR_INVALID, "%s: log binning needs kmin > 0 and dlog > 0", who);
+        if (float_shell_rule)
+            return fail(ctx, CM_ERR_UNSUPPORTED, "%s: the float32 shell rule applies to linear binning", who);
+        return CM_OK;
+    }
+    if (!(dk > 0)) return fail(ctx, CM_ERR_INVALID, "%s: binning (dk must be > 0)", who);
+    return CM_OK;
 }
 
 cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const void *dk, double **sums,
@@ -211,7 +255,8 @@ cm_status pk_accumulate(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dt, const voi
     if ((s = launch(ctx, "pk_zero", k_pk_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
                     ns)) != CM_OK)
         return s;
-    PkK p = make_pkk(o);
+    PkK p = make_pkk(ctx, o, &s);
+    if (s != CM_OK) return s;
     const size_t total = (size_t)o->N * o->N * M1;
     size_t lds = sizeof(double) * (2 * (size_t)M1 + 4 * (size_t)ns) + sizeof(unsigned) * (size_t)ns;
     if (lds > ctx->lds_per_block)
@@ -252,14 +297,16 @@ extern "C" cm_status cm_pk_shells(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dty
     if (!ctx) return CM_ERR_INVALID;
     if (!o || !dk) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: null pointer");
     if (o->N < 2 || (o->N & 1)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: N=%d must be even", o->N);
-    if (o->nshells < 1) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: nshells=%d", o->nshells);
-    if (!(o->dk > 0) || !(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: dk and L must be > 0");
+    if (!(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: L must be > 0");
+    cm_status s = check_binning(ctx, o->nshells, o->kmin, o->dk, o->log_binning, o->dlog, o->float_shell_rule,
+                                "cm_pk_shells");
+    if (s != CM_OK) return s;
     if ((int)o->scheme < 0 || (int)o->scheme > 3) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: scheme");
     if (!(o->normal_power > 0) && (pk0 || pk2 || pk4))
         return fail(ctx, CM_ERR_INVALID, "cm_pk_shells: normal_power must be > 0");
     double *sums;
     unsigned long long *cnts;
-    cm_status s = pk_accumulate(ctx, o, dtype, dk, &sums, &cnts);
+    s = pk_accumulate(ctx, o, dtype, dk, &sums, &cnts);
     if (s != CM_OK) return s;
     std::vector<double> hs(4 * (size_t)o->nshells);
     std::vector<unsigned long long> hc((size_t)o->nshells);

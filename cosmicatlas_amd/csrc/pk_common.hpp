@@ -23,7 +23,21 @@ struct PkK {
     double Dmin, ndel;  // kmin/k_f, dk/k_f
     float dkx_f, kmin_f, rdk_f;
     float rndel_f;  // 1/ndel for the integer rule's estimate
+    // logarithmic shells: shell s holds q = |k/k_f|^2 with qedge[s] <= q <
+    // qedge[s+1] (device table of nshells+1 thresholds); nullptr: linear
+    const int *qedge;
 };
+
+// Shell of q under the log rule: the s with qedge[s] <= q < qedge[s+1], else -1.
+__device__ __forceinline__ int log_shell(const int *__restrict__ qedge, int ns, int q) {
+    if (q < qedge[0] || q >= qedge[ns]) return -1;
+    int lo = 0, hi = ns;  // qedge[lo] <= q < qedge[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (q >= qedge[mid]) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 // Which stored half-spectrum modes power_spectrum_fkp visits
 // (FftwFunctions.cpp:1061-1084,1180,1273,1373,1474): the loop runs i,j < N/2,
@@ -66,6 +80,7 @@ __device__ __forceinline__ int shell_of(const PkK &p, int i, int j, int k) {
         return (s < 0 || s >= p.nshells) ? -1 : s;
     }
     const int q = i * i + j * j + k * k;
+    if (p.qedge) return log_shell(p.qedge, p.nshells, q);
     if (p.int_rule) {
         // integer DeltaKmin and ndel: float estimate (hardware sqrt and a
         // reciprocal are enough), then exact integer edges

@@ -433,7 +433,9 @@ void FftwFunctions::shells(std::vector<real_prec> &p0, std::vector<real_prec> *p
     check(cm_fft_r2c(ctx(), (int)N, CM_F32, din.p, dk.p, 0.0), "cm_fft_r2c");
     if (data_out_g) cmh::d2h(data_out_g, dk.p, sizeof(complex_prec) * nh);
     const int ns = params._d_Nnp_data();
-    if (params._type_of_binning() != "linear") throw cmh::Error("log binning is not part of this build");
+    const bool logb = params._type_of_binning() == "log";
+    // the log rule divides by kmin (FftwFunctions.cpp:1003, rkmin = 1/kmin)
+    if (logb && !(params._d_kmin() > 0)) throw cmh::Error("type_of_binning = log needs DeltaKmin > 0");
     cm_pk_opts o;
     o.N = (int)N;
     o.L = params._Lbox();
@@ -444,7 +446,9 @@ void FftwFunctions::shells(std::vector<real_prec> &p0, std::vector<real_prec> *p
     o.mas_correction = params._MAS_correction();
     o.shot_noise = params._SN_correction() ? shot_noise : 0.0;
     o.normal_power = normal_power;
-    o.float_shell_rule = params._float_shell_rule();
+    o.float_shell_rule = logb ? 0 : params._float_shell_rule();
+    o.log_binning = logb ? 1 : 0;
+    o.dlog = params._d_Deltal();
     std::vector<double> a0(ns), a2(ns), a4(ns);
     std::vector<int64_t> nm(ns);
     check(cm_pk_shells(ctx(), &o, CM_F32, dk.p, a0.data(), a2.data(), a4.data(), nm.data()), "cm_pk_shells");
@@ -467,7 +471,8 @@ void FftwFunctions::get_bispectrum_fkp(char, std::vector<real_prec> &bispect, st
                                        std::vector<int> &mod) {
     const ULONG N = params._Nft();
     const size_t n3 = (size_t)N * N * N, nh = params._NGRID_h();
-    if (params._type_of_binning() != "linear") throw cmh::Error("log binning is not part of this build");
+    if (params._type_of_binning() != "linear")
+        throw cmh::Error("Bk_fkp: log binning of the bispectrum is not part of this build");
     const int nb = params._d_Nnp_data();
     if (nb > 64) throw cmh::Error("Bk_fkp: at most 64 k bins (Nft/(2 ndel_data) = " + std::to_string(nb) + ")");
     DevBuf din(sizeof(real_prec) * n3), dk(sizeof(complex_prec) * nh);
@@ -506,7 +511,8 @@ void FftwFunctions::get_bispectrum_fkp_fast(std::vector<real_prec> &pk, std::vec
                                             std::vector<int> &mod, const std::string &file) {
     const ULONG N = params._Nft();
     const size_t n3 = (size_t)N * N * N, nh = params._NGRID_h();
-    if (params._type_of_binning() != "linear") throw cmh::Error("log binning is not part of this build");
+    if (params._type_of_binning() != "linear")
+        throw cmh::Error("Bk_fkp_fast: log binning of the bispectrum is not part of this build");
     // FftwFunctions::set_params (:5064-5095): DeltaK_Bis = DeltaK_data, Nshells_bk =
     // int(kmax_bk / DeltaK_Bis), kmin_bk raised to the fundamental mode; the
     // triangle loop starts at floor((kmin_bk - kmin)/DeltaK_Bis) (:3524)
@@ -784,10 +790,14 @@ void PowerSpectrumF::set_output_filenames() {
 }
 
 void PowerSpectrumF::kvector() {
-    // PowerSpectrumF.cpp:1266-1272 / 4259-4264: bin centres kmin + dk (i+1/2)
+    // PowerSpectrumF.cpp:1266-1272 / 4259-4264: bin centres kmin + dk (i+1/2);
+    // log binning: kmin 10^((i-1/2) Deltal), the log centre of shell i
+    // (:1273-1277 loops over the just-cleared vector and writes no k at all)
     kvector_data.resize(params._d_Nnp_data());
+    const bool logb = params._type_of_binning() == "log";
     for (int i = 0; i < params._d_Nnp_data(); ++i)
-        kvector_data[i] = (real_prec)(params._d_kmin() + params._d_DeltaK_data() * (i + 0.5));
+        kvector_data[i] = (real_prec)(logb ? params._d_kmin() * std::pow(10.0, (i - 0.5) * params._d_Deltal())
+                                           : params._d_kmin() + params._d_DeltaK_data() * (i + 0.5));
 }
 
 // -p with Input_type=catalog (PowerSpectrumF.cpp:741-1363, box, no randoms)

@@ -150,7 +150,14 @@ typedef struct {
     double shot_noise;    /* SN = V/N_gal, 0 to switch off (SN_correction) */
     double normal_power;  /* N_gal^2/V (FftwFunctions::raw_sampling, :159-174) */
     int float_shell_rule; /* 1 = the reference's float32 shell index
-                             (FftwFunctions.cpp:1097); 0 = exact integer rule */
+                             (FftwFunctions.cpp:1097); 0 = exact integer rule
+                             (linear binning only) */
+    int log_binning;      /* type_of_binning = log (Source/Params.cpp:4104-4106):
+                             shell s = floor(log10(|k|/kmin)/dlog) + 1 for
+                             s = 0..nshells-1 (FftwFunctions.cpp:1103-1108),
+                             evaluated exactly (integer |k|^2 thresholds); dk
+                             is unused; kmin must be > 0 */
+    double dlog;          /* Deltal = log10(kmax/kmin)/N_log_bins */
 } cm_pk_opts;
 
 /* Replaces FftwFunctions::power_spectrum_fkp box branch
@@ -248,6 +255,8 @@ typedef struct {
                             (d1 + exp(i k.(d/2)(1,1,1)) d2)/2 and the aliased shot
                             noise (prod C1 + prod A)/2 with A the alternating
                             aliasing sum; no counterpart in the reference */
+    int log_binning;     /* logarithmic shells, as in cm_pk_opts */
+    double dlog;
 } cm_measure_opts;
 
 typedef struct {

@@ -33,6 +33,8 @@ class PkOpts(C.Structure):
         ("shot_noise", C.c_double),
         ("normal_power", C.c_double),
         ("float_shell_rule", C.c_int),
+        ("log_binning", C.c_int),
+        ("dlog", C.c_double),
     ]
 
 
@@ -122,9 +124,9 @@ def fft_1d(x, sign=-1):
 
 
 def pk_opts(N, L, nshells, kmin, dk, scheme, shot_noise, normal_power, mas_correction=True,
-            float_shell_rule=False):
+            float_shell_rule=False, log_binning=False, dlog=0.0):
     return PkOpts(N, L, nshells, kmin, dk, scheme, int(mas_correction), shot_noise,
-                  normal_power, int(float_shell_rule))
+                  normal_power, int(float_shell_rule), int(log_binning), float(dlog))
 
 
 def pk_shells(dk, opts: PkOpts):

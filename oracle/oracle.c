@@ -290,6 +290,20 @@ static int near_int(double v, long *out)
 int or_shell_index(const or_pk_opts *o, int i, int j, int k)
 {
     const double kf = 2.0 * M_PI / o->L;
+    if (o->log_binning) {
+        /* FftwFunctions.cpp:1103-1108: kv = |k|^2/kmin^2, shell
+         * floor(log10(sqrt(kv))/Deltal) + 1 for kv != 0, evaluated in FP64;
+         * a quotient within 1e-10 of an integer is that integer (a mode on a
+         * shell edge goes to the shell above, as the exact rule says). */
+        long q = (long)i * i + (long)j * j + (long)k * k;
+        if (q == 0) return -1;
+        double t = log10(sqrt((double)q) / (o->kmin / kf)) / o->dlog;
+        double r = floor(t + 0.5);
+        if (fabs(t - r) < 1e-10) t = r;
+        long s = (long)floor(t) + 1;
+        if (s < 0 || s >= o->nshells) return -1;
+        return (int)s;
+    }
     if (o->float_shell_rule) {
         /* The reference's float32 arithmetic (real_prec=float):
          * Params members rounded to float (Params.h:745,760), terms

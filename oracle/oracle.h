@@ -70,6 +70,9 @@ typedef struct {
     double normal_power;  /* N_gal^2/V */
     int    float_shell_rule; /* 1: the reference's float32 floor (FftwFunctions.cpp:1097);
                                 0: exact (FP64) rule */
+    int    log_binning;   /* type_of_binning = log: shell floor(log10(|k|/kmin)/dlog) + 1
+                             (FftwFunctions.cpp:1103-1108), exact rule */
+    double dlog;          /* Deltal (Params.cpp:4106) */
 } or_pk_opts;
 
 /* power_spectrum_fkp box branch (Source/FftwFunctions.cpp:958-1657): enumerates

@@ -144,13 +144,15 @@ def _interlaced_ref(orc, pos, N, L, scheme, b):
     ph = np.exp(1j * np.pi * (sj[:, None, None] + sj[None, :, None] + kz[None, None, :]) / N)
     d = 0.5 * (dA + ph * dB)
     normal = nobj ** 2 / V
-    p0, p2, p4, nm = orc.pk_shells(d, orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, 0.0, normal))
+    lk = dict(log_binning=True, dlog=b["dlog"]) if b.get("log_binning") else {}
+    p0, p2, p4, nm = orc.pk_shells(d, orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, 0.0, normal, **lk))
     C, A = _alias_tables(N, scheme)
     f = np.where(j <= N // 2, j, N - j)
     Cx, Ax = C[f], A[f]
     X = 0.5 * (Cx[:, None, None] * Cx[None, :, None] * C[None, None, :]
                + Ax[:, None, None] * Ax[None, :, None] * A[None, None, :])
-    sn_shell = orc.pk_shells(np.sqrt(X), orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, 0.0, 1.0))[0]
+    sn_shell = orc.pk_shells(np.sqrt(X), orc.pk_opts(N, L, b["nshells"], b["kmin"], b["dk"], scheme, 0.0, 1.0,
+                                                    **lk))[0]
     SN = V / nobj
     return p0 - SN * sn_shell, p2, nm, SN, d, nobj, C, A
 
@@ -240,3 +242,30 @@ def test_measure_interlaced_ngp_dense_tiles(ctx, orc):
         denom = np.abs(ref0) + SN
         assert np.max(np.abs(out["pk0"] - ref0) / denom) < 1e-4
         assert np.max(np.abs(out["pk2"] - p2) / denom) < 1e-4
+
+
+@pytest.mark.parametrize("scheme,interlace", [(2, False), (1, True), (0, False)])
+def test_measure_log_binning_matches_oracle(ctx, orc, scheme, interlace):
+    """The fused x pass walks the log shells with the same integer thresholds
+    as the shell kernel: counts equal the oracle's log10 rule, P(k) within
+    1e-4 (type_of_binning = log, Params.cpp:4104-4106)."""
+    import cosmicatlas_amd as cm
+    N, L, n = 64, 300.0, 150000
+    rng = np.random.default_rng(31 + scheme)
+    pos = rng.uniform(0.0, L, (n, 3)).astype(np.float32)
+    b = cm.binning_log(N, L, DeltaKmin=1.0, N_log_bins=8)
+    out = ctx.measure(torch.from_numpy(pos).cuda(), N, L, scheme, b["nshells"], b["kmin"], 0.0, interlace=interlace,
+                      log_binning=True, dlog=b["dlog"])
+    if interlace:
+        ref0, _, nm, SN, _, _, _, _ = _interlaced_ref(orc, pos, N, L, scheme, b)
+    else:
+        mesh = orc.mas(pos, N, L, scheme)
+        nobj = mesh.sum()
+        V = L ** 3
+        SN = V / nobj
+        o = orc.pk_opts(N, L, b["nshells"], b["kmin"], 0.0, scheme, SN, nobj ** 2 / V, log_binning=True,
+                        dlog=b["dlog"])
+        ref0, _, _, nm = orc.pk_shells(orc.fft_r2c(mesh - nobj / N ** 3), o)
+    assert np.array_equal(out["nmodes"], nm)
+    assert nm.sum() > 0
+    assert np.max(np.abs(out["pk0"] - ref0) / (np.abs(ref0) + SN)) < 1e-4

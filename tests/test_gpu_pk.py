@@ -66,3 +66,34 @@ def test_single_mode_closed_form_on_gpu(ctx):
     exp = np.zeros(N // 2)
     exp[s] = (A * N ** 3 / 2) ** 2 / nm[s]
     assert np.max(np.abs(p0 - exp)) < 1e-9 * exp.max()
+
+
+@pytest.mark.parametrize("N,dmin,nlog", [(128, 1.0, 6), (64, 1.5, 10), (48, 2.0, 7), (96, 0.5, 12)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_log_shells_match_oracle(ctx, orc, N, dmin, nlog, dtype):
+    """type_of_binning = log: integer |k|^2 thresholds on the GPU against the
+    oracle's FP64 log10 rule (pinned by a 60-digit enumeration): mode counts
+    bit-exact, including the modes that sit on an edge (N = 128, 6 bins)."""
+    L = 400.0
+    kf = 2 * np.pi / L
+    dlog = np.log10((N / 2) / dmin) / nlog
+    dk = _spectrum(N, 3 * N + nlog)
+    SN, normal = 1.5, 2.0
+    o = orc.pk_opts(N, L, nlog, dmin * kf, 0.0, 2, SN, normal, log_binning=True, dlog=dlog)
+    r0, r2, r4, rn = orc.pk_shells(dk, o)
+    p0, p2, p4, nm = ctx.pk_shells(torch.from_numpy(dk).to(dtype).cuda(), N, L, nlog, dmin * kf, 0.0, 2, SN, normal,
+                                   log_binning=True, dlog=dlog)
+    assert np.array_equal(nm, rn)
+    tol = 1e-11 if dtype == torch.complex128 else 2e-6
+    scale = np.abs(r0).max()
+    for a, r in ((p0, r0), (p2, r2), (p4, r4)):
+        assert np.max(np.abs(a - r)) < tol * scale
+
+
+def test_log_binning_argument_checks(ctx):
+    import cosmicatlas_amd as cm
+    dk = torch.zeros((16, 16, 9), dtype=torch.complex64, device="cuda")
+    with pytest.raises(cm.CosmoError):  # kmin = 0 has no logarithm
+        ctx.pk_shells(dk, 16, 100.0, 4, 0.0, 0.0, 1, 0.0, 1.0, log_binning=True, dlog=0.3)
+    with pytest.raises(cm.CosmoError):  # the float32 rule is the linear one's
+        ctx.pk_shells(dk, 16, 100.0, 4, 0.1, 0.0, 1, 0.0, 1.0, log_binning=True, dlog=0.3, float_shell_rule=True)

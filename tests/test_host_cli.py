@@ -95,6 +95,37 @@ def test_cli_catalog_power_spectrum(tmp_path, orc):
 
 
 @pytest.mark.gpu
+def test_cli_log_binning(tmp_path, orc):
+    """type_of_binning = log through the -p workflow: N_log_bins shells with
+    the reference's rule floor(log10(|k|/kmin)/Deltal) + 1 (FftwFunctions.cpp:
+    1103-1108, Params.cpp:4104-4106), k written at the log centre
+    kmin 10^((s - 1/2) Deltal)."""
+    N, L, n, nlog = 64, 500.0, 100000, 8
+    pos = np.random.default_rng(23).uniform(0, L, (n, 3)).astype(np.float32)
+    np.savetxt(tmp_path / "cat.txt", pos, fmt="%.7f")
+    ini = _ini(tmp_path, Nft=str(N), mass_assignment_scheme="TSC", type_of_binning="log",
+               N_log_bins=str(nlog), DeltaKmin="1")
+    d = _show(ini)
+    kf = 2 * np.pi / L
+    dlog = math.log10((N / 2) * kf / kf) / nlog
+    assert int(d["Nnp_data"]) == nlog
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.loadtxt(tmp_path / f"Pk_fkp_BOX_Nft{N}_TSC_test.txt")
+    pos32 = np.loadtxt(tmp_path / "cat.txt").astype(np.float32)
+    mesh = orc.mas(pos32, N, L, orc.TSC)
+    V = L ** 3
+    dk = orc.fft_r2c(mesh - n / N ** 3)
+    o = orc.pk_opts(N, L, nlog, kf, 0.0, orc.TSC, V / n, n * n / V, log_binning=True, dlog=dlog)
+    p0, _, _, nm = orc.pk_shells(dk, o)
+    k = kf * 10.0 ** ((np.arange(nlog) - 0.5) * dlog)
+    first = 1 if p0[0] < 1 else 0
+    np.testing.assert_allclose(out[:, 0], k[first:], rtol=1e-5)
+    assert np.array_equal(out[:, 2].astype(int), nm[first:])
+    assert np.max(np.abs(out[:, 1] - p0[first:]) / (np.abs(p0[first:]) + V / n)) < 1e-4
+
+
+@pytest.mark.gpu
 def test_cli_density_grid(tmp_path, orc):
     N, L = 32, 300.0
     rng = np.random.default_rng(5)

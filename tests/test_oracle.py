@@ -178,6 +178,41 @@ def test_mode_counts_exact(orc, N, ndel, dmin):
     np.testing.assert_array_equal(nm, exp)
 
 
+@pytest.mark.parametrize("N,dmin,nlog", [(128, 1.0, 6), (64, 1.5, 10), (48, 2.0, 7), (32, 0.5, 5)])
+def test_log_shell_counts_exact(orc, N, dmin, nlog):
+    """type_of_binning = log: shell s = floor(log10(|k|/kmin)/Deltal) + 1
+    (FftwFunctions.cpp:1103-1108), Deltal = log10(kmax/kmin)/N_log_bins
+    (Params.cpp:4106).  Independent count: the edges
+    q_s = (DeltaKmin 10^((s-1) Deltal))^2 in 60-digit decimal arithmetic
+    against the integer q = |k/k_f|^2 of every visited mode; an edge within
+    1e-30 of an integer is that integer (the exact rule puts a mode on an edge
+    in the shell above).  N = 128, DeltaKmin = 1, 6 bins puts every edge on a
+    power of 4 (10^Deltal = 2)."""
+    import decimal
+    D = decimal.Decimal
+    decimal.getcontext().prec = 60
+    L = 400.0
+    kf = TWO_PI / L
+    dlog = np.log10((N / 2) / dmin) / nlog
+    o = orc.pk_opts(N, L, nlog, dmin * kf, 0.0, 1, 0.0, 1.0, log_binning=True, dlog=dlog)
+    _, _, _, nm = orc.pk_shells(np.ones((N, N, N // 2 + 1), complex), o)
+    dl = (D(N // 2) / D(str(dmin))).log10() / D(nlog)
+    edges = [(D(str(dmin)) * (D(10) ** ((D(s) - 1) * dl))) ** 2 for s in range(nlog + 1)]
+    edges = [e.to_integral_value() if abs(e - e.to_integral_value()) < D("1e-30") * max(e, D(1)) else e
+             for e in edges]
+    q = _folded_q(N)[_stored_mode_mask(N)]
+    vals, cnt = np.unique(q, return_counts=True)
+    exp = np.zeros(nlog, np.int64)
+    for v, c in zip(vals.tolist(), cnt.tolist()):
+        for s in range(nlog):
+            if edges[s] <= v < edges[s + 1]:
+                exp[s] += c
+    np.testing.assert_array_equal(nm, exp)
+    if (N, dmin, nlog) == (128, 1.0, 6):
+        # edges 4^(s-1): shell s >= 1 holds 4^(s-1) <= q < 4^s
+        assert [int(e) for e in edges[1:]] == [4 ** (s - 1) for s in range(1, nlog + 1)]
+
+
 def test_float_shell_rule_survey_counts(orc):
     """The reference's float32 shell index changes per-shell mode counts by a
     summed 2,612 (of 548,955 modes) in 47 of 64 shells at N=128, L=500, ndel=1,
