@@ -213,12 +213,12 @@ class Context:
 
     # ------------------------------------------------------------------- P(k)
     def pk_shells(self, dk, N, L, nshells, kmin, dk_bin, scheme, shot_noise, normal_power,
-                  mas_correction=True, float_shell_rule=False, log_binning=False, dlog=0.0):
+                  mas_correction=True, float_shell_rule=False, log_binning=False, dlog=0.0, window_set=False):
         torch = _torch()
         code = CM_F32 if dk.dtype == torch.complex64 else CM_F64
         o = _abi.PkOpts(N, float(L), nshells, float(kmin), float(dk_bin), _scheme(scheme),
                         int(mas_correction), float(shot_noise), float(normal_power),
-                        int(float_shell_rule), int(log_binning), float(dlog))
+                        int(float_shell_rule), int(log_binning), float(dlog), int(window_set))
         p0, p2, p4 = (np.zeros(nshells) for _ in range(3))
         nm = np.zeros(nshells, np.int64)
         self._check(self.lib.cm_pk_shells(self.h, C.byref(o), code, _ptr(dk),
@@ -276,12 +276,13 @@ class Context:
 
     def measure_fkp(self, pos, rand, N, L, scheme, nshells, kmin, dk_bin, weights=None, rweights=None, nbar=0.0,
                     P_est=0.0, fkp_weight=False, window=False, mas_correction=True, sn_correction=True,
-                    float_shell_rule=False, nbar_g=None, nbar_r=None):
+                    float_shell_rule=False, nbar_g=None, nbar_r=None, log_binning=False, dlog=0.0):
         """FKP estimator with a random catalogue ``rand`` ((nr,3) float32);
         ``nbar_g`` / ``nbar_r`` (device float32, one per object) give each
         object's mean density in place of the constant ``nbar``."""
         o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, mas_correction=mas_correction,
-                          sn_correction=sn_correction, float_shell_rule=float_shell_rule)
+                          sn_correction=sn_correction, float_shell_rule=float_shell_rule, log_binning=log_binning,
+                          dlog=dlog)
         torch = _torch()
         for a, m in ((nbar_g, pos), (nbar_r, rand)):
             if a is not None and (a.dtype != torch.float32 or not a.is_contiguous() or a.numel() != m.shape[0]):

@@ -48,6 +48,7 @@ class PkOpts(C.Structure):
         ("float_shell_rule", C.c_int),
         ("log_binning", C.c_int),
         ("dlog", C.c_double),
+        ("window_set", C.c_int),
     ]
 
 

@@ -187,8 +187,10 @@ cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f);
 // The fused tail of a measurement: mesh (float32, device) -> z/y/x passes with
 // the shells (+ potential and T-web labels when o->do_tweb).  meshB (nullable):
 // the interlaced mesh.  Results land in the FusedOut block of o->nshells.
+// window_set: the shells visit the window mode set of the random branch
+// (stored modes outside the first octant; the zero mode is the caller's).
 cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, FusedOut *out);
+                             int8_t *labels, FusedOut *out, bool window_set = false);
 // Copy a FusedOut block to the pinned host buffer and wait; returns its host copy.
 cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host);
 

@@ -115,15 +115,17 @@ static cm_status scaled_weights(cm_ctx *ctx, const char *name, const WeightRule 
     return CM_OK;
 }
 
-// mesh -> shells through the fused path, host sums and counts
+// mesh -> shells through the fused path, host sums, counts and the FP64 mesh sum
 static cm_status shells_of_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, double mean,
-                                std::vector<double> &hs, std::vector<unsigned long long> &hc) {
+                                std::vector<double> &hs, std::vector<unsigned long long> &hc,
+                                bool window_set = false, double *mesh_sum = nullptr) {
     FusedOut dev, h;
     cm_status s;
-    if ((s = fused_measure_mesh(ctx, o, mesh, nullptr, mean, nullptr, &dev)) != CM_OK) return s;
+    if ((s = fused_measure_mesh(ctx, o, mesh, nullptr, mean, nullptr, &dev, window_set)) != CM_OK) return s;
     if ((s = fused_readback(ctx, dev, o->nshells, &h)) != CM_OK) return s;
     hs.assign(h.sums, h.sums + 4 * (size_t)o->nshells);
     hc.assign(h.cnts, h.cnts + (size_t)o->nshells);
+    if (mesh_sum) *mesh_sum = *h.nobj;
     return CM_OK;
 }
 
@@ -190,10 +192,21 @@ static cm_status measure_fkp_impl(cm_ctx *ctx, const cm_measure_opts *o, const c
         info->tweb = cm_tweb_stats{};
     }
     if (f->window && pk_window) {
+        // the window's mode set (FftwFunctions.cpp:1248-1262, 1354-1568): the
+        // stored modes outside the first octant, plus the zero mode
+        // |W(0)|^2 = (mesh sum)^2 in the shell of k = 0 (shell 0 when kmin = 0
+        // under linear binning, always under log binning, :1110-1113).  The
+        // mean is removed on load (float32 transform of a small residual) and
+        // the zero mode added here from the FP64 mesh sum.
         const float *wr1;
         if ((s = scaled_weights(ctx, "fkp.wr", rr, nr, 1.0, &wr1)) != CM_OK) return s;
         if ((s = assign_randoms(ctx, &md, rpos, wr1, nr, mesh, 0)) != CM_OK) return s;
-        if ((s = shells_of_mesh(ctx, o, mesh, Wr / (double)n3, hs, hc)) != CM_OK) return s;
+        double msum = 0.0;
+        if ((s = shells_of_mesh(ctx, o, mesh, Wr / (double)n3, hs, hc, true, &msum)) != CM_OK) return s;
+        if (o->log_binning || o->kmin == 0.0) {
+            hs[0] += msum * msum;
+            hc[0] += 1;
+        }
         std::vector<double> q2((size_t)o->nshells), q4((size_t)o->nshells);
         std::vector<int64_t> nm((size_t)o->nshells);
         pk_finalize(o->nshells, normal_power / (alpha * alpha), 0.0, hs.data(), hc.data(), pk_window, q2.data(),

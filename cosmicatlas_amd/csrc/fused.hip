@@ -140,7 +140,7 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
             const int fx = xl ? ix : N - ix;
             // mode_counted: Nyquist x/y planes never; the kz = 0 plane only for
             // (i, j) with xl && yl && (i|j) != 0, or !xl && yl && j > 0
-            const bool excl = ix == h || iy == h;
+            const bool excl = ix == h || iy == h || (pk.window && xl && yl);
             const bool zero_ok = !excl && yl && (xl ? (ix | iy) != 0 : iy > 0);
             const int qxy = fx * fx + fy * fy;
             // shot-noise weight per mode: (wc prod C1 + wa prod A) / W^2
@@ -360,7 +360,7 @@ cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host)
 // the FusedOut block.  Host tables (MAS windows, interlacing phases, Gaussian
 // factors) are cached on the device, so a repeated call issues kernels only.
 cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, FusedOut *out) {
+                             int8_t *labels, FusedOut *out, bool window_set) {
     const int N = o->mas.N;
     const size_t n3 = (size_t)N * N * N;
     const int pitch = padded_pitch(N);  // internal spectrum: kz rows padded to 128-byte multiples
@@ -419,8 +419,10 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     po.float_shell_rule = o->float_shell_rule;
     po.log_binning = o->log_binning;
     po.dlog = o->dlog;
+    po.window_set = 0;
     PkK pk = make_pkk(ctx, &po, &s);
     if (s != CM_OK) return s;
+    pk.window = window_set ? 1 : 0;
     cm_status st;
     const void *tw = twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;

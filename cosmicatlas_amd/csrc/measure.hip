@@ -259,6 +259,7 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
     po.float_shell_rule = o->float_shell_rule;
     po.log_binning = o->log_binning;
     po.dlog = o->dlog;
+    po.window_set = 0;
     double *sums;
     unsigned long long *cnts;
     if ((s = pk_accumulate(ctx, &po, CM_F32, dk, &sums, &cnts)) != CM_OK) return s;

@@ -53,15 +53,19 @@ __global__ void __launch_bounds__(256) k_pk_shells(const v2pk<T> *__restrict__ d
             const size_t r = g / M1;
             const int iy = (int)(r % p.N), ix = (int)(r / p.N);
             int fx, fy;
-            if (mode_counted(p.N, ix, iy, iz, fx, fy)) {
-                sh = shell_of(p, fx, fy, iz);
+            bool counted = mode_counted(p.N, ix, iy, iz, fx, fy);
+            const bool dc = (ix | iy | iz) == 0;
+            if (p.window) counted = dc || (counted && !(ix < p.N / 2 && iy < p.N / 2));
+            if (counted) {
+                // the zero mode of a window takes shell 0 under log binning
+                sh = (dc && p.qedge) ? 0 : shell_of(p, fx, fy, iz);
                 if (sh >= 0) {
                     row = (int)r;
                     const v2pk<T> v = dk[g];
                     const double pw = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
                     const double ic = s_ic[fx] * s_ic[fy] * s_ic[iz];
                     const int q = fx * fx + fy * fy + iz * iz;
-                    const double mu2 = (double)iz * (double)iz / (double)q;
+                    const double mu2 = q > 0 ? (double)iz * (double)iz / (double)q : 0.0;
                     const double pc = pw * ic;
                     a0 = pc;
                     a1 = s_sq[fx] * s_sq[fy] * s_sq[iz] * ic;
@@ -199,6 +203,7 @@ PkK make_pkk(cm_ctx *ctx, const cm_pk_opts *o, cm_status *st) {
     *st = CM_OK;
     PkK p;
     p.qedge = nullptr;
+    p.window = o->window_set ? 1 : 0;
     p.N = o->N;
     p.nshells = o->nshells;
     const double kf = 2.0 * M_PI / o->L;

@@ -26,6 +26,9 @@ struct PkK {
     // logarithmic shells: shell s holds q = |k/k_f|^2 with qedge[s] <= q <
     // qedge[s+1] (device table of nshells+1 thresholds); nullptr: linear
     const int *qedge;
+    // window mode set of the random branch (FftwFunctions.cpp:1248-1262):
+    // the first octant (both folded x and y below N/2) is left out
+    int window;
 };
 
 // Shell of q under the log rule: the s with qedge[s] <= q < qedge[s+1], else -1.

@@ -79,6 +79,18 @@ class Params {
     std::string _delta_grid_file() const { return delta_grid_file; }
     std::string _file_catalogue() const { return Input_dir_cat + file_catalogue; }
     bool _use_random_catalog() const { return use_random_catalog; }
+    std::string _file_random() const { return Input_dir_cat + file_random; }
+    std::string _file_window() const { return file_window; }
+    // random-catalogue columns; an unset coordinate column (-1) takes the
+    // tracer catalogue's
+    int _i_coord1_r() const { return i_coord1_r >= 0 ? i_coord1_r : i_coord1_g; }
+    int _i_coord2_r() const { return i_coord2_r >= 0 ? i_coord2_r : i_coord2_g; }
+    int _i_coord3_r() const { return i_coord3_r >= 0 ? i_coord3_r : i_coord3_g; }
+    int _i_weight1_r() const { return i_weight1_r; }
+    int _i_mean_density_g() const { return i_mean_density_g; }
+    int _i_mean_density_r() const { return i_mean_density_r; }
+    bool _FKP_weight() const { return FKP_weight; }
+    real_prec _Pest() const { return Pest; }
     bool _measure_cross() const { return measure_cross; }
     int _i_coord1_g() const { return i_coord1_g; }
     int _i_coord2_g() const { return i_coord2_g; }
@@ -138,6 +150,12 @@ class Params {
     double kmax_bk = 0.2, kmin_bk = 0.1;
     std::string file_catalogue = "cat.dat", Input_dir_cat = "";
     bool use_random_catalog = false, measure_cross = false;
+    // random catalogue and FKP weights (Params.cpp:228,262-266,285,341-342,357)
+    std::string file_random = "random", file_window = "window";
+    int i_coord1_r = -1, i_coord2_r = -1, i_coord3_r = -1, i_weight1_r = -1;
+    int i_mean_density_g = -1, i_mean_density_r = -1;
+    bool FKP_weight = false;
+    real_prec Pest = 1000;
     int i_coord1_g = 0, i_coord2_g = 1, i_coord3_g = 2, i_weight1_g = -1;
     // velocity columns (-1: absent), their units and whether the coordinates
     // already hold the redshift-space displacement (Params.cpp:170,215-227)
@@ -219,12 +237,23 @@ class FftwFunctions {
     void set_w_g(real_prec w) { w_g = w; }
     void set_s_g(real_prec s) { s_g = s; }
     void set_normal_power(real_prec v) { normal_power = v; }
+    // random catalogue sums (Catalog::get_interpolated_density_field,
+    // Source/Catalog.cpp:14153-14171): W_r, S_r = sum w^2, normal_p = sum nbar w^2
+    void set_n_ran(real_prec n) { n_ran = n; }
+    void set_w_r(real_prec w) { w_r = w; }
+    void set_s_r(real_prec s) { s_r = s; }
+    void set_normal_p(real_prec v) { normal_p = v; }
     void raw_sampling(real_prec vol);
     void get_parameters_estimator(bool verbose);
     void get_fluctuation();
     // monopole/quadrupole/hexadecapole of the box FKP estimator (r2c inside)
     void get_power_spectrum_fkp(std::vector<real_prec> &power_g0, std::vector<real_prec> &power_g2,
                                 std::vector<real_prec> &power_g4, std::vector<int> &mod_g);
+    // the same with the window W(k) of the random catalogue (power_r, its modes
+    // mod_r) when use_random_catalog (FftwFunctions.cpp:1248-1262, 1635-1638)
+    void get_power_spectrum_fkp(std::vector<real_prec> &power_g0, std::vector<real_prec> &power_g2,
+                                std::vector<real_prec> &power_g4, std::vector<real_prec> &power_r,
+                                std::vector<int> &mod_g, std::vector<int> &mod_r);
     // grid-input monopole (r2c inside); the reference's D2 defect at
     // FftwFunctions.cpp:1853 is not reproduced
     void power_spectrum(std::vector<real_prec> &powerk_g0, std::vector<int> &mod_g);
@@ -241,16 +270,20 @@ class FftwFunctions {
                                  const std::string &file);
     real_prec _normal_power() const { return normal_power; }
     real_prec _shot_noise() const { return shot_noise; }
+    real_prec _alpha() const { return alpha; }
+    real_prec _normal_window() const { return normal_window; }
 
     std::vector<real_prec> data_g;    // FftwFunctions.h:1179
+    std::vector<real_prec> data_r;    // the random catalogue's mesh
     complex_prec *data_out_g = nullptr;
     real_prec shot_noise = 0;
 
    private:
-    void shells(std::vector<real_prec> &p0, std::vector<real_prec> *p2, std::vector<real_prec> *p4,
-                std::vector<int> &mod);
+    void shells(const std::vector<real_prec> &mesh, bool window, std::vector<real_prec> &p0,
+                std::vector<real_prec> *p2, std::vector<real_prec> *p4, std::vector<int> &mod);
     Params params;
     real_prec n_gal = 0, w_g = 0, s_g = 0, normal_power = 1, alpha = 1, normal_p = 1, s_r = 0;
+    real_prec n_ran = 0, w_r = 0, normal_window = 1;
 };
 
 // --------------------------------------------------------------------- Cwclass
@@ -284,12 +317,16 @@ class PowerSpectrumF {
     explicit PowerSpectrumF(const Params &p);
     void set_output_filenames();
     void compute_power_spectrum(bool verbose, bool mcut);
+    // use_random_catalog: tracer and random catalogues onto two meshes with
+    // their FKP weights and sums, the random branch of the estimator, P(k)
+    // and the window W(k) (PowerSpectrumF.cpp:741-1363)
+    void compute_power_spectrum_randoms(bool verbose);
     void compute_power_spectrum_grid();
     void write_power_and_modes();
     void write_bispectrum();
-    std::vector<real_prec> kvector_data, pk0, pk2, pk4;
-    std::vector<int> modes_g;
-    std::string file_power, file_cwc, file_bispectrum;
+    std::vector<real_prec> kvector_data, pk0, pk2, pk4, pk_w;
+    std::vector<int> modes_g, modes_r;
+    std::string file_power, file_cwc, file_bispectrum, file_window;
     std::vector<real_prec> kvector_data_b, bispectrum, sn_bispectrum;
     std::vector<int> modes_tri;
     FftwFunctions fftw_functions;
@@ -318,3 +355,9 @@ double rsd_conversion_factor(const Params &p);
 // ASCII catalogue: whitespace-separated columns (Catalog::read_catalog); a
 // velocity column i_v3_g moves objects along the third axis into redshift space.
 void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real_prec> &w);
+// The same reader for the random catalogue (Random_file, i_coord*_r,
+// i_weight1_r) and either catalogue's mean-density column (i_mean_density_g or
+// _r; nb stays empty without one).  Coordinates go through the tracer's
+// coordinate system (sys_of_coord_g).
+void read_catalog_columns(const Params &p, const std::string &file, const int c[3], int cw, int cnb,
+                          std::vector<real_prec> &xyz, std::vector<real_prec> &w, std::vector<real_prec> &nb);

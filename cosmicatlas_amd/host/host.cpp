@@ -106,6 +106,16 @@ void Params::read_pars(const std::string &file) {
         else if (name == "Catalogue_file") file_catalogue = value;
         else if (name == "Input_dir_cat") Input_dir_cat = value;
         else if (name == "use_random_catalog") use_random_catalog = to_bool(value);
+        else if (name == "Random_file") file_random = value;
+        else if (name == "file_window") file_window = value;
+        else if (name == "i_coord1_r") i_coord1_r = std::atoi(value.c_str());
+        else if (name == "i_coord2_r") i_coord2_r = std::atoi(value.c_str());
+        else if (name == "i_coord3_r") i_coord3_r = std::atoi(value.c_str());
+        else if (name == "i_weight1_r") i_weight1_r = std::atoi(value.c_str());
+        else if (name == "i_mean_density_g") i_mean_density_g = std::atoi(value.c_str());
+        else if (name == "i_mean_density_r") i_mean_density_r = std::atoi(value.c_str());
+        else if (name == "FKP_weight") FKP_weight = to_bool(value);
+        else if (name == "Pest") Pest = (real_prec)std::atoi(value.c_str());  // read as an integer (Params.cpp:3026)
         else if (name == "measure_cross") measure_cross = to_bool(value);
         else if (name == "i_coord1_g") i_coord1_g = std::atoi(value.c_str());
         else if (name == "i_coord2_g") i_coord2_g = std::atoi(value.c_str());
@@ -407,31 +417,53 @@ void FftwFunctions::raw_sampling(real_prec vol) {
     shot_noise = s_r;
 }
 
-// FftwFunctions.cpp:177-212 (box branch; randoms are not part of this path)
+// FftwFunctions.cpp:177-212: the random branch from the catalogue sums
+// (alpha = W_g/W_r, normal_power = alpha normal_p, shot noise
+// (alpha + 1) S_r/normal_p, window normalisation normal_power/alpha^2), the box
+// branch from raw_sampling
 void FftwFunctions::get_parameters_estimator(bool verbose) {
-    if (params._use_random_catalog())
-        throw cmh::Error("use_random_catalog: the FKP-with-randoms estimator is not part of this build");
+    if (params._use_random_catalog()) {
+        if (!(w_r > 0) || !(normal_p > 0))
+            throw cmh::Error("use_random_catalog: the random catalogue has no weight (W_r = 0 or sum nbar w^2 = 0)");
+        alpha = (real_prec)((double)w_g / (double)w_r);
+        normal_power = alpha * normal_p;
+        shot_noise = (real_prec)((alpha + 1.0) * (double)s_r / (double)normal_p);
+        normal_window = normal_power / (alpha * alpha);
+        if (verbose)
+            std::printf("** Ngal = %.9g\n** Nran = %.9g\n** Wg = %.9g\n** Wr = %.9g\n** Sg = %.9g\n** Sr = %.9g\n"
+                        "** alpha = %.9g\n** Normalization = %.9g\n** Shot Noise = %.9g\n",
+                        (double)n_gal, (double)n_ran, (double)w_g, (double)w_r, (double)s_g, (double)s_r,
+                        (double)alpha, (double)normal_power, (double)shot_noise);
+        return;
+    }
     alpha = 1.0;
     shot_noise = s_r;
+    normal_window = 1.0;
     if (verbose)
         std::printf("** Ngal = %.9g\n** Normalization = %.9g\n** Shot Noise = %.9g\n", (double)n_gal,
                     (double)normal_power, (double)shot_noise);
 }
 
-// FftwFunctions.cpp:293-294
+// FftwFunctions.cpp:275-294: delta = rho_g - alpha rho_r with randoms, else the
+// mean density removed
 void FftwFunctions::get_fluctuation() {
+    if (params._use_random_catalog()) {
+        if (data_r.size() != data_g.size()) throw cmh::Error("get_fluctuation: the random mesh is missing");
+        for (size_t i = 0; i < data_g.size(); ++i) data_g[i] -= alpha * data_r[i];
+        return;
+    }
     const real_prec m = n_gal / (real_prec)params._NGRID();
     for (auto &v : data_g) v -= m;
 }
 
-void FftwFunctions::shells(std::vector<real_prec> &p0, std::vector<real_prec> *p2, std::vector<real_prec> *p4,
-                           std::vector<int> &mod) {
+void FftwFunctions::shells(const std::vector<real_prec> &mesh, bool window, std::vector<real_prec> &p0,
+                           std::vector<real_prec> *p2, std::vector<real_prec> *p4, std::vector<int> &mod) {
     const ULONG N = params._Nft();
     const size_t n3 = (size_t)N * N * N, nh = params._NGRID_h();
     DevBuf din(sizeof(real_prec) * n3), dk(sizeof(complex_prec) * nh);
-    cmh::h2d(din.p, data_g.data(), sizeof(real_prec) * n3);
+    cmh::h2d(din.p, mesh.data(), sizeof(real_prec) * n3);
     check(cm_fft_r2c(ctx(), (int)N, CM_F32, din.p, dk.p, 0.0), "cm_fft_r2c");
-    if (data_out_g) cmh::d2h(data_out_g, dk.p, sizeof(complex_prec) * nh);
+    if (data_out_g && !window) cmh::d2h(data_out_g, dk.p, sizeof(complex_prec) * nh);
     const int ns = params._d_Nnp_data();
     const bool logb = params._type_of_binning() == "log";
     // the log rule divides by kmin (FftwFunctions.cpp:1003, rkmin = 1/kmin)
@@ -444,11 +476,12 @@ void FftwFunctions::shells(std::vector<real_prec> &p0, std::vector<real_prec> *p
     o.dk = params._d_DeltaK_data();
     o.scheme = (cm_mas)params._mass_assignment();
     o.mas_correction = params._MAS_correction();
-    o.shot_noise = params._SN_correction() ? shot_noise : 0.0;
-    o.normal_power = normal_power;
+    o.shot_noise = (params._SN_correction() && !window) ? shot_noise : 0.0;
+    o.normal_power = window ? normal_window : normal_power;
     o.float_shell_rule = logb ? 0 : params._float_shell_rule();
     o.log_binning = logb ? 1 : 0;
     o.dlog = params._d_Deltal();
+    o.window_set = window ? 1 : 0;
     std::vector<double> a0(ns), a2(ns), a4(ns);
     std::vector<int64_t> nm(ns);
     check(cm_pk_shells(ctx(), &o, CM_F32, dk.p, a0.data(), a2.data(), a4.data(), nm.data()), "cm_pk_shells");
@@ -459,12 +492,19 @@ void FftwFunctions::shells(std::vector<real_prec> &p0, std::vector<real_prec> *p
 }
 
 void FftwFunctions::get_power_spectrum_fkp(std::vector<real_prec> &power_g0, std::vector<real_prec> &power_g2,
+                                           std::vector<real_prec> &power_g4, std::vector<real_prec> &power_r,
+                                           std::vector<int> &mod_g, std::vector<int> &mod_r) {
+    shells(data_g, false, power_g0, &power_g2, &power_g4, mod_g);
+    if (params._use_random_catalog()) shells(data_r, true, power_r, nullptr, nullptr, mod_r);
+}
+
+void FftwFunctions::get_power_spectrum_fkp(std::vector<real_prec> &power_g0, std::vector<real_prec> &power_g2,
                                            std::vector<real_prec> &power_g4, std::vector<int> &mod_g) {
-    shells(power_g0, &power_g2, &power_g4, mod_g);
+    shells(data_g, false, power_g0, &power_g2, &power_g4, mod_g);
 }
 
 void FftwFunctions::power_spectrum(std::vector<real_prec> &powerk_g0, std::vector<int> &mod_g) {
-    shells(powerk_g0, nullptr, nullptr, mod_g);
+    shells(data_g, false, powerk_g0, nullptr, nullptr, mod_g);
 }
 
 void FftwFunctions::get_bispectrum_fkp(char, std::vector<real_prec> &bispect, std::vector<real_prec> &sn_bispect,
@@ -771,6 +811,30 @@ void read_catalog(const Params &p, std::vector<real_prec> &xyz, std::vector<real
     if (!vlos.empty()) to_redshift_space(p, xyz, vlos);
 }
 
+void read_catalog_columns(const Params &p, const std::string &file, const int c[3], int cw, int cnb,
+                          std::vector<real_prec> &xyz, std::vector<real_prec> &w, std::vector<real_prec> &nb) {
+    xyz.clear();
+    w.clear();
+    nb.clear();
+    std::ifstream f(file.c_str());
+    if (!f) throw cmh::Error("File " + file + " not found");
+    const int need = std::max(std::max(std::max(c[0], c[1]), std::max(c[2], cw)), cnb) + 1;
+    std::string line;
+    std::vector<double> col;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::stringstream ls(line);
+        col.clear();
+        double v;
+        while (ls >> v) col.push_back(v);
+        if ((int)col.size() < need) continue;
+        for (int a = 0; a < 3; ++a) xyz.push_back((real_prec)col[c[a]]);
+        if (cw >= 0) w.push_back((real_prec)col[cw]);
+        if (cnb >= 0) nb.push_back((real_prec)col[cnb]);
+    }
+    to_cartesian(p, xyz);
+}
+
 // ============================================================== PowerSpectrumF
 PowerSpectrumF::PowerSpectrumF(const Params &p) : fftw_functions(p), cwclass(p), params(p) {
     set_output_filenames();
@@ -787,6 +851,9 @@ void PowerSpectrumF::set_output_filenames() {
                       params._file_bispectrum() + ".txt";
     file_cwc = params._Output_directory() + "CWC_" + params._Name_survey() + "_Nft" + std::to_string(params._Nft()) +
                "_" + params._mass_assignment_scheme() + "_" + params._file_power();
+    file_window = params._Output_directory() + params._statistics() + "_" + params._Name_survey() + "_Nft" +
+                  std::to_string(params._Nft()) + "_" + params._mass_assignment_scheme() + "_" +
+                  params._file_window() + ".txt";
 }
 
 void PowerSpectrumF::kvector() {
@@ -800,8 +867,98 @@ void PowerSpectrumF::kvector() {
                                            : params._d_kmin() + params._d_DeltaK_data() * (i + 0.5));
 }
 
-// -p with Input_type=catalog (PowerSpectrumF.cpp:741-1363, box, no randoms)
+// Per-object sums of a catalogue (Catalog::get_interpolated_density_field,
+// Source/Catalog.cpp:14142-14171): total weight w = w_user (1 without a weight
+// column) times 1/(1 + Pest nbar) when FKP_weight, with nbar the object's
+// mean-density column or one constant; W = sum w, S = sum w^2,
+// normal = sum nbar w^2.  w receives the total weights.
+struct CatalogSums {
+    double n = 0, W = 0, S = 0, normal = 0;
+};
+static CatalogSums catalog_sums(const Params &p, size_t n, std::vector<real_prec> &w, const std::vector<real_prec> &nb,
+                                double nbar_const) {
+    CatalogSums s;
+    std::vector<real_prec> tot(n);
+    for (size_t i = 0; i < n; ++i) {
+        const double nbar = nb.empty() ? nbar_const : (double)nb[i];
+        double v = w.empty() ? 1.0 : (double)w[i];
+        if (p._FKP_weight()) v *= 1.0 / (1.0 + (double)p._Pest() * nbar);
+        tot[i] = (real_prec)v;
+        s.W += v;
+        s.S += v * v;
+        s.normal += nbar * v * v;
+    }
+    s.n = (double)n;
+    w.swap(tot);
+    return s;
+}
+
+// -p with Input_type=catalog and use_random_catalog (PowerSpectrumF.cpp:
+// 741-1363 random branch): both catalogues onto meshes with their total
+// weights, alpha and the normalisations from the sums, delta = rho_g -
+// alpha rho_r, P(k) with the shot noise (alpha + 1) S_r/normal_r and the
+// window of the randoms.  A catalogue without a mean-density column takes the
+// constant nbar = W_g / Lbox^3 (sum of the tracers' user weights over the box).
+void PowerSpectrumF::compute_power_spectrum_randoms(bool verbose) {
+    std::vector<real_prec> xg, wg, nbg, xr, wr, nbr;
+    if (params._i_mean_density_g() >= 0) {
+        const int cg[3] = {params._i_coord1_g(), params._i_coord2_g(), params._i_coord3_g()};
+        read_catalog_columns(params, params._file_catalogue(), cg, params._i_weight1_g(), params._i_mean_density_g(),
+                             xg, wg, nbg);
+    } else {
+        read_catalog(params, xg, wg);
+    }
+    const int cr[3] = {params._i_coord1_r(), params._i_coord2_r(), params._i_coord3_r()};
+    read_catalog_columns(params, params._file_random(), cr, params._i_weight1_r(), params._i_mean_density_r(), xr, wr,
+                         nbr);
+    const size_t ng = xg.size() / 3, nr = xr.size() / 3;
+    if (ng == 0) throw cmh::Error("the catalogue " + params._file_catalogue() + " holds no objects");
+    if (nr == 0) throw cmh::Error("the random catalogue " + params._file_random() + " holds no objects");
+    double wuser = 0;
+    if (wg.empty()) wuser = (double)ng;
+    else
+        for (real_prec v : wg) wuser += v;
+    const double V = (double)params._Lbox() * params._Lbox() * params._Lbox();
+    const double nbar_const = wuser / V;
+    const CatalogSums sg = catalog_sums(params, ng, wg, nbg, nbar_const);
+    const CatalogSums sr = catalog_sums(params, nr, wr, nbr, nbar_const);
+    fftw_functions.resize_fftw_vectors();
+    grid_assignment(params, xg, wg, fftw_functions.data_g);
+    fftw_functions.data_r.assign(params._NGRID(), 0);
+    grid_assignment(params, xr, wr, fftw_functions.data_r);
+    fftw_functions.set_n_gal((real_prec)sg.n);
+    fftw_functions.set_w_g((real_prec)sg.W);
+    fftw_functions.set_s_g((real_prec)sg.S);
+    fftw_functions.set_n_ran((real_prec)sr.n);
+    fftw_functions.set_w_r((real_prec)sr.W);
+    fftw_functions.set_s_r((real_prec)sr.S);
+    fftw_functions.set_normal_p((real_prec)sr.normal);
+    fftw_functions.get_parameters_estimator(verbose);
+    std::vector<real_prec> mesh;
+    if (params._compute_tweb()) mesh = fftw_functions.data_g;  // kept for the T-web
+    fftw_functions.get_fluctuation();
+    kvector();
+    if (params._statistics() != "Pk_fkp")
+        throw cmh::Error("use_random_catalog: Statistics = " + params._statistics() +
+                         " with randoms is not part of this build (Pk_fkp is)");
+    fftw_functions.get_power_spectrum_fkp(pk0, pk2, pk4, pk_w, modes_g, modes_r);
+    write_power_and_modes();
+    if (params._compute_tweb()) {
+        const real_prec mean = (real_prec)(sg.W / (double)params._NGRID());
+        for (auto &v : mesh) v = v / mean - 1;
+        cwclass.get_CWC(mesh);
+        std::vector<real_prec> lab(cwclass.CWClass.begin(), cwclass.CWClass.end());
+        write_array(file_cwc, lab);
+    }
+}
+
+// -p with Input_type=catalog (PowerSpectrumF.cpp:741-1363; box, or the random
+// branch when use_random_catalog)
 void PowerSpectrumF::compute_power_spectrum(bool verbose, bool) {
+    if (params._use_random_catalog()) {
+        compute_power_spectrum_randoms(verbose);
+        return;
+    }
     std::vector<real_prec> xyz, w;
     read_catalog(params, xyz, w);
     const size_t n = xyz.size() / 3;
@@ -869,7 +1026,12 @@ void PowerSpectrumF::compute_power_spectrum_grid() {
     write_power_and_modes();
 }
 
-void PowerSpectrumF::write_power_and_modes() { write_to_file2(file_power, kvector_data, pk0, modes_g, true); }
+// PowerSpectrumF.cpp:352-358: the window file carries the data shells' mode
+// counts, as the reference writes it
+void PowerSpectrumF::write_power_and_modes() {
+    write_to_file2(file_power, kvector_data, pk0, modes_g, true);
+    if (params._use_random_catalog()) write_to_file2(file_window, kvector_data, pk_w, modes_g, true);
+}
 
 void PowerSpectrumF::write_bispectrum() {
     std::ofstream out(file_bispectrum.c_str());

@@ -158,6 +158,13 @@ typedef struct {
                              evaluated exactly (integer |k|^2 thresholds); dk
                              is unused; kmin must be > 0 */
     double dlog;          /* Deltal = log10(kmax/kmin)/N_log_bins */
+    int window_set;       /* 1: the window W(k) of the random branch
+                             (FftwFunctions.cpp:1248-1262, 1354-1568): the
+                             zero mode (in the shell of k = 0) and the stored
+                             modes outside the first octant (kx, ky both
+                             below N/2 left out); pass the randoms' raw
+                             spectrum, shot_noise 0 and normal_window =
+                             normal_power/alpha^2 */
 } cm_pk_opts;
 
 /* Replaces FftwFunctions::power_spectrum_fkp box branch

@@ -62,6 +62,7 @@ def lib() -> C.CDLL:
         L.or_fft_c2r.argtypes = [C.c_int, P, P]
         L.or_fft_1d.argtypes = [C.c_int, P, P, C.c_int]
         L.or_pk_shells.argtypes = [P, C.POINTER(PkOpts), P, P, P, P]
+        L.or_window_shells.argtypes = [P, C.POINTER(PkOpts), P, P]
         L.or_shell_index.restype = C.c_int
         L.or_shell_index.argtypes = [C.POINTER(PkOpts), C.c_int, C.c_int, C.c_int]
         L.or_smooth_poisson.argtypes = [C.c_int, C.c_double, C.c_double, P]
@@ -137,6 +138,16 @@ def pk_shells(dk, opts: PkOpts):
     nm = np.zeros(ns, np.int64)
     lib().or_pk_shells(_p(dk), C.byref(opts), _p(p0), _p(p2), _p(p4), _p(nm))
     return p0, p2, p4, nm
+
+
+def window_shells(dk, opts: PkOpts):
+    """Window shells of power_spectrum_fkp's random branch (zero mode plus the
+    three non-first octant copies; FftwFunctions.cpp:1248-1262, 1354-1568)."""
+    dk = np.ascontiguousarray(dk, np.complex128)
+    pw = np.zeros(opts.nshells)
+    nm = np.zeros(opts.nshells, np.int64)
+    lib().or_window_shells(_p(dk), C.byref(opts), _p(pw), _p(nm))
+    return pw, nm
 
 
 def shell_index(opts: PkOpts, i, j, k):

@@ -399,6 +399,48 @@ void or_pk_shells(const double *dk, const or_pk_opts *o, double *pk0, double *pk
     free(s0); free(s2); free(s4); free(cnt);
 }
 
+/* Window shells of power_spectrum_fkp's random branch (FftwFunctions.cpp:
+ * 1248-1262, 1354-1362, 1454-1462, 1557-1568, normalised at :1635-1638): the
+ * first-octant loop adds only the zero mode to powerk_r (the else of
+ * if(lp != 0)); the (i, N-j, k), (N-i, j, k) and (N-i, N-j, k) copies add their
+ * modes.  |d|^2 over the squared MAS window, no shot noise; the zero mode
+ * lands in the shell of k = 0 (shell 0 when kmin = 0 under linear binning,
+ * always under log binning, :1110-1113).  Divided by n_modes * normal_power. */
+void or_window_shells(const double *dk, const or_pk_opts *o, double *pw, int64_t *nmodes)
+{
+    const int N = o->N, ns = o->nshells;
+    const size_t NN = (size_t)N, nh = NN / 2 + 1;
+    double *s0 = (double *)calloc((size_t)ns, sizeof(double));
+    int64_t *cnt = (int64_t *)calloc((size_t)ns, sizeof(int64_t));
+    const cplx *d = (const cplx *)dk;
+    for (int i = 0; i < N / 2; ++i)
+        for (int j = 0; j < N / 2; ++j)
+            for (int k = 0; k <= N / 2; ++k) {
+                int sh = or_shell_index(o, i, j, k);
+                if ((i | j | k) == 0 && o->log_binning) sh = 0;
+                if (sh < 0) continue;
+                double corr = 1.0;
+                if (o->mas_correction)
+                    corr = pow(sinc_pi(i, N) * sinc_pi(j, N) * sinc_pi(k, N), o->scheme + 1);
+                const double icorr2 = 1.0 / (corr * corr);
+                size_t lps[4];
+                int nl = 0;
+                if ((i | j | k) == 0) lps[nl++] = 0;
+                if (j > 0 && k > 0) lps[nl++] = ((size_t)i * NN + (NN - (size_t)j)) * nh + (size_t)k;
+                if (i > 0 && (j > 0 || k > 0)) lps[nl++] = ((NN - (size_t)i) * NN + (size_t)j) * nh + (size_t)k;
+                if (i > 0 && j > 0 && k > 0) lps[nl++] = ((NN - (size_t)i) * NN + (NN - (size_t)j)) * nh + (size_t)k;
+                for (int c = 0; c < nl; ++c) {
+                    s0[sh] += (d[lps[c]].re * d[lps[c]].re + d[lps[c]].im * d[lps[c]].im) * icorr2;
+                    cnt[sh] += 1;
+                }
+            }
+    for (int s = 0; s < ns; ++s) {
+        if (pw) pw[s] = cnt[s] == 0 ? 0.0 : s0[s] / ((double)cnt[s] * o->normal_power);
+        if (nmodes) nmodes[s] = cnt[s];
+    }
+    free(s0); free(cnt);
+}
+
 /* ------------------------------------------------------------------------- */
 /* T-web                                                                      */
 /* ------------------------------------------------------------------------- */

@@ -75,6 +75,10 @@ typedef struct {
     double dlog;          /* Deltal (Params.cpp:4106) */
 } or_pk_opts;
 
+/* Window shells of the random branch (FftwFunctions.cpp:1248-1262, 1354-1362,
+ * 1454-1462, 1557-1568): the zero mode and the three non-first octant copies. */
+void or_window_shells(const double *dk, const or_pk_opts *o, double *pw, int64_t *nmodes);
+
 /* power_spectrum_fkp box branch (Source/FftwFunctions.cpp:958-1657): enumerates
  * the four octant copies of every mode, bins |k| into shells, subtracts the
  * aliased shot noise (Jing 2005, Miscelanious.cpp:2532-2554), deconvolves the

@@ -97,3 +97,23 @@ def test_log_binning_argument_checks(ctx):
         ctx.pk_shells(dk, 16, 100.0, 4, 0.0, 0.0, 1, 0.0, 1.0, log_binning=True, dlog=0.3)
     with pytest.raises(cm.CosmoError):  # the float32 rule is the linear one's
         ctx.pk_shells(dk, 16, 100.0, 4, 0.1, 0.0, 1, 0.0, 1.0, log_binning=True, dlog=0.3, float_shell_rule=True)
+
+
+@pytest.mark.parametrize("N,dmin,logb", [(32, 0, False), (48, 1, False), (64, 1, True)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_window_set_matches_oracle(ctx, orc, N, dmin, logb, dtype):
+    """window_set: the zero mode and the stored modes outside the first octant
+    (FftwFunctions.cpp:1248-1262, 1354-1568) against the oracle's window loop."""
+    L = 300.0
+    kf = 2 * np.pi / L
+    ns = N // 2
+    lk = dict(log_binning=True, dlog=float(np.log10((N / 2) / dmin) / ns)) if logb else {}
+    dk = _spectrum(N, 5 * N + dmin)
+    dk[0, 0, 0] = 40.0 + 0j  # a dominant zero mode, as a random catalogue's
+    o = orc.pk_opts(N, L, ns, dmin * kf, kf, 1, 0.0, 3.0, **lk)
+    rw, rn = orc.window_shells(dk, o)
+    p0, _, _, nm = ctx.pk_shells(torch.from_numpy(dk).to(dtype).cuda(), N, L, ns, dmin * kf, kf, 1, 0.0, 3.0,
+                                 window_set=True, **lk)
+    assert np.array_equal(nm, rn)
+    tol = 1e-11 if dtype == torch.complex128 else 2e-6
+    assert np.max(np.abs(p0 - rw) / np.maximum(np.abs(rw), 1e-300)) < tol

@@ -126,6 +126,63 @@ def test_cli_log_binning(tmp_path, orc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nbar_cols,fkp", [(False, True), (True, True), (False, False)])
+def test_cli_random_catalogue(tmp_path, orc, nbar_cols, fkp):
+    """use_random_catalog through the -p workflow (PowerSpectrumF.cpp:741-1363
+    random branch; FftwFunctions.cpp:177-200, 275-281): per-object sums with
+    FKP weights 1/(1 + Pest nbar) (Catalog.cpp:14142-14171), alpha = W_g/W_r,
+    delta = rho_g - alpha rho_r, shot noise (alpha + 1) S_r/normal_r, and the
+    window file of the randoms' spectrum over normal_power/alpha^2.  Against the
+    oracle's deposits, FFT, shells and window mode set."""
+    N, L, scheme = 32, 200.0, "CIC"
+    rng = np.random.default_rng(41)
+    ng, nr = 8000, 30000
+    gal = rng.uniform(0, L, (ng, 3))
+    gal[:, 0] = L * rng.beta(1.5, 1.5, ng)  # a density gradient along x
+    ran = rng.uniform(0, L, (nr, 3))
+    wg = rng.uniform(0.5, 1.5, ng)
+    wr = rng.uniform(0.5, 1.5, nr)
+    nbg = 1e-3 * (1.0 + gal[:, 0] / L)
+    nbr = 1e-3 * (1.0 + ran[:, 0] / L)
+    np.savetxt(tmp_path / "cat.txt", np.column_stack([gal, wg, nbg]), fmt="%.8g")
+    np.savetxt(tmp_path / "ran.txt", np.column_stack([ran, wr, nbr]), fmt="%.8g")
+    Pest = 2000
+    kv = dict(Nft=str(N), Lbox=str(L), mass_assignment_scheme=scheme, DeltaKmin="0", use_random_catalog="true",
+              Random_file="ran.txt", i_coord1_r="0", i_coord2_r="1", i_coord3_r="2", i_weight1_g="3",
+              i_weight1_r="3", FKP_weight="true" if fkp else "false", Pest=str(Pest), file_window="win")
+    if nbar_cols:
+        kv.update(i_mean_density_g="4", i_mean_density_r="4")
+    ini = _ini(tmp_path, **kv)
+    r = subprocess.run([CLI, "-p", ini], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.loadtxt(tmp_path / f"Pk_fkp_BOX_Nft{N}_{scheme}_test.txt")
+    win = np.loadtxt(tmp_path / f"Pk_fkp_BOX_Nft{N}_{scheme}_win.txt")
+    # the catalogue values as the reader sees them (float32 columns)
+    G = np.loadtxt(tmp_path / "cat.txt").astype(np.float32).astype(np.float64)
+    R = np.loadtxt(tmp_path / "ran.txt").astype(np.float32).astype(np.float64)
+    V = L ** 3
+    nbar_c = G[:, 3].sum() / V
+    ng_, nr_ = (G[:, 4], R[:, 4]) if nbar_cols else (np.full(ng, nbar_c), np.full(nr, nbar_c))
+    fg = G[:, 3] / (1.0 + Pest * ng_) if fkp else G[:, 3]
+    fr = R[:, 3] / (1.0 + Pest * nr_) if fkp else R[:, 3]
+    alpha = fg.sum() / fr.sum()
+    normal_r = (nr_ * fr ** 2).sum()
+    SN = (alpha + 1) * (fr ** 2).sum() / normal_r
+    mr = orc.mas(R[:, :3].astype(np.float32), N, L, orc.CIC, weights=fr.astype(np.float32))
+    mesh = orc.mas(G[:, :3].astype(np.float32), N, L, orc.CIC, weights=fg.astype(np.float32)) - alpha * mr
+    kf = 2 * np.pi / L
+    p0, _, _, nm = orc.pk_shells(orc.fft_r2c(mesh), orc.pk_opts(N, L, N // 2, 0.0, kf, orc.CIC, SN, alpha * normal_r))
+    first = 1 if p0[0] < 1 else 0
+    assert np.array_equal(out[:, 2].astype(int), nm[first:])
+    assert np.max(np.abs(out[:, 1] - p0[first:]) / (np.abs(p0[first:]) + SN)) < 1e-4
+    pw, _ = orc.window_shells(orc.fft_r2c(mr), orc.pk_opts(N, L, N // 2, 0.0, kf, orc.CIC, 0.0,
+                                                             normal_r / alpha))
+    firstw = 1 if pw[0] < 1 else 0
+    assert np.array_equal(win[:, 2].astype(int), nm[firstw:])  # the data shells' counts, as the reference writes
+    np.testing.assert_allclose(win[:, 1], pw[firstw:], rtol=2e-4)
+
+
+@pytest.mark.gpu
 def test_cli_density_grid(tmp_path, orc):
     N, L = 32, 300.0
     rng = np.random.default_rng(5)

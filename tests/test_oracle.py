@@ -213,6 +213,52 @@ def test_log_shell_counts_exact(orc, N, dmin, nlog):
         assert [int(e) for e in edges[1:]] == [4 ** (s - 1) for s in range(1, nlog + 1)]
 
 
+@pytest.mark.parametrize("dmin,logb", [(0, False), (1, False), (1, True)])
+def test_window_mode_set(orc, dmin, logb):
+    """The window shells of the random branch visit the zero mode and the
+    stored modes outside the first octant (ix, iy < N/2), not the first
+    octant's other modes.  Independent mask over the stored half grid; and a
+    constant field, whose only mode is k = 0 with d = c N^3, gives
+    (c N^3)^2 / normal in shell 0 when that shell holds k = 0."""
+    N, L, ndel = 16, 200.0, 1
+    kf = TWO_PI / L
+    ns = N // 2
+    lk = dict(log_binning=True, dlog=np.log10(N / 2 / dmin) / ns) if logb else {}
+    o = orc.pk_opts(N, L, ns, dmin * kf, ndel * kf, 1, 0.0, 2.0, mas_correction=True, **lk)
+    rng = np.random.default_rng(4)
+    dk = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    pw, nm = orc.window_shells(dk, o)
+    ix = np.arange(N)[:, None, None]
+    iy = np.arange(N)[None, :, None]
+    mask = _stored_mode_mask(N) & ~((ix < N // 2) & (iy < N // 2))
+    mask[0, 0, 0] = True
+    q = _folded_q(N)
+    fx = np.minimum(np.arange(N), N - np.arange(N))
+    kz = np.arange(N // 2 + 1)
+    corr = (np.sinc(fx / N)[:, None, None] * np.sinc(fx / N)[None, :, None] * np.sinc(kz / N)[None, None, :]) ** 2
+    P = np.abs(dk) ** 2 / corr ** 2
+    if logb:
+        with np.errstate(divide="ignore"):
+            t = np.log10(np.sqrt(q) / dmin) / lk["dlog"]
+        sh = np.where(q == 0, 0, np.floor(t + 1e-12).astype(np.int64) + 1)
+    else:
+        with np.errstate(invalid="ignore"):
+            sh = np.floor((np.sqrt(q) - dmin) / ndel + 1e-12).astype(np.int64)
+            sh[np.sqrt(q) < dmin] = -1
+    exp_n = np.array([np.count_nonzero(mask & (sh == s)) for s in range(ns)])
+    exp_p = np.array([P[mask & (sh == s)].sum() for s in range(ns)]) / np.maximum(exp_n, 1) / 2.0
+    np.testing.assert_array_equal(nm, exp_n)
+    np.testing.assert_allclose(pw, exp_p, rtol=1e-12)
+    c = 0.7
+    const = np.zeros_like(dk)
+    const[0, 0, 0] = c * N ** 3
+    pw1, nm1 = orc.window_shells(const, o)
+    if dmin == 0 or logb:
+        assert np.isclose(pw1[0] * nm1[0] * 2.0, (c * N ** 3) ** 2, rtol=1e-14)
+    else:
+        assert np.all(pw1 == 0)
+
+
 def test_float_shell_rule_survey_counts(orc):
     """The reference's float32 shell index changes per-shell mode counts by a
     summed 2,612 (of 548,955 modes) in 47 of 64 shells at N=128, L=500, ndel=1,
