@@ -232,7 +232,12 @@ static void density_soa(int scheme, ULONG N, real_prec L, real_prec min1, real_p
     d.scheme = (cm_mas)scheme;
     d.interlace_shift = 0.0;
     d.algo = CM_MAS_AUTO;
-    check(cm_mesh_assign(ctx(), &d, dpos.p, dw, n, dmesh.p, CM_F32, 0), "cm_mesh_assign");
+    // the reference adds each particle's stencil onto delta (DELTAc(i,j,k) +=,
+    // Miscelanious.cpp:454-557): a mesh of the right size is accumulated onto,
+    // any other is replaced by the deposit alone
+    const bool acc = delta.size() == n3;
+    if (acc) cmh::h2d(dmesh.p, delta.data(), sizeof(real_prec) * n3);
+    check(cm_mesh_assign(ctx(), &d, dpos.p, dw, n, dmesh.p, CM_F32, acc ? 1 : 0), "cm_mesh_assign");
     delta.resize(n3);
     cmh::d2h(delta.data(), dmesh.p, sizeof(real_prec) * n3);
 }
