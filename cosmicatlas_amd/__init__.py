@@ -465,6 +465,14 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
+    def stats(self):
+        """Admission counters: evictions, bytes evicted, peak held, budget."""
+        e, eb, ph, b = C.c_int64(), C.c_size_t(), C.c_size_t(), C.c_size_t()
+        st = self.lib.cm_server_stats(self.h, C.byref(e), C.byref(eb), C.byref(ph), C.byref(b))
+        if st != 0:
+            raise CosmoError(st, "cm_server_stats failed")
+        return {"evictions": e.value, "evicted_bytes": eb.value, "peak_held": ph.value, "budget": b.value}
+
     def request(self, N, L, scheme, nshells, kmin, dk_bin, pos=None, gen=None, labels=None, **kw):
         """Build a request: device particles `pos` ((n,3) float32) or
         gen=dict(NL=, A=, ns=, seed=) for a GPU-generated mock in the same box."""

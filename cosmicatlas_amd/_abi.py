@@ -218,6 +218,8 @@ def _declare(L):
         "cm_server_submit": (st, [P, C.POINTER(Request), C.POINTER(C.c_int64)]),
         "cm_server_wait": (st, [P, C.c_int64, C.c_double, C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
         "cm_server_destroy": (st, [P]),
+        "cm_server_stats": (st, [P, C.POINTER(C.c_int64), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                 C.POINTER(C.c_size_t)]),
         "cm_measure_batch": (st, [P, C.POINTER(Request), C.c_int, P, P]),
         "cm_uniform": (st, [P, C.c_size_t, C.c_double, C.c_uint64, P]),
     }

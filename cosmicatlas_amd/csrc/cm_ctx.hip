@@ -106,6 +106,25 @@ cm_status scratch(cm_ctx *ctx, const char *name, size_t bytes, void **out) {
     return CM_OK;
 }
 
+size_t scratch_release(cm_ctx *ctx) {
+    size_t freed = 0;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &g : ctx->graph_cache)
+        if (g.exec) hipGraphExecDestroy(g.exec);
+    ctx->graph_cache.clear();
+    for (auto &kv : ctx->scratch) {
+        if (kv.second.ptr) {
+            hipFree(kv.second.ptr);
+            freed += kv.second.bytes;
+        }
+    }
+    ctx->scratch.clear();
+    ctx->resident -= freed;
+    ++ctx->scratch_gen;
+    return freed;
+}
+
 const void *find_table(cm_ctx *ctx, const std::string &key) {
     auto it = ctx->ctables.find(key);
     return it == ctx->ctables.end() ? nullptr : it->second.ptr;

@@ -90,6 +90,9 @@ cm_status hip_fail(cm_ctx *ctx, hipError_t e, const char *what);
 
 // Named persistent device buffer of at least `bytes` (grown on demand).
 cm_status scratch(cm_ctx *ctx, const char *name, size_t bytes, void **out);
+// Frees every work buffer of the context (after its stream drains) and drops
+// its captured graphs; returns the bytes freed.  Tables stay.
+size_t scratch_release(cm_ctx *ctx);
 
 // Device copy of a table computed on the host, uploaded on the first request of
 // `key` and kept for the context's lifetime (no host work or copy afterwards).

@@ -433,9 +433,14 @@ typedef struct cm_server cm_server;
 size_t cm_request_footprint(const cm_request *r);
 
 /* Continuous-batching server on `device`: `slots` (1..16) concurrent requests,
- * each on its own stream with its own resident meshes; a queued request starts
- * when a slot is free and its footprint fits in the HBM budget beside the
- * running ones (hbm_budget 0: 90% of the device). */
+ * each on its own stream with its own resident meshes.  All slots share one
+ * HBM budget (hbm_budget, 0: 90% of the device).  A queued request starts when
+ * a slot is free and its reservation (footprint + 25% + 256 MB, or what the
+ * slot already holds when more) fits beside what the other slots hold:
+ * running requests' reservations and idle slots' resident buffers, which are
+ * freed, largest first, when the request needs the room.  A running slot may
+ * not grow past its reservation; a request that would is queued again to run
+ * alone with the whole budget. */
 cm_status cm_server_create(int device, int slots, size_t hbm_budget, cm_server **out);
 
 /* Queue a request (thread-safe, returns at once).  CM_ERR_BUDGET if the
@@ -454,6 +459,12 @@ cm_status cm_server_destroy(cm_server *s);
 
 /* Submit n requests and wait for all: per-request status and latency (either
  * array may be NULL); returns the first failure or CM_OK. */
+/* Admission counters: idle-slot evictions and the bytes they freed, the most
+ * the slots held together (reservations of running requests plus idle slots'
+ * resident buffers, tables aside), and the budget. */
+cm_status cm_server_stats(cm_server *s, int64_t *evictions, size_t *evicted_bytes, size_t *peak_held,
+                          size_t *budget);
+
 cm_status cm_measure_batch(cm_server *s, const cm_request *reqs, int n, cm_status *statuses,
                            double *latency_ms);
 
