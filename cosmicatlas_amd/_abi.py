@@ -205,6 +205,7 @@ def _declare(L):
         "cm_poisson": (st, [P, C.POINTER(TwebOpts), P, P]),
         "cm_tweb_from_phi": (st, [P, C.POINTER(TwebOpts), P, P, P, C.POINTER(TwebStats)]),
         "cm_tweb_f64": (st, [P, C.POINTER(TwebOpts), P, P, P, P, C.POINTER(TwebStats)]),
+        "cm_pk_shells_noalloc": (st, [P, C.POINTER(PkOpts), C.c_int, P, P, P, P, P]),
         "cm_measure": (st, [P, C.POINTER(MeasureOpts), P, P, C.c_size_t, P, P, P, P, P,
                             C.POINTER(MeasureInfo)]),
         "cm_zeldovich": (st, [P, C.POINTER(ZeldovichOpts), P]),

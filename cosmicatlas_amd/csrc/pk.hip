@@ -12,7 +12,9 @@
 // are monotone along kz, so runs are contiguous); the last lane of each run adds
 // the run total into FP64 LDS accumulators, and each workgroup adds its
 // accumulators into the FP64 global sums once.
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "cm_internal.hpp"
@@ -187,8 +189,7 @@ static bool near_int(double v, long &out) {
 // x_s is evaluated in long double; a value within 1e-9 relative of an integer
 // (e.g. (N/2)^(2m/n) = 4^m) is that integer, so a mode on an edge goes to the
 // shell above as the exact rule says.  Thresholds are capped at qmax + 1.
-void log_shell_edges(double Dmin, double dlog, int ns, long qmax, std::vector<int> &Q) {
-    Q.resize((size_t)ns + 1);
+void log_shell_edges(double Dmin, double dlog, int ns, long qmax, int *Q) {
     for (int s = 0; s <= ns; ++s) {
         const long double e = (long double)Dmin * powl(10.0L, (long double)(s - 1) * (long double)dlog);
         const long double x = e * e;
@@ -227,7 +228,10 @@ PkK make_pkk(cm_ctx *ctx, const cm_pk_opts *o, cm_status *st) {
         p.int_rule = 0;
         p.float_rule = 0;
         p.qedge = cached_table<int>(ctx, table_key("logq:%d:%d:%.17g:%.17g", o->N, o->nshells, p.Dmin, o->dlog),
-                                    [&](std::vector<int> &q) { log_shell_edges(p.Dmin, o->dlog, o->nshells, qmax, q); },
+                                    [&](std::vector<int> &q) {
+                                        q.resize((size_t)o->nshells + 1);
+                                        log_shell_edges(p.Dmin, o->dlog, o->nshells, qmax, q.data());
+                                    },
                                     st);
     }
     return p;
@@ -321,5 +325,163 @@ extern "C" cm_status cm_pk_shells(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dty
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     pk_finalize(o->nshells, o->normal_power, o->shot_noise * o->normal_power, hs.data(), hc.data(), pk0, pk2, pk4,
                 nmodes);
+    return CM_OK;
+}
+
+// ------------------------------------------------- shells without allocation
+// cm_pk_shells_noalloc keeps everything it needs beyond its inputs and outputs
+// in one file-scope device array: the per-axis tables, the shell sums and
+// counts, the log-shell thresholds and the finalised results.  The array's
+// device address is looked up once per device (hipGetSymbolAddress), and only
+// that address is handed to kernels and copies.
+constexpr int kNaMaxN = 1024;       // mesh side
+constexpr int kNaMaxShells = 1024;  // shells (the shell kernel's LDS holds 4 FP64 sums each)
+constexpr int kNaM1 = kNaMaxN / 2 + 1;
+// layout in 8-byte words: tables 2*M1 | sums 4*ns | counts ns | pk0,pk2,pk4 3*ns | nmodes ns | thresholds (ns+1) ints
+constexpr size_t kNaWords = 2 * (size_t)kNaM1 + 9 * (size_t)kNaMaxShells + kNaMaxShells / 2 + 2;  // 78,880 bytes
+
+// The fixed work area of cm_pk_shells_noalloc: each device holds its own copy
+// for the life of the process.  Host code uses only the address
+// hipGetSymbolAddress returns for it on the current device.
+static __device__ __align__(256) unsigned long long scratch[kNaWords];
+
+namespace cm {
+
+static std::mutex na_mu;
+static void *na_ptr[64];  // per device: the array's device address
+
+static cm_status na_scratch(cm_ctx *ctx, void **out) {
+    if (ctx->device < 0 || ctx->device >= 64) return fail(ctx, CM_ERR_UNSUPPORTED, "device index %d", ctx->device);
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_ptr[ctx->device]) {
+        // with the context's device current; the caller's current device is restored
+        int prev = 0;
+        CM_HIP(ctx, hipGetDevice(&prev));
+        CM_HIP(ctx, hipSetDevice(ctx->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        hipSetDevice(prev);
+        if (e != hipSuccess || !p) return hip_fail(ctx, e, "hipGetSymbolAddress(scratch)");
+        na_ptr[ctx->device] = p;
+    }
+    *out = na_ptr[ctx->device];
+    return CM_OK;
+}
+
+// The tables of pk_tables on the device: 1/W^2 and the aliased shot-noise
+// factor C1 per axis index (no interlacing table: the shells do not use it).
+__global__ void k_na_tables(int N, int scheme, int mas_correction, double *__restrict__ tab) {
+    const int M1 = N / 2 + 1;
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < M1; i += blockDim.x * gridDim.x) {
+        const double a = M_PI * (double)i / (double)N;
+        const double sinc = i == 0 ? 1.0 : sin(a) / a;
+        double ic = 1.0;
+        if (mas_correction) {
+            const double cp = pow(sinc, scheme + 1);
+            ic = 1.0 / (cp * cp);
+        }
+        const double s = sin(a), s2 = s * s;
+        double sq = 1.0;
+        if (scheme == 1) sq = 1.0 - (2.0 / 3.0) * s2;
+        else if (scheme == 2) sq = 1.0 - s2 + (2.0 / 15.0) * s2 * s2;
+        else if (scheme == 3) sq = 1.0 - (4.0 / 3.0) * s2 + (2.0 / 5.0) * s2 * s2 - (4.0 / 315.0) * s2 * s2 * s2;
+        tab[i] = ic;
+        tab[M1 + i] = sq;
+    }
+}
+
+// pk_finalize on the device, into the results region.
+__global__ void k_na_finalize(int ns, double normal_power, double sn_norm, const double *__restrict__ sum,
+                              const unsigned long long *__restrict__ cnt, double *__restrict__ pk,
+                              long long *__restrict__ nm) {
+#pragma clang fp contract(off)
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        const double den = (double)cnt[i] * normal_power;
+        pk[i] = cnt[i] ? (sum[i] - sn_norm * sum[ns + i]) / den : 0.0;
+        pk[ns + i] = cnt[i] ? 5.0 * sum[2 * ns + i] / den : 0.0;
+        pk[2 * ns + i] = cnt[i] ? 9.0 * sum[3 * ns + i] / den : 0.0;
+        nm[i] = (long long)cnt[i];
+    }
+}
+
+}  // namespace cm
+
+extern "C" cm_status cm_pk_shells_noalloc(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dtype, const void *dk,
+                                          double *pk0, double *pk2, double *pk4, int64_t *nmodes) {
+    using namespace cm;
+    if (!ctx) return CM_ERR_INVALID;
+    if (!o || !dk) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells_noalloc: null pointer");
+    if (o->N < 2 || (o->N & 1)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells_noalloc: N=%d must be even", o->N);
+    if (o->N > kNaMaxN || o->nshells > kNaMaxShells)
+        return fail(ctx, CM_ERR_UNSUPPORTED,
+                    "cm_pk_shells_noalloc: N=%d, nshells=%d beyond its fixed scratch (N <= %d, nshells <= %d)", o->N,
+                    o->nshells, kNaMaxN, kNaMaxShells);
+    if (!(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells_noalloc: L must be > 0");
+    cm_status s = check_binning(ctx, o->nshells, o->kmin, o->dk, o->log_binning, o->dlog, o->float_shell_rule,
+                                "cm_pk_shells_noalloc");
+    if (s != CM_OK) return s;
+    if ((int)o->scheme < 0 || (int)o->scheme > 3) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells_noalloc: scheme");
+    if (dtype != CM_F32 && dtype != CM_F64) return fail(ctx, CM_ERR_INVALID, "cm_pk_shells_noalloc: dtype");
+    if (!(o->normal_power > 0) && (pk0 || pk2 || pk4))
+        return fail(ctx, CM_ERR_INVALID, "cm_pk_shells_noalloc: normal_power must be > 0");
+    void *base;
+    if ((s = na_scratch(ctx, &base)) != CM_OK) return s;
+    const int N = o->N, M1 = N / 2 + 1, ns = o->nshells;
+    auto *w = (unsigned long long *)base;
+    double *tab = (double *)w;                                 // 2*M1
+    double *sums = (double *)(w + 2 * (size_t)M1);             // 4*ns
+    unsigned long long *cnts = w + 2 * (size_t)M1 + 4 * (size_t)ns;
+    double *res = (double *)(cnts + ns);                       // 3*ns
+    long long *nm = (long long *)(res + 3 * (size_t)ns);       // ns
+    int *qedge = (int *)(nm + ns);                             // ns+1
+    // shell kernel LDS: the two tables in FP64, 4 FP64 sums and a count per shell
+    const size_t lds = sizeof(double) * (2 * (size_t)M1 + 4 * (size_t)ns) + sizeof(unsigned) * (size_t)ns;
+    if (lds > ctx->lds_per_block)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "cm_pk_shells_noalloc: %d shells need %zu bytes of LDS", ns, lds);
+    PkK p;
+    const double kf = 2.0 * M_PI / o->L;
+    {
+        // the linear rules of make_pkk; the log thresholds go into the scratch
+        cm_pk_opts lin = *o;
+        lin.log_binning = 0;
+        p = make_pkk(ctx, &lin, &s);
+        if (s != CM_OK) return s;
+    }
+    if (o->log_binning) {
+        int qh[kNaMaxShells + 1];  // host thresholds, copied into the work area
+        const long h = N / 2, qmax = 3 * h * h;
+        log_shell_edges(o->kmin / kf, o->dlog, ns, qmax, qh);
+        CM_HIP(ctx, hipMemcpyAsync(qedge, qh, sizeof(int) * (size_t)(ns + 1), hipMemcpyHostToDevice, ctx->stream));
+        CM_HIP(ctx, hipStreamSynchronize(ctx->stream));  // qh lives on this stack frame
+        p.int_rule = 0;
+        p.float_rule = 0;
+        p.qedge = qedge;
+    }
+    p.window = o->window_set ? 1 : 0;
+    // every call shares this device's array: the sums start from zero (fill
+    // kernel, no memset node), the tables are rewritten
+    if ((s = launch(ctx, "pk_zero", k_pk_zero, dim3(1), dim3(256), 0, sums, cnts, ns)) != CM_OK) return s;
+    if ((s = launch(ctx, "pk_na_tables", k_na_tables, dim3((M1 + 255) / 256), dim3(256), 0, N, (int)o->scheme,
+                    o->mas_correction, tab)) != CM_OK)
+        return s;
+    const size_t total = (size_t)N * N * M1;
+    const unsigned g = (unsigned)std::min<size_t>((total + 255) / 256, (size_t)ctx->cu_count * 8);
+    if (dtype == CM_F32)
+        s = launch(ctx, "pk_shells", k_pk_shells<float>, dim3(g), dim3(256), lds, (const v2pk<float> *)dk, p,
+                   (const double *)tab, sums, cnts);
+    else
+        s = launch(ctx, "pk_shells", k_pk_shells<double>, dim3(g), dim3(256), lds, (const v2pk<double> *)dk, p,
+                   (const double *)tab, sums, cnts);
+    if (s != CM_OK) return s;
+    if ((s = launch(ctx, "pk_na_finalize", k_na_finalize, dim3(1), dim3(256), 0, ns, o->normal_power,
+                    o->shot_noise * o->normal_power, (const double *)sums, (const unsigned long long *)cnts, res,
+                    nm)) != CM_OK)
+        return s;
+    const size_t b = sizeof(double) * (size_t)ns;
+    if (pk0) CM_HIP(ctx, hipMemcpyAsync(pk0, res, b, hipMemcpyDeviceToHost, ctx->stream));
+    if (pk2) CM_HIP(ctx, hipMemcpyAsync(pk2, res + ns, b, hipMemcpyDeviceToHost, ctx->stream));
+    if (pk4) CM_HIP(ctx, hipMemcpyAsync(pk4, res + 2 * ns, b, hipMemcpyDeviceToHost, ctx->stream));
+    if (nmodes) CM_HIP(ctx, hipMemcpyAsync(nmodes, nm, sizeof(int64_t) * (size_t)ns, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return CM_OK;
 }

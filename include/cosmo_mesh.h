@@ -176,6 +176,19 @@ typedef struct {
 cm_status cm_pk_shells(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dtype, const void *dk,
                        double *pk0, double *pk2, double *pk4, int64_t *nmodes);
 
+/* cm_pk_shells without any device allocation, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as cm_pk_shells; its work memory (tables, shell sums and counts,
+ * log thresholds, results: 78,880 bytes) is one fixed array per device that
+ * the library holds until the process ends.  Limits: N <= 1024 and
+ * nshells <= 1024 (CM_ERR_UNSUPPORTED beyond).  Every call on a device uses
+ * that device's one array, so calls that overlap on one device race; calls
+ * in sequence (any contexts and streams) are safe: each starts by clearing
+ * its sums.  Synchronises the context's stream like cm_pk_shells. */
+cm_status cm_pk_shells_noalloc(cm_ctx *ctx, const cm_pk_opts *o, cm_dtype dtype, const void *dk,
+                               double *pk0, double *pk2, double *pk4, int64_t *nmodes);
+
 /* ---------------------------------------------------------------------- T-web */
 typedef struct {
     int N;
