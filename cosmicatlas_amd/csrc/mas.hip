@@ -619,9 +619,10 @@ __global__ void __launch_bounds__(256) k_tile_zero_split(const unsigned *__restr
 //   unit weights: 32-bit cells with a per-item quantum 2^-k, k the largest
 //     integer with cnt * wmax * 2^k <= 2^31 (cnt = records of the item, wmax
 //     = the scheme's largest stencil weight), so no cell can overflow (the
-//     factor 2 covers rounding); an add rounds by at most 2^-(k+1) (2^-23 for
-//     a TSC chunk of 730 records, 2^-21 at the 4096-record maximum).  NGP adds
-//     2^k per record: counts stay exact.  32-bit atomics move half the LDS
+//     factor 2 covers rounding); an add is off by at most 3/4 of a quantum
+//     plus the float rounding of the product (k = 22 for a TSC chunk of 730
+//     records, 20 at the 4096-record maximum).  NGP adds 2^k per record:
+//     counts stay exact.  32-bit atomics move half the LDS
 //     words of 64-bit ones and halve the grid, so more workgroups fit a CU.
 //   general weights: 64-bit cells, quantum 2^-32.
 __device__ __forceinline__ float from_fixed(long long q) { return (float)((double)q * (1.0 / 4294967296.0)); }
@@ -639,7 +640,10 @@ __device__ __forceinline__ int unit_quantum_exp(unsigned cnt) {
 // come from up to R cells outside), so its stencil spans [-2R, T+2R): the LDS
 // grid has S = T+4R cells per axis with local cell u at index u+2R, every
 // contribution lands in it without a range test, and only [0,T) is written
-// out.  Unit weights: 32-bit cells, quantum 1/qs (qs = 2^k).
+// out.  Unit weights: 32-bit cells, quantum 1/qs (qs = 2^k); the quantum is
+// folded into the x weights, and a stencil value v (>= 0) becomes the integer
+// trunc(fl(v + 1/2)) with one fused multiply-add and one conversion: within
+// 3/4 of a quantum of v, plus the float rounding of the product.
 template <int SCHEME, bool UNIT, int S>
 __device__ __forceinline__ void deposit_record(void *lgrid, int base, const float f[3], float wt, float qs) {
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
@@ -647,18 +651,22 @@ __device__ __forceinline__ void deposit_record(void *lgrid, int base, const floa
     float wa[3][5];
 #pragma unroll
     for (int a = 0; a < 3; ++a) axis_weights<float>(SCHEME, f[a], wa[a]);
+    const float sx = UNIT ? qs : wt;
+#pragma unroll
+    for (int a = 0; a < W; ++a) wa[0][a + 2 - R] *= sx;
 #pragma unroll
     for (int a = 0; a < W; ++a) {
 #pragma unroll
         for (int b = 0; b < W; ++b) {
-            const float vab = wt * wa[0][a + 2 - R] * wa[1][b + 2 - R];
+            const float vab = wa[0][a + 2 - R] * wa[1][b + 2 - R];
             const int rowi = base + (a * S + b) * S;
 #pragma unroll
             for (int e = 0; e < W; ++e) {
-                const float v = vab * wa[2][e + 2 - R];
                 if constexpr (UNIT) {
-                    atomicAdd((unsigned *)lgrid + rowi + e, __float2uint_rn(v * qs));
+                    const float vq = fmaf(vab, wa[2][e + 2 - R], 0.5f);
+                    atomicAdd((unsigned *)lgrid + rowi + e, (unsigned)vq);
                 } else {
+                    const float v = vab * wa[2][e + 2 - R];
                     const long long q = __double2ll_rn((double)v * 4294967296.0);
                     atomicAdd((unsigned long long *)lgrid + rowi + e, (unsigned long long)q);
                 }
@@ -684,6 +692,18 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
     static_assert((S3 * WB) % 16 == 0, "LDS grid is cleared in 16-byte words");
     const int tid = threadIdx.x;
     const unsigned total = *nitems;
+    // half a cell up: fraction + 1/2; a fraction past 1 carries into the cell
+    // field above it (bits 27..31 stay below 32 - kLcBias)
+    const unsigned hadd = half ? 1u << (kFracBits - 1) : 0u;
+    // write-out: thread tid owns tile cells q = tid + k NT, k < NW; q and
+    // q + NT differ by NT / T^2 planes along x, so the LDS and mesh positions
+    // of cell k are the ones of cell 0 plus k times a constant stride
+    constexpr int TT = T * T * T, NW = (TT + NT - 1) / NT;
+    static_assert(NT % (T * T) == 0 || TT <= NT, "write-out stride");
+    constexpr int XS = NT / (T * T);
+    const int wz = tid % T, wy = (tid / T) % T, wx = tid / (T * T);
+    const int li0 = ((wx + 2 * R) * S + (wy + 2 * R)) * S + (wz + 2 * R);
+    const size_t mstride = (size_t)XS * N * N;
     unsigned item = blockIdx.x;
     uint4 it = item < total ? items[item] : make_uint4(0u, 0u, 0u, 0u);
     uint3 rec[RPT];
@@ -714,13 +734,10 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
         for (int r = 0; r < RPT; ++r) {
             const unsigned idx = it.y + (unsigned)(tid + r * NT);
             if (idx >= it.z) continue;
-            unsigned wd[3] = {rec[r].x, rec[r].y, rec[r].z};
+            unsigned wd[3] = {rec[r].x + hadd, rec[r].y + hadd, rec[r].z + hadd};
             bool inside = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                // half a cell up: fraction + 1/2; a fraction past 1 carries into
-                // the cell field above it (bits 27..31 stay below 32 - kLcBias)
-                if (half) wd[a] += 1u << (kFracBits - 1);
                 // a record whose stencil misses the tile's cells (the extra ghosts
                 // kept for the other mesh) is skipped
                 const int lc = (int)(wd[a] >> kFracBits) - kLcBias;
@@ -738,22 +755,20 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
         it = itn;
         load_item(it);  // in flight during the write-out
         __syncthreads();
-        constexpr int TT = T * T * T;
+        float *mp = mesh + ((size_t)(o[0] + wx) * N + (o[1] + wy)) * N + (o[2] + wz);
 #pragma unroll
-        for (int k = 0; k < (TT + NT - 1) / NT; ++k) {
-            const int q = tid + k * NT;
-            if (TT % NT != 0 && q >= TT) break;
-            const int iz = q % T, iy = (q / T) % T, ix = q / (T * T);
-            const int li = ((ix + 2 * R) * S + (iy + 2 * R)) * S + (iz + 2 * R);
+        for (int k = 0; k < NW; ++k) {
+            if (TT % NT != 0 && tid + k * NT >= TT) break;
+            const int li = li0 + k * XS * S * S;
             const float v = UNIT ? ldexpf(__uint2float_rn(((const unsigned *)lgrid)[li]), -qk)
                                  : from_fixed(((const long long *)lgrid)[li]);
-            const size_t idx = ((size_t)(o[0] + ix) * N + (o[1] + iy)) * N + (o[2] + iz);
+            float *dst = mp + k * mstride;
             if (split) {
-                if (v != 0.0f) unsafeAtomicAdd(&mesh[idx], v);
+                if (v != 0.0f) unsafeAtomicAdd(dst, v);
             } else if (accumulate) {
-                mesh[idx] += v;
+                *dst += v;
             } else {
-                mesh[idx] = v;
+                *dst = v;
             }
         }
         __syncthreads();

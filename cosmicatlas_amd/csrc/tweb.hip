@@ -459,7 +459,7 @@ cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, doub
 //     along a diagonal), so white noise of rms e moves it by rms <= 1.5 e / d^2.
 //   * The largest of the 3 N^3 eigenvalue errors of a Gaussian model lies
 //     below z = sqrt(2 ln(3 N^3)) standard deviations.
-// K(N) = 1.5 sqrt(log2 N^3) sqrt(2 ln(3 N^3)): 46.8 at N = 512.  The stencil
+// K(N) = 1.5 sqrt(log2 N^3) sqrt(2 ln(3 N^3)): 49.1 at N = 512.  The stencil
 // sums of the float32 T-web round in proportion to differences of nearby phi
 // values, far below the rounding of phi itself.
 double tweb_tol_coeff(int N) {

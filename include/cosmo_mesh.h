@@ -205,7 +205,7 @@ typedef struct {
 /* margin_tol value that counts the cells whose label the rounding of the
  * potential's arithmetic can change: tol = K(N) u ||phi||_rms / d^2 with
  * d = L/N, u the unit roundoff of phi (2^-24 float32, 2^-53 FP64) and
- * K(N) = 1.5 sqrt(log2 N^3) sqrt(2 ln(3 N^3)) (46.8 at N = 512), from a
+ * K(N) = 1.5 sqrt(log2 N^3) sqrt(2 ln(3 N^3)) (49.1 at N = 512), from a
  * random-rounding model: the FFT chain leaves phi a white error of rms
  * sqrt(log2 N^3) u ||phi||_rms; an eigenvalue moves by the second difference
  * of that error along its eigenvector, a stencil of L2 norm <= 1.5/d^2; and
