@@ -124,38 +124,43 @@ def traffic_from_pmc(fetch_csv, write_csv, out_json):
 
 
 def cpu_baseline(pos, quick=False):
-    """OpenMP restatement of the reference path on the host cores, on a bounded
-    sample: the whole pipeline at NS^3 with NS^3/8 particles (1/8 of the mesh and
-    particle count per axis-halving), scaled to the 512^3 mock by the ratio of
-    work (N^3 log N for the FFT, N^3 for the mesh passes, N_p for the deposit)."""
+    """OpenMP restatement of the reference path (oracle/cpu_ref.c) on the host
+    cores, measured on the 512^3 mock itself: the same particles, TSC onto
+    512^3, r2c, 128 shells, T-web; 1 warm-up and 3 timed runs on all cores of
+    the machine's share (16 per GPU), then 3 runs on the reference's 12 OpenMP
+    threads (_NTHREADS_).  value = 1 / the median run on all cores.  A cross-check
+    scaled from a 256^3 sample (N^3 for the mesh passes, N^3 log N for the
+    FFTs, N_p for the deposit) rides along.  quick: the scaled 128^3 sample
+    only (iteration runs)."""
     try:
         import oracle.cpu_ref as cr
     except Exception as e:  # pragma: no cover
         return {"value": None, "unit": "mocks/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
-    # all host cores of the machine's share (16 per GPU), and the reference's
-    # hard-coded 12 OpenMP threads (_NTHREADS_)
     threads = int(os.environ.get("CPU_BASELINE_THREADS", min(16, os.cpu_count() or 1)))
-    NS = 128 if quick else 256
-    reps = 1 if quick else 5
-    npart = (NS // 2) ** 3
-    sub = pos[:: max(1, len(pos) // npart)][:npart].astype(np.float32) * np.float32(1.0)
-    Ls = L * NS / N  # same cell size d = L/N
-    sub = np.mod(sub, Ls).astype(np.float32)
+    kw = dict(scheme=2, ndel=NDEL, R=R_SMOOTH, lambda_th=LAMBDA_TH)
 
-    def mocks_per_s(nthreads):
-        t = cr.time_pipeline(sub, NS, Ls, scheme=2, ndel=NDEL, R=R_SMOOTH, lambda_th=LAMBDA_TH, threads=nthreads,
-                             repeats=reps, warmup=1)
-        # scale: mesh passes ~N^3, FFT ~N^3 log N, deposit ~N_p
+    def stages(t):
+        return {k: round(float(x), 4) for k, x in t.items() if k != "runs"}
+
+    def say(tag):
+        return lambda i, sec: print(f"cpu baseline {tag}: run {i + 1} {sec:.2f} s", file=sys.stderr, flush=True)
+
+    def scaled(NS):
+        npart = (NS // 2) ** 3
+        sub = pos[:: max(1, len(pos) // npart)][:npart].astype(np.float32)
+        Ls = L * NS / N  # same cell size d = L/N
+        sub = np.mod(sub, Ls).astype(np.float32)
+        t = cr.time_pipeline(sub, NS, Ls, threads=threads, repeats=1, warmup=0, progress=say(f"{NS}^3 sample"), **kw)
         f_mesh = (N / NS) ** 3
         f_fft = f_mesh * math.log2(N) / math.log2(NS)
-        f_part = (NL ** 3) / npart
+        f_part = len(pos) / npart
+        tw_fft = t["tweb_fft"] / max(t["tweb"], 1e-12)
         est = t["mas"] * f_part + t["fft"] * f_fft + t["pk"] * f_mesh + t["tweb"] * (
-            f_mesh * (t["tweb_fft"] / max(t["tweb"], 1e-12)) * math.log2(N) / math.log2(NS)
-            + f_mesh * (1 - t["tweb_fft"] / max(t["tweb"], 1e-12)))
-        return 1.0 / est, t
+            f_mesh * tw_fft * math.log2(N) / math.log2(NS) + f_mesh * (1 - tw_fft))
+        return {"value": 1.0 / est, "sample": f"{NS}^3 mesh, {npart} particles, 1 run of "
+                f"{sum(t[k] for k in ('mas', 'fft', 'pk', 'tweb')):.2f} s, scaled by N^3, N^3 log N and N_p",
+                "stage_seconds_sample": stages(t)}
 
-    v, t = mocks_per_s(threads)
-    v12, t12 = mocks_per_s(12)
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -165,14 +170,26 @@ def cpu_baseline(pos, quick=False):
                     break
     except OSError:
         pass
-    return {"value": v, "unit": "mocks/s", "cores": threads, "kind": "port",
-            "sample": f"full pipeline at {NS}^3 mesh with {npart} particles (same cell size, TSC, "
-                      f"{NS // NDEL // 2} shells, T-web), median of {reps} runs after 1 warm-up, "
-                      f"{sum(t[k] for k in ('mas', 'fft', 'pk', 'tweb')):.2f} s per run, scaled to 512^3 / 256^3 "
-                      f"particles by N^3, N^3 log N and N_p; the port's own radix-2 CPU FFT (no FFTW on the host)",
-            "stage_seconds_sample": {k: round(x, 4) for k, x in t.items()},
-            "threads_12": {"value": v12, "stage_seconds_sample": {k: round(x, 4) for k, x in t12.items()}},
-            "host": {"nproc": os.cpu_count(), "model": model}}
+    host = {"nproc": os.cpu_count(), "model": model}
+    fft_note = "the port's own radix-2 CPU FFT (FFTW is not on the host)"
+    if quick:
+        x = scaled(128)
+        return {"value": x["value"], "unit": "mocks/s", "cores": threads, "kind": "port",
+                "sample": x["sample"] + "; " + fft_note, "stage_seconds_sample": x["stage_seconds_sample"],
+                "host": host}
+    full = np.ascontiguousarray(pos, np.float32)
+    t = cr.time_pipeline(full, N, L, threads=threads, repeats=3, warmup=1, progress=say(f"{threads} threads"), **kw)
+    t12 = cr.time_pipeline(full, N, L, threads=12, repeats=3, warmup=0, progress=say("12 threads"), **kw)
+    med = sorted(t["runs"])[1]
+    med12 = sorted(t12["runs"])[1]
+    cross = scaled(256)
+    return {"value": 1.0 / med, "unit": "mocks/s", "cores": threads, "kind": "port",
+            "sample": f"the full {N}^3 mock ({len(pos)} particles, TSC, {N // NDEL // 2} shells, T-web): median of "
+                      f"3 runs after 1 warm-up, {med:.2f} s per mock; {fft_note}",
+            "runs_s": [round(x, 3) for x in t["runs"]], "stage_seconds": stages(t),
+            "threads_12": {"value": 1.0 / med12, "runs_s": [round(x, 3) for x in t12["runs"]],
+                           "stage_seconds": stages(t12)},
+            "scaled_cross_check": cross, "host": host}
 
 
 def served(cm, posd, b, slots, nreq, rate=None, seed=0):
@@ -211,9 +228,11 @@ def served(cm, posd, b, slots, nreq, rate=None, seed=0):
             th.join()
         lat = np.array([t.result()["latency_ms"] for t in tickets])
         T = time.perf_counter() - t0
+        adm = srv.stats()
     return {"value": round(nreq / T, 2), "unit": "mocks/s", "slots": slots, "requests": nreq,
             "arrivals": "all at t=0" if rate is None else f"Poisson, {rate:.0f}/s",
-            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "admission": adm}
 
 
 MIXED_NL = (48, 64, 96, 128, 192, 256, 384)  # FFT sizes for the generator's Lagrangian grid
@@ -272,6 +291,7 @@ def served_mixed(cm, slots=3, nreq=96, rate=None, seed=5, specs=None):
             th.join()
         lat = np.array([t.result()["latency_ms"] for t in tickets])
         T = time.perf_counter() - t0
+        adm = srv.stats()
     npart = sum(sp[1] ** 3 for sp in specs)
     return {"value": round(nreq / T, 2), "unit": "mocks/s", "slots": slots, "requests": nreq,
             "particles": int(npart),
@@ -279,7 +299,7 @@ def served_mixed(cm, slots=3, nreq=96, rate=None, seed=5, specs=None):
                         "generated on the GPU, NGP/CIC/TSC, 64 shells, T-web on half",
             "arrivals": "all at t=0" if rate is None else f"Poisson, {rate:.0f}/s",
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3)}, specs
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "admission": adm}, specs
 
 
 def batched(cm, nmocks=64, slots=3, seed=7):
@@ -328,13 +348,15 @@ def batched(cm, nmocks=64, slots=3, seed=7):
         t0 = time.perf_counter()
         res = srv.batch(tickets())
         T = time.perf_counter() - t0
+        adm = srv.stats()
     lat = np.array([r["latency_ms"] for r in res])
     return {"value": round(nmocks / T, 2), "unit": "mocks/s", "mocks": nmocks, "slots": slots,
             "particles": int(npart), "seconds": round(T, 3),
             "workload": "64 uneven mocks generated on the GPU (N 128-512, NL^3 particles 1.1e5-5.7e7, "
                         "NGP/CIC/TSC, 64 shells, T-web)",
             "autotune": tuned, "graphs": "hipGraph replay of repeated request shapes per slot",
-            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "admission": adm}
 
 
 ROW_KERNELS = KERNELS + ["zel_spectra", "zel_displace", "uniform", "fkp_wscale", "fkp_wsums", "bk_masks", "bk_sums",
@@ -702,6 +724,13 @@ def main():
                                   "frac": round(bound_ms / dom[1]["mean_ms"], 3),
                                   "note": "instruction count from a rocprofv3 SQ_INSTS_VALU pass of this workload; "
                                           "frac = issue-bound time / measured mean"}
+            # the roof that binds: the larger of the VALU issue time and the
+            # time of the kernel's measured HBM traffic at peak bandwidth
+            hbm_ms = (traffic or ab) / (HBM_PEAK_GBS * 1e9) * 1e3
+            if bound_ms > hbm_ms:
+                roof["bound"] = "valu"
+                roof["note"] = ("bound by VALU issue (valu_issue.bound_ms > the HBM time of its traffic, "
+                                f"{hbm_ms:.4f} ms); achieved/peak/frac are its algorithmic bytes against HBM peak")
 
     if args.fused_only:
         print(json.dumps({"metric": METRIC, "value": round(value, 3), "unit": "mocks/s", "n_gpus": 1,

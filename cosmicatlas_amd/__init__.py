@@ -466,12 +466,14 @@ class Server:
         self.close()
 
     def stats(self):
-        """Admission counters: evictions, bytes evicted, peak held, budget."""
-        e, eb, ph, b = C.c_int64(), C.c_size_t(), C.c_size_t(), C.c_size_t()
-        st = self.lib.cm_server_stats(self.h, C.byref(e), C.byref(eb), C.byref(ph), C.byref(b))
+        """Admission counters: evictions, bytes evicted, peak held, budget, and
+        the requests re-run alone after outgrowing their reservation."""
+        e, eb, ph, b, rr = C.c_int64(), C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_int64()
+        st = self.lib.cm_server_stats(self.h, C.byref(e), C.byref(eb), C.byref(ph), C.byref(b), C.byref(rr))
         if st != 0:
             raise CosmoError(st, "cm_server_stats failed")
-        return {"evictions": e.value, "evicted_bytes": eb.value, "peak_held": ph.value, "budget": b.value}
+        return {"evictions": e.value, "evicted_bytes": eb.value, "peak_held": ph.value, "budget": b.value,
+                "reruns": rr.value}
 
     def request(self, N, L, scheme, nshells, kmin, dk_bin, pos=None, gen=None, labels=None, **kw):
         """Build a request: device particles `pos` ((n,3) float32) or
@@ -515,7 +517,10 @@ class Server:
         res = []
         for t, s_, l_ in zip(tickets, sts, lat):
             if s_ != 0:
-                raise CosmoError(int(s_), f"batch request failed: {self.lib.cm_status_string(int(s_)).decode()}")
+                msg = C.create_string_buffer(512)
+                self.lib.cm_server_last_error(self.h, msg, 512)
+                raise CosmoError(int(s_), f"batch request failed: {self.lib.cm_status_string(int(s_)).decode()}"
+                                          f" ({msg.value.decode()})")
             info = t.out["_info"]
             t.out.update({"n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
                           "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal,

@@ -220,8 +220,9 @@ def _declare(L):
         "cm_server_wait": (st, [P, C.c_int64, C.c_double, C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
         "cm_server_destroy": (st, [P]),
         "cm_server_stats": (st, [P, C.POINTER(C.c_int64), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
-                                 C.POINTER(C.c_size_t)]),
+                                 C.POINTER(C.c_size_t), C.POINTER(C.c_int64)]),
         "cm_measure_batch": (st, [P, C.POINTER(Request), C.c_int, P, P]),
+        "cm_server_last_error": (st, [P, C.c_char_p, C.c_size_t]),
         "cm_uniform": (st, [P, C.c_size_t, C.c_double, C.c_uint64, P]),
     }
     for name, (res, args) in sig.items():

@@ -65,12 +65,12 @@ struct Job {
     double t_submit = 0, t_start = 0, t_done = 0;
 };
 
-// Reservation of a running job: its footprint estimate with a quarter and
-// 256 MB of headroom (at most the budget), or what its slot already holds when
-// that is more (the resident buffers are reused, not reallocated).
+// Reservation of a running job: what its slot already holds plus its
+// footprint estimate with a quarter and 256 MB of headroom, at most the budget.
+// The resident buffers are kept: a request of another shape may grow any of
+// them by up to its own footprint, so they do not count towards it.
 static size_t reservation(size_t footprint, size_t resident, size_t budget) {
-    const size_t r = std::min(footprint + footprint / 4 + ((size_t)256 << 20), budget);
-    return std::max(r, resident);
+    return std::min(resident + footprint + footprint / 4 + ((size_t)256 << 20), budget);
 }
 
 }  // namespace cm
@@ -93,8 +93,9 @@ struct cm_server {
     std::unordered_map<int64_t, cm::Job> jobs;
     int64_t next = 1;
     bool stop = false;
-    int64_t evictions = 0;
+    int64_t evictions = 0, reruns = 0;
     size_t evicted_bytes = 0, peak_held = 0;
+    std::string last_err;  // message of the most recent failed request
 };
 
 namespace cm {
@@ -214,10 +215,12 @@ static void worker_main(cm_server *sv, int slot) {
             // the footprint estimate was short: run again with the whole budget
             // once nothing else runs
             done.alone = true;
+            ++sv->reruns;
             sv->queue.push_front(t);
         } else {
             done.status = st;
             done.err = err;
+            if (st != CM_OK) sv->last_err = "request " + std::to_string(t) + ": " + err;
             done.t_done = now_ms();
             done.done = true;
             sv->cv_done.notify_all();
@@ -325,13 +328,21 @@ cm_status cm_server_destroy(cm_server *sv) {
 }
 
 cm_status cm_server_stats(cm_server *sv, int64_t *evictions, size_t *evicted_bytes, size_t *peak_held,
-                          size_t *budget) {
+                          size_t *budget, int64_t *reruns) {
     if (!sv) return CM_ERR_INVALID;
     std::lock_guard<std::mutex> lk(sv->mu);
+    if (reruns) *reruns = sv->reruns;
     if (evictions) *evictions = sv->evictions;
     if (evicted_bytes) *evicted_bytes = sv->evicted_bytes;
     if (peak_held) *peak_held = sv->peak_held;
     if (budget) *budget = sv->budget;
+    return CM_OK;
+}
+
+cm_status cm_server_last_error(cm_server *sv, char *buf, size_t len) {
+    if (!sv || !buf || len == 0) return CM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::snprintf(buf, len, "%s", sv->last_err.c_str());
     return CM_OK;
 }
 

@@ -220,7 +220,11 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         const int yy = q / HZ, zz = q - (q / HZ) * HZ;
         foff[k] = q < HP ? ((ty0 - 4 + yy + N) % N) * N + (tz0 - 4 + zz + N) % N : 0;
     }
-    // interior cells: phi-plane index, G-plane index, in-plane output offset
+    // interior cells: phi-plane index, G-plane index, in-plane output offset.
+    // The stencil reads of a cell address the LDS from one base per cell
+    // (pb = its phi index - 2 HZ - 2, gb = its G index - 2 GZ - 2) plus
+    // non-negative compile-time offsets, which the LDS instructions encode as
+    // immediates: no per-read address arithmetic.
     int po[IP], gq[IP], coff[IP];
 #pragma unroll
     for (int k = 0; k < IP; ++k) {
@@ -229,6 +233,7 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         gq[k] = (ia + 2) * GZ + ib + 2;
         coff[k] = (ty0 + ia) * N + tz0 + ib;
     }
+    constexpr int PB = 2 * HZ + 2, GB = 2 * GZ + 2;  // base offsets below a cell's own index
     // at most one halo position per thread: kind 1 = y-halo row (G_y, G_z),
     // kind 2 = z-halo column (G_z)
     int hkind = 0, hq = 0, hpo = 0;
@@ -275,18 +280,19 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
             if (tid + k * NT < HP) cur[tid + k * NT] = pf[k];
         if (s + 1 < x1 + 4) fetch(s + 1);  // in flight while plane s is processed
         __syncthreads();
-        const float *pm3 = ring + ((J + 1) % 5) * HP, *pm2 = ring + ((J + 2) % 5) * HP;
-        const float *pm1 = ring + ((J + 3) % 5) * HP;
+        constexpr int S3 = ((J + 1) % 5) * HP, S2 = ((J + 2) % 5) * HP, S1 = ((J + 3) % 5) * HP,
+                      S0 = ((J + 4) % 5) * HP;
+        const float *pm2 = ring + S2;
         // registers are written unconditionally (before x0 the values come from
         // a partly filled ring and are never read), so no history value has to
         // survive a skipped write
 #pragma unroll
         for (int k = 0; k < IP; ++k) {
-            const int o = po[k];
-            p3[k][J] = pm3[o];
-            hx[k][J] = fd4s(p3[k][(J + 4) % 5], p3[k][J], pm1[o], cur[o]);
-            hy[k][J] = fd4s(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ]);
-            hz[k][J] = fd4s(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2]);
+            const float *b = ring + (po[k] - PB);  // loop-invariant per cell
+            p3[k][J] = b[S3 + PB];
+            hx[k][J] = fd4s(p3[k][(J + 4) % 5], p3[k][J], b[S1 + PB], b[S0 + PB]);
+            hy[k][J] = fd4s(b[S2 + 2], b[S2 + HZ + 2], b[S2 + 3 * HZ + 2], b[S2 + 4 * HZ + 2]);
+            hz[k][J] = fd4s(b[S2 + 2 * HZ], b[S2 + 2 * HZ + 1], b[S2 + 2 * HZ + 3], b[S2 + 2 * HZ + 4]);
         }
         if (s >= x0) {
             float *gyw = sgy + gslot(s - 2), *gzw = sgz + gslot(s - 2);
@@ -296,24 +302,26 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
                 gzw[gq[k]] = hz[k][J];
             }
             if (hkind) {
-                const int o = hpo;
-                if (hkind == 1) gyw[hq] = fd4s(pm2[o - 2 * HZ], pm2[o - HZ], pm2[o + HZ], pm2[o + 2 * HZ]);
-                gzw[hq] = fd4s(pm2[o - 2], pm2[o - 1], pm2[o + 1], pm2[o + 2]);
+                const float *b = pm2 + (hpo - PB);
+                if (hkind == 1) gyw[hq] = fd4s(b[2], b[HZ + 2], b[3 * HZ + 2], b[4 * HZ + 2]);
+                gzw[hq] = fd4s(b[2 * HZ], b[2 * HZ + 1], b[2 * HZ + 3], b[2 * HZ + 4]);
             }
         }
         const int c = s - 4;
         if (c < x0) return;
-        const float *gy = sgy + gslot(c), *gz = sgz + gslot(c);
+        const int gs = gslot(c);  // uniform
         int8_t *lplane = labels + (size_t)c * NN * NN;
-        // T of cell k: txx, tyy, tzz, a10 = D_x G_y, a20 = D_x G_z, a21 = D_y G_z
+        // T of cell k: txx, tyy, tzz, a10 = D_x G_y, a20 = D_x G_z, a21 = D_y G_z;
+        // G_y at gy[gs + i], G_z at gy[gs + 3 GP + i]
+        constexpr int ZO = 3 * GP;
         auto tensor = [&](int k, float t[6]) {
-            const int q = gq[k];
+            const float *g = sgy + gs + (gq[k] - GB);
             t[0] = fd4s(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J]);
-            t[1] = fd4s(gy[q - 2 * GZ], gy[q - GZ], gy[q + GZ], gy[q + 2 * GZ]);
-            t[2] = fd4s(gz[q - 2], gz[q - 1], gz[q + 1], gz[q + 2]);
+            t[1] = fd4s(g[2], g[GZ + 2], g[3 * GZ + 2], g[4 * GZ + 2]);
+            t[2] = fd4s(g[ZO + 2 * GZ], g[ZO + 2 * GZ + 1], g[ZO + 2 * GZ + 3], g[ZO + 2 * GZ + 4]);
             t[3] = fd4s(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J]);
             t[4] = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
-            t[5] = fd4s(gz[q - 2 * GZ], gz[q - GZ], gz[q + GZ], gz[q + 2 * GZ]);
+            t[5] = fd4s(g[ZO + 2], g[ZO + GZ + 2], g[ZO + 3 * GZ + 2], g[ZO + 4 * GZ + 2]);
         };
         int v[IP];
         bool unc[IP];

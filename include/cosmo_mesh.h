@@ -470,14 +470,19 @@ cm_status cm_server_wait(cm_server *s, int64_t ticket, double timeout_s, double 
  * running ones, free the slots. */
 cm_status cm_server_destroy(cm_server *s);
 
-/* Submit n requests and wait for all: per-request status and latency (either
- * array may be NULL); returns the first failure or CM_OK. */
 /* Admission counters: idle-slot evictions and the bytes they freed, the most
  * the slots held together (reservations of running requests plus idle slots'
- * resident buffers, tables aside), and the budget. */
+ * resident buffers, tables aside), the budget, and the requests that outgrew
+ * their reservation and ran again alone.  Any pointer may be NULL. */
 cm_status cm_server_stats(cm_server *s, int64_t *evictions, size_t *evicted_bytes, size_t *peak_held,
-                          size_t *budget);
+                          size_t *budget, int64_t *reruns);
 
+/* Message of the most recent request that failed on this server ("" if none),
+ * for callers of cm_measure_batch, which returns status codes only. */
+cm_status cm_server_last_error(cm_server *s, char *buf, size_t len);
+
+/* Submit n requests and wait for all: per-request status and latency (either
+ * array may be NULL); returns the first failure or CM_OK. */
 cm_status cm_measure_batch(cm_server *s, const cm_request *reqs, int n, cm_status *statuses,
                            double *latency_ms);
 

@@ -144,7 +144,11 @@ def test_c4_batched_64_uneven_mocks(ctx):
     counts = {}
     with cm.Server(0, slots=3) as srv:
         res = srv.batch([_request(srv, s) for s in specs])
+        adm = srv.stats()
     assert len(res) == 64
+    # with the default budget no request outgrows its reservation (a re-run
+    # alone would serialise the server)
+    assert adm["reruns"] == 0 and adm["peak_held"] <= adm["budget"]
     for spec, r in zip(specs, res):
         a, pos, bm = _alone(ctx, spec)
         _same(r, a, spec)
@@ -187,6 +191,8 @@ def test_c5_poisson_stream_mixed(ctx):
         th.start()
         th.join()
         res = [t.result(timeout=120.0) for t in tickets]
+        adm = srv.stats()
+    assert adm["reruns"] == 0 and adm["peak_held"] <= adm["budget"]
     assert max(sp[1] ** 3 for sp in specs) >= 1e8
     for spec, r in zip(specs, res):
         a, pos, _ = _alone(ctx, spec)
