@@ -350,13 +350,47 @@ def batched(cm, nmocks=64, slots=3, seed=7):
         T = time.perf_counter() - t0
         adm = srv.stats()
     lat = np.array([r["latency_ms"] for r in res])
+    # the same 64 mocks with their particles generated beforehand: through the
+    # slot server, and through cm_measure_packed (one deposit launch per scheme
+    # over the union (mock, tile) work list, one T-web launch for all mocks)
+    with cm.Context(0) as pctx:
+        args = []
+        for (Nm, NLm, Lm, sch, ndel, sd) in specs:
+            bm = cm.binning(Nm, Lm, ndel=ndel)
+            args.append(dict(pos=pctx.zeldovich(NLm, Lm, A=1e4, ns=-1.5, seed=sd), N=Nm, L=Lm, scheme=sch,
+                             nshells=64, kmin=bm["kmin"], dk_bin=bm["dk"], do_tweb=True, R_smooth=2.0 * Lm / Nm,
+                             margin_tol=MARGIN_TOL))
+        torch.cuda.synchronize()
+        with cm.Server(0, slots=slots) as srv:
+            def pre():
+                return [srv.request(a["N"], a["L"], a["scheme"], 64, a["kmin"], a["dk_bin"], pos=a["pos"],
+                                    do_tweb=True, R_smooth=a["R_smooth"], margin_tol=MARGIN_TOL) for a in args]
+            srv.batch(pre()[:slots * 2])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            srv.batch(pre())
+            T_pre = time.perf_counter() - t0
+        pctx.measure_packed(args)  # warm-up: buffers and tables
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pctx.measure_packed(args)
+        torch.cuda.synchronize()
+        T_packed = time.perf_counter() - t0
+        del args
+    torch.cuda.empty_cache()
+    pregen = {"server": {"value": round(nmocks / T_pre, 2), "seconds": round(T_pre, 3), "slots": slots},
+              "packed": {"value": round(nmocks / T_packed, 2), "seconds": round(T_packed, 3),
+                         "entry": "cm_measure_packed"},
+              "note": "particles generated on the GPU before the timed region; the packed path runs the deposits "
+                      "of all mocks as one launch per MAS scheme over the union (mock, tile) work list and the "
+                      "T-web of all mocks as one launch"}
     return {"value": round(nmocks / T, 2), "unit": "mocks/s", "mocks": nmocks, "slots": slots,
             "particles": int(npart), "seconds": round(T, 3),
             "workload": "64 uneven mocks generated on the GPU (N 128-512, NL^3 particles 1.1e5-5.7e7, "
                         "NGP/CIC/TSC, 64 shells, T-web)",
             "autotune": tuned, "graphs": "hipGraph replay of repeated request shapes per slot",
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "admission": adm}
+            "admission": adm, "pregenerated": pregen}
 
 
 ROW_KERNELS = KERNELS + ["zel_spectra", "zel_displace", "uniform", "fkp_wscale", "fkp_wsums", "bk_masks", "bk_sums",

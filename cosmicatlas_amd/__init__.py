@@ -384,6 +384,44 @@ class Context:
                 "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal,
                 "tweb_tol": info.tweb.margin_tol}
 
+    def measure_packed(self, mocks):
+        """cm_measure_packed over a list of mocks, each a dict with the
+        arguments of measure() (pos, N, L, scheme, nshells, kmin, dk_bin and the
+        keywords).  Returns one result dict per mock, as measure() does; a
+        failed mock raises after the batch with its index."""
+        torch = _torch()
+        arr = (_abi.PackedMock * len(mocks))()
+        outs = []
+        for i, m in enumerate(mocks):
+            m = dict(m)
+            pos, N, nshells = m.pop("pos"), m.pop("N"), m.pop("nshells")
+            weights, labels = m.pop("weights", None), m.pop("labels", None)
+            L, scheme, kmin, dk_bin = m.pop("L"), m.pop("scheme"), m.pop("kmin"), m.pop("dk_bin")
+            o = _measure_opts(N, L, scheme, nshells, kmin, dk_bin, **m)
+            if o.do_tweb and labels is None:
+                labels = torch.empty((N, N, N), dtype=torch.int8, device=pos.device)
+            out = {"pk0": np.zeros(nshells), "pk2": np.zeros(nshells), "pk4": np.zeros(nshells),
+                   "nmodes": np.zeros(nshells, np.int64), "labels": labels, "_info": _abi.MeasureInfo(),
+                   "_keep": (pos, weights)}
+            arr[i] = _abi.PackedMock(o, _ptr(pos), _ptr(weights), pos.shape[0],
+                                     out["pk0"].ctypes.data_as(C.c_void_p), out["pk2"].ctypes.data_as(C.c_void_p),
+                                     out["pk4"].ctypes.data_as(C.c_void_p), out["nmodes"].ctypes.data_as(C.c_void_p),
+                                     _ptr(labels), C.pointer(out["_info"]))
+            outs.append(out)
+        sts = np.zeros(len(mocks), np.int32)
+        self.lib.cm_measure_packed(self.h, arr, len(mocks), sts.ctypes.data_as(C.c_void_p))
+        res = []
+        for i, (out, s_) in enumerate(zip(outs, sts)):
+            if s_ != 0:
+                raise CosmoError(int(s_), f"packed mock {i}: {self.lib.cm_last_error(self.h).decode()}")
+            info = out.pop("_info")
+            out.pop("_keep")
+            out.update({"n_obj": info.n_obj, "shot_noise": info.shot_noise, "normal_power": info.normal_power,
+                        "tweb_counts": list(info.tweb.counts), "tweb_marginal": info.tweb.n_marginal,
+                        "tweb_tol": info.tweb.margin_tol})
+            res.append(out)
+        return res
+
 
 def _tweb_stats(st):
     return {"counts": list(st.counts), "n_marginal": st.n_marginal, "margin_tol": st.margin_tol}

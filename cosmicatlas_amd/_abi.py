@@ -166,6 +166,21 @@ class Request(C.Structure):
     ]
 
 
+class PackedMock(C.Structure):
+    _fields_ = [
+        ("opts", MeasureOpts),
+        ("pos", C.c_void_p),
+        ("w", C.c_void_p),
+        ("n", C.c_size_t),
+        ("pk0", C.c_void_p),
+        ("pk2", C.c_void_p),
+        ("pk4", C.c_void_p),
+        ("nmodes", C.c_void_p),
+        ("labels", C.c_void_p),
+        ("info", C.POINTER(MeasureInfo)),
+    ]
+
+
 _LIB = None
 
 
@@ -223,6 +238,7 @@ def _declare(L):
                                  C.POINTER(C.c_size_t), C.POINTER(C.c_int64)]),
         "cm_measure_batch": (st, [P, C.POINTER(Request), C.c_int, P, P]),
         "cm_server_last_error": (st, [P, C.c_char_p, C.c_size_t]),
+        "cm_measure_packed": (st, [P, P, C.c_int, P]),
         "cm_uniform": (st, [P, C.c_size_t, C.c_double, C.c_uint64, P]),
     }
     for name, (res, args) in sig.items():

@@ -171,6 +171,32 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
 cm_status mesh_sum_impl(cm_ctx *ctx, const void *mesh, cm_dtype dt, size_t n, double *out_dev,
                         double *host_out);
 
+// Packed deposits (several meshes, one launch per MAS scheme over the union of
+// their tile work lists).  One mock of such a launch: its records, mesh and
+// tile geometry.
+struct DepMock {
+    const uint3 *sorted;
+    const float *wsorted;
+    float *mesh;
+    int N, ntx, tx0, pad;
+};
+// Tile binning of one mock (records grouped by 16^3 tile, exact sizing, one
+// host synchronisation) into buffers named <tag>.*, kept for a later packed
+// deposit; the mock's mesh must bin in one range (N <= 512).
+struct TileBins {
+    int N = 0, T = 0, ntx = 0, ntiles = 0, scheme = 0;
+    bool weighted = false;
+    void *sorted = nullptr, *wsorted = nullptr, *items = nullptr, *nitems = nullptr, *offsets = nullptr;
+    size_t cap = 0, max_items = 0;
+};
+bool mas_packable(cm_ctx *ctx, const cm_mas_desc *d);
+cm_status mas_bin_mock(cm_ctx *ctx, const cm_mas_desc *d, const float *pos, const float *w, size_t n,
+                       const char *tag, unsigned *overflow, TileBins *b);
+// Zeroes the cells of b's split tiles in mesh (before a deposit that does not
+// accumulate), then deposits every mock of `bins` (all of one scheme and weight
+// kind) into its mesh with one launch over the union of their work lists.
+cm_status mas_deposit_packed(cm_ctx *ctx, const std::vector<TileBins> &bins, const std::vector<float *> &meshes);
+
 // Device results of one fused measurement in one block, so that a single copy
 // reads them back: shell sums[4 ns], mode counts[ns], object count, T-web
 // counters[5] (4 classes, marginal cells), deposit overflow flag, ||phi||_rms
@@ -192,8 +218,20 @@ cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f);
 // the interlaced mesh.  Results land in the FusedOut block of o->nshells.
 // window_set: the shells visit the window mode set of the random branch
 // (stored modes outside the first octant; the zero mode is the caller's).
+// ext (nullable, packed measurements): results go to the block *out already
+// set up by the caller (out_given), the potential to ext->phi, and the T-web
+// launch is left to the caller (tweb_deferred: ext->tweb receives its job).
+struct TwMock;
+struct FusedExt {
+    bool out_given = false;
+    float *phi = nullptr;
+    TwMock *tweb = nullptr;
+};
 cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, FusedOut *out, bool window_set = false);
+                             int8_t *labels, FusedOut *out, bool window_set = false, const FusedExt *ext = nullptr);
+// FusedOut views of a result block at `base` for ns shells (5 ns + 8 words).
+size_t fused_out_words(int ns);
+void fused_out_at(void *base, int ns, FusedOut *f);
 // Copy a FusedOut block to the pinned host buffer and wait; returns its host copy.
 cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host);
 

@@ -321,20 +321,26 @@ static cm_status x_fused_launch(cm_ctx *ctx, int nb, int xe, float2 *dk, const f
                                  ns, ztab);
 }
 
-cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f) {
-    const size_t words = 5 * (size_t)ns + 8;
-    void *b;
-    cm_status s;
-    if ((s = scratch(ctx, "f.out", 8 * words, &b)) != CM_OK) return s;
+size_t fused_out_words(int ns) { return 5 * (size_t)ns + 8; }
+
+void fused_out_at(void *b, int ns, FusedOut *f) {
     auto *w = (unsigned long long *)b;
     f->base = b;
-    f->bytes = 8 * words;
+    f->bytes = 8 * fused_out_words(ns);
     f->sums = (double *)w;
     f->cnts = w + 4 * (size_t)ns;
     f->nobj = (double *)(w + 5 * (size_t)ns);
     f->stats = w + 5 * (size_t)ns + 1;
     f->overflow = (unsigned *)(w + 5 * (size_t)ns + 6);
     f->phirms = (unsigned *)(w + 5 * (size_t)ns + 7);
+}
+
+cm_status fused_out(cm_ctx *ctx, int ns, FusedOut *f) {
+    const size_t words = fused_out_words(ns);
+    void *b;
+    cm_status s;
+    if ((s = scratch(ctx, "f.out", 8 * words, &b)) != CM_OK) return s;
+    fused_out_at(b, ns, f);
     return CM_OK;
 }
 
@@ -360,7 +366,7 @@ cm_status fused_readback(cm_ctx *ctx, const FusedOut &f, int ns, FusedOut *host)
 // the FusedOut block.  Host tables (MAS windows, interlacing phases, Gaussian
 // factors) are cached on the device, so a repeated call issues kernels only.
 cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, void *meshB, double mean_est,
-                             int8_t *labels, FusedOut *out, bool window_set) {
+                             int8_t *labels, FusedOut *out, bool window_set, const FusedExt *ext) {
     const int N = o->mas.N;
     const size_t n3 = (size_t)N * N * N;
     const int pitch = padded_pitch(N);  // internal spectrum: kz rows padded to 128-byte multiples
@@ -371,7 +377,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     const int nparts = fft_rows_blocks(ctx, N, CM_F32);
     if ((s = scratch(ctx, "m.dk", sizeof(float2) * nh, &dk)) != CM_OK) return s;
     if ((s = scratch(ctx, "f.part", sizeof(double) * (size_t)nparts, &part)) != CM_OK) return s;
-    if ((s = fused_out(ctx, ns, out)) != CM_OK) return s;
+    if (!(ext && ext->out_given) && (s = fused_out(ctx, ns, out)) != CM_OK) return s;
     double *dsum = out->sums, *dnobj = out->nobj;
     unsigned long long *dcnt = out->cnts, *dstats = out->stats;
     const double *dtab = pk_tables_dev(ctx, N, (int)o->mas.scheme, o->mas_correction, &s);
@@ -473,8 +479,8 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     }
     if (s != CM_OK) return s;
     if (do_tweb) {
-        void *phi;
-        if ((s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
+        void *phi = ext ? ext->phi : nullptr;
+        if (!phi && (s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
         if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false, pitch)) != CM_OK) return s;
         // ||phi||_rms (for a tolerance derived from it): sums of squares ride in
         // the c2r pass (the z pass's partial sums are consumed by now)
@@ -484,6 +490,16 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
             return s;
         if (rms && (s = phi_rms_final(ctx, (const double *)part, nparts, n3, out->phirms)) != CM_OK) return s;
         const TolK tk = tweb_tol(N, o->mas.L, o->margin_tol, false, rms ? out->phirms : nullptr);
+        if (ext && ext->tweb) {  // the caller launches it with other mocks
+            TwMock &t = *ext->tweb;
+            t.phi = (const float *)phi;
+            t.labels = labels;
+            t.stats = (unsigned long long *)dstats;
+            t.tk = tk;
+            t.lth = o->lambda_th;
+            t.N = N;
+            return CM_OK;
+        }
         if ((s = tweb_fused_launch(ctx, N, o->mas.L, (const float *)phi, o->lambda_th, tk, labels, nullptr,
                                    (unsigned long long *)dstats)) != CM_OK)
             return s;

@@ -677,13 +677,17 @@ __device__ __forceinline__ void deposit_record(void *lgrid, int base, const floa
 
 // One workgroup per work item (a tile, or a kTileCH-record chunk of a crowded
 // tile), persistent over items.  The records of the next item are loaded into
-// registers while the current tile is written out.
+// registers while the current tile is written out.  Packed deposits of several
+// meshes (dm non-null): item i belongs to mock item_mock[i], whose records,
+// mesh and geometry come from dm[]; the arguments sorted .. mesh are unused.
 template <int SCHEME, bool UNIT, int T>
 __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ sorted,
                                                       const float *__restrict__ wsorted,
                                                       const uint4 *__restrict__ items,
                                                       const unsigned *__restrict__ nitems, int N, int ntx, int tx0,
-                                                      float *__restrict__ mesh, int accumulate, int half) {
+                                                      float *__restrict__ mesh, int accumulate, int half,
+                                                      const DepMock *__restrict__ dm,
+                                                      const unsigned *__restrict__ item_mock) {
     extern __shared__ long long lgrid_raw[];
     void *lgrid = lgrid_raw;
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
@@ -703,30 +707,37 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
     constexpr int XS = NT / (T * T);
     const int wz = tid % T, wy = (tid / T) % T, wx = tid / (T * T);
     const int li0 = ((wx + 2 * R) * S + (wy + 2 * R)) * S + (wz + 2 * R);
-    const size_t mstride = (size_t)XS * N * N;
     unsigned item = blockIdx.x;
     uint4 it = item < total ? items[item] : make_uint4(0u, 0u, 0u, 0u);
+    // the mock of an item (uniform per workgroup)
+    auto mock_of = [&](unsigned i) -> DepMock {
+        if (dm && i < total) return dm[item_mock[i]];
+        return DepMock{sorted, wsorted, mesh, N, ntx, tx0, 0};
+    };
+    DepMock mk = mock_of(item);
     uint3 rec[RPT];
     float wr[RPT];
-    auto load_item = [&](const uint4 &x) {
+    auto load_item = [&](const uint4 &x, const DepMock &m) {
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const unsigned idx = x.y + (unsigned)(tid + r * NT);
             if (idx < x.z) {
-                rec[r] = sorted[idx];
-                wr[r] = UNIT ? 1.0f : wsorted[idx];
+                rec[r] = m.sorted[idx];
+                wr[r] = UNIT ? 1.0f : m.wsorted[idx];
             }
         }
     };
-    load_item(it);
+    load_item(it, mk);
     for (; item < total; item += gridDim.x) {
         const unsigned nxt = item + gridDim.x;
         const uint4 itn = nxt < total ? items[nxt] : make_uint4(0u, 0u, 0u, 0u);
+        const DepMock mkn = mock_of(nxt);
+        const int mN = mk.N, mntx = mk.ntx;
         uint4 *lz = reinterpret_cast<uint4 *>(lgrid);
         for (int q = tid; q < S3 * WB / 16; q += NT) lz[q] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
         const int tile = (int)it.x;
-        const int tz = tile % ntx, ty = (tile / ntx) % ntx, tx = tx0 + tile / (ntx * ntx);
+        const int tz = tile % mntx, ty = (tile / mntx) % mntx, tx = mk.tx0 + tile / (mntx * mntx);
         const int o[3] = {tx * T, ty * T, tz * T};
         const int qk = UNIT ? unit_quantum_exp<SCHEME>(it.z - it.y) : 0;
         const float qs = ldexpf(1.0f, qk);
@@ -752,10 +763,12 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
             deposit_record<SCHEME, UNIT, S>(lgrid, (l[0] * S + l[1]) * S + l[2], f, wr[r], qs);
         }
         const bool split = it.w != 0u;
+        float *mp = mk.mesh + ((size_t)(o[0] + wx) * mN + (o[1] + wy)) * mN + (o[2] + wz);
+        const size_t mstride = (size_t)XS * mN * mN;
         it = itn;
-        load_item(it);  // in flight during the write-out
+        mk = mkn;
+        load_item(it, mk);  // in flight during the write-out
         __syncthreads();
-        float *mp = mesh + ((size_t)(o[0] + wx) * N + (o[1] + wy)) * N + (o[2] + wz);
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             if (TT % NT != 0 && tid + k * NT >= TT) break;
@@ -845,20 +858,24 @@ static int tile_edge(int N) {
     return 0;
 }
 
-// meshB (nullable): the half-cell-shifted mesh of the same particles, deposited
-// from the same records (binned with one more ghost cell towards the tile above).
+// Binning of one particle range into tiles: records grouped by tile (buffers
+// <tag>.sorted/.wsorted), per-tile offsets (<tag>.offsets) and the deposit work
+// list (<tag>.items, <tag>.nitems); the histogram and scan buffers are shared.
+// interlaced: records also feed the half-cell-shifted mesh (one more ghost
+// cell towards the tile above).
 template <int T>
-static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
-                                 float *meshB, int accumulate, unsigned long long *dropped, bool exact,
-                                 unsigned *overflow, int tx0, int nxl) {
+static cm_status mas_bin_range(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, bool interlaced,
+                               unsigned long long *dropped, bool exact, unsigned *overflow, int tx0, int nxl,
+                               const char *tag, TileBins *b, TileK *tk) {
     TileK t;
     t.T = T;
     t.R = mas_radius(p.scheme);
-    t.Rup = t.R + (meshB ? 1 : 0);
+    t.Rup = t.R + (interlaced ? 1 : 0);
     t.ntx = p.N / T;
     t.CH = kTileCH;
     t.tx0 = tx0;
     t.nxl = nxl;
+    *tk = t;
     const int ntiles = nxl * t.ntx * t.ntx;
     // a particle is copied to at most two tiles per axis: below when its
     // stencil reaches R cells under the tile edge, above when it reaches Rup
@@ -873,16 +890,17 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
     const int nrows = lds_hist ? std::max(1, ctx->cu_count) : 1;
     const size_t chunk = (p.n + nrows - 1) / nrows;
     void *hist, *counts, *offsets, *item_off, *bsum, *items, *nitems, *sorted;
+    const std::string tg(tag);
     cm_status s;
     if ((s = scratch(ctx, "mas.hist", sizeof(unsigned) * (size_t)nrows * ntiles, &hist)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.counts", sizeof(unsigned) * ntiles, &counts)) != CM_OK) return s;
-    if ((s = scratch(ctx, "mas.offsets", sizeof(unsigned) * (ntiles + 1), &offsets)) != CM_OK) return s;
+    if ((s = scratch(ctx, (tg + ".offsets").c_str(), sizeof(unsigned) * (ntiles + 1), &offsets)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.item_off", sizeof(unsigned) * (ntiles + 1), &item_off)) != CM_OK) return s;
     if ((s = scratch(ctx, "mas.bsum", sizeof(unsigned long long) * 2 * (size_t)(nblk + 1), &bsum)) != CM_OK) return s;
     const size_t max_items = (size_t)ntiles + max_copies / kTileCH + 1;
     if (max_items > 0xffffffffull) return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: too many particles");
-    if ((s = scratch(ctx, "mas.items", sizeof(uint4) * max_items, &items)) != CM_OK) return s;
-    if ((s = scratch(ctx, "mas.nitems", sizeof(unsigned), &nitems)) != CM_OK) return s;
+    if ((s = scratch(ctx, (tg + ".items").c_str(), sizeof(uint4) * max_items, &items)) != CM_OK) return s;
+    if ((s = scratch(ctx, (tg + ".nitems").c_str(), sizeof(unsigned), &nitems)) != CM_OK) return s;
     auto *hp = (unsigned *)hist;
     if (lds_hist) {
         if ((s = launch(ctx, "mas_tile_count", p.soa ? k_tile_count<T, true, true> : k_tile_count<T, true, false>,
@@ -920,9 +938,10 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
         const double g1 = 1.0 + (double)(t.R + t.Rup) / T;
         cap = std::min(std::min(max_copies, max_cap), (size_t)(1.5 * g1 * g1 * g1 * (double)p.n) + 65536);
     }
-    if ((s = scratch(ctx, "mas.sorted", sizeof(uint3) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
+    if ((s = scratch(ctx, (tg + ".sorted").c_str(), sizeof(uint3) * (cap ? cap : 1), &sorted)) != CM_OK) return s;
     void *wsorted = nullptr;
-    if (w && (s = scratch(ctx, "mas.wsorted", sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK) return s;
+    if (w && (s = scratch(ctx, (tg + ".wsorted").c_str(), sizeof(float) * (cap ? cap : 1), &wsorted)) != CM_OK)
+        return s;
     if ((s = launch(ctx, "mas_tile_items", k_tile_items, dim3(nblk), dim3(256), 0, ntiles,
                     (const unsigned long long *)bsum, (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts,
                     (uint4 *)items, (unsigned *)nitems, (unsigned)cap, (unsigned)max_items, overflow)) != CM_OK)
@@ -942,39 +961,86 @@ static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, c
                         (unsigned *)counts, (uint3 *)sorted, (float *)wsorted, (unsigned)cap, overflow)) != CM_OK)
             return s;
     }
-    if (!accumulate) {
-        for (float *m : {mesh, meshB}) {
-            if (m && (s = launch(ctx, "mas_tile_zero_split", k_tile_zero_split<T>, dim3(ntiles), dim3(256), 0,
-                                 (const unsigned *)offsets, t, p.N, m)) != CM_OK)
-                return s;
-        }
-    }
-    const int S = T + 4 * t.R;
-    const size_t lds = (w ? sizeof(long long) : sizeof(unsigned)) * (size_t)S * S * S;
-    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "mas: deposit grid of %zu bytes", lds);
+    b->N = p.N;
+    b->T = T;
+    b->ntx = t.ntx;
+    b->ntiles = ntiles;
+    b->scheme = p.scheme;
+    b->weighted = w != nullptr;
+    b->sorted = sorted;
+    b->wsorted = wsorted;
+    b->items = items;
+    b->nitems = nitems;
+    b->offsets = offsets;
+    b->cap = cap;
+    b->max_items = max_items;
+    return CM_OK;
+}
+
+// Zero the cells of the tiles of b (range t) that several workgroups deposit.
+template <int T>
+static cm_status mas_zero_split(cm_ctx *ctx, const TileBins &b, const TileK &t, float *mesh) {
+    return launch(ctx, "mas_tile_zero_split", k_tile_zero_split<T>, dim3(b.ntiles), dim3(256), 0,
+                  (const unsigned *)b.offsets, t, b.N, mesh);
+}
+
+// Deposit grid: persistent workgroups, dep_wg (default 8) per CU, at most one
+// per work item.
+static unsigned deposit_grid(cm_ctx *ctx, size_t max_items) {
     const int wg_per_cu = ctx->tune.dep_wg > 0 && ctx->tune.dep_wg <= 64 ? ctx->tune.dep_wg : 8;
-    const unsigned dg = (unsigned)std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * wg_per_cu);
-    const float *ws = (const float *)wsorted;
-    const dim3 gd(dg), bd(512);
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * wg_per_cu));
+}
+
+// Deposit launch of one scheme and weight kind: a single mock (dm null: the
+// buffers of b, geometry of t) or the union work list of packed mocks.
+template <int T>
+static cm_status deposit_launch(cm_ctx *ctx, int scheme, bool weighted, const TileBins &b, const TileK &t,
+                                const uint4 *items, const unsigned *nitems, size_t max_items, float *dst,
+                                int accumulate, int half, const DepMock *dm, const unsigned *item_mock) {
+    const int S = T + 4 * mas_radius(scheme);
+    const size_t lds = (weighted ? sizeof(long long) : sizeof(unsigned)) * (size_t)S * S * S;
+    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "mas: deposit grid of %zu bytes", lds);
+    const dim3 gd(deposit_grid(ctx, max_items)), bd(512);
+    const float *ws = (const float *)b.wsorted;
 #define CM_DEPOSIT(SCH, UNIT)                                                                                    \
-    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const uint3 *)sorted, ws,     \
-           (const uint4 *)items, (const unsigned *)nitems, p.N, t.ntx, t.tx0, dst, accumulate, half)
-    for (int half = 0; half < (meshB ? 2 : 1); ++half) {
-        float *dst = half ? meshB : mesh;
-        switch (p.scheme * 2 + (w ? 0 : 1)) {
-        case 0: s = CM_DEPOSIT(0, false); break;
-        case 1: s = CM_DEPOSIT(0, true); break;
-        case 2: s = CM_DEPOSIT(1, false); break;
-        case 3: s = CM_DEPOSIT(1, true); break;
-        case 4: s = CM_DEPOSIT(2, false); break;
-        case 5: s = CM_DEPOSIT(2, true); break;
-        case 6: s = CM_DEPOSIT(3, false); break;
-        case 7: s = CM_DEPOSIT(3, true); break;
-        default: return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", p.scheme);
-        }
-        if (s != CM_OK) return s;
+    launch(ctx, "mas_tile_deposit", k_tile_deposit<SCH, UNIT, T>, gd, bd, lds, (const uint3 *)b.sorted, ws,   \
+           items, nitems, b.N, b.ntx, t.tx0, dst, accumulate, half, dm, item_mock)
+    switch (scheme * 2 + (weighted ? 0 : 1)) {
+    case 0: return CM_DEPOSIT(0, false);
+    case 1: return CM_DEPOSIT(0, true);
+    case 2: return CM_DEPOSIT(1, false);
+    case 3: return CM_DEPOSIT(1, true);
+    case 4: return CM_DEPOSIT(2, false);
+    case 5: return CM_DEPOSIT(2, true);
+    case 6: return CM_DEPOSIT(3, false);
+    case 7: return CM_DEPOSIT(3, true);
     }
 #undef CM_DEPOSIT
+    return fail(ctx, CM_ERR_INVALID, "mas: scheme %d", scheme);
+}
+
+// meshB (nullable): the half-cell-shifted mesh of the same particles, deposited
+// from the same records (binned with one more ghost cell towards the tile above).
+template <int T>
+static cm_status mas_tiled_range(cm_ctx *ctx, const MasK &p, const float *pos, const float *w, float *mesh,
+                                 float *meshB, int accumulate, unsigned long long *dropped, bool exact,
+                                 unsigned *overflow, int tx0, int nxl) {
+    TileBins b;
+    TileK t;
+    cm_status s;
+    if ((s = mas_bin_range<T>(ctx, p, pos, w, meshB != nullptr, dropped, exact, overflow, tx0, nxl, "mas", &b,
+                              &t)) != CM_OK)
+        return s;
+    if (!accumulate) {
+        for (float *m : {mesh, meshB})
+            if (m && (s = mas_zero_split<T>(ctx, b, t, m)) != CM_OK) return s;
+    }
+    for (int half = 0; half < (meshB ? 2 : 1); ++half) {
+        if ((s = deposit_launch<T>(ctx, p.scheme, w != nullptr, b, t, (const uint4 *)b.items,
+                                   (const unsigned *)b.nitems, b.max_items, half ? meshB : mesh, accumulate, half,
+                                   nullptr, nullptr)) != CM_OK)
+            return s;
+    }
     return CM_OK;
 }
 
@@ -1066,6 +1132,99 @@ static cm_status mas_tiled(cm_ctx *ctx, const MasK &p, const float *pos, const f
     case 8: return mas_tiled_t<8>(ctx, p, pos, w, mesh, meshB, accumulate, dropped, exact, overflow);
     }
     return fail(ctx, CM_ERR_UNSUPPORTED, "mas tiled: N=%d", p.N);
+}
+
+// ------------------------------------------------------ packed deposits
+// Union of the work lists of several mocks: mock m's items go to
+// out[off_m + i] with off_m the sum of the earlier mocks' item counts (device
+// values); item_mock records the mock of every item.
+struct PackSrc {
+    const uint4 *items;
+    const unsigned *nitems;
+};
+
+__global__ void __launch_bounds__(256) k_pack_items(const PackSrc *__restrict__ src, int nm, uint4 *__restrict__ out,
+                                                    unsigned *__restrict__ item_mock, unsigned *__restrict__ total) {
+    const int m = blockIdx.y;
+    unsigned off = 0;
+    for (int k = 0; k < m; ++k) off += *src[k].nitems;
+    const unsigned cnt = *src[m].nitems;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        out[off + i] = src[m].items[i];
+        item_mock[off + i] = (unsigned)m;
+    }
+    if (m == nm - 1 && blockIdx.x == 0 && threadIdx.x == 0) *total = off + cnt;
+}
+
+bool mas_packable(cm_ctx *ctx, const cm_mas_desc *d) {
+    if (d->pos_dtype != CM_F32 || tile_edge(d->N) != 16) return false;
+    const size_t ntx = (size_t)d->N / 16;
+    return sizeof(unsigned) * ntx * ntx * ntx <= ctx->lds_per_block;  // one binning range
+}
+
+cm_status mas_bin_mock(cm_ctx *ctx, const cm_mas_desc *d, const float *pos, const float *w, size_t n,
+                       const char *tag, unsigned *overflow, TileBins *b) {
+    if (!mas_packable(ctx, d)) return fail(ctx, CM_ERR_UNSUPPORTED, "packed binning: N=%d", d->N);
+    MasK p = make_mask(d, n);
+    void *dropped;
+    cm_status s;
+    if ((s = scratch(ctx, "mas.dropped", sizeof(unsigned long long), &dropped)) != CM_OK) return s;
+    if ((s = fill(ctx, (unsigned long long *)dropped, 1, 0ull)) != CM_OK) return s;
+    TileK t;
+    if ((s = mas_bin_range<16>(ctx, p, pos, w, false, (unsigned long long *)dropped, true, overflow, 0, p.N / 16,
+                               tag, b, &t)) != CM_OK)
+        return s;
+    return CM_OK;
+}
+
+cm_status mas_deposit_packed(cm_ctx *ctx, const std::vector<TileBins> &bins, const std::vector<float *> &meshes) {
+    const int nm = (int)bins.size();
+    if (nm == 0) return CM_OK;
+    if (meshes.size() != bins.size()) return fail(ctx, CM_ERR_INVALID, "packed deposit: %d meshes", (int)meshes.size());
+    const int scheme = bins[0].scheme;
+    const bool weighted = bins[0].weighted;
+    std::vector<DepMock> dm((size_t)nm);
+    std::vector<PackSrc> src((size_t)nm);
+    size_t max_items = 0, per_mock = 0;
+    cm_status s;
+    for (int m = 0; m < nm; ++m) {
+        const TileBins &b = bins[(size_t)m];
+        if (b.scheme != scheme || b.weighted != weighted || b.T != 16)
+            return fail(ctx, CM_ERR_INVALID, "packed deposit: mixed schemes or weights");
+        TileK t;
+        t.T = 16;
+        t.R = mas_radius(scheme);
+        t.Rup = t.R;
+        t.ntx = b.ntx;
+        t.CH = kTileCH;
+        t.tx0 = 0;
+        t.nxl = b.ntx;
+        if ((s = mas_zero_split<16>(ctx, b, t, meshes[(size_t)m])) != CM_OK) return s;
+        dm[(size_t)m] = DepMock{(const uint3 *)b.sorted, (const float *)b.wsorted, meshes[(size_t)m], b.N, b.ntx, 0, 0};
+        src[(size_t)m] = PackSrc{(const uint4 *)b.items, (const unsigned *)b.nitems};
+        max_items += b.max_items;
+        per_mock = std::max(per_mock, b.max_items);
+    }
+    void *ddm, *dsrc, *uitems, *umock, *utotal;
+    if ((s = scratch(ctx, "mas.pk.dm", sizeof(DepMock) * (size_t)nm, &ddm)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.pk.src", sizeof(PackSrc) * (size_t)nm, &dsrc)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.pk.items", sizeof(uint4) * max_items, &uitems)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.pk.mock", sizeof(unsigned) * max_items, &umock)) != CM_OK) return s;
+    if ((s = scratch(ctx, "mas.pk.total", sizeof(unsigned), &utotal)) != CM_OK) return s;
+    // the descriptor tables are small; a synchronous copy keeps the host
+    // vectors out of the stream's lifetime
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CM_HIP(ctx, hipMemcpy(ddm, dm.data(), sizeof(DepMock) * (size_t)nm, hipMemcpyHostToDevice));
+    CM_HIP(ctx, hipMemcpy(dsrc, src.data(), sizeof(PackSrc) * (size_t)nm, hipMemcpyHostToDevice));
+    ++ctx->host_syncs;
+    const unsigned gx = (unsigned)std::min<size_t>(1024, (per_mock + 255) / 256);
+    if ((s = launch(ctx, "mas_pack_items", k_pack_items, dim3(gx, (unsigned)nm), dim3(256), 0, (const PackSrc *)dsrc,
+                    nm, (uint4 *)uitems, (unsigned *)umock, (unsigned *)utotal)) != CM_OK)
+        return s;
+    TileK t0;
+    t0.tx0 = 0;
+    return deposit_launch<16>(ctx, scheme, weighted, bins[0], t0, (const uint4 *)uitems, (const unsigned *)utotal,
+                              max_items, meshes[0], 0, 0, (const DepMock *)ddm, (const unsigned *)umock);
 }
 
 cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, const void *w, size_t n,

@@ -17,6 +17,11 @@
 namespace cm {
 
 cm_status poisson_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, float *phi, double subtract);
+cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::vector<double> &L);
+
+__global__ void k_zero_u32(unsigned *a, int n) {
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0u;
+}
 cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi, int8_t *labels, float *lam,
                              cm_tweb_stats *st, bool plain);
 
@@ -317,7 +322,185 @@ cm_status measure_run(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, co
     TuneScope ts(ctx, o->mas.N);
     return measure_impl(ctx, o, pos, w, n, pk0, pk2, pk4, nmodes, labels, info);
 }
+// ------------------------------------------------------------ packed mocks
+// Argument checks of one mock (cm_measure's).
+static cm_status check_mock(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, size_t n, const int8_t *labels) {
+    if (!o || (!pos && n)) return fail(ctx, CM_ERR_INVALID, "cm_measure: null pointer");
+    if (!fft_size_ok(o->mas.N)) return fail(ctx, CM_ERR_INVALID, "cm_measure: N=%d not supported", o->mas.N);
+    if (o->mas.pos_dtype != CM_F32) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_measure: float32 positions only");
+    cm_status sb = check_binning(ctx, o->nshells, o->kmin, o->dk, o->log_binning, o->dlog, o->float_shell_rule,
+                                 "cm_measure");
+    if (sb != CM_OK) return sb;
+    if (o->do_tweb && !labels) return fail(ctx, CM_ERR_INVALID, "cm_measure: do_tweb needs a labels buffer");
+    return CM_OK;
+}
+
+// The packed path takes fused, non-interlaced mocks whose mesh bins in one
+// range (16^3 tiles, N <= 512) and, with the T-web, N a multiple of 64.
+static bool packable(cm_ctx *ctx, const cm_measure_opts *o) {
+    return o->fused && !o->interlace && mas_packable(ctx, &o->mas) && (!o->do_tweb || o->mas.N % 64 == 0);
+}
+
+// HBM a mock holds from its binning to its results: mesh, potential, records
+// (ghost copies of 16^3 tiles with headroom), work list and result block.
+static size_t packed_bytes(const cm_packed_mock &m) {
+    const cm_measure_opts &o = m.opts;
+    const size_t n3 = (size_t)o.mas.N * o.mas.N * o.mas.N;
+    const double R = o.mas.scheme == CM_NGP ? 0.0 : (o.mas.scheme == CM_PCS ? 2.0 : 1.0);
+    const double g = 1.0 + 2.0 * R / 16.0;
+    const size_t recs = (size_t)(1.25 * g * g * g * (double)m.n) + 65536;
+    return 4 * n3 * (o.do_tweb ? 2 : 1) + (12 + (m.w ? 4 : 0)) * recs + 16 * (n3 / 4096 + recs / 4096 + 2) +
+           8 * fused_out_words(o.nshells) + 4 * (n3 / 4096 + 1);
+}
+
+// One group of packable mocks: bin each, deposit all (one launch per scheme
+// and weight kind), spectra and shells per mock, one T-web launch, one
+// read-back of every result block.
+static cm_status packed_group(cm_ctx *ctx, const cm_packed_mock *m, const std::vector<int> &idx, cm_status *st) {
+    const int g = (int)idx.size();
+    cm_status s;
+    std::vector<size_t> off((size_t)g + 1, 0);
+    for (int k = 0; k < g; ++k) off[(size_t)k + 1] = off[(size_t)k] + fused_out_words(m[idx[(size_t)k]].opts.nshells);
+    void *outs;
+    if ((s = scratch(ctx, "pk.out", 8 * off[(size_t)g], &outs)) != CM_OK) return s;
+    std::vector<FusedOut> fo((size_t)g);
+    std::vector<TileBins> bins((size_t)g);
+    std::vector<float *> mesh((size_t)g), phi((size_t)g, nullptr);
+    char tag[32];
+    for (int k = 0; k < g; ++k) {
+        const cm_packed_mock &q = m[idx[(size_t)k]];
+        const size_t n3 = (size_t)q.opts.mas.N * q.opts.mas.N * q.opts.mas.N;
+        fused_out_at((unsigned long long *)outs + off[(size_t)k], q.opts.nshells, &fo[(size_t)k]);
+        void *p;
+        std::snprintf(tag, sizeof tag, "pk.mesh.%d", k);
+        if ((s = scratch(ctx, tag, 4 * n3, &p)) != CM_OK) return s;
+        mesh[(size_t)k] = (float *)p;
+        if (q.opts.do_tweb) {
+            std::snprintf(tag, sizeof tag, "pk.phi.%d", k);
+            if ((s = scratch(ctx, tag, 4 * n3, &p)) != CM_OK) return s;
+            phi[(size_t)k] = (float *)p;
+        }
+    }
+    // binning of every mock (exact record sizing)
+    for (int k = 0; k < g; ++k) {
+        const cm_packed_mock &q = m[idx[(size_t)k]];
+        std::snprintf(tag, sizeof tag, "pk.%d", k);
+        if ((s = launch(ctx, "fill", k_zero_u32, dim3(1), dim3(64), 0, fo[(size_t)k].overflow, 1)) != CM_OK)
+            return s;
+        if ((s = mas_bin_mock(ctx, &q.opts.mas, (const float *)q.pos, (const float *)q.w, q.n, tag,
+                              fo[(size_t)k].overflow, &bins[(size_t)k])) != CM_OK)
+            return s;
+    }
+    // one deposit launch per scheme and weight kind over the union work list
+    for (int sch = 0; sch < 4; ++sch)
+        for (int wt = 0; wt < 2; ++wt) {
+            std::vector<TileBins> bs;
+            std::vector<float *> ms;
+            for (int k = 0; k < g; ++k)
+                if (bins[(size_t)k].scheme == sch && (int)bins[(size_t)k].weighted == wt) {
+                    bs.push_back(bins[(size_t)k]);
+                    ms.push_back(mesh[(size_t)k]);
+                }
+            if (!bs.empty() && (s = mas_deposit_packed(ctx, bs, ms)) != CM_OK) return s;
+        }
+    // spectra, shells and potentials per mock; the T-web of all in one launch
+    std::vector<TwMock> tw;
+    std::vector<double> twL;
+    std::vector<TwMock> twk((size_t)g);
+    for (int k = 0; k < g; ++k) {
+        const cm_packed_mock &q = m[idx[(size_t)k]];
+        const int N = q.opts.mas.N;
+        const double n3 = (double)N * N * N;
+        TuneScope ts(ctx, N);
+        FusedExt ext;
+        ext.out_given = true;
+        ext.phi = phi[(size_t)k];
+        ext.tweb = q.opts.do_tweb ? &twk[(size_t)k] : nullptr;
+        if ((s = fused_measure_mesh(ctx, &q.opts, mesh[(size_t)k], nullptr, (double)q.n / n3, q.labels,
+                                    &fo[(size_t)k], false, &ext)) != CM_OK)
+            return s;
+        if (q.opts.do_tweb) {
+            tw.push_back(twk[(size_t)k]);
+            twL.push_back(q.opts.mas.L);
+        }
+    }
+    if (!tw.empty() && (s = tweb_fused_packed(ctx, tw, twL)) != CM_OK) return s;
+    // one read-back of all result blocks
+    void *h;
+    if ((s = pinned_host(ctx, 8 * off[(size_t)g], &h)) != CM_OK) return s;
+    CM_HIP(ctx, hipMemcpyAsync(h, outs, 8 * off[(size_t)g], hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < g; ++k) {
+        const int i = idx[(size_t)k];
+        const cm_packed_mock &q = m[i];
+        const cm_measure_opts *o = &q.opts;
+        FusedOut hb;
+        fused_out_at((unsigned long long *)h + off[(size_t)k], o->nshells, &hb);
+        if (*hb.overflow) {  // not expected with exact sizing: measure it alone
+            st[i] = fail(ctx, CM_ERR_DEVICE, "cm_measure_packed: record buffer overflow in mock %d", i);
+            continue;
+        }
+        if (q.info) {
+            for (int c = 0; c < 4; ++c) q.info->tweb.counts[c] = o->do_tweb ? (int64_t)hb.stats[c] : 0;
+            q.info->tweb.n_marginal = o->do_tweb ? (int64_t)hb.stats[4] : 0;
+            q.info->tweb.margin_tol =
+                o->do_tweb ? tweb_tol_applied(o->mas.N, o->mas.L, o->margin_tol, *hb.phirms) : 0.0;
+        }
+        const std::vector<double> hs(hb.sums, hb.sums + 4 * (size_t)o->nshells);
+        const std::vector<unsigned long long> hc(hb.cnts, hb.cnts + (size_t)o->nshells);
+        finalize(o, *hb.nobj, hs, hc, q.pk0, q.pk2, q.pk4, q.nmodes, q.info);
+        st[i] = CM_OK;
+    }
+    return CM_OK;
+}
+
+cm_status measure_packed(cm_ctx *ctx, const cm_packed_mock *m, int nm, cm_status *statuses) {
+    if (!ctx) return CM_ERR_INVALID;
+    if (nm < 0 || (nm > 0 && !m)) return fail(ctx, CM_ERR_INVALID, "cm_measure_packed: bad mock list");
+    std::vector<cm_status> st((size_t)std::max(nm, 0), CM_OK);
+    std::vector<int> pk;
+    for (int i = 0; i < nm; ++i) {
+        const cm_packed_mock &q = m[i];
+        if ((st[(size_t)i] = check_mock(ctx, &q.opts, q.pos, q.n, q.labels)) != CM_OK) continue;
+        if (packable(ctx, &q.opts)) {
+            pk.push_back(i);
+        } else {
+            st[(size_t)i] = measure_run(ctx, &q.opts, q.pos, q.w, q.n, q.pk0, q.pk2, q.pk4, q.nmodes, q.labels,
+                                        q.info);
+        }
+    }
+    // groups that hold at most half of the context's free budget at once
+    const size_t avail = ctx->budget > ctx->resident ? ctx->budget - ctx->resident : 0;
+    size_t k0 = 0;
+    while (k0 < pk.size()) {
+        size_t k1 = k0, held = 0;
+        while (k1 < pk.size() && (k1 == k0 || held + packed_bytes(m[pk[k1]]) <= avail / 2))
+            held += packed_bytes(m[pk[k1++]]);
+        const std::vector<int> grp(pk.begin() + (long)k0, pk.begin() + (long)k1);
+        const cm_status s = packed_group(ctx, m, grp, st.data());
+        if (s != CM_OK) {
+            for (int i : grp) st[(size_t)i] = s;
+            const std::string e = ctx->err;
+            for (int i : grp)  // the group failed as a whole: each mock alone
+                st[(size_t)i] = measure_run(ctx, &m[i].opts, m[i].pos, m[i].w, m[i].n, m[i].pk0, m[i].pk2,
+                                            m[i].pk4, m[i].nmodes, m[i].labels, m[i].info);
+            (void)e;
+        }
+        k0 = k1;
+    }
+    cm_status first = CM_OK;
+    for (int i = 0; i < nm; ++i) {
+        if (statuses) statuses[i] = st[(size_t)i];
+        if (first == CM_OK && st[(size_t)i] != CM_OK) first = st[(size_t)i];
+    }
+    return first;
+}
+
 }  // namespace cm
+
+extern "C" cm_status cm_measure_packed(cm_ctx *ctx, const cm_packed_mock *mocks, int nm, cm_status *statuses) {
+    return cm::measure_packed(ctx, mocks, nm, statuses);
+}
 
 extern "C" cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, const void *w, size_t n,
                                 double *pk0, double *pk2, double *pk4, int64_t *nmodes, int8_t *labels,

@@ -187,9 +187,10 @@ constexpr size_t tweb_lds_floats() {
 }
 
 template <int TY, int TZ, int NT, bool INL>
-__global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi, int N, int segLen, float ih2,
-                                                   double lth, TolK tk, int8_t *__restrict__ labels,
-                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats) {
+__global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi_a, int N_a, int segLen_a,
+                                                   float ih2_a, double lth_a, TolK tk_a, int8_t *__restrict__ labels_a,
+                                                   float *__restrict__ lam, unsigned long long *__restrict__ stats_a,
+                                                   const TwMock *__restrict__ tm, int nm) {
     constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
     constexpr int GZ = TZ + 4, GP = (TY + 4) * GZ;         // G plane (halo 2)
     constexpr int IP = TY * TZ / NT;                       // interior cells per thread
@@ -203,9 +204,35 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     __shared__ unsigned scnt[5];
     const int tid = threadIdx.x;
     if (tid < 5) scnt[tid] = 0;
+    // the workgroup's potential and work item: (tile column, x segment) of the
+    // one potential (tm null), or of the mock that owns it in a packed launch
+    const float *phi = phi_a;
+    int8_t *labels = labels_a;
+    unsigned long long *stats = stats_a;
+    int N = N_a, segLen = segLen_a;
+    float ih2 = ih2_a;
+    double lth = lth_a;
+    TolK tk = tk_a;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (tm) {
+        int m = 0;
+        while (m + 1 < nm && (int)blockIdx.x >= tm[m + 1].first) ++m;
+        const TwMock &t = tm[m];
+        phi = t.phi;
+        labels = t.labels;
+        stats = t.stats;
+        N = t.N;
+        segLen = t.segLen;
+        ih2 = t.ih2;
+        lth = t.lth;
+        tk = t.tk;
+        const int local = (int)blockIdx.x - t.first, tiles = (N / TY) * (N / TZ);
+        bx = local % tiles;
+        by = local / tiles;
+    }
     const int ntz = N / TZ;
-    const int ty0 = (blockIdx.x / ntz) * TY, tz0 = (blockIdx.x % ntz) * TZ;
-    const int x0 = blockIdx.y * segLen, x1 = min(N, x0 + segLen);
+    const int ty0 = (bx / ntz) * TY, tz0 = (bx % ntz) * TZ;
+    const int x0 = by * segLen, x1 = min(N, x0 + segLen);
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
     // T' = T/ih^2: threshold and margin scaled to match
     const double tol = tol_value(tk);
@@ -399,20 +426,11 @@ static unsigned grid_for(size_t n, int block, size_t cap) {
 }
 
 
-template <int TY, int TZ, int NT>
-static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol,
-                                int8_t *labels, float *lam, unsigned long long *stats) {
-    const bool inl = lam != nullptr;
-    auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
-    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
-    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
-    // x segments: a segment re-reads 8 halo planes; pick the count that
-    // minimises (rounds of resident workgroups) x (planes per segment)
-    const int tiles = (N / TY) * (N / TZ);
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    const int slots = std::max(1, ctx->cu_count) * per_cu;
+// x segments of the fused T-web: a segment re-reads 8 halo planes; the count
+// minimises (rounds of resident workgroups) x (planes per segment).  Returns
+// the segment length.
+static int tweb_seg_len(cm_ctx *ctx, int N, int tiles, int per_cu) {
+    const int slots = std::max(1, ctx->cu_count) * std::max(1, per_cu);
     int nseg = 1;
     double best = 1e300;
     for (int g = 1; g <= std::max(1, N / 16); ++g) {
@@ -425,14 +443,70 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
         }
     }
     if (ctx->tune.tseg > 0 && ctx->tune.tseg <= N / 4) nseg = ctx->tune.tseg;
-    const int segLen = (N + nseg - 1) / nseg;
-    nseg = (N + segLen - 1) / segLen;
+    return (N + nseg - 1) / nseg;
+}
+
+template <int TY, int TZ, int NT>
+static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol,
+                                int8_t *labels, float *lam, unsigned long long *stats) {
+    const bool inl = lam != nullptr;
+    auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
+    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
+    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
+    const int tiles = (N / TY) * (N / TZ);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int segLen = tweb_seg_len(ctx, N, tiles, per_cu);
+    const int nseg = (N + segLen - 1) / segLen;
     dim3 grid((unsigned)tiles, (unsigned)nseg);
     const double ih = 1.0 / (12.0 * (L / (double)N));
     const float ih2 = (float)(ih * ih);
     cm_status s;
     if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
-    return launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats);
+    return launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats,
+                  (const TwMock *)nullptr, 0);
+}
+
+// Packed T-web: one launch over the (tile column, x segment) items of every
+// mock (16 x 64 tiles, so N a multiple of 64).  mocks[i].first and .segLen
+// are set here; each mock's counters are cleared first.
+cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::vector<double> &L) {
+    constexpr int TY = 16, TZ = 64, NT = 512;
+    const int nm = (int)mocks.size();
+    if (nm == 0) return CM_OK;
+    auto kern = k_tweb_fused<TY, TZ, NT, false>;
+    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
+    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    long long blocks = 0;
+    cm_status s;
+    for (int m = 0; m < nm; ++m) {
+        TwMock &t = mocks[(size_t)m];
+        if (t.N % TZ != 0 || t.N % TY != 0) return fail(ctx, CM_ERR_UNSUPPORTED, "packed tweb: N=%d", t.N);
+        const int tiles = (t.N / TY) * (t.N / TZ);
+        {
+            TuneScope ts(ctx, t.N);
+            t.segLen = tweb_seg_len(ctx, t.N, tiles, per_cu);
+        }
+        const int nseg = (t.N + t.segLen - 1) / t.segLen;
+        const double ih = 1.0 / (12.0 * (L[(size_t)m] / (double)t.N));
+        t.ih2 = (float)(ih * ih);
+        t.first = (int)blocks;
+        blocks += (long long)tiles * nseg;
+        if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, t.stats, 5)) != CM_OK) return s;
+    }
+    if (blocks > 0x7fffffffLL) return fail(ctx, CM_ERR_UNSUPPORTED, "packed tweb: %lld workgroups", blocks);
+    void *dtm;
+    if ((s = scratch(ctx, "tweb.pk.mocks", sizeof(TwMock) * (size_t)nm, &dtm)) != CM_OK) return s;
+    CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CM_HIP(ctx, hipMemcpy(dtm, mocks.data(), sizeof(TwMock) * (size_t)nm, hipMemcpyHostToDevice));
+    ++ctx->host_syncs;
+    return launch(ctx, "tweb_fused", kern, dim3((unsigned)blocks), dim3(NT), lds, (const float *)nullptr, 0, 0, 0.0f,
+                  0.0, TolK{0.0, 0.0, nullptr}, (int8_t *)nullptr, (float *)nullptr, (unsigned long long *)nullptr,
+                  (const TwMock *)dtm, nm);
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.

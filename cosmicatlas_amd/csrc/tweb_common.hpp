@@ -77,6 +77,19 @@ __device__ __forceinline__ double tol_value(const TolK &t) {
     return t.norm ? t.per_norm * (double)__uint_as_float(*t.norm) : t.tol;
 }
 
+// One mock of a packed T-web launch (several potentials, one launch over the
+// union of their (tile column, x segment) work items): its potential, labels,
+// counters, tolerance and geometry; first = its first workgroup.
+struct TwMock {
+    const float *phi;
+    int8_t *labels;
+    unsigned long long *stats;
+    TolK tk;
+    double lth;
+    float ih2;
+    int N, segLen, first;
+};
+
 // The same stencil in float (fused kernel): 3 FMAs and a multiply.
 __device__ __forceinline__ float fd4f(float fll, float fl, float fr, float frr, float ih) {
     return ((fll - frr) + 8.0f * (fr - fl)) * ih;

@@ -296,6 +296,31 @@ cm_status cm_measure(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, con
                      size_t n, double *pk0, double *pk2, double *pk4, int64_t *nmodes,
                      int8_t *labels, cm_measure_info *info);
 
+/* One mock of a packed measurement: the arguments of cm_measure. */
+typedef struct {
+    cm_measure_opts opts;
+    const void *pos;     /* device particles (float32) */
+    const void *w;       /* device weights or NULL */
+    size_t n;
+    double *pk0, *pk2, *pk4; /* host, opts.nshells each (pk2/pk4 may be NULL) */
+    int64_t *nmodes;     /* host, opts.nshells (may be NULL) */
+    int8_t *labels;      /* device int8 N^3 when opts.do_tweb */
+    cm_measure_info *info; /* host, may be NULL */
+} cm_packed_mock;
+
+/* Packed measurement of a batch of uneven mocks (no reference counterpart;
+ * the batched configuration).  Each mock gets exactly what cm_measure gives it,
+ * but the tile deposits of all mocks run as one persistent launch per MAS
+ * scheme over the union of their (mock, tile) work lists, and the T-web of all
+ * mocks as one launch over their (mock, tile column, x segment) items; the
+ * binning (exact record sizing, one host synchronisation per mock), the FFT
+ * passes and the fused shell pass run per mock in between.  Mocks are taken
+ * in groups that hold at most half of the context's free budget.  Mocks the
+ * packed path does not take (plain or interlaced, N above 512, N not a
+ * multiple of 64 with the T-web) run through cm_measure.  statuses (nullable):
+ * per mock; returns the first failure. */
+cm_status cm_measure_packed(cm_ctx *ctx, const cm_packed_mock *mocks, int nm, cm_status *statuses);
+
 /* ------------------------------------------------------------- autotuner */
 /* Launch shapes of the fused path; 0 = built-in choice. */
 typedef struct {

@@ -166,6 +166,27 @@ def test_c4_batched_64_uneven_mocks(ctx):
         assert np.max(np.abs(raw32[ok] - p64[ok]) / np.abs(p64[ok])) < 1e-4
 
 
+def test_c4_packed_64_uneven_mocks(ctx):
+    """The same 64 mocks through cm_measure_packed (one deposit launch per
+    scheme over the union (mock, tile) work list, one T-web launch for all):
+    each equals its alone run."""
+    import cosmicatlas_amd as cm
+    specs = _c4_specs()
+    args = []
+    for (Nm, NLm, Lm, sch, ndel, tw, sd) in specs:
+        bm = cm.binning(Nm, Lm, ndel=ndel)
+        args.append(dict(pos=ctx.zeldovich(NLm, Lm, A=1e4, ns=-1.5, seed=sd), N=Nm, L=Lm, scheme=sch, nshells=64,
+                         kmin=bm["kmin"], dk_bin=bm["dk"], do_tweb=True, R_smooth=2.0 * Lm / Nm,
+                         margin_tol=cm.TOL_ROUNDING))
+    res = ctx.measure_packed(args)
+    del args
+    torch.cuda.empty_cache()
+    for spec, r in zip(specs, res):
+        a, pos, _ = _alone(ctx, spec)
+        del pos
+        _same(r, a, spec)
+
+
 def test_c5_poisson_stream_mixed(ctx):
     import cosmicatlas_amd as cm
     rng = np.random.default_rng(11)
