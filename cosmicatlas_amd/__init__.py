@@ -227,9 +227,11 @@ class Context:
         return p0, p2, p4, nm
 
     # ------------------------------------------------------------------ T-web
-    def _tweb_opts(self, N, L, R, lambda_th, margin_tol, delta_scale):
+    def _tweb_opts(self, N, L, R, lambda_th, margin_tol, delta_scale, hessian="fd"):
+        if hessian not in ("fd", "spectral"):
+            raise ValueError(f"hessian must be 'fd' or 'spectral', not {hessian!r}")
         return _abi.TwebOpts(N, float(L), float(R), float(lambda_th), float(margin_tol),
-                             float(delta_scale))
+                             float(delta_scale), 1 if hessian == "spectral" else 0)
 
     def poisson(self, delta, L, R=0.0, delta_scale=1.0):
         torch = _torch()
@@ -239,26 +241,30 @@ class Context:
         self._check(self.lib.cm_poisson(self.h, C.byref(o), _ptr(delta), _ptr(phi)))
         return phi
 
-    def tweb(self, delta, L, R, lambda_th, margin_tol=0.0, delta_scale=1.0, want_lambdas=False):
+    def tweb(self, delta, L, R, lambda_th, margin_tol=0.0, delta_scale=1.0, want_lambdas=False, hessian="fd"):
+        """T-web of a float32 delta: (labels, lambdas or None, stats).  hessian:
+        "fd" (the 4th-order FD stencil applied twice to phi) or "spectral"
+        (T_ab = c2r(k_a k_b / k^2 W(k) delta_k))."""
         torch = _torch()
         N = delta.shape[0]
         labels = torch.empty((N, N, N), dtype=torch.int8, device=delta.device)
         lam = torch.empty((3, N, N, N), dtype=torch.float32, device=delta.device) if want_lambdas else None
         st = _abi.TwebStats()
-        o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale)
+        o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale, hessian)
         self._check(self.lib.cm_tweb(self.h, C.byref(o), _ptr(delta), _ptr(labels), _ptr(lam), C.byref(st)))
         return labels, lam, _tweb_stats(st)
 
     def tweb_f64(self, delta, L, R, lambda_th, margin_tol=0.0, delta_scale=1.0, want_lambdas=False,
-                 want_phi=False):
-        """FP64 T-web of a float64 delta: (labels, lambdas or None, phi or None, stats)."""
+                 want_phi=False, hessian="fd"):
+        """FP64 T-web of a float64 delta: (labels, lambdas or None, phi or None, stats).
+        want_phi needs hessian="fd" (the spectral Hessian forms no potential)."""
         torch = _torch()
         N = delta.shape[0]
         labels = torch.empty((N, N, N), dtype=torch.int8, device=delta.device)
         lam = torch.empty((3, N, N, N), dtype=torch.float64, device=delta.device) if want_lambdas else None
         phi = torch.empty((N, N, N), dtype=torch.float64, device=delta.device) if want_phi else None
         st = _abi.TwebStats()
-        o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale)
+        o = self._tweb_opts(N, L, R, lambda_th, margin_tol, delta_scale, hessian)
         self._check(self.lib.cm_tweb_f64(self.h, C.byref(o), _ptr(delta), _ptr(labels), _ptr(lam), _ptr(phi),
                                          C.byref(st)))
         return labels, lam, phi, _tweb_stats(st)

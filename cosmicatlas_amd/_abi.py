@@ -60,6 +60,7 @@ class TwebOpts(C.Structure):
         ("lambda_th", C.c_double),
         ("margin_tol", C.c_double),
         ("delta_scale", C.c_double),
+        ("hessian", C.c_int),
     ]
 
 

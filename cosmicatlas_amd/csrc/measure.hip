@@ -292,6 +292,7 @@ static cm_status measure_impl(cm_ctx *ctx, const cm_measure_opts *o, const void 
         to.lambda_th = o->lambda_th;
         to.margin_tol = o->margin_tol;
         to.delta_scale = 1.0;
+        to.hessian = CM_HESSIAN_FD;
         cm_tweb_stats ts;
         if ((s = tweb_from_phi_impl(ctx, &to, (const float *)phi, labels, nullptr, info ? &ts : nullptr,
                                     !o->fused)) != CM_OK)

@@ -31,6 +31,82 @@ __global__ void __launch_bounds__(256) k_kmul(typename std::conditional<sizeof(T
     }
 }
 
+// Spectral Hessian component (calc_LapPhiv, Source/NumericalMethods.cpp:
+// 2605-2679): out = scale W(k) k_a k_b / k^2 delta_k, zero at k = 0, with the
+// wavenumbers of kcoord (index N/2 at +N/2 k_f, as the reference's coords).
+template <typename T>
+__global__ void __launch_bounds__(256) k_kab(const typename std::conditional<sizeof(T) == 4, float2, double2>::type
+                                                 *__restrict__ dk,
+                                             typename std::conditional<sizeof(T) == 4, float2, double2>::type
+                                                 *__restrict__ out,
+                                             int N, double kf, double R2h, double scale, int a, int b) {
+    const int M1 = N / 2 + 1;
+    const size_t total = (size_t)N * N * M1;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(g % M1);
+        const size_t r = g / M1;
+        const int iy = (int)(r % N), ix = (int)(r / N);
+        const double k[3] = {kcoord(ix, N, kf), kcoord(iy, N, kf), kf * (double)iz};
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const double f = k2 > 0.0 ? scale * exp(-k2 * R2h) * (k[a] * k[b]) / k2 : 0.0;
+        auto v = dk[g];
+        v.x = (T)((double)v.x * f);
+        v.y = (T)((double)v.y * f);
+        out[g] = v;
+    }
+}
+
+// Labels, lambdas and counters from the six tensor fields T (planes xx, yy,
+// zz, yx, zx, zy of n3 values each): FP64 closed-form eigenvalues per cell,
+// as the plain FD path.
+template <typename T>
+__global__ void __launch_bounds__(256) k_tensor_eig(const T *__restrict__ t6, size_t n3, double lth, TolK tk,
+                                                    int8_t *__restrict__ labels, T *__restrict__ lam,
+                                                    unsigned long long *__restrict__ stats) {
+    __shared__ unsigned cnt[5];
+    if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const double tol = tol_value(tk);
+    unsigned loc[5] = {0, 0, 0, 0, 0};
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += (size_t)gridDim.x * blockDim.x) {
+        double e1, e2, e3;
+        eig3(t6[c], t6[n3 + c], t6[2 * n3 + c], t6[3 * n3 + c], t6[4 * n3 + c], t6[5 * n3 + c], e1, e2, e3);
+        const int v = (e1 > lth) + (e2 > lth) + (e3 > lth);
+        const int lab = 4 - v;
+        labels[c] = (int8_t)lab;
+        if (lam) {
+            lam[c] = (T)e1;
+            lam[n3 + c] = (T)e2;
+            lam[2 * n3 + c] = (T)e3;
+        }
+        loc[lab - 1]++;
+        const double m = fmin(fabs(e1 - lth), fmin(fabs(e2 - lth), fabs(e3 - lth)));
+        if (m < tol) loc[4]++;
+    }
+    for (int i = 0; i < 5; ++i)
+        if (loc[i]) atomicAdd(&cnt[i], loc[i]);
+    __syncthreads();
+    if (threadIdx.x < 5 && cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+// ||T||_F,rms = sqrt(sum over cells of sum_ab T_ab^2 / n3) from the per-
+// workgroup sums of squares of the six planes (part[c * np + i]; the three
+// off-diagonal planes count twice), as float bits.
+__global__ void __launch_bounds__(1024) k_rms_tensor(const double *__restrict__ part, int np, double n,
+                                                     unsigned *__restrict__ out) {
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (int c = 0; c < 6; ++c)
+        for (int i = threadIdx.x; i < np; i += blockDim.x) s += (c < 3 ? 1.0 : 2.0) * part[(size_t)c * np + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = __float_as_uint((float)sqrt(red[0] / n));
+}
+
 // G_a = D_a phi, stored in the potential's type (float32: the stencil in
 // double, rounded to float, as the reference's float build; FP64: all double).
 template <typename T>
@@ -656,8 +732,84 @@ cm_status tweb_from_phi_impl(cm_ctx *ctx, const cm_tweb_opts *o, const float *ph
     return tweb_from_phi_t<float>(ctx, o, phi, labels, lam, st, plain);
 }
 
+// Spectral T-web (the reference's _USE_GFFT_EIGENV_ branch of EigenValuesTweb,
+// Source/Miscelanious.cpp:1703-1718): T_ab = c2r(scale W(k) k_a k_b / k^2
+// r2c(delta)) for the six independent components, then labels from the FP64
+// eigenvalues of each cell's tensor.  A negative margin_tol asks for
+// |margin_tol| K_s u ||T||_F,rms with K_s = sqrt(log2 N^3) sqrt(2 ln(3 N^3)):
+// each component carries the white rounding of its FFT pair, rms
+// sqrt(log2 N^3) u times its own rms, an eigenvalue moves by at most the
+// Frobenius norm of that error, and the last factor bounds the largest of the
+// 3 N^3 eigenvalue errors of a Gaussian model.
+template <typename T>
+static cm_status tweb_spectral_t(cm_ctx *ctx, const cm_tweb_opts *o, const T *delta, int8_t *labels, T *lam,
+                                 cm_tweb_stats *st) {
+    using C2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    constexpr cm_dtype DT = sizeof(T) == 4 ? CM_F32 : CM_F64;
+    constexpr bool F64 = sizeof(T) == 8;
+    const int N = o->N;
+    const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * (N / 2 + 1);
+    void *dk, *dab, *t6, *stats;
+    cm_status s;
+    if ((s = scratch(ctx, F64 ? "tweb.dk64" : "tweb.dk", sizeof(C2) * nh, &dk)) != CM_OK) return s;
+    if ((s = scratch(ctx, F64 ? "tweb.dkab64" : "tweb.dkab", sizeof(C2) * nh, &dab)) != CM_OK) return s;
+    if ((s = scratch(ctx, F64 ? "tweb.t6_64" : "tweb.t6", sizeof(T) * 6 * n3, &t6)) != CM_OK) return s;
+    if ((s = scratch(ctx, "tweb.stats", sizeof(unsigned long long) * 6, &stats)) != CM_OK) return s;
+    auto *w = (unsigned long long *)stats;
+    auto *norm = (unsigned *)(w + 5);
+    if ((s = tweb_zero_words(ctx, w, 6)) != CM_OK) return s;
+    if ((s = fft_r2c_impl(ctx, N, DT, delta, dk, 0.0)) != CM_OK) return s;
+    const double kf = 2.0 * M_PI / o->L;
+    const double R2h = o->R_smooth > 0 ? 0.5 * o->R_smooth * o->R_smooth : 0.0;
+    const double scale = o->delta_scale == 0.0 ? 1.0 : o->delta_scale;
+    static const int ab[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 0}, {2, 0}, {2, 1}};
+    T *tp = (T *)t6;
+    for (int c = 0; c < 6; ++c) {
+        if ((s = launch(ctx, "tweb_kab", k_kab<T>, dim3(grid_for(nh, 256, 16384)), dim3(256), 0, (const C2 *)dk,
+                        (C2 *)dab, N, kf, R2h, scale, ab[c][0], ab[c][1])) != CM_OK)
+            return s;
+        if ((s = fft_c2r_impl(ctx, N, DT, dab, tp + (size_t)c * n3, 1.0 / (double)n3)) != CM_OK) return s;
+    }
+    const unsigned gr = grid_for(n3, 256, 65536);
+    TolK tk{o->margin_tol, 0.0, nullptr};
+    if (o->margin_tol < 0.0) {
+        constexpr unsigned NP = 2048;
+        void *part;
+        if ((s = scratch(ctx, "tweb.sq6", sizeof(double) * 6 * NP, &part)) != CM_OK) return s;
+        const unsigned g = std::min(gr, NP);
+        for (int c = 0; c < 6; ++c)
+            if ((s = launch(ctx, "tweb_sumsq", k_sumsq<T>, dim3(g), dim3(256), 0, (const T *)(tp + (size_t)c * n3),
+                            n3, (double *)part + (size_t)c * g)) != CM_OK)
+                return s;
+        if ((s = launch(ctx, "tweb_rms", k_rms_tensor, dim3(1), dim3(1024), 0, (const double *)part, (int)g,
+                        (double)n3, norm)) != CM_OK)
+            return s;
+        const double n3d = (double)n3, u = F64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+        tk.tol = 0.0;
+        tk.per_norm = -o->margin_tol * std::sqrt(std::log2(n3d)) * std::sqrt(2.0 * std::log(3.0 * n3d)) * u;
+        tk.norm = norm;
+    }
+    if ((s = launch(ctx, "tweb_tensor_eig", k_tensor_eig<T>, dim3(gr), dim3(256), 0, (const T *)tp, n3,
+                    o->lambda_th, tk, labels, lam, w)) != CM_OK)
+        return s;
+    if (st) {
+        unsigned long long h[6];
+        CM_HIP(ctx, hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+        CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < 4; ++i) st->counts[i] = (int64_t)h[i];
+        st->n_marginal = (int64_t)h[4];
+        float pn;
+        const unsigned bits = (unsigned)h[5];
+        std::memcpy(&pn, &bits, sizeof pn);
+        st->margin_tol = tk.norm ? tk.per_norm * (double)pn : tk.tol;
+    }
+    return CM_OK;
+}
+
 static cm_status check_opts(cm_ctx *ctx, const cm_tweb_opts *o) {
     if (!o) return fail(ctx, CM_ERR_INVALID, "tweb: null options");
+    if (o->hessian != CM_HESSIAN_FD && o->hessian != CM_HESSIAN_SPECTRAL)
+        return fail(ctx, CM_ERR_INVALID, "tweb: hessian %d (0: FD, 1: spectral)", o->hessian);
     if (!fft_size_ok(o->N)) return fail(ctx, CM_ERR_INVALID, "tweb: N=%d not supported by the FFT", o->N);
     if (!(o->L > 0)) return fail(ctx, CM_ERR_INVALID, "tweb: L must be > 0");
     if (!std::isfinite(o->margin_tol)) return fail(ctx, CM_ERR_INVALID, "tweb: margin_tol must be finite");
@@ -684,6 +836,8 @@ cm_status cm_tweb_from_phi(cm_ctx *ctx, const cm_tweb_opts *o, const float *phi,
     cm_status s = check_opts(ctx, o);
     if (s != CM_OK) return s;
     if (!phi || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb_from_phi: null pointer");
+    if (o->hessian != CM_HESSIAN_FD)
+        return fail(ctx, CM_ERR_UNSUPPORTED, "cm_tweb_from_phi: the spectral Hessian is formed from delta (cm_tweb)");
     return tweb_from_phi_impl(ctx, o, phi, labels, lambdas, stats, false);
 }
 
@@ -693,6 +847,7 @@ cm_status cm_tweb(cm_ctx *ctx, const cm_tweb_opts *o, const float *delta, int8_t
     cm_status s = check_opts(ctx, o);
     if (s != CM_OK) return s;
     if (!delta || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb: null pointer");
+    if (o->hessian == CM_HESSIAN_SPECTRAL) return tweb_spectral_t<float>(ctx, o, delta, labels, lambdas, stats);
     void *phi;
     const size_t n3 = (size_t)o->N * o->N * o->N;
     if ((s = scratch(ctx, "tweb.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
@@ -706,6 +861,10 @@ cm_status cm_tweb_f64(cm_ctx *ctx, const cm_tweb_opts *o, const double *delta, i
     cm_status s = check_opts(ctx, o);
     if (s != CM_OK) return s;
     if (!delta || !labels) return fail(ctx, CM_ERR_INVALID, "cm_tweb_f64: null pointer");
+    if (o->hessian == CM_HESSIAN_SPECTRAL) {
+        if (phi_out) return fail(ctx, CM_ERR_INVALID, "cm_tweb_f64: the spectral Hessian forms no potential");
+        return tweb_spectral_t<double>(ctx, o, delta, labels, lambdas, stats);
+    }
     void *phi = phi_out;
     const size_t n3 = (size_t)o->N * o->N * o->N;
     if (!phi && (s = scratch(ctx, "tweb.phi64", sizeof(double) * n3, &phi)) != CM_OK) return s;

@@ -305,6 +305,10 @@ class Cwclass {
     real_prec volume_knots = 0, volume_filaments = 0, volume_sheets = 0, volume_voids = 0;
     int64_t n_marginal = 0;   // cells with an eigenvalue within margin_tol of lambdath
     double margin_tol = 0.0;  // the tolerance applied (float32 rounding of phi)
+    // tidal tensor of get_CWC: the FD Hessian of phi (the reference's default
+    // branch) or T_ab = k_a k_b / k^2 delta_k (its _USE_GFFT_EIGENV_ branch,
+    // a compile-time choice there, a run-time one here)
+    bool spectral_hessian = false;
 
    private:
     Params params;

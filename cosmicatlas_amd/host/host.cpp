@@ -377,7 +377,7 @@ void Cwclass::get_CWC(std::vector<real_prec> &delta) {
     // margin: the cells whose label the float32 rounding of the potential can
     // change are counted (tol = K(N) u ||phi||_rms / d^2)
     cm_tweb_opts o{(int)N, (double)params._Lbox(), (double)params._smscale(), (double)params._lambdath(),
-                   CM_TWEB_TOL_ROUNDING, 1.0};
+                   CM_TWEB_TOL_ROUNDING, 1.0, spectral_hessian ? CM_HESSIAN_SPECTRAL : CM_HESSIAN_FD};
     cm_tweb_stats st;
     check(cm_tweb(ctx(), &o, (const float *)dd.p, (int8_t *)dlab.p, (float *)dlam.p, &st), "cm_tweb");
     std::vector<int8_t> lab(n3);

@@ -200,7 +200,17 @@ typedef struct {
                             for s times the rounding tolerance
                             K(N) u ||phi||_rms / d^2 (see CM_TWEB_TOL_ROUNDING) */
     double delta_scale;  /* input multiplied by this before the Poisson solve (1/mean for rho-mean) */
+    int hessian;         /* CM_HESSIAN_FD (0): gradfindif applied twice to phi, the reference's default
+                            branch (Source/Miscelanious.cpp:1684-1700); CM_HESSIAN_SPECTRAL (1): T_ab =
+                            c2r(k_a k_b / k^2 W(k) delta_k), its _USE_GFFT_EIGENV_ branch (calc_LapPhiv,
+                            Source/NumericalMethods.cpp:2605-2679; Source/Miscelanious.cpp:1703-1718), with
+                            FP64 closed-form eigenvalues per cell and, for a negative margin_tol, the
+                            rounding tolerance s K_s(N) u ||T||_F,rms, K_s = sqrt(log2 N^3)
+                            sqrt(2 ln(3 N^3)).  cm_tweb and cm_tweb_f64 take either; the fused
+                            measurement and cm_tweb_from_phi the FD Hessian only. */
 } cm_tweb_opts;
+#define CM_HESSIAN_FD 0
+#define CM_HESSIAN_SPECTRAL 1
 
 /* margin_tol value that counts the cells whose label the rounding of the
  * potential's arithmetic can change: tol = K(N) u ||phi||_rms / d^2 with

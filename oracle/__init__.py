@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
         L.or_eigen3.argtypes = [P, P]
         L.or_tweb.argtypes = [C.c_int, C.c_double, P, C.c_double, C.c_double, P, P, P, P]
         L.or_tweb_from_phi.argtypes = [C.c_int, C.c_double, P, C.c_double, P, P, P]
+        L.or_tweb_spectral.argtypes = [C.c_int, C.c_double, P, C.c_double, C.c_double, P, P, P]
         L.or_bk_direct.argtypes = [C.c_int, P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, P, P, P, P, P]
         _LIB = L
     return _LIB
@@ -184,6 +185,19 @@ def tweb(delta, L, R, lambda_th, want_lambdas=True, want_phi=False):
     counts = np.zeros(4, np.int64)
     lib().or_tweb(N, L, _p(delta), R, lambda_th, _p(labels), _p(lam), _p(phi), _p(counts))
     return labels, lam, phi, counts
+
+
+def tweb_spectral(delta, L, R, lambda_th):
+    """T-web with the spectral tidal tensor T_ab = c2r(W k_a k_b / k^2 delta_k)
+    (EigenValuesTweb _USE_GFFT_EIGENV_ branch, Miscelanious.cpp:1703-1718;
+    calc_LapPhiv, NumericalMethods.cpp:2605-2679): (labels, lambdas, counts)."""
+    delta = np.ascontiguousarray(delta, np.float64)
+    N = delta.shape[0]
+    labels = np.zeros((N, N, N), np.int8)
+    lam = np.zeros((3, N, N, N))
+    counts = np.zeros(4, np.int64)
+    lib().or_tweb_spectral(N, L, _p(delta), R, lambda_th, _p(labels), _p(lam), _p(counts))
+    return labels, lam, counts
 
 
 def tweb_from_phi(phi, L, lambda_th):

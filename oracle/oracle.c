@@ -583,6 +583,59 @@ void or_tweb(int N, double L, const double *delta, double R, double lambda_th,
     free(dk);
 }
 
+void or_tweb_spectral(int N, double L, const double *delta, double R, double lambda_th,
+                      int8_t *labels, double *lambdas, int64_t counts[4])
+{
+    /* EigenValuesTweb _USE_GFFT_EIGENV_ branch (Miscelanious.cpp:1703-1718):
+     * T_ab = c2r(k_a k_b / k^2 delta_k) per component (calc_LapPhiv,
+     * NumericalMethods.cpp:2605-2679: coords +i k_f for i <= N/2, zero at
+     * k = 0), the Gaussian of the pre-smoothing applied to delta_k; the matrix
+     * is symmetric, so the six components ab = xx, yy, zz, yx, zx, zy fill it. */
+    const size_t NN = (size_t)N, n3 = NN * NN * NN, nh = NN / 2 + 1, nhh = NN * NN * nh;
+    const double kf = 2.0 * M_PI / L;
+    static const int ab[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 0}, {2, 0}, {2, 1}};
+    cplx *dk = (cplx *)malloc(sizeof(cplx) * nhh), *tmp = (cplx *)malloc(sizeof(cplx) * nhh);
+    double *T[6];
+    or_fft_r2c(N, delta, (double *)dk);
+    for (int c = 0; c < 6; ++c) {
+        T[c] = (double *)malloc(sizeof(double) * n3);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j)
+                for (int k = 0; k < (int)nh; ++k) {
+                    const double kv[3] = {kcoord(i, N, kf), kcoord(j, N, kf), kcoord(k, N, kf)};
+                    const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                    double f = 0.0;
+                    if (k2 > 0) {
+                        f = kv[ab[c][0]] * kv[ab[c][1]] / k2;
+                        if (R > 0) f *= exp(-0.5 * k2 * R * R);
+                    }
+                    const size_t idx = ((size_t)i * NN + (size_t)j) * nh + (size_t)k;
+                    tmp[idx].re = dk[idx].re * f;
+                    tmp[idx].im = dk[idx].im * f;
+                }
+        or_fft_c2r(N, (const double *)tmp, T[c]);
+    }
+    if (counts) memset(counts, 0, 4 * sizeof(int64_t));
+    for (size_t c = 0; c < n3; ++c) {
+        double m[9];
+        m[0] = T[0][c]; m[4] = T[1][c]; m[8] = T[2][c];
+        m[1] = m[3] = T[3][c];
+        m[2] = m[6] = T[4][c];
+        m[5] = m[7] = T[5][c];
+        double ev[3];
+        or_eigen3(m, ev);
+        int v = 0;
+        for (int q = 0; q < 3; ++q) if (ev[q] > lambda_th) ++v;
+        const int lab = 4 - v;
+        if (labels) labels[c] = (int8_t)lab;
+        if (lambdas) { lambdas[c] = ev[0]; lambdas[n3 + c] = ev[1]; lambdas[2 * n3 + c] = ev[2]; }
+        if (counts) counts[lab - 1]++;
+    }
+    for (int c = 0; c < 6; ++c) free(T[c]);
+    free(tmp);
+    free(dk);
+}
+
 /* bin of |q|^2 = q2 (units of k_f), -1 outside [Dmin, Dmin + nb ndel) */
 static int bkd_bin(long q2, double Dmin, double ndel, int nb, int int_rule, long Di, long ni)
 {

@@ -118,6 +118,13 @@ void or_eigen3(const double m[9], double ev[3]);
 void or_tweb(int N, double L, const double *delta, double R, double lambda_th,
              int8_t *labels, double *lambdas, double *phi, int64_t counts[4]);
 
+/* T-web with the spectral tidal tensor (EigenValuesTweb _USE_GFFT_EIGENV_
+ * branch, Source/Miscelanious.cpp:1703-1718; calc_LapPhiv,
+ * Source/NumericalMethods.cpp:2605-2679): T_ab = c2r(W(k) k_a k_b / k^2
+ * delta_k); eigenvalues, labels and counts as or_tweb. */
+void or_tweb_spectral(int N, double L, const double *delta, double R, double lambda_th,
+                      int8_t *labels, double *lambdas, int64_t counts[4]);
+
 /* T-web from a given potential (EigenValuesTweb FD branch + labels only). */
 void or_tweb_from_phi(int N, double L, const double *phi, double lambda_th,
                       int8_t *labels, double *lambdas, int64_t counts[4]);

@@ -149,3 +149,87 @@ def test_tweb_marginal_count_matches_eigenvalues(ctx, frac):
     assert exact > 0
     assert abs(st["n_marginal"] - exact) <= edge
     assert st["n_marginal"] == st_l["n_marginal"]
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("N,L,R,lth", [(32, 64.0, 3.0, 0.0), (48, 100.0, 4.0, 0.02)])
+def test_spectral_hessian_matches_oracle(ctx, orc, dtype, N, L, R, lth):
+    """hessian="spectral": T_ab = c2r(W k_a k_b / k^2 delta_k) (the reference's
+    _USE_GFFT_EIGENV_ branch) against the oracle's restatement: eigenvalues to
+    float32 (FP64) round-off, labels equal outside the margin, counts
+    consistent.  The fields carry no power at the Nyquist planes (smoothing),
+    where the mixed multipliers with +N/2 k_f are not Hermitian."""
+    delta = _smooth_field(N, N + 1)
+    lab_ref, lam_ref, cnt_ref = orc.tweb_spectral(delta, L, R, lth)
+    scale = np.abs(lam_ref).max()
+    if dtype == "f32":
+        tol = 1e-4 * scale
+        labels, lam, st = ctx.tweb(torch.from_numpy(delta.astype(np.float32)).cuda(), L, R, lth, margin_tol=tol,
+                                   want_lambdas=True, hessian="spectral")
+        err = 2e-5
+    else:
+        # the closed-form (trigonometric) eigenvalues carry errors up to about
+        # sqrt(u) |lambda| where two eigenvalues nearly coincide (acos near
+        # +-1): measured 1e-8 of the largest here, against the oracle's
+        # iterative solve
+        tol = 1e-6 * scale
+        labels, lam, _, st = ctx.tweb_f64(torch.from_numpy(delta).cuda(), L, R, lth, margin_tol=tol,
+                                          want_lambdas=True, hessian="spectral")
+        err = 1e-7
+    lam = lam.cpu().numpy()
+    labels = labels.cpu().numpy()
+    assert np.max(np.abs(lam - lam_ref)) < err * scale
+    ok = np.abs(lam_ref - lth).min(axis=0) >= tol
+    assert np.array_equal(labels[ok], lab_ref[ok])
+    assert sum(st["counts"]) == N ** 3
+    assert np.array_equal(np.bincount(labels.ravel(), minlength=5)[1:], st["counts"])
+    assert int(np.count_nonzero(labels != lab_ref)) <= st["n_marginal"] + int((~ok).sum())
+
+
+def test_spectral_hessian_plane_wave_and_rounding_tol(ctx):
+    """Plane wave delta = A cos(2 pi q.i / N) (FP64): eigenvalues {A W cos, 0,
+    0} in closed form, labels from them; the marginal count and the rounding
+    tolerance of the spectral tensor."""
+    import cosmicatlas_amd as cm
+    N, L, A, q, R = 32, 100.0, 0.5, (3, -1, 2), 2.0
+    i = np.arange(N)
+    c = np.cos(2 * np.pi * (q[0] * i[:, None, None] + q[1] * i[None, :, None] + q[2] * i[None, None, :]) / N)
+    kf = 2 * np.pi / L
+    lam1 = A * np.exp(-0.5 * kf * kf * sum(x * x for x in q) * R * R) * c
+    labels, lam, _, st = ctx.tweb_f64(torch.from_numpy(A * c).cuda(), L, R, -0.3 * A, want_lambdas=True,
+                                      hessian="spectral")
+    lam = lam.cpu().numpy()
+    # a double zero eigenvalue is the worst case of the closed form: its roots
+    # come out within about sqrt(u) |lambda_1| (measured 4e-9 here)
+    eps = 1e-7 * A
+    np.testing.assert_allclose(lam[0], np.maximum(lam1, 0.0), atol=eps)
+    np.testing.assert_allclose(lam[1], 0.0, atol=eps)
+    np.testing.assert_allclose(lam[2], np.minimum(lam1, 0.0), atol=eps)
+    sel = np.abs(lam1 + 0.3 * A) > eps
+    assert np.array_equal(labels.cpu().numpy()[sel], (2 - (lam1 > -0.3 * A).astype(int))[sel])
+    # lambda_th = 0: two eigenvalues vanish analytically, so every cell is
+    # within 1e-6 of the threshold (the closed form puts them within 4e-9)
+    _, _, _, st1 = ctx.tweb_f64(torch.from_numpy(A * c).cuda(), L, R, 0.0, margin_tol=1e-6, hessian="spectral")
+    assert st1["n_marginal"] == N ** 3
+    _, _, _, st0 = ctx.tweb_f64(torch.from_numpy(A * c).cuda(), L, R, 0.0, margin_tol=cm.TOL_ROUNDING,
+                                hessian="spectral")
+    assert 0.0 < st0["margin_tol"] < 1e-12
+
+
+def test_hessian_selector_errors(ctx):
+    import ctypes as C
+    import cosmicatlas_amd as cm
+    from cosmicatlas_amd import _abi
+    N, L = 16, 50.0
+    delta = torch.zeros((N, N, N), dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        ctx.tweb(delta, L, 0.0, 0.0, hessian="fourier")
+    labels = torch.empty((N, N, N), dtype=torch.int8, device="cuda")
+    o = _abi.TwebOpts(N, L, 0.0, 0.0, 0.0, 1.0, 5)
+    assert ctx.lib.cm_tweb(ctx.h, C.byref(o), C.c_void_p(delta.data_ptr()), C.c_void_p(labels.data_ptr()), None,
+                           None) == _abi.CM_ERR_INVALID
+    o = _abi.TwebOpts(N, L, 0.0, 0.0, 0.0, 1.0, 1)
+    assert ctx.lib.cm_tweb_from_phi(ctx.h, C.byref(o), C.c_void_p(delta.data_ptr()), C.c_void_p(labels.data_ptr()),
+                                    None, None) == _abi.CM_ERR_UNSUPPORTED
+    with pytest.raises(cm.CosmoError):
+        ctx.tweb_f64(delta.double(), L, 0.0, 0.0, want_phi=True, hessian="spectral")

@@ -445,3 +445,65 @@ def test_tweb_plane_wave(orc):
     assert np.array_equal(lab_hi[sel, 0, 0], np.where(c > 0, 3, 4)[sel])
     assert np.array_equal(lab_lo[sel, 0, 0], np.where(c > 0, 1, 2)[sel])
     assert np.all(lab_hi == lab_hi[:, :1, :1]) and np.all(lab_lo == lab_lo[:, :1, :1])
+
+
+@pytest.mark.parametrize("q,R", [((2, 1, 3), 0.0), ((1, -2, 1), 0.0), ((3, 0, -1), 5.0)])
+def test_tweb_spectral_plane_wave(orc, q, R):
+    """Spectral tidal tensor of delta = A cos(2 pi q.i / N): T_ab = A W(k)
+    k_a k_b / k^2 cos(.) exactly (k = q k_f), so one eigenvalue is A W cos(.)
+    along q and two are zero; a negative component of q pins the sign of the
+    off-diagonal products.  With lambda_th = -0.3 A the zero eigenvalues are
+    above the threshold: label = 2 - [A W cos > -0.3 A]."""
+    N, L, A = 16, 80.0, 0.7
+    i = np.arange(N)
+    ph = TWO_PI * (q[0] * i[:, None, None] + q[1] * i[None, :, None] + q[2] * i[None, None, :]) / N
+    c = np.cos(ph)
+    delta = A * c
+    kf = TWO_PI / L
+    k2 = kf * kf * (q[0] ** 2 + q[1] ** 2 + q[2] ** 2)
+    lam1 = A * np.exp(-0.5 * k2 * R * R) * c
+    lth = -0.3 * A
+    labels, lam, counts = orc.tweb_spectral(delta, L, R, lth)
+    np.testing.assert_allclose(lam[0], np.maximum(lam1, 0.0), atol=1e-12)
+    np.testing.assert_allclose(lam[1], 0.0, atol=1e-12)
+    np.testing.assert_allclose(lam[2], np.minimum(lam1, 0.0), atol=1e-12)
+    sel = np.abs(lam1 - lth) > 1e-9
+    assert np.array_equal(labels[sel], (2 - (lam1 > lth).astype(int))[sel])
+    assert counts.sum() == N ** 3
+
+
+def test_tweb_spectral_trace_is_smoothed_delta(orc):
+    """The trace of T_ab = k_a k_b / k^2 W delta_k is W delta_k without its k = 0
+    mode: the three eigenvalues sum to the smoothed, mean-free field (numpy's
+    FFT), on a random field; and the spectral tensor of the oracle agrees with
+    a numpy restatement built from numpy's FFT (all nine components)."""
+    rng = np.random.default_rng(5)
+    N, L, R = 16, 60.0, 4.0
+    delta = rng.standard_normal((N, N, N))
+    labels, lam, counts = orc.tweb_spectral(delta, L, R, 0.0)
+    kf = TWO_PI / L
+    kx = np.fft.fftfreq(N, 1.0 / N) * kf
+    kz = np.fft.rfftfreq(N, 1.0 / N) * kf
+    KX, KY, KZ = np.meshgrid(kx, kx, kz, indexing="ij")
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    dk = np.fft.rfftn(delta) * np.exp(-0.5 * k2 * R * R)
+    dk[0, 0, 0] = 0.0
+    np.testing.assert_allclose(lam.sum(axis=0), np.fft.irfftn(dk, s=(N, N, N)), atol=1e-12)
+    # the Nyquist index of numpy's fftfreq is -N/2; the reference's coords take
+    # +N/2; the mixed products differ only there, and a field with no power at
+    # the Nyquist planes (zeroed here) makes both agree
+    K = [KX, KY, KZ]
+    K[0] = np.where(np.abs(KX) >= N // 2 * kf, 0.0, KX)
+    K[1] = np.where(np.abs(KY) >= N // 2 * kf, 0.0, KY)
+    nyq = (np.abs(KX) >= N // 2 * kf) | (np.abs(KY) >= N // 2 * kf) | (KZ >= N // 2 * kf)
+    dk2 = np.where(nyq, 0.0, dk)
+    delta2 = np.fft.irfftn(np.fft.rfftn(delta) * np.where(nyq, 0.0, 1.0), s=(N, N, N))
+    _, lam2, _ = orc.tweb_spectral(delta2, L, R, 0.0)
+    H = np.empty((N, N, N, 3, 3))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for a in range(3):
+            for b in range(3):
+                m = np.where(k2 > 0, K[a] * K[b] / k2, 0.0)
+                H[..., a, b] = np.fft.irfftn(m * dk2, s=(N, N, N))
+    ev = np.linalg.eigvalsh(H)[..., ::-1]
+    np.testing.assert_allclose(np.moveaxis(lam2, 0, -1), ev, atol=1e-11)
