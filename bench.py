@@ -89,6 +89,39 @@ def lib_name(kname):
     return None
 
 
+def _largest_grid(rows):
+    """Per library kernel, the rows of its largest grid: the measured mock's
+    dispatches, without the smaller ones the input generator makes of the same
+    kernels (its 256^3 FFT passes)."""
+    big = {}
+    for r in rows:
+        name = lib_name(r["Kernel_Name"])
+        if name:
+            big[name] = max(big.get(name, 0), int(r["Grid_Size"]))
+    return [r for r in rows if lib_name(r["Kernel_Name"]) and int(r["Grid_Size"]) == big[lib_name(r["Kernel_Name"])]]
+
+
+def valu_from_pmc(csv_path, out_json):
+    """Median SQ_INSTS_VALU per dispatch and library kernel from a rocprofv3
+    --pmc pass (SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES) of
+    `bench.py --fused-only`; the counter is summed over the per-XCD/SE
+    instances of each dispatch."""
+    import csv
+    per = {}
+    rows = [r for r in csv.DictReader(open(csv_path)) if r["Counter_Name"] == "SQ_INSTS_VALU"]
+    for r in _largest_grid(rows):
+        key = (lib_name(r["Kernel_Name"]), r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    vals = {}
+    for (name, _), v in per.items():
+        vals.setdefault(name, []).append(v)
+    out = {"source": "SQ_INSTS_VALU, wave instructions per dispatch, 512^3 TSC single mock (bench.py --fused-only)",
+           "sq_insts_valu": {k: float(np.median(v)) for k, v in sorted(vals.items())}}
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
 def traffic_from_pmc(fetch_csv, write_csv, out_json):
     """HBM bytes per dispatch from two rocprofv3 --pmc passes (FETCH_SIZE, then
     WRITE_SIZE, KiB per dispatch) of `bench.py --fused-only --copy-calibration`.
@@ -99,16 +132,12 @@ def traffic_from_pmc(fetch_csv, write_csv, out_json):
 
     def load(path, counter):
         vals, copy = {}, []
-        for r in csv.DictReader(open(path)):
-            if r["Counter_Name"] != counter:
-                continue
-            v = float(r["Counter_Value"])
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        for r in rows:
             if "copyBuffer" in r["Kernel_Name"]:
-                copy.append(v)
-                continue
-            name = lib_name(r["Kernel_Name"])
-            if name:
-                vals.setdefault(name, []).append(v)
+                copy.append(float(r["Counter_Value"]))
+        for r in _largest_grid(rows):
+            vals.setdefault(lib_name(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
         return {k: float(np.median(v)) for k, v in vals.items()}, max(copy)
 
     fetch, cf = load(fetch_csv, "FETCH_SIZE")
@@ -671,7 +700,12 @@ def main():
                     help="run the large interlaced 1024^3 configuration instead (one JSON line)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="convert two rocprofv3 --pmc counter files into profiles/traffic.json and exit")
+    ap.add_argument("--valu-from", metavar="VALU_CSV",
+                    help="convert a rocprofv3 --pmc SQ_INSTS_VALU counter file into profiles/valu.json and exit")
     args = ap.parse_args()
+    if args.valu_from:
+        print(json.dumps(valu_from_pmc(args.valu_from, os.path.join(ROOT, "profiles", "valu.json"))))
+        return
     if args.traffic_from:
         print(json.dumps(traffic_from_pmc(args.traffic_from[0], args.traffic_from[1],
                                           os.path.join(ROOT, "profiles", "traffic.json"))))
