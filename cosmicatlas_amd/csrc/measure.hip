@@ -478,14 +478,10 @@ cm_status measure_packed(cm_ctx *ctx, const cm_packed_mock *m, int nm, cm_status
         while (k1 < pk.size() && (k1 == k0 || held + packed_bytes(m[pk[k1]]) <= avail / 2))
             held += packed_bytes(m[pk[k1++]]);
         const std::vector<int> grp(pk.begin() + (long)k0, pk.begin() + (long)k1);
-        const cm_status s = packed_group(ctx, m, grp, st.data());
-        if (s != CM_OK) {
-            for (int i : grp) st[(size_t)i] = s;
-            const std::string e = ctx->err;
-            for (int i : grp)  // the group failed as a whole: each mock alone
+        if (packed_group(ctx, m, grp, st.data()) != CM_OK) {
+            for (int i : grp)  // the group failed as a whole (e.g. its buffers): each mock alone
                 st[(size_t)i] = measure_run(ctx, &m[i].opts, m[i].pos, m[i].w, m[i].n, m[i].pk0, m[i].pk2,
                                             m[i].pk4, m[i].nmodes, m[i].labels, m[i].info);
-            (void)e;
         }
         k0 = k1;
     }

@@ -89,6 +89,21 @@ def test_packed_repeat_is_reproducible(ctx):
         _check_same(q, p, spec)
 
 
+def test_packed_empty_mock(ctx):
+    """A mock without particles beside others: the same result as alone."""
+    import cosmicatlas_amd as cm
+    a = _args(ctx, SPECS[1])
+    e = dict(_args(ctx, SPECS[2]))
+    e["pos"] = e["pos"][:0]
+    res = ctx.measure_packed([a, e])
+    kw = {k: v for k, v in e.items() if k not in ("pos", "N", "L", "scheme", "nshells", "kmin", "dk_bin")}
+    alone = ctx.measure(e["pos"], e["N"], e["L"], e["scheme"], e["nshells"], e["kmin"], e["dk_bin"], **kw)
+    assert res[1]["n_obj"] == alone["n_obj"] == 0.0
+    assert np.array_equal(res[1]["nmodes"], alone["nmodes"])
+    np.testing.assert_array_equal(res[1]["pk0"], alone["pk0"])
+    assert torch.equal(res[1]["labels"], alone["labels"])
+
+
 def test_packed_errors_are_per_mock(ctx):
     """A mock that fails its checks fails alone; the others are measured."""
     import cosmicatlas_amd as cm
