@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/cosmo_mesh.h"
+#include "memspace.hpp"
 
 namespace cm {
 

@@ -68,8 +68,13 @@ constexpr int ct_row_stride(int M) { return M + 1 + ((M + 1) >> 3) + 1; }
 // The first stage reads through src(b, e), the last writes through
 // dst(b, e, v); the caller owns the barrier before the first stage.
 template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int S, int Ns, int E, class Src, class Dst>
-__device__ __forceinline__ void ct_stage(v2<T> *lds, const v2<T> *__restrict__ tw, const Src &src, const Dst &dst) {
+__device__ __forceinline__ void ct_stage(v2<T> *lds_p, const v2<T> *__restrict__ tw_p, const Src &src,
+                                         const Dst &dst) {
     using C = v2<T>;
+    // LDS and global views: the pointers may arrive through a non-inlined
+    // function, where they would otherwise be accessed with flat instructions
+    const auto lds = as_local(lds_p);
+    const auto tw = as_global(tw_p);
     constexpr int R = ct_radix(n, S), NST = ct_nst(n);
     constexpr bool FIRST = S == 0, LAST = S == NST - 1;
     constexpr int IT = E / R, nR = n / R, NT = NB * n / E, tws = n / (Ns * R);
@@ -121,7 +126,7 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds, const v2<T> *__restrict__ t
     }
     if constexpr (!LAST) {
         __syncthreads();
-        ct_stage<T, n, NB, STRIDE, COLS, FWD, S + 1, Ns * R, E>(lds, tw, src, dst);
+        ct_stage<T, n, NB, STRIDE, COLS, FWD, S + 1, Ns * R, E>(lds_p, tw_p, src, dst);
     }
 }
 

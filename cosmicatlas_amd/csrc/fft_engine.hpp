@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "memspace.hpp"
+
 namespace cm {
 
 template <typename T> struct vec2;
@@ -120,10 +122,12 @@ template <> struct Items<false> {
 };
 
 template <typename T, int E, int R, bool COLS, bool FWD, class Src, class Dst>
-__device__ __forceinline__ void fft_stage(v2<T> *lds, int ldsStride, int n, int nb, int Ns,
-                                          bool first, bool last, const v2<T> *__restrict__ tw,
+__device__ __forceinline__ void fft_stage(v2<T> *lds_p, int ldsStride, int n, int nb, int Ns,
+                                          bool first, bool last, const v2<T> *__restrict__ tw_p,
                                           const Src &src, const Dst &dst) {
     using C = v2<T>;
+    const auto lds = as_local(lds_p);
+    const auto tw = as_global(tw_p);
     constexpr int IT = E / R;
     const int NT = blockDim.x;
     const int tid = threadIdx.x;

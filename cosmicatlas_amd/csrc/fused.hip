@@ -31,8 +31,11 @@ template <int N, int NB, int E>
 __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restrict__ data, size_t base,
                                                             size_t xstride, int valid, float2 *lds,
                                                             const float2 *__restrict__ tw) {
-    auto srcG = [&](int b, int e) {
-        return b < valid ? data[base + (size_t)e * xstride + b] : float2{0.f, 0.f};
+    const auto g = as_global(data);
+    auto srcG = [&](int b, int e) -> float2 {
+        if (b >= valid) return float2{0.f, 0.f};
+        const float2 v = g[base + (size_t)e * xstride + b];
+        return v;
     };
     auto dstL = [&](int b, int e, float2 v) { lds[e * (NB + 1) + b] = v; };
     ct_fft<float, N, NB, NB + 1, true, true, E>(lds, tw, srcG, dstL);
@@ -43,8 +46,9 @@ __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict_
                                                               int valid, float2 *lds,
                                                               const float2 *__restrict__ tw) {
     auto srcL = [&](int b, int e) { return lds[e * (NB + 1) + b]; };
+    const auto g = as_global(data);
     auto dstG = [&](int b, int e, float2 v) {
-        if (b < valid) data[base + (size_t)e * xstride + b] = v;
+        if (b < valid) g[base + (size_t)e * xstride + b] = v;
     };
     ct_fft<float, N, NB, NB + 1, true, false, E>(lds, tw, srcL, dstG);
 }

@@ -722,8 +722,8 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
         for (int r = 0; r < RPT; ++r) {
             const unsigned idx = x.y + (unsigned)(tid + r * NT);
             if (idx < x.z) {
-                rec[r] = m.sorted[idx];
-                wr[r] = UNIT ? 1.0f : m.wsorted[idx];
+                rec[r] = as_global(m.sorted)[idx];
+                wr[r] = UNIT ? 1.0f : as_global(m.wsorted)[idx];
             }
         }
     };
@@ -777,11 +777,11 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
                                  : from_fixed(((const long long *)lgrid)[li]);
             float *dst = mp + k * mstride;
             if (split) {
-                if (v != 0.0f) unsafeAtomicAdd(dst, v);
+                if (v != 0.0f) global_add(dst, v);
             } else if (accumulate) {
-                *dst += v;
+                *as_global(dst) += v;
             } else {
-                *dst = v;
+                *as_global(dst) = v;
             }
         }
         __syncthreads();
@@ -984,10 +984,10 @@ static cm_status mas_zero_split(cm_ctx *ctx, const TileBins &b, const TileK &t, 
                   (const unsigned *)b.offsets, t, b.N, mesh);
 }
 
-// Deposit grid: persistent workgroups, dep_wg (default 8) per CU, at most one
+// Deposit grid: persistent workgroups, dep_wg (default 16) per CU, at most one
 // per work item.
 static unsigned deposit_grid(cm_ctx *ctx, size_t max_items) {
-    const int wg_per_cu = ctx->tune.dep_wg > 0 && ctx->tune.dep_wg <= 64 ? ctx->tune.dep_wg : 8;
+    const int wg_per_cu = ctx->tune.dep_wg > 0 && ctx->tune.dep_wg <= 64 ? ctx->tune.dep_wg : 16;
     return (unsigned)std::max<size_t>(1, std::min<size_t>(max_items, (size_t)std::max(1, ctx->cu_count) * wg_per_cu));
 }
 

@@ -1,0 +1,35 @@
+// Address-space views of generic pointers (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cm {
+
+// Global-memory view of a generic pointer.  A pointer that reaches a kernel
+// through a table in memory, or a parameter of a non-inlined device function,
+// is generic: its loads and stores are issued as flat instructions, which
+// count against the LDS wait counter as well as the vector-memory one, so a
+// wait for an LDS result (or the wait before a barrier) also waits for every
+// flat load or store in flight, and a prefetch no longer overlaps the work
+// before it.  Every buffer these kernels address lives in device global
+// memory, so their accesses go through this view as global loads and stores.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+template <typename T> using lmem = __attribute__((address_space(3))) T;
+#else
+// host pass: device functions are parsed there but never compiled for the GPU
+template <typename T> using gmem = T;
+template <typename T> using lmem = T;
+#endif
+template <typename T> __device__ __forceinline__ gmem<T> *as_global(T *p) { return (gmem<T> *)p; }
+
+// Float atomic add to global memory without return (the hardware add, as
+// unsafeAtomicAdd), issued as a global atomic.
+__device__ __forceinline__ void global_add(float *p, float v) { (void)__builtin_amdgcn_global_atomic_fadd_f32(as_global(p), v); }
+
+// LDS view of a generic pointer that points into the workgroup's shared
+// memory: ds_ instructions instead of flat ones where the compiler cannot see
+// that a pointer (a parameter of a non-inlined function) addresses LDS.
+template <typename T> __device__ __forceinline__ lmem<T> *as_local(T *p) { return (lmem<T> *)p; }
+
+}  // namespace cm

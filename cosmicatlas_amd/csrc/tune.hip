@@ -110,7 +110,7 @@ cm_status cm_autotune(cm_ctx *ctx, const cm_measure_opts *o, const void *pos, co
     if ((s = timed(best, 2, &best_ms)) != CM_OK) return restore(s);
     t0 = best_ms;
     // candidate values per knob (0 = built-in)
-    const std::vector<int> xnb_c = {16, 8}, xe_c = {16, 8}, dep_c = {4, 16};
+    const std::vector<int> xnb_c = {16, 8}, xe_c = {16, 8}, dep_c = {4, 8};
     std::vector<int> tz_c, tseg_c = {1, 2, 4, 8};
     for (int tz : {64, 32, 16})
         if (N % tz == 0) tz_c.push_back(tz);

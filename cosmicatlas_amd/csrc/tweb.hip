@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     float pf[PF];
     auto fetch = [&](int s) {
         const int sp = (s % N + N) % N;
-        const float *plane = phi + (size_t)sp * NN * NN;
+        const auto plane = as_global(phi) + (size_t)sp * NN * NN;
 #pragma unroll
         for (int k = 0; k < PF; ++k)
             if (tid + k * NT < HP) pf[k] = plane[foff[k]];
@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         const int c = s - 4;
         if (c < x0) return;
         const int gs = gslot(c);  // uniform
-        int8_t *lplane = labels + (size_t)c * NN * NN;
+        const auto lplane = as_global(labels) + (size_t)c * NN * NN;
         // T of cell k: txx, tyy, tzz, a10 = D_x G_y, a20 = D_x G_z, a21 = D_y G_z;
         // G_y at gy[gs + i], G_z at gy[gs + 3 GP + i]
         constexpr int ZO = 3 * GP;
