@@ -244,14 +244,16 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
     }
 }
 
-__global__ void k_fused_zero(double *s, unsigned long long *c, int ns) {
-    for (int i = threadIdx.x; i < 4 * ns; i += blockDim.x) s[i] = 0.0;
-    for (int i = threadIdx.x; i < ns; i += blockDim.x) c[i] = 0ull;
-}
-
 // FP64 total of the z pass's partial sums (one workgroup, four independent
-// accumulators per thread so the loads overlap).
-__global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ part, int np, double *__restrict__ out) {
+// accumulators per thread so the loads overlap).  The same launch clears the
+// accumulators of the passes after it: shell sums zs[0, 4 ns) and counts
+// zc[0, ns) of the fused x pass, the five T-web counters zt (nullable).
+__global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ part, int np, double *__restrict__ out,
+                                                    double *__restrict__ zs, unsigned long long *__restrict__ zc,
+                                                    int ns, unsigned long long *__restrict__ zt) {
+    for (int i = threadIdx.x; i < 4 * ns; i += blockDim.x) zs[i] = 0.0;
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) zc[i] = 0ull;
+    if (zt && threadIdx.x < 5) zt[threadIdx.x] = 0ull;
     __shared__ double red[1024];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int i = threadIdx.x;
@@ -273,7 +275,7 @@ __global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ p
 
 PkK make_pkk(cm_ctx *ctx, const cm_pk_opts *o, cm_status *st);
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
-                            float *lam, unsigned long long *stats);
+                            float *lam, unsigned long long *stats, bool stats_zeroed);
 TolK tweb_tol(int N, double L, double margin_tol, bool fp64, const unsigned *norm);
 cm_status phi_rms_final(cm_ctx *ctx, const double *part, int np, size_t n, unsigned *out);
 
@@ -390,7 +392,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     // z pass with the mesh sum, then the FP64 total
     if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part, pitch)) != CM_OK) return s;
     if ((s = launch(ctx, "fused_sum", k_fused_sum, dim3(1), dim3(1024), 0, (const double *)part, nparts,
-                    (double *)dnobj)) != CM_OK)
+                    (double *)dnobj, dsum, dcnt, ns, dstats)) != CM_OK)
         return s;
     // y pass
     if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch)) != CM_OK) return s;
@@ -412,10 +414,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
         if ((s = fft_cols(ctx, N, CM_F32, dkB, 1, true, pitch, phase)) != CM_OK) return s;
         if ((s = fft_cols(ctx, N, CM_F32, dkB, 0, true, pitch, phase)) != CM_OK) return s;
     }
-    // x pass + shells + multiply + inverse x
-    if ((s = launch(ctx, "fused_zero", k_fused_zero, dim3(1), dim3(256), 0, (double *)dsum, (unsigned long long *)dcnt,
-                    ns)) != CM_OK)
-        return s;
+    // x pass + shells + multiply + inverse x (sums cleared by fused_sum)
     cm_pk_opts po;
     po.N = N;
     po.L = o->mas.L;
@@ -505,7 +504,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
             return CM_OK;
         }
         if ((s = tweb_fused_launch(ctx, N, o->mas.L, (const float *)phi, o->lambda_th, tk, labels, nullptr,
-                                   (unsigned long long *)dstats)) != CM_OK)
+                                   (unsigned long long *)dstats, true)) != CM_OK)
             return s;
     }
     return CM_OK;

@@ -72,6 +72,14 @@ __global__ void __launch_bounds__(256) k_mas_plain(const PT *__restrict__ pos, c
     }
 }
 
+// Clears a particle-drop counter and (nullable) an overflow flag in one launch.
+__global__ void k_clear_flags(unsigned long long *__restrict__ dropped, unsigned *__restrict__ ovf) {
+    if (threadIdx.x == 0) {
+        *dropped = 0ull;
+        if (ovf) *ovf = 0u;
+    }
+}
+
 template <typename T>
 __global__ void k_fill(T *__restrict__ a, size_t n, T v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1236,7 +1244,6 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
     void *dropped;
     cm_status s;
     if ((s = scratch(ctx, "mas.dropped", sizeof(unsigned long long), &dropped)) != CM_OK) return s;
-    if ((s = fill(ctx, (unsigned long long *)dropped, 1, 0ull)) != CM_OK) return s;
     bool tiled_ok = d->pos_dtype == CM_F32 && mdt == CM_F32 && tile_edge(d->N) > 0;
     bool use_tiled = d->algo == CM_MAS_TILED || (d->algo == CM_MAS_AUTO && tiled_ok);
     if (use_tiled && !tiled_ok)
@@ -1244,11 +1251,14 @@ cm_status mesh_assign_impl(cm_ctx *ctx, const cm_mas_desc *d, const void *pos, c
     if (use_tiled) {
         void *ovf = overflow_at;
         if (!ovf && (s = scratch(ctx, "mas.overflow", sizeof(unsigned), &ovf)) != CM_OK) return s;
-        if (reset_overflow && (s = fill(ctx, (unsigned *)ovf, 1, 0u)) != CM_OK) return s;
+        if ((s = launch(ctx, "fill", k_clear_flags, dim3(1), dim3(64), 0, (unsigned long long *)dropped,
+                        reset_overflow ? (unsigned *)ovf : (unsigned *)nullptr)) != CM_OK)
+            return s;
         if (overflow_out) *overflow_out = (unsigned *)ovf;
         return mas_tiled(ctx, p, (const float *)pos, (const float *)w, (float *)mesh, (float *)meshB, accumulate,
                          (unsigned long long *)dropped, exact || !overflow_out, (unsigned *)ovf);
     }
+    if ((s = fill(ctx, (unsigned long long *)dropped, 1, 0ull)) != CM_OK) return s;
     if (meshB) {  // plain path: the shifted mesh is a second deposit
         cm_mas_desc db = *d;
         db.interlace_shift += 0.5;

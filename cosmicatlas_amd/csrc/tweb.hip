@@ -524,7 +524,7 @@ static int tweb_seg_len(cm_ctx *ctx, int N, int tiles, int per_cu) {
 
 template <int TY, int TZ, int NT>
 static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol,
-                                int8_t *labels, float *lam, unsigned long long *stats) {
+                                int8_t *labels, float *lam, unsigned long long *stats, bool stats_zeroed) {
     const bool inl = lam != nullptr;
     auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
     const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
@@ -539,7 +539,7 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
     const double ih = 1.0 / (12.0 * (L / (double)N));
     const float ih2 = (float)(ih * ih);
     cm_status s;
-    if ((s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
+    if (!stats_zeroed && (s = launch(ctx, "tweb_zero", k_zero_u64, dim3(1), dim3(64), 0, stats, 5)) != CM_OK) return s;
     return launch(ctx, "tweb_fused", kern, grid, dim3(NT), lds, phi, N, segLen, ih2, lth, tol, labels, lam, stats,
                   (const TwMock *)nullptr, 0);
 }
@@ -588,17 +588,18 @@ cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.
 // Tile (TY x TZ cells, threads): 16x64/512 when 64 | N, else 8x32/256,
 // 8x16/128 or 8x8/64 by the largest of 32, 16, 8 dividing N.
+// stats_zeroed: the counters were cleared by an earlier launch on the stream.
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
-                            float *lam, unsigned long long *stats) {
+                            float *lam, unsigned long long *stats, bool stats_zeroed) {
     int TZ = N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : (N % 16 == 0 ? 16 : (N % 8 == 0 ? 8 : 0)));
     if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 && ctx->tune.tz <= TZ &&
         (ctx->tune.tz == 64 || ctx->tune.tz == 32 || ctx->tune.tz == 16 || ctx->tune.tz == 8))
         TZ = ctx->tune.tz;
     switch (TZ) {
-    case 64: return tweb_fused_run<16, 64, 512>(ctx, N, L, phi, lth, tol, labels, lam, stats);
-    case 32: return tweb_fused_run<8, 32, 256>(ctx, N, L, phi, lth, tol, labels, lam, stats);
-    case 16: return tweb_fused_run<8, 16, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats);
-    case 8: return tweb_fused_run<8, 8, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats);
+    case 64: return tweb_fused_run<16, 64, 512>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 32: return tweb_fused_run<8, 32, 256>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 16: return tweb_fused_run<8, 16, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 8: return tweb_fused_run<8, 8, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     }
     return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
 }
@@ -709,8 +710,9 @@ static cm_status tweb_from_phi_t(cm_ctx *ctx, const cm_tweb_opts *o, const T *ph
                         (const T *)gz, N, ih, o->lambda_th, tk, labels, lam, w)) != CM_OK)
             return s;
     } else {
-        if ((s = tweb_fused_launch(ctx, N, o->L, (const float *)phi, o->lambda_th, tk, labels, (float *)lam, w)) !=
-            CM_OK)
+        // the counters were cleared above
+        if ((s = tweb_fused_launch(ctx, N, o->L, (const float *)phi, o->lambda_th, tk, labels, (float *)lam, w,
+                                   true)) != CM_OK)
             return s;
     }
     if (st) {
