@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
     double acc = 0.0;  // sum of squares of this thread's outputs
     auto dst = [&](int b, int m, float2 z) {
         const float2 o = float2{s2 * z.x, s2 * z.y};
-        orow[b * M + m] = o;
+        store_stream(&orow[b * M + m], o);
         acc += (double)o.x * o.x + (double)o.y * o.y;
     };
     ct_fft<float, M, NB, STRIDE, false, false>(lds, twM, src, dst);
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__res
     auto src = [&](int b, int e) { return b < valid ? data[base + (size_t)e * stride + b] : float2{0.f, 0.f}; };
     auto dst = [&](int b, int e, float2 v) {
         if (phase) v = cmul(v, phase[e]);
-        if (b < valid) data[base + (size_t)e * stride + b] = v;
+        if (b < valid) store_stream(&data[base + (size_t)e * stride + b], v);
     };
     ct_fft<float, N, NB, NB, true, FWD>(lds, tw, src, dst);
 }

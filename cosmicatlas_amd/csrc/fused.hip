@@ -46,9 +46,8 @@ __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict_
                                                               int valid, float2 *lds,
                                                               const float2 *__restrict__ tw) {
     auto srcL = [&](int b, int e) { return lds[e * (NB + 1) + b]; };
-    const auto g = as_global(data);
     auto dstG = [&](int b, int e, float2 v) {
-        if (b < valid) g[base + (size_t)e * xstride + b] = v;
+        if (b < valid) store_stream(&data[base + (size_t)e * xstride + b], v);
     };
     ct_fft<float, N, NB, NB + 1, true, false, E>(lds, tw, srcL, dstG);
 }

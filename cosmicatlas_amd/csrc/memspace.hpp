@@ -27,6 +27,16 @@ template <typename T> __device__ __forceinline__ gmem<T> *as_global(T *p) { retu
 // unsafeAtomicAdd), issued as a global atomic.
 __device__ __forceinline__ void global_add(float *p, float v) { (void)__builtin_amdgcn_global_atomic_fadd_f32(as_global(p), v); }
 
+// Store with the nontemporal hint, for a pass output that the next pass reads
+// only after the whole array has been written (the cache lines would be
+// evicted long before a re-read): measured 2% faster per 512^3 measurement
+// in the y, fused x and c2r passes (slower in the r2c z pass, which keeps
+// plain stores).
+__device__ __forceinline__ void store_stream(float2 *p, float2 v) {
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2f{v.x, v.y}, (gmem<v2f> *)p);
+}
+
 // LDS view of a generic pointer that points into the workgroup's shared
 // memory: ds_ instructions instead of flat ones where the compiler cannot see
 // that a pointer (a parameter of a non-inlined function) addresses LDS.
