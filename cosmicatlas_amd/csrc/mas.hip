@@ -789,7 +789,7 @@ __global__ void __launch_bounds__(512) k_tile_deposit(const uint3 *__restrict__ 
             } else if (accumulate) {
                 *as_global(dst) += v;
             } else {
-                *as_global(dst) = v;
+                store_stream(dst, v);
             }
         }
         __syncthreads();

@@ -30,12 +30,15 @@ __device__ __forceinline__ void global_add(float *p, float v) { (void)__builtin_
 // Store with the nontemporal hint, for a pass output that the next pass reads
 // only after the whole array has been written (the cache lines would be
 // evicted long before a re-read): measured 2% faster per 512^3 measurement
-// in the y, fused x and c2r passes (slower in the r2c z pass, which keeps
-// plain stores).
+// in the y, fused x and c2r passes and the deposit's mesh write-out (slower
+// in the r2c z pass and for the partial-line record stores of the binning,
+// which keep plain stores).
 __device__ __forceinline__ void store_stream(float2 *p, float2 v) {
     typedef float v2f __attribute__((ext_vector_type(2)));
     __builtin_nontemporal_store(v2f{v.x, v.y}, (gmem<v2f> *)p);
 }
+
+__device__ __forceinline__ void store_stream(float *p, float v) { __builtin_nontemporal_store(v, (gmem<float> *)p); }
 
 // LDS view of a generic pointer that points into the workgroup's shared
 // memory: ds_ instructions instead of flat ones where the compiler cannot see
