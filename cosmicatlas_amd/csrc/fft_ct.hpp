@@ -92,6 +92,21 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds_p, const v2<T> *__restrict__
             jj[i] = it % nR;
             bb[i] = it / nR;
         }
+    }
+    // twiddles of this thread's butterflies, all loaded before the stage's
+    // barrier so that their latency overlaps it (loaded one at a time next to
+    // their use, each load waited for the previous one)
+    C w[IT][R];
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int k = jj[i] % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[i][r] = tw[r * k * tws];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int e = jj[i] + r * nR;
@@ -108,9 +123,9 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds_p, const v2<T> *__restrict__
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                C w = tw[r * k * tws];
-                if (!FWD) w.y = -w.y;
-                v[i * R + r] = cmul(v[i * R + r], w);
+                C wr = w[i][r];
+                if (!FWD) wr.y = -wr.y;
+                v[i * R + r] = cmul(v[i * R + r], wr);
             }
         }
         Bfly<R, FWD, C>::run(&v[i * R]);
