@@ -264,6 +264,7 @@ cm_status cm_ctx_create(int device, size_t hbm_budget, cm_ctx **out) {
                 int v = atoi(kv.c_str() + eq + 1);
                 if (k == "xnb") c->tune.xnb = v;
                 else if (k == "xe") c->tune.xe = v;
+                else if (k == "xtw") c->tune.xtw = v;
                 else if (k == "colnb") c->tune.colnb = v;
                 else if (k == "zrows") c->tune.zrows = v;
                 else if (k == "ty") c->tune.ty = v;

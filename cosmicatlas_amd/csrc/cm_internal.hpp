@@ -68,6 +68,7 @@ struct cm_ctx {
     struct Tune {
         int xnb = 0;    // kz columns per workgroup in the fused x pass
         int xe = 0;     // elements per thread in the fused x pass (8: twice the threads)
+        int xtw = 0;    // fused x pass twiddles: 0 in LDS when the occupancy allows it, 2 never
         int colnb = 0;  // kz columns per workgroup in the y/x column passes
         int zrows = 0;  // rows per workgroup in the z passes
         int ty = 0, tz = 0, tseg = 0;  // T-web tile and x segments

@@ -67,14 +67,24 @@ constexpr int ct_row_stride(int M) { return M + 1 + ((M + 1) >> 3) + 1; }
 // per-element rows on distinct banks); rows: b*STRIDE + rpad(e).
 // The first stage reads through src(b, e), the last writes through
 // dst(b, e, v); the caller owns the barrier before the first stage.
-template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int S, int Ns, int E, class Src, class Dst>
+// TWL: the twiddle table is in LDS (else global memory).
+template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int S, int Ns, int E, bool TWL, class Src,
+          class Dst>
 __device__ __forceinline__ void ct_stage(v2<T> *lds_p, const v2<T> *__restrict__ tw_p, const Src &src,
                                          const Dst &dst) {
     using C = v2<T>;
     // LDS and global views: the pointers may arrive through a non-inlined
     // function, where they would otherwise be accessed with flat instructions
     const auto lds = as_local(lds_p);
-    const auto tw = as_global(tw_p);
+    auto twv = [&](int i) -> C {
+        if constexpr (TWL) {
+            const C x = as_local(tw_p)[i];
+            return x;
+        } else {
+            const C x = as_global(tw_p)[i];
+            return x;
+        }
+    };
     constexpr int R = ct_radix(n, S), NST = ct_nst(n);
     constexpr bool FIRST = S == 0, LAST = S == NST - 1;
     constexpr int IT = E / R, nR = n / R, NT = NB * n / E, tws = n / (Ns * R);
@@ -102,7 +112,7 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds_p, const v2<T> *__restrict__
         for (int i = 0; i < IT; ++i) {
             const int k = jj[i] % Ns;
 #pragma unroll
-            for (int r = 1; r < R; ++r) w[i][r] = tw[r * k * tws];
+            for (int r = 1; r < R; ++r) w[i][r] = twv(r * k * tws);
         }
     }
 #pragma unroll
@@ -141,15 +151,16 @@ __device__ __forceinline__ void ct_stage(v2<T> *lds_p, const v2<T> *__restrict__
     }
     if constexpr (!LAST) {
         __syncthreads();
-        ct_stage<T, n, NB, STRIDE, COLS, FWD, S + 1, Ns * R, E>(lds_p, tw_p, src, dst);
+        ct_stage<T, n, NB, STRIDE, COLS, FWD, S + 1, Ns * R, E, TWL>(lds_p, tw_p, src, dst);
     }
 }
 
 // E: elements per thread (ct_E(n) by default; any multiple of every radix of
-// the plan), so the workgroup has NB*n/E threads.
-template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int E = ct_E(n), class Src, class Dst>
+// the plan), so the workgroup has NB*n/E threads.  TWL: tw points into LDS.
+template <typename T, int n, int NB, int STRIDE, bool COLS, bool FWD, int E = ct_E(n), bool TWL = false, class Src,
+          class Dst>
 __device__ __forceinline__ void ct_fft(v2<T> *lds, const v2<T> *__restrict__ tw, const Src &src, const Dst &dst) {
-    ct_stage<T, n, NB, STRIDE, COLS, FWD, 0, 1, E>(lds, tw, src, dst);
+    ct_stage<T, n, NB, STRIDE, COLS, FWD, 0, 1, E, TWL>(lds, tw, src, dst);
 }
 
 // Workgroup shapes of the three passes for a float32 N^3 transform.

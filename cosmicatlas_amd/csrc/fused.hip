@@ -27,7 +27,7 @@ namespace cm {
 // The two FFT phases of a tile live in separate non-inlined functions so that
 // each gets its own register allocation (inlined together with the shell code
 // they exceed the VGPR budget and spill).
-template <int N, int NB, int E>
+template <int N, int NB, int E, bool TWL>
 __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restrict__ data, size_t base,
                                                             size_t xstride, int valid, float2 *lds,
                                                             const float2 *__restrict__ tw) {
@@ -38,10 +38,10 @@ __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restr
         return v;
     };
     auto dstL = [&](int b, int e, float2 v) { lds[e * (NB + 1) + b] = v; };
-    ct_fft<float, N, NB, NB + 1, true, true, E>(lds, tw, srcG, dstL);
+    ct_fft<float, N, NB, NB + 1, true, true, E, TWL>(lds, tw, srcG, dstL);
 }
 
-template <int N, int NB, int E>
+template <int N, int NB, int E, bool TWL>
 __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict__ data, size_t base, size_t xstride,
                                                               int valid, float2 *lds,
                                                               const float2 *__restrict__ tw) {
@@ -49,7 +49,7 @@ __device__ __attribute__((noinline)) void x_fft_inv_from_lds(float2 *__restrict_
     auto dstG = [&](int b, int e, float2 v) {
         if (b < valid) store_stream(&data[base + (size_t)e * xstride + b], v);
     };
-    ct_fft<float, N, NB, NB + 1, true, false, E>(lds, tw, srcL, dstG);
+    ct_fft<float, N, NB, NB + 1, true, false, E, TWL>(lds, tw, srcL, dstG);
 }
 
 // Shell accumulator of one thread: the modes a thread visits along kz have
@@ -72,7 +72,10 @@ struct ShellAcc {
     }
 };
 
-template <int N, int NB, bool IL, int E>
+// TWL: the twiddle table is copied into LDS once per workgroup (a stage's
+// twiddle loads then wait on LDS, not on the L2), when that leaves the number
+// of resident workgroups unchanged.
+template <int N, int NB, bool IL, int E, bool TWL>
 __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__restrict__ data,
                                                  const float2 *__restrict__ dataB, int pitch,
                                                  const float2 *__restrict__ tw, PkK pk,
@@ -87,21 +90,24 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
     constexpr int M1 = N / 2 + 1;
     const int ns = pk.nshells;
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
-    double *s_acc = reinterpret_cast<double *>(lds + (size_t)(NB + 1) * N);  // 4*ns
-    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);           // ns
-    float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                      // M1
-    float *s_sq = s_ic + M1;                                                  // M1: C1 (aliasing sum)
-    float *s_al = s_sq + M1;                                                  // M1: alternating sum
-    float *s_g = s_al + M1;                                                   // N
+    float2 *s_tw = lds + (size_t)(NB + 1) * N;                                   // N (TWL)
+    double *s_acc = reinterpret_cast<double *>(s_tw + (TWL ? N : 0));           // 4*ns
+    unsigned *s_cnt = reinterpret_cast<unsigned *>(s_acc + 4 * ns);             // ns
+    float *s_ic = reinterpret_cast<float *>(s_cnt + ns);                        // M1
+    float *s_sq = s_ic + M1;                                                    // M1: C1 (aliasing sum)
+    float *s_al = s_sq + M1;                                                    // M1: alternating sum (IL)
+    const auto g_gauss = as_global(gauss);  // Gaussian factor per index (two reads per row)
     constexpr int NT = ColShape<N, NB, E>::NT;
     const int ncols = N / 2 + 1;
     const int tid = threadIdx.x;
     for (int i = tid; i < M1; i += NT) {
         s_ic[i] = (float)tabs[i];
         s_sq[i] = (float)tabs[M1 + i];
-        s_al[i] = (float)tabs[2 * M1 + i];
+        if constexpr (IL) s_al[i] = (float)tabs[2 * M1 + i];
     }
-    for (int i = tid; i < N; i += NT) s_g[i] = gauss[i];
+    if constexpr (TWL)
+        for (int i = tid; i < N; i += NT) s_tw[i] = tw[i];
+    const float2 *twp = TWL ? s_tw : tw;
     for (int i = tid; i < 4 * ns; i += NT) s_acc[i] = 0.0;
     for (int i = tid; i < ns; i += NT) s_cnt[i] = 0u;
     const double n3 = (double)N * N * N;
@@ -117,7 +123,7 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
         const int kz0 = kzb * NB;
         const size_t base = (size_t)iy * pitch + kz0;
         const int valid = min(NB, ncols - kz0);
-        x_fft_fwd_to_lds<N, NB, E>(data, base, xstride, valid, lds, tw);
+        x_fft_fwd_to_lds<N, NB, E, TWL>(data, base, xstride, valid, lds, twp);
         __syncthreads();
         if constexpr (IL) {
             // interlaced meshes: average with the second mesh's finished
@@ -137,7 +143,7 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
         const int ky = iy <= h ? iy : iy - N;
         const bool yl = iy < h;
         const int fy = yl ? iy : N - iy;
-        const float gy = s_g[iy];
+        const float gy = g_gauss[iy];
         for (int ix = tid; ix < N; ix += NT) {
             const bool xl = ix < h;
             const int fx = xl ? ix : N - ix;
@@ -149,10 +155,11 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
             // shot-noise weight per mode: (wc prod C1 + wa prod A) / W^2
             // (wa = 0 for one mesh; wc = wa = 1/2 for interlaced meshes)
             const float icxy = s_ic[fx] * s_ic[fy];
-            const float sqxy = wc * s_sq[fx] * s_sq[fy] * icxy, alxy = wa * s_al[fx] * s_al[fy] * icxy;
+            const float sqxy = wc * s_sq[fx] * s_sq[fy] * icxy,
+                        alxy = IL ? wa * s_al[fx] * s_al[fy] * icxy : 0.0f;
             const int kx = xl ? ix : ix - N;
             const int q2xy = kx * kx + ky * ky;
-            const float gxy = fscale * s_g[ix] * gy;
+            const float gxy = fscale * g_gauss[ix] * gy;
             float2 *row = lds + ix * (NB + 1);
             ShellAcc acc{-1, 0.f, 0.f, 0.f, 0.f, 0u};
             // integer shell rule: raw index r with (r ndel + Dmin)^2 <= q < hiq;
@@ -229,7 +236,7 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
             acc.flush(s_acc, s_cnt, ns);
         }
         __syncthreads();
-        if (do_tweb) x_fft_inv_from_lds<N, NB, E>(data, base, xstride, valid, lds, tw);
+        if (do_tweb) x_fft_inv_from_lds<N, NB, E, TWL>(data, base, xstride, valid, lds, twp);
         __syncthreads();
     }
     for (int i = tid; i < ns; i += NT) {
@@ -284,13 +291,25 @@ static cm_status x_fused_run(cm_ctx *ctx, float2 *dk, const float2 *dkB, int pit
                              unsigned long long *dcnt, int do_tweb, float inv_kf2, const float *dgauss,
                              const double *dnobj, int ns, const float4 *ztab) {
     constexpr int M1 = N / 2 + 1, NT = ColShape<N, NB, E>::NT;
-    const size_t lds = sizeof(float2) * (size_t)(NB + 1) * N + sizeof(double) * 4 * (size_t)ns +
-                       sizeof(unsigned) * (size_t)ns + sizeof(float) * (3 * (size_t)M1 + (size_t)N);
+    size_t lds = sizeof(float2) * (size_t)(NB + 1) * N + sizeof(double) * 4 * (size_t)ns +
+                 sizeof(unsigned) * (size_t)ns + sizeof(float) * ((dkB ? 3 : 2) * (size_t)M1);
     if (lds > ctx->lds_per_block)
         return fail(ctx, CM_ERR_UNSUPPORTED, "fused x pass needs %zu bytes of LDS (N=%d, %d shells)", lds, N, ns);
-    auto kern = dkB ? k_x_fused<N, NB, true, E> : k_x_fused<N, NB, false, E>;
+    auto kern = dkB ? k_x_fused<N, NB, true, E, false> : k_x_fused<N, NB, false, E, false>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    // twiddles in LDS when the workgroups per CU stay the same
+    {
+        const size_t lds_tw = lds + sizeof(float2) * (size_t)N;
+        auto kern_tw = dkB ? k_x_fused<N, NB, true, E, true> : k_x_fused<N, NB, false, E, true>;
+        int per_cu_tw = 0;
+        if (ctx->tune.xtw != 2 && lds_tw <= ctx->lds_per_block &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_tw, kern_tw, NT, lds_tw) == hipSuccess &&
+            per_cu_tw >= per_cu) {
+            kern = kern_tw;
+            lds = lds_tw;
+        }
+    }
     const int ntiles = ((M1 + NB - 1) / NB) * N;
     const unsigned grid = (unsigned)std::min(ntiles, std::max(1, ctx->cu_count) * per_cu);
     const float wc = dkB ? 0.5f : 1.0f, wa = dkB ? 0.5f : 0.0f;
