@@ -510,8 +510,10 @@ def rows(cm):
     t = _timed(fkp, 5)
     kt = _kernels(ctx, fkp)
     dom = max(kt.items(), key=lambda kv: kv[1]["mean_ms"] * kv[1]["launches_per_call"])[0]
-    n_avg = (NL ** 3 + nr) / 2.0  # the MAS kernels run once on the data and once on the randoms
-    ab = algorithmic_bytes(dom, n_avg, 1.42 * n_avg)
+    # a MAS kernel runs once on the data and once per x-slab group of the
+    # randoms: algorithmic bytes of the mean particle count per launch
+    n_launch = (NL ** 3 + nr) / max(1.0, kt[dom]["launches_per_call"])
+    ab = algorithmic_bytes(dom, n_launch, 1.42 * n_launch)
     # CPU: the oracle's serial MAS, FFT and shells on a 128^3 sample with 1/64 of both catalogues
     Ns = 128
     sub = posd[:: 64].cpu().numpy()
@@ -533,8 +535,9 @@ def rows(cm):
         "value": round(1.0 / t, 3), "unit": "measurements/s", "ms_per_call": round(1e3 * t, 4),
         "kernels_ms_per_call": {k: round(v["mean_ms"] * v["launches_per_call"], 4) for k, v in kt.items()},
         "roofline": _hbm_roof(dom, kt, ab) if ab else None,
-        "roofline_note": "per launch of the dominant kernel at the mean catalogue size (data and randoms launch once "
-                         "each), 1.42 records per particle for TSC on 16^3 tiles",
+        "roofline_note": "per launch of the dominant kernel (the data and each x-slab group of the randoms launch "
+                         "it once) at the mean particle count per launch, 1.42 records per particle for TSC on 16^3 "
+                         "tiles",
         "cpu_baseline": {"value": round(1.0 / tcpu, 5), "unit": "measurements/s", "cores": 1, "kind": "port",
                          "sample": f"the oracle's serial MAS of both catalogues, r2c and shells at {Ns}^3 with 1/64 "
                                    f"of the particles, {tcs:.2f} s, scaled by the larger of the particle (x64) and "
