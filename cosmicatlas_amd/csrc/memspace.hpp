@@ -45,4 +45,13 @@ __device__ __forceinline__ void store_stream(float *p, float v) { __builtin_nont
 // that a pointer (a parameter of a non-inlined function) addresses LDS.
 template <typename T> __device__ __forceinline__ lmem<T> *as_local(T *p) { return (lmem<T> *)p; }
 
+// Raw buffer resource over the n bytes at p (word 3: 32-bit data format, as
+// gfx9 raw buffers take it).  Buffer instructions take a 32-bit per-lane
+// offset beside the uniform base, so an array of fewer than 4 GiB is addressed
+// without 64-bit address arithmetic per lane; an access at or beyond n reads
+// zero and a store there is dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, unsigned n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)n, 0x00020000);
+}
+
 }  // namespace cm

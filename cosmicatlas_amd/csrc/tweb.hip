@@ -217,27 +217,32 @@ __global__ void __launch_bounds__(1024) k_rms_final(const double *__restrict__ p
 // Fused FD Hessian + labels.  One workgroup owns a TY x TZ column of cells in
 // (y,z) and a segment of x planes and marches along x, one barrier per plane:
 //   step s: phi plane s enters the LDS ring; barrier; G_a = D_a phi of plane
-//   s-2 (halo 2 in y,z for G_y, G_z) from ring planes s-3..s and the thread's
-//   own plane s-4 values (kept in registers from the step before), G_y and
-//   G_z written to the G slot of plane s-2; T of plane c = s-4 from G_x, G_y,
-//   G_z of planes c-2..c+2 (x derivative, register histories) and the G slot
-//   of plane c (y, z derivatives, written two steps earlier).
+//   s-2 (halo 2 in y,z for G_y, G_z) from ring planes s-2..s and the centre
+//   values of planes s-4, s-3 (kept in registers from the two steps before),
+//   G_y and G_z written to the G slot of plane s-2; T of plane c = s-4 from
+//   G_x, G_y, G_z of planes c-2..c+2 (x derivative, register histories) and
+//   the G slot of plane c (y, z derivatives, written two steps earlier).
 // Five phi slots (plane s-4 is no longer read from LDS in step s, so plane
 // s+1 may replace it before the next barrier) and three G slots make every
-// LDS slot's next write follow the barrier after its last read.  Each thread owns IP interior cells and keeps
-// their last five G_x, G_y, G_z values in registers; the march is unrolled by
-// five so the history slots are compile-time indices.  The next phi plane is
-// loaded into registers while the current one is processed.  G and T never
-// touch HBM.  Composition order of the reference: G_a = D_a phi, then D_x G_x,
-// D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z (Miscelanious.cpp:1684-1700).
+// LDS slot's next write follow the barrier after its last read.  Each thread
+// owns two z-adjacent cells (z even, z+1), so every LDS access is one aligned
+// 8-byte pair and the z stencils of the two cells share their reads; it keeps
+// the pair's last five G_x, G_y, G_z values in registers, and the march is
+// unrolled by five so the history slots are compile-time indices.  The next
+// phi plane is loaded into registers while the current one is processed.  G
+// and T never touch HBM.  Composition order of the reference: G_a = D_a phi,
+// then D_x G_x, D_x G_y, D_x G_z, D_y G_y, D_y G_z, D_z G_z
+// (Miscelanious.cpp:1684-1700).
 // Labels from the inertia of T - lambda_th I: float signs with a certified
 // rounding bound, recounted in FP64 in the kernel where the float signs are
 // not certain.  The same coefficients, shifted by -+ tol, give the sign
 // counts of T - (lambda_th -+ tol) I: the cell is marginal when they differ
 // (an eigenvalue lies within tol of lambda_th), decided in FP64 when a shifted
-// sign is not certain either.  The marching loop makes no calls except for the
-// lambda output (INL = true), which takes the out-of-line FP64 branch for
-// every cell (a call site inside the loop costs ~35 VGPRs of saved registers).
+// sign is not certain either.  Class and margin counts are kept per wave from
+// the sign masks of the wave's cells (scalar population counts).  The
+// marching loop makes no calls except for the lambda output (INL = true),
+// which takes the out-of-line FP64 branch for every cell (a call site inside
+// the loop costs ~35 VGPRs of saved registers).
 
 // Lambda output of the fused kernel (kept out of line so its FP64 registers do
 // not count against the marching loop): closed-form FP64 eigenvalues of T =
@@ -257,9 +262,44 @@ __device__ __attribute__((noinline)) int positive_eigs_band_d(float b00, float b
     return positive_eigs_band64(b00, b11, b22, a10, a20, a21, tol, band);
 }
 
-template <int TY, int TZ>
+// LDS of the fused T-web: five phi slots, each padded to a whole number of
+// the workgroup's loads ((TY+8)(TZ+8) rounded up to a multiple of NT), and
+// three G_y and three G_z planes of (TY+4)(TZ+4).
+template <int TY, int TZ, int NT>
 constexpr size_t tweb_lds_floats() {
-    return 5 * (size_t)(TY + 8) * (TZ + 8) + 6 * (size_t)(TY + 4) * (TZ + 4);
+    return 5 * (((size_t)(TY + 8) * (TZ + 8) + NT - 1) / NT * NT) + 6 * (size_t)(TY + 4) * (TZ + 4);
+}
+
+// An LDS address the compiler treats as opaque: the reads through it take it
+// as their own base register with small immediate offsets.  Without it every
+// ring slot's reads share one base, and the compiler pairs reads whose
+// offsets do not fit the paired instruction by adding a new base per pair.
+__device__ __forceinline__ const lmem<float> *lds_base(const lmem<float> *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(p));
+#endif
+    return p;
+}
+
+// Per lane: the sum of the lane's bits in three lane masks of the wave (a
+// select and two adds with carry-in from the masks; written out so that the
+// compiler keeps the masks for the scalar counts instead of re-deriving them
+// from the sum).
+__device__ __forceinline__ unsigned lane_bits3(unsigned long long a, unsigned long long b, unsigned long long c) {
+    unsigned r = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("v_cndmask_b32_e64 %0, 0, 1, %1\n\t"
+        "v_addc_co_u32_e64 %0, vcc, 0, %0, %2\n\t"
+        "v_addc_co_u32_e64 %0, vcc, 0, %0, %3"
+        : "=&v"(r)
+        : "s"(a), "s"(b), "s"(c)
+        : "vcc");
+#endif
+    return r;
+}
+
+__device__ __forceinline__ float2 lds_pair(const lmem<float> *p, int off) {
+    return *(const lmem<float2> *)(p + off);
 }
 
 template <int TY, int TZ, int NT, bool INL>
@@ -269,14 +309,15 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
                                                    const TwMock *__restrict__ tm, int nm) {
     constexpr int HY = TY + 8, HZ = TZ + 8, HP = HY * HZ;  // haloed phi plane (halo 4)
     constexpr int GZ = TZ + 4, GP = (TY + 4) * GZ;         // G plane (halo 2)
-    constexpr int IP = TY * TZ / NT;                       // interior cells per thread
+    constexpr int TZ2 = TZ / 2;                            // cell pairs per row
     constexpr int PF = (HP + NT - 1) / NT;                 // phi values per thread per plane
-    constexpr int NH1 = 4 * TZ, NH2 = 4 * TY;              // halo positions: y rows, z columns
-    static_assert(IP * NT == TY * TZ && NH1 + NH2 <= NT, "tile does not match the workgroup");
-    extern __shared__ float sm[];
-    float *ring = sm;             // 5 phi planes
-    float *sgy = ring + 5 * HP;   // 3 G_y planes
-    float *sgz = sgy + 3 * GP;    // 3 G_z planes
+    constexpr int HPP = PF * NT;                           // ring slot stride (padded)
+    constexpr int NH1 = 4 * TZ2, NH2 = 2 * TY;             // halo pairs: y rows, z column pairs
+    static_assert(TZ % 2 == 0 && TY * TZ == 2 * NT && NH1 + NH2 <= NT, "tile does not match the workgroup");
+    extern __shared__ __attribute__((aligned(16))) float tw_sm[];
+    const lmem<float> *ring = as_local(tw_sm);  // 5 phi planes
+    lmem<float> *sgy = as_local(tw_sm) + 5 * HPP;  // 3 G_y planes
+    lmem<float> *sgz = sgy + 3 * GP;              // 3 G_z planes
     __shared__ unsigned scnt[5];
     const int tid = threadIdx.x;
     if (tid < 5) scnt[tid] = 0;
@@ -312,158 +353,208 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     const size_t NN = (size_t)N, n3 = NN * NN * NN;
     // T' = T/ih^2: threshold and margin scaled to match
     const double tol = tol_value(tk);
-    const float lthf = (float)(lth / (double)ih2);
-    const float tols = (float)(tol / (double)ih2);
+    // (uniform: the tolerance may come through a generic pointer, whose load
+    // the compiler takes as per-lane, and a per-lane tolerance would turn the
+    // band test's branch and the wave masks into per-lane code)
+    const float lthf = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)(lth / (double)ih2))));
+    const float tols = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)(tol / (double)ih2))));
 
-    // this thread's phi loads: in-plane offsets with periodic wrap in y and z
-    int foff[PF];
+    // this thread's phi loads: byte offsets in a plane (N^2 floats, below
+    // 4 GiB for every supported N) with periodic wrap in y and z; the slot's
+    // padding positions take an offset beyond the plane, which the buffer
+    // load's range check reads as zero
+    unsigned foff[PF];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
         const int q = tid + k * NT;
         const int yy = q / HZ, zz = q - (q / HZ) * HZ;
-        foff[k] = q < HP ? ((ty0 - 4 + yy + N) % N) * N + (tz0 - 4 + zz + N) % N : 0;
+        foff[k] = q < HP ? 4u * (unsigned)(((ty0 - 4 + yy + N) % N) * N + (tz0 - 4 + zz + N) % N) : 0xfffffff0u;
     }
-    // interior cells: phi-plane index, G-plane index, in-plane output offset.
-    // The stencil reads of a cell address the LDS from one base per cell
-    // (pb = its phi index - 2 HZ - 2, gb = its G index - 2 GZ - 2) plus
-    // non-negative compile-time offsets, which the LDS instructions encode as
-    // immediates: no per-read address arithmetic.
-    int po[IP], gq[IP], coff[IP];
-#pragma unroll
-    for (int k = 0; k < IP; ++k) {
-        const int i = tid + k * NT, ia = i / TZ, ib = i % TZ;
-        po[k] = (ia + 4) * HZ + ib + 4;
-        gq[k] = (ia + 2) * GZ + ib + 2;
-        coff[k] = (ty0 + ia) * N + tz0 + ib;
-    }
+    // the cell pair: row ia, columns ib and ib+1 of the tile.  rb[j] points
+    // at (row - 2, column - 2) of the pair in ring slot j: every read of a
+    // slot lies within 1.2 KB above it.
+    const int ia = tid / TZ2, ib = 2 * (tid - (tid / TZ2) * TZ2);
     constexpr int PB = 2 * HZ + 2, GB = 2 * GZ + 2;  // base offsets below a cell's own index
-    // at most one halo position per thread: kind 1 = y-halo row (G_y, G_z),
-    // kind 2 = z-halo column (G_z)
+    const int po = (ia + 4) * HZ + ib + 4, gq = (ia + 2) * GZ + ib + 2;
+    const unsigned coff = (unsigned)((ty0 + ia) * N + tz0 + ib);  // output offset in a plane
+    const lmem<float> *rb[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) rb[j] = lds_base(ring + j * HPP + (po - PB));
+    // at most one halo pair per thread: kind 1 = y-halo row (G_y, G_z),
+    // kind 2 = z-halo column pair (G_z)
     int hkind = 0, hq = 0, hpo = 0;
     if (tid < NH1) {
-        const int r = tid / TZ, a = r < 2 ? r : TY + r, b = 2 + tid % TZ;
+        const int r = tid / TZ2, a = r < 2 ? r : TY + r, b = 2 + 2 * (tid - r * TZ2);
         hkind = 1;
         hq = a * GZ + b;
         hpo = (a + 2) * HZ + b + 2;
     } else if (tid < NH1 + NH2) {
-        const int j = tid - NH1, r = j & 3, a = 2 + (j >> 2), b = r < 2 ? r : TZ + r;
+        const int j = tid - NH1, a = 2 + (j >> 1), b = (j & 1) ? TZ + 2 : 0;
         hkind = 2;
         hq = a * GZ + b;
         hpo = (a + 2) * HZ + b + 2;
     }
 
-    float hx[IP][5], hy[IP][5], hz[IP][5], p3[IP][5];
+    // histories of the pair: G_x, G_y, G_z of the last five planes, phi at
+    // the cells of the last five planes
+    float hx[2][5], hy[2][5], hz[2][5], pc[2][5];
 #pragma unroll
-    for (int k = 0; k < IP; ++k)
+    for (int k = 0; k < 2; ++k)
 #pragma unroll
-        for (int i = 0; i < 5; ++i) hx[k][i] = hy[k][i] = hz[k][i] = p3[k][i] = 0.f;
-    unsigned long long cls = 0;  // class counts, 16 bits each (at most 2048 cells per thread)
-    unsigned nmarg = 0;
+        for (int i = 0; i < 5; ++i) hx[k][i] = hy[k][i] = hz[k][i] = pc[k][i] = 0.f;
+    // per-wave counts (uniform): cells with 3 and with at least one positive
+    // eigenvalue, with an odd count, marginal cells
+    unsigned wn3 = 0, wany = 0, wodd = 0, wmarg = 0;
     float pf[PF];
-    auto fetch = [&](int s) {
-        const int sp = (s % N + N) % N;
-        const auto plane = as_global(phi) + (size_t)sp * NN * NN;
+    // planes are fetched in order x0-4, x0-3, ... (periodic in x): fp is the
+    // next one's index, fptr its first value
+    const unsigned plane_bytes = (unsigned)(NN * NN * sizeof(float));
+    int fp = ((x0 - 4) % N + N) % N;
+    const float *fptr = phi + (size_t)fp * NN * NN;
+    auto fetch = [&]() {
+        const auto rs = buffer_rsrc(fptr, plane_bytes);
 #pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (tid + k * NT < HP) pf[k] = plane[foff[k]];
+        for (int k = 0; k < PF; ++k) pf[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, foff[k], 0, 0));
+        if (++fp == N) {
+            fp = 0;
+            fptr = phi;
+        } else {
+            fptr += NN * NN;
+        }
     };
-    // G of plane p (p >= x0 - 2) in G slot (p - x0 + 3) % 3
-    auto gslot = [&](int p) { return (p - x0 + 3) % 3 * GP; };
+    // G of plane p (p >= x0 - 2) in G slot (p - x0 + 3) mod 3; gk is the slot
+    // of plane s-2 in step s (0 in the first step, s = x0-4)
+    int gk = 0;
+    // labels of plane c, the next output plane
+    int8_t *lptr = labels + (size_t)x0 * NN * NN;
 
     // Step s (J = (s - x0 + 4) mod 5): plane s enters ring slot (J+4)%5 (the
-    // slot of plane s-5); G at plane s-2 goes to history slot J; cells of
-    // plane c = s-4 are output, with G(c-2..c+2) in history slots (J+1)%5,
-    // (J+2)%5, (J+3)%5, (J+4)%5, J.  p3[k][J] keeps plane s-3 at the thread's
-    // cell for step s+1 (where it is plane s-4).
+    // slot of plane s-5); phi of plane s-2 at the pair and G at plane s-2 go
+    // to history slot J; cells of plane c = s-4 are output, with G(c-2..c+2)
+    // in history slots (J+1)%5, (J+2)%5, (J+3)%5, (J+4)%5, J.
     auto step = [&](auto Jc, int s) {
         constexpr int J = decltype(Jc)::value;
-        float *cur = ring + ((J + 4) % 5) * HP;
+        constexpr int S2 = (J + 2) % 5, S1 = (J + 3) % 5, S0 = (J + 4) % 5;
+        lmem<float> *cur = as_local(tw_sm) + S0 * HPP;
 #pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (tid + k * NT < HP) cur[tid + k * NT] = pf[k];
-        if (s + 1 < x1 + 4) fetch(s + 1);  // in flight while plane s is processed
+        for (int k = 0; k < PF; ++k) cur[tid + k * NT] = pf[k];
+        if (s + 1 < x1 + 4) fetch();  // plane s+1, in flight while plane s is processed
+        // G slots of planes s-2 (written) and s-4 (read)
+        const int gw = gk * GP, gs = (gk == 2 ? 0 : gk + 1) * GP;
+        gk = gk == 2 ? 0 : gk + 1;
         __syncthreads();
-        constexpr int S3 = ((J + 1) % 5) * HP, S2 = ((J + 2) % 5) * HP, S1 = ((J + 3) % 5) * HP,
-                      S0 = ((J + 4) % 5) * HP;
-        const float *pm2 = ring + S2;
         // registers are written unconditionally (before x0 the values come from
         // a partly filled ring and are never read), so no history value has to
         // survive a skipped write
-#pragma unroll
-        for (int k = 0; k < IP; ++k) {
-            const float *b = ring + (po[k] - PB);  // loop-invariant per cell
-            p3[k][J] = b[S3 + PB];
-            hx[k][J] = fd4s(p3[k][(J + 4) % 5], p3[k][J], b[S1 + PB], b[S0 + PB]);
-            hy[k][J] = fd4s(b[S2 + 2], b[S2 + HZ + 2], b[S2 + 3 * HZ + 2], b[S2 + 4 * HZ + 2]);
-            hz[k][J] = fd4s(b[S2 + 2 * HZ], b[S2 + 2 * HZ + 1], b[S2 + 2 * HZ + 3], b[S2 + 2 * HZ + 4]);
+        {
+            const lmem<float> *b = rb[S2];
+            const float2 y0 = lds_pair(b, 2), y1 = lds_pair(b, HZ + 2), y3 = lds_pair(b, 3 * HZ + 2),
+                         y4 = lds_pair(b, 4 * HZ + 2);
+            const float2 z0 = lds_pair(b, 2 * HZ), zc = lds_pair(b, 2 * HZ + 2), z2 = lds_pair(b, 2 * HZ + 4);
+            const float2 c1 = lds_pair(rb[S1], 2 * HZ + 2), c0 = lds_pair(rb[S0], 2 * HZ + 2);
+            // phi of planes s-4 and s-3 at the pair: read in the two steps before
+            hx[0][J] = fd4s(pc[0][(J + 3) % 5], pc[0][(J + 4) % 5], c1.x, c0.x);
+            hx[1][J] = fd4s(pc[1][(J + 3) % 5], pc[1][(J + 4) % 5], c1.y, c0.y);
+            pc[0][J] = zc.x;
+            pc[1][J] = zc.y;
+            hy[0][J] = fd4s(y0.x, y1.x, y3.x, y4.x);
+            hy[1][J] = fd4s(y0.y, y1.y, y3.y, y4.y);
+            hz[0][J] = fd4s(z0.x, z0.y, zc.y, z2.x);
+            hz[1][J] = fd4s(z0.y, zc.x, z2.x, z2.y);
         }
         if (s >= x0) {
-            float *gyw = sgy + gslot(s - 2), *gzw = sgz + gslot(s - 2);
-#pragma unroll
-            for (int k = 0; k < IP; ++k) {
-                gyw[gq[k]] = hy[k][J];
-                gzw[gq[k]] = hz[k][J];
-            }
+            *(lmem<float2> *)(sgy + gw + gq) = make_float2(hy[0][J], hy[1][J]);
+            *(lmem<float2> *)(sgz + gw + gq) = make_float2(hz[0][J], hz[1][J]);
             if (hkind) {
-                const float *b = pm2 + (hpo - PB);
-                if (hkind == 1) gyw[hq] = fd4s(b[2], b[HZ + 2], b[3 * HZ + 2], b[4 * HZ + 2]);
-                gzw[hq] = fd4s(b[2 * HZ], b[2 * HZ + 1], b[2 * HZ + 3], b[2 * HZ + 4]);
+                const lmem<float> *b = lds_base(ring + S2 * HPP + (hpo - PB));
+                const float2 z0 = lds_pair(b, 2 * HZ), zc = lds_pair(b, 2 * HZ + 2), z2 = lds_pair(b, 2 * HZ + 4);
+                *(lmem<float2> *)(sgz + gw + hq) =
+                    make_float2(fd4s(z0.x, z0.y, zc.y, z2.x), fd4s(z0.y, zc.x, z2.x, z2.y));
+                if (hkind == 1) {
+                    const float2 y0 = lds_pair(b, 2), y1 = lds_pair(b, HZ + 2), y3 = lds_pair(b, 3 * HZ + 2),
+                                 y4 = lds_pair(b, 4 * HZ + 2);
+                    *(lmem<float2> *)(sgy + gw + hq) =
+                        make_float2(fd4s(y0.x, y1.x, y3.x, y4.x), fd4s(y0.y, y1.y, y3.y, y4.y));
+                }
             }
         }
         const int c = s - 4;
         if (c < x0) return;
-        const int gs = gslot(c);  // uniform
-        const auto lplane = as_global(labels) + (size_t)c * NN * NN;
-        // T of cell k: txx, tyy, tzz, a10 = D_x G_y, a20 = D_x G_z, a21 = D_y G_z;
-        // G_y at gy[gs + i], G_z at gy[gs + 3 GP + i]
-        constexpr int ZO = 3 * GP;
-        auto tensor = [&](int k, float t[6]) {
-            const float *g = sgy + gs + (gq[k] - GB);
-            t[0] = fd4s(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J]);
-            t[1] = fd4s(g[2], g[GZ + 2], g[3 * GZ + 2], g[4 * GZ + 2]);
-            t[2] = fd4s(g[ZO + 2 * GZ], g[ZO + 2 * GZ + 1], g[ZO + 2 * GZ + 3], g[ZO + 2 * GZ + 4]);
-            t[3] = fd4s(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J]);
-            t[4] = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
-            t[5] = fd4s(g[ZO + 2], g[ZO + GZ + 2], g[ZO + 3 * GZ + 2], g[ZO + 4 * GZ + 2]);
+        const lmem<float> *gyb = lds_base(sgy + gs + (gq - GB)), *gzb = lds_base(sgz + gs + (gq - GB));
+        constexpr int JC = (J + 3) % 5;  // history slot of plane c
+        // T of the pair: txx, tyy, tzz, a10 = D_x G_y, a20 = D_x G_z,
+        // a21 = D_y G_z; the z stencil of G_z takes the pair's own values of
+        // plane c from the history
+        auto tensor = [&](float t[2][6]) {
+            const float2 a0 = lds_pair(gyb, 2), a1 = lds_pair(gyb, GZ + 2), a3 = lds_pair(gyb, 3 * GZ + 2),
+                         a4 = lds_pair(gyb, 4 * GZ + 2);
+            const float2 b0 = lds_pair(gzb, 2), b1 = lds_pair(gzb, GZ + 2), b3 = lds_pair(gzb, 3 * GZ + 2),
+                         b4 = lds_pair(gzb, 4 * GZ + 2);
+            const float2 w0 = lds_pair(gzb, 2 * GZ), w2 = lds_pair(gzb, 2 * GZ + 4);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                t[k][0] = fd4s(hx[k][(J + 1) % 5], hx[k][(J + 2) % 5], hx[k][(J + 4) % 5], hx[k][J]);
+                t[k][3] = fd4s(hy[k][(J + 1) % 5], hy[k][(J + 2) % 5], hy[k][(J + 4) % 5], hy[k][J]);
+                t[k][4] = fd4s(hz[k][(J + 1) % 5], hz[k][(J + 2) % 5], hz[k][(J + 4) % 5], hz[k][J]);
+            }
+            t[0][1] = fd4s(a0.x, a1.x, a3.x, a4.x);
+            t[1][1] = fd4s(a0.y, a1.y, a3.y, a4.y);
+            t[0][5] = fd4s(b0.x, b1.x, b3.x, b4.x);
+            t[1][5] = fd4s(b0.y, b1.y, b3.y, b4.y);
+            t[0][2] = fd4s(w0.x, w0.y, hz[1][JC], w2.x);
+            t[1][2] = fd4s(w0.y, hz[0][JC], w2.x, w2.y);
         };
-        int v[IP];
-        bool unc[IP];
-        bool any = INL && lam != nullptr;
+        float t[2][6];
+        tensor(t);
+        // sign count and margin band, certified in float, as lane masks
+        // (count = ma + mb + mc per lane)
+        unsigned long long ma[2], mb[2], mc[2], mu[2], md[2];
 #pragma unroll
-        for (int k = 0; k < IP; ++k) {
-            float t[6];
-            tensor(k, t);
-            // sign count and margin band, certified in float
-            bool bd;
-            v[k] = positive_eigs_band(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], tols, unc[k], bd);
-            nmarg += (!unc[k] && bd) ? 1u : 0u;
-            any = any || unc[k];
-        }
-        if (__builtin_expect(any, 0)) {
+        for (int k = 0; k < 2; ++k)
+            positive_eigs_band_wave(t[k][0] - lthf, t[k][1] - lthf, t[k][2] - lthf, t[k][3], t[k][4], t[k][5], tols,
+                                    ma[k], mb[k], mc[k], mu[k], md[k]);
+        const auto lrs = buffer_rsrc(lptr, (unsigned)(NN * NN));
+        lptr += NN * NN;
+        // uniform branch: a wave with an uncertain cell (or the lambda output)
+        // takes the per-lane path, the others count from the masks
+        if (__builtin_expect(!(INL && lam) && (mu[0] | mu[1]) == 0ull, 1)) {
+            const unsigned v0 = lane_bits3(ma[0], mb[0], mc[0]), v1 = lane_bits3(ma[1], mb[1], mc[1]);
+            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(0x0404u - (v0 | (v1 << 8))), lrs, coff, 0, 0);
 #pragma unroll
-            for (int k = 0; k < IP; ++k) {
-                if (!(unc[k] || (INL && lam))) continue;
-                // T is recomputed here, so it is not live across the common path
-                float t[6];
-                tensor(k, t);
-                if (unc[k]) {
-                    bool bd;
+            for (int k = 0; k < 2; ++k) {
+                wn3 += (unsigned)__popcll(ma[k] & mb[k] & mc[k]);
+                wany += (unsigned)__popcll(ma[k] | mb[k] | mc[k]);
+                wodd += (unsigned)__popcll(ma[k] ^ mb[k] ^ mc[k]);
+                wmarg += (unsigned)__popcll(md[k]);
+            }
+        } else {
+            // T is recomputed here, so it is not live across the common path
+            float u[2][6];
+            tensor(u);
+            unsigned v[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                v[k] = lane_bits3(ma[k], mb[k], mc[k]);
+                bool marg = lane_bits3(md[k], 0ull, 0ull) != 0u;
+                if (lane_bits3(mu[k], 0ull, 0ull) != 0u) {
                     if constexpr (INL)
-                        v[k] = positive_eigs_band_d(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], tols, bd);
+                        v[k] = (unsigned)positive_eigs_band_d(u[k][0] - lthf, u[k][1] - lthf, u[k][2] - lthf, u[k][3],
+                                                              u[k][4], u[k][5], tols, marg);
                     else
-                        v[k] = positive_eigs_band64(t[0] - lthf, t[1] - lthf, t[2] - lthf, t[3], t[4], t[5], tols, bd);
-                    nmarg += bd ? 1u : 0u;
+                        v[k] = (unsigned)positive_eigs_band64(u[k][0] - lthf, u[k][1] - lthf, u[k][2] - lthf,
+                                                              u[k][3], u[k][4], u[k][5], tols, marg);
                 }
                 if constexpr (INL) {
-                    if (lam) eig_store(t[0], t[1], t[2], t[3], t[4], t[5], ih2, lam, (size_t)c * NN * NN + coff[k], n3);
+                    if (lam)
+                        eig_store(u[k][0], u[k][1], u[k][2], u[k][3], u[k][4], u[k][5], ih2, lam,
+                                  (size_t)c * NN * NN + coff + k, n3);
                 }
+                wn3 += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(v[k] == 3u));
+                wany += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(v[k] != 0u));
+                wodd += (unsigned)__popcll(__builtin_amdgcn_ballot_w64((v[k] & 1u) != 0u));
+                wmarg += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(marg));
             }
-        }
-#pragma unroll
-        for (int k = 0; k < IP; ++k) {
-            const int lab = 4 - v[k];
-            lplane[coff[k]] = (int8_t)lab;
-            cls += 1ull << (16 * (lab - 1));
+            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(0x0404u - (v[0] | (v[1] << 8))), lrs, coff, 0, 0);
         }
     };
     using I0 = std::integral_constant<int, 0>;
@@ -471,7 +562,8 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     using I2 = std::integral_constant<int, 2>;
     using I3 = std::integral_constant<int, 3>;
     using I4 = std::integral_constant<int, 4>;
-    fetch(x0 - 4);
+    const unsigned lanes = (unsigned)__popcll(__builtin_amdgcn_ballot_w64(true));
+    fetch();
     const int send = x1 + 4;
     for (int s = x0 - 4; s < send; s += 5) {
         step(I0{}, s);
@@ -480,12 +572,16 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
         if (s + 3 < send) step(I3{}, s + 3);
         if (s + 4 < send) step(I4{}, s + 4);
     }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const unsigned ci = (unsigned)((cls >> (16 * i)) & 0xffffull);
-        if (ci) atomicAdd(&scnt[i], ci);
+    // labels 1..4 = 4 - count: count 3 -> scnt[0], 2 -> [1], 1 -> [2], 0 -> [3]
+    if ((tid & 63) == 0) {
+        const unsigned wall = 2u * lanes * (unsigned)max(0, x1 - x0);
+        const unsigned wn0 = wall - wany, wn1 = wodd - wn3, wn2 = wany - wn1 - wn3;
+        if (wn3) atomicAdd(&scnt[0], wn3);
+        if (wn2) atomicAdd(&scnt[1], wn2);
+        if (wn1) atomicAdd(&scnt[2], wn1);
+        if (wn0) atomicAdd(&scnt[3], wn0);
+        if (wmarg) atomicAdd(&scnt[4], wmarg);
     }
-    if (nmarg) atomicAdd(&scnt[4], nmarg);
     __syncthreads();
     if (tid < 5 && scnt[tid]) atomicAdd(&stats[tid], (unsigned long long)scnt[tid]);
 }
@@ -527,7 +623,7 @@ static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, 
                                 int8_t *labels, float *lam, unsigned long long *stats, bool stats_zeroed) {
     const bool inl = lam != nullptr;
     auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
-    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
+    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ, NT>();
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
     const int tiles = (N / TY) * (N / TZ);
     int per_cu = 0;
@@ -552,7 +648,7 @@ cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::
     const int nm = (int)mocks.size();
     if (nm == 0) return CM_OK;
     auto kern = k_tweb_fused<TY, TZ, NT, false>;
-    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ>();
+    const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ, NT>();
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
@@ -586,8 +682,8 @@ cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.
-// Tile (TY x TZ cells, threads): 16x64/512 when 64 | N, else 8x32/256,
-// 8x16/128 or 8x8/64 by the largest of 32, 16, 8 dividing N.
+// Tile (TY x TZ cells, threads; two cells per thread): 16x64/512 when 64 | N,
+// else 8x32/128, 8x16/64 or 8x8/32 by the largest of 32, 16, 8 dividing N.
 // stats_zeroed: the counters were cleared by an earlier launch on the stream.
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
                             float *lam, unsigned long long *stats, bool stats_zeroed) {
@@ -597,9 +693,9 @@ cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, doub
         TZ = ctx->tune.tz;
     switch (TZ) {
     case 64: return tweb_fused_run<16, 64, 512>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
-    case 32: return tweb_fused_run<8, 32, 256>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
-    case 16: return tweb_fused_run<8, 16, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
-    case 8: return tweb_fused_run<8, 8, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 32: return tweb_fused_run<8, 32, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 16: return tweb_fused_run<8, 16, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 8: return tweb_fused_run<8, 8, 32>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     }
     return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: N=%d has no tile decomposition", N);
 }

@@ -248,8 +248,16 @@ __device__ __forceinline__ int positive_eigs_count64(float b00, float b11, float
 // determinants are taken in float as they come: a cell whose eigenvalue lies
 // within their rounding of -+ tol may be counted either way, the float
 // rounding of the boundary itself.  tol = 0: no band test.
-__device__ __forceinline__ int positive_eigs_band(float b00, float b11, float b22, float a10, float a20, float a21,
-                                                  float tol, bool &unc, bool &band) {
+// Wave form: called by every active lane together, it returns lane masks of
+// the wave (bit l for lane l): the count's three sign changes ma (1 -> -e1),
+// mb (-e1 -> e2) and mc (e2 -> -e3), so count = ma + mb + mc per lane; munc
+// (the float signs are not all certain) and mband (an eigenvalue within tol of
+// zero).  Each mask is taken straight from a compare and combined in scalar
+// registers.
+__device__ __forceinline__ void positive_eigs_band_wave(float b00, float b11, float b22, float a10, float a20,
+                                                        float a21, float tol, unsigned long long &ma,
+                                                        unsigned long long &mb, unsigned long long &mc,
+                                                        unsigned long long &munc, unsigned long long &mband) {
     const float e1 = b00 + b11 + b22;
     const float q10 = a10 * a10, q20 = a20 * a20, q21 = a21 * a21;
     const float e2 = (b00 * b11 + b00 * b22 + b11 * b22) - (q10 + q20 + q21);
@@ -258,21 +266,25 @@ __device__ __forceinline__ int positive_eigs_band(float b00, float b11, float b2
     const float sc = (fabsf(b00) + fabsf(b11) + fabsf(b22)) + (fabsf(a10) + fabsf(a20) + fabsf(a21));
     const float u = 5.9604645e-8f;
     const float t1 = 8.0f * u * sc, t2 = 2.0f * t1 * sc, t3 = 1.5f * t2 * sc;
-    unc = (fabsf(e1) <= t1) || (fabsf(e2) <= t2) || (fabsf(e3) <= t3) || !(sc < 1e12f);
-    const int v = (e1 > 0.f) + ((e1 < 0.f) != (e2 > 0.f)) + ((e2 > 0.f) != (e3 < 0.f));
-    band = false;
-    if (tol > 0.f) {
+    const unsigned long long p1 = __builtin_amdgcn_ballot_w64(e1 > 0.f), n1 = __builtin_amdgcn_ballot_w64(e1 < 0.f),
+                             p2 = __builtin_amdgcn_ballot_w64(e2 > 0.f), n3 = __builtin_amdgcn_ballot_w64(e3 < 0.f);
+    ma = p1;
+    mb = n1 ^ p2;
+    mc = p2 ^ n3;
+    munc = __builtin_amdgcn_ballot_w64(fabsf(e1) <= t1) | __builtin_amdgcn_ballot_w64(fabsf(e2) <= t2) |
+           __builtin_amdgcn_ballot_w64(fabsf(e3) <= t3) | __builtin_amdgcn_ballot_w64(!(sc < 1e12f));
+    mband = 0ull;
+    if (tol > 0.f) {  // uniform
         const float dm = e3 + tol * (e2 + tol * (e1 + tol));  // det(B + tol I)
         const float dp = e3 - tol * (e2 - tol * (e1 - tol));  // det(B - tol I)
         const float m = fabsf(e1) + 2.0f * tol + t1;
-        const bool two = (fabsf(e3) <= tol * tol * m + t3) && (fabsf(e2) <= tol * (tol + 2.0f * m) + t2);
-        unc = unc || two;
-        band = (dm > 0.f) != (dp > 0.f);
+        munc |= __builtin_amdgcn_ballot_w64(fabsf(e3) <= tol * tol * m + t3) &
+                __builtin_amdgcn_ballot_w64(fabsf(e2) <= tol * (tol + 2.0f * m) + t2);
+        mband = __builtin_amdgcn_ballot_w64(dm > 0.f) ^ __builtin_amdgcn_ballot_w64(dp > 0.f);
     }
-    return v;
 }
 
-// FP64 decision of positive_eigs_band for a cell whose float signs are not
+// FP64 decision of positive_eigs_band_wave for a cell whose float signs are not
 // all certain: the count of B and the band test from B -+ tol I in FP64.
 __device__ __forceinline__ int positive_eigs_band64(float b00, float b11, float b22, float a10, float a20,
                                                     float a21, float tol, bool &band) {
