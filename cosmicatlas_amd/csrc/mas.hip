@@ -100,25 +100,11 @@ struct TileK {
 
 constexpr int kTileCH = 4096;
 
-// Tiles touched by a particle in cell c along one axis: home tile and, when the
-// stencil crosses the tile edge, the neighbour.  Returns the count (1 or 2).
-// The stencil spans cells c-R..c+R; shifted by half a cell (the interlaced
-// mesh) it spans c-R..c+R+1, so records shared by both meshes go to the tile
-// above from Rup = R+1 cells below its edge.
-template <int T>
-__device__ __forceinline__ int axis_tiles(int c, int R, int Rup, int ntx, int out[2]) {
-    const int home = c / T, lc = c % T;
-    out[0] = home;
-    if (lc < R) {
-        out[1] = home == 0 ? ntx - 1 : home - 1;
-        return 2;
-    }
-    if (lc >= T - Rup) {
-        out[1] = home + 1 == ntx ? 0 : home + 1;
-        return 2;
-    }
-    return 1;
-}
+// A particle in cell c touches along one axis its home tile and, when the
+// stencil crosses the tile edge, the neighbour.  The stencil spans cells
+// c-R..c+R; shifted by half a cell (the interlaced mesh) it spans c-R..c+R+1,
+// so records shared by both meshes go to the tile above from Rup = R+1 cells
+// below its edge (axis_pair).
 
 // Cell and in-cell offset f = x/d - c - 1/2 (FP64, as the plain kernel) of a
 // particle from its loaded coordinates; s[] receives the offsets in float.
@@ -155,6 +141,49 @@ __device__ __forceinline__ bool cells_from(const float v[3], const MasK &p, int 
         fr[a] = qf < (1u << kFracBits) ? qf : (1u << kFracBits) - 1u;
     }
     return ok;
+}
+
+// The same cells and fractions with the common case in straight-line code.
+// A coordinate in the box whose q = x rd lies at least 2^-39 from a cell edge
+// takes c = floor(q) and the fraction q - c: x rd is within 2^-40 of x/d for
+// every N <= 4096 (two roundings of 2^-53 relative, q < 2^12), so floor(q) is
+// get_bin's cell, cell_fast's edge test does not fire for it and cells_from
+// returns the same c and fraction.  A particle with any other coordinate
+// (outside the box, non-finite, near an edge) goes through cells_from.
+__device__ __forceinline__ bool particle_cells(const float v[3], const MasK &p, int c[3], unsigned fr[3]) {
+    const float Lf = (float)p.L;
+    bool slow = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float u = (float)((double)v[a] - p.xmin[a]);
+        double x = (double)u;
+        if (p.ishift != 0.0) {  // uniform; shifted()
+            x += p.ishift;
+            if (x >= p.L) x -= p.L;
+        }
+        const double q = x * p.rd, fq = floor(q), f = q - fq;
+        slow = slow | !(u >= 0.0f && u < Lf) | !(f >= 0x1p-39 && f <= 1.0 - 0x1p-39);
+        c[a] = (int)fq;
+        fr[a] = (unsigned)(f * (double)(1u << kFracBits));
+    }
+    if (__builtin_expect(slow, 0)) return cells_from(v, p, c, fr);
+    return true;
+}
+
+// Destination tiles of a cell c along one axis (axis_tiles in closed form, for
+// ntx >= 2): home tile h and, where the stencil crosses the tile edge (two),
+// the neighbour nb; lh and ln are the cell's coordinates relative to the two
+// tiles' origins (the record words' local coordinates, periodic images
+// included: lh + T in the tile below, lh - T in the tile above).
+template <int T>
+__device__ __forceinline__ void axis_pair(int c, int R, int Rup, int ntx, int &h, int &nb, int &lh, int &ln,
+                                          bool &two) {
+    h = c / T;
+    lh = c - h * T;
+    const bool lo = lh < R, hi = lh >= T - Rup;
+    two = lo | hi;
+    nb = lo ? (h == 0 ? ntx - 1 : h - 1) : (h + 1 == ntx ? 0 : h + 1);
+    ln = lo ? lh + T : lh - T;
 }
 
 // Sorted record: one 32-bit word per axis.  Bits 27..31 hold the cell's
@@ -243,19 +272,29 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
             if (base + (size_t)u * step >= i1) continue;
             int c[3];
             unsigned f[3];
-            if (!cells_from(v[u], p, c, f)) {
+            if (!particle_cells(v[u], p, c, f)) {
                 ++ndrop;
                 continue;
             }
-            int tx[2], ty[2], tz[2];
-            const int nx = axis_tiles<T>(c[0], t.R, t.Rup, t.ntx, tx),
-                      ny = axis_tiles<T>(c[1], t.R, t.Rup, t.ntx, ty),
-                      nz = axis_tiles<T>(c[2], t.R, t.Rup, t.ntx, tz);
-            for (int a = 0; a < nx; ++a) {
-                const int lx = tx[a] - t.tx0;  // tiles outside the x range belong to another pass
-                if ((unsigned)lx >= (unsigned)t.nxl) continue;
-                for (int b = 0; b < ny; ++b)
-                    for (int e = 0; e < nz; ++e) atomicAdd(&h[(lx * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
+            int th[3], tn[3], lh[3], ln[3];
+            bool two[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) axis_pair<T>(c[a], t.R, t.Rup, t.ntx, th[a], tn[a], lh[a], ln[a], two[a]);
+            // tiles outside the x range belong to another pass
+            const int lx0 = th[0] - t.tx0, lx1 = tn[0] - t.tx0;
+            const bool x0 = (unsigned)lx0 < (unsigned)t.nxl, x1 = two[0] && (unsigned)lx1 < (unsigned)t.nxl;
+            auto tile = [&](int lx, int y, int z) { return (lx * t.ntx + y) * t.ntx + z; };
+            if (x0) atomicAdd(&h[tile(lx0, th[1], th[2])], 1u);
+            if (two[0] | two[1] | two[2]) {
+                if (x0 && two[2]) atomicAdd(&h[tile(lx0, th[1], tn[2])], 1u);
+                if (x0 && two[1]) atomicAdd(&h[tile(lx0, tn[1], th[2])], 1u);
+                if (x0 && two[1] && two[2]) atomicAdd(&h[tile(lx0, tn[1], tn[2])], 1u);
+                if (x1) {
+                    atomicAdd(&h[tile(lx1, th[1], th[2])], 1u);
+                    if (two[2]) atomicAdd(&h[tile(lx1, th[1], tn[2])], 1u);
+                    if (two[1]) atomicAdd(&h[tile(lx1, tn[1], th[2])], 1u);
+                    if (two[1] && two[2]) atomicAdd(&h[tile(lx1, tn[1], tn[2])], 1u);
+                }
             }
         }
     }
@@ -440,26 +479,40 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
             if (base + (size_t)u * step >= i1) continue;
             int c[3];
             unsigned f[3];
-            if (!cells_from(v[u], p, c, f)) continue;
-            int tx[2], ty[2], tz[2];
-            const int nx = axis_tiles<T>(c[0], t.R, t.Rup, t.ntx, tx),
-                      ny = axis_tiles<T>(c[1], t.R, t.Rup, t.ntx, ty),
-                      nz = axis_tiles<T>(c[2], t.R, t.Rup, t.ntx, tz);
-            for (int a = 0; a < nx; ++a) {
-                const int lx = tx[a] - t.tx0;
-                if ((unsigned)lx >= (unsigned)t.nxl) continue;
-                const unsigned wx = record_word(c[0], tx[a] * T, f[0], p.N, t.Rup);
-                for (int b = 0; b < ny; ++b) {
-                    const unsigned wy = record_word(c[1], ty[b] * T, f[1], p.N, t.Rup);
-                    for (int e = 0; e < nz; ++e) {
-                        const unsigned slot = atomicAdd(&cu[(lx * t.ntx + ty[b]) * t.ntx + tz[e]], 1u);
-                        if (slot < cap) {
-                            sorted[slot] = make_uint3(wx, wy, record_word(c[2], tz[e] * T, f[2], p.N, t.Rup));
-                            if (w) wsorted[slot] = wv[u];
-                        } else {
-                            ovf = 1u;
-                        }
-                    }
+            if (!particle_cells(v[u], p, c, f)) continue;
+            int th[3], tn[3], lh[3], ln[3];
+            bool two[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) axis_pair<T>(c[a], t.R, t.Rup, t.ntx, th[a], tn[a], lh[a], ln[a], two[a]);
+            // record words (record_word) for the home and the neighbour tile
+            unsigned wh[3], wn[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                wh[a] = ((unsigned)(lh[a] + kLcBias) << kFracBits) | f[a];
+                wn[a] = ((unsigned)(ln[a] + kLcBias) << kFracBits) | f[a];
+            }
+            const int lx0 = th[0] - t.tx0, lx1 = tn[0] - t.tx0;
+            const bool x0 = (unsigned)lx0 < (unsigned)t.nxl, x1 = two[0] && (unsigned)lx1 < (unsigned)t.nxl;
+            const float wu = wv[u];
+            auto emit = [&](int lx, int y, int z, unsigned wx, unsigned wy, unsigned wz) {
+                const unsigned slot = atomicAdd(&cu[(lx * t.ntx + y) * t.ntx + z], 1u);
+                if (slot < cap) {
+                    sorted[slot] = make_uint3(wx, wy, wz);
+                    if (w) wsorted[slot] = wu;
+                } else {
+                    ovf = 1u;
+                }
+            };
+            if (x0) emit(lx0, th[1], th[2], wh[0], wh[1], wh[2]);
+            if (two[0] | two[1] | two[2]) {
+                if (x0 && two[2]) emit(lx0, th[1], tn[2], wh[0], wh[1], wn[2]);
+                if (x0 && two[1]) emit(lx0, tn[1], th[2], wh[0], wn[1], wh[2]);
+                if (x0 && two[1] && two[2]) emit(lx0, tn[1], tn[2], wh[0], wn[1], wn[2]);
+                if (x1) {
+                    emit(lx1, th[1], th[2], wn[0], wh[1], wh[2]);
+                    if (two[2]) emit(lx1, th[1], tn[2], wn[0], wh[1], wn[2]);
+                    if (two[1]) emit(lx1, tn[1], th[2], wn[0], wn[1], wh[2]);
+                    if (two[1] && two[2]) emit(lx1, tn[1], tn[2], wn[0], wn[1], wn[2]);
                 }
             }
         }
@@ -512,7 +565,7 @@ __global__ void __launch_bounds__(1024) k_group_count(const float *__restrict__ 
             if (base + (size_t)u * blockDim.x >= i1) continue;
             int c[3];
             unsigned f[3];
-            if (!cells_from(v[u], p, c, f)) {
+            if (!particle_cells(v[u], p, c, f)) {
                 ++ndrop;
                 continue;
             }
@@ -593,7 +646,7 @@ __global__ void __launch_bounds__(1024) k_group_scatter(const float *__restrict_
             if (i >= i1) continue;
             int c[3];
             unsigned f[3];
-            if (!cells_from(v[u], p, c, f)) continue;
+            if (!particle_cells(v[u], p, c, f)) continue;
             int g[2];
             const int ng = particle_rows<T>(c, R, Rup, ntx, gs, G, g);
             const float wi = w ? w[i] : 0.0f;

@@ -302,7 +302,7 @@ __device__ __forceinline__ float2 lds_pair(const lmem<float> *p, int off) {
     return *(const lmem<float2> *)(p + off);
 }
 
-template <int TY, int TZ, int NT, bool INL>
+template <int TY, int TZ, int NT, bool INL, bool LZ>
 __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi_a, int N_a, int segLen_a,
                                                    float ih2_a, double lth_a, TolK tk_a, int8_t *__restrict__ labels_a,
                                                    float *__restrict__ lam, unsigned long long *__restrict__ stats_a,
@@ -356,7 +356,9 @@ __global__ void __launch_bounds__(NT) k_tweb_fused(const float *__restrict__ phi
     // (uniform: the tolerance may come through a generic pointer, whose load
     // the compiler takes as per-lane, and a per-lane tolerance would turn the
     // band test's branch and the wave masks into per-lane code)
-    const float lthf = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)(lth / (double)ih2))));
+    // (LZ: lambda_th = 0, whose subtraction is the identity in float)
+    const float lthf =
+        LZ ? 0.0f : __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)(lth / (double)ih2))));
     const float tols = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)(tol / (double)ih2))));
 
     // this thread's phi loads: byte offsets in a plane (N^2 floats, below
@@ -618,11 +620,14 @@ static int tweb_seg_len(cm_ctx *ctx, int N, int tiles, int per_cu) {
     return (N + nseg - 1) / nseg;
 }
 
-template <int TY, int TZ, int NT>
+// LZ: a variant for lambda_th = 0 (the common threshold), without the
+// subtraction of the threshold from the diagonal.
+template <int TY, int TZ, int NT, bool LZ = false>
 static cm_status tweb_fused_run(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol,
                                 int8_t *labels, float *lam, unsigned long long *stats, bool stats_zeroed) {
     const bool inl = lam != nullptr;
-    auto kern = inl ? k_tweb_fused<TY, TZ, NT, true> : k_tweb_fused<TY, TZ, NT, false>;
+    auto kern = inl ? k_tweb_fused<TY, TZ, NT, true, false>
+                    : (LZ && lth == 0.0 ? k_tweb_fused<TY, TZ, NT, false, LZ> : k_tweb_fused<TY, TZ, NT, false, false>);
     const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ, NT>();
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
     const int tiles = (N / TY) * (N / TZ);
@@ -647,7 +652,9 @@ cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::
     constexpr int TY = 16, TZ = 64, NT = 512;
     const int nm = (int)mocks.size();
     if (nm == 0) return CM_OK;
-    auto kern = k_tweb_fused<TY, TZ, NT, false>;
+    bool lz = true;  // every mock thresholds at lambda_th = 0
+    for (const TwMock &t : mocks) lz = lz && t.lth == 0.0;
+    auto kern = lz ? k_tweb_fused<TY, TZ, NT, false, true> : k_tweb_fused<TY, TZ, NT, false, false>;
     const size_t lds = sizeof(float) * tweb_lds_floats<TY, TZ, NT>();
     if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "tweb: tile %dx%d needs too much LDS", TY, TZ);
     int per_cu = 0;
@@ -692,7 +699,7 @@ cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, doub
         (ctx->tune.tz == 64 || ctx->tune.tz == 32 || ctx->tune.tz == 16 || ctx->tune.tz == 8))
         TZ = ctx->tune.tz;
     switch (TZ) {
-    case 64: return tweb_fused_run<16, 64, 512>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 64: return tweb_fused_run<16, 64, 512, true>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     case 32: return tweb_fused_run<8, 32, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     case 16: return tweb_fused_run<8, 16, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     case 8: return tweb_fused_run<8, 8, 32>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
