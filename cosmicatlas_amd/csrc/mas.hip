@@ -220,10 +220,12 @@ __device__ __forceinline__ void load_batch(const float *__restrict__ pos, size_t
             w[1] = pos[n + i];
             w[2] = pos[2 * n + i];
         } else {
-            const float3 q = reinterpret_cast<const float3 *>(pos)[i];
-            w[0] = q.x;
-            w[1] = q.y;
-            w[2] = q.z;
+            // nontemporal: the positions are read once per pass (0.28 -> 0.266 ms
+            // in the record scatter at 512^3, whose record stores keep the L2)
+            const auto g = as_global(pos) + 3 * i;
+            w[0] = __builtin_nontemporal_load(g);
+            w[1] = __builtin_nontemporal_load(g + 1);
+            w[2] = __builtin_nontemporal_load(g + 2);
         }
     };
     if (base + (size_t)(U - 1) * stride < i1) {
