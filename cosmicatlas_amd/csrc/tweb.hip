@@ -689,18 +689,20 @@ cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::
 }
 
 // Fused T-web from phi: labels (and optional lambdas) plus the five counters.
-// Tile (TY x TZ cells, threads; two cells per thread): 16x64/512 when 64 | N,
-// else 8x32/128, 8x16/64 or 8x8/32 by the largest of 32, 16, 8 dividing N.
+// Tile (TY x TZ cells, threads; two cells per thread): 16x32/256 when 32 | N
+// (four workgroups per CU; 0.462 against 0.473 ms for 16x64/512 at 512^3,
+// which the tuning knob tz = 64 selects), else 8x16/64 or 8x8/32 by the
+// largest of 16, 8 dividing N.
 // stats_zeroed: the counters were cleared by an earlier launch on the stream.
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
                             float *lam, unsigned long long *stats, bool stats_zeroed) {
-    int TZ = N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : (N % 16 == 0 ? 16 : (N % 8 == 0 ? 8 : 0)));
-    if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 && ctx->tune.tz <= TZ &&
+    int TZ = N % 32 == 0 ? 32 : (N % 16 == 0 ? 16 : (N % 8 == 0 ? 8 : 0));
+    if (ctx->tune.tz > 0 && N % ctx->tune.tz == 0 &&
         (ctx->tune.tz == 64 || ctx->tune.tz == 32 || ctx->tune.tz == 16 || ctx->tune.tz == 8))
         TZ = ctx->tune.tz;
     switch (TZ) {
     case 64: return tweb_fused_run<16, 64, 512, true>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
-    case 32: return tweb_fused_run<8, 32, 128>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
+    case 32: return tweb_fused_run<16, 32, 256, true>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     case 16: return tweb_fused_run<8, 16, 64>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     case 8: return tweb_fused_run<8, 8, 32>(ctx, N, L, phi, lth, tol, labels, lam, stats, stats_zeroed);
     }
