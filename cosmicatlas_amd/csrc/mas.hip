@@ -96,6 +96,7 @@ struct TileK {
     int CH;     // max records per deposit workgroup
     int tx0;    // first tile along x of the binned range
     int nxl;    // tiles along x in the range (ntx: the whole mesh)
+    int rl;     // lattice row length of the scatter's blocked walk (0: particle order)
 };
 
 constexpr int kTileCH = 4096;
@@ -240,6 +241,25 @@ __device__ __forceinline__ void load_batch(const float *__restrict__ pos, size_t
             } else {
                 v[u][0] = v[u][1] = v[u][2] = 0.0f;
             }
+        }
+    }
+}
+
+// Particles idx[u] (u < U, where valid) loaded before any is used.
+template <int U, bool SOA>
+__device__ __forceinline__ void load_idx(const float *__restrict__ pos, size_t n, const size_t idx[U], float v[U][3]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t i = idx[u];
+        if constexpr (SOA) {
+            v[u][0] = pos[i];
+            v[u][1] = pos[n + i];
+            v[u][2] = pos[2 * n + i];
+        } else {
+            const auto g = as_global(pos) + 3 * i;
+            v[u][0] = __builtin_nontemporal_load(g);
+            v[u][1] = __builtin_nontemporal_load(g + 1);
+            v[u][2] = __builtin_nontemporal_load(g + 2);
         }
     }
 }
@@ -468,54 +488,81 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
         i0 = (size_t)blockIdx.x * blockDim.x;
     }
     unsigned ovf = 0;
-    for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * step) {
-        float v[kBatch][3], wv[kBatch];
-        load_batch<kBatch, SOA>(pos, p.n, base, step, i1, v);
+    // one particle: its records to the tiles its stencil touches
+    auto scatter_one = [&](const float vv[3], float wu) {
+        int c[3];
+        unsigned f[3];
+        if (!particle_cells(vv, p, c, f)) return;
+        int th[3], tn[3], lh[3], ln[3];
+        bool two[3];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const size_t i = base + (size_t)u * step;
-            wv[u] = (w && i < i1) ? w[i] : 0.0f;
+        for (int a = 0; a < 3; ++a) axis_pair<T>(c[a], t.R, t.Rup, t.ntx, th[a], tn[a], lh[a], ln[a], two[a]);
+        // record words (record_word) for the home and the neighbour tile
+        unsigned wh[3], wn[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            wh[a] = ((unsigned)(lh[a] + kLcBias) << kFracBits) | f[a];
+            wn[a] = ((unsigned)(ln[a] + kLcBias) << kFracBits) | f[a];
         }
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            if (base + (size_t)u * step >= i1) continue;
-            int c[3];
-            unsigned f[3];
-            if (!particle_cells(v[u], p, c, f)) continue;
-            int th[3], tn[3], lh[3], ln[3];
-            bool two[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) axis_pair<T>(c[a], t.R, t.Rup, t.ntx, th[a], tn[a], lh[a], ln[a], two[a]);
-            // record words (record_word) for the home and the neighbour tile
-            unsigned wh[3], wn[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                wh[a] = ((unsigned)(lh[a] + kLcBias) << kFracBits) | f[a];
-                wn[a] = ((unsigned)(ln[a] + kLcBias) << kFracBits) | f[a];
+        const int lx0 = th[0] - t.tx0, lx1 = tn[0] - t.tx0;
+        const bool x0 = (unsigned)lx0 < (unsigned)t.nxl, x1 = two[0] && (unsigned)lx1 < (unsigned)t.nxl;
+        auto emit = [&](int lx, int y, int z, unsigned wx, unsigned wy, unsigned wz) {
+            const unsigned slot = atomicAdd(&cu[(lx * t.ntx + y) * t.ntx + z], 1u);
+            if (slot < cap) {
+                sorted[slot] = make_uint3(wx, wy, wz);
+                if (w) wsorted[slot] = wu;
+            } else {
+                ovf = 1u;
             }
-            const int lx0 = th[0] - t.tx0, lx1 = tn[0] - t.tx0;
-            const bool x0 = (unsigned)lx0 < (unsigned)t.nxl, x1 = two[0] && (unsigned)lx1 < (unsigned)t.nxl;
-            const float wu = wv[u];
-            auto emit = [&](int lx, int y, int z, unsigned wx, unsigned wy, unsigned wz) {
-                const unsigned slot = atomicAdd(&cu[(lx * t.ntx + y) * t.ntx + z], 1u);
-                if (slot < cap) {
-                    sorted[slot] = make_uint3(wx, wy, wz);
-                    if (w) wsorted[slot] = wu;
-                } else {
-                    ovf = 1u;
-                }
-            };
-            if (x0) emit(lx0, th[1], th[2], wh[0], wh[1], wh[2]);
-            if (two[0] | two[1] | two[2]) {
-                if (x0 && two[2]) emit(lx0, th[1], tn[2], wh[0], wh[1], wn[2]);
-                if (x0 && two[1]) emit(lx0, tn[1], th[2], wh[0], wn[1], wh[2]);
-                if (x0 && two[1] && two[2]) emit(lx0, tn[1], tn[2], wh[0], wn[1], wn[2]);
-                if (x1) {
-                    emit(lx1, th[1], th[2], wn[0], wh[1], wh[2]);
-                    if (two[2]) emit(lx1, th[1], tn[2], wn[0], wh[1], wn[2]);
-                    if (two[1]) emit(lx1, tn[1], th[2], wn[0], wn[1], wh[2]);
-                    if (two[1] && two[2]) emit(lx1, tn[1], tn[2], wn[0], wn[1], wn[2]);
-                }
+        };
+        if (x0) emit(lx0, th[1], th[2], wh[0], wh[1], wh[2]);
+        if (two[0] | two[1] | two[2]) {
+            if (x0 && two[2]) emit(lx0, th[1], tn[2], wh[0], wh[1], wn[2]);
+            if (x0 && two[1]) emit(lx0, tn[1], th[2], wh[0], wn[1], wh[2]);
+            if (x0 && two[1] && two[2]) emit(lx0, tn[1], tn[2], wh[0], wn[1], wn[2]);
+            if (x1) {
+                emit(lx1, th[1], th[2], wn[0], wh[1], wh[2]);
+                if (two[2]) emit(lx1, th[1], tn[2], wn[0], wh[1], wn[2]);
+                if (two[1]) emit(lx1, tn[1], th[2], wn[0], wn[1], wh[2]);
+                if (two[1] && two[2]) emit(lx1, tn[1], tn[2], wn[0], wn[1], wn[2]);
+            }
+        }
+    };
+    if (LDSH && t.rl > 0) {
+        // blocked walk of a lattice-ordered chunk (rows of t.rl particles):
+        // blocks of 32 rows x 32 columns, one particle per thread, column
+        // blocks fastest, so that a tile's records from this chunk are
+        // written within one or two blocks (a scattered walk of whole rows
+        // keeps so many record runs open that their cache lines are evicted
+        // and written back several times before they fill)
+        const int RL = t.rl, nbz = RL / 32, r_in = threadIdx.x / 32, c_in = threadIdx.x % 32;
+        const size_t nblocks = (i1 - i0) / 1024;
+        for (size_t b0 = 0; b0 < nblocks; b0 += kBatch) {
+            size_t idx[kBatch];
+            float v[kBatch][3];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const size_t b = min(b0 + (size_t)u, nblocks - 1), rb = b / nbz, cb = b - rb * nbz;
+                idx[u] = i0 + (rb * 32 + r_in) * (size_t)RL + cb * 32 + c_in;
+            }
+            load_idx<kBatch, SOA>(pos, p.n, idx, v);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (b0 + (size_t)u < nblocks) scatter_one(v[u], w ? w[idx[u]] : 0.0f);
+        }
+    } else {
+        for (size_t base = i0 + threadIdx.x; base < i1; base += kBatch * step) {
+            float v[kBatch][3], wv[kBatch];
+            load_batch<kBatch, SOA>(pos, p.n, base, step, i1, v);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const size_t i = base + (size_t)u * step;
+                wv[u] = (w && i < i1) ? w[i] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                if (base + (size_t)u * step >= i1) continue;
+                scatter_one(v[u], wv[u]);
             }
         }
     }
@@ -938,7 +985,6 @@ static cm_status mas_bin_range(cm_ctx *ctx, const MasK &p, const float *pos, con
     t.CH = kTileCH;
     t.tx0 = tx0;
     t.nxl = nxl;
-    *tk = t;
     const int ntiles = nxl * t.ntx * t.ntx;
     // a particle is copied to at most two tiles per axis: below when its
     // stencil reaches R cells under the tile edge, above when it reaches Rup
@@ -952,6 +998,16 @@ static cm_status mas_bin_range(cm_ctx *ctx, const MasK &p, const float *pos, con
     const bool lds_hist = hist_lds <= ctx->lds_per_block;
     const int nrows = lds_hist ? std::max(1, ctx->cu_count) : 1;
     const size_t chunk = (p.n + nrows - 1) / nrows;
+    // n = NL^3 particles (a lattice-based catalogue, rows of NL particles): the
+    // scatter walks each chunk in blocks of 32 rows x 32 columns when the
+    // chunks are whole multiples of 32 rows (any walk of a chunk gives the same
+    // records; this one keeps few record runs open at a time for lattice order)
+    t.rl = 0;
+    if (lds_hist && nrows > 1 && p.n % (size_t)nrows == 0) {
+        size_t nl = (size_t)std::llround(std::cbrt((double)p.n));
+        if (nl >= 32 && nl * nl * nl == p.n && nl % 32 == 0 && chunk % (32 * nl) == 0) t.rl = (int)nl;
+    }
+    *tk = t;
     void *hist, *counts, *offsets, *item_off, *bsum, *items, *nitems, *sorted;
     const std::string tg(tag);
     cm_status s;
@@ -1262,6 +1318,7 @@ cm_status mas_deposit_packed(cm_ctx *ctx, const std::vector<TileBins> &bins, con
         t.CH = kTileCH;
         t.tx0 = 0;
         t.nxl = b.ntx;
+        t.rl = 0;
         if ((s = mas_zero_split<16>(ctx, b, t, meshes[(size_t)m])) != CM_OK) return s;
         dm[(size_t)m] = DepMock{(const uint3 *)b.sorted, (const float *)b.wsorted, meshes[(size_t)m], b.N, b.ntx, 0, 0};
         src[(size_t)m] = PackSrc{(const uint4 *)b.items, (const unsigned *)b.nitems};

@@ -57,6 +57,28 @@ def test_c1_cic_f64_mesh_matches_oracle(ctx, orc):
     assert np.max(np.abs(mesh - ref)) < 1e-12
 
 
+@pytest.mark.parametrize("scheme", ["CIC", "TSC"])
+def test_lattice_order_blocked_walk(ctx, orc, scheme):
+    """128^3 particles in lattice order (rows of 128 along z, displaced by up to
+    3 cells): the record scatter walks each workgroup's chunk in blocks of 32
+    rows x 32 columns; the mesh must equal the oracle's as for any order."""
+    NL, N, L = 128, 128, 500.0
+    rng = np.random.default_rng(11)
+    g = (np.arange(NL) + 0.5) * (L / NL)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    pos = ((lat + rng.uniform(-3, 3, lat.shape) * (L / N)) % L).astype(np.float32)
+    sch = {"CIC": orc.CIC, "TSC": orc.TSC}[scheme]
+    mesh = ctx.mesh_assign(torch.from_numpy(pos).cuda(), N, L, scheme, algo="tiled").cpu().numpy()
+    assert rel_err(mesh, orc.mas(pos, N, L, sch)) < 6e-6
+    # the same particles in random order (the walk then has no lattice rows):
+    # tiles of more than 4096 records are deposited in chunks, each rounded to
+    # its own fixed-point quantum, and which records share a chunk depends on
+    # the order, so the two meshes agree to the quantisation bound above
+    perm = rng.permutation(len(pos))
+    mp = ctx.mesh_assign(torch.from_numpy(pos[perm]).cuda(), N, L, scheme, algo="tiled").cpu().numpy()
+    assert rel_err(mp, mesh) < 6e-6
+
+
 def test_tiled_equals_plain_on_clustered_input(ctx):
     """Clumps put >4096 particles in a tile: chunked tiles with global atomics."""
     N, L, n = 64, 100.0, 600000
