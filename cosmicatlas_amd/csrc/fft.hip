@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_r2c_ct(const float *__
     const float2 *inrow = reinterpret_cast<const float2 *>(in + row0 * (size_t)N);
     float acc = 0.f;  // at most E values per thread; the workgroup total is FP64
     auto src = [&](int b, int m) {
-        float2 z = inrow[b * M + m];
+        float2 z = load_stream(&inrow[b * M + m]);
         acc += z.x + z.y;
         z.x -= sub;
         z.y -= sub;
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
         const int t = threadIdx.x + i * NT;
         if (t < NB * (M + 1)) {
             const int b = t / (M + 1), k = t - b * (M + 1);
-            ld[i] = irow[(size_t)b * pitch + k];
+            ld[i] = load_stream(&irow[(size_t)b * pitch + k]);
         }
     }
 #pragma unroll
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__res
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     const size_t base = (size_t)blockIdx.y * planeStride + (size_t)blockIdx.x * NB;
     const int valid = min(NB, ncols - (int)blockIdx.x * NB);
-    auto src = [&](int b, int e) { return b < valid ? data[base + (size_t)e * stride + b] : float2{0.f, 0.f}; };
+    auto src = [&](int b, int e) { return b < valid ? load_stream(&data[base + (size_t)e * stride + b]) : float2{0.f, 0.f}; };
     auto dst = [&](int b, int e, float2 v) {
         if (phase) v = cmul(v, phase[e]);
         if (b < valid) store_stream(&data[base + (size_t)e * stride + b], v);

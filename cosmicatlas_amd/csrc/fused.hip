@@ -31,11 +31,9 @@ template <int N, int NB, int E, bool TWL>
 __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restrict__ data, size_t base,
                                                             size_t xstride, int valid, float2 *lds,
                                                             const float2 *__restrict__ tw) {
-    const auto g = as_global(data);
     auto srcG = [&](int b, int e) -> float2 {
         if (b >= valid) return float2{0.f, 0.f};
-        const float2 v = g[base + (size_t)e * xstride + b];
-        return v;
+        return load_stream(&data[base + (size_t)e * xstride + b]);
     };
     auto dstL = [&](int b, int e, float2 v) { lds[e * (NB + 1) + b] = v; };
     ct_fft<float, N, NB, NB + 1, true, true, E, TWL>(lds, tw, srcG, dstL);

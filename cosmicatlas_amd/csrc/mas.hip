@@ -264,6 +264,17 @@ __device__ __forceinline__ void load_idx(const float *__restrict__ pos, size_t n
     }
 }
 
+// Particle chunk of binning workgroup b of n: workgroups are dispatched to the
+// eight XCDs in turn (b mod 8), so consecutive chunks go to workgroups of one
+// XCD.  Consecutive chunks of an ordered catalogue share tiles, and their
+// record runs of a tile are adjacent: the cache lines where one run ends and
+// the next begins are then written through one L2 (scatter 0.215 -> 0.210 ms
+// at 512^3).  The count pass uses the same map, so its histogram row of
+// chunk c is the one the scatter reads.
+__device__ __forceinline__ unsigned chunk_of(unsigned b, unsigned n) {
+    return n % 8u == 0u ? (b % 8u) * (n / 8u) + b / 8u : b;
+}
+
 // Count records per tile.  LDS form: workgroup g owns particles
 // [g*chunk, (g+1)*chunk) and writes its histogram row hist[g*ntiles + tile];
 // global form (hist_lds == false, too many tiles for LDS): one row, global
@@ -279,7 +290,7 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
         for (int i = threadIdx.x; i < ntiles; i += blockDim.x) lh[i] = 0u;
         __syncthreads();
         h = lh;
-        i0 = (size_t)blockIdx.x * chunk;
+        i0 = (size_t)chunk_of(blockIdx.x, gridDim.x) * chunk;
         i1 = min(p.n, i0 + chunk);
         step = blockDim.x;
     } else {
@@ -323,7 +334,7 @@ __global__ void __launch_bounds__(1024) k_tile_count(const float *__restrict__ p
     if (ndrop) atomicAdd(dropped, (unsigned long long)ndrop);
     if constexpr (LDSH) {
         __syncthreads();
-        unsigned *row = hist + (size_t)blockIdx.x * ntiles;
+        unsigned *row = hist + (size_t)chunk_of(blockIdx.x, gridDim.x) * ntiles;
         for (int i = threadIdx.x; i < ntiles; i += blockDim.x) row[i] = lh[i];
     }
 }
@@ -477,11 +488,11 @@ __global__ void __launch_bounds__(1024) k_tile_scatter(const float *__restrict__
     unsigned *cu = gcursor;
     size_t i0 = 0, i1 = p.n, step = (size_t)gridDim.x * blockDim.x;
     if constexpr (LDSH) {
-        const unsigned *row = pref + (size_t)blockIdx.x * ntiles;
+        const unsigned *row = pref + (size_t)chunk_of(blockIdx.x, gridDim.x) * ntiles;
         for (int i = threadIdx.x; i < ntiles; i += blockDim.x) cur[i] = offsets[i] + row[i];
         __syncthreads();
         cu = cur;
-        i0 = (size_t)blockIdx.x * chunk;
+        i0 = (size_t)chunk_of(blockIdx.x, gridDim.x) * chunk;
         i1 = min(p.n, i0 + chunk);
         step = blockDim.x;
     } else {
