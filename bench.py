@@ -72,7 +72,7 @@ KERNELS = ["mas_group_count", "mas_group_scan", "mas_group_scatter", "mas_tile_c
 # rocprofv3 kernel names (demangled prefix) -> the library's kernel names, for
 # the fused path (the only path run with --fused-only)
 ROCPROF_NAMES = [("cm::k_tile_count_lds", "mas_tile_count"), ("cm::k_tile_count", "mas_tile_count"),
-                 ("cm::k_tile_prefix", "mas_tile_prefix"), ("cm::k_tile_scan", "mas_tile_scan"),
+                 ("cm::k_tile_prefix", "mas_tile_prefix"), ("cm::k_tile_total", "mas_tile_scan"),
                  ("cm::k_tile_items", "mas_tile_items"), ("cm::k_tile_scatter", "mas_tile_scatter"),
                  ("cm::k_tile_zero_split", "mas_tile_zero_split"), ("cm::k_tile_deposit", "mas_tile_deposit"),
                  ("cm::k_rows_r2c", "fft_z_r2c"), ("cm::k_rows_c2r", "fft_z_c2r"), ("cm::k_x_fused", "fused_x"),

@@ -393,45 +393,34 @@ __global__ void __launch_bounds__(256) k_tile_prefix(unsigned *__restrict__ hist
     }
 }
 
-// Exclusive scan of the workgroup sums (one workgroup, nblk <= 4096), in
-// 64-bit so that a total of 2^32 records or more is seen and reported.
-__global__ void __launch_bounds__(1024) k_tile_bscan(unsigned long long *__restrict__ bsum, int nblk) {
-    __shared__ unsigned long long sa[1024], sb[1024];
-    const int tid = threadIdx.x;
-    constexpr int PER = 4;
-    unsigned long long a[PER], b[PER], ta = 0, tb = 0;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid * PER + k;
-        a[k] = i < nblk ? bsum[2 * i] : 0ull;
-        b[k] = i < nblk ? bsum[2 * i + 1] : 0ull;
-        ta += a[k];
-        tb += b[k];
+// Totals of the workgroup sums (one workgroup) into bsum[2 nblk], [2 nblk + 1],
+// in 64-bit so that a total of 2^32 records or more is seen and reported: the
+// host reads them when it sizes the record buffer exactly.
+__global__ void __launch_bounds__(1024) k_tile_total(unsigned long long *__restrict__ bsum, int nblk) {
+    __shared__ unsigned long long sa[16], sb[16];
+    unsigned long long ta = 0, tb = 0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        ta += bsum[2 * i];
+        tb += bsum[2 * i + 1];
     }
-    sa[tid] = ta;
-    sb[tid] = tb;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ta += __shfl_xor(ta, o, 64);
+        tb += __shfl_xor(tb, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = ta;
+        sb[threadIdx.x >> 6] = tb;
+    }
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned long long va = tid >= off ? sa[tid - off] : 0ull, vb = tid >= off ? sb[tid - off] : 0ull;
-        __syncthreads();
-        sa[tid] += va;
-        sb[tid] += vb;
-        __syncthreads();
-    }
-    unsigned long long ra = sa[tid] - ta, rb = sb[tid] - tb;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid * PER + k;
-        if (i < nblk) {
-            bsum[2 * i] = ra;
-            bsum[2 * i + 1] = rb;
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            a += sa[w];
+            b += sb[w];
         }
-        ra += a[k];
-        rb += b[k];
-    }
-    if (tid == 1023) {
-        bsum[2 * nblk] = sa[1023];
-        bsum[2 * nblk + 1] = sb[1023];
+        bsum[2 * nblk] = a;
+        bsum[2 * nblk + 1] = b;
     }
 }
 
@@ -439,6 +428,9 @@ __global__ void __launch_bounds__(1024) k_tile_bscan(unsigned long long *__restr
 // {tile, first record, end record, split} per chunk of at most kTileCH records,
 // clamped to the record buffer's capacity.  counts[tile] holds the tile total
 // on entry and the slot cursor (= first record) on exit.
+// bsum holds the per-workgroup sums of k_tile_prefix; each workgroup adds up
+// the sums of the workgroups before it (the exclusive scan, at most 4096
+// values) and tile 0's workgroup all of them.
 __global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned long long *__restrict__ bsum,
                                                     unsigned *__restrict__ offsets, unsigned *__restrict__ item_off,
                                                     unsigned *__restrict__ counts, uint4 *__restrict__ items,
@@ -446,19 +438,47 @@ __global__ void __launch_bounds__(256) k_tile_items(int ntiles, const unsigned l
                                                     unsigned max_items, unsigned *__restrict__ overflow) {
     const int tile = blockIdx.x * 256 + threadIdx.x;
     const int nblk = (ntiles + 255) / 256;
+    const int blk = blockIdx.x;
+    __shared__ unsigned long long sred[4][4];
+    unsigned long long pa = 0, pb = 0, sa = 0, sb = 0;
+    for (int j = threadIdx.x; j < (blk == 0 ? nblk : blk); j += 256) {
+        const unsigned long long a = bsum[2 * j], b = bsum[2 * j + 1];
+        sa += a;
+        sb += b;
+        if (j < blk) {
+            pa += a;
+            pb += b;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        pa += __shfl_xor(pa, o, 64);
+        pb += __shfl_xor(pb, o, 64);
+        sa += __shfl_xor(sa, o, 64);
+        sb += __shfl_xor(sb, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sred[threadIdx.x >> 6][0] = pa;
+        sred[threadIdx.x >> 6][1] = pb;
+        sred[threadIdx.x >> 6][2] = sa;
+        sred[threadIdx.x >> 6][3] = sb;
+    }
+    __syncthreads();
+    pa = (sred[0][0] + sred[1][0]) + (sred[2][0] + sred[3][0]);
+    pb = (sred[0][1] + sred[1][1]) + (sred[2][1] + sred[3][1]);
     if (tile == 0) {
-        const unsigned long long tot = bsum[2 * nblk], nit = bsum[2 * nblk + 1];
+        const unsigned long long tot = (sred[0][2] + sred[1][2]) + (sred[2][2] + sred[3][2]),
+                                 nit = (sred[0][3] + sred[1][3]) + (sred[2][3] + sred[3][3]);
         offsets[ntiles] = (unsigned)min(tot, (unsigned long long)cap);
         item_off[ntiles] = (unsigned)min(nit, (unsigned long long)max_items);
         *nitems = (unsigned)min(nit, (unsigned long long)max_items);
         if (tot > cap || nit > max_items) atomicOr(overflow, 1u);
     }
     if (tile >= ntiles) return;
-    const int blk = tile / 256;
     const unsigned cnt = counts[tile];
-    const unsigned long long tb64 = offsets[tile] + bsum[2 * blk];
+    const unsigned long long tb64 = offsets[tile] + pa;
     const unsigned tb = (unsigned)min(tb64, (unsigned long long)cap);
-    const unsigned ib = item_off[tile] + (unsigned)bsum[2 * blk + 1];
+    const unsigned ib = item_off[tile] + (unsigned)pb;
     const unsigned long long te64 = tb64 + cnt;
     offsets[tile] = tb;
     item_off[tile] = ib;
@@ -1047,13 +1067,13 @@ static cm_status mas_bin_range(cm_ctx *ctx, const MasK &p, const float *pos, con
                     (unsigned *)offsets, (unsigned *)item_off, (unsigned *)counts,
                     (unsigned long long *)bsum)) != CM_OK)
         return s;
-    if ((s = launch(ctx, "mas_tile_scan", k_tile_bscan, dim3(1), dim3(1024), 0, (unsigned long long *)bsum, nblk)) !=
-        CM_OK)
-        return s;
     // record indices are 32-bit
     const size_t max_cap = 0xffffffffull;
     size_t cap;
     if (exact) {
+        if ((s = launch(ctx, "mas_tile_scan", k_tile_total, dim3(1), dim3(1024), 0, (unsigned long long *)bsum,
+                        nblk)) != CM_OK)
+            return s;
         unsigned long long total = 0;
         CM_HIP(ctx, hipMemcpyAsync(&total, (unsigned long long *)bsum + 2 * nblk, sizeof(total),
                                    hipMemcpyDeviceToHost, ctx->stream));
