@@ -790,11 +790,84 @@ __device__ __forceinline__ void deposit_record(void *lgrid, int base, const floa
     constexpr int R = SCHEME == 0 ? 0 : (SCHEME == 3 ? 2 : 1);
     constexpr int W = 2 * R + 1;
     float wa[3][5];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) axis_weights<float>(SCHEME, f[a], wa[a]);
     const float sx = UNIT ? qs : wt;
+    if constexpr (SCHEME == 1) {
+        // CIC: the two nodes with weight along each axis (c-1 and c when
+        // f < 0, else c and c+1; the third node of the 3-node stencil has
+        // weight zero), the same weights and products as the general loop: 8
+        // LDS adds per record instead of 27
+        float wx[3][2];
+        int off = 0;
 #pragma unroll
-    for (int a = 0; a < W; ++a) wa[0][a + 2 - R] *= sx;
+        for (int a = 0; a < 3; ++a) {
+            const bool neg = f[a] < 0.0f;
+            const float c0 = 1.0f - fabsf(f[a]);
+            wx[a][0] = neg ? -f[a] : c0;
+            wx[a][1] = neg ? c0 : f[a];
+            off = off * S + (neg ? 0 : 1);
+        }
+        wx[0][0] *= sx;
+        wx[0][1] *= sx;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const float vab = wx[0][a] * wx[1][b];
+                const int rowi = base + off + (a * S + b) * S;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    if constexpr (UNIT) {
+                        const float vq = fmaf(vab, wx[2][e], 0.5f);
+                        atomicAdd((unsigned *)lgrid + rowi + e, (unsigned)vq);
+                    } else {
+                        const float v = vab * wx[2][e];
+                        const long long q = __double2ll_rn((double)v * 4294967296.0);
+                        atomicAdd((unsigned long long *)lgrid + rowi + e, (unsigned long long)q);
+                    }
+                }
+            }
+        }
+        return;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) axis_weights<float>(SCHEME, f[a], wa[a]);
+#pragma unroll
+        for (int a = 0; a < W; ++a) wa[0][a + 2 - R] *= sx;
+    }
+    if constexpr (UNIT && W > 1 && S % 2 == 0) {
+        // 32-bit cells: two adjacent z nodes whose pair of cells is 8-byte
+        // aligned take one 64-bit add (no carry crosses between the two
+        // cells: every cell's total stays below 2^32 by the choice of the
+        // quantum), so a row of W nodes takes (W+1)/2 adds instead of W.  The
+        // row's first node has the parity of base (S is even); the z weights
+        // are permuted once so that the pairs come first and the single node
+        // last.
+        constexpr int NP = W / 2;
+        const int p = base & 1;
+        float wp[W];
+#pragma unroll
+        for (int e = 0; e < W - 1; ++e) wp[e] = p ? wa[2][e + 3 - R] : wa[2][e + 2 - R];
+        wp[W - 1] = p ? wa[2][2 - R] : wa[2][2 + R];
+        unsigned *const g32 = (unsigned *)lgrid;
+        const int pbase = base + p, sbase = base + (p ? 0 : W - 1);
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                const float vab = wa[0][a + 2 - R] * wa[1][b + 2 - R];
+                const int ro = (a * S + b) * S;
+#pragma unroll
+                for (int k = 0; k < NP; ++k) {
+                    const unsigned lo = (unsigned)fmaf(vab, wp[2 * k], 0.5f),
+                                   hi = (unsigned)fmaf(vab, wp[2 * k + 1], 0.5f);
+                    atomicAdd((unsigned long long *)(g32 + pbase + ro + 2 * k),
+                              ((unsigned long long)hi << 32) | (unsigned long long)lo);
+                }
+                atomicAdd(g32 + sbase + ro, (unsigned)fmaf(vab, wp[W - 1], 0.5f));
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < W; ++a) {
 #pragma unroll
