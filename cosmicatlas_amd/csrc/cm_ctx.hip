@@ -130,6 +130,14 @@ const void *find_table(cm_ctx *ctx, const std::string &key) {
     return it == ctx->ctables.end() ? nullptr : it->second.ptr;
 }
 
+hipError_t host_upload(cm_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    hipError_t e;
+    if (!ctx->upload_stream && (e = hipStreamCreateWithFlags(&ctx->upload_stream, hipStreamNonBlocking)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->upload_stream)) != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->upload_stream);
+}
+
 const void *upload_table(cm_ctx *ctx, const std::string &key, const void *host, size_t bytes, cm_status *st) {
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
@@ -137,7 +145,7 @@ const void *upload_table(cm_ctx *ctx, const std::string &key, const void *host, 
         *st = hip_fail(ctx, e, "hipMalloc(table)");
         return nullptr;
     }
-    if (bytes && (e = hipMemcpy(p, host, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+    if (bytes && (e = host_upload(ctx, p, host, bytes)) != hipSuccess) {
         hipFree(p);
         *st = hip_fail(ctx, e, "table upload");
         return nullptr;
@@ -197,9 +205,9 @@ const void *twiddles(cm_ctx *ctx, int n, cm_dtype dt, cm_status *st) {
     hipError_t e;
     if (dt == CM_F32) {
         std::vector<float> f(d.begin(), d.end());
-        e = hipMemcpy(p, f.data(), bytes, hipMemcpyHostToDevice);
+        e = host_upload(ctx, p, f.data(), bytes);
     } else {
-        e = hipMemcpy(p, d.data(), bytes, hipMemcpyHostToDevice);
+        e = host_upload(ctx, p, d.data(), bytes);
     }
     if (e != hipSuccess) {
         hipFree(p);
@@ -305,6 +313,7 @@ void cm_ctx_destroy(cm_ctx *ctx) {
         if (kv.second.ptr) hipFree(kv.second.ptr);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    if (ctx->upload_stream) hipStreamDestroy(ctx->upload_stream);
     delete ctx;
 }
 

@@ -43,6 +43,7 @@ struct GraphEntry {
 struct cm_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
+    hipStream_t upload_stream = nullptr;         // non-blocking: host-to-device copies of tables (host_upload)
     hipStream_t stream = nullptr;
     size_t budget = 0;
     size_t resident = 0;  // bytes held in `scratch` and tables
@@ -100,6 +101,11 @@ size_t scratch_release(cm_ctx *ctx);
 // `key` and kept for the context's lifetime (no host work or copy afterwards).
 const void *find_table(cm_ctx *ctx, const std::string &key);
 const void *upload_table(cm_ctx *ctx, const std::string &key, const void *host, size_t bytes, cm_status *st);
+// Synchronous host-to-device copy on the context's non-blocking upload stream:
+// unlike hipMemcpy on the legacy stream it takes no implicit dependency on
+// other blocking streams, so it is legal while another thread's stream is
+// being captured into a graph (server slots).
+hipError_t host_upload(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
 template <typename T, typename F>
 const T *cached_table(cm_ctx *ctx, const std::string &key, F make, cm_status *st) {
     if (const void *p = find_table(ctx, key)) {

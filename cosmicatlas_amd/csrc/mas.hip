@@ -1438,8 +1438,8 @@ cm_status mas_deposit_packed(cm_ctx *ctx, const std::vector<TileBins> &bins, con
     // the descriptor tables are small; a synchronous copy keeps the host
     // vectors out of the stream's lifetime
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    CM_HIP(ctx, hipMemcpy(ddm, dm.data(), sizeof(DepMock) * (size_t)nm, hipMemcpyHostToDevice));
-    CM_HIP(ctx, hipMemcpy(dsrc, src.data(), sizeof(PackSrc) * (size_t)nm, hipMemcpyHostToDevice));
+    CM_HIP(ctx, host_upload(ctx, ddm, dm.data(), sizeof(DepMock) * (size_t)nm));
+    CM_HIP(ctx, host_upload(ctx, dsrc, src.data(), sizeof(PackSrc) * (size_t)nm));
     ++ctx->host_syncs;
     const unsigned gx = (unsigned)std::min<size_t>(1024, (per_mock + 255) / 256);
     if ((s = launch(ctx, "mas_pack_items", k_pack_items, dim3(gx, (unsigned)nm), dim3(256), 0, (const PackSrc *)dsrc,

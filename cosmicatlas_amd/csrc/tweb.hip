@@ -681,7 +681,7 @@ cm_status tweb_fused_packed(cm_ctx *ctx, std::vector<TwMock> &mocks, const std::
     void *dtm;
     if ((s = scratch(ctx, "tweb.pk.mocks", sizeof(TwMock) * (size_t)nm, &dtm)) != CM_OK) return s;
     CM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    CM_HIP(ctx, hipMemcpy(dtm, mocks.data(), sizeof(TwMock) * (size_t)nm, hipMemcpyHostToDevice));
+    CM_HIP(ctx, host_upload(ctx, dtm, mocks.data(), sizeof(TwMock) * (size_t)nm));
     ++ctx->host_syncs;
     return launch(ctx, "tweb_fused", kern, dim3((unsigned)blocks), dim3(NT), lds, (const float *)nullptr, 0, 0, 0.0f,
                   0.0, TolK{0.0, 0.0, nullptr}, (int8_t *)nullptr, (float *)nullptr, (unsigned long long *)nullptr,
