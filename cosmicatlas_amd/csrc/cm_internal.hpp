@@ -15,6 +15,48 @@
 
 namespace cm {
 
+// Work that rides at the end of a spectrum pass in one workgroup, in place of
+// a launch of its own: the FP64 total of part[0, np) into out[0] and cleared
+// accumulators zs[0, 4 ns), zc[0, ns) and zt[0, 5) (nullable) for the passes
+// after it.  part null: nothing.
+struct SumTail {
+    const double *part;
+    int np;
+    double *out;
+    double *zs;
+    unsigned long long *zc;
+    int ns;
+    unsigned long long *zt;
+};
+
+// SumTail for the calling workgroup (every thread calls it).
+__device__ inline void sum_tail(const SumTail &t) {
+    __shared__ double red[16];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < 4 * t.ns; i += nt) t.zs[i] = 0.0;
+    for (int i = tid; i < t.ns; i += nt) t.zc[i] = 0ull;
+    if (t.zt && tid < 5) t.zt[tid] = 0ull;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = tid;
+    for (; i + 3 * nt < t.np; i += 4 * nt) {
+        s0 += t.part[i];
+        s1 += t.part[i + nt];
+        s2 += t.part[i + 2 * nt];
+        s3 += t.part[i + 3 * nt];
+    }
+    for (; i < t.np; i += nt) s0 += t.part[i];
+    double sv = (s0 + s1) + (s2 + s3);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sv += __shfl_xor(sv, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = sv;
+    __syncthreads();
+    if (tid == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < (nt + 63) / 64; ++w) tot += red[w];
+        t.out[0] = tot;
+    }
+}
+
 // Per-kernel event timing (cm_profile_enable / cm_kernel_time).
 struct KernelTimes {
     double sum_ms = 0.0;
@@ -156,8 +198,11 @@ cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
 // float32 only may use a padded pitch).
 // phase (float32 only, nullable): per-frequency factors float2[N] multiplied
 // into the output of the pass (kz index for the z pass, ky for the y pass).
+// tail (float32 only, nullable): a reduction and clears that workgroup (0,0)
+// of the pass runs after its transform (SumTail).
+struct SumTail;
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch = 0,
-                   const void *phase = nullptr);
+                   const void *phase = nullptr, const SumTail *tail = nullptr);
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double subtract, double *part, int pitch = 0, const void *phase = nullptr);
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);

@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
 template <int N, int NB, bool FWD>
 __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__restrict__ data, int ncols, size_t stride,
                                                                  size_t planeStride, const float2 *__restrict__ tw,
-                                                                 const float2 *__restrict__ phase) {
+                                                                 const float2 *__restrict__ phase, SumTail tail) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     const size_t base = (size_t)blockIdx.y * planeStride + (size_t)blockIdx.x * NB;
@@ -297,6 +297,10 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__res
         if (b < valid) store_stream(&data[base + (size_t)e * stride + b], v);
     };
     ct_fft<float, N, NB, NB, true, FWD>(lds, tw, src, dst);
+    if (tail.part && blockIdx.x == 0 && blockIdx.y == 0) {  // uniform per workgroup
+        __syncthreads();
+        sum_tail(tail);
+    }
 }
 
 // column block width of the float32 y/x passes
@@ -321,7 +325,8 @@ static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitc
 }
 
 template <int N, bool FWD>
-static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw, const float2 *phase) {
+static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw, const float2 *phase,
+                         const SumTail *tail) {
     constexpr int NB = col_nb<N>();
     using CS = ColShape<N, NB>;
     if (CS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: LDS");
@@ -336,8 +341,9 @@ static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const f
     }
     dim3 grid((unsigned)((ncols + NB - 1) / NB), (unsigned)N);
     const char *name = axis == 1 ? (FWD ? "fft_y_fwd" : "fft_y_inv") : (FWD ? "fft_x_fwd" : "fft_x_inv");
+    const SumTail t = tail ? *tail : SumTail{nullptr, 0, nullptr, nullptr, nullptr, 0, nullptr};
     return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw,
-                  phase);
+                  phase, t);
 }
 
 static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **twN) {
@@ -381,7 +387,8 @@ static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int
     return fail(ctx, CM_ERR_UNSUPPORTED, "fft: N=%d", N);
 }
 
-static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch, const float2 *phase) {
+static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch, const float2 *phase,
+                          const SumTail *tail) {
     cm_status st;
     const float2 *tw = (const float2 *)twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;
@@ -389,8 +396,8 @@ static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward
     switch (N) {
 #define CM_CASE(n)                                                                     \
     case n:                                                                            \
-        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase)                                        \
-                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase);
+        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase, tail)                                  \
+                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase, tail);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -557,10 +564,10 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 }
 
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch,
-                   const void *phase) {
+                   const void *phase, const SumTail *tail) {
     if (dt == CM_F32)
-        return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1, (const float2 *)phase);
-    if (phase) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: phase on an FP64 spectrum");
+        return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1, (const float2 *)phase, tail);
+    if (phase || tail) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: phase or tail on an FP64 spectrum");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return forward ? cols_t<double, true>(ctx, N, data, axis) : cols_t<double, false>(ctx, N, data, axis);
 }

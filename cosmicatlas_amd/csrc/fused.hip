@@ -248,35 +248,6 @@ __global__ void __launch_bounds__((ColShape<N, NB, E>::NT)) k_x_fused(float2 *__
     }
 }
 
-// FP64 total of the z pass's partial sums (one workgroup, four independent
-// accumulators per thread so the loads overlap).  The same launch clears the
-// accumulators of the passes after it: shell sums zs[0, 4 ns) and counts
-// zc[0, ns) of the fused x pass, the five T-web counters zt (nullable).
-__global__ void __launch_bounds__(1024) k_fused_sum(const double *__restrict__ part, int np, double *__restrict__ out,
-                                                    double *__restrict__ zs, unsigned long long *__restrict__ zc,
-                                                    int ns, unsigned long long *__restrict__ zt) {
-    for (int i = threadIdx.x; i < 4 * ns; i += blockDim.x) zs[i] = 0.0;
-    for (int i = threadIdx.x; i < ns; i += blockDim.x) zc[i] = 0ull;
-    if (zt && threadIdx.x < 5) zt[threadIdx.x] = 0ull;
-    __shared__ double red[1024];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3 * 1024 < np; i += 4 * 1024) {
-        s0 += part[i];
-        s1 += part[i + 1024];
-        s2 += part[i + 2 * 1024];
-        s3 += part[i + 3 * 1024];
-    }
-    for (; i < np; i += 1024) s0 += part[i];
-    red[threadIdx.x] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = red[0];
-}
-
 PkK make_pkk(cm_ctx *ctx, const cm_pk_opts *o, cm_status *st);
 cm_status tweb_fused_launch(cm_ctx *ctx, int N, double L, const float *phi, double lth, TolK tol, int8_t *labels,
                             float *lam, unsigned long long *stats, bool stats_zeroed);
@@ -405,13 +376,11 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     const double *dtab = pk_tables_dev(ctx, N, (int)o->mas.scheme, o->mas_correction, &s);
     if (!dtab) return s;
 
-    // z pass with the mesh sum, then the FP64 total
+    // z pass with the mesh sum; the y pass's first workgroup takes the FP64
+    // total and clears the shell sums and T-web counters of the passes after it
     if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part, pitch)) != CM_OK) return s;
-    if ((s = launch(ctx, "fused_sum", k_fused_sum, dim3(1), dim3(1024), 0, (const double *)part, nparts,
-                    (double *)dnobj, dsum, dcnt, ns, dstats)) != CM_OK)
-        return s;
-    // y pass
-    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch)) != CM_OK) return s;
+    const SumTail tail{(const double *)part, nparts, (double *)dnobj, dsum, dcnt, ns, dstats};
+    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch, nullptr, &tail)) != CM_OK) return s;
     // interlaced mesh (particles shifted by d/2 on every axis): its full
     // spectrum, multiplied by exp(+i k.(d/2)(1,1,1)) = prod_a exp(i pi k_a/N)
     // one axis per pass, is averaged with the first in the fused x pass
