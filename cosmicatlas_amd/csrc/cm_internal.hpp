@@ -199,10 +199,12 @@ cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
 // phase (float32 only, nullable): per-frequency factors float2[N] multiplied
 // into the output of the pass (kz index for the z pass, ky for the y pass).
 // tail (float32 only, nullable): a reduction and clears that workgroup (0,0)
-// of the pass runs after its transform (SumTail).
+// of the pass runs after its transform (SumTail).  stream_loads (float32, y
+// axis, N >= 512): read the input with the nontemporal hint (the fused
+// measurement's y passes, where it speeds up the passes after them).
 struct SumTail;
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch = 0,
-                   const void *phase = nullptr, const SumTail *tail = nullptr);
+                   const void *phase = nullptr, const SumTail *tail = nullptr, bool stream_loads = false);
 cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
                        double subtract, double *part, int pitch = 0, const void *phase = nullptr);
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_r2c_ct(const float *__
     const float2 *inrow = reinterpret_cast<const float2 *>(in + row0 * (size_t)N);
     float acc = 0.f;  // at most E values per thread; the workgroup total is FP64
     auto src = [&](int b, int m) {
-        float2 z = load_stream(&inrow[b * M + m]);
+        float2 z = load_pass<N>(&inrow[b * M + m]);
         acc += z.x + z.y;
         z.x -= sub;
         z.y -= sub;
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
         const int t = threadIdx.x + i * NT;
         if (t < NB * (M + 1)) {
             const int b = t / (M + 1), k = t - b * (M + 1);
-            ld[i] = load_stream(&irow[(size_t)b * pitch + k]);
+            ld[i] = load_pass<N>(&irow[(size_t)b * pitch + k]);
         }
     }
 #pragma unroll
@@ -286,12 +286,21 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
 template <int N, int NB, bool FWD>
 __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__restrict__ data, int ncols, size_t stride,
                                                                  size_t planeStride, const float2 *__restrict__ tw,
-                                                                 const float2 *__restrict__ phase, SumTail tail) {
+                                                                 const float2 *__restrict__ phase, SumTail tail,
+                                                                 int ntl) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     const size_t base = (size_t)blockIdx.y * planeStride + (size_t)blockIdx.x * NB;
     const int valid = min(NB, ncols - (int)blockIdx.x * NB);
-    auto src = [&](int b, int e) { return b < valid ? load_stream(&data[base + (size_t)e * stride + b]) : float2{0.f, 0.f}; };
+    // ntl (uniform): nontemporal loads (the fused measurement's y passes of
+    // meshes from 512^3 up; in the bispectrum's shell transforms they made
+    // the column passes slower, the x passes (columns of planes 1 MB apart)
+    // 2.2x)
+    auto src = [&](int b, int e) {
+        if (b >= valid) return float2{0.f, 0.f};
+        const float2 *q = &data[base + (size_t)e * stride + b];
+        return ntl ? load_stream(q) : *q;
+    };
     auto dst = [&](int b, int e, float2 v) {
         if (phase) v = cmul(v, phase[e]);
         if (b < valid) store_stream(&data[base + (size_t)e * stride + b], v);
@@ -326,7 +335,7 @@ static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitc
 
 template <int N, bool FWD>
 static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw, const float2 *phase,
-                         const SumTail *tail) {
+                         const SumTail *tail, bool stream_loads) {
     constexpr int NB = col_nb<N>();
     using CS = ColShape<N, NB>;
     if (CS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: LDS");
@@ -343,7 +352,7 @@ static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const f
     const char *name = axis == 1 ? (FWD ? "fft_y_fwd" : "fft_y_inv") : (FWD ? "fft_x_fwd" : "fft_x_inv");
     const SumTail t = tail ? *tail : SumTail{nullptr, 0, nullptr, nullptr, nullptr, 0, nullptr};
     return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw,
-                  phase, t);
+                  phase, t, (stream_loads && N >= 512 && axis == 1) ? 1 : 0);
 }
 
 static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **twN) {
@@ -388,7 +397,7 @@ static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int
 }
 
 static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch, const float2 *phase,
-                          const SumTail *tail) {
+                          const SumTail *tail, bool stream_loads) {
     cm_status st;
     const float2 *tw = (const float2 *)twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;
@@ -396,8 +405,8 @@ static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward
     switch (N) {
 #define CM_CASE(n)                                                                     \
     case n:                                                                            \
-        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase, tail)                                  \
-                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase, tail);
+        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase, tail, stream_loads)                    \
+                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase, tail, stream_loads);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -564,9 +573,10 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 }
 
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch,
-                   const void *phase, const SumTail *tail) {
+                   const void *phase, const SumTail *tail, bool stream_loads) {
     if (dt == CM_F32)
-        return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1, (const float2 *)phase, tail);
+        return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1, (const float2 *)phase, tail,
+                        stream_loads);
     if (phase || tail) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: phase or tail on an FP64 spectrum");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return forward ? cols_t<double, true>(ctx, N, data, axis) : cols_t<double, false>(ctx, N, data, axis);

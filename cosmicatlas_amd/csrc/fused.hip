@@ -33,7 +33,7 @@ __device__ __attribute__((noinline)) void x_fft_fwd_to_lds(const float2 *__restr
                                                             const float2 *__restrict__ tw) {
     auto srcG = [&](int b, int e) -> float2 {
         if (b >= valid) return float2{0.f, 0.f};
-        return load_stream(&data[base + (size_t)e * xstride + b]);
+        return load_pass<N>(&data[base + (size_t)e * xstride + b]);
     };
     auto dstL = [&](int b, int e, float2 v) { lds[e * (NB + 1) + b] = v; };
     ct_fft<float, N, NB, NB + 1, true, true, E, TWL>(lds, tw, srcG, dstL);
@@ -380,7 +380,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     // total and clears the shell sums and T-web counters of the passes after it
     if ((s = fft_rows_r2c(ctx, N, CM_F32, mesh, dk, mean_est, (double *)part, pitch)) != CM_OK) return s;
     const SumTail tail{(const double *)part, nparts, (double *)dnobj, dsum, dcnt, ns, dstats};
-    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch, nullptr, &tail)) != CM_OK) return s;
+    if ((s = fft_cols(ctx, N, CM_F32, dk, 1, true, pitch, nullptr, &tail, true)) != CM_OK) return s;
     // interlaced mesh (particles shifted by d/2 on every axis): its full
     // spectrum, multiplied by exp(+i k.(d/2)(1,1,1)) = prod_a exp(i pi k_a/N)
     // one axis per pass, is averaged with the first in the fused x pass
@@ -469,7 +469,7 @@ cm_status fused_measure_mesh(cm_ctx *ctx, const cm_measure_opts *o, void *mesh, 
     if (do_tweb) {
         void *phi = ext ? ext->phi : nullptr;
         if (!phi && (s = scratch(ctx, "m.phi", sizeof(float) * n3, &phi)) != CM_OK) return s;
-        if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false, pitch)) != CM_OK) return s;
+        if ((s = fft_cols(ctx, N, CM_F32, dk, 1, false, pitch, nullptr, nullptr, true)) != CM_OK) return s;
         // ||phi||_rms (for a tolerance derived from it): sums of squares ride in
         // the c2r pass (the z pass's partial sums are consumed by now)
         const bool rms = o->margin_tol < 0.0;

@@ -39,15 +39,25 @@ __device__ __forceinline__ void store_stream(float2 *p, float2 v) {
 }
 
 __device__ __forceinline__ void store_stream(float *p, float v) { __builtin_nontemporal_store(v, (gmem<float> *)p); }
-// Load with the nontemporal hint, for a pass input read once (the z, y and
-// fused x passes of the spectrum).  512^3, all five passes together: 2.42 ->
-// 2.37 ms per measurement (r2c 0.227 -> 0.218, c2r 0.229 -> 0.203, fused x
-// 0.414 -> 0.397, the forward y pass 0.218 -> 0.226 ms); with the hint on the
-// r2c pass alone, the forward y pass after it took 0.242 ms.
+// Load with the nontemporal hint, for a pass input read once.
 __device__ __forceinline__ float2 load_stream(const float2 *p) {
     typedef float v2f __attribute__((ext_vector_type(2)));
     const v2f v = __builtin_nontemporal_load((const gmem<v2f> *)p);
     return float2{v.x, v.y};
+}
+
+// Input of a spectrum pass of an N^3 mesh: nontemporal from N = 512 up, whose
+// arrays (0.54 GB) exceed the memory-side cache (512^3, all five passes:
+// 2.42 -> 2.37 ms per measurement; r2c 0.227 -> 0.218, c2r 0.229 -> 0.203,
+// fused x 0.414 -> 0.397, forward y 0.218 -> 0.226 ms), plain below: a 256^3
+// spectrum (68 MB) is read back from that cache, and the hint took the
+// generator's 256^3 column passes from 0.045 to 0.080 ms.
+template <int N>
+__device__ __forceinline__ float2 load_pass(const float2 *p) {
+    if constexpr (N >= 512)
+        return load_stream(p);
+    else
+        return *p;
 }
 
 // LDS view of a generic pointer that points into the workgroup's shared
