@@ -559,7 +559,8 @@ def rows(cm):
         "value": round(1.0 / t, 3), "unit": "bispectra/s", "ms_per_call": round(1e3 * t, 3),
         "kernels_ms_per_call": {k: round(v["mean_ms"] * v["launches_per_call"], 4) for k, v in kt.items()},
         "roofline": _hbm_roof("bk_sums", kt, 3 * nsb * N ** 3 * 4),
-        "roofline_note": "bk_sums reads the three float32 fields of every shell once (12 ns N^3 bytes)",
+        "roofline_note": "bk_sums reads the three float32 fields of every shell once (12 ns N^3 bytes); it is bound "
+                         "by VALU issue, not HBM: bins in tiles of 2 j x 4 k shells, 8 VALU per cell and tile slot",
         "cpu_baseline": None,
         "cpu_note": "no CPU restatement of the shell-transform estimator; the direct estimator below carries the CPU leg"}
     del mesh, dkd

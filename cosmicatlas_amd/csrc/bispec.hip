@@ -13,8 +13,8 @@
 // 1/nbar^2, with the triangle average taken the same way.  normal_b = n^3/V^2
 // (raw_sampling, :159-174).
 //
-// One kernel holds, for a chunk of 128 cells, the three fields of every shell
-// in LDS and accumulates all triangle bins at once, so each field is read once.
+// One kernel holds, for a chunk of cells, the three fields of every shell in
+// LDS and accumulates all triangle bins at once, so each field is read once.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -67,79 +67,148 @@ __global__ void __launch_bounds__(256) k_bk_masks(const float2 *__restrict__ dk,
     }
 }
 
-constexpr int kBkChunk = 128;
-// LDS row pitch of one field: one word of padding, so that lanes reading the
-// same cell of different shells hit different banks
-constexpr int kBkPitch = kBkChunk + 1;
+// Triangle sums over cells, tiled over the bins.  A tile holds the bins
+// (i, j, k) of one shell i, two consecutive shells j and four consecutive
+// shells k (slots not in the bin list are computed and never read).  Per cell
+// it forms the pair products DD = D_i D_j, UU = U_i U_j and Q = P_i U_j + U_i P_j
+// once per j and adds, for every k,
+//   DDD += DD D_k,   UUU += UU U_k,   PUU += Q U_k + UU P_k
+// (PUU is the cyclic sum P_i U_j U_k + U_i P_j U_k + U_i U_j P_k).  A workgroup
+// holds a chunk of C cells of all 3 ns fields in LDS (row f: C floats); each
+// wave takes every nw-th tile, each lane two adjacent cells per step, so the
+// 24 sums of a tile sit in registers for the whole chunk and are reduced
+// across the wave once per chunk (two lane-half swaps, then row sums) into
+// FP64 tile sums in LDS.  part[blockIdx][tile][slot][3] receives them.
+constexpr int kBkJW = 2, kBkKW = 4, kBkSlots = kBkJW * kBkKW, kBkAcc = 3 * kBkSlots;
 
-// Triangle sums over cells.  fields: [3][ns][n3] (D, U, P); tri: (i, j, k) per
-// bin; part[blockIdx][bin][3] = partial sums of DDD, UUU and the cyclic PUU.
-__global__ void __launch_bounds__(256) k_bk_sums(const float *__restrict__ fields, int ns, size_t n3,
-                                                 const int4 *__restrict__ tri, int nbins,
-                                                 double *__restrict__ part) {
-    extern __shared__ float sf[];  // [3][ns][kBkPitch]
-    const int tid = threadIdx.x;
-    const int NT = blockDim.x;
-    // each thread owns bins tid, tid + NT, ... (at most 8 per thread here)
-    double acc[8][3];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) acc[a][0] = acc[a][1] = acc[a][2] = 0.0;
-    const size_t nchunks = (n3 + kBkChunk - 1) / kBkChunk;
-    for (size_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const size_t x0 = c * kBkChunk;
-        const int len = (int)std::min<size_t>(kBkChunk, n3 - x0);
-        __syncthreads();
-        for (int t = tid; t < 3 * ns * kBkChunk; t += NT) {
-            const int f = t / kBkChunk, xi = t % kBkChunk;
-            sf[f * kBkPitch + xi] = xi < len ? fields[(size_t)f * n3 + x0 + xi] : 0.f;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const int b = tid + a * NT;
-            if (b >= nbins) break;
-            const int4 t = tri[b];
-            const float *Di = sf + t.x * kBkPitch, *Dj = sf + t.y * kBkPitch, *Dk = sf + t.z * kBkPitch;
-            const float *Ui = sf + (ns + t.x) * kBkPitch, *Uj = sf + (ns + t.y) * kBkPitch,
-                        *Uk = sf + (ns + t.z) * kBkPitch;
-            const float *Pi = sf + (2 * ns + t.x) * kBkPitch, *Pj = sf + (2 * ns + t.y) * kBkPitch,
-                        *Pk = sf + (2 * ns + t.z) * kBkPitch;
-            float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-            for (int xi = 0; xi < kBkChunk; ++xi) {
-                const float ui = Ui[xi], uj = Uj[xi], uk = Uk[xi];
-                s0 += Di[xi] * Dj[xi] * Dk[xi];
-                s1 += ui * uj * uk;
-                s2 += Pi[xi] * uj * uk + Pj[xi] * ui * uk + Pk[xi] * ui * uj;
-                if ((xi & 15) == 15) {  // FP64 every 16 terms
-                    d0 += s0;
-                    d1 += s1;
-                    d2 += s2;
-                    s0 = s1 = s2 = 0.f;
-                }
-            }
-            acc[a][0] += d0;
-            acc[a][1] += d1;
-            acc[a][2] += d2;
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const int b = tid + a * NT;
-        if (b >= nbins) break;
-        double *o = part + ((size_t)blockIdx.x * nbins + b) * 3;
-        o[0] = acc[a][0];
-        o[1] = acc[a][1];
-        o[2] = acc[a][2];
+struct alignas(32) BkTile {
+    int i, j[kBkJW], k[kBkKW], pad;  // shell indices (clamped to ns - 1)
+};
+
+__device__ __forceinline__ float bk_dpp_add(float v, int ctrl) {
+    // v + (v of the lane that ctrl names); quad_perm, row_half_mirror and
+    // row_mirror patterns stay inside a row of 16 lanes
+    switch (ctrl) {
+        case 0xB1: return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+        case 0x4E: return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
+        case 0x141: return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true));
+        default: return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, true));
     }
 }
 
-__global__ void __launch_bounds__(256) k_bk_reduce(const double *__restrict__ part, int nblk, int nbins,
+template <int C>
+__global__ void __launch_bounds__(1024) k_bk_sums(const float *__restrict__ fields, int ns, size_t n3,
+                                                  const BkTile *__restrict__ tiles, int ntiles,
+                                                  double *__restrict__ part) {
+    static_assert(C % 128 == 0, "two cells per lane per step");
+    extern __shared__ __attribute__((aligned(16))) float sf[];  // [3 ns][C], then double [ntiles][kBkAcc]
+    const int nf = 3 * ns, nth = blockDim.x;
+    double *accd = (double *)(sf + (size_t)nf * C);
+    for (int q = threadIdx.x; q < ntiles * kBkAcc; q += nth) accd[q] = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = nth >> 6;
+    const size_t nchunks = (n3 + C - 1) / C;
+    for (size_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const size_t x0 = c * C;
+        __syncthreads();
+        if ((n3 & 3) == 0 && x0 + C <= n3) {
+            for (int t = threadIdx.x; t < nf * (C / 4); t += nth) {
+                const int f = t / (C / 4), q = t % (C / 4);
+                *(float4 *)(sf + f * C + 4 * q) = *(const float4 *)(fields + (size_t)f * n3 + x0 + 4 * q);
+            }
+        } else {
+            for (int t = threadIdx.x; t < nf * C; t += nth) {
+                const int f = t / C, xi = t % C;
+                sf[f * C + xi] = x0 + xi < n3 ? fields[(size_t)f * n3 + x0 + xi] : 0.f;
+            }
+        }
+        __syncthreads();
+        for (int t = wave; t < ntiles; t += nw) {
+            const BkTile tl = tiles[t];
+            const float2 *base = (const float2 *)sf + lane;
+            const float2 *Di = base + tl.i * (C / 2), *Ui = base + (ns + tl.i) * (C / 2),
+                         *Pi = base + (2 * ns + tl.i) * (C / 2);
+            const float2 *Dj[kBkJW], *Uj[kBkJW], *Pj[kBkJW], *Dk[kBkKW], *Uk[kBkKW], *Pk[kBkKW];
+#pragma unroll
+            for (int a = 0; a < kBkJW; ++a) {
+                Dj[a] = base + tl.j[a] * (C / 2);
+                Uj[a] = base + (ns + tl.j[a]) * (C / 2);
+                Pj[a] = base + (2 * ns + tl.j[a]) * (C / 2);
+            }
+#pragma unroll
+            for (int a = 0; a < kBkKW; ++a) {
+                Dk[a] = base + tl.k[a] * (C / 2);
+                Uk[a] = base + (ns + tl.k[a]) * (C / 2);
+                Pk[a] = base + (2 * ns + tl.k[a]) * (C / 2);
+            }
+            float acc[kBkAcc];
+#pragma unroll
+            for (int a = 0; a < kBkAcc; ++a) acc[a] = 0.f;
+#pragma unroll
+            for (int st = 0; st < C / 128; ++st) {
+                const int o = st * 64;
+                const float2 di = Di[o], ui = Ui[o], pi = Pi[o];
+                float2 dk[kBkKW], uk[kBkKW], pk[kBkKW];
+#pragma unroll
+                for (int b = 0; b < kBkKW; ++b) {
+                    dk[b] = Dk[b][o];
+                    uk[b] = Uk[b][o];
+                    pk[b] = Pk[b][o];
+                }
+#pragma unroll
+                for (int a = 0; a < kBkJW; ++a) {
+                    const float2 dj = Dj[a][o], uj = Uj[a][o], pj = Pj[a][o];
+                    const float ddx = di.x * dj.x, uux = ui.x * uj.x, qx = fmaf(pi.x, uj.x, ui.x * pj.x);
+                    const float ddy = di.y * dj.y, uuy = ui.y * uj.y, qy = fmaf(pi.y, uj.y, ui.y * pj.y);
+#pragma unroll
+                    for (int b = 0; b < kBkKW; ++b) {
+                        float *r = acc + 3 * (a * kBkKW + b);
+                        r[0] = fmaf(ddy, dk[b].y, fmaf(ddx, dk[b].x, r[0]));
+                        r[1] = fmaf(uuy, uk[b].y, fmaf(uux, uk[b].x, r[1]));
+                        r[2] = fmaf(qy, uk[b].y, fmaf(uuy, pk[b].y, fmaf(qx, uk[b].x, fmaf(uux, pk[b].x, r[2]))));
+                    }
+                }
+            }
+            // 24 sums x 64 lanes -> lane 16 r + a holds sum a + 6 r (a < 6):
+            // lanes 32-63 of acc[a] trade with lanes 0-31 of acc[a + 12], then
+            // odd rows of r1[a] with even rows of r1[a + 6]; rows then sum up
+            float r1[kBkAcc / 2], r2[kBkAcc / 4];
+#pragma unroll
+            for (int a = 0; a < kBkAcc / 2; ++a) {
+                const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[a]),
+                                                                __float_as_uint(acc[a + kBkAcc / 2]), false, false);
+                r1[a] = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+            }
+#pragma unroll
+            for (int a = 0; a < kBkAcc / 4; ++a) {
+                const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(r1[a]),
+                                                                __float_as_uint(r1[a + kBkAcc / 4]), false, false);
+                float v = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+                v = bk_dpp_add(v, 0xB1);
+                v = bk_dpp_add(v, 0x4E);
+                v = bk_dpp_add(v, 0x141);
+                r2[a] = bk_dpp_add(v, 0x140);
+            }
+            const int a16 = lane & 15;
+            float mine = r2[0];
+#pragma unroll
+            for (int a = 1; a < kBkAcc / 4; ++a) mine = a16 == a ? r2[a] : mine;
+            if (a16 < kBkAcc / 4) accd[t * kBkAcc + a16 + (kBkAcc / 4) * (lane >> 4)] += (double)mine;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < ntiles * kBkAcc; q += nth) part[(size_t)blockIdx.x * ntiles * kBkAcc + q] = accd[q];
+}
+
+// out[3 b + q] = sum over workgroups of the tile sum at slot pos[b].
+__global__ void __launch_bounds__(256) k_bk_reduce(const double *__restrict__ part, int nblk, int rowlen,
+                                                   const int *__restrict__ pos, int nbins,
                                                    double *__restrict__ out) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= 3 * nbins) return;
+    const int at = 3 * pos[b / 3] + b % 3;
     double s = 0.0;
-    for (int g = 0; g < nblk; ++g) s += part[(size_t)g * 3 * nbins + b];
+    for (int g = 0; g < nblk; ++g) s += part[(size_t)g * rowlen + at];
     out[b] = s;
 }
 
@@ -155,6 +224,44 @@ static void triangle_bins(int ns, int smin, std::vector<int4> &tri) {
             if (nstep3 <= 0) kstart = smin;
             for (int k = kstart; k <= j; ++k) tri.push_back(int4{i, j, k, 0});
         }
+    }
+}
+
+// Tiles of kBkJW x kBkKW bins over the bin list (bins of one shell i; j
+// windows from the first j of that i, k windows from the first k of the j
+// window); pos[b] = tile * kBkSlots + slot of bin b.
+static void bin_tiles(const std::vector<int4> &tri, int ns, std::vector<BkTile> &tiles, std::vector<int> &pos) {
+    tiles.clear();
+    pos.assign(tri.size(), -1);
+    size_t b0 = 0;
+    while (b0 < tri.size()) {
+        const int i = tri[b0].x;
+        size_t b1 = b0;
+        while (b1 < tri.size() && tri[b1].x == i) ++b1;
+        for (int j0 = tri[b0].y; j0 <= i; j0 += kBkJW) {
+            int kmin = ns, kmax = -1;
+            for (size_t b = b0; b < b1; ++b)
+                if (tri[b].y >= j0 && tri[b].y < j0 + kBkJW) {
+                    kmin = std::min(kmin, tri[b].z);
+                    kmax = std::max(kmax, tri[b].z);
+                }
+            for (int k0 = kmin; k0 <= kmax; k0 += kBkKW) {
+                BkTile t{};
+                t.i = i;
+                for (int a = 0; a < kBkJW; ++a) t.j[a] = std::min(j0 + a, ns - 1);
+                for (int a = 0; a < kBkKW; ++a) t.k[a] = std::min(k0 + a, ns - 1);
+                bool any = false;
+                for (size_t b = b0; b < b1; ++b) {
+                    const int jj = tri[b].y - j0, kk = tri[b].z - k0;
+                    if (jj >= 0 && jj < kBkJW && kk >= 0 && kk < kBkKW) {
+                        pos[b] = (int)tiles.size() * kBkSlots + jj * kBkKW + kk;
+                        any = true;
+                    }
+                }
+                if (any) tiles.push_back(t);
+            }
+        }
+        b0 = b1;
     }
 }
 
@@ -178,8 +285,24 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
     if (max_bins < nb || !ijk || !bk || !sn || !ntri)
         return fail(ctx, CM_ERR_INVALID, "cm_bispectrum: output arrays hold %d bins, %d needed", max_bins, nb);
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * (N / 2 + 1);
-    const size_t lds = sizeof(float) * 3 * (size_t)ns * kBkPitch;
-    if (lds > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum: %d shells need %zu B LDS", ns, lds);
+    std::vector<BkTile> tiles;
+    std::vector<int> pos;
+    bin_tiles(tri, ns, tiles, pos);
+    const int ntl = (int)tiles.size();
+    for (int v : pos)
+        if (v < 0) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum: triangle bin outside the tiles");
+    // cells per chunk: the largest of 512, 256, 128 whose fields and FP64 tile
+    // sums fit the LDS of one workgroup
+    int C = 0;
+    size_t lds = 0;
+    for (int c : {512, 256, 128}) {
+        lds = sizeof(float) * 3 * (size_t)ns * c + sizeof(double) * kBkAcc * (size_t)ntl;
+        if (lds <= ctx->lds_per_block) {
+            C = c;
+            break;
+        }
+    }
+    if (!C) return fail(ctx, CM_ERR_UNSUPPORTED, "cm_bispectrum: %d shells need %zu B LDS", ns, lds);
     cm_status s;
     void *fields, *specs, *dtri, *part, *dsum;
     if ((s = scratch(ctx, "bk.fields", sizeof(float) * 3 * (size_t)ns * n3, &fields)) != CM_OK) return s;
@@ -214,18 +337,22 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
                 CM_OK)
                 return s;
     }
-    if ((s = scratch(ctx, "bk.tri", sizeof(int4) * (size_t)nb, &dtri)) != CM_OK) return s;
-    CM_HIP(ctx, hipMemcpyAsync(dtri, tri.data(), sizeof(int4) * (size_t)nb, hipMemcpyHostToDevice, ctx->stream));
-    const int nthreads = 256;
-    const size_t nchunks = (n3 + kBkChunk - 1) / kBkChunk;
-    const int nblk = (int)std::min<size_t>(nchunks, (size_t)std::max(1, ctx->cu_count) * 4);
-    if ((s = scratch(ctx, "bk.part", sizeof(double) * 3 * (size_t)nblk * nb, &part)) != CM_OK) return s;
+    void *dpos;
+    if ((s = scratch(ctx, "bk.tri", sizeof(BkTile) * (size_t)ntl, &dtri)) != CM_OK) return s;
+    if ((s = scratch(ctx, "bk.pos", sizeof(int) * (size_t)nb, &dpos)) != CM_OK) return s;
+    CM_HIP(ctx, host_upload(ctx, dtri, tiles.data(), sizeof(BkTile) * (size_t)ntl));
+    CM_HIP(ctx, host_upload(ctx, dpos, pos.data(), sizeof(int) * (size_t)nb));
+    const size_t nchunks = (n3 + C - 1) / C;
+    const int nblk = (int)std::min<size_t>(nchunks, (size_t)std::max(1, ctx->cu_count));
+    const int rowlen = kBkAcc * ntl;
+    if ((s = scratch(ctx, "bk.part", sizeof(double) * (size_t)nblk * rowlen, &part)) != CM_OK) return s;
     if ((s = scratch(ctx, "bk.sum", sizeof(double) * 3 * (size_t)nb, &dsum)) != CM_OK) return s;
-    if ((s = launch(ctx, "bk_sums", k_bk_sums, dim3(nblk), dim3(nthreads), lds, (const float *)fl, ns, n3,
-                    (const int4 *)dtri, nb, (double *)part)) != CM_OK)
+    auto kern = C == 512 ? k_bk_sums<512> : C == 256 ? k_bk_sums<256> : k_bk_sums<128>;
+    if ((s = launch(ctx, "bk_sums", kern, dim3(nblk), dim3(1024), lds, (const float *)fl, ns, n3,
+                    (const BkTile *)dtri, ntl, (double *)part)) != CM_OK)
         return s;
     if ((s = launch(ctx, "bk_reduce", k_bk_reduce, dim3((3 * nb + 255) / 256), dim3(256), 0, (const double *)part, nblk,
-                    nb, (double *)dsum)) != CM_OK)
+                    rowlen, (const int *)dpos, nb, (double *)dsum)) != CM_OK)
         return s;
     std::vector<double> h(3 * (size_t)nb);
     CM_HIP(ctx, hipMemcpyAsync(h.data(), dsum, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
