@@ -287,10 +287,13 @@ template <int N, int NB, bool FWD>
 __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__restrict__ data, int ncols, size_t stride,
                                                                  size_t planeStride, const float2 *__restrict__ tw,
                                                                  const float2 *__restrict__ phase, SumTail tail,
-                                                                 int ntl) {
+                                                                 int ntl, int pk, int pskip) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
-    const size_t base = (size_t)blockIdx.y * planeStride + (size_t)blockIdx.x * NB;
+    // planes above pk are offset by pskip (a pass over the two ends of the
+    // plane range only)
+    const size_t plane = blockIdx.y + ((int)blockIdx.y > pk ? pskip : 0);
+    const size_t base = plane * planeStride + (size_t)blockIdx.x * NB;
     const int valid = min(NB, ncols - (int)blockIdx.x * NB);
     // ntl (uniform): nontemporal loads (the fused measurement's y passes of
     // meshes from 512^3 up; in the bispectrum's shell transforms they made
@@ -335,11 +338,18 @@ static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitc
 
 template <int N, bool FWD>
 static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const float2 *tw, const float2 *phase,
-                         const SumTail *tail, bool stream_loads) {
+                         const SumTail *tail, bool stream_loads, int ncols_used, int band) {
     constexpr int NB = col_nb<N>();
     using CS = ColShape<N, NB>;
     if (CS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: LDS");
-    const int ncols = N / 2 + 1;
+    const int ncols = ncols_used > 0 ? std::min(ncols_used, N / 2 + 1) : N / 2 + 1;
+    unsigned planes = (unsigned)N;
+    int pk = N, pskip = 0;
+    if (axis == 0 && band >= 0 && 2 * band + 1 < N) {  // y planes 0..band and N-band..N-1
+        planes = (unsigned)(2 * band + 1);
+        pk = band;
+        pskip = N - 2 * band - 1;
+    }
     size_t stride, planeStride;
     if (axis == 1) {  // y: fixed x plane, stride over kz rows
         stride = (size_t)pitch;
@@ -348,11 +358,11 @@ static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const f
         stride = (size_t)N * pitch;
         planeStride = (size_t)pitch;
     }
-    dim3 grid((unsigned)((ncols + NB - 1) / NB), (unsigned)N);
+    dim3 grid((unsigned)((ncols + NB - 1) / NB), planes);
     const char *name = axis == 1 ? (FWD ? "fft_y_fwd" : "fft_y_inv") : (FWD ? "fft_x_fwd" : "fft_x_inv");
     const SumTail t = tail ? *tail : SumTail{nullptr, 0, nullptr, nullptr, nullptr, 0, nullptr};
     return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw,
-                  phase, t, (stream_loads && N >= 512 && axis == 1) ? 1 : 0);
+                  phase, t, (stream_loads && N >= 512 && axis == 1) ? 1 : 0, pk, pskip);
 }
 
 static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **twN) {
@@ -397,7 +407,7 @@ static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int
 }
 
 static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward, int pitch, const float2 *phase,
-                          const SumTail *tail, bool stream_loads) {
+                          const SumTail *tail, bool stream_loads, int ncols, int band) {
     cm_status st;
     const float2 *tw = (const float2 *)twiddles(ctx, N, CM_F32, &st);
     if (!tw) return st;
@@ -405,8 +415,8 @@ static cm_status cols_f32(cm_ctx *ctx, int N, void *data, int axis, bool forward
     switch (N) {
 #define CM_CASE(n)                                                                     \
     case n:                                                                            \
-        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase, tail, stream_loads)                    \
-                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase, tail, stream_loads);
+        return forward ? cols_ct<n, true>(ctx, dp, axis, pitch, tw, phase, tail, stream_loads, ncols, band)       \
+                       : cols_ct<n, false>(ctx, dp, axis, pitch, tw, phase, tail, stream_loads, ncols, band);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -573,10 +583,10 @@ cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 }
 
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch,
-                   const void *phase, const SumTail *tail, bool stream_loads) {
+                   const void *phase, const SumTail *tail, bool stream_loads, int ncols, int band) {
     if (dt == CM_F32)
         return cols_f32(ctx, N, data, axis, forward, pitch > 0 ? pitch : N / 2 + 1, (const float2 *)phase, tail,
-                        stream_loads);
+                        stream_loads, ncols, band);
     if (phase || tail) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: phase or tail on an FP64 spectrum");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return forward ? cols_t<double, true>(ctx, N, data, axis) : cols_t<double, false>(ctx, N, data, axis);
@@ -593,6 +603,14 @@ cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
     cm_status s = fft_cols(ctx, N, dt, in, 0, false);
     if (s != CM_OK) return s;
     if ((s = fft_cols(ctx, N, dt, in, 1, false)) != CM_OK) return s;
+    return fft_rows_c2r(ctx, N, dt, in, out, scale);
+}
+
+cm_status fft_c2r_band(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale, int K) {
+    if (K < 0 || K >= N / 2) return fft_c2r_impl(ctx, N, dt, in, out, scale);
+    cm_status s = fft_cols(ctx, N, dt, in, 0, false, 0, nullptr, nullptr, false, K + 1, K);
+    if (s != CM_OK) return s;
+    if ((s = fft_cols(ctx, N, dt, in, 1, false, 0, nullptr, nullptr, false, K + 1)) != CM_OK) return s;
     return fft_rows_c2r(ctx, N, dt, in, out, scale);
 }
 
