@@ -329,12 +329,12 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
     auto *fl = (float *)fields;
     size_t g = (nh + 255) / 256;
     if (g > 16384) g = 16384;
-    // every shell mode has |kx|, |ky|, kz < Dmin + ns ndel <= K: the first
-    // shell writes the whole spectra (zeros outside the shell), later shells
-    // only kz <= K, where the band-limited c2r leaves its results
+    // every shell mode has |kx|, |ky|, kz < Dmin + ns ndel <= K: the masks
+    // write the modes kz <= K (zeros outside the shell), the only ones the
+    // band-limited c2r reads
     const int K = std::min(N / 2, (int)std::ceil(Dmin + ns * ndel));
+    const int nz = std::min(K + 1, N / 2 + 1);
     for (int sh = 0; sh < ns; ++sh) {
-        const int nz = sh == 0 ? N / 2 + 1 : std::min(K + 1, N / 2 + 1);
         if ((s = launch(ctx, "bk_masks", k_bk_masks, dim3((unsigned)g), dim3(256), 0, (const float2 *)dk, N, sh,
                         int_rule, (int)Di, (int)ndi, Dmin, ndel, dtab, sn_norm, 1.0 / normal, sp,
                         sp + nh, sp + 2 * nh, nz)) != CM_OK)

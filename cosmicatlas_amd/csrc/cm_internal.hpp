@@ -195,9 +195,9 @@ cm_status fft_r2c_impl(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale);
 // c2r of a spectrum that is zero wherever a wave-vector component exceeds K
 // (|kx|, |ky|, kz <= K < N/2 hold every non-zero mode): the x pass transforms
-// only the lines with |ky| <= K and kz <= K, the y pass only kz <= K; the
-// lines left out are zero before and after.  The input keeps non-zero values
-// only at kz <= K afterwards.
+// only the lines with |ky| <= K and kz <= K, the y pass only kz <= K, the z
+// pass reads kz <= K only.  Modes kz > K of the input are never read, nor
+// written.
 cm_status fft_c2r_band(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale, int K);
 // Strided column passes over the half spectrum, axis 1 (y) or 0 (x).  pitch:
 // kz row pitch of the spectrum in complex values (0 = N/2+1, the FFTW layout;
@@ -218,9 +218,10 @@ cm_status fft_rows_r2c(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
                        double subtract, double *part, int pitch = 0, const void *phase = nullptr);
 int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt);
 // sq (float32 only, nullable): per-workgroup sums of squares of the output,
-// fft_rows_blocks() FP64 words.
+// fft_rows_blocks() FP64 words.  nk (float32, > 0): only the modes kz < nk of
+// each row are read, the others taken as zero.
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out,
-                       double scale, int pitch = 0, double *sq = nullptr);
+                       double scale, int pitch = 0, double *sq = nullptr, int nk = 0);
 
 // exact = false (tiled path only, needs overflow_out): the record buffer is
 // sized from an estimate instead of a read-back of the count (no host

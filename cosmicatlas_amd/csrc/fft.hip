@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
                                                                  float *__restrict__ out, int pitch, float scale,
                                                                  const float2 *__restrict__ twM,
                                                                  const float2 *__restrict__ twN,
-                                                                 double *__restrict__ sq) {
+                                                                 double *__restrict__ sq, int nk) {
     using RS = RowShape<N>;
     constexpr int M = RS::M, NB = RS::NB, STRIDE = RS::STRIDE, NT = RS::NT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
         const int t = threadIdx.x + i * NT;
         if (t < NB * (M + 1)) {
             const int b = t / (M + 1), k = t - b * (M + 1);
-            ld[i] = load_pass<N>(&irow[(size_t)b * pitch + k]);
+            ld[i] = k < nk ? load_pass<N>(&irow[(size_t)b * pitch + k]) : float2{0.f, 0.f};
         }
     }
 #pragma unroll
@@ -329,11 +329,11 @@ static cm_status rows_r2c_ct(cm_ctx *ctx, const float *in, float2 *out, int pitc
 
 template <int N>
 static cm_status rows_c2r_ct(cm_ctx *ctx, const float2 *in, float *out, int pitch, float scale, const float2 *twM,
-                             const float2 *twN, double *sq) {
+                             const float2 *twN, double *sq, int nk) {
     using RS = RowShape<N>;
     if (RS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft z pass: LDS");
     return launch(ctx, "fft_z_c2r", k_rows_c2r_ct<N>, dim3((unsigned)((size_t)N * N / RS::NB)), dim3(RS::NT), RS::LDS,
-                  in, out, pitch, scale, twM, twN, sq);
+                  in, out, pitch, scale, twM, twN, sq, nk > 0 ? nk : N / 2 + 1);
 }
 
 template <int N, bool FWD>
@@ -391,7 +391,7 @@ static cm_status rows_r2c_f32(cm_ctx *ctx, int N, const void *in, void *out, int
 }
 
 static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int pitch, double scale,
-                              double *sq) {
+                              double *sq, int nk) {
     const float2 *twM, *twN;
     cm_status s = tw_pair(ctx, N, &twM, &twN);
     if (s != CM_OK) return s;
@@ -399,7 +399,7 @@ static cm_status rows_c2r_f32(cm_ctx *ctx, int N, const void *in, void *out, int
     auto *op = (float *)out;
     switch (N) {
 #define CM_CASE(n) \
-    case n: return rows_c2r_ct<n>(ctx, ip, op, pitch, (float)scale, twM, twN, sq);
+    case n: return rows_c2r_ct<n>(ctx, ip, op, pitch, (float)scale, twM, twN, sq, nk);
         CM_FFT_SIZES(CM_CASE)
 #undef CM_CASE
     }
@@ -575,8 +575,8 @@ int fft_rows_blocks(cm_ctx *ctx, int N, cm_dtype dt) {
 }
 
 cm_status fft_rows_c2r(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *out, double scale, int pitch,
-                       double *sq) {
-    if (dt == CM_F32) return rows_c2r_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, scale, sq);
+                       double *sq, int nk) {
+    if (dt == CM_F32) return rows_c2r_f32(ctx, N, in, out, pitch > 0 ? pitch : N / 2 + 1, scale, sq, nk);
     if (sq) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: sum of squares of an FP64 c2r");
     if (pitch > 0 && pitch != N / 2 + 1) return fail(ctx, CM_ERR_UNSUPPORTED, "fft: padded FP64 spectrum");
     return rows_c2r_t<double>(ctx, N, in, out, scale);
@@ -611,7 +611,7 @@ cm_status fft_c2r_band(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
     cm_status s = fft_cols(ctx, N, dt, in, 0, false, 0, nullptr, nullptr, false, K + 1, K);
     if (s != CM_OK) return s;
     if ((s = fft_cols(ctx, N, dt, in, 1, false, 0, nullptr, nullptr, false, K + 1)) != CM_OK) return s;
-    return fft_rows_c2r(ctx, N, dt, in, out, scale);
+    return fft_rows_c2r(ctx, N, dt, in, out, scale, 0, nullptr, K + 1);
 }
 
 }  // namespace cm
