@@ -86,6 +86,31 @@ def test_single_particle_closed_form():
     assert np.allclose(out["bk"][ok], V ** 2, rtol=1e-4)
 
 
+def test_bin_subset_independent_of_shell_count():
+    """Bins (i, j, k) with i < 16 are the same triangles whether 16 or 24 shells
+    are asked for; 24 shells take 256-cell chunks in LDS, 16 shells 512-cell
+    chunks, so the two sums run through different tile and chunk layouts."""
+    import cosmicatlas_amd as cm
+    N, L = 96, 300.0
+    rng = np.random.default_rng(11)
+    mesh = rng.poisson(2.0, (N, N, N)).astype(np.float32)
+    kf = 2 * np.pi / L
+    with cm.Context(0) as ctx:
+        dk = ctx.fft_r2c(torch.from_numpy(mesh).cuda(), subtract=float(mesh.mean()))
+        n = float(mesh.sum())
+        a = ctx.bispectrum(dk, N, L, 16, 2 * kf, kf, "CIC", n)
+        b = ctx.bispectrum(dk, N, L, 24, 2 * kf, kf, "CIC", n)
+    sub = b["ijk"][:, 0] < 16
+    assert len(a["bk"]) == 507 and len(b["bk"]) == 1521 and sub.sum() == 507
+    np.testing.assert_array_equal(a["ijk"], b["ijk"][sub])
+    assert np.all(np.abs(a["ntri"] - b["ntri"][sub]) <= 1 + 1e-6 * a["ntri"])
+    # float32 sums in two orders: a bin's rounding scales with its terms, not with
+    # its (often cancelling) total, hence the floor of 1e-7 of the largest bin
+    scale = np.abs(a["bk"]).max()
+    assert np.all(np.abs(a["bk"] - b["bk"][sub]) <= 1e-5 * (np.abs(a["bk"]) + 1e-2 * scale))
+    assert np.all(np.abs(a["sn"] - b["sn"][sub]) <= 1e-5 * np.abs(a["sn"]).max())
+
+
 def test_shells_beyond_third_of_grid_rejected():
     import cosmicatlas_amd as cm
     N, L = 32, 100.0
