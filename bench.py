@@ -82,7 +82,8 @@ ROCPROF_NAMES = [("cm::k_tile_count_lds", "mas_tile_count"), ("cm::k_tile_count"
 def lib_name(kname):
     k = kname.replace("void ", "", 1)
     if k.startswith("cm::k_cols<") or k.startswith("cm::k_cols_ct<"):
-        return "fft_y_fwd" if ", true>" in k.split("(")[0] else "fft_y_inv"
+        targs = [a.strip() for a in k[k.index("<") + 1:k.index(">")].split(",")]
+        return "fft_y_fwd" if targs[2] == "true" else "fft_y_inv"
     for pre, name in ROCPROF_NAMES:
         if k.startswith(pre):
             return name

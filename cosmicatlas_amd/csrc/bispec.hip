@@ -30,15 +30,17 @@ __global__ void __launch_bounds__(256) k_bk_masks(const float2 *__restrict__ dk,
                                                   int Dmin_i, int ndel_i, double Dmin, double ndel,
                                                   const double *__restrict__ tabs, double sn_norm, double inv_normal,
                                                   float2 *__restrict__ fd, float2 *__restrict__ fu,
-                                                  float2 *__restrict__ fp, int nz) {
-    // writes the kz columns 0..nz-1 of every (kx, ky) row
-    const int M1 = N / 2 + 1, h = N / 2;
-    const size_t total = (size_t)N * N * nz;
+                                                  float2 *__restrict__ fp, int nz, int ny) {
+    // writes the kz columns 0..nz-1 of the rows (kx, ky) with ky in
+    // 0..(ny-1)/2 and N-(ny-1)/2..N-1 (ny odd, or ny = N: every row)
+    const int M1 = N / 2 + 1, h = N / 2, yh = (ny - 1) / 2;
+    const size_t total = (size_t)N * ny * nz;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const int iz = (int)(t % nz);
-        const size_t r = t / nz;
-        const size_t g = r * M1 + iz;
-        const int iy = (int)(r % N), ix = (int)(r / N);
+        const size_t rb = t / nz;
+        const int yb = (int)(rb % ny), ix = (int)(rb / ny);
+        const int iy = ny == N || yb <= yh ? yb : yb + N - ny;
+        const size_t g = ((size_t)ix * N + iy) * M1 + iz;
         const int fx = ix <= h ? ix : N - ix, fy = iy <= h ? iy : N - iy;
         const int q = fx * fx + fy * fy + iz * iz;
         bool in = false;
@@ -330,14 +332,14 @@ extern "C" cm_status cm_bispectrum(cm_ctx *ctx, const cm_bk_opts *o, const void 
     size_t g = (nh + 255) / 256;
     if (g > 16384) g = 16384;
     // every shell mode has |kx|, |ky|, kz < Dmin + ns ndel <= K: the masks
-    // write the modes kz <= K (zeros outside the shell), the only ones the
-    // band-limited c2r reads
+    // write the modes |ky| <= K, kz <= K (zeros outside the shell), the only
+    // ones the band-limited c2r reads
     const int K = std::min(N / 2, (int)std::ceil(Dmin + ns * ndel));
-    const int nz = std::min(K + 1, N / 2 + 1);
+    const int nz = std::min(K + 1, N / 2 + 1), ny = 2 * K + 1 < N ? 2 * K + 1 : N;
     for (int sh = 0; sh < ns; ++sh) {
         if ((s = launch(ctx, "bk_masks", k_bk_masks, dim3((unsigned)g), dim3(256), 0, (const float2 *)dk, N, sh,
                         int_rule, (int)Di, (int)ndi, Dmin, ndel, dtab, sn_norm, 1.0 / normal, sp,
-                        sp + nh, sp + 2 * nh, nz)) != CM_OK)
+                        sp + nh, sp + 2 * nh, nz, ny)) != CM_OK)
             return s;
         for (int f = 0; f < 3; ++f)
             if ((s = fft_c2r_band(ctx, N, CM_F32, sp + f * nh, fl + ((size_t)f * ns + sh) * n3, 1.0 / (double)n3,

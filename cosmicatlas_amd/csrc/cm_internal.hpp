@@ -195,9 +195,9 @@ cm_status fft_r2c_impl(cm_ctx *ctx, int N, cm_dtype dt, const void *in, void *ou
 cm_status fft_c2r_impl(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale);
 // c2r of a spectrum that is zero wherever a wave-vector component exceeds K
 // (|kx|, |ky|, kz <= K < N/2 hold every non-zero mode): the x pass transforms
-// only the lines with |ky| <= K and kz <= K, the y pass only kz <= K, the z
-// pass reads kz <= K only.  Modes kz > K of the input are never read, nor
-// written.
+// only the lines with |ky| <= K and kz <= K, the y pass only kz <= K and reads
+// only |ky| <= K, the z pass reads kz <= K only.  Modes with |ky| > K or
+// kz > K of the input are never read; those with kz > K are not written.
 cm_status fft_c2r_band(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, double scale, int K);
 // Strided column passes over the half spectrum, axis 1 (y) or 0 (x).  pitch:
 // kz row pitch of the spectrum in complex values (0 = N/2+1, the FFTW layout;
@@ -208,8 +208,9 @@ cm_status fft_c2r_band(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
 // of the pass runs after its transform (SumTail).  stream_loads (float32, y
 // axis, N >= 512): read the input with the nontemporal hint (the fused
 // measurement's y passes, where it speeds up the passes after them).
-// ncols (float32, > 0): only the first ncols kz columns; band (float32, x axis,
-// >= 0): only the lines with |ky| <= band (skipped lines must be zero).
+// ncols (float32, > 0): only the first ncols kz columns; band (float32, >= 0):
+// the input is zero at |ky| > band, so the x pass transforms only the lines
+// with |ky| <= band and the y pass reads only |ky| <= band.
 struct SumTail;
 cm_status fft_cols(cm_ctx *ctx, int N, cm_dtype dt, void *data, int axis, bool forward, int pitch = 0,
                    const void *phase = nullptr, const SumTail *tail = nullptr, bool stream_loads = false,

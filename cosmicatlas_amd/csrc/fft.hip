@@ -283,11 +283,11 @@ __global__ void __launch_bounds__(RowShape<N>::NT) k_rows_c2r_ct(const float2 *_
     }
 }
 
-template <int N, int NB, bool FWD>
+template <int N, int NB, bool FWD, bool EL = false>
 __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__restrict__ data, int ncols, size_t stride,
                                                                  size_t planeStride, const float2 *__restrict__ tw,
                                                                  const float2 *__restrict__ phase, SumTail tail,
-                                                                 int ntl, int pk, int pskip) {
+                                                                 int ntl, int pk, int pskip, int elim) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *lds = reinterpret_cast<float2 *>(smem_raw);
     // planes above pk are offset by pskip (a pass over the two ends of the
@@ -300,7 +300,8 @@ __global__ void __launch_bounds__((ColShape<N, NB>::NT)) k_cols_ct(float2 *__res
     // the column passes slower, the x passes (columns of planes 1 MB apart)
     // 2.2x)
     auto src = [&](int b, int e) {
-        if (b >= valid) return float2{0.f, 0.f};
+        // EL: elements elim < e < N - elim are zero (not read)
+        if (b >= valid || (EL && e > elim && e < N - elim)) return float2{0.f, 0.f};
         const float2 *q = &data[base + (size_t)e * stride + b];
         return ntl ? load_stream(q) : *q;
     };
@@ -344,11 +345,15 @@ static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const f
     if (CS::LDS > ctx->lds_per_block) return fail(ctx, CM_ERR_UNSUPPORTED, "fft column pass: LDS");
     const int ncols = ncols_used > 0 ? std::min(ncols_used, N / 2 + 1) : N / 2 + 1;
     unsigned planes = (unsigned)N;
-    int pk = N, pskip = 0;
-    if (axis == 0 && band >= 0 && 2 * band + 1 < N) {  // y planes 0..band and N-band..N-1
-        planes = (unsigned)(2 * band + 1);
-        pk = band;
-        pskip = N - 2 * band - 1;
+    int pk = N, pskip = 0, elim = N;
+    if (band >= 0 && 2 * band + 1 < N) {
+        if (axis == 0) {  // y planes 0..band and N-band..N-1
+            planes = (unsigned)(2 * band + 1);
+            pk = band;
+            pskip = N - 2 * band - 1;
+        } else {  // y elements band < e < N - band are zero
+            elim = band;
+        }
     }
     size_t stride, planeStride;
     if (axis == 1) {  // y: fixed x plane, stride over kz rows
@@ -361,8 +366,9 @@ static cm_status cols_ct(cm_ctx *ctx, float2 *data, int axis, int pitch, const f
     dim3 grid((unsigned)((ncols + NB - 1) / NB), planes);
     const char *name = axis == 1 ? (FWD ? "fft_y_fwd" : "fft_y_inv") : (FWD ? "fft_x_fwd" : "fft_x_inv");
     const SumTail t = tail ? *tail : SumTail{nullptr, 0, nullptr, nullptr, nullptr, 0, nullptr};
-    return launch(ctx, name, k_cols_ct<N, NB, FWD>, grid, dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw,
-                  phase, t, (stream_loads && N >= 512 && axis == 1) ? 1 : 0, pk, pskip);
+    return launch(ctx, name, elim < N ? k_cols_ct<N, NB, FWD, true> : k_cols_ct<N, NB, FWD, false>, grid,
+                  dim3(CS::NT), CS::LDS, data, ncols, stride, planeStride, tw, phase, t,
+                  (stream_loads && N >= 512 && axis == 1) ? 1 : 0, pk, pskip, elim);
 }
 
 static cm_status tw_pair(cm_ctx *ctx, int N, const float2 **twM, const float2 **twN) {
@@ -610,7 +616,7 @@ cm_status fft_c2r_band(cm_ctx *ctx, int N, cm_dtype dt, void *in, void *out, dou
     if (K < 0 || K >= N / 2) return fft_c2r_impl(ctx, N, dt, in, out, scale);
     cm_status s = fft_cols(ctx, N, dt, in, 0, false, 0, nullptr, nullptr, false, K + 1, K);
     if (s != CM_OK) return s;
-    if ((s = fft_cols(ctx, N, dt, in, 1, false, 0, nullptr, nullptr, false, K + 1)) != CM_OK) return s;
+    if ((s = fft_cols(ctx, N, dt, in, 1, false, 0, nullptr, nullptr, false, K + 1, K)) != CM_OK) return s;
     return fft_rows_c2r(ctx, N, dt, in, out, scale, 0, nullptr, K + 1);
 }
 
